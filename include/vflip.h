@@ -1,0 +1,169 @@
+/*
+ * vflip.h — C ABI of the MI355X-native vFLIP hot path (libvflip.so).
+ *
+ * Plain pointers and sizes only. Unless a function says "host", every array
+ * argument is a DEVICE pointer (HBM), laid out linear with x fastest:
+ * a cell grid ni x nj x nk has index i + ni*(j + nj*k); MAC faces use the
+ * staggered shapes u:(ni+1,nj,nk), v:(ni,nj+1,nk), w:(ni,nj,nk+1) and the nodal
+ * solid SDF is (ni+1,nj+1,nk+1), all with the reference's world placement
+ * (cell (i,j,k) centre at (i,j,k)*h, u(i,j,k) at ((i-.5)h, jh, kh), nodal
+ * phi(i,j,k) at ((i-.5)h,(j-.5)h,(k-.5)h)).
+ *
+ * Status codes: 0 ok/converged, 1 not converged, <0 error (vflip_last_error()).
+ * No exceptions cross the ABI. Work is enqueued on the stream set on the
+ * object (default: the null stream); calls return after the work completes
+ * unless documented otherwise.
+ *
+ * Reference interfaces replaced (paths relative to the vFLIP checkout,
+ * S/ = Smoke_Solver/) are cited on each declaration.
+ */
+#ifndef VFLIP_H
+#define VFLIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VFLIP_OK 0
+#define VFLIP_NOT_CONVERGED 1
+#define VFLIP_ERR_ARG (-1)
+#define VFLIP_ERR_HIP (-2)
+#define VFLIP_ERR_NOMEM (-3)
+#define VFLIP_ERR_BREAKDOWN (-4) /* rho == 0 or NaN after the first preconditioner apply */
+
+/* Last error message of the calling thread ("" if none). */
+const char* vflip_last_error(void);
+/* Library version string. */
+const char* vflip_version(void);
+/* Number of visible HIP devices (0 when none). */
+int vflip_device_count(void);
+
+/* ------------------------------------------------------------------ solver */
+typedef struct vflip_solver vflip_solver;
+
+enum vflip_precision {
+    VFLIP_PREC_FP32 = 0,  /* plain: FP32 storage, FP32 smoothing, FP32 CG reductions */
+    VFLIP_PREC_MIXED = 1  /* FP32 smoothing, FP16-stored levels >= 2, FP64 CG dots and
+                             alpha/beta, FP64 true-residual refinement every refine_every */
+};
+
+enum vflip_variant {
+    VFLIP_VARIANT_COARSE = 0, /* SmokeSolver3D::pcg_projection, S/Smoke_solver3D.cpp:945-1200 */
+    VFLIP_VARIANT_SUB = 1     /* SubDomainSolver::pcg_projection, S/sub_domain.cpp:556-820 */
+};
+
+typedef struct {
+    double tolerance;   /* stop when |r|_inf <= tolerance * |b|_inf (recursive r) */
+    int max_iterations; /* 50 coarse grid / 1000 subdomain in the reference */
+    int precision;      /* enum vflip_precision */
+    int pre_sweeps;     /* RB-GS sweeps before restriction (reference: 4) */
+    int post_sweeps;    /* RB-GS sweeps after prolongation (reference: 4) */
+    int coarse_sweeps;  /* RB-GS sweeps on the coarsest level (reference: 200) */
+    int refine_every;   /* mixed: recompute r = b - A x in FP64 every N iterations (8) */
+    int warm_start;     /* 1: the x passed to vflip_solver_solve is the initial guess */
+} vflip_solve_opts;
+
+typedef struct {
+    int iterations;       /* as AMGPCGSolve iterations_out */
+    int converged;        /* 1 when the stop test was met */
+    double residual;      /* final recursive |r|_inf, absolute (AMGPCGSolve residual_out) */
+    double rel_residual;  /* residual / |b|_inf */
+    double true_residual; /* |b - A x|_inf / |b|_inf recomputed in FP64 on the GPU */
+    double ms_setup;      /* hierarchy build (coarsening kernels) */
+    double ms_solve;      /* PCG loop */
+} vflip_solve_stats;
+
+/* Reference defaults: tol 1e-6, 50 iterations, FP32, 4/4/200 sweeps, refine 8. */
+void vflip_default_opts(vflip_solve_opts* opts);
+
+/* Allocates the multigrid hierarchy (levels ceil-halved while > 16^3 unknowns,
+ * S/GeometricLevelGen.h:28-71) and the PCG workspace for an ni x nj x nk grid. */
+int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vflip_solver** out);
+int vflip_solver_destroy(vflip_solver* s);
+int vflip_solver_set_opts(vflip_solver* s, const vflip_solve_opts* opts);
+/* hip_stream: a hipStream_t (NULL = null stream). */
+int vflip_solver_set_stream(vflip_solver* s, void* hip_stream);
+/* Number of levels and the dims of level l (for tests and tooling). */
+int vflip_solver_levels(const vflip_solver* s);
+int vflip_solver_level_dims(const vflip_solver* s, int level, int* ni, int* nj, int* nk);
+
+/* Level-0 operator from device FP32 arrays: diag and the couplings to the +x,
+ * +y, +z neighbours (ax[c] = A(c,c+1) = A(c+1,c); 0 where no entry exists).
+ * leak may be NULL; otherwise leak[c] = diag[c] - sum of |couplings| of row c
+ * (the Dirichlet part), which keeps coarse diagonals cancellation-free. */
+int vflip_solver_set_matrix(vflip_solver* s, const float* diag, const float* ax, const float* ay,
+                            const float* az, const float* leak);
+
+/* Builds coarse levels A_{l+1} = 0.5 R A_l P (R = 1/8 average, P = injection),
+ * S/GeometricLevelGen.cpp:8-76, in closed form on the GPU. */
+int vflip_solver_setup(vflip_solver* s);
+
+/* Preconditioned CG with one V-cycle per iteration (S/AlgebraicMultigrid.h:189-301).
+ * rhs, x: device FP32 arrays of ni*nj*nk. Returns 0 converged / 1 not converged /
+ * VFLIP_ERR_BREAKDOWN. stats may be NULL. */
+int vflip_solver_solve(vflip_solver* s, const float* rhs, float* x, vflip_solve_stats* stats);
+
+/* One V-cycle z = M^-1 r (zero initial guess) — the preconditioner alone,
+ * S/AlgebraicMultigrid.h:130-188. */
+int vflip_solver_vcycle(vflip_solver* s, const float* r, float* z);
+
+/* y = A x with the level-0 operator (S/sparse_matrix.h:341-366). */
+int vflip_solver_matvec(vflip_solver* s, const float* x, float* y);
+
+/* Copies level l's operator to device arrays (diag, ax, ay, az; FP32). */
+int vflip_solver_get_level(const vflip_solver* s, int level, float* diag, float* ax, float* ay,
+                           float* az);
+
+/* Drop-in for ivock::AMGPCGSolve<double> (S/AlgebraicMultigrid.h:189-197), HOST
+ * arrays: the 7-point SPD matrix as diag + couplings (the reference SparseMatrix
+ * rows restricted to a 7-point stencil), rhs and result of ni*nj*nk doubles.
+ * Solves on the GPU in FP32 (opts->precision selects the mixed mode) with the
+ * reference stop test; result, residual_out and iterations_out as the reference.
+ * Returns 0 converged, 1 not converged, <0 error. opts may be NULL (defaults
+ * with tolerance_factor/max_iterations applied). */
+int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const double* ax,
+                            const double* ay, const double* az, const double* rhs, double* result,
+                            double tolerance_factor, int max_iterations, double* residual_out,
+                            int* iterations_out, const vflip_solve_opts* opts);
+
+/* -------------------------------------------------------------- projection */
+/* Cut-cell face weights w = clamp(1 - fraction_inside(4 nodal phi), 0, 1);
+ * bit-exact with S/Smoke_solver3D.cpp:874-944 / S/levelset_util.cpp:23-103. */
+int vflip_face_weights(int ni, int nj, int nk, const float* phi, float* uw, float* vw, float* ww,
+                       void* hip_stream);
+
+/* Inflow boundary: faces of cells with bdesc==1 get u=u_in, v=w=0
+ * (S/Smoke_solver3D.cpp:964-980). bdesc: ni*nj*nk bytes. */
+int vflip_inflow_bc(int ni, int nj, int nk, const int8_t* bdesc, float u_in, float* u, float* v,
+                    float* w, void* hip_stream);
+
+/* Assembles the level-0 operator of the solver's grid and the right-hand side
+ * rhs (device FP32, ni*nj*nk) from the MAC velocity, the face weights and the
+ * solid velocity (S/Smoke_solver3D.cpp:983-1111; variant SUB: S/sub_domain.cpp:612-729). */
+int vflip_assemble(vflip_solver* s, int variant, float h, float dt, const float* u, const float* v,
+                   const float* w, const float* uw, const float* vw, const float* ww,
+                   const float* us, const float* vs, const float* ws, float* rhs);
+
+/* Velocity update u -= grad p (S/Smoke_solver3D.cpp:1126-1200; variant SUB with
+ * dt and wider face ranges: S/sub_domain.cpp:745-819). p: ni*nj*nk FP32. */
+int vflip_grad_apply(int ni, int nj, int nk, int variant, float h, float dt, const float* p,
+                     const float* uw, const float* vw, const float* ww, const float* us,
+                     const float* vs, const float* ws, float* u, float* v, float* w,
+                     void* hip_stream);
+
+/* Whole pressure projection — the body of SmokeSolver3D::pcg_projection /
+ * SubDomainSolver::pcg_projection after face weights: assemble, setup, PCG,
+ * gradient update. pressure (device, ni*nj*nk) receives p; it is also the warm
+ * start when opts.warm_start is set. rhs_scratch: device ni*nj*nk FP32. */
+int vflip_project(vflip_solver* s, int variant, float h, float dt, float* u, float* v, float* w,
+                  const float* uw, const float* vw, const float* ww, const float* us,
+                  const float* vs, const float* ws, float* rhs_scratch, float* pressure,
+                  vflip_solve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VFLIP_H */

@@ -1,0 +1,236 @@
+"""ctypes bindings of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU baseline,
+never as part of the product path.  Each wrapper names the reference code it
+restates; the restatement itself (with file:line citations) is
+oracle/vflip_oracle.cpp.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+c_int, c_float, c_double = ctypes.c_int, ctypes.c_float, ctypes.c_double
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: run `make -C oracle`")
+        L = ctypes.CDLL(path)
+        L.or_fraction_inside2.restype = c_float
+        L.or_fraction_inside2.argtypes = [c_float, c_float]
+        L.or_fraction_inside4.restype = c_float
+        L.or_fraction_inside4.argtypes = [c_float] * 4
+        L.or_face_weights.argtypes = [c_int, c_int, c_int, f32p, f32p, f32p, f32p]
+        L.or_assemble.argtypes = [c_int, c_int, c_int, c_float, c_float, c_int] + [f32p] * 9 + [f64p] * 5
+        L.or_level_dims.restype = c_int
+        L.or_level_dims.argtypes = [c_int, c_int, c_int, i32p, c_int]
+        L.or_coarsen.argtypes = [c_int, c_int, c_int] + [f64p] * 8
+        L.or_matvec.argtypes = [c_int, c_int, c_int] + [f64p] * 6
+        L.or_rbgs.argtypes = [c_int, c_int, c_int] + [f64p] * 6 + [c_int]
+        L.or_restrict_residual.argtypes = [c_int, c_int, c_int] + [f64p] * 7
+        L.or_prolong_add.argtypes = [c_int, c_int, c_int, f64p, f64p]
+        L.or_vcycle.argtypes = [c_int, c_int, c_int] + [f64p] * 6 + [c_int] * 3
+        L.or_amgpcg.restype = c_int
+        L.or_amgpcg.argtypes = ([c_int, c_int, c_int] + [f64p] * 6 + [c_double, c_int,
+                                ctypes.POINTER(c_double), ctypes.POINTER(c_int), c_int, c_int, c_int,
+                                ctypes.c_void_p])
+        L.or_grad_apply.argtypes = [c_int, c_int, c_int, c_float, c_float, c_int, f64p] + [f32p] * 9
+        L.or_sample_linear.restype = c_float
+        L.or_sample_linear.argtypes = [f32p, c_int, c_int, c_int] + [c_double] * 7
+        L.or_sample_cubic.restype = c_float
+        L.or_sample_cubic.argtypes = [f32p, c_int, c_int, c_int] + [c_double] * 7
+        L.or_sample_cube_lerp.restype = c_float
+        L.or_sample_cube_lerp.argtypes = [f32p, c_int, c_int, c_int] + [c_double] * 7 + [f32p]
+        L.or_advect_rk2.argtypes = ([c_int, c_int, c_int, c_float, c_float, c_int, f32p, f32p, f32p, f32p,
+                                    c_int, c_int, c_int, c_double, c_double, c_double, f32p])
+        L.or_hash.restype = c_int
+        L.or_hash.argtypes = ([c_int, f32p, c_float, c_int, c_int, c_int, c_int, c_float, c_float, c_float,
+                              i32p, i32p, i32p])
+        L.or_p2g.argtypes = ([c_int, f32p, f32p, c_int, c_float, c_int, c_int, c_int, i32p, i32p,
+                             c_int, c_int, c_int, c_double, c_double, c_double, c_int, c_float, c_float,
+                             c_float, f32p, f32p])
+        L.or_g2p.argtypes = ([c_int, f32p, f32p, c_float, c_int, c_int, c_int] + [f32p] * 6 +
+                             [c_float, c_float, c_float, c_float])
+        L.or_fine_to_coarse.argtypes = [f32p, c_int, c_int, c_int, f32p, c_int, c_int, c_int] + [c_int] * 4
+        L.or_coarse_to_fine.argtypes = ([f32p, c_int, c_int, c_int, c_double, c_double, c_double, c_float,
+                                        f32p, c_int, c_int, c_int, c_double, c_double, c_double, c_double])
+        L.or_set_face_boundary.argtypes = [f32p, c_int, c_int, c_int, f32p, c_int, c_int, c_int] + [c_int] * 4
+        L.or_to_blocked.argtypes = [f32p, c_int, c_int, c_int, f32p]
+        L.or_randhash.restype = ctypes.c_uint
+        L.or_randhash.argtypes = [ctypes.c_uint]
+        L.or_num_threads.restype = c_int
+        _LIB = L
+    return _LIB
+
+
+def ref_lib():
+    """oracle/_ref/libvflipref.so (reference sources compiled as-is), or None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libvflipref.so")
+        if not os.path.exists(path):
+            return None
+        R = ctypes.CDLL(path)
+        R.ref_fraction_inside2.restype = c_float
+        R.ref_fraction_inside2.argtypes = [c_float, c_float]
+        R.ref_fraction_inside4.restype = c_float
+        R.ref_fraction_inside4.argtypes = [c_float] * 4
+        R.ref_randhash.restype = ctypes.c_uint
+        R.ref_randhash.argtypes = [ctypes.c_uint]
+        R.ref_sample.argtypes = ([c_int, f32p, c_int, c_int, c_int, c_double, c_double, c_double, c_double,
+                                 c_int, f64p, f32p, f32p, f32p])
+        R.ref_blocked_index.restype = ctypes.c_uint
+        R.ref_blocked_index.argtypes = [c_int] * 6
+        _REF = R
+    return _REF
+
+
+# --------------------------------------------------------------------------- helpers
+def c32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def face_weights(phi):
+    """phi: nodal array shaped (nk+1, nj+1, ni+1). Returns uw, vw, ww (z,y,x arrays)."""
+    nk, nj, ni = (s - 1 for s in phi.shape)
+    uw = np.zeros((nk, nj, ni + 1), np.float32)
+    vw = np.zeros((nk, nj + 1, ni), np.float32)
+    ww = np.zeros((nk + 1, nj, ni), np.float32)
+    lib().or_face_weights(ni, nj, nk, c32(phi), uw, vw, ww)
+    return uw, vw, ww
+
+
+def assemble(h, u, v, w, uw, vw, ww, us, vs, ws, dt=1.0, sub=False):
+    nk, nj, ni1 = u.shape
+    ni = ni1 - 1
+    shp = (nk, nj, ni)
+    out = [np.zeros(shp, np.float64) for _ in range(5)]
+    lib().or_assemble(ni, nj, nk, h, dt, int(sub), c32(u), c32(v), c32(w), c32(uw), c32(vw), c32(ww),
+                      c32(us), c32(vs), c32(ws), *out)
+    return out  # diag, ax, ay, az, rhs
+
+
+def level_dims(ni, nj, nk):
+    d = np.zeros(3 * 32, np.int32)
+    n = lib().or_level_dims(ni, nj, nk, d, 32)
+    return [tuple(int(x) for x in d[3 * l:3 * l + 3]) for l in range(n)]
+
+
+def coarsen(d, ax, ay, az):
+    nk, nj, ni = d.shape
+    ci, cj, ck = (int(np.ceil(np.float32(n) / 2.0)) for n in (ni, nj, nk))
+    out = [np.zeros((ck, cj, ci), np.float64) for _ in range(4)]
+    lib().or_coarsen(ni, nj, nk, c64(d), c64(ax), c64(ay), c64(az), *out)
+    return out
+
+
+def matvec(d, ax, ay, az, x):
+    nk, nj, ni = d.shape
+    y = np.zeros_like(d)
+    lib().or_matvec(ni, nj, nk, c64(d), c64(ax), c64(ay), c64(az), c64(x), y)
+    return y
+
+
+def rbgs(d, ax, ay, az, b, x, sweeps):
+    nk, nj, ni = d.shape
+    x = c64(x).copy()
+    lib().or_rbgs(ni, nj, nk, c64(d), c64(ax), c64(ay), c64(az), c64(b), x, sweeps)
+    return x
+
+
+def vcycle(d, ax, ay, az, r, pre=4, post=4, coarse=200):
+    nk, nj, ni = d.shape
+    z = np.zeros_like(d)
+    lib().or_vcycle(ni, nj, nk, c64(d), c64(ax), c64(ay), c64(az), c64(r), z, pre, post, coarse)
+    return z
+
+
+def amgpcg(d, ax, ay, az, rhs, tol=1e-6, maxit=50, pre=4, post=4, coarse=200, history=False):
+    nk, nj, ni = d.shape
+    x = np.zeros_like(d)
+    res = c_double(0)
+    its = c_int(0)
+    hist = np.zeros(maxit + 1, np.float64) if history else None
+    ok = lib().or_amgpcg(ni, nj, nk, c64(d), c64(ax), c64(ay), c64(az), c64(rhs), x, tol, maxit,
+                         ctypes.byref(res), ctypes.byref(its), pre, post, coarse,
+                         hist.ctypes.data if history else None)
+    out = dict(x=x, ok=bool(ok), residual=res.value, iterations=its.value)
+    if history:
+        out["history"] = hist[: its.value + 1]
+    return out
+
+
+def grad_apply(h, p, uw, vw, ww, us, vs, ws, u, v, w, dt=1.0, sub=False):
+    nk, nj, ni = p.shape
+    u, v, w = c32(u).copy(), c32(v).copy(), c32(w).copy()
+    lib().or_grad_apply(ni, nj, nk, h, dt, int(sub), c64(p), c32(uw), c32(vw), c32(ww), c32(us), c32(vs),
+                        c32(ws), u, v, w)
+    return u, v, w
+
+
+def advect_rk2(h, dt, u, v, w, field, offs, mode=0):
+    nk, nj, ni1 = u.shape
+    ni = ni1 - 1
+    fz, fy, fx = field.shape
+    out = np.zeros_like(c32(field))
+    lib().or_advect_rk2(ni, nj, nk, h, dt, mode, c32(u), c32(v), c32(w), c32(field), fx, fy, fz,
+                        offs[0], offs[1], offs[2], out)
+    return out
+
+
+def bin_particles(pos, h, dims, clamp=False, bmin=(0.0, 0.0, 0.0)):
+    ni, nj, nk = dims
+    pos = c32(pos).reshape(-1)
+    npart = pos.size // 3
+    key = np.zeros(npart, np.int32)
+    start = np.zeros(ni * nj * nk + 1, np.int32)
+    order = np.zeros(max(npart, 1), np.int32)
+    nb = lib().or_hash(npart, pos, h, ni, nj, nk, int(clamp), *[float(b) for b in bmin], key, start, order)
+    return key, start, order[:nb]
+
+
+def p2g(pos, vel, comp, h, dims, start, order, fshape, offs, sub=False, bmin=(0.0, 0.0, 0.0)):
+    ni, nj, nk = dims
+    fz, fy, fx = fshape
+    field = np.zeros(fshape, np.float32)
+    coef = np.zeros(fshape, np.float32)
+    pos = c32(pos).reshape(-1)
+    order = np.ascontiguousarray(order, np.int32)
+    if order.size == 0:
+        order = np.zeros(1, np.int32)
+    lib().or_p2g(pos.size // 3, pos, c32(vel).reshape(-1), comp, h, ni, nj, nk,
+                 np.ascontiguousarray(start, np.int32), order, fx, fy, fz,
+                 offs[0], offs[1], offs[2], int(sub), *[float(b) for b in bmin], field, coef)
+    return field, coef
+
+
+def g2p(pos, vel, h, dims, du, dv, dw, u=None, v=None, w=None, flip=1.0, bmin=(0.0, 0.0, 0.0)):
+    ni, nj, nk = dims
+    vel = c32(vel).reshape(-1).copy()
+    pos = c32(pos).reshape(-1)
+    u = du if u is None else u
+    v = dv if v is None else v
+    w = dw if w is None else w
+    lib().or_g2p(pos.size // 3, pos, vel, h, ni, nj, nk, c32(du), c32(dv), c32(dw), c32(u), c32(v), c32(w),
+                 flip, *[float(b) for b in bmin])
+    return vel.reshape(-1, 3)
+
+
+def randhash(s):
+    return lib().or_randhash(s)

@@ -1,0 +1,903 @@
+// vflip oracle: CPU FP64 restatement of the reference vFLIP per-step hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP product path:
+// only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// it, and only to check or to time the reference algorithm on the host. The
+// product library (vflip_amd/libvflip.so) never links or calls it.
+//
+// Every function cites the reference file:line it restates (paths relative to
+// the reference repository root, S/ = Smoke_Solver/).
+//
+// Parity status of each function (how it is pinned; see tests/):
+//   fraction_inside2/4, face_weights .... oracle/_ref (reference levelset_util.cpp
+//                                         compiled as-is) + closed-form areas
+//   sample_linear/cubic/cube_lerp ....... oracle/_ref (reference fluid_buffer3D.h
+//                                         compiled as-is) + affine/cubic exactness
+//   randhash ............................ oracle/_ref (reference util.h)
+//   assemble ............................ closed form (uniform Laplacian) + scipy
+//                                         divergence/Laplacian operators
+//   coarsen ............................. scipy 0.5*R*A*P with the reference's R/P
+//   matvec/rbgs/vcycle/amgpcg ........... scipy spsolve, RB-GS colour-order brute
+//                                         force, PCG invariants
+//   grad_apply .......................... discrete divergence after projection
+//   hash (binning) ...................... brute-force membership
+//   p2g ................................. brute force over all particles; constant
+//                                         velocity closed form
+//   g2p ................................. affine reproduction
+//   advect_rk2_linear/cubic_clamp ....... affine translation / RK2 exactness
+//   coarse/fine operators ............... constant-field round trip, flux closed form
+//
+// Floating point: compiled with -ffp-contract=off so float expressions round
+// exactly as written (the reference's MSVC /fp:precise build does the same).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdlib>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+inline size_t lin(int i, int j, int k, int nx, int ny) {
+    return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k);
+}
+
+// S/util.h:173 clamp
+template <class T> inline T clampv(T a, T lo, T hi) {
+    if (a < lo) return lo;
+    if (a > hi) return hi;
+    return a;
+}
+
+// S/levelset_util.cpp:15-20 cycle_array
+inline void rot4(float* a) {
+    float t = a[0];
+    a[0] = a[1]; a[1] = a[2]; a[2] = a[3]; a[3] = t;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Level-set cut fractions. S/levelset_util.cpp:4-13 (segment) and 23-103 (square)
+// ---------------------------------------------------------------------------
+float or_fraction_inside2(float l, float r) {
+    if (l < 0 && r < 0) return 1;
+    if (l < 0 && r >= 0) return l / (l - r);
+    if (l >= 0 && r < 0) return r / (r - l);
+    return 0;
+}
+
+// The reference rotates with unbounded while loops; with corners exactly 0 some
+// of them never terminate (e.g. two zeros and two positives). The rotations are
+// bounded at one full turn here, which is identical wherever the reference ends.
+float or_fraction_inside4(float bl, float br, float tl, float tr) {
+    int inside = (bl <= 0 ? 1 : 0) + (tl <= 0 ? 1 : 0) + (br <= 0 ? 1 : 0) + (tr <= 0 ? 1 : 0);
+    // corner order walked around the square: bl, br, tr, tl
+    float c[4] = {bl, br, tr, tl};
+    if (inside == 4) return 1;
+    if (inside == 3) {
+        for (int q = 0; q < 4 && c[0] < 0; ++q) rot4(c);
+        float s0 = 1 - or_fraction_inside2(c[0], c[3]);
+        float s1 = 1 - or_fraction_inside2(c[0], c[1]);
+        return 1 - 0.5f * s0 * s1;
+    }
+    if (inside == 2) {
+        for (int q = 0; q < 4 && (c[0] >= 0 || !(c[1] < 0 || c[2] < 0)); ++q) rot4(c);
+        if (c[1] < 0) {
+            float sl = or_fraction_inside2(c[0], c[3]);
+            float sr = or_fraction_inside2(c[1], c[2]);
+            return 0.5f * (sl + sr);
+        }
+        float mid = 0.25f * (c[0] + c[1] + c[2] + c[3]);
+        if (mid < 0) {
+            float area = 0;
+            float s1 = 1 - or_fraction_inside2(c[0], c[3]);
+            float s3 = 1 - or_fraction_inside2(c[2], c[3]);
+            area += 0.5f * s1 * s3;
+            float s2 = 1 - or_fraction_inside2(c[2], c[1]);
+            float s0 = 1 - or_fraction_inside2(c[0], c[1]);
+            area += 0.5f * s0 * s2;
+            return 1 - area;
+        }
+        float area = 0;
+        float s0 = or_fraction_inside2(c[0], c[1]);
+        float s1 = or_fraction_inside2(c[0], c[3]);
+        area += 0.5f * s0 * s1;
+        float s2 = or_fraction_inside2(c[2], c[1]);
+        float s3 = or_fraction_inside2(c[2], c[3]);
+        area += 0.5f * s2 * s3;
+        return area;
+    }
+    if (inside == 1) {
+        for (int q = 0; q < 4 && c[0] >= 0; ++q) rot4(c);
+        float s0 = or_fraction_inside2(c[0], c[3]);
+        float s1 = or_fraction_inside2(c[0], c[1]);
+        return 0.5f * s0 * s1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Cut-cell face weights from the nodal solid SDF.
+// S/Smoke_solver3D.cpp:874-944 (identical in S/sub_domain.cpp:491-554).
+// phi: nodal (ni+1)x(nj+1)x(nk+1); uw: (ni+1)xnjxnk; vw: nix(nj+1)xnk; ww: nixnjx(nk+1).
+// All arrays linear, x fastest.
+// ---------------------------------------------------------------------------
+void or_face_weights(int ni, int nj, int nk, const float* phi, float* uw, float* vw, float* ww) {
+    const int px = ni + 1, py = nj + 1;
+    auto P = [&](int i, int j, int k) { return phi[lin(i, j, k, px, py)]; };
+#pragma omp parallel for
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i <= ni; ++i) {
+                float f = 1 - or_fraction_inside4(P(i, j, k), P(i, j + 1, k), P(i, j, k + 1), P(i, j + 1, k + 1));
+                uw[lin(i, j, k, ni + 1, nj)] = clampv(f, 0.0f, 1.0f);
+            }
+#pragma omp parallel for
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j <= nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                float f = 1 - or_fraction_inside4(P(i, j, k), P(i, j, k + 1), P(i + 1, j, k), P(i + 1, j, k + 1));
+                vw[lin(i, j, k, ni, nj + 1)] = clampv(f, 0.0f, 1.0f);
+            }
+#pragma omp parallel for
+    for (int k = 0; k <= nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                float f = 1 - or_fraction_inside4(P(i, j, k), P(i, j + 1, k), P(i + 1, j, k), P(i + 1, j + 1, k));
+                ww[lin(i, j, k, ni, nj)] = clampv(f, 0.0f, 1.0f);
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Pressure system assembly over interior cells 1..n-2 (outer ring = empty rows,
+// Dirichlet p=0). S/Smoke_solver3D.cpp:983-1111 (coarse, sub=0) and
+// S/sub_domain.cpp:612-729 (sub=1: terms carry dt, solid term uses float 1/h).
+// Output: symmetric 7-point matrix as diag + couplings to +x/+y/+z neighbours
+// (ax[c] = A(c, c+1) = A(c+1, c), etc.; zero where the reference stores no entry),
+// plus the FP64 right-hand side. All nixnjxnk linear.
+// ---------------------------------------------------------------------------
+void or_assemble(int ni, int nj, int nk, float h, float dt, int sub,
+                 const float* u, const float* v, const float* w,
+                 const float* uw, const float* vw, const float* ww,
+                 const float* us, const float* vs, const float* ws,
+                 double* diag, double* ax, double* ay, double* az, double* rhs) {
+    const size_t n = (size_t)ni * nj * nk;
+    std::memset(diag, 0, n * sizeof(double));
+    std::memset(ax, 0, n * sizeof(double));
+    std::memset(ay, 0, n * sizeof(double));
+    std::memset(az, 0, n * sizeof(double));
+    std::memset(rhs, 0, n * sizeof(double));
+    auto U = [&](const float* a, int i, int j, int k) { return a[lin(i, j, k, ni + 1, nj)]; };
+    auto V = [&](const float* a, int i, int j, int k) { return a[lin(i, j, k, ni, nj + 1)]; };
+    auto W = [&](const float* a, int i, int j, int k) { return a[lin(i, j, k, ni, nj)]; };
+    const float h2 = h * h;  // sqr(_hx), S/util.h:26
+#pragma omp parallel for
+    for (int k = 1; k < nk - 1; ++k)
+        for (int j = 1; j < nj - 1; ++j)
+            for (int i = 1; i < ni - 1; ++i) {
+                size_t c = lin(i, j, k, ni, nj);
+                double d = 0, r = 0;
+                float t;
+                // +x face
+                t = sub ? U(uw, i + 1, j, k) * dt / h2 : U(uw, i + 1, j, k) / h2;
+                d += t;
+                if (i < ni - 2) ax[c] = -(double)t;
+                r -= (double)(U(uw, i + 1, j, k) * U(u, i + 1, j, k) / h);
+                // -x face
+                t = sub ? U(uw, i, j, k) * dt / h2 : U(uw, i, j, k) / h2;
+                d += t;
+                r += (double)(U(uw, i, j, k) * U(u, i, j, k) / h);
+                // +y face
+                t = sub ? V(vw, i, j + 1, k) * dt / h2 : V(vw, i, j + 1, k) / h2;
+                d += t;
+                if (j < nj - 2) ay[c] = -(double)t;
+                r -= (double)(V(vw, i, j + 1, k) * V(v, i, j + 1, k) / h);
+                // -y face
+                t = sub ? V(vw, i, j, k) * dt / h2 : V(vw, i, j, k) / h2;
+                d += t;
+                r += (double)(V(vw, i, j, k) * V(v, i, j, k) / h);
+                // +z face
+                t = sub ? W(ww, i, j, k + 1) * dt / h2 : W(ww, i, j, k + 1) / h2;
+                d += t;
+                if (k < nk - 2) az[c] = -(double)t;
+                r -= (double)(W(ww, i, j, k + 1) * W(w, i, j, k + 1) / h);
+                // -z face
+                t = sub ? W(ww, i, j, k) * dt / h2 : W(ww, i, j, k) / h2;
+                d += t;
+                r += (double)(W(ww, i, j, k) * W(w, i, j, k) / h);
+                if (d > 0) {
+                    // solid-velocity term, evaluated left to right in float
+                    float s = (U(uw, i + 1, j, k) - 1.0f) * U(us, i + 1, j, k);
+                    s = s - (U(uw, i, j, k) - 1.0f) * U(us, i, j, k);
+                    s = s + (V(vw, i, j + 1, k) - 1.0f) * V(vs, i, j + 1, k);
+                    s = s - (V(vw, i, j, k) - 1.0f) * V(vs, i, j, k);
+                    s = s + (W(ww, i, j, k + 1) - 1.0f) * W(ws, i, j, k + 1);
+                    s = s - (W(ww, i, j, k) - 1.0f) * W(ws, i, j, k);
+                    if (sub)
+                        r += (double)((1 / h) * s);  // S/sub_domain.cpp:721: int/float
+                    else
+                        r += 1.0 / (double)h * (double)s;  // S/Smoke_solver3D.cpp:1101: double
+                }
+                diag[c] = d;
+                rhs[c] = r;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// AMG hierarchy shapes: ceil-halving while unknowns > 16^3.
+// S/GeometricLevelGen.h:36-58.
+// ---------------------------------------------------------------------------
+int or_level_dims(int ni, int nj, int nk, int* dims, int max_levels) {
+    int L = 1;
+    dims[0] = ni; dims[1] = nj; dims[2] = nk;
+    unsigned int unknowns = (unsigned)ni * nj * nk;
+    int a = ni, b = nj, c = nk;
+    while (unknowns > 16u * 16u * 16u && L < max_levels) {
+        a = (int)std::ceil((float)a / 2.0);
+        b = (int)std::ceil((float)b / 2.0);
+        c = (int)std::ceil((float)c / 2.0);
+        dims[3 * L] = a; dims[3 * L + 1] = b; dims[3 * L + 2] = c;
+        unknowns = (unsigned)a * b * c;
+        ++L;
+    }
+    return L;
+}
+
+// ---------------------------------------------------------------------------
+// Galerkin coarsening A_c = 0.5 * R * A * P with R = 1/8 per child (average)
+// and P = injection, S/GeometricLevelGen.cpp:8-76 and S/GeometricLevelGen.h:60-68.
+// Piecewise-constant aggregates only touch across faces, so A_c is again a
+// symmetric 7-point operator; its entries in closed form are
+//   A_c(I,I)   = 1/16 * sum_{i,k in I} A(i,k)
+//   A_c(I,I+e) = 1/16 * sum_{i in I, k in I+e} A(i,k)
+// which is what the sparse triple product evaluates (up to FP64 summation order).
+// ---------------------------------------------------------------------------
+void or_coarsen(int ni, int nj, int nk, const double* d, const double* ax, const double* ay,
+                const double* az, double* dc, double* axc, double* ayc, double* azc) {
+    const int ci = (int)std::ceil((float)ni / 2.0), cj = (int)std::ceil((float)nj / 2.0),
+              ck = (int)std::ceil((float)nk / 2.0);
+    const double s = 0.5 * 0.125;  // 0.5 * R weight * P weight (1.0)
+#pragma omp parallel for
+    for (int K = 0; K < ck; ++K)
+        for (int J = 0; J < cj; ++J)
+            for (int I = 0; I < ci; ++I) {
+                double sd = 0, sx = 0, sy = 0, sz = 0;
+                for (int kk = 0; kk < 2; ++kk)
+                    for (int jj = 0; jj < 2; ++jj)
+                        for (int ii = 0; ii < 2; ++ii) {
+                            int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+                            if (i >= ni || j >= nj || k >= nk) continue;
+                            size_t c = lin(i, j, k, ni, nj);
+                            sd += d[c];
+                            // couplings from child i to a neighbour k
+                            if (i + 1 < ni) { if (ii == 0) sd += 2 * ax[c]; else sx += ax[c]; }
+                            if (j + 1 < nj) { if (jj == 0) sd += 2 * ay[c]; else sy += ay[c]; }
+                            if (k + 1 < nk) { if (kk == 0) sd += 2 * az[c]; else sz += az[c]; }
+                        }
+                size_t C = lin(I, J, K, ci, cj);
+                dc[C] = s * sd;
+                axc[C] = (I + 1 < ci) ? s * sx : 0.0;
+                ayc[C] = (J + 1 < cj) ? s * sy : 0.0;
+                azc[C] = (K + 1 < ck) ? s * sz : 0.0;
+            }
+}
+
+// 7-point operator application in the reference CSR column order
+// (-z, -y, -x, diag, +x, +y, +z). S/sparse_matrix.h:341-366.
+static inline double row_dot(int i, int j, int k, int ni, int nj, int nk, const double* d,
+                             const double* ax, const double* ay, const double* az, const double* x,
+                             bool with_diag) {
+    size_t c = lin(i, j, k, ni, nj);
+    double s = 0;
+    if (k > 0) s += az[c - (size_t)ni * nj] * x[c - (size_t)ni * nj];
+    if (j > 0) s += ay[c - ni] * x[c - ni];
+    if (i > 0) s += ax[c - 1] * x[c - 1];
+    if (with_diag) s += d[c] * x[c];
+    if (i + 1 < ni) s += ax[c] * x[c + 1];
+    if (j + 1 < nj) s += ay[c] * x[c + ni];
+    if (k + 1 < nk) s += az[c] * x[c + (size_t)ni * nj];
+    return s;
+}
+
+void or_matvec(int ni, int nj, int nk, const double* d, const double* ax, const double* ay,
+               const double* az, const double* x, double* y) {
+#pragma omp parallel for
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i)
+                y[lin(i, j, k, ni, nj)] = row_dot(i, j, k, ni, nj, nk, d, ax, ay, az, x, true);
+}
+
+// Red-black Gauss-Seidel, odd colour (i+j+k odd) first. S/AlgebraicMultigrid.h:22-103.
+void or_rbgs(int ni, int nj, int nk, const double* d, const double* ax, const double* ay,
+             const double* az, const double* b, double* x, int sweeps) {
+    for (int it = 0; it < sweeps; ++it) {
+        for (int colour = 1; colour >= 0; --colour) {
+#pragma omp parallel for
+            for (int k = 0; k < nk; ++k)
+                for (int j = 0; j < nj; ++j)
+                    for (int i = 0; i < ni; ++i) {
+                        if (((i + j + k) & 1) != colour) continue;
+                        size_t c = lin(i, j, k, ni, nj);
+                        double s = row_dot(i, j, k, ni, nj, nk, d, ax, ay, az, x, false);
+                        x[c] = (d[c] != 0) ? (b[c] - s) / d[c] : 0.0;
+                    }
+        }
+    }
+}
+
+// b_c = R (b - A x), residual formed entry by entry as multiply_and_subtract does.
+// S/AlgebraicMultigrid.h:105-117, S/sparse_matrix.h:459-480.
+void or_restrict_residual(int ni, int nj, int nk, const double* d, const double* ax,
+                          const double* ay, const double* az, const double* x, const double* b,
+                          double* bc) {
+    const int ci = (int)std::ceil((float)ni / 2.0), cj = (int)std::ceil((float)nj / 2.0),
+              ck = (int)std::ceil((float)nk / 2.0);
+    std::vector<double> r((size_t)ni * nj * nk);
+#pragma omp parallel for
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                size_t c = lin(i, j, k, ni, nj);
+                double rr = b[c];
+                const size_t sl = (size_t)ni * nj;
+                if (k > 0) rr -= az[c - sl] * x[c - sl];
+                if (j > 0) rr -= ay[c - ni] * x[c - ni];
+                if (i > 0) rr -= ax[c - 1] * x[c - 1];
+                rr -= d[c] * x[c];
+                if (i + 1 < ni) rr -= ax[c] * x[c + 1];
+                if (j + 1 < nj) rr -= ay[c] * x[c + ni];
+                if (k + 1 < nk) rr -= az[c] * x[c + sl];
+                r[c] = rr;
+            }
+#pragma omp parallel for
+    for (int K = 0; K < ck; ++K)
+        for (int J = 0; J < cj; ++J)
+            for (int I = 0; I < ci; ++I) {
+                double s = 0;
+                for (int kk = 0; kk < 2; ++kk)
+                    for (int jj = 0; jj < 2; ++jj)
+                        for (int ii = 0; ii < 2; ++ii) {
+                            int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
+                            if (i < ni && j < nj && k < nk) s += 0.125 * r[lin(i, j, k, ni, nj)];
+                        }
+                bc[lin(I, J, K, ci, cj)] = s;
+            }
+}
+
+// x += P x_c (injection). S/AlgebraicMultigrid.h:118-129.
+void or_prolong_add(int ni, int nj, int nk, const double* xc, double* x) {
+    const int ci = (int)std::ceil((float)ni / 2.0), cj = (int)std::ceil((float)nj / 2.0);
+#pragma omp parallel for
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) x[lin(i, j, k, ni, nj)] += xc[lin(i / 2, j / 2, k / 2, ci, cj)];
+}
+
+}  // extern "C"
+
+namespace {
+
+struct Level {
+    int n[3];
+    std::vector<double> d, ax, ay, az, x, b;
+};
+
+struct Hierarchy {
+    std::vector<Level> L;
+};
+
+void build_hierarchy(Hierarchy& H, int ni, int nj, int nk, const double* d, const double* ax,
+                     const double* ay, const double* az) {
+    int dims[3 * 32];
+    int nl = or_level_dims(ni, nj, nk, dims, 32);
+    H.L.resize(nl);
+    for (int l = 0; l < nl; ++l) {
+        Level& lv = H.L[l];
+        lv.n[0] = dims[3 * l]; lv.n[1] = dims[3 * l + 1]; lv.n[2] = dims[3 * l + 2];
+        size_t n = (size_t)lv.n[0] * lv.n[1] * lv.n[2];
+        lv.d.resize(n); lv.ax.resize(n); lv.ay.resize(n); lv.az.resize(n);
+        lv.x.assign(n, 0.0); lv.b.assign(n, 0.0);
+        if (l == 0) {
+            std::copy(d, d + n, lv.d.begin());
+            std::copy(ax, ax + n, lv.ax.begin());
+            std::copy(ay, ay + n, lv.ay.begin());
+            std::copy(az, az + n, lv.az.begin());
+        } else {
+            Level& f = H.L[l - 1];
+            or_coarsen(f.n[0], f.n[1], f.n[2], f.d.data(), f.ax.data(), f.ay.data(), f.az.data(),
+                       lv.d.data(), lv.ax.data(), lv.ay.data(), lv.az.data());
+        }
+    }
+}
+
+// One V-cycle applied to r with zero initial guess: z = M^-1 r.
+// S/AlgebraicMultigrid.h:130-188 (4 pre / 4 post sweeps, 200 on the coarsest level).
+void vcycle(Hierarchy& H, const double* r, double* z, int pre, int post, int coarse) {
+    int nl = (int)H.L.size();
+    for (int l = 0; l < nl; ++l) std::fill(H.L[l].x.begin(), H.L[l].x.end(), 0.0);
+    std::copy(r, r + H.L[0].b.size(), H.L[0].b.begin());
+    for (int l = 0; l < nl - 1; ++l) {
+        Level& a = H.L[l];
+        or_rbgs(a.n[0], a.n[1], a.n[2], a.d.data(), a.ax.data(), a.ay.data(), a.az.data(), a.b.data(),
+                a.x.data(), pre);
+        or_restrict_residual(a.n[0], a.n[1], a.n[2], a.d.data(), a.ax.data(), a.ay.data(), a.az.data(),
+                             a.x.data(), a.b.data(), H.L[l + 1].b.data());
+    }
+    {
+        Level& a = H.L[nl - 1];
+        or_rbgs(a.n[0], a.n[1], a.n[2], a.d.data(), a.ax.data(), a.ay.data(), a.az.data(), a.b.data(),
+                a.x.data(), coarse);
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        Level& a = H.L[l];
+        or_prolong_add(a.n[0], a.n[1], a.n[2], H.L[l + 1].x.data(), a.x.data());
+        or_rbgs(a.n[0], a.n[1], a.n[2], a.d.data(), a.ax.data(), a.ay.data(), a.az.data(), a.b.data(),
+                a.x.data(), post);
+    }
+    std::copy(H.L[0].x.begin(), H.L[0].x.end(), z);
+}
+
+// serial dot and abs-max, S/blas_wrapper.h:13-43
+double sdot(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+double absmax(const std::vector<double>& a) {
+    double m = 0;
+    for (size_t i = 0; i < a.size(); ++i)
+        if (std::fabs(a[i]) > m) m = std::fabs(a[i]);
+    return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+// One V-cycle preconditioner application (for kernel-level parity checks).
+void or_vcycle(int ni, int nj, int nk, const double* d, const double* ax, const double* ay,
+               const double* az, const double* r, double* z, int pre, int post, int coarse) {
+    Hierarchy H;
+    build_hierarchy(H, ni, nj, nk, d, ax, ay, az);
+    vcycle(H, r, z, pre, post, coarse);
+}
+
+// Aggregation-AMG preconditioned CG with the reference's infinity-norm stop
+// |r|_inf <= tol * |b|_inf on the recursively updated residual.
+// S/AlgebraicMultigrid.h:189-301. Returns 1 converged, 0 not converged.
+// history (optional, length maxit+1): |r|_inf after each iteration (history[0] = |b|_inf).
+int or_amgpcg(int ni, int nj, int nk, const double* d, const double* ax, const double* ay,
+              const double* az, const double* rhs, double* result, double tol, int maxit,
+              double* residual_out, int* iterations_out, int pre, int post, int coarse,
+              double* history) {
+    const size_t n = (size_t)ni * nj * nk;
+    Hierarchy H;
+    build_hierarchy(H, ni, nj, nk, d, ax, ay, az);
+    std::vector<double> r(rhs, rhs + n), z(n), s(n), x(n, 0.0);
+    double res = absmax(r);
+    *residual_out = res;
+    if (history) history[0] = res;
+    if (res == 0) {
+        *iterations_out = 0;
+        std::fill(result, result + n, 0.0);
+        return 1;
+    }
+    double tol_abs = tol * res;
+    vcycle(H, r.data(), z.data(), pre, post, coarse);
+    double rho = sdot(z, r);
+    if (rho == 0 || rho != rho) {
+        *iterations_out = 0;
+        std::fill(result, result + n, 0.0);
+        return 0;
+    }
+    s = z;
+    int it;
+    for (it = 0; it < maxit; ++it) {
+        or_matvec(ni, nj, nk, d, ax, ay, az, s.data(), z.data());
+        double alpha = rho / sdot(s, z);
+#pragma omp parallel for
+        for (long long q = 0; q < (long long)n; ++q) {
+            x[q] += alpha * s[q];
+            r[q] += -alpha * z[q];
+        }
+        res = absmax(r);
+        if (history) history[it + 1] = res;
+        if (res <= tol_abs) {
+            *residual_out = res;
+            *iterations_out = it + 1;
+            std::copy(x.begin(), x.end(), result);
+            return 1;
+        }
+        vcycle(H, r.data(), z.data(), pre, post, coarse);
+        double rho_new = sdot(z, r);
+        double beta = rho_new / rho;
+#pragma omp parallel for
+        for (long long q = 0; q < (long long)n; ++q) s[q] = z[q] + beta * s[q];
+        rho = rho_new;
+    }
+    *residual_out = res;
+    *iterations_out = it;
+    std::copy(x.begin(), x.end(), result);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Pressure-gradient update. S/Smoke_solver3D.cpp:1126-1200 (sub=0: faces
+// 2..n-2 along the face axis, no dt) and S/sub_domain.cpp:745-819 (sub=1: faces
+// 1..n-1, multiplied by dt). Faces with weight 0 take the solid velocity.
+// ---------------------------------------------------------------------------
+void or_grad_apply(int ni, int nj, int nk, float h, float dt, int sub, const double* p,
+                   const float* uw, const float* vw, const float* ww, const float* us,
+                   const float* vs, const float* ws, float* u, float* v, float* w) {
+    const int lo = sub ? 1 : 2;
+    const size_t sl = (size_t)ni * nj;
+#pragma omp parallel for
+    for (int k = 1; k < nk - 1; ++k)
+        for (int j = 1; j < nj - 1; ++j)
+            for (int i = lo; i <= (sub ? ni - 1 : ni - 2); ++i) {
+                size_t f = lin(i, j, k, ni + 1, nj), c = lin(i, j, k, ni, nj);
+                if (uw[f] > 0) {
+                    float g = (float)(p[c] - p[c - 1]);
+                    u[f] -= sub ? dt * g / h : g / h;
+                } else {
+                    u[f] = us[f];
+                }
+            }
+#pragma omp parallel for
+    for (int k = 1; k < nk - 1; ++k)
+        for (int j = lo; j <= (sub ? nj - 1 : nj - 2); ++j)
+            for (int i = 1; i < ni - 1; ++i) {
+                size_t f = lin(i, j, k, ni, nj + 1), c = lin(i, j, k, ni, nj);
+                if (vw[f] > 0) {
+                    float g = (float)(p[c] - p[c - ni]);
+                    v[f] -= sub ? dt * g / h : g / h;
+                } else {
+                    v[f] = vs[f];
+                }
+            }
+#pragma omp parallel for
+    for (int k = lo; k <= (sub ? nk - 1 : nk - 2); ++k)
+        for (int j = 1; j < nj - 1; ++j)
+            for (int i = 1; i < ni - 1; ++i) {
+                size_t f = lin(i, j, k, ni, nj), c = lin(i, j, k, ni, nj);
+                if (ww[f] > 0) {
+                    float g = (float)(p[c] - p[c - sl]);
+                    w[f] -= sub ? dt * g / h : g / h;
+                } else {
+                    w[f] = ws[f];
+                }
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Samplers on a linear (x-fastest) nx x ny x nz grid with grid offset (ox,oy,oz)
+// and spacing h; index clamping as Buffer3D::at. Interpolation in FP64.
+// S/fluid_buffer3D.h:205-248 (at, sample_linear), 259-326 (sample_cubic),
+// 329-356 (sample_cube_lerp), 22-29 (cubic weights).
+// ---------------------------------------------------------------------------
+static inline float at_c(const float* f, int nx, int ny, int nz, int i, int j, int k) {
+    int ti = std::min(std::max(i, 0), nx - 1), tj = std::min(std::max(j, 0), ny - 1),
+        tk = std::min(std::max(k, 0), nz - 1);
+    return f[lin(ti, tj, tk, nx, ny)];
+}
+static inline double lerpd(double a, double b, double c) { return (1.0 - c) * a + c * b; }
+
+float or_sample_linear(const float* f, int nx, int ny, int nz, double h, double ox, double oy,
+                       double oz, double wx, double wy, double wz) {
+    double gx = wx / h + ox, gy = wy / h + oy, gz = wz / h + oz;
+    int i = (int)std::floor(gx), j = (int)std::floor(gy), k = (int)std::floor(gz);
+    double cx = gx - (double)i, cy = gy - (double)j, cz = gz - (double)k;
+    double v1 = lerpd(lerpd(at_c(f, nx, ny, nz, i, j, k), at_c(f, nx, ny, nz, i + 1, j, k), cx),
+                      lerpd(at_c(f, nx, ny, nz, i, j + 1, k), at_c(f, nx, ny, nz, i + 1, j + 1, k), cx), cy);
+    double v2 = lerpd(lerpd(at_c(f, nx, ny, nz, i, j, k + 1), at_c(f, nx, ny, nz, i + 1, j, k + 1), cx),
+                      lerpd(at_c(f, nx, ny, nz, i, j + 1, k + 1), at_c(f, nx, ny, nz, i + 1, j + 1, k + 1), cx),
+                      cy);
+    return (float)lerpd(v1, v2, cz);
+}
+
+// Cubic Lagrange weights on {-1,0,1,2}, evaluated in double as the sampler does.
+static inline double cubic_interp(double m1, double v0, double v1, double v2, double f) {
+    double f2 = f * f, f3 = f2 * f;
+    double wm1 = -(1. / 3) * f + (1. / 2) * f2 - (1. / 6) * f3;
+    double w0 = 1 - f2 + (1. / 2) * (f3 - f);
+    double w1 = f + (1. / 2) * (f2 - f3);
+    double w2 = (1. / 6) * (f3 - f);
+    return wm1 * m1 + w0 * v0 + w1 * v1 + w2 * v2;
+}
+
+float or_sample_cubic(const float* f, int nx, int ny, int nz, double h, double ox, double oy,
+                      double oz, double wx, double wy, double wz) {
+    double gx = wx / h + ox, gy = wy / h + oy, gz = wz / h + oz;
+    int i = (int)std::floor(gx), j = (int)std::floor(gy), k = (int)std::floor(gz);
+    double x = gx - (double)i, y = gy - (double)j, z = gz - (double)k;
+    double fz[4];
+    for (int dk = -1; dk <= 2; ++dk) {
+        double fy[4];
+        for (int dj = -1; dj <= 2; ++dj)
+            fy[dj + 1] = cubic_interp(at_c(f, nx, ny, nz, i - 1, j + dj, k + dk), at_c(f, nx, ny, nz, i, j + dj, k + dk),
+                                      at_c(f, nx, ny, nz, i + 1, j + dj, k + dk),
+                                      at_c(f, nx, ny, nz, i + 2, j + dj, k + dk), x);
+        fz[dk + 1] = cubic_interp(fy[0], fy[1], fy[2], fy[3], y);
+    }
+    return (float)cubic_interp(fz[0], fz[1], fz[2], fz[3], z);
+}
+
+float or_sample_cube_lerp(const float* f, int nx, int ny, int nz, double h, double ox, double oy,
+                          double oz, double wx, double wy, double wz, float* corners) {
+    double gx = wx / h + ox, gy = wy / h + oy, gz = wz / h + oz;
+    int i = (int)std::floor(gx), j = (int)std::floor(gy), k = (int)std::floor(gz);
+    for (int q = 0; q < 8; ++q)
+        corners[q] = at_c(f, nx, ny, nz, i + (q & 1), j + ((q >> 1) & 1), k + ((q >> 2) & 1));
+    return or_sample_linear(f, nx, ny, nz, h, ox, oy, oz, wx, wy, wz);
+}
+
+// ---------------------------------------------------------------------------
+// Semi-Lagrangian advection, RK2 midpoint backtrace.
+// mode 0: trilinear gather  -- SubDomainSolver::advect_field, S/sub_domain.cpp:191-235
+// mode 1: clamped cubic     -- SmokeSolver3D::advect_field_cubic_clamp,
+//                              S/Smoke_solver3D.cpp:657-707
+// Velocity grids u (nx+1,ny,nz; offset .5,0,0), v, w as in S/Smoke_solver3D.h:299-307.
+// field: fnx x fny x fnz with offsets (fox,foy,foz); output field_new same shape.
+// ---------------------------------------------------------------------------
+void or_advect_rk2(int nx, int ny, int nz, float h, float dt, int mode, const float* u,
+                   const float* v, const float* w, const float* field, int fnx, int fny, int fnz,
+                   double fox, double foy, double foz, float* out) {
+    const double hd = h;
+#pragma omp parallel for
+    for (int k = 0; k < fnz; ++k)
+        for (int j = 0; j < fny; ++j)
+            for (int i = 0; i < fnx; ++i) {
+                float wx = (float)(((double)(float)i - fox) * hd);
+                float wy = (float)(((double)(float)j - foy) * hd);
+                float wz = (float)(((double)(float)k - foz) * hd);
+                float uu = or_sample_linear(u, nx + 1, ny, nz, hd, 0.5, 0, 0, wx, wy, wz);
+                float vv = or_sample_linear(v, nx, ny + 1, nz, hd, 0, 0.5, 0, wx, wy, wz);
+                float ww = or_sample_linear(w, nx, ny, nz + 1, hd, 0, 0, 0.5, wx, wy, wz);
+                float px = (float)(wx - 0.5 * dt * uu), py = (float)(wy - 0.5 * dt * vv),
+                      pz = (float)(wz - 0.5 * dt * ww);
+                uu = or_sample_linear(u, nx + 1, ny, nz, hd, 0.5, 0, 0, px, py, pz);
+                vv = or_sample_linear(v, nx, ny + 1, nz, hd, 0, 0.5, 0, px, py, pz);
+                ww = or_sample_linear(w, nx, ny, nz + 1, hd, 0, 0, 0.5, px, py, pz);
+                if (mode == 0) {
+                    px = wx - dt * uu; py = wy - dt * vv; pz = wz - dt * ww;
+                } else {
+                    // advect_field_cubic_clamp: px = world_x - dt*u computed as float
+                    px = wx - dt * uu; py = wy - dt * vv; pz = wz - dt * ww;
+                }
+                float val;
+                if (mode == 0) {
+                    val = or_sample_linear(field, fnx, fny, fnz, hd, fox, foy, foz, px, py, pz);
+                } else {
+                    float cn[8];
+                    float lin_v = or_sample_cube_lerp(field, fnx, fny, fnz, hd, fox, foy, foz, px, py, pz, cn);
+                    float cub = or_sample_cubic(field, fnx, fny, fnz, hd, fox, foy, foz, px, py, pz);
+                    float mn = cn[0], mx = cn[0];
+                    for (int q = 1; q < 8; ++q) { mn = std::min(mn, cn[q]); mx = std::max(mx, cn[q]); }
+                    val = cub;
+                    if (cub <= mn || cub >= mx) val = lin_v;
+                }
+                out[lin(i, j, k, fnx, fny)] = val;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Particle binning: cell = trunc((p - bmin + 0.5h)/h) per axis (float ops,
+// truncating conversion, S/vec.h:37-39). clamp=0 drops out-of-range particles
+// (coarse grid, S/Smoke_solver3D.cpp:1327-1346); clamp=1 clamps them into the
+// grid (subdomain, S/sub_domain.cpp:1087-1107). Within a cell, particles keep
+// their input order (append order of the reference's per-cell lists).
+// Outputs: cell_start[ncell+1] (prefix offsets), order[...] particle ids; returns
+// the number of binned particles. pos is xyz-interleaved float.
+// ---------------------------------------------------------------------------
+int or_hash(int np, const float* pos, float h, int nx, int ny, int nz, int clamp, float bx,
+            float by, float bz, int* cell_key, int* cell_start, int* order) {
+    const float half = 0.5f * h;  // Vec3f(0.5,0.5,0.5)*_hx
+    const int ncell = nx * ny * nz;
+    std::vector<int> cnt(ncell + 1, 0);
+    for (int p = 0; p < np; ++p) {
+        float x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+        if (clamp) { x = x - bx; y = y - by; z = z - bz; }
+        x = x + half; y = y + half; z = z + half;
+        int i = (int)(x / h), j = (int)(y / h), k = (int)(z / h);
+        int key = -1;
+        if (clamp) {
+            i = clampv(i, 0, nx - 1); j = clampv(j, 0, ny - 1); k = clampv(k, 0, nz - 1);
+            key = k * nx * ny + j * nx + i;
+        } else if (i >= 0 && j >= 0 && k >= 0 && i < nx && j < ny && k < nz) {
+            key = k * nx * ny + j * nx + i;
+        }
+        cell_key[p] = key;
+        if (key >= 0) cnt[key + 1]++;
+    }
+    for (int c = 0; c < ncell; ++c) cnt[c + 1] += cnt[c];
+    std::memcpy(cell_start, cnt.data(), sizeof(int) * (ncell + 1));
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int p = 0; p < np; ++p)
+        if (cell_key[p] >= 0) order[fill[cell_key[p]]++] = p;
+    return cnt[ncell];
+}
+
+// ---------------------------------------------------------------------------
+// P2G gather for one MAC component. Each face sample sums tent-weighted
+// particle velocities over the (2l+1)^3 cells around its index and divides by
+// (1e-5 + sum w). S/Smoke_solver3D.cpp:1451-1510 and S/Smoke_solver3D.h:211-229.
+// Cells visited k-outer, i-inner; particles in binned order; float accumulation.
+// field/coef: fnx x fny x fnz with offsets (fox,foy,foz) (u: nx+1,ny,nz,.5,0,0).
+// sub=1 adds the subdomain's widened kernel near its boundary (l=4, support 4h
+// within 4 cells of a face, S/sub_domain.h:89-103, S/sub_domain.cpp:1209-1265)
+// and subtracts bmin from particle positions.
+// ---------------------------------------------------------------------------
+static inline float tent(float r) {
+    float res = 0;
+    if (r >= -1 && r < 0) res = 1 + r;
+    if (r >= 0 && r < 1) res = 1 - r;
+    return res;
+}
+
+void or_p2g(int np, const float* pos, const float* vel, int comp, float h, int nx, int ny, int nz,
+            const int* cell_start, const int* order, int fnx, int fny, int fnz, double fox,
+            double foy, double foz, int sub, float bx, float by, float bz, float* field,
+            float* coef) {
+    (void)np;
+#pragma omp parallel for
+    for (int k = 0; k < fnz; ++k)
+        for (int j = 0; j < fny; ++j)
+            for (int i = 0; i < fnx; ++i) {
+                float gx = (float)(((double)(float)i - fox) * (double)h);
+                float gy = (float)(((double)(float)j - foy) * (double)h);
+                float gz = (float)(((double)(float)k - foz) * (double)h);
+                float tw = 1e-5f, tv = 0;
+                int l = 1;
+                float hk = h;
+                if (sub) {
+                    int dist = std::min(std::min(std::min(i, nx - i), std::min(j, ny - j)), std::min(k, nz - k));
+                    if (dist <= 4) { l = 4; hk = 4.0f * h; }
+                }
+                for (int dk = -l; dk <= l; ++dk)
+                    for (int dj = -l; dj <= l; ++dj)
+                        for (int di = -l; di <= l; ++di) {
+                            int a = i + di, b = j + dj, c = k + dk;
+                            if (a < 0 || a >= nx || b < 0 || b >= ny || c < 0 || c >= nz) continue;
+                            int cell = c * nx * ny + b * nx + a;
+                            for (int q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
+                                int p = order[q];
+                                float px = pos[3 * p], py = pos[3 * p + 1], pz = pos[3 * p + 2];
+                                if (sub) { px = px - bx; py = py - by; pz = pz - bz; }
+                                float wgt = tent((px - gx) / hk) * tent((py - gy) / hk) * tent((pz - gz) / hk);
+                                tw += wgt;
+                                tv += wgt * vel[3 * p + comp];
+                            }
+                        }
+                size_t o = lin(i, j, k, fnx, fny);
+                coef[o] = tw;
+                field[o] = tv / tw;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// G2P, pure FLIP: vel += trilinear(u_new - u_old) at the particle position.
+// S/Smoke_solver3D.cpp:1298-1325 (getDelta) and 1398-1406 (particle_from_grid).
+// du/dv/dw are the delta grids (MAC shapes). flip_ratio blends with PIC when < 1
+// (PIC/FLIP mechanism of S/fluid_particle.h:297-336); flip_ratio=1 is the
+// reference's pure FLIP path and ignores u/v/w.
+// ---------------------------------------------------------------------------
+void or_g2p(int np, const float* pos, float* vel, float h, int nx, int ny, int nz,
+            const float* du, const float* dv, const float* dw, const float* u, const float* v,
+            const float* w, float flip_ratio, float bx, float by, float bz) {
+    const double hd = h;
+#pragma omp parallel for
+    for (int p = 0; p < np; ++p) {
+        float x = pos[3 * p] - bx, y = pos[3 * p + 1] - by, z = pos[3 * p + 2] - bz;
+        float a = or_sample_linear(du, nx + 1, ny, nz, hd, 0.5, 0, 0, x, y, z);
+        float b = or_sample_linear(dv, nx, ny + 1, nz, hd, 0, 0.5, 0, x, y, z);
+        float c = or_sample_linear(dw, nx, ny, nz + 1, hd, 0, 0, 0.5, x, y, z);
+        if (flip_ratio >= 1.0f) {
+            vel[3 * p] += a; vel[3 * p + 1] += b; vel[3 * p + 2] += c;
+        } else {
+            float pu = or_sample_linear(u, nx + 1, ny, nz, hd, 0.5, 0, 0, x, y, z);
+            float pv = or_sample_linear(v, nx, ny + 1, nz, hd, 0, 0.5, 0, x, y, z);
+            float pw = or_sample_linear(w, nx, ny, nz + 1, hd, 0, 0, 0.5, x, y, z);
+            vel[3 * p] = flip_ratio * (vel[3 * p] + a) + (1.0f - flip_ratio) * pu;
+            vel[3 * p + 1] = flip_ratio * (vel[3 * p + 1] + b) + (1.0f - flip_ratio) * pv;
+            vel[3 * p + 2] = flip_ratio * (vel[3 * p + 2] + c) + (1.0f - flip_ratio) * pw;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coarse/fine operators. S/coarse_fine_solver.cpp:73-108 (fine->coarse box
+// average with a face offset), 146-170 (coarse->fine trilinear), 182-230
+// (flux-matching face correction). Ratio N, fine spacing hf.
+// ---------------------------------------------------------------------------
+void or_fine_to_coarse(const float* fine, int fnx, int fny, int fnz, float* coarse, int cnx,
+                       int cny, int cnz, int N, int offx, int offy, int offz) {
+#pragma omp parallel for
+    for (int k = 0; k < cnz; ++k)
+        for (int j = 0; j < cny; ++j)
+            for (int i = 0; i < cnx; ++i) {
+                int count = 0;
+                float q = 0;
+                for (int a = std::max(0, i * N - offx); a < std::min(fnx, i * N + N - offx); ++a)
+                    for (int b = std::max(0, j * N - offy); b < std::min(fny, j * N + N - offy); ++b)
+                        for (int c = std::max(0, k * N - offz); c < std::min(fnz, k * N + N - offz); ++c) {
+                            ++count;
+                            q += fine[lin(a, b, c, fnx, fny)];
+                        }
+                coarse[lin(i, j, k, cnx, cny)] = count > 0 ? q / (float)count : 0;
+            }
+}
+
+void or_coarse_to_fine(float* fine, int fnx, int fny, int fnz, double fox, double foy, double foz,
+                       float hf, const float* coarse, int cnx, int cny, int cnz, double cox,
+                       double coy, double coz, double hc) {
+#pragma omp parallel for
+    for (int k = 0; k < fnz; ++k)
+        for (int j = 0; j < fny; ++j)
+            for (int i = 0; i < fnx; ++i) {
+                float px = ((float)i - (float)fox) * hf, py = ((float)j - (float)foy) * hf,
+                      pz = ((float)k - (float)foz) * hf;
+                fine[lin(i, j, k, fnx, fny)] = or_sample_linear(coarse, cnx, cny, cnz, hc, cox, coy, coz, px, py, pz);
+            }
+}
+
+void or_set_face_boundary(float* fine, int fnx, int fny, int fnz, const float* coarse, int cnx,
+                          int cny, int cnz, int N, int offx, int offy, int offz) {
+    // Each coarse cell owns a disjoint fine block, so the loop is write-disjoint.
+#pragma omp parallel for
+    for (int k = 0; k < cnz; ++k)
+        for (int j = 0; j < cny; ++j)
+            for (int i = 0; i < cnx; ++i) {
+                float q = 0;
+                for (int a = std::max(0, i * N); a < std::min(fnx, i * N + offx); ++a)
+                    for (int b = std::max(0, j * N); b < std::min(fny, j * N + offy); ++b)
+                        for (int c = std::max(0, k * N); c < std::min(fnz, k * N + offz); ++c) q += fine[lin(a, b, c, fnx, fny)];
+                float diff = coarse[lin(i, j, k, cnx, cny)] - q / (float)(N * N);
+                for (int a = std::max(0, i * N); a < std::min(fnx, i * N + offx); ++a)
+                    for (int b = std::max(0, j * N); b < std::min(fny, j * N + offy); ++b)
+                        for (int c = std::max(0, k * N); c < std::min(fnz, k * N + offz); ++c) fine[lin(a, b, c, fnx, fny)] += diff;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// Storage helpers: the reference Buffer3D 8^3-brick layout.
+// idx = ((K*bx*by + J*bx + I) << 9) + (kk << 6) + (jj << 3) + ii, S/fluid_buffer3D.h:186-202.
+// ---------------------------------------------------------------------------
+void or_to_blocked(const float* src, int nx, int ny, int nz, float* dst) {
+    int bx = (nx + 7) / 8, by = (ny + 7) / 8, bz = (nz + 7) / 8;
+    std::memset(dst, 0, sizeof(float) * (size_t)bx * by * bz * 512);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t idx = (((size_t)(k >> 3) * bx * by + (size_t)(j >> 3) * bx + (i >> 3)) << 9) +
+                             ((k & 7) << 6) + ((j & 7) << 3) + (i & 7);
+                dst[idx] = src[lin(i, j, k, nx, ny)];
+            }
+}
+
+unsigned int or_randhash(unsigned int seed) {
+    // S/util.h:238-246
+    unsigned int i = (seed ^ 0xA3C59AC3u) * 2654435769u;
+    i ^= (i >> 16);
+    i *= 2654435769u;
+    i ^= (i >> 16);
+    i *= 2654435769u;
+    return i;
+}
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

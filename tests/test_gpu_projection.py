@@ -1,0 +1,199 @@
+"""GPU parity of the pressure projection (HIP, via the C ABI) against the oracle.
+
+Bars: face weights bit-exact; assembled operator within FP32 rounding of the
+oracle's FP64 values; one V-cycle and the full PCG solve within 1e-5 relative
+L2 of the oracle (same iteration count, +-1 for FP32 vs FP64 round-off), and the
+returned pressure's FP64 true residual at the requested tolerance level.
+"""
+import numpy as np
+import pytest
+
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def oracle_iterate(O, d, ax, ay, az, rhs, iterations):
+    """The oracle's PCG iterate after exactly `iterations` iterations (tol 0)."""
+    return O.amgpcg(d, ax, ay, az, rhs, tol=0.0, maxit=iterations)["x"]
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from oracle import oracle as O
+    from vflip_amd import api
+    return O, api
+
+
+def build_problem(n, seed, O):
+    pb = scenes.projection_problem(n, seed)
+    uw, vw, ww = O.face_weights(pb["phi"])
+    pb.update(uw=uw, vw=vw, ww=ww)
+    return pb
+
+
+@pytest.mark.parametrize("n,seed", [(24, 1), (33, 2), (64, 3)])
+def test_face_weights_bitexact(cuda, mods, n, seed):
+    O, api = mods
+    pb = scenes.projection_problem(n, seed)
+    print(f"face weights n={n} seed={seed}", flush=True)
+    ref = O.face_weights(pb["phi"])
+    got = api.face_weights(_t(pb["phi"]))
+    for r, g in zip(ref, got):
+        assert np.array_equal(r.view(np.uint32), _np(g).view(np.uint32))
+
+
+@pytest.mark.parametrize("n,sub", [(32, False), (31, True)])
+def test_assembly(cuda, mods, n, sub):
+    O, api = mods
+    pb = build_problem(n, 5, O)
+    dt = np.float32(0.01) if sub else np.float32(1.0)
+    d, ax, ay, az, rhs = O.assemble(pb["h"], pb["u"], pb["v"], pb["w"], pb["uw"], pb["vw"], pb["ww"],
+                                    pb["us"], pb["vs"], pb["ws"], dt=dt, sub=sub)
+    S = api.Solver(n, n, n)
+    rhs_g = _t(np.zeros((n, n, n), np.float32))
+    args = [_t(pb[k]) for k in ("u", "v", "w", "uw", "vw", "ww", "us", "vs", "ws")]
+    S.assemble(api.SUB if sub else api.COARSE, float(pb["h"]), float(dt), *args, rhs_g)
+    gd, gx, gy, gz = S.get_level(0)
+    for ref, got in ((d, gd), (ax, gx), (ay, gy), (az, gz), (rhs, rhs_g)):
+        got = _np(got).astype(np.float64)
+        # FP64 accumulation rounded once to FP32
+        assert np.all(np.abs(got - ref) <= 1.2e-7 * np.abs(ref) + 1e-30)
+
+
+@pytest.mark.parametrize("n", [40, 64])
+def test_coarse_levels(cuda, mods, n):
+    O, api = mods
+    pb = build_problem(n, 7, O)
+    d, ax, ay, az, rhs = O.assemble(pb["h"], pb["u"], pb["v"], pb["w"], pb["uw"], pb["vw"], pb["ww"],
+                                    pb["us"], pb["vs"], pb["ws"])
+    S = api.Solver(n, n, n)
+    S.set_matrix(*[_t(a.astype(np.float32)) for a in (d, ax, ay, az)])
+    S.setup()
+    levels = S.levels()
+    assert levels == O.level_dims(n, n, n)
+    cur = (d, ax, ay, az)
+    for l in range(1, len(levels)):
+        cur = O.coarsen(*cur)
+        got = S.get_level(l)
+        for ref, g in zip(cur, got):
+            g = _np(g).astype(np.float64)
+            scale = np.abs(cur[0]).max()
+            assert np.abs(g - ref).max() <= 2e-6 * scale
+
+
+def _solver_with_matrix(api, O, pb, n, **opts):
+    d, ax, ay, az, rhs = O.assemble(pb["h"], pb["u"], pb["v"], pb["w"], pb["uw"], pb["vw"], pb["ww"],
+                                    pb["us"], pb["vs"], pb["ws"])
+    S = api.Solver(n, n, n, **opts)
+    S.set_matrix(*[_t(a.astype(np.float32)) for a in (d, ax, ay, az)])
+    return S, (d, ax, ay, az, rhs)
+
+
+@pytest.mark.parametrize("n", [32, 48])
+def test_vcycle(cuda, mods, n):
+    O, api = mods
+    pb = build_problem(n, 11, O)
+    S, (d, ax, ay, az, rhs) = _solver_with_matrix(api, O, pb, n)
+    z_ref = O.vcycle(d, ax, ay, az, rhs)
+    z = _t(np.zeros((n, n, n), np.float32))
+    S.vcycle(_t(rhs.astype(np.float32)), z)
+    assert rel_l2(_np(z), z_ref) < 1e-5
+
+
+@pytest.mark.parametrize("n,seed", [(32, 1), (64, 2), (50, 3)])
+def test_pcg_parity(cuda, mods, n, seed):
+    O, api = mods
+    print(f"pcg parity n={n} seed={seed}", flush=True)
+    pb = build_problem(n, seed, O)
+    S, (d, ax, ay, az, rhs) = _solver_with_matrix(api, O, pb, n)
+    ref = O.amgpcg(d, ax, ay, az, rhs, tol=1e-6, maxit=50)
+    x = _t(np.zeros((n, n, n), np.float32))
+    st = S.solve(_t(rhs.astype(np.float32)), x)
+    assert ref["ok"] and st["status"] == 0
+    assert abs(st["iterations"] - ref["iterations"]) <= 1
+    # same algorithm, same iteration count -> same iterate up to FP32 round-off
+    assert rel_l2(_np(x), oracle_iterate(O, d, ax, ay, az, rhs, st["iterations"])) < 1e-5
+    # FP64 residual of the GPU answer against the FP64 operator
+    r = rhs - O.matvec(d, ax, ay, az, _np(x).astype(np.float64))
+    assert np.abs(r).max() <= 5e-6 * np.abs(rhs).max()
+
+
+def test_pcg_mixed(cuda, mods):
+    O, api = mods
+    n = 64
+    pb = build_problem(n, 4, O)
+    S, (d, ax, ay, az, rhs) = _solver_with_matrix(api, O, pb, n, precision=api.MIXED)
+    ref = O.amgpcg(d, ax, ay, az, rhs, tol=1e-6, maxit=50)
+    x = _t(np.zeros((n, n, n), np.float32))
+    st = S.solve(_t(rhs.astype(np.float32)), x)
+    assert st["status"] == 0
+    assert st["iterations"] <= ref["iterations"] + 2
+    assert st["true_residual"] < 5e-6
+    assert rel_l2(_np(x), oracle_iterate(O, d, ax, ay, az, rhs, st["iterations"])) < 1e-5
+
+
+def test_zero_rhs_and_nonconvergence(cuda, mods):
+    O, api = mods
+    n = 24
+    pb = build_problem(n, 9, O)
+    S, (d, ax, ay, az, rhs) = _solver_with_matrix(api, O, pb, n)
+    x = _t(np.ones((n, n, n), np.float32))
+    st = S.solve(_t(np.zeros((n, n, n), np.float32)), x)
+    assert st["status"] == 0 and st["iterations"] == 0 and float(x.abs().max()) == 0.0
+    S.set_opts(max_iterations=1)
+    st = S.solve(_t(rhs.astype(np.float32)), x)
+    ref = O.amgpcg(d, ax, ay, az, rhs, tol=1e-6, maxit=1)
+    assert st["status"] == 1 and not ref["ok"] and st["iterations"] == 1
+
+
+def test_project_matches_oracle(cuda, mods):
+    O, api = mods
+    n = 48
+    pb = build_problem(n, 13, O)
+    h = float(pb["h"])
+    d, ax, ay, az, rhs = O.assemble(pb["h"], pb["u"], pb["v"], pb["w"], pb["uw"], pb["vw"], pb["ww"],
+                                    pb["us"], pb["vs"], pb["ws"])
+    ref = O.amgpcg(d, ax, ay, az, rhs)
+    S = api.Solver(n, n, n)
+    g = {k: _t(pb[k]) for k in ("u", "v", "w", "uw", "vw", "ww", "us", "vs", "ws")}
+    rhs_g = _t(np.zeros((n, n, n), np.float32))
+    p = _t(np.zeros((n, n, n), np.float32))
+    st = S.project(api.COARSE, h, 1.0, g["u"], g["v"], g["w"], g["uw"], g["vw"], g["ww"], g["us"], g["vs"],
+                   g["ws"], rhs_g, p)
+    assert st["status"] == 0 and abs(st["iterations"] - ref["iterations"]) <= 1
+    p_ref = oracle_iterate(O, d, ax, ay, az, rhs, st["iterations"])
+    assert rel_l2(_np(p), p_ref) < 1e-5
+    ru, rv, rw = O.grad_apply(pb["h"], p_ref, pb["uw"], pb["vw"], pb["ww"], pb["us"], pb["vs"], pb["ws"],
+                              pb["u"], pb["v"], pb["w"])
+    for a, b in ((g["u"], ru), (g["v"], rv), (g["w"], rw)):
+        assert rel_l2(_np(a), b) < 1e-5
+
+
+def test_host_dropin(cuda, mods):
+    O, api = mods
+    n = 32
+    pb = build_problem(n, 21, O)
+    d, ax, ay, az, rhs = O.assemble(pb["h"], pb["u"], pb["v"], pb["w"], pb["uw"], pb["vw"], pb["ww"],
+                                    pb["us"], pb["vs"], pb["ws"])
+    ref = O.amgpcg(d, ax, ay, az, rhs)
+    rc, x, res, its = api.amgpcg_solve_host(d, ax, ay, az, rhs)
+    assert rc == 0 and abs(its - ref["iterations"]) <= 1
+    assert res <= 1e-6 * np.abs(rhs).max()
+    assert rel_l2(x, oracle_iterate(O, d, ax, ay, az, rhs, its)) < 1e-5

@@ -1,0 +1,107 @@
+"""ctypes loader for libvflip.so, declaring every entry point of include/vflip.h.
+
+The library is the product path: there is no CPU fallback. Loading fails with
+an error when the HIP build is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvflip.so")
+
+c_int, c_float, c_double, c_void_p = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p
+P = ctypes.c_void_p  # device or host pointer
+PI = ctypes.POINTER(ctypes.c_int)
+PD = ctypes.POINTER(ctypes.c_double)
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [
+        ("tolerance", c_double),
+        ("max_iterations", c_int),
+        ("precision", c_int),
+        ("pre_sweeps", c_int),
+        ("post_sweeps", c_int),
+        ("coarse_sweeps", c_int),
+        ("refine_every", c_int),
+        ("warm_start", c_int),
+    ]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", c_int),
+        ("converged", c_int),
+        ("residual", c_double),
+        ("rel_residual", c_double),
+        ("true_residual", c_double),
+        ("ms_setup", c_double),
+        ("ms_solve", c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "vflip_last_error": (ctypes.c_char_p, []),
+    "vflip_version": (ctypes.c_char_p, []),
+    "vflip_device_count": (c_int, []),
+    "vflip_default_opts": (None, [ctypes.POINTER(SolveOpts)]),
+    "vflip_solver_create": (c_int, [c_int, c_int, c_int, ctypes.POINTER(SolveOpts), ctypes.POINTER(c_void_p)]),
+    "vflip_solver_destroy": (c_int, [P]),
+    "vflip_solver_set_opts": (c_int, [P, ctypes.POINTER(SolveOpts)]),
+    "vflip_solver_set_stream": (c_int, [P, P]),
+    "vflip_solver_levels": (c_int, [P]),
+    "vflip_solver_level_dims": (c_int, [P, c_int, PI, PI, PI]),
+    "vflip_solver_set_matrix": (c_int, [P, P, P, P, P, P]),
+    "vflip_solver_setup": (c_int, [P]),
+    "vflip_solver_solve": (c_int, [P, P, P, ctypes.POINTER(SolveStats)]),
+    "vflip_solver_vcycle": (c_int, [P, P, P]),
+    "vflip_solver_matvec": (c_int, [P, P, P]),
+    "vflip_solver_get_level": (c_int, [P, c_int, P, P, P, P]),
+    "vflip_amgpcg_solve_host": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P, c_double, c_int, PD, PI,
+                                        ctypes.POINTER(SolveOpts)]),
+    "vflip_face_weights": (c_int, [c_int, c_int, c_int, P, P, P, P, P]),
+    "vflip_inflow_bc": (c_int, [c_int, c_int, c_int, P, c_float, P, P, P, P]),
+    "vflip_assemble": (c_int, [P, c_int, c_float, c_float] + [P] * 10),
+    "vflip_grad_apply": (c_int, [c_int, c_int, c_int, c_int, c_float, c_float] + [P] * 10 + [P]),
+    "vflip_project": (c_int, [P, c_int, c_float, c_float] + [P] * 11 + [ctypes.POINTER(SolveStats)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libvflip.so; raises if the HIP build is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} not found: build the HIP library first (python -m vflip_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(SIGNATURES)
+
+
+class VflipError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    """Raises VflipError for negative status codes; returns rc otherwise."""
+    if rc < 0:
+        msg = lib().vflip_last_error().decode(errors="replace")
+        raise VflipError(f"{what} failed ({rc}): {msg}")
+    return rc

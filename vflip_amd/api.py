@@ -1,0 +1,157 @@
+"""Thin Python view of the C ABI over torch device tensors.
+
+torch only provides device memory and the current stream here; every
+computation runs in libvflip.so's HIP kernels. Arrays are torch tensors shaped
+(nk, nj, ni) (z, y, x) so that the memory layout is the ABI's linear x-fastest.
+"""
+import ctypes
+
+import torch
+
+from ._lib import SolveOpts, SolveStats, check, lib
+
+FP32 = 0
+MIXED = 1
+COARSE = 0
+SUB = 1
+
+
+def _ptr(t, dtype=torch.float32, numel=None, name="array"):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def default_opts(**kw):
+    o = SolveOpts()
+    lib().vflip_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Solver:
+    """Owns a vflip_solver (MG hierarchy + PCG workspace) for an ni x nj x nk grid."""
+
+    def __init__(self, ni, nj, nk, **opts):
+        self.ni, self.nj, self.nk = ni, nj, nk
+        self.n = ni * nj * nk
+        self.opts = default_opts(**opts)
+        h = ctypes.c_void_p()
+        check(lib().vflip_solver_create(ni, nj, nk, ctypes.byref(self.opts), ctypes.byref(h)), "solver_create")
+        self.h = h
+        check(lib().vflip_solver_set_stream(self.h, _stream()), "set_stream")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vflip_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            setattr(self.opts, k, v)
+        check(lib().vflip_solver_set_opts(self.h, ctypes.byref(self.opts)), "set_opts")
+
+    def levels(self):
+        out = []
+        for l in range(lib().vflip_solver_levels(self.h)):
+            a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            check(lib().vflip_solver_level_dims(self.h, l, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+            out.append((a.value, b.value, c.value))
+        return out
+
+    def set_matrix(self, d, ax, ay, az, leak=None):
+        n = self.n
+        check(lib().vflip_solver_set_matrix(self.h, _ptr(d, numel=n), _ptr(ax, numel=n), _ptr(ay, numel=n),
+                                            _ptr(az, numel=n), _ptr(leak, numel=n)), "set_matrix")
+
+    def setup(self):
+        check(lib().vflip_solver_setup(self.h), "setup")
+
+    def solve(self, rhs, x):
+        st = SolveStats()
+        rc = check(lib().vflip_solver_solve(self.h, _ptr(rhs, numel=self.n, name="rhs"),
+                                            _ptr(x, numel=self.n, name="x"), ctypes.byref(st)), "solve")
+        d = st.as_dict()
+        d["status"] = rc
+        return d
+
+    def vcycle(self, r, z):
+        check(lib().vflip_solver_vcycle(self.h, _ptr(r, numel=self.n), _ptr(z, numel=self.n)), "vcycle")
+
+    def matvec(self, x, y):
+        check(lib().vflip_solver_matvec(self.h, _ptr(x, numel=self.n), _ptr(y, numel=self.n)), "matvec")
+
+    def get_level(self, l):
+        a, b, c = self.levels()[l]
+        outs = [torch.empty((c, b, a), device="cuda", dtype=torch.float32) for _ in range(4)]
+        check(lib().vflip_solver_get_level(self.h, l, *[_ptr(o) for o in outs]), "get_level")
+        return outs
+
+    def assemble(self, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs):
+        check(lib().vflip_assemble(self.h, variant, h, dt, *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
+                                   _ptr(rhs, numel=self.n)), "assemble")
+
+    def project(self, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
+        st = SolveStats()
+        rc = check(lib().vflip_project(self.h, variant, h, dt,
+                                       *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
+                                       _ptr(rhs, numel=self.n), _ptr(p, numel=self.n), ctypes.byref(st)),
+                   "project")
+        d = st.as_dict()
+        d["status"] = rc
+        return d
+
+
+def face_weights(phi):
+    nk, nj, ni = (s - 1 for s in phi.shape)
+    uw = torch.empty((nk, nj, ni + 1), device=phi.device, dtype=torch.float32)
+    vw = torch.empty((nk, nj + 1, ni), device=phi.device, dtype=torch.float32)
+    ww = torch.empty((nk + 1, nj, ni), device=phi.device, dtype=torch.float32)
+    check(lib().vflip_face_weights(ni, nj, nk, _ptr(phi), _ptr(uw), _ptr(vw), _ptr(ww), _stream()), "face_weights")
+    return uw, vw, ww
+
+
+def inflow_bc(bdesc, u_in, u, v, w):
+    nk, nj, ni = bdesc.shape
+    check(lib().vflip_inflow_bc(ni, nj, nk, _ptr(bdesc, torch.int8), u_in, _ptr(u), _ptr(v), _ptr(w), _stream()),
+          "inflow_bc")
+
+
+def grad_apply(variant, h, dt, p, uw, vw, ww, us, vs, ws, u, v, w):
+    nk, nj, ni = p.shape
+    check(lib().vflip_grad_apply(ni, nj, nk, variant, h, dt, _ptr(p), *[_ptr(a) for a in (uw, vw, ww, us, vs, ws)],
+                                 _ptr(u), _ptr(v), _ptr(w), _stream()), "grad_apply")
+
+
+def amgpcg_solve_host(diag, ax, ay, az, rhs, tol=1e-6, maxit=50, precision=FP32):
+    """AMGPCGSolve drop-in on host float64 numpy arrays shaped (nk, nj, ni)."""
+    import numpy as np
+    nk, nj, ni = diag.shape
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (diag, ax, ay, az, rhs)]
+    res = np.zeros_like(arrs[0])
+    r = ctypes.c_double()
+    it = ctypes.c_int()
+    o = default_opts(precision=precision)
+    rc = check(lib().vflip_amgpcg_solve_host(ni, nj, nk, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs],
+                                             res.ctypes.data_as(ctypes.c_void_p), tol, maxit, ctypes.byref(r),
+                                             ctypes.byref(it), ctypes.byref(o)), "amgpcg_solve_host")
+    return rc, res, r.value, it.value

@@ -1,0 +1,95 @@
+// Shared device/host helpers of libvflip (internal header).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/vflip.h"
+
+namespace vflip {
+
+void set_error(const char* fmt, ...);
+
+#define VF_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t _e = (call);                                                             \
+        if (_e != hipSuccess) {                                                             \
+            ::vflip::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(_e)); \
+            return VFLIP_ERR_HIP;                                                           \
+        }                                                                                   \
+    } while (0)
+
+#define VF_CHECK_LAUNCH()                                                                   \
+    do {                                                                                    \
+        hipError_t _e = hipGetLastError();                                                  \
+        if (_e != hipSuccess) {                                                             \
+            ::vflip::set_error("%s:%d kernel launch: %s", __FILE__, __LINE__, hipGetErrorString(_e)); \
+            return VFLIP_ERR_HIP;                                                           \
+        }                                                                                   \
+    } while (0)
+
+#define VF_ARG(cond, msg)                                                                   \
+    do {                                                                                    \
+        if (!(cond)) {                                                                      \
+            ::vflip::set_error("invalid argument: %s", msg);                                \
+            return VFLIP_ERR_ARG;                                                           \
+        }                                                                                   \
+    } while (0)
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(size_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+__host__ __device__ inline size_t lin3(int i, int j, int k, int nx, int ny) {
+    return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k);
+}
+
+// Wave64 reductions (CDNA wavefront = 64 lanes).
+template <typename T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename T> __device__ inline T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+
+// Block reduction for blockDim.x == kBlock (4 waves); result valid in thread 0.
+template <typename T> __device__ inline T block_sum(T v) {
+    __shared__ T red[kBlock / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < kBlock / 64; ++q) r += red[q];
+    }
+    __syncthreads();
+    return r;
+}
+template <typename T> __device__ inline T block_max(T v) {
+    __shared__ T red[kBlock / 64];
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) {
+        r = red[0];
+        for (int q = 1; q < kBlock / 64; ++q) r = r > red[q] ? r : red[q];
+    }
+    __syncthreads();
+    return r;
+}
+
+}  // namespace vflip

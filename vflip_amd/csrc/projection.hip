@@ -1,0 +1,304 @@
+// Pressure-projection kernels around the solver: cut-cell face weights, inflow
+// boundary, matrix-free assembly and the gradient update.
+//
+// Face weights and cell indices must match the reference bit for bit, so this
+// file is compiled without FMA contraction and with IEEE fp32 division.
+#pragma clang fp contract(off)
+
+#include "common.h"
+
+namespace vflip {
+
+// fraction_inside for a segment, S/levelset_util.cpp:4-13
+__device__ inline float frac2(float l, float r) {
+    if (l < 0 && r < 0) return 1.f;
+    if (l < 0 && r >= 0) return l / (l - r);
+    if (l >= 0 && r < 0) return r / (r - l);
+    return 0.f;
+}
+
+// fraction_inside for a square, S/levelset_util.cpp:23-103. Corners are walked
+// bl, br, tr, tl and rotated until the case's anchor corner comes first.
+__device__ inline float frac4(float bl, float br, float tl, float tr) {
+    int inside = (bl <= 0 ? 1 : 0) + (tl <= 0 ? 1 : 0) + (br <= 0 ? 1 : 0) + (tr <= 0 ? 1 : 0);
+    float c0 = bl, c1 = br, c2 = tr, c3 = tl;
+    auto rot = [&]() {
+        float t = c0;
+        c0 = c1; c1 = c2; c2 = c3; c3 = t;
+    };
+    if (inside == 4) return 1.f;
+    if (inside == 3) {
+        for (int q = 0; q < 4 && c0 < 0; ++q) rot();
+        float s0 = 1 - frac2(c0, c3);
+        float s1 = 1 - frac2(c0, c1);
+        return 1 - 0.5f * s0 * s1;
+    }
+    if (inside == 2) {
+        for (int q = 0; q < 4 && (c0 >= 0 || !(c1 < 0 || c2 < 0)); ++q) rot();
+        if (c1 < 0) {
+            float sl = frac2(c0, c3);
+            float sr = frac2(c1, c2);
+            return 0.5f * (sl + sr);
+        }
+        float mid = 0.25f * (c0 + c1 + c2 + c3);
+        if (mid < 0) {
+            float area = 0;
+            float s1 = 1 - frac2(c0, c3);
+            float s3 = 1 - frac2(c2, c3);
+            area += 0.5f * s1 * s3;
+            float s2 = 1 - frac2(c2, c1);
+            float s0 = 1 - frac2(c0, c1);
+            area += 0.5f * s0 * s2;
+            return 1 - area;
+        }
+        float area = 0;
+        float s0 = frac2(c0, c1);
+        float s1 = frac2(c0, c3);
+        area += 0.5f * s0 * s1;
+        float s2 = frac2(c2, c1);
+        float s3 = frac2(c2, c3);
+        area += 0.5f * s2 * s3;
+        return area;
+    }
+    if (inside == 1) {
+        for (int q = 0; q < 4 && c0 >= 0; ++q) rot();
+        float s0 = frac2(c0, c3);
+        float s1 = frac2(c0, c1);
+        return 0.5f * s0 * s1;
+    }
+    return 0.f;
+}
+
+__device__ inline float clamp01(float a) { return a < 0.f ? 0.f : (a > 1.f ? 1.f : a); }
+
+// axis 0: u faces (ni+1,nj,nk), corners (i,j,k),(i,j+1,k),(i,j,k+1),(i,j+1,k+1)
+// axis 1: v faces (ni,nj+1,nk), corners (i,j,k),(i,j,k+1),(i+1,j,k),(i+1,j,k+1)
+// axis 2: w faces (ni,nj,nk+1), corners (i,j,k),(i,j+1,k),(i+1,j,k),(i+1,j+1,k)
+// S/Smoke_solver3D.cpp:882-941
+__global__ __launch_bounds__(kBlock) void k_face_weights(int axis, int ni, int nj, int nk, const float* __restrict__ phi,
+                                                         float* __restrict__ out) {
+    const int fx = ni + (axis == 0), fy = nj + (axis == 1), fz = nk + (axis == 2);
+    const size_t nf = (size_t)fx * fy * fz;
+    size_t f = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    int i = (int)(f % fx);
+    size_t t = f / fx;
+    int j = (int)(t % fy), k = (int)(t / fy);
+    const int px = ni + 1, py = nj + 1;
+    auto P = [&](int a, int b, int c) { return phi[lin3(a, b, c, px, py)]; };
+    float fr;
+    if (axis == 0) fr = frac4(P(i, j, k), P(i, j + 1, k), P(i, j, k + 1), P(i, j + 1, k + 1));
+    else if (axis == 1) fr = frac4(P(i, j, k), P(i, j, k + 1), P(i + 1, j, k), P(i + 1, j, k + 1));
+    else fr = frac4(P(i, j, k), P(i, j + 1, k), P(i + 1, j, k), P(i + 1, j + 1, k));
+    out[f] = clamp01(1 - fr);
+}
+
+// S/Smoke_solver3D.cpp:964-980
+__global__ __launch_bounds__(kBlock) void k_inflow(int ni, int nj, int nk, const int8_t* __restrict__ bdesc, float uin,
+                                                   float* __restrict__ u, float* __restrict__ v, float* __restrict__ w) {
+    const size_t n = (size_t)ni * nj * nk;
+    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n || bdesc[c] != 1) return;
+    int i = (int)(c % ni);
+    size_t t = c / ni;
+    int j = (int)(t % nj), k = (int)(t / nj);
+    u[lin3(i, j, k, ni + 1, nj)] = uin;
+    u[lin3(i + 1, j, k, ni + 1, nj)] = uin;
+    v[lin3(i, j, k, ni, nj + 1)] = 0.f;
+    v[lin3(i, j + 1, k, ni, nj + 1)] = 0.f;
+    w[lin3(i, j, k, ni, nj)] = 0.f;
+    w[lin3(i, j, k + 1, ni, nj)] = 0.f;
+}
+
+// Matrix-free assembly over interior cells 1..n-2 (ring rows empty).
+// S/Smoke_solver3D.cpp:986-1111; sub variant S/sub_domain.cpp:612-729.
+// Writes diag/ax/ay/az/leak (FP32, from FP64 accumulation of the FP32 terms)
+// and rhs (FP64 accumulation rounded to FP32).
+__global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, float h, float dt, int sub,
+                                                     const float* __restrict__ u, const float* __restrict__ v,
+                                                     const float* __restrict__ w, const float* __restrict__ uw,
+                                                     const float* __restrict__ vw, const float* __restrict__ ww,
+                                                     const float* __restrict__ us, const float* __restrict__ vs,
+                                                     const float* __restrict__ ws, float* __restrict__ D,
+                                                     float* __restrict__ AX, float* __restrict__ AY,
+                                                     float* __restrict__ AZ, float* __restrict__ LK,
+                                                     float* __restrict__ rhs) {
+    const size_t n = (size_t)ni * nj * nk;
+    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    int i = (int)(c % ni);
+    size_t t = c / ni;
+    int j = (int)(t % nj), k = (int)(t / nj);
+    if (!(i >= 1 && i < ni - 1 && j >= 1 && j < nj - 1 && k >= 1 && k < nk - 1)) {
+        D[c] = 0.f; AX[c] = 0.f; AY[c] = 0.f; AZ[c] = 0.f; LK[c] = 0.f; rhs[c] = 0.f;
+        return;
+    }
+    const float h2 = h * h;
+    const size_t fxp = lin3(i + 1, j, k, ni + 1, nj), fxm = lin3(i, j, k, ni + 1, nj);
+    const size_t fyp = lin3(i, j + 1, k, ni, nj + 1), fym = lin3(i, j, k, ni, nj + 1);
+    const size_t fzp = lin3(i, j, k + 1, ni, nj), fzm = lin3(i, j, k, ni, nj);
+    auto term = [&](float wgt) { return sub ? wgt * dt / h2 : wgt / h2; };
+    const float txp = term(uw[fxp]), txm = term(uw[fxm]);
+    const float typ = term(vw[fyp]), tym = term(vw[fym]);
+    const float tzp = term(ww[fzp]), tzm = term(ww[fzm]);
+    double d = 0, r = 0, lk = 0;
+    d += txp; r -= (double)(uw[fxp] * u[fxp] / h);
+    d += txm; r += (double)(uw[fxm] * u[fxm] / h);
+    d += typ; r -= (double)(vw[fyp] * v[fyp] / h);
+    d += tym; r += (double)(vw[fym] * v[fym] / h);
+    d += tzp; r -= (double)(ww[fzp] * w[fzp] / h);
+    d += tzm; r += (double)(ww[fzm] * w[fzm] / h);
+    // terms whose neighbour is a ring cell carry no off-diagonal (Dirichlet leak)
+    if (i == ni - 2) lk += txp;
+    if (i == 1) lk += txm;
+    if (j == nj - 2) lk += typ;
+    if (j == 1) lk += tym;
+    if (k == nk - 2) lk += tzp;
+    if (k == 1) lk += tzm;
+    if (d > 0) {
+        float s = (uw[fxp] - 1.0f) * us[fxp];
+        s = s - (uw[fxm] - 1.0f) * us[fxm];
+        s = s + (vw[fyp] - 1.0f) * vs[fyp];
+        s = s - (vw[fym] - 1.0f) * vs[fym];
+        s = s + (ww[fzp] - 1.0f) * ws[fzp];
+        s = s - (ww[fzm] - 1.0f) * ws[fzm];
+        if (sub) r += (double)((1 / h) * s);
+        else r += 1.0 / (double)h * (double)s;
+    }
+    D[c] = (float)d;
+    AX[c] = (i < ni - 2) ? -txp : 0.f;
+    AY[c] = (j < nj - 2) ? -typ : 0.f;
+    AZ[c] = (k < nk - 2) ? -tzp : 0.f;
+    LK[c] = (float)lk;
+    rhs[c] = (float)r;
+}
+
+// u -= grad p, S/Smoke_solver3D.cpp:1126-1200 and S/sub_domain.cpp:745-819.
+__global__ __launch_bounds__(kBlock) void k_grad(int axis, int ni, int nj, int nk, int sub, float h, float dt,
+                                                 const float* __restrict__ p, const float* __restrict__ wgt,
+                                                 const float* __restrict__ sol, float* __restrict__ vel) {
+    const int fx = ni + (axis == 0), fy = nj + (axis == 1), fz = nk + (axis == 2);
+    const size_t nf = (size_t)fx * fy * fz;
+    size_t f = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (f >= nf) return;
+    int i = (int)(f % fx);
+    size_t t = f / fx;
+    int j = (int)(t % fy), k = (int)(t / fy);
+    const int lo = sub ? 1 : 2;
+    int a[3] = {i, j, k};
+    int nn[3] = {ni, nj, nk};
+    for (int q = 0; q < 3; ++q) {
+        if (q == axis) {
+            int hi = sub ? nn[q] - 1 : nn[q] - 2;
+            if (a[q] < lo || a[q] > hi) return;
+        } else if (a[q] < 1 || a[q] > nn[q] - 2) {
+            return;
+        }
+    }
+    if (wgt[f] > 0) {
+        size_t c = lin3(i, j, k, ni, nj);
+        size_t cm = (axis == 0) ? c - 1 : (axis == 1) ? c - ni : c - (size_t)ni * nj;
+        float g = p[c] - p[cm];
+        vel[f] -= sub ? dt * g / h : g / h;
+    } else {
+        vel[f] = sol[f];
+    }
+}
+
+}  // namespace vflip
+
+using namespace vflip;
+
+extern "C" {
+
+int vflip_face_weights(int ni, int nj, int nk, const float* phi, float* uw, float* vw, float* ww, void* stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && phi && uw && vw && ww, "face_weights args");
+    hipStream_t s = (hipStream_t)stream;
+    size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+    k_face_weights<<<grid_for(nu), kBlock, 0, s>>>(0, ni, nj, nk, phi, uw);
+    k_face_weights<<<grid_for(nv), kBlock, 0, s>>>(1, ni, nj, nk, phi, vw);
+    k_face_weights<<<grid_for(nw), kBlock, 0, s>>>(2, ni, nj, nk, phi, ww);
+    VF_CHECK_LAUNCH();
+    VF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int vflip_inflow_bc(int ni, int nj, int nk, const int8_t* bdesc, float u_in, float* u, float* v, float* w,
+                    void* stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && bdesc && u && v && w, "inflow args");
+    hipStream_t s = (hipStream_t)stream;
+    k_inflow<<<grid_for((size_t)ni * nj * nk), kBlock, 0, s>>>(ni, nj, nk, bdesc, u_in, u, v, w);
+    VF_CHECK_LAUNCH();
+    VF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int vflip_grad_apply(int ni, int nj, int nk, int variant, float h, float dt, const float* p, const float* uw,
+                     const float* vw, const float* ww, const float* us, const float* vs, const float* ws, float* u,
+                     float* v, float* w, void* stream) {
+    VF_ARG(ni > 2 && nj > 2 && nk > 2, "grid dims");
+    VF_ARG(p && uw && vw && ww && us && vs && ws && u && v && w, "grad arrays");
+    VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
+    hipStream_t s = (hipStream_t)stream;
+    int sub = variant == VFLIP_VARIANT_SUB;
+    size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+    k_grad<<<grid_for(nu), kBlock, 0, s>>>(0, ni, nj, nk, sub, h, dt, p, uw, us, u);
+    k_grad<<<grid_for(nv), kBlock, 0, s>>>(1, ni, nj, nk, sub, h, dt, p, vw, vs, v);
+    k_grad<<<grid_for(nw), kBlock, 0, s>>>(2, ni, nj, nk, sub, h, dt, p, ww, ws, w);
+    VF_CHECK_LAUNCH();
+    VF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // extern "C"
+
+// Assembly and the whole projection need the solver's level-0 storage.
+struct vflip_solver;
+namespace vflip {
+int solver_level0(vflip_solver* s, int* ni, int* nj, int* nk, float** d, float** ax, float** ay, float** az,
+                  float** leak, hipStream_t* stream);
+void solver_mark_matrix_changed(vflip_solver* s);
+}  // namespace vflip
+
+extern "C" {
+
+int vflip_assemble(vflip_solver* S, int variant, float h, float dt, const float* u, const float* v, const float* w,
+                   const float* uw, const float* vw, const float* ww, const float* us, const float* vs,
+                   const float* ws, float* rhs) {
+    VF_ARG(S, "solver");
+    VF_ARG(u && v && w && uw && vw && ww && us && vs && ws && rhs, "assembly arrays");
+    VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
+    int ni, nj, nk;
+    float *d, *ax, *ay, *az, *lk;
+    hipStream_t s;
+    int rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s);
+    if (rc) return rc;
+    k_assemble<<<grid_for((size_t)ni * nj * nk), kBlock, 0, s>>>(ni, nj, nk, h, dt, variant == VFLIP_VARIANT_SUB, u,
+                                                                 v, w, uw, vw, ww, us, vs, ws, d, ax, ay, az, lk,
+                                                                 rhs);
+    VF_CHECK_LAUNCH();
+    solver_mark_matrix_changed(S);
+    return 0;
+}
+
+int vflip_project(vflip_solver* S, int variant, float h, float dt, float* u, float* v, float* w, const float* uw,
+                  const float* vw, const float* ww, const float* us, const float* vs, const float* ws,
+                  float* rhs_scratch, float* pressure, vflip_solve_stats* stats) {
+    VF_ARG(S && pressure && rhs_scratch, "solver/pressure/rhs");
+    int rc = vflip_assemble(S, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs_scratch);
+    if (rc) return rc;
+    rc = vflip_solver_setup(S);
+    if (rc) return rc;
+    int status = vflip_solver_solve(S, rhs_scratch, pressure, stats);
+    if (status < 0 && status != VFLIP_ERR_BREAKDOWN) return status;
+    int ni, nj, nk;
+    float *d, *ax, *ay, *az, *lk;
+    hipStream_t s;
+    rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s);
+    if (rc) return rc;
+    rc = vflip_grad_apply(ni, nj, nk, variant, h, dt, pressure, uw, vw, ww, us, vs, ws, u, v, w, s);
+    if (rc) return rc;
+    return status;
+}
+
+}  // extern "C"
