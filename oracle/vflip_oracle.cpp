@@ -603,13 +603,16 @@ float or_sample_linear(const float* f, int nx, int ny, int nz, double h, double 
     return (float)lerpd(v1, v2, cz);
 }
 
-// Cubic Lagrange weights on {-1,0,1,2}, evaluated in double as the sampler does.
-static inline double cubic_interp(double m1, double v0, double v1, double v2, double f) {
-    double f2 = f * f, f3 = f2 * f;
-    double wm1 = -(1. / 3) * f + (1. / 2) * f2 - (1. / 6) * f3;
-    double w0 = 1 - f2 + (1. / 2) * (f3 - f);
-    double w1 = f + (1. / 2) * (f2 - f3);
-    double w2 = (1. / 6) * (f3 - f);
+// Cubic Lagrange weights on {-1,0,1,2}. Buffer3D<float>::cubic_interp takes the
+// fraction as the buffer's element type, so the weights are evaluated in float
+// and combined with the (double) values in double.
+static inline double cubic_interp(double m1, double v0, double v1, double v2, double fd) {
+    const float f = (float)fd;
+    const float f2 = f * f, f3 = f2 * f;
+    const float wm1 = -float(1. / 3) * f + float(1. / 2) * f2 - float(1. / 6) * f3;
+    const float w0 = 1 - f2 + float(1. / 2) * (f3 - f);
+    const float w1 = f + float(1. / 2) * (f2 - f3);
+    const float w2 = float(1. / 6) * (f3 - f);
     return wm1 * m1 + w0 * v0 + w1 * v1 + w2 * v2;
 }
 

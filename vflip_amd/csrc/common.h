@@ -63,15 +63,18 @@ template <typename T> __device__ inline T wave_max(T v) {
     return v;
 }
 
-// Block reduction for blockDim.x == kBlock (4 waves); result valid in thread 0.
+// Block reductions for kBlock threads in any (x, y) shape; result valid in the
+// first thread.
+__device__ inline int flat_tid() { return threadIdx.x + threadIdx.y * blockDim.x; }
+
 template <typename T> __device__ inline T block_sum(T v) {
     __shared__ T red[kBlock / 64];
     v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int t = flat_tid(), lane = t & 63, wid = t >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     T r = 0;
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         for (int q = 0; q < kBlock / 64; ++q) r += red[q];
     }
     __syncthreads();
@@ -80,11 +83,11 @@ template <typename T> __device__ inline T block_sum(T v) {
 template <typename T> __device__ inline T block_max(T v) {
     __shared__ T red[kBlock / 64];
     v = wave_max(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int t = flat_tid(), lane = t & 63, wid = t >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     T r = 0;
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         r = red[0];
         for (int q = 1; q < kBlock / 64; ++q) r = r > red[q] ? r : red[q];
     }

@@ -1,4 +1,4 @@
-// Multigrid-preconditioned CG pressure solver for gfx950.
+// Multigrid-preconditioned CG pressure solver for gfx950 (host side).
 //
 // Restates the reference aggregation-AMG PCG (S/AlgebraicMultigrid.h:22-301,
 // S/GeometricLevelGen.{h,cpp}) matrix-free: every level is a symmetric 7-point
@@ -9,464 +9,18 @@
 //   d, ax, ay, az, leak  FP32 (leak = diag - sum|couplings|, the Dirichlet part,
 //                        only read when building the next level)
 //   hd, hax, hay, haz    FP16 copies scaled by 1/hscale (mixed precision, l >= 2)
-//   x, b                 FP32 V-cycle vectors (level 0 uses the PCG r and z)
+//   x, xt, b             FP32 V-cycle vectors; x/xt ping-pong for the fused
+//                        sweeps (level 0 uses the PCG vectors r and z/zt)
 #include "common.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
-namespace vflip {
+#include "mg_kernels.h"
 
-// ----------------------------------------------------------------------------
-// Operator view used by the kernels. CT = float (FP32 storage) or __half.
-// ----------------------------------------------------------------------------
-template <typename CT> struct Op {
-    const CT* d;
-    const CT* ax;
-    const CT* ay;
-    const CT* az;
-    float scale;
-    int ni, nj, nk;
-    __device__ inline float D(size_t c) const { return scale * (float)d[c]; }
-    __device__ inline float AX(size_t c) const { return scale * (float)ax[c]; }
-    __device__ inline float AY(size_t c) const { return scale * (float)ay[c]; }
-    __device__ inline float AZ(size_t c) const { return scale * (float)az[c]; }
-};
-
-__device__ inline void unflat(size_t c, int ni, int nj, int& i, int& j, int& k) {
-    size_t t = c / (size_t)ni;
-    i = (int)(c - t * ni);
-    k = (int)(t / (size_t)nj);
-    j = (int)(t - (size_t)k * nj);
-}
-
-// Off-diagonal row sum sum_{j != c} A(c,j) x_j in the reference CSR column order
-// (-z, -y, -x, +x, +y, +z), S/AlgebraicMultigrid.h:41-53.
-template <typename CT>
-__device__ inline float offdiag_sum(const Op<CT>& op, const float* __restrict__ x, size_t c, int i, int j, int k) {
-    const size_t sl = (size_t)op.ni * op.nj;
-    float s = 0.f;
-    if (k > 0) s += op.AZ(c - sl) * x[c - sl];
-    if (j > 0) s += op.AY(c - op.ni) * x[c - op.ni];
-    if (i > 0) s += op.AX(c - 1) * x[c - 1];
-    if (i + 1 < op.ni) s += op.AX(c) * x[c + 1];
-    if (j + 1 < op.nj) s += op.AY(c) * x[c + op.ni];
-    if (k + 1 < op.nk) s += op.AZ(c) * x[c + sl];
-    return s;
-}
-
-// ----------------------------------------------------------------------------
-// Smoother: one red-black half sweep, colour (i+j+k)&1 == colour, in place.
-// x = (b - offdiag)/diag, diag == 0 -> 0 (S/AlgebraicMultigrid.h:54-62).
-// ----------------------------------------------------------------------------
-template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_rbgs_half(Op<CT> op, const float* __restrict__ b, float* __restrict__ x,
-                                                      int colour) {
-    const size_t n = (size_t)op.ni * op.nj * op.nk;
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    int i, j, k;
-    unflat(c, op.ni, op.nj, i, j, k);
-    if (((i + j + k) & 1) != colour) return;
-    float s = offdiag_sum(op, x, c, i, j, k);
-    float d = op.D(c);
-    x[c] = (d != 0.f) ? (b[c] - s) / d : 0.f;
-}
-
-// First half sweep from a zero guess: odd cells get b/diag, even cells 0.
-template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_rbgs_first(Op<CT> op, const float* __restrict__ b, float* __restrict__ x) {
-    const size_t n = (size_t)op.ni * op.nj * op.nk;
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    int i, j, k;
-    unflat(c, op.ni, op.nj, i, j, k);
-    float v = 0.f;
-    if ((i + j + k) & 1) {
-        float d = op.D(c);
-        v = (d != 0.f) ? (b[c] - 0.f) / d : 0.f;
-    }
-    x[c] = v;
-}
-
-// Odd half sweep fused with the preceding prolongation x += P xc: odd cells read
-// their even neighbours as x + xc(parent) (S/AlgebraicMultigrid.h:118-129, 161-165).
-// Even cells are left uncorrected: the following even half sweep overwrites them
-// without reading their old value.
-template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_rbgs_odd_prolong(Op<CT> op, const float* __restrict__ b,
-                                                             float* __restrict__ x, const float* __restrict__ xc,
-                                                             int ci, int cj) {
-    const size_t n = (size_t)op.ni * op.nj * op.nk;
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    int i, j, k;
-    unflat(c, op.ni, op.nj, i, j, k);
-    if (!((i + j + k) & 1)) return;
-    const size_t sl = (size_t)op.ni * op.nj;
-    auto X = [&](size_t q, int a, int bb, int cc) { return x[q] + xc[lin3(a >> 1, bb >> 1, cc >> 1, ci, cj)]; };
-    float s = 0.f;
-    if (k > 0) s += op.AZ(c - sl) * X(c - sl, i, j, k - 1);
-    if (j > 0) s += op.AY(c - op.ni) * X(c - op.ni, i, j - 1, k);
-    if (i > 0) s += op.AX(c - 1) * X(c - 1, i - 1, j, k);
-    if (i + 1 < op.ni) s += op.AX(c) * X(c + 1, i + 1, j, k);
-    if (j + 1 < op.nj) s += op.AY(c) * X(c + op.ni, i, j + 1, k);
-    if (k + 1 < op.nk) s += op.AZ(c) * X(c + sl, i, j, k + 1);
-    float d = op.D(c);
-    x[c] = (d != 0.f) ? (b[c] - s) / d : 0.f;
-}
-
-// x += P xc (injection), used when there are no post-smoothing sweeps.
-__global__ __launch_bounds__(kBlock) void k_prolong_add(int ni, int nj, int nk, float* __restrict__ x,
-                                                        const float* __restrict__ xc, int ci, int cj) {
-    const size_t n = (size_t)ni * nj * nk;
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    int i, j, k;
-    unflat(c, ni, nj, i, j, k);
-    x[c] += xc[lin3(i >> 1, j >> 1, k >> 1, ci, cj)];
-}
-
-// b_c = R (b - A x), R = 1/8 per existing child (S/AlgebraicMultigrid.h:105-117,
-// S/GeometricLevelGen.cpp:50-63). One thread per coarse cell.
-template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_residual_restrict(Op<CT> op, const float* __restrict__ b,
-                                                              const float* __restrict__ x, float* __restrict__ bc,
-                                                              int ci, int cj, int ck) {
-    const size_t nc = (size_t)ci * cj * ck;
-    size_t C = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (C >= nc) return;
-    int I, J, K;
-    unflat(C, ci, cj, I, J, K);
-    float acc = 0.f;
-#pragma unroll
-    for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-        for (int jj = 0; jj < 2; ++jj)
-#pragma unroll
-            for (int ii = 0; ii < 2; ++ii) {
-                int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
-                if (i >= op.ni || j >= op.nj || k >= op.nk) continue;
-                size_t c = lin3(i, j, k, op.ni, op.nj);
-                float r = b[c] - (offdiag_sum(op, x, c, i, j, k) + op.D(c) * x[c]);
-                acc += 0.125f * r;
-            }
-    bc[C] = acc;
-}
-
-// ----------------------------------------------------------------------------
-// Coarsest level: all RB-GS sweeps in one workgroup, x resident in LDS,
-// coefficients in registers (n <= 4096 unknowns, S/GeometricLevelGen.h:46).
-// ----------------------------------------------------------------------------
-constexpr int kCoarseThreads = 1024;
-constexpr int kCoarseMax = 4096;
-constexpr int kCoarsePer = kCoarseMax / kCoarseThreads;
-
-template <typename CT>
-__global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const float* __restrict__ b,
-                                                             float* __restrict__ x, int sweeps) {
-    __shared__ float xs[kCoarseMax + 1];
-    const int n = op.ni * op.nj * op.nk;
-    const int sl = op.ni * op.nj;
-    float cd[kCoarsePer], cb[kCoarsePer], ca[kCoarsePer][6];
-    int nb[kCoarsePer][6];
-    int colour[kCoarsePer];
-#pragma unroll
-    for (int q = 0; q < kCoarsePer; ++q) {
-        int c = threadIdx.x + q * kCoarseThreads;
-        colour[q] = -1;
-        if (c < n) {
-            int i = c % op.ni, t = c / op.ni, j = t % op.nj, k = t / op.nj;
-            colour[q] = (i + j + k) & 1;
-            cd[q] = op.D(c);
-            cb[q] = b[c];
-            nb[q][0] = k > 0 ? c - sl : kCoarseMax;
-            ca[q][0] = k > 0 ? op.AZ(c - sl) : 0.f;
-            nb[q][1] = j > 0 ? c - op.ni : kCoarseMax;
-            ca[q][1] = j > 0 ? op.AY(c - op.ni) : 0.f;
-            nb[q][2] = i > 0 ? c - 1 : kCoarseMax;
-            ca[q][2] = i > 0 ? op.AX(c - 1) : 0.f;
-            nb[q][3] = i + 1 < op.ni ? c + 1 : kCoarseMax;
-            ca[q][3] = i + 1 < op.ni ? op.AX(c) : 0.f;
-            nb[q][4] = j + 1 < op.nj ? c + op.ni : kCoarseMax;
-            ca[q][4] = j + 1 < op.nj ? op.AY(c) : 0.f;
-            nb[q][5] = k + 1 < op.nk ? c + sl : kCoarseMax;
-            ca[q][5] = k + 1 < op.nk ? op.AZ(c) : 0.f;
-            xs[c] = 0.f;
-        }
-    }
-    if (threadIdx.x == 0) xs[kCoarseMax] = 0.f;
-    __syncthreads();
-    for (int it = 0; it < sweeps; ++it) {
-#pragma unroll
-        for (int col = 1; col >= 0; --col) {
-#pragma unroll
-            for (int q = 0; q < kCoarsePer; ++q) {
-                if (colour[q] == col) {
-                    float s = 0.f;
-#pragma unroll
-                    for (int e = 0; e < 6; ++e) s += ca[q][e] * xs[nb[q][e]];
-                    xs[threadIdx.x + q * kCoarseThreads] = (cd[q] != 0.f) ? (cb[q] - s) / cd[q] : 0.f;
-                }
-            }
-            __syncthreads();
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < kCoarsePer; ++q) {
-        int c = threadIdx.x + q * kCoarseThreads;
-        if (c < n) x[c] = xs[c];
-    }
-}
-
-// ----------------------------------------------------------------------------
-// Closed-form Galerkin coarsening A_c = 0.5 R A P (R = 1/8, P = injection):
-//   coupling(I, I+e) = 1/16 * sum of the fine couplings across the aggregate face
-//   leak(I)          = 1/16 * sum of the children's leaks
-//   diag(I)          = leak(I) + sum of the 6 coarse couplings' magnitudes
-// (equal to 1/16 * sum_{i,k in I} A(i,k); written this way no term cancels).
-// ----------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_coarsen(int ni, int nj, int nk, const float* __restrict__ ax,
-                                                    const float* __restrict__ ay, const float* __restrict__ az,
-                                                    const float* __restrict__ leak, int ci, int cj, int ck,
-                                                    float* __restrict__ dc, float* __restrict__ axc,
-                                                    float* __restrict__ ayc, float* __restrict__ azc,
-                                                    float* __restrict__ leakc) {
-    const size_t nc = (size_t)ci * cj * ck;
-    size_t C = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (C >= nc) return;
-    int I, J, K;
-    unflat(C, ci, cj, I, J, K);
-    const float s = 0.0625f;
-    // coupling across the aggregate face between fine planes p and p+1 along an axis
-    auto face = [&](int axis, int plane) -> float {
-        // plane: fine index of the lower side of the face
-        float acc = 0.f;
-        for (int u = 0; u < 2; ++u)
-            for (int v = 0; v < 2; ++v) {
-                int i, j, k;
-                if (axis == 0) { i = plane; j = 2 * J + u; k = 2 * K + v; }
-                else if (axis == 1) { i = 2 * I + u; j = plane; k = 2 * K + v; }
-                else { i = 2 * I + u; j = 2 * J + v; k = plane; }
-                if (i < 0 || j < 0 || k < 0 || i >= ni || j >= nj || k >= nk) continue;
-                size_t c = lin3(i, j, k, ni, nj);
-                acc += (axis == 0 ? ax[c] : axis == 1 ? ay[c] : az[c]);
-            }
-        return s * acc;
-    };
-    float cxp = (I + 1 < ci) ? face(0, 2 * I + 1) : 0.f;
-    float cyp = (J + 1 < cj) ? face(1, 2 * J + 1) : 0.f;
-    float czp = (K + 1 < ck) ? face(2, 2 * K + 1) : 0.f;
-    float cxm = (I > 0) ? face(0, 2 * I - 1) : 0.f;
-    float cym = (J > 0) ? face(1, 2 * J - 1) : 0.f;
-    float czm = (K > 0) ? face(2, 2 * K - 1) : 0.f;
-    float lk = 0.f;
-    for (int kk = 0; kk < 2; ++kk)
-        for (int jj = 0; jj < 2; ++jj)
-            for (int ii = 0; ii < 2; ++ii) {
-                int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
-                if (i < ni && j < nj && k < nk) lk += leak[lin3(i, j, k, ni, nj)];
-            }
-    lk *= s;
-    leakc[C] = lk;
-    axc[C] = cxp;
-    ayc[C] = cyp;
-    azc[C] = czp;
-    dc[C] = lk + ((-czm) + (-cym) + (-cxm) + (-cxp) + (-cyp) + (-czp));
-}
-
-// leak = diag - sum|couplings| evaluated in FP64 (for operators given without it).
-__global__ __launch_bounds__(kBlock) void k_leak_from_diag(int ni, int nj, int nk, const float* __restrict__ d,
-                                                           const float* __restrict__ ax, const float* __restrict__ ay,
-                                                           const float* __restrict__ az, float* __restrict__ leak) {
-    const size_t n = (size_t)ni * nj * nk;
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    int i, j, k;
-    unflat(c, ni, nj, i, j, k);
-    const size_t sl = (size_t)ni * nj;
-    double s = 0;
-    if (k > 0) s -= az[c - sl];
-    if (j > 0) s -= ay[c - ni];
-    if (i > 0) s -= ax[c - 1];
-    if (i + 1 < ni) s -= ax[c];
-    if (j + 1 < nj) s -= ay[c];
-    if (k + 1 < nk) s -= az[c];
-    double l = (double)d[c] - s;
-    leak[c] = (float)(l > 0 ? l : 0.0);
-}
-
-// FP16 copy of a level scaled by 1/scale.
-__global__ __launch_bounds__(kBlock) void k_to_half(size_t n, const float* __restrict__ src, __half* __restrict__ dst,
-                                                    float inv_scale) {
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) dst[c] = __float2half(src[c] * inv_scale);
-}
-
-// ----------------------------------------------------------------------------
-// Krylov kernels. Reductions: per-block partials (grid-stride, fixed grid), then
-// one single-block finalize kernel computes the scalars on the device. Acc is
-// float (plain FP32 mode) or double (mixed mode: FP64 dots and alpha/beta).
-// ----------------------------------------------------------------------------
-constexpr int kRedBlocks = 1024;
-
-struct Scalars {
-    double rho, rho_new, sigma, alpha, beta, res, bnorm, tol_abs, true_res;
-    int breakdown;
-};
-
-template <typename CT, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* __restrict__ s, float* __restrict__ q,
-                                                       double* __restrict__ partial) {
-    const size_t n = (size_t)op.ni * op.nj * op.nk;
-    Acc acc = 0;
-    for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock) {
-        int i, j, k;
-        unflat(c, op.ni, op.nj, i, j, k);
-        const size_t sl = (size_t)op.ni * op.nj;
-        // reference multiply order: -z, -y, -x, diag, +x, +y, +z
-        float y = 0.f;
-        if (k > 0) y += op.AZ(c - sl) * s[c - sl];
-        if (j > 0) y += op.AY(c - op.ni) * s[c - op.ni];
-        if (i > 0) y += op.AX(c - 1) * s[c - 1];
-        y += op.D(c) * s[c];
-        if (i + 1 < op.ni) y += op.AX(c) * s[c + 1];
-        if (j + 1 < op.nj) y += op.AY(c) * s[c + op.ni];
-        if (k + 1 < op.nk) y += op.AZ(c) * s[c + sl];
-        q[c] = y;
-        acc += (Acc)s[c] * (Acc)y;
-    }
-    Acc t = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
-}
-
-template <typename Acc>
-__global__ __launch_bounds__(kBlock) void k_dot(size_t n, const float* __restrict__ a, const float* __restrict__ b,
-                                                double* __restrict__ partial) {
-    Acc acc = 0;
-    for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock)
-        acc += (Acc)a[c] * (Acc)b[c];
-    Acc t = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
-}
-
-__global__ __launch_bounds__(kBlock) void k_absmax(size_t n, const float* __restrict__ a, double* __restrict__ partial) {
-    float m = 0.f;
-    for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock)
-        m = fmaxf(m, fabsf(a[c]));
-    float t = block_max(m);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
-}
-
-// x += alpha s; r -= alpha q; partial max|r|
-template <typename Acc>
-__global__ __launch_bounds__(kBlock) void k_update_xr(size_t n, const Scalars* __restrict__ sc, float* __restrict__ x,
-                                                      const float* __restrict__ s, float* __restrict__ r,
-                                                      const float* __restrict__ q, double* __restrict__ partial) {
-    const Acc alpha = (Acc)sc->alpha;
-    float m = 0.f;
-    for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock) {
-        x[c] = (float)((Acc)x[c] + alpha * (Acc)s[c]);
-        float rv = (float)((Acc)r[c] - alpha * (Acc)q[c]);
-        r[c] = rv;
-        m = fmaxf(m, fabsf(rv));
-    }
-    float t = block_max(m);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
-}
-
-// s = z + beta s
-template <typename Acc>
-__global__ __launch_bounds__(kBlock) void k_update_s(size_t n, const Scalars* __restrict__ sc, float* __restrict__ s,
-                                                     const float* __restrict__ z) {
-    const Acc beta = (Acc)sc->beta;
-    for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock)
-        s[c] = (float)((Acc)z[c] + beta * (Acc)s[c]);
-}
-
-// r = b - A x evaluated in FP64 (true residual), partial max|r| / |b|.
-template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_true_residual(Op<CT> op, const float* __restrict__ b,
-                                                          const float* __restrict__ x, float* __restrict__ r,
-                                                          double* __restrict__ partial) {
-    const size_t n = (size_t)op.ni * op.nj * op.nk;
-    double m = 0;
-    for (size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock) {
-        int i, j, k;
-        unflat(c, op.ni, op.nj, i, j, k);
-        const size_t sl = (size_t)op.ni * op.nj;
-        double y = (double)op.D(c) * x[c];
-        if (k > 0) y += (double)op.AZ(c - sl) * x[c - sl];
-        if (j > 0) y += (double)op.AY(c - op.ni) * x[c - op.ni];
-        if (i > 0) y += (double)op.AX(c - 1) * x[c - 1];
-        if (i + 1 < op.ni) y += (double)op.AX(c) * x[c + 1];
-        if (j + 1 < op.nj) y += (double)op.AY(c) * x[c + op.ni];
-        if (k + 1 < op.nk) y += (double)op.AZ(c) * x[c + sl];
-        double rv = (double)b[c] - y;
-        if (r) r[c] = (float)rv;
-        m = fmax(m, fabs(rv));
-    }
-    double t = block_max(m);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
-
-// Finalize kernels (single block).
-template <typename Acc> __device__ inline double reduce_partials_sum(const double* p, int np) {
-    Acc a = 0;
-    for (int q = threadIdx.x; q < np; q += kBlock) a += (Acc)p[q];
-    return (double)block_sum(a);
-}
-__device__ inline double reduce_partials_max(const double* p, int np) {
-    double a = 0;
-    for (int q = threadIdx.x; q < np; q += kBlock) a = fmax(a, p[q]);
-    return block_max(a);
-}
-
-// what: 0 rho (initial) -> also s/beta unused; 1 sigma -> alpha; 2 res; 3 rho_new -> beta;
-// 4 bnorm (with tol factor); 5 true_res
-template <typename Acc>
-__global__ __launch_bounds__(kBlock) void k_finalize(Scalars* sc, const double* __restrict__ partial, int np, int what,
-                                                     double tol) {
-    double v = (what == 2 || what == 4 || what == 5) ? reduce_partials_max(partial, np)
-                                                     : reduce_partials_sum<Acc>(partial, np);
-    if (threadIdx.x != 0) return;
-    if (what == 0) {
-        sc->rho = v;
-        sc->breakdown = (v == 0 || v != v) ? 1 : 0;
-    } else if (what == 1) {
-        sc->sigma = v;
-        sc->alpha = (double)((Acc)sc->rho / (Acc)v);
-    } else if (what == 2) {
-        sc->res = v;
-    } else if (what == 3) {
-        sc->rho_new = v;
-        sc->beta = (double)((Acc)v / (Acc)sc->rho);
-        sc->rho = v;
-    } else if (what == 4) {
-        sc->bnorm = v;
-        sc->tol_abs = tol * v;
-        sc->res = v;
-    } else {
-        sc->true_res = v;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_fill(size_t n, float* __restrict__ a, float v) {
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) a[c] = v;
-}
-__global__ __launch_bounds__(kBlock) void k_copy(size_t n, float* __restrict__ dst, const float* __restrict__ src) {
-    size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c < n) dst[c] = src[c];
-}
-
-}  // namespace vflip
-
-// ============================================================================
-// Host side
-// ============================================================================
 using namespace vflip;
 
 struct LevelBuf {
@@ -476,7 +30,8 @@ struct LevelBuf {
     __half *hd = nullptr, *hax = nullptr, *hay = nullptr, *haz = nullptr;
     float hscale = 1.f;
     bool half = false;
-    float *x = nullptr, *b = nullptr;
+    float *x = nullptr, *xt = nullptr, *b = nullptr;
+    void* vec_alloc = nullptr;
 };
 
 struct vflip_solver {
@@ -485,12 +40,16 @@ struct vflip_solver {
     vflip_solve_opts opts;
     hipStream_t stream = nullptr;
     std::vector<LevelBuf> L;
-    float *r = nullptr, *z = nullptr, *s = nullptr, *q = nullptr, *xw = nullptr;
+    float *r = nullptr, *z = nullptr, *zt = nullptr, *s = nullptr, *q = nullptr, *xw = nullptr;
+    void* pcg_alloc = nullptr;
     double* partial = nullptr;
+    int np_partial = 0;  // partial buffer capacity
+    int np_last = 0;     // partials written by the last reduction
     Scalars* sc = nullptr;
     Scalars* h_sc = nullptr;  // pinned
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool setup_done = false;
+    int zsweep_min_n = 32768;  // levels at least this large use the fused z-marching sweep
 };
 
 namespace {
@@ -513,35 +72,96 @@ int alloc(void** p, size_t bytes) {
     return 0;
 }
 
-template <typename CT> int smooth_level(vflip_solver* S, int l, bool pre, const float* xc) {
+bool mixed(const vflip_solver* S) { return S->opts.precision == VFLIP_PREC_MIXED; }
+
+dim3 grid3(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, nk); }
+dim3 grid3z(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, (nk + kMatvecKZ - 1) / kMatvecKZ); }
+int nblocks(dim3 g) { return (int)(g.x * g.y * g.z); }
+
+// Fused z-marching sweep geometry for a level.
+struct ZGeom {
+    int ty, tiles_x, tiles_y, zchunk, blocks;
+};
+ZGeom zgeom(const LevelBuf& lv) {
+    ZGeom g;
+    g.ty = lv.nj >= 128 ? 8 : 4;
+    g.tiles_x = (lv.ni + kZTX - 1) / kZTX;
+    g.tiles_y = (lv.nj + g.ty - 1) / g.ty;
+    const int tiles = g.tiles_x * g.tiles_y;
+    // about 1024 workgroups (4 per CU), z chunks of at least 4 planes
+    int zc = std::max(4, (int)((long long)lv.nk * tiles / 1024));
+    const char* env = getenv("VFLIP_ZCHUNK");
+    if (env && atoi(env) > 0) zc = atoi(env);
+    zc = std::min(zc, lv.nk);
+    g.zchunk = zc;
+    g.blocks = tiles * ((lv.nk + zc - 1) / zc);
+    return g;
+}
+
+template <typename Acc>
+void launch_zsweep(vflip_solver* S, const LevelBuf& lv, const float* b, const float* xin, float* xout, int mode,
+                   const float* xc, int ci, int cj, double* dot) {
+    ZGeom g = zgeom(lv);
+    Op<float> op = make_op<float>(lv);
+    if (g.ty == 8)
+        k_rbgs_zsweep<8, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, mode, xc, ci, cj, g.tiles_x,
+                                                                g.tiles_y, g.zchunk, dot);
+    else
+        k_rbgs_zsweep<4, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, mode, xc, ci, cj, g.tiles_x,
+                                                                g.tiles_y, g.zchunk, dot);
+}
+
+bool use_zsweep(const vflip_solver* S, const LevelBuf& lv) {
+    return !lv.half && (int)lv.n >= S->zsweep_min_n && lv.ni >= 16 && lv.nj >= 4 && lv.nk >= 4;
+}
+
+// Pre- or post-smoothing of level l. With dot_done (level 0, post) the last
+// fused sweep also leaves z.r partials in S->partial.
+template <typename CT> int smooth_level(vflip_solver* S, int l, bool pre, const float* xc, bool* dot_done) {
     LevelBuf& lv = S->L[l];
-    Op<CT> op = make_op<CT>(lv);
     const float* b = (l == 0) ? S->r : lv.b;
-    float* x = (l == 0) ? S->z : lv.x;
-    unsigned g = grid_for(lv.n);
-    int sweeps = pre ? S->opts.pre_sweeps : S->opts.post_sweeps;
-    int start = 0;
-    if (pre) {
-        if (sweeps == 0) {
-            k_fill<<<g, kBlock, 0, S->stream>>>(lv.n, x, 0.f);
-            VF_CHECK_LAUNCH();
-            return 0;
+    float** xp = (l == 0) ? &S->z : &lv.x;
+    float** xtp = (l == 0) ? &S->zt : &lv.xt;
+    const int sweeps = pre ? S->opts.pre_sweeps : S->opts.post_sweeps;
+    const LevelBuf* c = pre ? nullptr : &S->L[l + 1];
+    if (dot_done) *dot_done = false;
+    if (sweeps == 0) {
+        if (pre) {
+            k_fill<<<grid_for(lv.n), kBlock, 0, S->stream>>>((unsigned)lv.n, *xp, 0.f);
+        } else {
+            k_prolong_add<<<grid_for(lv.n), kBlock, 0, S->stream>>>(lv.ni, lv.nj, lv.nk, *xp, xc, c->ni, c->nj);
         }
-        k_rbgs_first<CT><<<g, kBlock, 0, S->stream>>>(op, b, x);
-        k_rbgs_half<CT><<<g, kBlock, 0, S->stream>>>(op, b, x, 0);
-        start = 1;
-    } else {
-        const LevelBuf& c = S->L[l + 1];
-        if (sweeps == 0) {
-            k_prolong_add<<<g, kBlock, 0, S->stream>>>(lv.ni, lv.nj, lv.nk, x, xc, c.ni, c.nj);
-            VF_CHECK_LAUNCH();
-            return 0;
-        }
-        k_rbgs_odd_prolong<CT><<<g, kBlock, 0, S->stream>>>(op, b, x, xc, c.ni, c.nj);
-        k_rbgs_half<CT><<<g, kBlock, 0, S->stream>>>(op, b, x, 0);
-        start = 1;
+        VF_CHECK_LAUNCH();
+        return 0;
     }
-    for (int it = start; it < sweeps; ++it) {
+    if (use_zsweep(S, lv)) {
+        for (int it = 0; it < sweeps; ++it) {
+            const int mode = (it == 0) ? (pre ? 1 : 2) : 0;
+            const bool last = (it == sweeps - 1);
+            double* dot = (dot_done && last && l == 0) ? S->partial : nullptr;
+            if (mixed(S))
+                launch_zsweep<double>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
+            else
+                launch_zsweep<float>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
+            VF_CHECK_LAUNCH();
+            std::swap(*xp, *xtp);
+            if (dot) {
+                S->np_last = zgeom(lv).blocks;
+                *dot_done = true;
+            }
+        }
+        return 0;
+    }
+    Op<CT> op = make_op<CT>(lv);
+    float* x = *xp;
+    const unsigned g = grid_for(lv.n);
+    if (pre) {
+        k_rbgs_first<CT><<<g, kBlock, 0, S->stream>>>(op, b, x);
+    } else {
+        k_rbgs_odd_prolong<CT><<<g, kBlock, 0, S->stream>>>(op, b, x, xc, c->ni, c->nj);
+    }
+    k_rbgs_half<CT><<<g, kBlock, 0, S->stream>>>(op, b, x, 0);
+    for (int it = 1; it < sweeps; ++it) {
         k_rbgs_half<CT><<<g, kBlock, 0, S->stream>>>(op, b, x, 1);
         k_rbgs_half<CT><<<g, kBlock, 0, S->stream>>>(op, b, x, 0);
     }
@@ -554,16 +174,22 @@ template <typename CT> int restrict_level(vflip_solver* S, int l) {
     LevelBuf& c = S->L[l + 1];
     const float* b = (l == 0) ? S->r : lv.b;
     const float* x = (l == 0) ? S->z : lv.x;
-    k_residual_restrict<CT><<<grid_for(c.n), kBlock, 0, S->stream>>>(make_op<CT>(lv), b, x, c.b, c.ni, c.nj, c.nk);
+    k_residual_restrict<CT><<<grid3(c.ni, c.nj, c.nk), dim3(64, 4), 0, S->stream>>>(make_op<CT>(lv), b, x, c.b, c.ni,
+                                                                                    c.nj, c.nk);
     VF_CHECK_LAUNCH();
     return 0;
 }
 
-int run_vcycle(vflip_solver* S) {
+// z = M^-1 r, one V-cycle (S/AlgebraicMultigrid.h:130-188). When want_dot is set
+// and the level-0 post-smoother fused it, partials of z.r are left in
+// S->partial and *dot_done is set.
+int run_vcycle(vflip_solver* S, bool want_dot, bool* dot_done) {
     const int nl = (int)S->L.size();
     int rc;
+    if (dot_done) *dot_done = false;
     for (int l = 0; l < nl - 1; ++l) {
-        rc = S->L[l].half ? smooth_level<__half>(S, l, true, nullptr) : smooth_level<float>(S, l, true, nullptr);
+        rc = S->L[l].half ? smooth_level<__half>(S, l, true, nullptr, nullptr)
+                          : smooth_level<float>(S, l, true, nullptr, nullptr);
         if (rc) return rc;
         rc = S->L[l].half ? restrict_level<__half>(S, l) : restrict_level<float>(S, l);
         if (rc) return rc;
@@ -572,6 +198,12 @@ int run_vcycle(vflip_solver* S) {
         LevelBuf& lv = S->L[nl - 1];
         const float* b = (nl == 1) ? S->r : lv.b;
         float* x = (nl == 1) ? S->z : lv.x;
+        const int ppr = (lv.ni + 1) / 2;
+        const int npairs = ppr * lv.nj * lv.nk;
+        if (lv.n > (size_t)kCoarseMax || npairs > kCoarsePairs * kCoarseThreads) {
+            set_error("coarsest level %dx%dx%d exceeds the single-workgroup solver", lv.ni, lv.nj, lv.nk);
+            return VFLIP_ERR_ARG;
+        }
         if (lv.half)
             k_coarsest<__half><<<1, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), b, x, S->opts.coarse_sweeps);
         else
@@ -580,19 +212,18 @@ int run_vcycle(vflip_solver* S) {
     }
     for (int l = nl - 2; l >= 0; --l) {
         const float* xc = S->L[l + 1].x;
-        rc = S->L[l].half ? smooth_level<__half>(S, l, false, xc) : smooth_level<float>(S, l, false, xc);
+        bool* dd = (l == 0 && want_dot) ? dot_done : nullptr;
+        rc = S->L[l].half ? smooth_level<__half>(S, l, false, xc, dd) : smooth_level<float>(S, l, false, xc, dd);
         if (rc) return rc;
     }
     return 0;
 }
 
-bool mixed(const vflip_solver* S) { return S->opts.precision == VFLIP_PREC_MIXED; }
-
 int finalize(vflip_solver* S, int what, double tol = 0) {
     if (mixed(S))
-        k_finalize<double><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial, kRedBlocks, what, tol);
+        k_finalize<double><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial, S->np_last, what, tol);
     else
-        k_finalize<float><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial, kRedBlocks, what, tol);
+        k_finalize<float><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial, S->np_last, what, tol);
     VF_CHECK_LAUNCH();
     return 0;
 }
@@ -605,8 +236,57 @@ int fetch_scalars(vflip_solver* S) {
 
 int check_dims(int ni, int nj, int nk) {
     if (ni < 1 || nj < 1 || nk < 1) return 0;
-    if ((double)ni * nj * nk > 2.0e9) return 0;
+    if ((double)ni * nj * nk > 2147483647.0) return 0;
     return 1;
+}
+
+// Reductions record how many partials they wrote.
+void launch_absmax(vflip_solver* S, unsigned n, const float* a) {
+    k_absmax<<<kVecBlocks, kBlock, 0, S->stream>>>(n, a, S->partial);
+    S->np_last = kVecBlocks;
+}
+void launch_dot(vflip_solver* S, unsigned n, const float* a, const float* b) {
+    if (mixed(S)) k_dot<double><<<kVecBlocks, kBlock, 0, S->stream>>>(n, a, b, S->partial);
+    else k_dot<float><<<kVecBlocks, kBlock, 0, S->stream>>>(n, a, b, S->partial);
+    S->np_last = kVecBlocks;
+}
+void launch_true_residual(vflip_solver* S, const float* b, const float* x, float* r) {
+    const LevelBuf& l0 = S->L[0];
+    dim3 g = grid3z(l0.ni, l0.nj, l0.nk);
+    k_true_residual<float><<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), b, x, r, S->partial);
+    S->np_last = nblocks(g);
+}
+void launch_matvec_dot(vflip_solver* S, const float* s, float* q) {
+    const LevelBuf& l0 = S->L[0];
+    dim3 g = grid3z(l0.ni, l0.nj, l0.nk);
+    if (mixed(S))
+        k_matvec_dot<float, double><<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), s, q, S->partial);
+    else
+        k_matvec_dot<float, float><<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), s, q, S->partial);
+    S->np_last = nblocks(g);
+}
+void launch_update_xr(vflip_solver* S) {
+    if (mixed(S))
+        k_update_xr<double><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->xw, S->s, S->r, S->q,
+                                                                  S->partial);
+    else
+        k_update_xr<float><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->xw, S->s, S->r, S->q,
+                                                                 S->partial);
+    S->np_last = kVecBlocks;
+}
+void launch_update_s(vflip_solver* S) {
+    if (mixed(S)) k_update_s<double><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->s, S->z);
+    else k_update_s<float><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->s, S->z);
+}
+
+// z = M^-1 r followed by rho = z.r (fused into the last smoothing sweep when possible).
+int precond_and_dot(vflip_solver* S, int what) {
+    bool dot_done = false;
+    int rc = run_vcycle(S, true, &dot_done);
+    if (rc) return rc;
+    if (!dot_done) launch_dot(S, (unsigned)S->n, S->z, S->r);
+    VF_CHECK_LAUNCH();
+    return finalize(S, what);
 }
 
 }  // namespace
@@ -633,9 +313,9 @@ int vflip_solver_destroy(vflip_solver* S) {
     for (auto& l : S->L) {
         if (l.d) (void)hipFree(l.d);
         if (l.hd) (void)hipFree(l.hd);
-        if (l.x) (void)hipFree(l.x);
+        if (l.vec_alloc) (void)hipFree(l.vec_alloc);
     }
-    if (S->r) (void)hipFree(S->r);
+    if (S->pcg_alloc) (void)hipFree(S->pcg_alloc);
     if (S->partial) (void)hipFree(S->partial);
     if (S->sc) (void)hipFree(S->sc);
     if (S->h_sc) (void)hipHostFree(S->h_sc);
@@ -650,14 +330,11 @@ int vflip_solver_set_opts(vflip_solver* S, const vflip_solve_opts* o) {
     VF_ARG(o->max_iterations >= 0 && o->pre_sweeps >= 0 && o->post_sweeps >= 0 && o->coarse_sweeps >= 0,
            "negative iteration or sweep count");
     VF_ARG(o->precision == VFLIP_PREC_FP32 || o->precision == VFLIP_PREC_MIXED, "precision");
-    bool need_half = (o->precision == VFLIP_PREC_MIXED);
-    bool had_half = (S->opts.precision == VFLIP_PREC_MIXED);
+    const bool need_half = (o->precision == VFLIP_PREC_MIXED);
+    const bool had_half = (S->opts.precision == VFLIP_PREC_MIXED);
     S->opts = *o;
     if (S->opts.refine_every < 0) S->opts.refine_every = 0;
-    if (need_half != had_half) {
-        for (size_t l = 0; l < S->L.size(); ++l) S->L[l].half = need_half && l >= 2 && S->L[l].hd;
-        S->setup_done = false;
-    }
+    if (need_half != had_half) S->setup_done = false;
     return 0;
 }
 
@@ -665,11 +342,18 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
     VF_ARG(out, "out");
     *out = nullptr;
     VF_ARG(check_dims(ni, nj, nk), "grid dims");
+    if (opts) {
+        VF_ARG(opts->precision == VFLIP_PREC_FP32 || opts->precision == VFLIP_PREC_MIXED, "precision");
+        VF_ARG(opts->max_iterations >= 0 && opts->pre_sweeps >= 0 && opts->post_sweeps >= 0 &&
+                   opts->coarse_sweeps >= 0,
+               "negative iteration or sweep count");
+    }
     vflip_solver* S = new vflip_solver();
     S->ni = ni; S->nj = nj; S->nk = nk;
     S->n = (size_t)ni * nj * nk;
     vflip_default_opts(&S->opts);
     if (opts) S->opts = *opts;
+    if (const char* e = getenv("VFLIP_ZSWEEP_MIN_N")) S->zsweep_min_n = atoi(e);
     // level shapes, S/GeometricLevelGen.h:36-58
     {
         int a = ni, b = nj, c = nk;
@@ -694,25 +378,31 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
         if (rc) break;
         lv.d = (float*)p; lv.ax = lv.d + lv.n; lv.ay = lv.ax + lv.n; lv.az = lv.ay + lv.n; lv.leak = lv.az + lv.n;
         if (l >= 1) {
-            rc = alloc(&p, 2 * lv.n * sizeof(float));
+            rc = alloc(&lv.vec_alloc, 3 * lv.n * sizeof(float));
             if (rc) break;
-            lv.x = (float*)p; lv.b = lv.x + lv.n;
+            lv.x = (float*)lv.vec_alloc; lv.xt = lv.x + lv.n; lv.b = lv.xt + lv.n;
         }
         if (l >= 2) {
             rc = alloc(&p, 4 * lv.n * sizeof(__half));
             if (rc) break;
             lv.hd = (__half*)p; lv.hax = lv.hd + lv.n; lv.hay = lv.hax + lv.n; lv.haz = lv.hay + lv.n;
-            lv.half = (S->opts.precision == VFLIP_PREC_MIXED);
         }
     }
     if (!rc) {
-        void* p = nullptr;
-        rc = alloc(&p, 5 * S->n * sizeof(float));
+        rc = alloc(&S->pcg_alloc, 6 * S->n * sizeof(float));
         if (!rc) {
-            S->r = (float*)p; S->z = S->r + S->n; S->s = S->z + S->n; S->q = S->s + S->n; S->xw = S->q + S->n;
+            float* p = (float*)S->pcg_alloc;
+            S->r = p; S->z = p + S->n; S->zt = p + 2 * S->n; S->s = p + 3 * S->n; S->q = p + 4 * S->n;
+            S->xw = p + 5 * S->n;
         }
     }
-    if (!rc) rc = alloc((void**)&S->partial, kRedBlocks * sizeof(double));
+    if (!rc) {
+        const LevelBuf& l0 = S->L[0];
+        int np = std::max(kVecBlocks, nblocks(grid3z(l0.ni, l0.nj, l0.nk)));
+        np = std::max(np, zgeom(l0).blocks);
+        S->np_partial = np;
+        rc = alloc((void**)&S->partial, (size_t)np * sizeof(double));
+    }
     if (!rc) rc = alloc((void**)&S->sc, sizeof(Scalars));
     if (!rc) {
         if (hipHostMalloc((void**)&S->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
@@ -754,7 +444,7 @@ int vflip_solver_set_matrix(vflip_solver* S, const float* d, const float* ax, co
                             const float* leak) {
     VF_ARG(S && d && ax && ay && az, "matrix arrays");
     LevelBuf& l0 = S->L[0];
-    size_t bytes = l0.n * sizeof(float);
+    const size_t bytes = l0.n * sizeof(float);
     VF_HIP(hipMemcpyAsync(l0.d, d, bytes, hipMemcpyDeviceToDevice, S->stream));
     VF_HIP(hipMemcpyAsync(l0.ax, ax, bytes, hipMemcpyDeviceToDevice, S->stream));
     VF_HIP(hipMemcpyAsync(l0.ay, ay, bytes, hipMemcpyDeviceToDevice, S->stream));
@@ -784,20 +474,21 @@ int vflip_solver_setup(vflip_solver* S) {
         // FP16 copies of levels >= 2, scaled so the largest diagonal maps to 2^10.
         for (size_t l = 2; l < S->L.size(); ++l) {
             LevelBuf& lv = S->L[l];
-            k_absmax<<<kRedBlocks, kBlock, 0, S->stream>>>(lv.n, lv.d, S->partial);
+            launch_absmax(S, (unsigned)lv.n, lv.d);
             VF_CHECK_LAUNCH();
             int rc = finalize(S, 5);
             if (rc) return rc;
             rc = fetch_scalars(S);
             if (rc) return rc;
-            double mx = S->h_sc->true_res;
-            float scale = (mx > 0) ? (float)(mx / 1024.0) : 1.f;
+            const double mx = S->h_sc->true_res;
+            const float scale = (mx > 0) ? (float)(mx / 1024.0) : 1.f;
             lv.hscale = scale;
-            float inv = 1.f / scale;
-            k_to_half<<<grid_for(lv.n), kBlock, 0, S->stream>>>(lv.n, lv.d, lv.hd, inv);
-            k_to_half<<<grid_for(lv.n), kBlock, 0, S->stream>>>(lv.n, lv.ax, lv.hax, inv);
-            k_to_half<<<grid_for(lv.n), kBlock, 0, S->stream>>>(lv.n, lv.ay, lv.hay, inv);
-            k_to_half<<<grid_for(lv.n), kBlock, 0, S->stream>>>(lv.n, lv.az, lv.haz, inv);
+            const float inv = 1.f / scale;
+            const unsigned g = grid_for(lv.n);
+            k_to_half<<<g, kBlock, 0, S->stream>>>((unsigned)lv.n, lv.d, lv.hd, inv);
+            k_to_half<<<g, kBlock, 0, S->stream>>>((unsigned)lv.n, lv.ax, lv.hax, inv);
+            k_to_half<<<g, kBlock, 0, S->stream>>>((unsigned)lv.n, lv.ay, lv.hay, inv);
+            k_to_half<<<g, kBlock, 0, S->stream>>>((unsigned)lv.n, lv.az, lv.haz, inv);
             VF_CHECK_LAUNCH();
             lv.half = true;
         }
@@ -812,7 +503,7 @@ int vflip_solver_setup(vflip_solver* S) {
 int vflip_solver_get_level(const vflip_solver* S, int l, float* d, float* ax, float* ay, float* az) {
     VF_ARG(S && l >= 0 && l < (int)S->L.size(), "level");
     const LevelBuf& lv = S->L[l];
-    size_t bytes = lv.n * sizeof(float);
+    const size_t bytes = lv.n * sizeof(float);
     if (d) VF_HIP(hipMemcpyAsync(d, lv.d, bytes, hipMemcpyDeviceToDevice, S->stream));
     if (ax) VF_HIP(hipMemcpyAsync(ax, lv.ax, bytes, hipMemcpyDeviceToDevice, S->stream));
     if (ay) VF_HIP(hipMemcpyAsync(ay, lv.ay, bytes, hipMemcpyDeviceToDevice, S->stream));
@@ -828,7 +519,7 @@ int vflip_solver_vcycle(vflip_solver* S, const float* r, float* z) {
         if (rc) return rc;
     }
     VF_HIP(hipMemcpyAsync(S->r, r, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
-    int rc = run_vcycle(S);
+    int rc = run_vcycle(S, false, nullptr);
     if (rc) return rc;
     VF_HIP(hipMemcpyAsync(z, S->z, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
     VF_HIP(hipStreamSynchronize(S->stream));
@@ -837,7 +528,9 @@ int vflip_solver_vcycle(vflip_solver* S, const float* r, float* z) {
 
 int vflip_solver_matvec(vflip_solver* S, const float* x, float* y) {
     VF_ARG(S && x && y, "solver/x/y");
-    k_matvec_dot<float, float><<<kRedBlocks, kBlock, 0, S->stream>>>(make_op<float>(S->L[0]), x, y, S->partial);
+    const LevelBuf& l0 = S->L[0];
+    k_matvec_dot<float, float><<<grid3z(l0.ni, l0.nj, l0.nk), dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), x, y,
+                                                                                           S->partial);
     VF_CHECK_LAUNCH();
     VF_HIP(hipStreamSynchronize(S->stream));
     return 0;
@@ -853,17 +546,16 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
         VF_HIP(hipEventSynchronize(S->e1));
         VF_HIP(hipEventElapsedTime(&ms_setup, S->e0, S->e1));
     }
-    const size_t n = S->n;
+    const unsigned n = (unsigned)S->n;
     const bool mx = mixed(S);
-    Op<float> A = make_op<float>(S->L[0]);
     VF_HIP(hipEventRecord(S->e0, S->stream));
     // |b|_inf and the stop threshold (S/AlgebraicMultigrid.h:212-233)
-    k_absmax<<<kRedBlocks, kBlock, 0, S->stream>>>(n, rhs, S->partial);
+    launch_absmax(S, n, rhs);
     VF_CHECK_LAUNCH();
     if ((rc = finalize(S, 4, S->opts.tolerance))) return rc;
     if (S->opts.warm_start) {
-        VF_HIP(hipMemcpyAsync(S->xw, x, n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
-        k_true_residual<float><<<kRedBlocks, kBlock, 0, S->stream>>>(A, rhs, S->xw, S->r, S->partial);
+        VF_HIP(hipMemcpyAsync(S->xw, x, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+        launch_true_residual(S, rhs, S->xw, S->r);
         VF_CHECK_LAUNCH();
         if ((rc = finalize(S, 2))) return rc;
     } else {
@@ -874,8 +566,8 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
     if ((rc = fetch_scalars(S))) return rc;
     vflip_solve_stats stats{};
     stats.ms_setup = ms_setup;
-    double bnorm = S->h_sc->bnorm;
-    double tol_abs = S->h_sc->tol_abs;
+    const double bnorm = S->h_sc->bnorm;
+    const double tol_abs = S->h_sc->tol_abs;
     double res = S->h_sc->res;
     int status = VFLIP_OK;
     int it = 0;
@@ -887,11 +579,7 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
     } else if (res <= tol_abs && S->opts.warm_start) {
         stats.converged = 1;
     } else {
-        if ((rc = run_vcycle(S))) return rc;
-        if (mx) k_dot<double><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->z, S->r, S->partial);
-        else k_dot<float><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->z, S->r, S->partial);
-        VF_CHECK_LAUNCH();
-        if ((rc = finalize(S, 0))) return rc;
+        if ((rc = precond_and_dot(S, 0))) return rc;
         if ((rc = fetch_scalars(S))) return rc;
         if (S->h_sc->breakdown) {
             status = VFLIP_ERR_BREAKDOWN;
@@ -900,16 +588,14 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
             VF_CHECK_LAUNCH();
             status = VFLIP_NOT_CONVERGED;
             for (it = 0; it < S->opts.max_iterations; ++it) {
-                if (mx) k_matvec_dot<float, double><<<kRedBlocks, kBlock, 0, S->stream>>>(A, S->s, S->q, S->partial);
-                else k_matvec_dot<float, float><<<kRedBlocks, kBlock, 0, S->stream>>>(A, S->s, S->q, S->partial);
+                launch_matvec_dot(S, S->s, S->q);
                 VF_CHECK_LAUNCH();
                 if ((rc = finalize(S, 1))) return rc;
-                if (mx) k_update_xr<double><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->sc, S->xw, S->s, S->r, S->q, S->partial);
-                else k_update_xr<float><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->sc, S->xw, S->s, S->r, S->q, S->partial);
+                launch_update_xr(S);
                 VF_CHECK_LAUNCH();
                 if ((rc = finalize(S, 2))) return rc;
                 if (mx && S->opts.refine_every > 0 && ((it + 1) % S->opts.refine_every) == 0) {
-                    k_true_residual<float><<<kRedBlocks, kBlock, 0, S->stream>>>(A, rhs, S->xw, S->r, S->partial);
+                    launch_true_residual(S, rhs, S->xw, S->r);
                     VF_CHECK_LAUNCH();
                     if ((rc = finalize(S, 2))) return rc;
                 }
@@ -920,22 +606,17 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
                     ++it;
                     break;
                 }
-                if ((rc = run_vcycle(S))) return rc;
-                if (mx) k_dot<double><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->z, S->r, S->partial);
-                else k_dot<float><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->z, S->r, S->partial);
-                VF_CHECK_LAUNCH();
-                if ((rc = finalize(S, 3))) return rc;
-                if (mx) k_update_s<double><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->sc, S->s, S->z);
-                else k_update_s<float><<<kRedBlocks, kBlock, 0, S->stream>>>(n, S->sc, S->s, S->z);
+                if ((rc = precond_and_dot(S, 3))) return rc;
+                launch_update_s(S);
                 VF_CHECK_LAUNCH();
             }
             stats.converged = (status == VFLIP_OK);
         }
-        VF_HIP(hipMemcpyAsync(x, S->xw, n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+        VF_HIP(hipMemcpyAsync(x, S->xw, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
     }
     VF_HIP(hipEventRecord(S->e1, S->stream));
     // FP64 true residual of the returned solution, for the stats
-    k_true_residual<float><<<kRedBlocks, kBlock, 0, S->stream>>>(A, rhs, x, nullptr, S->partial);
+    launch_true_residual(S, rhs, x, nullptr);
     VF_CHECK_LAUNCH();
     if ((rc = finalize(S, 5))) return rc;
     if ((rc = fetch_scalars(S))) return rc;
@@ -978,13 +659,13 @@ int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const do
     for (int k = 0; k < nk; ++k)
         for (int j = 0; j < nj; ++j)
             for (int i = 0; i < ni; ++i) {
-                size_t c = lin3(i, j, k, ni, nj);
+                const size_t c = lin3(i, j, k, ni, nj);
                 double s = 0;
                 if (i > 0) s -= ax[c - 1];
                 if (j > 0) s -= ay[c - ni];
                 if (k > 0) s -= az[c - (size_t)ni * nj];
                 s -= ax[c] + ay[c] + az[c];
-                double l = diag[c] - s;
+                const double l = diag[c] - s;
                 hl[c] = (float)(l > 0 ? l : 0.0);
             }
     float* dev = nullptr;
