@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the vFLIP per-step hot path on one MI355X.
+
+Workload: smoke plume on a 256^3 MAC grid (h = 1/256, dt = 0.01), 1-4 seeded
+sphere obstacles (radius U[0.05,0.15]), sphere source r = 0.08 at
+(0.5, 0.1, 0.5) with rho = T = 1 + U[0, 0.05] jitter (seed 0), buoyancy
+alpha = 0.1, beta = 0.3, RK2 semi-Lagrangian advection of u, v, w, rho, T and
+the MGPCG pressure projection to 1e-6 relative residual (infinity norm,
+recursive residual, as the reference). One step = advection + forcing +
+projection; inputs are resident in HBM. The headline `value` is ms/step of the
+mixed-precision projection path; the plain FP32 MGPCG path is timed beside it.
+
+The `roofline` object describes the dominant kernel (the fused level-0
+red-black smoothing sweep): algorithmic bytes per launch (28 B/cell) over the
+average launch time measured with HIP events on the solver's stream inside the
+timed steps. `cpu_baseline` times the FP64 CPU oracle (a restatement of the
+reference's TBB path) on one step of the same workload on the host cores.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ms/step & iters to 1e-6 rel. residual, 256^3: learned+mixed vs plain MGPCG"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_scene(n, seed=0):
+    from tests import scenes
+    h = 1.0 / n
+    spheres = scenes.random_spheres(seed)
+    phi = scenes.solid_phi(n, h, spheres)
+    return phi, spheres
+
+
+def run_gpu(n, steps, warmup, precision, phi, init_state=None, profile=False):
+    import torch
+    from vflip_amd import api
+    from vflip_amd.plume import PlumeSim
+    sim = PlumeSim(n, phi, precision=precision)
+    if init_state is not None:
+        sim.load_state(init_state)
+    for _ in range(warmup):
+        sim.step()
+    torch.cuda.synchronize()
+    if profile:
+        sim.solver.profile(True)
+    its, proj_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = sim.step()
+        its.append(st["iterations"])
+        proj_ms.append(st["ms_solve"])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = sim.solver.profile_read() if profile else None
+    if profile:
+        sim.solver.profile(False)
+    return sim, (t1 - t0) * 1e3 / steps, its, proj_ms, prof, st
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the fused sweep from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "zsweep_pmc_256.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(sim_state, phi, n, dt=0.01):
+    """One plume step of the same state on the FP64 CPU oracle (all host cores)."""
+    import numpy as np
+    from oracle import oracle as O
+    h = np.float32(1.0 / n)
+    st = {k: v.detach().cpu().numpy() for k, v in sim_state.items()}
+    uw, vw, ww = O.face_weights(phi)
+    z = [np.zeros_like(st[k]) for k in ("u", "v", "w")]
+    t0 = time.perf_counter()
+    u2 = O.advect_rk2(h, np.float32(dt), st["u"], st["v"], st["w"], st["u"], (0.5, 0, 0))
+    v2 = O.advect_rk2(h, np.float32(dt), st["u"], st["v"], st["w"], st["v"], (0, 0.5, 0))
+    w2 = O.advect_rk2(h, np.float32(dt), st["u"], st["v"], st["w"], st["w"], (0, 0, 0.5))
+    r2 = O.advect_rk2(h, np.float32(dt), st["u"], st["v"], st["w"], st["rho"], (0, 0, 0))
+    t2 = O.advect_rk2(h, np.float32(dt), st["u"], st["v"], st["w"], st["temp"], (0, 0, 0))
+    c = [x - 0.5 * float(h) for x in (0.5, 0.1, 0.5)]
+    r2, t2, v2 = O.plume_force(float(h), dt, c, 0.08, 0, 0.05, 0.1, 0.3, r2, t2, v2)
+    d, ax, ay, az, rhs = O.assemble(h, u2, v2, w2, uw, vw, ww, *z)
+    t_asm = time.perf_counter()
+    out = O.amgpcg(d, ax, ay, az, rhs, tol=1e-6, maxit=50)
+    t_solve = time.perf_counter()
+    uo, vo, wo = O.grad_apply(h, out["x"], uw, vw, ww, *z, u2, v2, w2)
+    t1 = time.perf_counter()
+    return {
+        "ms_per_step": (t1 - t0) * 1e3,
+        "ms_solve": (t_solve - t_asm) * 1e3,
+        "iterations": out["iterations"],
+        "p": out["x"], "u": uo, "v": vo, "w": wo,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from vflip_amd import api
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    n = args.n
+    phi, spheres = make_scene(n)
+    log(f"scene n={n} spheres={spheres}")
+
+    # plain FP32 MGPCG
+    sim_p, ms_plain, its_plain, proj_plain, _, _ = run_gpu(n, args.steps, args.warmup, api.FP32, phi)
+    log(f"plain: {ms_plain:.3f} ms/step its={its_plain}")
+    state0 = {k: t.clone() for k, t in sim_p.state().items()}
+    del sim_p
+    # mixed precision (headline), with kernel timing of the dominant kernel
+    sim_m, ms_mixed, its_mixed, proj_mixed, prof, st = run_gpu(n, args.steps, args.warmup, api.MIXED, phi,
+                                                               profile=True)
+    log(f"mixed: {ms_mixed:.3f} ms/step its={its_mixed} prof={prof}")
+
+    roofline = None
+    if prof and prof["launches"] > 0:
+        avg_ms = prof["total_ms"] / prof["launches"]
+        achieved = prof["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
+                    "kernel": "k_rbgs_zsweep (fused red-black sweep, level 0)",
+                    "avg_launch_us": round(avg_ms * 1e3, 2), "launches": prof["launches"],
+                    "algorithmic_bytes_per_launch": prof["bytes_per_launch"]}
+
+    cpu = None
+    oracle_err = None
+    if not args.no_cpu:
+        # one step from the plain run's final state on the CPU oracle, and the same step on the GPU
+        log("cpu baseline: one FP64 oracle step ...")
+        base = cpu_baseline(state0, phi, n)
+        from oracle import oracle as O
+        cores = O.lib().or_num_threads()
+        cpu = {"value": round(base["ms_per_step"], 1), "unit": "ms/step", "cores": cores, "kind": "port",
+               "sample": f"one plume step at {n}^3 (advection of u,v,w,rho,T + forcing + assembly + FP64 AMGPCG "
+                         f"to 1e-6, {base['iterations']} its + gradient), FP64 oracle with OpenMP",
+               "ms_solve": round(base["ms_solve"], 1)}
+        from vflip_amd.plume import PlumeSim
+        g = PlumeSim(n, phi, precision=api.FP32)
+        g.load_state(state0)
+        stg = g.step()
+        torch.cuda.synchronize()
+
+        def rel(a, b):
+            a = a.detach().cpu().numpy().astype(np.float64)
+            return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        oracle_err = {"p_rel_l2": rel(g.p, base["p"]), "u_rel_l2": rel(g.u, base["u"]),
+                      "v_rel_l2": rel(g.v, base["v"]), "w_rel_l2": rel(g.w, base["w"]),
+                      "gpu_iterations": stg["iterations"], "oracle_iterations": base["iterations"]}
+        log(f"cpu: {cpu} err: {oracle_err}")
+
+    voxels = n ** 3
+    out = {
+        "metric": METRIC,
+        "value": round(ms_mixed, 3),
+        "unit": "ms/step",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_mixed, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"smoke plume {n}^3 (1-4 seeded sphere obstacles), RK2 semi-Lagrangian advection + "
+                               f"MGPCG projection to 1e-6; headline = mixed-precision projection",
+                   "grid": n, "dt": 0.01, "tolerance": 1e-6, "spheres": len(spheres)},
+        "iterations": {"mixed": its_mixed, "plain": its_plain},
+        "plain_ms_per_step": round(ms_plain, 3),
+        "mixed_ms_per_step": round(ms_mixed, 3),
+        "projection_ms": {"plain": round(float(np.mean(proj_plain)), 3), "mixed": round(float(np.mean(proj_mixed)), 3)},
+        "speedup_mixed_vs_plain": round(ms_plain / ms_mixed, 3),
+        "voxels_per_s": round(voxels / (ms_mixed * 1e-3), 1),
+        "true_residual_last": st["true_residual"],
+        "learned_warm_start": None,
+        "fp64_oracle_error": oracle_err,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

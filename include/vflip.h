@@ -113,6 +113,13 @@ int vflip_solver_vcycle(vflip_solver* s, const float* r, float* z);
 /* y = A x with the level-0 operator (S/sparse_matrix.h:341-366). */
 int vflip_solver_matvec(vflip_solver* s, const float* x, float* y);
 
+/* Kernel timing of the dominant kernel (the fused level-0 smoothing sweep):
+ * enable=1 records HIP events around each launch on the solver's stream;
+ * _read returns the launches since enabling/the last read, their summed
+ * duration and the algorithmic HBM bytes of one launch (28 B per cell). */
+int vflip_solver_profile(vflip_solver* s, int enable);
+int vflip_solver_profile_read(vflip_solver* s, int* launches, double* total_ms, double* bytes_per_launch);
+
 /* Copies level l's operator to device arrays (diag, ax, ay, az; FP32). */
 int vflip_solver_get_level(const vflip_solver* s, int level, float* diag, float* ax, float* ay,
                            float* az);
@@ -162,6 +169,31 @@ int vflip_project(vflip_solver* s, int variant, float h, float dt, float* u, flo
                   const float* uw, const float* vw, const float* ww, const float* us,
                   const float* vs, const float* ws, float* rhs_scratch, float* pressure,
                   vflip_solve_stats* stats);
+
+/* --------------------------------------------------------------- advection */
+/* Semi-Lagrangian advection of one field (and optionally a second field on the
+ * same grid, sharing the backtrace) by the MAC velocity u,v,w of the ni x nj x nk
+ * cell grid: RK2 midpoint backtrace x - dt*u(x - dt/2*u(x)), then
+ *   mode 0: trilinear gather — SubDomainSolver::advect_field, S/sub_domain.cpp:191-235
+ *   mode 1: cubic gather clamped to the trilinear value on overshoot —
+ *           SmokeSolver3D::advect_field_cubic_clamp, S/Smoke_solver3D.cpp:657-707.
+ * field: fnx x fny x fnz with grid offsets (fox,foy,foz) in cells (u: .5,0,0;
+ * cell-centred scalars: 0,0,0). Samplers interpolate in FP64 with index clamping
+ * (Buffer3D::sample_linear/sample_cubic, S/fluid_buffer3D.h:226-326); results are
+ * bit-identical with that arithmetic. field2/out2 may be NULL. Asynchronous on
+ * hip_stream. */
+int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const float* u, const float* v,
+                     const float* w, const float* field, int fnx, int fny, int fnz, float fox, float foy,
+                     float foz, float* out, const float* field2, float* out2, void* hip_stream);
+
+/* Smoke-plume forcing of the synthetic workload (replaces the reference's
+ * gen_heat/add_smokeforce, S/Smoke_solver3D.cpp:1624-1668, with the workload's
+ * sphere source and face buoyancy): cells with |x - c| <= r get
+ * rho = T = 1 + jitter*U(seed, cell); then v faces 1..nj-1 get
+ * v += dt*(-alpha*rho_f + beta*T_f). Asynchronous on hip_stream. */
+int vflip_plume_force(int ni, int nj, int nk, float h, float dt, float cx, float cy, float cz, float r,
+                      unsigned seed, float jitter, float alpha, float beta, float* rho, float* temp, float* v,
+                      void* hip_stream);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,8 @@ def lib():
         L.or_randhash.restype = ctypes.c_uint
         L.or_randhash.argtypes = [ctypes.c_uint]
         L.or_num_threads.restype = c_int
+        L.or_plume_force.argtypes = ([c_int, c_int, c_int] + [c_float] * 6 + [ctypes.c_uint] + [c_float] * 3 +
+                                     [f32p] * 3)
         _LIB = L
     return _LIB
 
@@ -234,3 +236,11 @@ def g2p(pos, vel, h, dims, du, dv, dw, u=None, v=None, w=None, flip=1.0, bmin=(0
 
 def randhash(s):
     return lib().or_randhash(s)
+
+
+def plume_force(h, dt, center, r, seed, jitter, alpha, beta, rho, temp, v):
+    nk, nj, ni = rho.shape
+    rho, temp, v = c32(rho).copy(), c32(temp).copy(), c32(v).copy()
+    lib().or_plume_force(ni, nj, nk, h, dt, center[0], center[1], center[2], r, seed, jitter, alpha, beta, rho, temp,
+                         v)
+    return rho, temp, v

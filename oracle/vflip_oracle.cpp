@@ -904,3 +904,35 @@ int or_num_threads(void) {
 }
 
 }  // extern "C"
+
+extern "C" {
+// ---------------------------------------------------------------------------
+// Plume forcing of the synthetic smoke workload (not reference code: the
+// workload re-specifies the reference's cylinder source on a 2x grid and its
+// particle buoyancy, S/Smoke_solver3D.cpp:1624-1668, as a sphere source and
+// face buoyancy). Source: rho = T = 1 + jitter*U where U = randhash(seed*0x9E3779B1
+// + cell)/UINT_MAX inside |x - c| <= r (float ops); buoyancy on v faces 1..nj-1:
+// v += dt*(-alpha*rho_f + beta*T_f) with face averages 0.5*(a + b).
+// ---------------------------------------------------------------------------
+void or_plume_force(int ni, int nj, int nk, float h, float dt, float cx, float cy, float cz, float r,
+                    unsigned seed, float jitter, float alpha, float beta, float* rho, float* temp, float* v) {
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                float dx = i * h - cx, dy = j * h - cy, dz = k * h - cz;
+                if (dx * dx + dy * dy + dz * dz > r * r) continue;
+                size_t c = lin(i, j, k, ni, nj);
+                float u = (float)or_randhash(seed * 0x9E3779B1u + (unsigned)c) * (1.0f / 4294967295.0f);
+                rho[c] = 1.0f + jitter * u;
+                temp[c] = 1.0f + jitter * u;
+            }
+    for (int k = 0; k < nk; ++k)
+        for (int j = 1; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                size_t c = lin(i, j, k, ni, nj), cm = c - ni;
+                float rf = 0.5f * (rho[c] + rho[cm]);
+                float tf = 0.5f * (temp[c] + temp[cm]);
+                v[lin(i, j, k, ni, nj + 1)] += dt * (-alpha * rf + beta * tf);
+            }
+}
+}  // extern "C"

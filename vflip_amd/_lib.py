@@ -61,6 +61,8 @@ SIGNATURES = {
     "vflip_solver_vcycle": (c_int, [P, P, P]),
     "vflip_solver_matvec": (c_int, [P, P, P]),
     "vflip_solver_get_level": (c_int, [P, c_int, P, P, P, P]),
+    "vflip_solver_profile": (c_int, [P, c_int]),
+    "vflip_solver_profile_read": (c_int, [P, PI, PD, PD]),
     "vflip_amgpcg_solve_host": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P, c_double, c_int, PD, PI,
                                         ctypes.POINTER(SolveOpts)]),
     "vflip_face_weights": (c_int, [c_int, c_int, c_int, P, P, P, P, P]),
@@ -68,6 +70,10 @@ SIGNATURES = {
     "vflip_assemble": (c_int, [P, c_int, c_float, c_float] + [P] * 10),
     "vflip_grad_apply": (c_int, [c_int, c_int, c_int, c_int, c_float, c_float] + [P] * 10 + [P]),
     "vflip_project": (c_int, [P, c_int, c_float, c_float] + [P] * 11 + [ctypes.POINTER(SolveStats)]),
+    "vflip_advect_rk2": (c_int, [c_int, c_int, c_int, c_float, c_float, c_int, P, P, P, P, c_int, c_int, c_int,
+                                 c_float, c_float, c_float, P, P, P, P]),
+    "vflip_plume_force": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float,
+                                  ctypes.c_uint, c_float, c_float, c_float, P, P, P, P]),
 }
 
 _lib = None

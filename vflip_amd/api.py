@@ -100,6 +100,15 @@ class Solver:
     def matvec(self, x, y):
         check(lib().vflip_solver_matvec(self.h, _ptr(x, numel=self.n), _ptr(y, numel=self.n)), "matvec")
 
+    def profile(self, enable=True):
+        check(lib().vflip_solver_profile(self.h, int(enable)), "profile")
+
+    def profile_read(self):
+        n, ms, b = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(lib().vflip_solver_profile_read(self.h, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)),
+              "profile_read")
+        return {"launches": n.value, "total_ms": ms.value, "bytes_per_launch": b.value}
+
     def get_level(self, l):
         a, b, c = self.levels()[l]
         outs = [torch.empty((c, b, a), device="cuda", dtype=torch.float32) for _ in range(4)]
@@ -155,3 +164,18 @@ def amgpcg_solve_host(diag, ax, ay, az, rhs, tol=1e-6, maxit=50, precision=FP32)
                                              res.ctypes.data_as(ctypes.c_void_p), tol, maxit, ctypes.byref(r),
                                              ctypes.byref(it), ctypes.byref(o)), "amgpcg_solve_host")
     return rc, res, r.value, it.value
+
+
+def advect_rk2(h, dt, u, v, w, field, offs, out, mode=0, field2=None, out2=None):
+    """RK2 semi-Lagrangian advection of field (and field2) on a grid shaped like field."""
+    nk, nj, ni1 = u.shape
+    fz, fy, fx = field.shape
+    check(lib().vflip_advect_rk2(ni1 - 1, nj, nk, h, dt, mode, _ptr(u), _ptr(v), _ptr(w), _ptr(field), fx, fy, fz,
+                                 offs[0], offs[1], offs[2], _ptr(out), _ptr(field2), _ptr(out2), _stream()),
+          "advect_rk2")
+
+
+def plume_force(h, dt, center, r, seed, jitter, alpha, beta, rho, temp, v):
+    nk, nj, ni = rho.shape
+    check(lib().vflip_plume_force(ni, nj, nk, h, dt, center[0], center[1], center[2], r, seed, jitter, alpha, beta,
+                                  _ptr(rho), _ptr(temp), _ptr(v), _stream()), "plume_force")

@@ -63,6 +63,14 @@ template <typename T> __device__ inline T wave_max(T v) {
     return v;
 }
 
+// Workgroup barrier that orders LDS only: outstanding global loads stay in
+// flight across it (a full __syncthreads() also waits for them).
+__device__ inline void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Block reductions for kBlock threads in any (x, y) shape; result valid in the
 // first thread.
 __device__ inline int flat_tid() { return threadIdx.x + threadIdx.y * blockDim.x; }

@@ -118,28 +118,43 @@ __global__ __launch_bounds__(kBlock) void k_prolong_add(int ni, int nj, int nk, 
 // Large-level smoother: one full red-black sweep per launch, x ping-ponged
 // (xin -> xout). A workgroup owns a 64 x TY column tile and a z range and
 // marches up it: at plane p it computes the odd (red) cells of p on the tile plus
-// a one-cell ring from xin (global, L1/L2 resident), keeps the last three red
-// planes in LDS, and then finishes the even (black) cells of plane p-1 from those
-// red planes, writing both colours of plane p-1. Ring red cells are recomputed by
-// both neighbouring workgroups (bit-identical values), so there is no grid-wide
-// barrier between the two half sweeps and each cell's operator, b and x are
-// streamed from HBM once per sweep.
-//   mode 0: xin is the current iterate
-//   mode 1: the current iterate is zero (first pre-smoothing sweep; xin unused)
-//   mode 2: the current iterate is xin + P xc (first post-smoothing sweep)
+// a one-cell ring from xin, keeps the last three red planes in LDS, and then
+// finishes the even (black) cells of plane p-1 from those red planes, writing
+// both colours of plane p-1. Ring red cells are recomputed by both neighbouring
+// workgroups (bit-identical values), so there is no grid-wide barrier between the
+// two half sweeps and each cell's operator, b and x are streamed from HBM once
+// per sweep. The loop is software-pipelined: the global inputs of the next
+// plane's red cells and of this plane's black cells are loaded one step ahead,
+// so each step's memory latency overlaps the previous step's work.
+//   MODE 0: xin is the current iterate
+//   MODE 1: the current iterate is zero (first pre-smoothing sweep; xin unused)
+//   MODE 2: the current iterate is xin + P xc (first post-smoothing sweep)
 // dot != nullptr accumulates sum(xout * b) per workgroup (z.r in PCG).
 // ----------------------------------------------------------------------------
 constexpr int kZTX = 64;
 
-template <int TY, typename Acc>
+struct RedIn {
+    float b, d, a[6], x[6];
+    int slot;  // LDS index, -1 when the slot has no cell
+};
+struct BlackIn {
+    float b, d, a[6], bp;
+    int c;  // cell index, -1 when absent
+    int c2; // partner (odd) cell index, -1 when absent
+};
+
+template <int TY, int MODE, typename Acc>
 __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* __restrict__ b,
-                                                     const float* __restrict__ xin, float* __restrict__ xout, int mode,
+                                                     const float* __restrict__ xin, float* __restrict__ xout,
                                                      const float* __restrict__ xc, int ci, int cj, int tiles_x,
                                                      int tiles_y, int zchunk, double* __restrict__ dot) {
     constexpr int TX = kZTX;
     constexpr int RW = TX + 2, RH = TY + 2;
-    constexpr int RSLOTS = (RW / 2) * RH;
+    constexpr int RHALF = RW / 2;
+    constexpr int RSLOTS = RHALF * RH;
+    constexpr int RPER = (RSLOTS + 255) / 256;
     constexpr int BSLOTS = (TX / 2) * TY;
+    constexpr int BPER = (BSLOTS + 255) / 256;
     __shared__ float sr[3][RH * RW];
     const int ni = op.ni, nj = op.nj, nk = op.nk;
     const int sl = ni * nj;
@@ -155,70 +170,147 @@ __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* 
 
     auto xold = [&](int c, int i, int j, int k) -> float {
         float v = xin[c];
-        if (mode == 2) v += xc[((k >> 1) * cj + (j >> 1)) * ci + (i >> 1)];
+        if (MODE == 2) v += xc[((k >> 1) * cj + (j >> 1)) * ci + (i >> 1)];
         return v;
     };
+    // Loads are branch-free (clamped in-bounds addresses, masked values) so the
+    // compiler can keep the prefetched loads of the next step in flight.
+    auto load_red = [&](int p, int s, RedIn& in) {
+        const int sc = s < RSLOTS ? s : 0;
+        const int jr = sc / RHALF, m = sc - jr * RHALF;
+        const int j = j0 - 1 + jr;
+        const int ir = 2 * m + ((i0 + j + p) & 1);
+        const int i = i0 - 1 + ir;
+        const bool ok = s < RSLOTS && p >= 0 && p < nk && i >= 0 && i < ni && j >= 0 && j < nj;
+        const int c = ok ? (p * nj + j) * ni + i : 0;
+        const int pi = ok ? i : 0, pj = ok ? j : 0, pp = ok ? p : 0;
+        in.slot = ok ? jr * RW + ir : -1;
+        const bool zm = pp > 0, ym = pj > 0, xm = pi > 0, xp = pi + 1 < ni, yp = pj + 1 < nj, zp = pp + 1 < nk;
+        const int czm = zm ? c - sl : c, cym = ym ? c - ni : c, cxm = xm ? c - 1 : c;
+        const int cxp = xp ? c + 1 : c, cyp = yp ? c + ni : c, czp = zp ? c + sl : c;
+        in.b = b[c];
+        in.d = op.D(c);
+        const float a0 = op.AZ(czm), a1 = op.AY(cym), a2 = op.AX(cxm), a3 = op.AX(c), a4 = op.AY(c), a5 = op.AZ(c);
+        in.a[0] = zm ? a0 : 0.f;
+        in.a[1] = ym ? a1 : 0.f;
+        in.a[2] = xm ? a2 : 0.f;
+        in.a[3] = xp ? a3 : 0.f;
+        in.a[4] = yp ? a4 : 0.f;
+        in.a[5] = zp ? a5 : 0.f;
+        if (MODE != 1) {
+            float x0 = xin[czm], x1 = xin[cym], x2 = xin[cxm], x3 = xin[cxp], x4 = xin[cyp], x5 = xin[czp];
+            if (MODE == 2) {
+                auto par = [&](int ii, int jj, int kk) { return ((kk >> 1) * cj + (jj >> 1)) * ci + (ii >> 1); };
+                x0 += xc[par(pi, pj, zm ? pp - 1 : pp)];
+                x1 += xc[par(pi, ym ? pj - 1 : pj, pp)];
+                x2 += xc[par(xm ? pi - 1 : pi, pj, pp)];
+                x3 += xc[par(xp ? pi + 1 : pi, pj, pp)];
+                x4 += xc[par(pi, yp ? pj + 1 : pj, pp)];
+                x5 += xc[par(pi, pj, zp ? pp + 1 : pp)];
+            }
+            in.x[0] = zm ? x0 : 0.f;
+            in.x[1] = ym ? x1 : 0.f;
+            in.x[2] = xm ? x2 : 0.f;
+            in.x[3] = xp ? x3 : 0.f;
+            in.x[4] = yp ? x4 : 0.f;
+            in.x[5] = zp ? x5 : 0.f;
+        }
+    };
+    auto load_black = [&](int q, int s, BlackIn& in) {
+        const int sc = s < BSLOTS ? s : 0;
+        const int jt = sc / (TX / 2), m = sc - jt * (TX / 2);
+        const int j = j0 + jt;
+        const int it = 2 * m + ((i0 + j + q) & 1);
+        const int i = i0 + it, ip = i0 + (it ^ 1);
+        const bool rowok = s < BSLOTS && q >= k0 && q < k1 && j < nj;
+        const bool ok = rowok && i < ni, okp = rowok && ip < ni;
+        const int row = rowok ? (q * nj + j) * ni : 0;
+        const int c = ok ? row + i : 0;
+        in.c = ok ? c : -1;
+        in.c2 = okp ? row + ip : -1;
+        in.bp = dot ? b[okp ? row + ip : 0] : 0.f;
+        const int pi = ok ? i : 0, pj = ok ? j : 0, pq = ok ? q : 0;
+        const bool zm = pq > 0, ym = pj > 0, xm = pi > 0, xp = pi + 1 < ni, yp = pj + 1 < nj, zp = pq + 1 < nk;
+        in.b = b[c];
+        in.d = op.D(c);
+        const float a0 = op.AZ(zm ? c - sl : c), a1 = op.AY(ym ? c - ni : c), a2 = op.AX(xm ? c - 1 : c);
+        const float a3 = op.AX(c), a4 = op.AY(c), a5 = op.AZ(c);
+        in.a[0] = zm ? a0 : 0.f;
+        in.a[1] = ym ? a1 : 0.f;
+        in.a[2] = xm ? a2 : 0.f;
+        in.a[3] = xp ? a3 : 0.f;
+        in.a[4] = yp ? a4 : 0.f;
+        in.a[5] = zp ? a5 : 0.f;
+    };
+
+    for (int s = tid; s < 3 * RH * RW; s += 256) (&sr[0][0])[s] = 0.f;
+    lds_barrier();
+    RedIn rin[RPER], rnext[RPER];
+    BlackIn bin[BPER], bnext[BPER];
+#pragma unroll
+    for (int r = 0; r < RPER; ++r) load_red(k0 - 1, tid + 256 * r, rin[r]);
+#pragma unroll
+    for (int r = 0; r < BPER; ++r) bin[r].c = bin[r].c2 = -1;
 
     for (int p = k0 - 1; p <= k1; ++p) {
-        if (p >= 0 && p < nk) {
-            float* R = sr[(p + 3) % 3];
-            for (int s = tid; s < RSLOTS; s += 256) {
-                const int jr = s / (RW / 2), m = s - jr * (RW / 2);
-                const int j = j0 - 1 + jr;
-                const int ir = 2 * m + ((i0 + j + p) & 1);
-                const int i = i0 - 1 + ir;
-                if (i < 0 || i >= ni || j < 0 || j >= nj) continue;
-                const int c = (p * nj + j) * ni + i;
-                float sum = 0.f;
-                if (mode != 1) {
-                    if (p > 0) sum += op.AZ(c - sl) * xold(c - sl, i, j, p - 1);
-                    if (j > 0) sum += op.AY(c - ni) * xold(c - ni, i, j - 1, p);
-                    if (i > 0) sum += op.AX(c - 1) * xold(c - 1, i - 1, j, p);
-                    if (i + 1 < ni) sum += op.AX(c) * xold(c + 1, i + 1, j, p);
-                    if (j + 1 < nj) sum += op.AY(c) * xold(c + ni, i, j + 1, p);
-                    if (p + 1 < nk) sum += op.AZ(c) * xold(c + sl, i, j, p + 1);
-                }
-                R[jr * RW + ir] = gs_value(b[c], sum, op.D(c));
+        // prefetch: red inputs of plane p+1, black inputs of plane p
+#pragma unroll
+        for (int r = 0; r < RPER; ++r) load_red(p + 1 <= k1 ? p + 1 : nk, tid + 256 * r, rnext[r]);
+#pragma unroll
+        for (int r = 0; r < BPER; ++r) load_black(p, tid + 256 * r, bnext[r]);
+        // red cells of plane p
+        float* R = sr[(p + 3) % 3];
+#pragma unroll
+        for (int r = 0; r < RPER; ++r) {
+            if (rin[r].slot < 0) continue;
+            float s = 0.f;
+            if (MODE != 1) {
+#pragma unroll
+                for (int e = 0; e < 6; ++e) s += rin[r].a[e] * rin[r].x[e];
             }
+            R[rin[r].slot] = gs_value(rin[r].b, s, rin[r].d);
         }
-        __syncthreads();
+        lds_barrier();
+        // black cells of plane q = p - 1 (inputs loaded during the previous step)
         const int q = p - 1;
         if (q >= k0 && q < k1) {
             const float* Rm = sr[(q + 2) % 3];
             const float* R0 = sr[(q + 3) % 3];
             const float* Rp = sr[(p + 3) % 3];
-            for (int s = tid; s < BSLOTS; s += 256) {
+#pragma unroll
+            for (int r = 0; r < BPER; ++r) {
+                const int s = tid + 256 * r;
+                if (s >= BSLOTS) continue;
                 const int jt = s / (TX / 2), m = s - jt * (TX / 2);
                 const int j = j0 + jt;
-                if (j >= nj) continue;
                 const int it = 2 * m + ((i0 + j + q) & 1);
-                const int i = i0 + it;
                 const int jr = jt + 1, ir = it + 1;
-                const int row = (q * nj + j) * ni;
-                if (i < ni) {
-                    const int c = row + i;
+                if (bin[r].c >= 0) {
+                    // red neighbours outside the grid were never computed: skip them
+                    const int i = i0 + it;
                     float sum = 0.f;
-                    if (q > 0) sum += op.AZ(c - sl) * Rm[jr * RW + ir];
-                    if (j > 0) sum += op.AY(c - ni) * R0[(jr - 1) * RW + ir];
-                    if (i > 0) sum += op.AX(c - 1) * R0[jr * RW + ir - 1];
-                    if (i + 1 < ni) sum += op.AX(c) * R0[jr * RW + ir + 1];
-                    if (j + 1 < nj) sum += op.AY(c) * R0[(jr + 1) * RW + ir];
-                    if (q + 1 < nk) sum += op.AZ(c) * Rp[jr * RW + ir];
-                    const float bc = b[c];
-                    const float v = gs_value(bc, sum, op.D(c));
-                    xout[c] = v;
-                    if (dot) dacc += (Acc)v * (Acc)bc;
+                    if (q > 0) sum += bin[r].a[0] * Rm[jr * RW + ir];
+                    if (j > 0) sum += bin[r].a[1] * R0[(jr - 1) * RW + ir];
+                    if (i > 0) sum += bin[r].a[2] * R0[jr * RW + ir - 1];
+                    if (i + 1 < ni) sum += bin[r].a[3] * R0[jr * RW + ir + 1];
+                    if (j + 1 < nj) sum += bin[r].a[4] * R0[(jr + 1) * RW + ir];
+                    if (q + 1 < nk) sum += bin[r].a[5] * Rp[jr * RW + ir];
+                    const float v = gs_value(bin[r].b, sum, bin[r].d);
+                    xout[bin[r].c] = v;
+                    if (dot) dacc += (Acc)v * (Acc)bin[r].b;
                 }
-                const int ip = i0 + (it ^ 1);
-                if (ip < ni) {
-                    const int c2 = row + ip;
+                if (bin[r].c2 >= 0) {
                     const float vr = R0[jr * RW + (it ^ 1) + 1];
-                    xout[c2] = vr;
-                    if (dot) dacc += (Acc)vr * (Acc)b[c2];
+                    xout[bin[r].c2] = vr;
+                    if (dot) dacc += (Acc)vr * (Acc)bin[r].bp;
                 }
             }
         }
-        __syncthreads();
+        lds_barrier();
+#pragma unroll
+        for (int r = 0; r < RPER; ++r) rin[r] = rnext[r];
+#pragma unroll
+        for (int r = 0; r < BPER; ++r) bin[r] = bnext[r];
     }
     if (dot) {
         Acc t = block_sum(dacc);
@@ -228,29 +320,33 @@ __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* 
 
 // ----------------------------------------------------------------------------
 // b_c = R (b - A x), R = 1/8 per existing child (S/AlgebraicMultigrid.h:105-117,
-// S/GeometricLevelGen.cpp:50-63). 3D launch over coarse cells (64 x 4 threads),
-// so a wavefront reads 128 consecutive fine cells per row.
+// S/GeometricLevelGen.cpp:50-63). A 64 x 2 x 2 block computes one fine residual
+// per thread (coalesced rows) and reduces each 2x2x2 aggregate: the x pair with
+// a lane shuffle, the y/z pairs through LDS.
 // ----------------------------------------------------------------------------
 template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_residual_restrict(Op<CT> op, const float* __restrict__ b,
-                                                              const float* __restrict__ x, float* __restrict__ bc,
-                                                              int ci, int cj, int ck) {
-    const int I = blockIdx.x * 64 + threadIdx.x, J = blockIdx.y * 4 + threadIdx.y, K = blockIdx.z;
-    if (I >= ci || J >= cj || K >= ck) return;
-    float acc = 0.f;
-#pragma unroll
-    for (int kk = 0; kk < 2; ++kk)
-#pragma unroll
-        for (int jj = 0; jj < 2; ++jj)
-#pragma unroll
-            for (int ii = 0; ii < 2; ++ii) {
-                const int i = 2 * I + ii, j = 2 * J + jj, k = 2 * K + kk;
-                if (i >= op.ni || j >= op.nj || k >= op.nk) continue;
-                const int c = (k * op.nj + j) * op.ni + i;
-                const float r = b[c] - (offdiag_sum(op, x, c, i, j, k) + op.D(c) * x[c]);
-                acc += 0.125f * r;
-            }
-    bc[(K * cj + J) * ci + I] = acc;
+__global__ __launch_bounds__(256) void k_residual_restrict(Op<CT> op, const float* __restrict__ b,
+                                                           const float* __restrict__ x, float* __restrict__ bc,
+                                                           int ci, int cj, int ck) {
+    __shared__ float part[4][32];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 2 + threadIdx.y;
+    const int k = blockIdx.z * 2 + threadIdx.z;
+    float r = 0.f;
+    if (i < op.ni && j < op.nj && k < op.nk) {
+        const int c = (k * op.nj + j) * op.ni + i;
+        r = 0.125f * (b[c] - (offdiag_sum(op, x, c, i, j, k) + op.D(c) * x[c]));
+    }
+    r += __shfl_xor(r, 1, 64);
+    const int row = threadIdx.y + 2 * threadIdx.z;
+    if ((threadIdx.x & 1) == 0) part[row][threadIdx.x >> 1] = r;
+    __syncthreads();
+    if (row == 0 && (threadIdx.x & 1) == 0) {
+        const int m = threadIdx.x >> 1;
+        const int I = blockIdx.x * 32 + m, J = blockIdx.y, K = blockIdx.z;
+        if (I < ci && J < cj && K < ck)
+            bc[(K * cj + J) * ci + I] = (part[0][m] + part[1][m]) + (part[2][m] + part[3][m]);
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -260,22 +356,43 @@ __global__ __launch_bounds__(kBlock) void k_residual_restrict(Op<CT> op, const f
 // lane has work in both half sweeps.
 // ----------------------------------------------------------------------------
 constexpr int kCoarseMax = 4096;
-constexpr int kCoarseThreads = 256;
-constexpr int kCoarsePairs = 8;  // pairs per thread: up to 2048 pairs
+constexpr int kCoarseThreads = 1024;
+constexpr int kCoarsePairs = 2;  // pairs per thread: up to 2048 pairs
 
 template <typename CT>
+__device__ inline void coarse_cell(const Op<CT>& op, const float* __restrict__ b, int unit, int i, int j, int k,
+                                   int& cell, float& d, float& bb, float (&a)[6], int (&nb)[6]) {
+    const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
+    if (i >= ni) return;
+    const int c = (k * nj + j) * ni + i;
+    cell = c;
+    d = op.D(c);
+    bb = unit >= 0 ? (c == unit ? 1.f : 0.f) : b[c];
+    if (k > 0) { nb[0] = c - sl; a[0] = op.AZ(c - sl); }
+    if (j > 0) { nb[1] = c - ni; a[1] = op.AY(c - ni); }
+    if (i > 0) { nb[2] = c - 1; a[2] = op.AX(c - 1); }
+    if (i + 1 < ni) { nb[3] = c + 1; a[3] = op.AX(c); }
+    if (j + 1 < nj) { nb[4] = c + ni; a[4] = op.AY(c); }
+    if (k + 1 < nk) { nb[5] = c + sl; a[5] = op.AZ(c); }
+}
+
+// tab != nullptr: tabulation mode. Workgroup u solves with b = e_u and writes
+// the result to row u of tab (tab[u*n + r] = (RBGS^sweeps e_u)[r]); the solve
+// from a zero guess is linear in b, so y = tab^T b reproduces it (k_coarse_apply).
+template <typename CT>
 __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const float* __restrict__ b,
-                                                             float* __restrict__ x, int sweeps) {
+                                                             float* __restrict__ x, int sweeps,
+                                                             float* __restrict__ tab) {
     __shared__ float xs[kCoarseMax + 1];
+    const int unit = tab ? (int)blockIdx.x : -1;
     const int ni = op.ni, nj = op.nj, nk = op.nk;
-    const int sl = ni * nj;
     const int ppr = (ni + 1) / 2;
     const int npairs = ppr * nj * nk;
+    // [q][colour]: colour 1 = odd (i+j+k), updated first
     float cd[kCoarsePairs][2], cb[kCoarsePairs][2], ca[kCoarsePairs][2][6];
     int nbr[kCoarsePairs][2][6], cell[kCoarsePairs][2];
 #pragma unroll
     for (int q = 0; q < kCoarsePairs; ++q) {
-        // empty slots read the zero pad at xs[kCoarseMax] with zero couplings
 #pragma unroll
         for (int col = 0; col < 2; ++col) {
             cell[q][col] = -1;
@@ -283,37 +400,22 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
             cb[q][col] = 0.f;
 #pragma unroll
             for (int e = 0; e < 6; ++e) {
-                nbr[q][col][e] = kCoarseMax;
+                nbr[q][col][e] = kCoarseMax;  // zero pad
                 ca[q][col][e] = 0.f;
             }
         }
         const int P = threadIdx.x + q * kCoarseThreads;
-        if (P >= npairs) continue;
-        const int row = P / ppr, m = P - row * ppr;
-        const int j = row % nj, k = row / nj;
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            const int i = 2 * m + e;
-            if (i >= ni) continue;
-            const int c = (k * nj + j) * ni + i;
-            const int col = (i + j + k) & 1;
-            cell[q][col] = c;
-            cd[q][col] = op.D(c);
-            cb[q][col] = b[c];
-            nbr[q][col][0] = k > 0 ? c - sl : kCoarseMax;
-            ca[q][col][0] = k > 0 ? op.AZ(c - sl) : 0.f;
-            nbr[q][col][1] = j > 0 ? c - ni : kCoarseMax;
-            ca[q][col][1] = j > 0 ? op.AY(c - ni) : 0.f;
-            nbr[q][col][2] = i > 0 ? c - 1 : kCoarseMax;
-            ca[q][col][2] = i > 0 ? op.AX(c - 1) : 0.f;
-            nbr[q][col][3] = i + 1 < ni ? c + 1 : kCoarseMax;
-            ca[q][col][3] = i + 1 < ni ? op.AX(c) : 0.f;
-            nbr[q][col][4] = j + 1 < nj ? c + ni : kCoarseMax;
-            ca[q][col][4] = j + 1 < nj ? op.AY(c) : 0.f;
-            nbr[q][col][5] = k + 1 < nk ? c + sl : kCoarseMax;
-            ca[q][col][5] = k + 1 < nk ? op.AZ(c) : 0.f;
-            xs[c] = 0.f;
+        if (P < npairs) {
+            const int row = P / ppr, m = P - row * ppr;
+            const int j = row % nj, k = row / nj;
+            // cell 2m+e is odd when e == 1 - ((j+k)&1)
+            const int eodd = ((j + k) & 1) ? 0 : 1;
+            coarse_cell(op, b, unit, 2 * m + eodd, j, k, cell[q][1], cd[q][1], cb[q][1], ca[q][1], nbr[q][1]);
+            coarse_cell(op, b, unit, 2 * m + 1 - eodd, j, k, cell[q][0], cd[q][0], cb[q][0], ca[q][0], nbr[q][0]);
         }
+#pragma unroll
+        for (int col = 0; col < 2; ++col)
+            if (cell[q][col] >= 0) xs[cell[q][col]] = 0.f;
     }
     if (threadIdx.x == 0) xs[kCoarseMax] = 0.f;
     __syncthreads();
@@ -326,11 +428,11 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
                 float s = 0.f;
 #pragma unroll
                 for (int e = 0; e < 6; ++e) s += ca[q][col][e] * xs[nbr[q][col][e]];
-                nv[q] = s;
+                nv[q] = gs_value(cb[q][col], s, cd[q][col]);
             }
 #pragma unroll
             for (int q = 0; q < kCoarsePairs; ++q)
-                if (cell[q][col] >= 0) xs[cell[q][col]] = gs_value(cb[q][col], nv[q], cd[q][col]);
+                if (cell[q][col] >= 0) xs[cell[q][col]] = nv[q];
             __syncthreads();
         }
     }
@@ -338,7 +440,36 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
     for (int q = 0; q < kCoarsePairs; ++q)
 #pragma unroll
         for (int col = 0; col < 2; ++col)
-            if (cell[q][col] >= 0) x[cell[q][col]] = xs[cell[q][col]];
+            if (cell[q][col] >= 0) {
+                if (tab) tab[(size_t)unit * (op.ni * op.nj * op.nk) + cell[q][col]] = xs[cell[q][col]];
+                else x[cell[q][col]] = xs[cell[q][col]];
+            }
+}
+
+// y = tab^T b in two passes (64-column chunks, then a fixed-order sum of the
+// chunk partials): deterministic and spread over the whole GPU.
+constexpr int kCoarseChunk = 64;
+__global__ __launch_bounds__(kBlock) void k_coarse_apply_partial(const float* __restrict__ tab,
+                                                                 const float* __restrict__ b,
+                                                                 float* __restrict__ part, int n) {
+    __shared__ float bs[kCoarseChunk];
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    const int c0 = blockIdx.y * kCoarseChunk;
+    const int cn = min(kCoarseChunk, n - c0);
+    if (threadIdx.x < cn) bs[threadIdx.x] = b[c0 + threadIdx.x];
+    __syncthreads();
+    if (r >= n) return;
+    float acc = 0.f;
+    for (int q = 0; q < cn; ++q) acc += tab[(size_t)(c0 + q) * n + r] * bs[q];
+    part[(size_t)blockIdx.y * n + r] = acc;
+}
+__global__ __launch_bounds__(kBlock) void k_coarse_apply_reduce(const float* __restrict__ part, float* __restrict__ y,
+                                                                int n, int nchunks) {
+    const int r = blockIdx.x * kBlock + threadIdx.x;
+    if (r >= n) return;
+    float s = 0.f;
+    for (int q = 0; q < nchunks; ++q) s += part[(size_t)q * n + r];
+    y[r] = s;
 }
 
 // ----------------------------------------------------------------------------

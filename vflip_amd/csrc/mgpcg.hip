@@ -50,6 +50,19 @@ struct vflip_solver {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool setup_done = false;
     int zsweep_min_n = 32768;  // levels at least this large use the fused z-marching sweep
+    // coarsest-level operator RBGS^coarse_sweeps tabulated as a dense matrix
+    float* ctab = nullptr;
+    float* cpart = nullptr;
+    int ctab_sweeps = -1;
+    bool use_ctab = true;
+    // operator-change detection between assemblies
+    int* d_changed = nullptr;
+    int* h_changed = nullptr;
+    bool check_changed = false;
+    // level-0 smoother timing (profiling)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    int prof_used = 0;
 };
 
 namespace {
@@ -85,30 +98,45 @@ struct ZGeom {
 ZGeom zgeom(const LevelBuf& lv) {
     ZGeom g;
     g.ty = lv.nj >= 128 ? 8 : 4;
+    if (const char* e = getenv("VFLIP_ZTY")) {
+        const int t = atoi(e);
+        if ((t == 4 || t == 8 || t == 16) && lv.nj >= t) g.ty = t;
+    }
     g.tiles_x = (lv.ni + kZTX - 1) / kZTX;
     g.tiles_y = (lv.nj + g.ty - 1) / g.ty;
     const int tiles = g.tiles_x * g.tiles_y;
-    // about 1024 workgroups (4 per CU), z chunks of at least 4 planes
-    int zc = std::max(4, (int)((long long)lv.nk * tiles / 1024));
-    const char* env = getenv("VFLIP_ZCHUNK");
-    if (env && atoi(env) > 0) zc = atoi(env);
+    // about 2048 workgroups (8 per CU), z chunks of at least 4 planes
+    int zc = std::max(4, (int)((long long)lv.nk * tiles / 2048));
+    const char* env = getenv("VFLIP_ZCHUNK");  // applies to the finest level only (tuning)
+    if (env && atoi(env) > 0 && lv.nk >= 256) zc = atoi(env);
     zc = std::min(zc, lv.nk);
     g.zchunk = zc;
     g.blocks = tiles * ((lv.nk + zc - 1) / zc);
     return g;
 }
 
+template <int MODE, typename Acc>
+void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<float>& op, const float* b, const float* xin,
+                        float* xout, const float* xc, int ci, int cj, double* dot) {
+    if (g.ty == 16)
+        k_rbgs_zsweep<16, MODE, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
+                                                                       g.tiles_y, g.zchunk, dot);
+    else if (g.ty == 8)
+        k_rbgs_zsweep<8, MODE, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
+                                                                      g.tiles_y, g.zchunk, dot);
+    else
+        k_rbgs_zsweep<4, MODE, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
+                                                                      g.tiles_y, g.zchunk, dot);
+}
+
 template <typename Acc>
 void launch_zsweep(vflip_solver* S, const LevelBuf& lv, const float* b, const float* xin, float* xout, int mode,
                    const float* xc, int ci, int cj, double* dot) {
-    ZGeom g = zgeom(lv);
-    Op<float> op = make_op<float>(lv);
-    if (g.ty == 8)
-        k_rbgs_zsweep<8, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, mode, xc, ci, cj, g.tiles_x,
-                                                                g.tiles_y, g.zchunk, dot);
-    else
-        k_rbgs_zsweep<4, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, mode, xc, ci, cj, g.tiles_x,
-                                                                g.tiles_y, g.zchunk, dot);
+    const ZGeom g = zgeom(lv);
+    const Op<float> op = make_op<float>(lv);
+    if (mode == 1) launch_zsweep_mode<1, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
+    else if (mode == 2) launch_zsweep_mode<2, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
+    else launch_zsweep_mode<0, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
 }
 
 bool use_zsweep(const vflip_solver* S, const LevelBuf& lv) {
@@ -139,11 +167,21 @@ template <typename CT> int smooth_level(vflip_solver* S, int l, bool pre, const 
             const int mode = (it == 0) ? (pre ? 1 : 2) : 0;
             const bool last = (it == sweeps - 1);
             double* dot = (dot_done && last && l == 0) ? S->partial : nullptr;
+            const bool prof = S->prof && l == 0;
+            if (prof) {
+                while ((int)S->prof_ev.size() < S->prof_used + 2) {
+                    hipEvent_t e;
+                    VF_HIP(hipEventCreate(&e));
+                    S->prof_ev.push_back(e);
+                }
+                VF_HIP(hipEventRecord(S->prof_ev[S->prof_used++], S->stream));
+            }
             if (mixed(S))
                 launch_zsweep<double>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
             else
                 launch_zsweep<float>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
             VF_CHECK_LAUNCH();
+            if (prof) VF_HIP(hipEventRecord(S->prof_ev[S->prof_used++], S->stream));
             std::swap(*xp, *xtp);
             if (dot) {
                 S->np_last = zgeom(lv).blocks;
@@ -174,8 +212,8 @@ template <typename CT> int restrict_level(vflip_solver* S, int l) {
     LevelBuf& c = S->L[l + 1];
     const float* b = (l == 0) ? S->r : lv.b;
     const float* x = (l == 0) ? S->z : lv.x;
-    k_residual_restrict<CT><<<grid3(c.ni, c.nj, c.nk), dim3(64, 4), 0, S->stream>>>(make_op<CT>(lv), b, x, c.b, c.ni,
-                                                                                    c.nj, c.nk);
+    const dim3 g((lv.ni + 63) / 64, (lv.nj + 1) / 2, (lv.nk + 1) / 2);
+    k_residual_restrict<CT><<<g, dim3(64, 2, 2), 0, S->stream>>>(make_op<CT>(lv), b, x, c.b, c.ni, c.nj, c.nk);
     VF_CHECK_LAUNCH();
     return 0;
 }
@@ -204,10 +242,19 @@ int run_vcycle(vflip_solver* S, bool want_dot, bool* dot_done) {
             set_error("coarsest level %dx%dx%d exceeds the single-workgroup solver", lv.ni, lv.nj, lv.nk);
             return VFLIP_ERR_ARG;
         }
-        if (lv.half)
-            k_coarsest<__half><<<1, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), b, x, S->opts.coarse_sweeps);
-        else
-            k_coarsest<float><<<1, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), b, x, S->opts.coarse_sweeps);
+        if (S->ctab && S->ctab_sweeps == S->opts.coarse_sweeps) {
+            const int nc = (int)lv.n;
+            const int nch = (nc + kCoarseChunk - 1) / kCoarseChunk;
+            k_coarse_apply_partial<<<dim3((nc + kBlock - 1) / kBlock, nch), kBlock, 0, S->stream>>>(S->ctab, b,
+                                                                                                  S->cpart, nc);
+            k_coarse_apply_reduce<<<(nc + kBlock - 1) / kBlock, kBlock, 0, S->stream>>>(S->cpart, x, nc, nch);
+        } else if (lv.half) {
+            k_coarsest<__half><<<1, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), b, x, S->opts.coarse_sweeps,
+                                                                    nullptr);
+        } else {
+            k_coarsest<float><<<1, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), b, x, S->opts.coarse_sweeps,
+                                                                   nullptr);
+        }
         VF_CHECK_LAUNCH();
     }
     for (int l = nl - 2; l >= 0; --l) {
@@ -316,6 +363,11 @@ int vflip_solver_destroy(vflip_solver* S) {
         if (l.vec_alloc) (void)hipFree(l.vec_alloc);
     }
     if (S->pcg_alloc) (void)hipFree(S->pcg_alloc);
+    if (S->ctab) (void)hipFree(S->ctab);
+    if (S->cpart) (void)hipFree(S->cpart);
+    if (S->d_changed) (void)hipFree(S->d_changed);
+    if (S->h_changed) (void)hipHostFree(S->h_changed);
+    for (auto e : S->prof_ev) (void)hipEventDestroy(e);
     if (S->partial) (void)hipFree(S->partial);
     if (S->sc) (void)hipFree(S->sc);
     if (S->h_sc) (void)hipHostFree(S->h_sc);
@@ -404,6 +456,12 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
         rc = alloc((void**)&S->partial, (size_t)np * sizeof(double));
     }
     if (!rc) rc = alloc((void**)&S->sc, sizeof(Scalars));
+    if (!rc) rc = alloc((void**)&S->d_changed, sizeof(int));
+    if (!rc && hipHostMalloc((void**)&S->h_changed, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        set_error("hipHostMalloc failed");
+        rc = VFLIP_ERR_NOMEM;
+    }
+    if (const char* e = getenv("VFLIP_COARSE_TABLE")) S->use_ctab = atoi(e) != 0;
     if (!rc) {
         if (hipHostMalloc((void**)&S->h_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
             set_error("hipHostMalloc failed");
@@ -462,6 +520,14 @@ int vflip_solver_set_matrix(vflip_solver* S, const float* d, const float* ax, co
 
 int vflip_solver_setup(vflip_solver* S) {
     VF_ARG(S, "solver");
+    if (S->check_changed && S->setup_done) {
+        // the last assembly compared the new operator with the previous one
+        VF_HIP(hipMemcpyAsync(S->h_changed, S->d_changed, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+        VF_HIP(hipStreamSynchronize(S->stream));
+        S->check_changed = false;
+        if (*S->h_changed == 0) return 0;  // same operator: the hierarchy stays valid
+    }
+    S->check_changed = false;
     VF_HIP(hipEventRecord(S->e0, S->stream));
     for (size_t l = 0; l + 1 < S->L.size(); ++l) {
         LevelBuf& f = S->L[l];
@@ -495,8 +561,51 @@ int vflip_solver_setup(vflip_solver* S) {
     } else {
         for (auto& lv : S->L) lv.half = false;
     }
+    // Tabulate the coarsest solve (coarse_sweeps RB-GS sweeps from a zero guess,
+    // a linear map of b): one workgroup per unit right-hand side.
+    if (S->use_ctab) {
+        LevelBuf& lv = S->L.back();
+        const int nc = (int)lv.n;
+        if (!S->ctab) {
+            int rc = alloc((void**)&S->ctab, (size_t)nc * nc * sizeof(float));
+            if (!rc) rc = alloc((void**)&S->cpart, (size_t)nc * ((nc + kCoarseChunk - 1) / kCoarseChunk) * sizeof(float));
+            if (rc) return rc;
+        }
+        if (lv.half)
+            k_coarsest<__half><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), nullptr, nullptr,
+                                                                     S->opts.coarse_sweeps, S->ctab);
+        else
+            k_coarsest<float><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), nullptr, nullptr,
+                                                                    S->opts.coarse_sweeps, S->ctab);
+        VF_CHECK_LAUNCH();
+        S->ctab_sweeps = S->opts.coarse_sweeps;
+    }
     VF_HIP(hipEventRecord(S->e1, S->stream));
     S->setup_done = true;
+    return 0;
+}
+
+int vflip_solver_profile(vflip_solver* S, int enable) {
+    VF_ARG(S, "solver");
+    S->prof = enable != 0;
+    S->prof_used = 0;
+    return 0;
+}
+
+int vflip_solver_profile_read(vflip_solver* S, int* launches, double* total_ms, double* bytes_per_launch) {
+    VF_ARG(S, "solver");
+    VF_HIP(hipStreamSynchronize(S->stream));
+    double tot = 0;
+    for (int q = 0; q + 1 < S->prof_used; q += 2) {
+        float ms = 0.f;
+        VF_HIP(hipEventElapsedTime(&ms, S->prof_ev[q], S->prof_ev[q + 1]));
+        tot += ms;
+    }
+    if (launches) *launches = S->prof_used / 2;
+    if (total_ms) *total_ms = tot;
+    // algorithmic bytes of one fused sweep on level 0: b, x_in, diag + 3 couplings read, x_out written
+    if (bytes_per_launch) *bytes_per_launch = 28.0 * (double)S->n;
+    S->prof_used = 0;
     return 0;
 }
 
@@ -714,4 +823,12 @@ int solver_level0(vflip_solver* S, int* ni, int* nj, int* nk, float** d, float**
     return 0;
 }
 void solver_mark_matrix_changed(vflip_solver* S) { S->setup_done = false; }
+// Before an assembly that rewrites level 0: clears the device change flag; the
+// next setup only rebuilds the hierarchy when the assembly set it.
+int* solver_begin_assembly(vflip_solver* S) {
+    int* f = S->d_changed;
+    k_fill<<<1, kBlock, 0, S->stream>>>(1, (float*)f, 0.f);  // 0.0f and int 0 share the bit pattern
+    S->check_changed = true;
+    return f;
+}
 }  // namespace vflip

@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
                                                      const float* __restrict__ ws, float* __restrict__ D,
                                                      float* __restrict__ AX, float* __restrict__ AY,
                                                      float* __restrict__ AZ, float* __restrict__ LK,
-                                                     float* __restrict__ rhs) {
+                                                     float* __restrict__ rhs, int* __restrict__ changed) {
     const size_t n = (size_t)ni * nj * nk;
     size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -130,6 +130,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     size_t t = c / ni;
     int j = (int)(t % nj), k = (int)(t / nj);
     if (!(i >= 1 && i < ni - 1 && j >= 1 && j < nj - 1 && k >= 1 && k < nk - 1)) {
+        if (changed && (D[c] != 0.f || AX[c] != 0.f || AY[c] != 0.f || AZ[c] != 0.f || LK[c] != 0.f))
+            atomicOr(changed, 1);
         D[c] = 0.f; AX[c] = 0.f; AY[c] = 0.f; AZ[c] = 0.f; LK[c] = 0.f; rhs[c] = 0.f;
         return;
     }
@@ -165,11 +167,18 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
         if (sub) r += (double)((1 / h) * s);
         else r += 1.0 / (double)h * (double)s;
     }
-    D[c] = (float)d;
-    AX[c] = (i < ni - 2) ? -txp : 0.f;
-    AY[c] = (j < nj - 2) ? -typ : 0.f;
-    AZ[c] = (k < nk - 2) ? -tzp : 0.f;
-    LK[c] = (float)lk;
+    const float nd = (float)d, nx = (i < ni - 2) ? -txp : 0.f, ny = (j < nj - 2) ? -typ : 0.f;
+    const float nz = (k < nk - 2) ? -tzp : 0.f, nl = (float)lk;
+    // flag an operator change against the previous assembly (bitwise compare)
+    if (changed && (__float_as_uint(D[c]) != __float_as_uint(nd) || __float_as_uint(AX[c]) != __float_as_uint(nx) ||
+                    __float_as_uint(AY[c]) != __float_as_uint(ny) || __float_as_uint(AZ[c]) != __float_as_uint(nz) ||
+                    __float_as_uint(LK[c]) != __float_as_uint(nl)))
+        atomicOr(changed, 1);
+    D[c] = nd;
+    AX[c] = nx;
+    AY[c] = ny;
+    AZ[c] = nz;
+    LK[c] = nl;
     rhs[c] = (float)r;
 }
 
@@ -258,6 +267,7 @@ namespace vflip {
 int solver_level0(vflip_solver* s, int* ni, int* nj, int* nk, float** d, float** ax, float** ay, float** az,
                   float** leak, hipStream_t* stream);
 void solver_mark_matrix_changed(vflip_solver* s);
+int* solver_begin_assembly(vflip_solver* s);
 }  // namespace vflip
 
 extern "C" {
@@ -273,11 +283,11 @@ int vflip_assemble(vflip_solver* S, int variant, float h, float dt, const float*
     hipStream_t s;
     int rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s);
     if (rc) return rc;
+    int* changed = solver_begin_assembly(S);
     k_assemble<<<grid_for((size_t)ni * nj * nk), kBlock, 0, s>>>(ni, nj, nk, h, dt, variant == VFLIP_VARIANT_SUB, u,
                                                                  v, w, uw, vw, ww, us, vs, ws, d, ax, ay, az, lk,
-                                                                 rhs);
+                                                                 rhs, changed);
     VF_CHECK_LAUNCH();
-    solver_mark_matrix_changed(S);
     return 0;
 }
 
