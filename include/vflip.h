@@ -106,6 +106,18 @@ int vflip_solver_setup(vflip_solver* s);
  * VFLIP_ERR_BREAKDOWN. stats may be NULL. */
 int vflip_solver_solve(vflip_solver* s, const float* rhs, float* x, vflip_solve_stats* stats);
 
+/* FP64 right-hand side and solution (device double arrays). In mixed precision
+ * the iterate is accumulated in FP64 and the stop test is on the FP64 true
+ * residual |b - A x|_inf (FP64 diagonal from vflip_solver_set_diag64 or the
+ * assembly when available); in FP32 mode the rhs is rounded to FP32 and the
+ * FP32 solution is returned widened. Same return codes as vflip_solver_solve. */
+int vflip_solver_solve_f64(vflip_solver* s, const double* rhs, double* x, vflip_solve_stats* stats);
+
+/* Exact FP64 level-0 diagonal (device, ni*nj*nk doubles) for the FP64 true
+ * residual; the FP32 couplings are exact when they are single FP32 terms, as
+ * the reference's assembly produces. Valid until the next set_matrix. */
+int vflip_solver_set_diag64(vflip_solver* s, const double* diag);
+
 /* One V-cycle z = M^-1 r (zero initial guess) — the preconditioner alone,
  * S/AlgebraicMultigrid.h:130-188. */
 int vflip_solver_vcycle(vflip_solver* s, const float* r, float* z);
@@ -160,6 +172,13 @@ int vflip_grad_apply(int ni, int nj, int nk, int variant, float h, float dt, con
                      const float* uw, const float* vw, const float* ww, const float* us,
                      const float* vs, const float* ws, float* u, float* v, float* w,
                      void* hip_stream);
+
+/* Same update from an FP64 pressure: (float)(p[c] - p[c-1]) as the reference,
+ * whose pressure vector is double. */
+int vflip_grad_apply_f64(int ni, int nj, int nk, int variant, float h, float dt, const double* p,
+                         const float* uw, const float* vw, const float* ww, const float* us,
+                         const float* vs, const float* ws, float* u, float* v, float* w,
+                         void* hip_stream);
 
 /* Whole pressure projection — the body of SmokeSolver3D::pcg_projection /
  * SubDomainSolver::pcg_projection after face weights: assemble, setup, PCG,

@@ -61,6 +61,9 @@ SIGNATURES = {
     "vflip_solver_vcycle": (c_int, [P, P, P]),
     "vflip_solver_matvec": (c_int, [P, P, P]),
     "vflip_solver_get_level": (c_int, [P, c_int, P, P, P, P]),
+    "vflip_solver_solve_f64": (c_int, [P, P, P, ctypes.POINTER(SolveStats)]),
+    "vflip_solver_set_diag64": (c_int, [P, P]),
+    "vflip_grad_apply_f64": (c_int, [c_int, c_int, c_int, c_int, c_float, c_float] + [P] * 10 + [P]),
     "vflip_solver_profile": (c_int, [P, c_int]),
     "vflip_solver_profile_read": (c_int, [P, PI, PD, PD]),
     "vflip_amgpcg_solve_host": (c_int, [c_int, c_int, c_int, P, P, P, P, P, P, c_double, c_int, PD, PI,

@@ -94,6 +94,17 @@ class Solver:
         d["status"] = rc
         return d
 
+    def solve_f64(self, rhs, x):
+        st = SolveStats()
+        rc = check(lib().vflip_solver_solve_f64(self.h, _ptr(rhs, torch.float64, self.n, "rhs"),
+                                                _ptr(x, torch.float64, self.n, "x"), ctypes.byref(st)), "solve_f64")
+        d = st.as_dict()
+        d["status"] = rc
+        return d
+
+    def set_diag64(self, d):
+        check(lib().vflip_solver_set_diag64(self.h, _ptr(d, torch.float64, self.n, "diag")), "set_diag64")
+
     def vcycle(self, r, z):
         check(lib().vflip_solver_vcycle(self.h, _ptr(r, numel=self.n), _ptr(z, numel=self.n)), "vcycle")
 

@@ -591,11 +591,83 @@ __global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* _
             acc += (Acc)s[c] * (Acc)y;
         }
     }
-    const int tflat = threadIdx.y * 64 + threadIdx.x;
-    (void)tflat;
     Acc t = block_sum(acc);
     if (threadIdx.x == 0 && threadIdx.y == 0)
         partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (double)t;
+}
+
+// FP64 true residual r = b - A x of the mixed-precision refinement. The level-0
+// couplings are exact in FP32 (each is one FP32 face term); the diagonal and
+// the right-hand side come in FP64 when the assembly provided them (d64, b64),
+// else from the FP32 arrays. x is the FP64 iterate. Writes r rounded to FP32
+// and partial max|r|.
+__global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op, const double* __restrict__ d64,
+                                                            const double* __restrict__ b64,
+                                                            const float* __restrict__ b32,
+                                                            const double* __restrict__ x, float* __restrict__ r,
+                                                            double* __restrict__ partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int kb = blockIdx.z * kMatvecKZ;
+    const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
+    double m = 0;
+    if (i < ni && j < nj) {
+        for (int k = kb; k < min(kb + kMatvecKZ, nk); ++k) {
+            const int c = (k * nj + j) * ni + i;
+            double y = (d64 ? d64[c] : (double)op.D(c)) * x[c];
+            if (k > 0) y += (double)op.AZ(c - sl) * x[c - sl];
+            if (j > 0) y += (double)op.AY(c - ni) * x[c - ni];
+            if (i > 0) y += (double)op.AX(c - 1) * x[c - 1];
+            if (i + 1 < ni) y += (double)op.AX(c) * x[c + 1];
+            if (j + 1 < nj) y += (double)op.AY(c) * x[c + ni];
+            if (k + 1 < nk) y += (double)op.AZ(c) * x[c + sl];
+            const double rv = (b64 ? b64[c] : (double)b32[c]) - y;
+            if (r) r[c] = (float)rv;
+            m = fmax(m, fabs(rv));
+        }
+    }
+    double t = block_max(m);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+}
+
+// x64 += alpha s; r -= alpha q; partial max|r| (mixed mode, FP64 iterate)
+__global__ __launch_bounds__(kBlock) void k_update_x64r(unsigned n, const Scalars* __restrict__ sc,
+                                                        double* __restrict__ x, const float* __restrict__ s,
+                                                        float* __restrict__ r, const float* __restrict__ q,
+                                                        double* __restrict__ partial) {
+    const double alpha = sc->alpha;
+    float m = 0.f;
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) {
+        x[c] += alpha * (double)s[c];
+        const float rv = (float)((double)r[c] - alpha * (double)q[c]);
+        r[c] = rv;
+        m = fmaxf(m, fabsf(rv));
+    }
+    float t = block_max(m);
+    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_absmax64(unsigned n, const double* __restrict__ a,
+                                                     double* __restrict__ partial) {
+    double m = 0;
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) m = fmax(m, fabs(a[c]));
+    double t = block_max(m);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cast_f64_f32(unsigned n, const double* __restrict__ a,
+                                                         float* __restrict__ o) {
+    unsigned c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) o[c] = (float)a[c];
+}
+__global__ __launch_bounds__(kBlock) void k_cast_f32_f64(unsigned n, const float* __restrict__ a,
+                                                         double* __restrict__ o) {
+    unsigned c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) o[c] = (double)a[c];
+}
+__global__ __launch_bounds__(kBlock) void k_fill64(unsigned n, double* __restrict__ a, double v) {
+    unsigned c = blockIdx.x * kBlock + threadIdx.x;
+    if (c < n) a[c] = v;
 }
 
 // r = b - A x in FP64 (true residual); partial max|r|. r may be null.

@@ -40,8 +40,12 @@ struct vflip_solver {
     vflip_solve_opts opts;
     hipStream_t stream = nullptr;
     std::vector<LevelBuf> L;
-    float *r = nullptr, *z = nullptr, *zt = nullptr, *s = nullptr, *q = nullptr, *xw = nullptr;
+    float *r = nullptr, *z = nullptr, *zt = nullptr, *s = nullptr, *q = nullptr, *xw = nullptr, *b32 = nullptr;
     void* pcg_alloc = nullptr;
+    // FP64 iterate / level-0 diagonal / right-hand side (mixed-precision refinement)
+    double *x64 = nullptr, *d64 = nullptr, *b64 = nullptr;
+    bool d64_valid = false;  // d64 holds the assembled FP64 diagonal of the current operator
+    bool b64_valid = false;  // b64 holds the assembled FP64 right-hand side
     double* partial = nullptr;
     int np_partial = 0;  // partial buffer capacity
     int np_last = 0;     // partials written by the last reduction
@@ -66,6 +70,8 @@ struct vflip_solver {
 };
 
 namespace {
+
+int ensure_f64(vflip_solver* S);
 
 template <typename CT> Op<CT> make_op(const LevelBuf& l);
 template <> Op<float> make_op<float>(const LevelBuf& l) {
@@ -363,6 +369,7 @@ int vflip_solver_destroy(vflip_solver* S) {
         if (l.vec_alloc) (void)hipFree(l.vec_alloc);
     }
     if (S->pcg_alloc) (void)hipFree(S->pcg_alloc);
+    if (S->x64) (void)hipFree(S->x64);
     if (S->ctab) (void)hipFree(S->ctab);
     if (S->cpart) (void)hipFree(S->cpart);
     if (S->d_changed) (void)hipFree(S->d_changed);
@@ -441,11 +448,11 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
         }
     }
     if (!rc) {
-        rc = alloc(&S->pcg_alloc, 6 * S->n * sizeof(float));
+        rc = alloc(&S->pcg_alloc, 7 * S->n * sizeof(float));
         if (!rc) {
             float* p = (float*)S->pcg_alloc;
             S->r = p; S->z = p + S->n; S->zt = p + 2 * S->n; S->s = p + 3 * S->n; S->q = p + 4 * S->n;
-            S->xw = p + 5 * S->n;
+            S->xw = p + 5 * S->n; S->b32 = p + 6 * S->n;
         }
     }
     if (!rc) {
@@ -515,6 +522,17 @@ int vflip_solver_set_matrix(vflip_solver* S, const float* d, const float* ax, co
         VF_CHECK_LAUNCH();
     }
     S->setup_done = false;
+    S->d64_valid = false;
+    S->b64_valid = false;
+    return 0;
+}
+
+int vflip_solver_set_diag64(vflip_solver* S, const double* d64) {
+    VF_ARG(S && d64, "solver/diag");
+    int rc = ensure_f64(S);
+    if (rc) return rc;
+    VF_HIP(hipMemcpyAsync(S->d64, d64, S->n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+    S->d64_valid = true;
     return 0;
 }
 
@@ -645,8 +663,37 @@ int vflip_solver_matvec(vflip_solver* S, const float* x, float* y) {
     return 0;
 }
 
-int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_stats* st) {
-    VF_ARG(S && rhs && x, "solver/rhs/x");
+}  // extern "C"
+
+namespace {
+
+int ensure_f64(vflip_solver* S) {
+    if (S->x64) return 0;
+    int rc = alloc((void**)&S->x64, 3 * S->n * sizeof(double));
+    if (rc) return rc;
+    S->d64 = S->x64 + S->n;
+    S->b64 = S->d64 + S->n;
+    return 0;
+}
+
+void launch_true_residual64(vflip_solver* S, const double* b64, const float* b32, const double* x, float* r) {
+    const LevelBuf& l0 = S->L[0];
+    dim3 g = grid3z(l0.ni, l0.nj, l0.nk);
+    k_true_residual64<<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr, b64, b32,
+                                                         x, r, S->partial);
+    S->np_last = nblocks(g);
+}
+
+// PCG driver for both precisions (S/AlgebraicMultigrid.h:189-301).
+//   plain : FP32 iterate; stop on the recursive |r|_inf as the reference.
+//   mixed : FP64 iterate x64 (x += alpha s in FP64), FP64 dots and scalars; the
+//           residual is replaced by the FP64 true residual b - A x (FP64 diagonal
+//           and right-hand side when the assembly provided them) every
+//           refine_every iterations and whenever the recursive residual meets
+//           the tolerance; the stop test is on that true residual.
+// rhs32 is required; rhs64 (optional) is the FP64 right-hand side.
+int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* x32, double* x64out,
+               vflip_solve_stats* st) {
     int rc;
     float ms_setup = 0.f;
     if (!S->setup_done) {
@@ -655,21 +702,32 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
         VF_HIP(hipEventSynchronize(S->e1));
         VF_HIP(hipEventElapsedTime(&ms_setup, S->e0, S->e1));
     }
+    if ((rc = ensure_f64(S))) return rc;
     const unsigned n = (unsigned)S->n;
     const bool mx = mixed(S);
+    const unsigned gn = grid_for(n);
     VF_HIP(hipEventRecord(S->e0, S->stream));
     // |b|_inf and the stop threshold (S/AlgebraicMultigrid.h:212-233)
-    launch_absmax(S, n, rhs);
+    if (rhs64) {
+        k_absmax64<<<kVecBlocks, kBlock, 0, S->stream>>>(n, rhs64, S->partial);
+        S->np_last = kVecBlocks;
+    } else {
+        launch_absmax(S, n, rhs32);
+    }
     VF_CHECK_LAUNCH();
     if ((rc = finalize(S, 4, S->opts.tolerance))) return rc;
     if (S->opts.warm_start) {
-        VF_HIP(hipMemcpyAsync(S->xw, x, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
-        launch_true_residual(S, rhs, S->xw, S->r);
+        // initial guess from x32 (or x64out when given)
+        if (x64out) VF_HIP(hipMemcpyAsync(S->x64, x64out, n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+        else k_cast_f32_f64<<<gn, kBlock, 0, S->stream>>>(n, x32, S->x64);
+        if (!mx) k_cast_f64_f32<<<gn, kBlock, 0, S->stream>>>(n, S->x64, S->xw);
+        launch_true_residual64(S, mx ? rhs64 : nullptr, rhs32, S->x64, S->r);
         VF_CHECK_LAUNCH();
         if ((rc = finalize(S, 2))) return rc;
     } else {
-        k_fill<<<grid_for(n), kBlock, 0, S->stream>>>(n, S->xw, 0.f);
-        k_copy<<<grid_for(n), kBlock, 0, S->stream>>>(n, S->r, rhs);
+        if (mx) k_fill64<<<gn, kBlock, 0, S->stream>>>(n, S->x64, 0.0);
+        else k_fill<<<gn, kBlock, 0, S->stream>>>(n, S->xw, 0.f);
+        k_copy<<<gn, kBlock, 0, S->stream>>>(n, S->r, rhs32);
         VF_CHECK_LAUNCH();
     }
     if ((rc = fetch_scalars(S))) return rc;
@@ -681,7 +739,8 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
     int status = VFLIP_OK;
     int it = 0;
     if (bnorm == 0) {
-        k_fill<<<grid_for(n), kBlock, 0, S->stream>>>(n, x, 0.f);
+        if (mx) k_fill64<<<gn, kBlock, 0, S->stream>>>(n, S->x64, 0.0);
+        else k_fill<<<gn, kBlock, 0, S->stream>>>(n, S->xw, 0.f);
         VF_CHECK_LAUNCH();
         stats.converged = 1;
         res = 0;
@@ -693,23 +752,35 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
         if (S->h_sc->breakdown) {
             status = VFLIP_ERR_BREAKDOWN;
         } else {
-            k_copy<<<grid_for(n), kBlock, 0, S->stream>>>(n, S->s, S->z);
+            k_copy<<<gn, kBlock, 0, S->stream>>>(n, S->s, S->z);
             VF_CHECK_LAUNCH();
             status = VFLIP_NOT_CONVERGED;
             for (it = 0; it < S->opts.max_iterations; ++it) {
                 launch_matvec_dot(S, S->s, S->q);
                 VF_CHECK_LAUNCH();
                 if ((rc = finalize(S, 1))) return rc;
-                launch_update_xr(S);
+                if (mx) {
+                    k_update_x64r<<<kVecBlocks, kBlock, 0, S->stream>>>(n, S->sc, S->x64, S->s, S->r, S->q,
+                                                                        S->partial);
+                    S->np_last = kVecBlocks;
+                } else {
+                    launch_update_xr(S);
+                }
                 VF_CHECK_LAUNCH();
                 if ((rc = finalize(S, 2))) return rc;
-                if (mx && S->opts.refine_every > 0 && ((it + 1) % S->opts.refine_every) == 0) {
-                    launch_true_residual(S, rhs, S->xw, S->r);
-                    VF_CHECK_LAUNCH();
-                    if ((rc = finalize(S, 2))) return rc;
-                }
                 if ((rc = fetch_scalars(S))) return rc;
                 res = S->h_sc->res;
+                if (mx) {
+                    const bool periodic = S->opts.refine_every > 0 && ((it + 1) % S->opts.refine_every) == 0;
+                    if (periodic || res <= tol_abs) {
+                        // residual replacement with the FP64 true residual
+                        launch_true_residual64(S, rhs64, rhs32, S->x64, S->r);
+                        VF_CHECK_LAUNCH();
+                        if ((rc = finalize(S, 2))) return rc;
+                        if ((rc = fetch_scalars(S))) return rc;
+                        res = S->h_sc->res;
+                    }
+                }
                 if (res <= tol_abs) {
                     status = VFLIP_OK;
                     ++it;
@@ -721,11 +792,22 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
             }
             stats.converged = (status == VFLIP_OK);
         }
-        VF_HIP(hipMemcpyAsync(x, S->xw, S->n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
     }
+    // outputs
+    if (mx) {
+        if (x32) k_cast_f64_f32<<<gn, kBlock, 0, S->stream>>>(n, S->x64, x32);
+        if (x64out && x64out != S->x64)
+            VF_HIP(hipMemcpyAsync(x64out, S->x64, n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+    } else {
+        if (x32) VF_HIP(hipMemcpyAsync(x32, S->xw, n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+        k_cast_f32_f64<<<gn, kBlock, 0, S->stream>>>(n, S->xw, S->x64);
+        if (x64out && x64out != S->x64)
+            VF_HIP(hipMemcpyAsync(x64out, S->x64, n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+    }
+    VF_CHECK_LAUNCH();
     VF_HIP(hipEventRecord(S->e1, S->stream));
-    // FP64 true residual of the returned solution, for the stats
-    launch_true_residual(S, rhs, x, nullptr);
+    // FP64 true residual of the returned solution (FP64 operator/rhs when available), for the stats
+    launch_true_residual64(S, rhs64, rhs32, S->x64, nullptr);
     VF_CHECK_LAUNCH();
     if ((rc = finalize(S, 5))) return rc;
     if ((rc = fetch_scalars(S))) return rc;
@@ -738,6 +820,25 @@ int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_
     stats.true_residual = bnorm > 0 ? S->h_sc->true_res / bnorm : 0.0;
     if (st) *st = stats;
     return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_stats* st) {
+    VF_ARG(S && rhs && x, "solver/rhs/x");
+    return solve_impl(S, rhs, nullptr, x, nullptr, st);
+}
+
+int vflip_solver_solve_f64(vflip_solver* S, const double* rhs, double* x, vflip_solve_stats* st) {
+    VF_ARG(S && rhs && x, "solver/rhs/x");
+    int rc = ensure_f64(S);
+    if (rc) return rc;
+    // FP32 copy of the right-hand side for the FP32 parts of the iteration
+    k_cast_f64_f32<<<grid_for(S->n), kBlock, 0, S->stream>>>((unsigned)S->n, rhs, S->b32);
+    VF_CHECK_LAUNCH();
+    return solve_impl(S, S->b32, rhs, nullptr, x, st);
 }
 
 int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const double* ax, const double* ay,
@@ -778,7 +879,10 @@ int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const do
                 hl[c] = (float)(l > 0 ? l : 0.0);
             }
     float* dev = nullptr;
-    if (hipMalloc((void**)&dev, 7 * n * sizeof(float)) != hipSuccess) {
+    double* dev64 = nullptr;
+    if (hipMalloc((void**)&dev, 6 * n * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&dev64, 3 * n * sizeof(double)) != hipSuccess) {
+        if (dev) (void)hipFree(dev);
         vflip_solver_destroy(S);
         set_error("hipMalloc failed");
         return VFLIP_ERR_NOMEM;
@@ -786,26 +890,31 @@ int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const do
     vflip_solve_stats st{};
     int status = 0;
     do {
-        if (hipMemcpy(dev, h.data(), 6 * n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        // FP32 operator for the preconditioner and Krylov products; the exact FP64
+        // diagonal and right-hand side for the true residual (mixed mode)
+        if (hipMemcpy(dev, h.data(), 6 * n * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dev64, diag, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dev64 + n, rhs, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
             set_error("H2D copy failed");
             status = VFLIP_ERR_HIP;
             break;
         }
         status = vflip_solver_set_matrix(S, dev, dev + n, dev + 2 * n, dev + 3 * n, dev + 4 * n);
         if (status < 0) break;
-        status = vflip_solver_solve(S, dev + 5 * n, dev + 6 * n, &st);
+        status = vflip_solver_set_diag64(S, dev64);
+        if (status < 0) break;
+        status = vflip_solver_solve_f64(S, dev64 + n, dev64 + 2 * n, &st);
         if (status < 0 && status != VFLIP_ERR_BREAKDOWN) break;
-        std::vector<float> xo(n);
-        if (hipMemcpy(xo.data(), dev + 6 * n, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+        if (hipMemcpy(result, dev64 + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
             set_error("D2H copy failed");
             status = VFLIP_ERR_HIP;
             break;
         }
-        for (size_t c = 0; c < n; ++c) result[c] = xo[c];
         if (residual_out) *residual_out = st.residual;
         if (iterations_out) *iterations_out = st.iterations;
     } while (0);
     (void)hipFree(dev);
+    (void)hipFree(dev64);
     vflip_solver_destroy(S);
     return status;
 }
@@ -823,6 +932,25 @@ int solver_level0(vflip_solver* S, int* ni, int* nj, int* nk, float** d, float**
     return 0;
 }
 void solver_mark_matrix_changed(vflip_solver* S) { S->setup_done = false; }
+// FP64 level-0 buffers for the assembly (diagonal, right-hand side) and the FP64 iterate.
+int solver_f64(vflip_solver* S, double** d64, double** b64, double** x64) {
+    int rc = ensure_f64(S);
+    if (rc) return rc;
+    *d64 = S->d64;
+    *b64 = S->b64;
+    *x64 = S->x64;
+    return 0;
+}
+void solver_mark_f64_assembled(vflip_solver* S) {
+    S->d64_valid = true;
+    S->b64_valid = true;
+}
+bool solver_is_mixed(const vflip_solver* S) { return S->opts.precision == VFLIP_PREC_MIXED; }
+// Solve the assembled system: FP32 rhs, plus the FP64 rhs in mixed mode.
+int solver_solve_assembled(vflip_solver* S, const float* rhs32, float* x32, vflip_solve_stats* st) {
+    const double* b64 = (S->opts.precision == VFLIP_PREC_MIXED && S->b64_valid) ? S->b64 : nullptr;
+    return solve_impl(S, rhs32, b64, x32, nullptr, st);
+}
 // Before an assembly that rewrites level 0: clears the device change flag; the
 // next setup only rebuilds the hierarchy when the assembly set it.
 int* solver_begin_assembly(vflip_solver* S) {

@@ -122,7 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
                                                      const float* __restrict__ ws, float* __restrict__ D,
                                                      float* __restrict__ AX, float* __restrict__ AY,
                                                      float* __restrict__ AZ, float* __restrict__ LK,
-                                                     float* __restrict__ rhs, int* __restrict__ changed) {
+                                                     float* __restrict__ rhs, int* __restrict__ changed,
+                                                     double* __restrict__ D64, double* __restrict__ R64) {
     const size_t n = (size_t)ni * nj * nk;
     size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -131,8 +132,9 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     int j = (int)(t % nj), k = (int)(t / nj);
     if (!(i >= 1 && i < ni - 1 && j >= 1 && j < nj - 1 && k >= 1 && k < nk - 1)) {
         if (changed && (D[c] != 0.f || AX[c] != 0.f || AY[c] != 0.f || AZ[c] != 0.f || LK[c] != 0.f))
-            atomicOr(changed, 1);
+            *changed = 1;  // every writer stores the same value
         D[c] = 0.f; AX[c] = 0.f; AY[c] = 0.f; AZ[c] = 0.f; LK[c] = 0.f; rhs[c] = 0.f;
+        if (D64) { D64[c] = 0.0; R64[c] = 0.0; }
         return;
     }
     const float h2 = h * h;
@@ -173,7 +175,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     if (changed && (__float_as_uint(D[c]) != __float_as_uint(nd) || __float_as_uint(AX[c]) != __float_as_uint(nx) ||
                     __float_as_uint(AY[c]) != __float_as_uint(ny) || __float_as_uint(AZ[c]) != __float_as_uint(nz) ||
                     __float_as_uint(LK[c]) != __float_as_uint(nl)))
-        atomicOr(changed, 1);
+        *changed = 1;  // every writer stores the same value
+    if (D64) {
+        D64[c] = d;  // the reference's FP64 diagonal and right-hand side
+        R64[c] = r;
+    }
     D[c] = nd;
     AX[c] = nx;
     AY[c] = ny;
@@ -183,8 +189,11 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
 }
 
 // u -= grad p, S/Smoke_solver3D.cpp:1126-1200 and S/sub_domain.cpp:745-819.
+// P = double reproduces the reference exactly: (float)(p[c] - p[c-1]) with an
+// FP64 pressure. P = float is the plain FP32 path.
+template <typename P>
 __global__ __launch_bounds__(kBlock) void k_grad(int axis, int ni, int nj, int nk, int sub, float h, float dt,
-                                                 const float* __restrict__ p, const float* __restrict__ wgt,
+                                                 const P* __restrict__ p, const float* __restrict__ wgt,
                                                  const float* __restrict__ sol, float* __restrict__ vel) {
     const int fx = ni + (axis == 0), fy = nj + (axis == 1), fz = nk + (axis == 2);
     const size_t nf = (size_t)fx * fy * fz;
@@ -207,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(int axis, int ni, int nj, int n
     if (wgt[f] > 0) {
         size_t c = lin3(i, j, k, ni, nj);
         size_t cm = (axis == 0) ? c - 1 : (axis == 1) ? c - ni : c - (size_t)ni * nj;
-        float g = p[c] - p[cm];
+        const float g = (float)(p[c] - p[cm]);
         vel[f] -= sub ? dt * g / h : g / h;
     } else {
         vel[f] = sol[f];
@@ -251,9 +260,26 @@ int vflip_grad_apply(int ni, int nj, int nk, int variant, float h, float dt, con
     hipStream_t s = (hipStream_t)stream;
     int sub = variant == VFLIP_VARIANT_SUB;
     size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
-    k_grad<<<grid_for(nu), kBlock, 0, s>>>(0, ni, nj, nk, sub, h, dt, p, uw, us, u);
-    k_grad<<<grid_for(nv), kBlock, 0, s>>>(1, ni, nj, nk, sub, h, dt, p, vw, vs, v);
-    k_grad<<<grid_for(nw), kBlock, 0, s>>>(2, ni, nj, nk, sub, h, dt, p, ww, ws, w);
+    k_grad<float><<<grid_for(nu), kBlock, 0, s>>>(0, ni, nj, nk, sub, h, dt, p, uw, us, u);
+    k_grad<float><<<grid_for(nv), kBlock, 0, s>>>(1, ni, nj, nk, sub, h, dt, p, vw, vs, v);
+    k_grad<float><<<grid_for(nw), kBlock, 0, s>>>(2, ni, nj, nk, sub, h, dt, p, ww, ws, w);
+    VF_CHECK_LAUNCH();
+    VF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int vflip_grad_apply_f64(int ni, int nj, int nk, int variant, float h, float dt, const double* p, const float* uw,
+                         const float* vw, const float* ww, const float* us, const float* vs, const float* ws,
+                         float* u, float* v, float* w, void* stream) {
+    VF_ARG(ni > 2 && nj > 2 && nk > 2, "grid dims");
+    VF_ARG(p && uw && vw && ww && us && vs && ws && u && v && w, "grad arrays");
+    VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
+    hipStream_t s = (hipStream_t)stream;
+    int sub = variant == VFLIP_VARIANT_SUB;
+    size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
+    k_grad<double><<<grid_for(nu), kBlock, 0, s>>>(0, ni, nj, nk, sub, h, dt, p, uw, us, u);
+    k_grad<double><<<grid_for(nv), kBlock, 0, s>>>(1, ni, nj, nk, sub, h, dt, p, vw, vs, v);
+    k_grad<double><<<grid_for(nw), kBlock, 0, s>>>(2, ni, nj, nk, sub, h, dt, p, ww, ws, w);
     VF_CHECK_LAUNCH();
     VF_HIP(hipStreamSynchronize(s));
     return 0;
@@ -268,6 +294,10 @@ int solver_level0(vflip_solver* s, int* ni, int* nj, int* nk, float** d, float**
                   float** leak, hipStream_t* stream);
 void solver_mark_matrix_changed(vflip_solver* s);
 int* solver_begin_assembly(vflip_solver* s);
+int solver_f64(vflip_solver* s, double** d64, double** b64, double** x64);
+void solver_mark_f64_assembled(vflip_solver* s);
+bool solver_is_mixed(const vflip_solver* s);
+int solver_solve_assembled(vflip_solver* s, const float* rhs32, float* x32, vflip_solve_stats* st);
 }  // namespace vflip
 
 extern "C" {
@@ -283,11 +313,15 @@ int vflip_assemble(vflip_solver* S, int variant, float h, float dt, const float*
     hipStream_t s;
     int rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s);
     if (rc) return rc;
+    double *d64, *b64, *x64;
+    rc = solver_f64(S, &d64, &b64, &x64);
+    if (rc) return rc;
     int* changed = solver_begin_assembly(S);
     k_assemble<<<grid_for((size_t)ni * nj * nk), kBlock, 0, s>>>(ni, nj, nk, h, dt, variant == VFLIP_VARIANT_SUB, u,
                                                                  v, w, uw, vw, ww, us, vs, ws, d, ax, ay, az, lk,
-                                                                 rhs, changed);
+                                                                 rhs, changed, d64, b64);
     VF_CHECK_LAUNCH();
+    solver_mark_f64_assembled(S);
     return 0;
 }
 
@@ -299,14 +333,22 @@ int vflip_project(vflip_solver* S, int variant, float h, float dt, float* u, flo
     if (rc) return rc;
     rc = vflip_solver_setup(S);
     if (rc) return rc;
-    int status = vflip_solver_solve(S, rhs_scratch, pressure, stats);
+    int status = solver_solve_assembled(S, rhs_scratch, pressure, stats);
     if (status < 0 && status != VFLIP_ERR_BREAKDOWN) return status;
     int ni, nj, nk;
     float *d, *ax, *ay, *az, *lk;
     hipStream_t s;
     rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s);
     if (rc) return rc;
-    rc = vflip_grad_apply(ni, nj, nk, variant, h, dt, pressure, uw, vw, ww, us, vs, ws, u, v, w, s);
+    if (solver_is_mixed(S)) {
+        // gradient of the FP64 pressure, as the reference (double pressure vector)
+        double *d64, *b64, *x64;
+        rc = solver_f64(S, &d64, &b64, &x64);
+        if (rc) return rc;
+        rc = vflip_grad_apply_f64(ni, nj, nk, variant, h, dt, x64, uw, vw, ww, us, vs, ws, u, v, w, s);
+    } else {
+        rc = vflip_grad_apply(ni, nj, nk, variant, h, dt, pressure, uw, vw, ww, us, vs, ws, u, v, w, s);
+    }
     if (rc) return rc;
     return status;
 }
