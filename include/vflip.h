@@ -214,6 +214,70 @@ int vflip_plume_force(int ni, int nj, int nk, float h, float dt, float cx, float
                       unsigned seed, float jitter, float alpha, float beta, float* rho, float* temp, float* v,
                       void* hip_stream);
 
+/* -------------------------------------------------------------------- FLIP */
+/* Particles are xyz-interleaved float arrays pos[3*np], vel[3*np]: the pos and
+ * vel members of the reference's particle (S/fluid_particle.h:14-65). The
+ * FLIP calls below are asynchronous on their stream unless stated. */
+typedef struct vflip_particles vflip_particles;
+
+/* Binning and transfer workspace for an ni x nj x nk cell grid of spacing h
+ * (the reference's per-cell particle_hash lists, S/Smoke_solver3D.h:205-212).
+ * Device scratch grows with the particle count. */
+int vflip_particles_create(int ni, int nj, int nk, float h, vflip_particles** out);
+int vflip_particles_destroy(vflip_particles* P);
+int vflip_particles_set_stream(vflip_particles* P, void* hip_stream);
+
+/* hash_FLIP — SmokeSolver3D, S/Smoke_solver3D.cpp:1327-1346 (clamp = 0:
+ * particles outside the grid are dropped) and SubDomainSolver,
+ * S/sub_domain.cpp:1087-1107 (clamp = 1: positions relative to bmin, cell
+ * indices clamped into the grid). cell = trunc((p [- bmin] + h/2) / h) in float.
+ * Outputs: cell_key[np] (linear cell index, -1 when dropped), cell_start[ncell+1]
+ * (offsets into order), order[np] (particle ids grouped by cell, ascending id
+ * within a cell = the append order of the reference's per-cell lists).
+ * Bit-exact. *nbinned (host, may be NULL) receives cell_start[ncell]; passing it
+ * makes the call synchronous. */
+int vflip_hash(vflip_particles* P, int np, const float* pos, int clamp, float bx, float by, float bz,
+               int* cell_key, int* cell_start, int* order, int* nbinned);
+
+/* particle_to_grid — S/Smoke_solver3D.cpp:1451-1510 (VFLIP_VARIANT_COARSE) and
+ * S/sub_domain.cpp:1204-1265 (VFLIP_VARIANT_SUB: positions relative to bmin;
+ * faces within 4 cells of the boundary use a 4h kernel over 9^3 cells,
+ * S/sub_domain.h:89-103). For every MAC face: coef = 1e-5 + sum w,
+ * field = sum w*vel_c / coef, w = tent weights H(dx/h)H(dy/h)H(dz/h)
+ * (S/Smoke_solver3D.h:217-233) over the particles of the 3^3 cells around the
+ * face. Takes vflip_hash's cell_start/order. method 0: LDS-binned scatter, then
+ * global atomics (FP32 sums in another order than the reference's); method 1:
+ * per-face gather in the reference's summation order (bit-exact). A component
+ * with NULL field and coef is skipped. */
+int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* pos, const float* vel,
+              const int* cell_start, const int* order, float bx, float by, float bz, float* u, float* u_coef,
+              float* v, float* v_coef, float* w, float* w_coef);
+
+/* getDelta — S/Smoke_solver3D.cpp:1297-1318: out = fnew - fold (n floats). */
+int vflip_get_delta(size_t n, const float* fnew, const float* fold, float* out, void* hip_stream);
+
+/* particle_from_grid — S/Smoke_solver3D.cpp:1398-1406: vel += trilinear(u_new -
+ * u_old) at pos - (bx,by,bz), per component (the reference samples _utemp after
+ * getDelta; the difference is formed per corner here, with the same float
+ * rounding). With u_old == NULL, u_new/v_new/w_new are the delta grids.
+ * flip_ratio < 1 blends with PIC in double as S/fluid_particle.h:297-336:
+ * vel = (1 - r)*u_new(pos) + r*(vel + du(pos)); it needs the old grids.
+ * Bit-exact. */
+int vflip_g2p(int ni, int nj, int nk, float h, int np, const float* pos, float* vel, const float* u_new,
+              const float* v_new, const float* w_new, const float* u_old, const float* v_old,
+              const float* w_old, float flip_ratio, float bx, float by, float bz, void* hip_stream);
+
+/* advect_FLIP — traceRK3 then solid push-out (pos -= phi * normalize(grad phi)
+ * where phi < 0). VFLIP_VARIANT_COARSE: S/Smoke_solver3D.cpp:141-154, 1408-1430
+ * (c3 = (1 - c1 - c2)*dt with c1 = 2/9 dt, c2 = 3/9 dt already scaled by dt);
+ * VFLIP_VARIANT_SUB: S/sub_domain.cpp:116-129, 1180-1202 (c3 = 4/9 dt, velocity
+ * at pos - bmin, phi at the unshifted pos). Trilinear velocity (without the
+ * subdomain blend of S/Smoke_solver3D.cpp:71-88); solid_phi is the nodal SDF
+ * (ni+1, nj+1, nk+1). Bit-exact. */
+int vflip_advect_particles(int ni, int nj, int nk, float h, float dt, int variant, float bx, float by, float bz,
+                           int np, float* pos, const float* u, const float* v, const float* w,
+                           const float* solid_phi, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

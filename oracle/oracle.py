@@ -73,6 +73,11 @@ def lib():
         L.or_num_threads.restype = c_int
         L.or_plume_force.argtypes = ([c_int, c_int, c_int] + [c_float] * 6 + [ctypes.c_uint] + [c_float] * 3 +
                                      [f32p] * 3)
+        L.or_sample_grad.argtypes = [f32p, c_int, c_int, c_int] + [c_double] * 7 + [f32p]
+        L.or_normalize3.argtypes = [f32p]
+        L.or_advect_particles.argtypes = ([c_int, f32p, f32p, f32p, f32p, f32p, c_int, c_int, c_int, c_float,
+                                          c_float, c_int, c_float, c_float, c_float])
+        L.or_get_delta.argtypes = [c_int, f32p, f32p, f32p]
         _LIB = L
     return _LIB
 
@@ -95,6 +100,8 @@ def ref_lib():
                                  c_int, f64p, f32p, f32p, f32p])
         R.ref_blocked_index.restype = ctypes.c_uint
         R.ref_blocked_index.argtypes = [c_int] * 6
+        R.ref_normalize.argtypes = [c_int, f32p, f32p]
+        R.ref_cell_index.argtypes = [c_int, f32p, c_float, c_int, c_float, c_float, c_float, i32p]
         _REF = R
     return _REF
 
@@ -244,3 +251,39 @@ def plume_force(h, dt, center, r, seed, jitter, alpha, beta, rho, temp, v):
     lib().or_plume_force(ni, nj, nk, h, dt, center[0], center[1], center[2], r, seed, jitter, alpha, beta, rho, temp,
                          v)
     return rho, temp, v
+
+
+def sample_grad(f, h, offs, pts):
+    """Buffer3D::sample_grad at each world point of pts (n,3 float64)."""
+    fz, fy, fx = f.shape
+    f = c32(f)
+    out = np.zeros((len(pts), 3), np.float32)
+    g = np.zeros(3, np.float32)
+    for n, (x, y, z) in enumerate(np.asarray(pts, np.float64)):
+        lib().or_sample_grad(f, fx, fy, fz, h, offs[0], offs[1], offs[2], x, y, z, g)
+        out[n] = g
+    return out
+
+
+def normalize(vecs):
+    out = c32(vecs).reshape(-1, 3).copy()
+    for n in range(len(out)):
+        row = np.ascontiguousarray(out[n])
+        lib().or_normalize3(row)
+        out[n] = row
+    return out
+
+
+def advect_particles(pos, h, dt, u, v, w, phi, sub=False, bmin=(0.0, 0.0, 0.0)):
+    """traceRK3 + solid push-out (advect_FLIP) of xyz positions; returns new positions."""
+    nk, nj, ni1 = u.shape
+    out = c32(pos).reshape(-1).copy()
+    lib().or_advect_particles(out.size // 3, out, c32(u), c32(v), c32(w), c32(phi), ni1 - 1, nj, nk, h, dt,
+                              int(sub), *[float(b) for b in bmin])
+    return out.reshape(-1, 3)
+
+
+def get_delta(new, old):
+    out = np.zeros_like(c32(new))
+    lib().or_get_delta(out.size, c32(new), c32(old), out)
+    return out

@@ -6,6 +6,7 @@
 //   fraction_inside(2), fraction_inside(4)  S/levelset_util.cpp:4-103
 //   Buffer3D<float>::sample_linear/_cubic/_cube_lerp/_grad   S/fluid_buffer3D.h:86-376
 //   randhash                                S/util.h:238-246
+//   normalize, Vec3f/Vec3i arithmetic       S/vec.h (hash_FLIP cell index)
 // The standard headers below are included first because the reference headers
 // rely on the MSVC toolchain pulling them in implicitly.
 #include <climits>
@@ -45,6 +46,29 @@ void ref_sample(int kind, const float* field, int nx, int ny, int nz, double h, 
         }
     }
     b.free();
+}
+
+// normalize(Vec3f) of S/vec.h:218-219 on n xyz triples.
+void ref_normalize(int n, const float* in, float* out) {
+    for (int p = 0; p < n; ++p) {
+        ivock::Vec3f v(in[3 * p], in[3 * p + 1], in[3 * p + 2]);
+        ivock::normalize(v);
+        out[3 * p] = v[0]; out[3 * p + 1] = v[1]; out[3 * p + 2] = v[2];
+    }
+}
+
+// The cell-index arithmetic of hash_FLIP evaluated with the reference's own
+// Vec3f/Vec3i types (S/Smoke_solver3D.cpp:1337-1339; S/sub_domain.cpp:1097-1099
+// subtracts bmin first): ijk = Vec3i((pos [- bmin] + Vec3f(.5,.5,.5)*h) / h).
+void ref_cell_index(int n, const float* pos, float h, int sub, float bx, float by, float bz, int* ijk) {
+    const ivock::Vec3f bmin(bx, by, bz);
+    for (int p = 0; p < n; ++p) {
+        ivock::Vec3f xx(pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]);
+        if (sub) xx = xx - bmin;
+        xx += ivock::Vec3f(0.5, 0.5, 0.5) * h;
+        ivock::Vec3i c(xx / h);
+        ijk[3 * p] = c[0]; ijk[3 * p + 1] = c[1]; ijk[3 * p + 2] = c[2];
+    }
 }
 
 // Blocked storage index of the reference Buffer3D for cell (i,j,k).

@@ -53,6 +53,14 @@ template <class T> inline T clampv(T a, T lo, T hi) {
     return a;
 }
 
+// Truncating float->int conversion of Vec3i(Vec3f) (S/vec.h:37-39) with the
+// x86-64 result for values outside int range and NaN (the "integer indefinite"
+// INT_MIN of cvttss2si), written out so it does not rest on undefined behaviour.
+inline int trunc_i32(float t) {
+    if (!(t > -2147483648.0f && t < 2147483648.0f)) return INT32_MIN;
+    return (int)t;
+}
+
 // S/levelset_util.cpp:15-20 cycle_array
 inline void rot4(float* a) {
     float t = a[0];
@@ -709,7 +717,7 @@ int or_hash(int np, const float* pos, float h, int nx, int ny, int nz, int clamp
         float x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
         if (clamp) { x = x - bx; y = y - by; z = z - bz; }
         x = x + half; y = y + half; z = z + half;
-        int i = (int)(x / h), j = (int)(y / h), k = (int)(z / h);
+        int i = trunc_i32(x / h), j = trunc_i32(y / h), k = trunc_i32(z / h);
         int key = -1;
         if (clamp) {
             i = clampv(i, 0, nx - 1); j = clampv(j, 0, ny - 1); k = clampv(k, 0, nz - 1);
@@ -808,9 +816,12 @@ void or_g2p(int np, const float* pos, float* vel, float h, int nx, int ny, int n
             float pu = or_sample_linear(u, nx + 1, ny, nz, hd, 0.5, 0, 0, x, y, z);
             float pv = or_sample_linear(v, nx, ny + 1, nz, hd, 0, 0.5, 0, x, y, z);
             float pw = or_sample_linear(w, nx, ny, nz + 1, hd, 0, 0, 0.5, x, y, z);
-            vel[3 * p] = flip_ratio * (vel[3 * p] + a) + (1.0f - flip_ratio) * pu;
-            vel[3 * p + 1] = flip_ratio * (vel[3 * p + 1] + b) + (1.0f - flip_ratio) * pv;
-            vel[3 * p + 2] = flip_ratio * (vel[3 * p + 2] + c) + (1.0f - flip_ratio) * pw;
+            // pic_w*u_new + flip_w*(vel + du) with double weights, as the
+            // 0.03/0.97 blend of S/fluid_particle.h:316-329
+            const double fw = (double)flip_ratio, pw_ = 1.0 - fw;
+            vel[3 * p] = (float)(pw_ * pu + fw * (vel[3 * p] + a));
+            vel[3 * p + 1] = (float)(pw_ * pv + fw * (vel[3 * p + 1] + b));
+            vel[3 * p + 2] = (float)(pw_ * pw + fw * (vel[3 * p + 2] + c));
         }
     }
 }
@@ -934,5 +945,90 @@ void or_plume_force(int ni, int nj, int nk, float h, float dt, float cx, float c
                 float tf = 0.5f * (temp[c] + temp[cm]);
                 v[lin(i, j, k, ni, nj + 1)] += dt * (-alpha * rf + beta * tf);
             }
+}
+
+// ---------------------------------------------------------------------------
+// Buffer3D::sample_grad (S/fluid_buffer3D.h:249-258): central differences of
+// sample_linear at +-h (the buffer's double spacing), each difference taken in
+// float, divided by h in double, stored as float and scaled by 0.5f.
+// ---------------------------------------------------------------------------
+void or_sample_grad(const float* f, int nx, int ny, int nz, double h, double ox, double oy, double oz,
+                    double wx, double wy, double wz, float* g) {
+    const float right = or_sample_linear(f, nx, ny, nz, h, ox, oy, oz, wx + h, wy, wz);
+    const float left = or_sample_linear(f, nx, ny, nz, h, ox, oy, oz, wx - h, wy, wz);
+    const float up = or_sample_linear(f, nx, ny, nz, h, ox, oy, oz, wx, wy + h, wz);
+    const float bottom = or_sample_linear(f, nx, ny, nz, h, ox, oy, oz, wx, wy - h, wz);
+    const float front = or_sample_linear(f, nx, ny, nz, h, ox, oy, oz, wx, wy, wz - h);
+    const float back = or_sample_linear(f, nx, ny, nz, h, ox, oy, oz, wx, wy, wz + h);
+    g[0] = 0.5f * (float)((double)(right - left) / h);
+    g[1] = 0.5f * (float)((double)(up - bottom) / h);
+    g[2] = 0.5f * (float)((double)(back - front) / h);
+}
+
+// normalize(Vec3f): a /= mag(a), mag = sqrt(mag2) + 1e-6 (S/vec.h:194-203,
+// 218-219). mag2 accumulates the squares in float; the reference's unqualified
+// sqrt(float) binds to the double ::sqrt when its headers are compiled with
+// g++/clang (checked against oracle/_ref), so the root is taken in double.
+void or_normalize3(float* a) {
+    float l = a[0] * a[0];
+    l += a[1] * a[1];
+    l += a[2] * a[2];
+    const float m = (float)(std::sqrt((double)l) + 1e-6);
+    a[0] /= m;
+    a[1] /= m;
+    a[2] /= m;
+}
+
+// ---------------------------------------------------------------------------
+// FLIP particle advection: traceRK3 then the solid push-out of advect_FLIP.
+// Coarse grid (sub=0): S/Smoke_solver3D.cpp:141-154 (traceRK3, where
+// c3 = (1 - c1 - c2)*dt with c1, c2 already scaled by dt) and 1408-1430
+// (advect_FLIP: if phi(pos) < 0, pos -= phi*normalize(grad phi)).
+// Subdomain (sub=1): S/sub_domain.cpp:116-129 (c3 = 4/9 dt; velocity sampled at
+// pos - bmin) and 1180-1202 (phi and its gradient sampled at the unshifted pos).
+// Velocity: trilinear u (nx+1,ny,nz; .5,0,0), v, w (S/Smoke_solver3D.cpp:71-88,
+// without the subdomain blend). phi: nodal (nx+1,ny+1,nz+1; .5,.5,.5).
+// pos is xyz-interleaved float, updated in place.
+// ---------------------------------------------------------------------------
+void or_advect_particles(int np, float* pos, const float* u, const float* v, const float* w,
+                         const float* phi, int nx, int ny, int nz, float h, float dt, int sub,
+                         float bx, float by, float bz) {
+    const double hd = h;
+    const float c1 = (float)(2.0 / 9.0 * dt), c2 = (float)(3.0 / 9.0 * dt);
+    const float c3 = sub ? (float)(4.0 / 9.0 * dt) : (float)((1.0 - c1 - c2) * dt);
+    const float t1 = (float)(0.5 * dt), t2 = (float)(0.75 * dt);
+    auto vel = [&](const float* x, float* o) {
+        float px = x[0], py = x[1], pz = x[2];
+        if (sub) { px = px - bx; py = py - by; pz = pz - bz; }
+        o[0] = or_sample_linear(u, nx + 1, ny, nz, hd, 0.5, 0, 0, px, py, pz);
+        o[1] = or_sample_linear(v, nx, ny + 1, nz, hd, 0, 0.5, 0, px, py, pz);
+        o[2] = or_sample_linear(w, nx, ny, nz + 1, hd, 0, 0, 0.5, px, py, pz);
+    };
+#pragma omp parallel for
+    for (int p = 0; p < np; ++p) {
+        float x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        float v1[3], v2[3], v3[3], m1[3], m2[3];
+        vel(x, v1);
+        for (int a = 0; a < 3; ++a) m1[a] = x[a] + t1 * v1[a];
+        vel(m1, v2);
+        for (int a = 0; a < 3; ++a) m2[a] = x[a] + t2 * v2[a];
+        vel(m2, v3);
+        for (int a = 0; a < 3; ++a) x[a] = x[a] + c1 * v1[a] + c2 * v2[a] + c3 * v3[a];
+        const float ph = or_sample_linear(phi, nx + 1, ny + 1, nz + 1, hd, 0.5, 0.5, 0.5, x[0], x[1], x[2]);
+        if (ph < 0) {
+            float g[3];
+            or_sample_grad(phi, nx + 1, ny + 1, nz + 1, hd, 0.5, 0.5, 0.5, x[0], x[1], x[2], g);
+            or_normalize3(g);
+            for (int a = 0; a < 3; ++a) x[a] -= ph * g[a];
+        }
+        pos[3 * p] = x[0];
+        pos[3 * p + 1] = x[1];
+        pos[3 * p + 2] = x[2];
+    }
+}
+
+// getDelta (S/Smoke_solver3D.cpp:1297-1318): out = new - old, elementwise float.
+void or_get_delta(int n, const float* fnew, const float* fold, float* out) {
+    for (int c = 0; c < n; ++c) out[c] = fnew[c] - fold[c];
 }
 }  // extern "C"

@@ -81,6 +81,31 @@ def main():
             if i < a and j < b and k < c:
                 bi.append((a, b, c, i, j, k, R.ref_blocked_index(a, b, c, i, j, k)))
     out["blocked_index"] = np.array(bi, np.int64)
+    # normalize(Vec3f): random, tiny, huge and zero vectors
+    nv = np.concatenate([rng.standard_normal((2000, 3)), rng.standard_normal((100, 3)) * 1e-7,
+                         rng.standard_normal((100, 3)) * 1e6, np.zeros((1, 3))]).astype(np.float32)
+    no = np.zeros_like(nv)
+    R.ref_normalize(len(nv), np.ascontiguousarray(nv), no)
+    out["normalize_in"] = nv
+    out["normalize_out"] = no
+    # hash_FLIP cell indices: positions around cell faces, negatives in (-h, 0),
+    # far outside, for h = 1/64 and 1/48, coarse and subdomain (bmin) forms
+    ci_pos, ci_meta, ci_out = [], [], []
+    for h, sub, bmin in ((1.0 / 64, 0, (0.0, 0.0, 0.0)), (1.0 / 48, 1, (0.3, 0.25, 0.4))):
+        hf = np.float32(h)
+        base = rng.integers(-3, 70, (3000, 3)).astype(np.float32)
+        frac = rng.choice(np.array([-0.5, -0.5000001, -0.4999999, 0.0, 0.4999999, 0.5, rng.uniform(-1, 1)],
+                                   np.float32), (3000, 3))
+        p = ((base + frac) * hf).astype(np.float32) + np.float32(bmin[0] if sub else 0)
+        p = np.concatenate([p, rng.uniform(-0.02, 1.05, (3000, 3)).astype(np.float32)])
+        o = np.zeros(p.shape, np.int32)
+        R.ref_cell_index(len(p), np.ascontiguousarray(p), float(hf), sub, *bmin, o)
+        ci_pos.append(p)
+        ci_out.append(o)
+        ci_meta.append([float(hf), sub, *bmin])
+    out["cellidx_pos"] = np.stack(ci_pos)
+    out["cellidx_out"] = np.stack(ci_out)
+    out["cellidx_meta"] = np.array(ci_meta, np.float64)
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, {k: v.shape for k, v in out.items()})

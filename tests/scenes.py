@@ -102,3 +102,26 @@ def projection_problem(n, seed=0, spheres=None, open_top=True, vel_scale=1.0):
     u, v, w = random_mac(n, seed, vel_scale)
     us, vs, ws = mac_zeros(n)
     return dict(n=n, h=np.float32(h), phi=phi, u=u, v=v, w=w, us=us, vs=vs, ws=ws, spheres=spheres)
+
+
+def block_particles(n, ppc, seed, lo=(0.0, 0.0, 0.0), hi=(0.4, 0.6, 1.0), jitter=1.0):
+    """ppc jittered particles in every cell whose centre lies in the config-space
+    block [lo, hi] of an n^3 grid (the dam-break fill: [0,.4]x[0,.6]x[0,1]).
+
+    Positions are in reference coordinates (cell (i,j,k) centre at (i,j,k)*h),
+    cell-major (x fastest), jitter U[-0.5, 0.5)*jitter*h per axis from the counter
+    hash keyed by (seed, axis, particle id). Returns float32 (np, 3).
+    """
+    h = 1.0 / n
+    c = (np.arange(n) + 0.5) * h
+    sel = [np.nonzero((c >= lo[a]) & (c <= hi[a]))[0] for a in range(3)]
+    k, j, i = np.meshgrid(sel[2], sel[1], sel[0], indexing="ij")
+    cells = np.repeat(np.stack([i.ravel(), j.ravel(), k.ravel()], 1), ppc, axis=0)
+    idx = np.arange(len(cells))
+    jit = np.stack([uniform01(seed, 200 + a, idx) - 0.5 for a in range(3)], 1) * jitter
+    return ((cells + jit) * h).astype(np.float32)
+
+
+def particle_velocities(npart, seed, scale=1.0):
+    idx = np.arange(npart)
+    return (np.stack([uniform01(seed, 300 + a, idx) * 2 - 1 for a in range(3)], 1) * scale).astype(np.float32)

@@ -1,0 +1,224 @@
+"""GPU FLIP transfer against the oracle.
+
+Bit-exact: binning (keys, cell offsets, per-cell order), the exact-order P2G
+gather, getDelta, G2P and particle advection. The LDS-binned scatter P2G sums
+the same float weights in another order, so it is held to FP32 rounding
+(relative L2 <= 1e-6 against the oracle, tolerance stated per test).
+"""
+import numpy as np
+import pytest
+
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _np(t):
+    import torch
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def _cloud(n, ppc, seed, spread=1.0, shuffle=False, bmin=(0.0, 0.0, 0.0)):
+    """ppc particles per cell over the whole grid; spread > 1 pushes some out of the grid."""
+    pos = scenes.block_particles(n, ppc, seed, hi=(1.0, 1.0, 1.0), jitter=spread)
+    pos = pos + np.asarray(bmin, np.float32)
+    vel = scenes.particle_velocities(len(pos), seed)
+    if shuffle:
+        perm = np.random.default_rng(seed).permutation(len(pos))
+        pos, vel = pos[perm], vel[perm]
+    return np.ascontiguousarray(pos), np.ascontiguousarray(vel)
+
+
+def _hash_both(n, pos, clamp=False, bmin=(0.0, 0.0, 0.0), h=None):
+    from oracle import oracle as O
+    from vflip_amd import api
+    h = np.float32(1.0 / n) if h is None else h
+    P = api.Particles(n, n, n, float(h))
+    key, start, order, nb = P.hash(_t(pos), clamp=clamp, bmin=bmin)
+    rk, rs, ro = O.bin_particles(pos, h, (n, n, n), clamp=clamp, bmin=bmin)
+    return P, (_np(key), _np(start), _np(order)[:nb], nb), (rk, rs, ro)
+
+
+@pytest.mark.parametrize("n,clamp,shuffle", [(20, False, False), (20, False, True), (33, True, True),
+                                             (33, True, False)])
+def test_hash_bitexact(cuda, n, clamp, shuffle):
+    print(f"hash n={n} clamp={clamp} shuffle={shuffle}", flush=True)
+    bmin = (0.1, 0.05, 0.2) if clamp else (0.0, 0.0, 0.0)
+    pos, _ = _cloud(n, 8, n, spread=1.8, shuffle=shuffle, bmin=bmin)
+    _, (k, s, o, nb), (rk, rs, ro) = _hash_both(n, pos, clamp, bmin)
+    assert np.array_equal(k, rk)
+    assert np.array_equal(s, rs)
+    assert nb == len(ro) and np.array_equal(o, ro)
+    if not clamp:
+        assert nb < len(pos)  # some particles were dropped
+
+
+def test_hash_long_segments(cuda):
+    """Clamping piles thousands of far-away particles into a few boundary cells:
+    segments longer than the per-thread fix-up take the workgroup merge sort."""
+    print("hash long segments", flush=True)
+    n = 16
+    rng = np.random.default_rng(3)
+    pos, _ = _cloud(n, 4, 1, shuffle=True)
+    far = np.concatenate([rng.uniform(-5, -1, (3000, 3)), rng.uniform(2, 3, (1500, 3)),
+                          np.stack([rng.uniform(0, 1, 5000), np.full(5000, -2.0), np.full(5000, 9.0)], 1)])
+    pos = np.ascontiguousarray(np.concatenate([pos, far.astype(np.float32)])[rng.permutation(len(pos) + len(far))])
+    _, (k, s, o, nb), (rk, rs, ro) = _hash_both(n, pos, clamp=True)
+    assert np.diff(rs).max() > 1000
+    assert np.array_equal(k, rk) and np.array_equal(s, rs) and np.array_equal(o, ro)
+
+
+def test_hash_empty_and_all_dropped(cuda):
+    print("hash empty", flush=True)
+    from vflip_amd import api
+    n = 8
+    P = api.Particles(n, n, n, 1.0 / n)
+    pos = _t(np.zeros((0, 3), np.float32))
+    key, start, order, nb = P.hash(pos)
+    assert nb == 0 and not _np(start).any()
+    out = np.full((100, 3), 5.0, np.float32)
+    out[:50] = -3.0
+    out[0] = np.nan
+    key, start, order, nb = P.hash(_t(out))
+    assert nb == 0 and np.all(_np(key) == -1) and not _np(start).any()
+
+
+def test_hash_dam_break_128(cuda):
+    """The 128^3 dam-break fill (4.02M particles), shuffled: full-size keys,
+    offsets and order against the oracle, and order is a stable sort by key."""
+    print("hash dam break 128", flush=True)
+    n = 128
+    pos = scenes.block_particles(n, 8, 0)
+    perm = np.random.default_rng(0).permutation(len(pos))
+    pos = np.ascontiguousarray(pos[perm])
+    _, (k, s, o, nb), (rk, rs, ro) = _hash_both(n, pos)
+    assert nb == len(pos) == 4021248
+    assert np.array_equal(k, rk) and np.array_equal(s, rs) and np.array_equal(o, ro)
+    assert np.array_equal(o, np.argsort(k, kind="stable"))
+
+
+def _p2g_case(variant):
+    if variant == 0:
+        n, bmin = 20, (0.0, 0.0, 0.0)
+        pos, vel = _cloud(n, 8, 11, spread=1.4, shuffle=True)
+    else:
+        n, bmin = 24, (0.3, 0.25, 0.1)
+        pos, vel = _cloud(n, 8, 12, spread=1.2, shuffle=True, bmin=bmin)
+    return n, bmin, pos, vel
+
+
+def _oracle_p2g(n, pos, vel, start, order, sub, bmin):
+    from oracle import oracle as O
+    h = np.float32(1.0 / n)
+    out = []
+    for comp, fshape, offs in ((0, (n, n, n + 1), (0.5, 0, 0)), (1, (n, n + 1, n), (0, 0.5, 0)),
+                               (2, (n + 1, n, n), (0, 0, 0.5))):
+        out.append(O.p2g(pos, vel, comp, h, (n, n, n), start, order, fshape, offs, sub=sub, bmin=bmin))
+    return out
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_p2g_exact_order_bitexact(cuda, variant):
+    print(f"p2g exact variant={variant}", flush=True)
+    from vflip_amd import api
+    n, bmin, pos, vel = _p2g_case(variant)
+    P, (k, s, o, nb), (rk, rs, ro) = _hash_both(n, pos, clamp=variant == 1, bmin=bmin)
+    gs, go = _t(s), _t(np.concatenate([o, np.zeros(1, np.int32)]))
+    got = P.p2g(_t(pos), _t(vel), gs, go, variant=variant, method=1, bmin=bmin)
+    ref = _oracle_p2g(n, pos, vel, rs, ro, variant == 1, bmin)
+    for (gf, gc), (rf, rc) in zip(got, ref):
+        assert np.array_equal(bits(_np(gc)), bits(rc))
+        assert np.array_equal(bits(_np(gf)), bits(rf))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_p2g_scatter_matches_oracle(cuda, variant):
+    """Scatter sums the reference's float weights in another order: FP32 rounding only.
+    Tolerance: relative L2 <= 1e-6 per array, max |diff| <= 1e-5 * max |ref|."""
+    print(f"p2g scatter variant={variant}", flush=True)
+    n, bmin, pos, vel = _p2g_case(variant)
+    P, (k, s, o, nb), (rk, rs, ro) = _hash_both(n, pos, clamp=variant == 1, bmin=bmin)
+    gs, go = _t(s), _t(np.concatenate([o, np.zeros(1, np.int32)]))
+    got = P.p2g(_t(pos), _t(vel), gs, go, variant=variant, method=0, bmin=bmin)
+    ref = _oracle_p2g(n, pos, vel, rs, ro, variant == 1, bmin)
+    for (gf, gc), (rf, rc) in zip(got, ref):
+        for g, r in ((_np(gc), rc), (_np(gf), rf)):
+            assert np.linalg.norm(g - r) <= 1e-6 * np.linalg.norm(r)
+            assert np.abs(g - r).max() <= 1e-5 * np.abs(r).max()
+
+
+def test_p2g_scatter_dam_break_128(cuda):
+    """Full-size scatter P2G (4.02M particles, 128^3) against the exact-order gather."""
+    print("p2g scatter 128", flush=True)
+    from vflip_amd import api
+    n = 128
+    pos = scenes.block_particles(n, 8, 0)
+    vel = scenes.particle_velocities(len(pos), 0)
+    P = api.Particles(n, n, n, 1.0 / n)
+    gp, gv = _t(pos), _t(vel)
+    key, start, order, nb = P.hash(gp)
+    fast = P.p2g(gp, gv, start, order, method=0)
+    exact = P.p2g(gp, gv, start, order, method=1)
+    for (ff, fc), (ef, ec) in zip(fast, exact):
+        for g, r in ((_np(fc), _np(ec)), (_np(ff), _np(ef))):
+            assert np.linalg.norm(g - r) <= 1e-6 * np.linalg.norm(r)
+
+
+def test_get_delta_and_g2p_bitexact(cuda):
+    print("g2p", flush=True)
+    from oracle import oracle as O
+    from vflip_amd import api
+    n = 24
+    h = np.float32(1.0 / n)
+    new = scenes.random_mac(n, 5)
+    old = scenes.random_mac(n, 6)
+    pos, vel = _cloud(n, 4, 7, spread=1.5, shuffle=True)
+    d = [O.get_delta(a, b) for a, b in zip(new, old)]
+    gd = [api.get_delta(_t(a), _t(b)) for a, b in zip(new, old)]
+    for g, r in zip(gd, d):
+        assert np.array_equal(bits(_np(g)), bits(r))
+    for flip, with_old in ((1.0, True), (1.0, False), (0.95, True)):
+        gv = _t(vel)
+        if with_old:
+            api.g2p(float(h), _t(pos), gv, [_t(a) for a in new], [_t(a) for a in old], flip=flip)
+        else:
+            api.g2p(float(h), _t(pos), gv, gd, None, flip=flip)
+        ref = O.g2p(pos, vel, h, (n, n, n), *d, u=new[0], v=new[1], w=new[2], flip=flip)
+        assert np.array_equal(bits(_np(gv)), bits(ref)), (flip, with_old)
+    # subdomain form: positions relative to bmin
+    gv = _t(vel)
+    api.g2p(float(h), _t(pos), gv, [_t(a) for a in new], [_t(a) for a in old], bmin=(0.1, 0.2, 0.05))
+    ref = O.g2p(pos, vel, h, (n, n, n), *d, bmin=(0.1, 0.2, 0.05))
+    assert np.array_equal(bits(_np(gv)), bits(ref))
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_advect_particles_bitexact(cuda, variant):
+    print(f"advect particles variant={variant}", flush=True)
+    from oracle import oracle as O
+    from vflip_amd import api
+    n = 32
+    h = np.float32(1.0 / n)
+    u, v, w = scenes.random_mac(n, 9, scale=2.0)
+    phi = scenes.solid_phi(n, 1.0 / n, [((0.5, 0.5, 0.5), 0.2)])
+    bmin = (0.05, 0.1, 0.02) if variant == 1 else (0.0, 0.0, 0.0)
+    pos, _ = _cloud(n, 2, 13, spread=1.0, shuffle=True)
+    dt = np.float32(0.02)
+    gp = _t(pos)
+    api.advect_particles(float(h), float(dt), gp, _t(u), _t(v), _t(w), _t(phi), variant=variant, bmin=bmin)
+    ref = O.advect_particles(pos, h, dt, u, v, w, phi, sub=variant == 1, bmin=bmin)
+    got = _np(gp)
+    assert np.array_equal(bits(got), bits(ref))
+    # the solid push-out changed a good number of particles
+    free = O.advect_particles(pos, h, dt, u, v, w, np.ones_like(phi), sub=variant == 1, bmin=bmin)
+    assert (ref != free).any(axis=1).sum() > 20

@@ -86,6 +86,14 @@ def test_samplers_golden():
     for p, ref_v, ref_c in zip(GOLD["samp_pts"], GOLD["samp_cube_lerp"], GOLD["samp_corners"]):
         v = O.lib().or_sample_cube_lerp(f, nx, ny, nz, h, ox, oy, oz, *p, c8)
         assert bits(v) == bits(ref_v) and np.array_equal(bits(c8), bits(ref_c))
+    grad = O.sample_grad(f, h, (ox, oy, oz), GOLD["samp_pts"])
+    assert np.array_equal(bits(grad), bits(GOLD["samp_grad"]))
+
+
+def test_normalize_golden():
+    # the reference's mag() takes the root in double under g++/clang (a float
+    # root differs in the last bit for about a third of these vectors)
+    assert np.array_equal(bits(O.normalize(GOLD["normalize_in"])), bits(GOLD["normalize_out"]))
 
 
 def test_trilinear_exact_on_affine_fields():
@@ -417,6 +425,21 @@ def test_binning_matches_brute_force_and_is_stable():
     assert np.all(key2 >= 0) and len(order2) == len(pos)
 
 
+def test_binning_cell_index_golden():
+    """Cell keys against hash_FLIP's arithmetic run with the reference's own Vec3f/Vec3i."""
+    n = 64
+    for pos, ijk, meta in zip(GOLD["cellidx_pos"], GOLD["cellidx_out"], GOLD["cellidx_meta"]):
+        h, sub, bmin = np.float32(meta[0]), bool(meta[1]), tuple(meta[2:5])
+        key, _, _ = O.bin_particles(pos, h, (n, n, n), clamp=sub, bmin=bmin)
+        if sub:
+            c = np.clip(ijk, 0, n - 1)
+            want = (c[:, 2] * n + c[:, 1]) * n + c[:, 0]
+        else:
+            inside = np.all((ijk >= 0) & (ijk < n), axis=1)
+            want = np.where(inside, (ijk[:, 2] * n + ijk[:, 1]) * n + ijk[:, 0], -1)
+        assert np.array_equal(key, want)
+
+
 def _brute_p2g(pos, vel, comp, h, fshape, offs):
     fz, fy, fx = fshape
     field = np.zeros(fshape)
@@ -479,6 +502,98 @@ def test_g2p_reproduces_affine_delta():
     for a, cc in enumerate((cu, cv, cw)):
         exact = vel[:, a] + cc[0] + cc[1] * p[:, 0] + cc[2] * p[:, 1] + cc[3] * p[:, 2]
         assert np.abs(out[:, a] - exact).max() < 3e-6
+    # PIC/FLIP blend: vel = (1-r)*u_new(x) + r*(vel + du(x)), u_new affine too
+    cn = (0.3, -0.5, 0.25, 0.75)
+    un = faces((n, n, n + 1), (0.5, 0, 0), cn)
+    r = 0.95
+    out = O.g2p(pos, vel, h, (n, n, n), du, dv, dw, u=un, v=dv, w=dw, flip=r)
+    new_u = cn[0] + cn[1] * p[:, 0] + cn[2] * p[:, 1] + cn[3] * p[:, 2]
+    du_u = cu[0] + cu[1] * p[:, 0] + cu[2] * p[:, 1] + cu[3] * p[:, 2]
+    assert np.abs(out[:, 0] - ((1 - r) * new_u + r * (vel[:, 0] + du_u))).max() < 3e-6
+
+
+# --------------------------------------------------------------------- particle advection
+def _affine_mac(n, h, cu, cv, cw):
+    """MAC velocity grids of u = cu0 + cu1 x + cu2 y + cu3 z (and v, w) at the reference's face positions."""
+
+    def faces(shape, off, c):
+        k, j, i = np.meshgrid(*[np.arange(s) for s in shape], indexing="ij")
+        x, y, z = (i - off[0]) * h, (j - off[1]) * h, (k - off[2]) * h
+        return (c[0] + c[1] * x + c[2] * y + c[3] * z).astype(np.float32)
+
+    return (faces((n, n, n + 1), (0.5, 0, 0), cu), faces((n, n + 1, n), (0, 0.5, 0), cv),
+            faces((n + 1, n, n), (0, 0, 0.5), cw))
+
+
+def _open_phi(n):
+    return np.full((n + 1, n + 1, n + 1), 1.0, np.float32)
+
+
+@pytest.mark.parametrize("sub", [False, True])
+def test_rk3_constant_velocity_displacement(sub):
+    """Constant velocity V moves a particle by (c1+c2+c3)*V: the coarse grid's
+    c3 = (1 - c1 - c2)*dt (c1, c2 already scaled by dt) vs 4/9 dt in the subdomain."""
+    n, dt = 16, 0.01
+    h = np.float32(1.0 / n)
+    V = (0.7, -0.4, 1.1)
+    u, v, w = _affine_mac(n, h, (V[0], 0, 0, 0), (V[1], 0, 0, 0), (V[2], 0, 0, 0))
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0.3, 0.7, (300, 3)).astype(np.float32)
+    out = O.advect_particles(pos, h, dt, u, v, w, _open_phi(n), sub=sub, bmin=(0.05, 0.1, 0.0))
+    c1, c2 = 2.0 / 9.0 * dt, 3.0 / 9.0 * dt
+    scale = c1 + c2 + (4.0 / 9.0 * dt if sub else (1.0 - c1 - c2) * dt)
+    assert np.abs((out - pos) - scale * np.array(V)).max() < 2e-7
+    if not sub:  # the quirk moves particles about 1.55x further than dt*V
+        assert np.abs((out - pos) - dt * np.array(V)).max() > 1e-3
+
+
+@pytest.mark.parametrize("sub", [False, True])
+def test_rk3_linear_velocity_closed_form(sub):
+    """u = a (x - x0): RK3 of a linear ODE is a polynomial in z = a dt; Ralston's
+    weights (sub) give exactly 1 + z + z^2/2 + z^3/6."""
+    n, dt, a, x0 = 16, 0.05, 1.7, 0.45
+    h = np.float32(1.0 / n)
+    u, v, w = _affine_mac(n, h, (-a * x0, a, 0, 0), (0, 0, 0, 0), (0, 0, 0, 0))
+    rng = np.random.default_rng(6)
+    pos = rng.uniform(0.3, 0.6, (300, 3)).astype(np.float32)
+    out = O.advect_particles(pos, h, dt, u, v, w, _open_phi(n), sub=sub)
+    z = a * dt
+    c1, c2 = np.float32(2.0 / 9.0 * dt), np.float32(3.0 / 9.0 * dt)
+    c3 = np.float32(4.0 / 9.0 * dt) if sub else np.float32((1.0 - float(c1) - float(c2)) * dt)
+    gain = 1 + a * (c1 + c2 * (1 + z / 2) + c3 * (1 + 3 * z / 4 + 3 * z * z / 8))
+    if sub:
+        assert abs(gain - (1 + z + z * z / 2 + z ** 3 / 6)) < 1e-6
+    d = pos[:, 0].astype(np.float64) - x0
+    assert np.abs(out[:, 0] - (x0 + d * gain)).max() < 1e-6
+    assert np.array_equal(out[:, 1:], pos[:, 1:])
+
+
+def test_solid_pushout_projects_onto_plane():
+    """Planar solid phi = n.(x - x_p) (affine, so sampled exactly): particles with
+    phi < 0 land on the plane; the subdomain form samples phi at the unshifted
+    position, so with zero velocity it pushes exactly as the coarse grid does."""
+    n = 16
+    h = np.float32(1.0 / n)
+    nrm = np.array([1.0, 2.0, 2.0]) / 3.0
+    xp = np.array([0.5, 0.45, 0.55])
+    k, j, i = np.meshgrid(*[np.arange(n + 1)] * 3, indexing="ij")
+    xs = np.stack([(i - 0.5) * h, (j - 0.5) * h, (k - 0.5) * h], -1)
+    phi = ((xs - xp) @ nrm).astype(np.float32)
+    u, v, w = _affine_mac(n, h, (0, 0, 0, 0), (0, 0, 0, 0), (0, 0, 0, 0))
+    rng = np.random.default_rng(7)
+    pos = rng.uniform(0.3, 0.7, (400, 3)).astype(np.float32)
+    out = O.advect_particles(pos, h, 0.01, u, v, w, phi)
+    before = (pos.astype(np.float64) - xp) @ nrm
+    after = (out.astype(np.float64) - xp) @ nrm
+    inside = before < 0
+    assert inside.sum() > 50 and (~inside).sum() > 50
+    assert np.abs(after[inside]).max() < 2e-6
+    assert np.array_equal(out[~inside], pos[~inside])
+    # the displacement is along the normal
+    dirs = (out[inside] - pos[inside]).astype(np.float64)
+    assert np.abs(np.cross(dirs, nrm)).max() < 2e-6
+    out_sub = O.advect_particles(pos, h, 0.01, u, v, w, phi, sub=True, bmin=(0.2, 0.1, 0.3))
+    assert np.array_equal(out_sub, out)
 
 
 # --------------------------------------------------------------------- coarse/fine coupling

@@ -77,6 +77,15 @@ SIGNATURES = {
                                  c_float, c_float, c_float, P, P, P, P]),
     "vflip_plume_force": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float,
                                   ctypes.c_uint, c_float, c_float, c_float, P, P, P, P]),
+    "vflip_particles_create": (c_int, [c_int, c_int, c_int, c_float, ctypes.POINTER(c_void_p)]),
+    "vflip_particles_destroy": (c_int, [P]),
+    "vflip_particles_set_stream": (c_int, [P, P]),
+    "vflip_hash": (c_int, [P, c_int, P, c_int, c_float, c_float, c_float, P, P, P, PI]),
+    "vflip_p2g": (c_int, [P, c_int, c_int, c_int, P, P, P, P, c_float, c_float, c_float] + [P] * 6),
+    "vflip_get_delta": (c_int, [ctypes.c_size_t, P, P, P, P]),
+    "vflip_g2p": (c_int, [c_int, c_int, c_int, c_float, c_int, P, P] + [P] * 6 + [c_float] * 4 + [P]),
+    "vflip_advect_particles": (c_int, [c_int, c_int, c_int, c_float, c_float, c_int, c_float, c_float, c_float,
+                                       c_int, P, P, P, P, P, P]),
 }
 
 _lib = None

@@ -190,3 +190,84 @@ def plume_force(h, dt, center, r, seed, jitter, alpha, beta, rho, temp, v):
     nk, nj, ni = rho.shape
     check(lib().vflip_plume_force(ni, nj, nk, h, dt, center[0], center[1], center[2], r, seed, jitter, alpha, beta,
                                   _ptr(rho), _ptr(temp), _ptr(v), _stream()), "plume_force")
+
+
+# ---------------------------------------------------------------------- FLIP
+class Particles:
+    """Binning/transfer workspace (vflip_particles) for an ni x nj x nk grid of spacing h.
+
+    Particle arrays are device float32 tensors shaped (np, 3); grids are (z, y, x)
+    tensors in the MAC shapes u:(nk, nj, ni+1), v:(nk, nj+1, ni), w:(nk+1, nj, ni).
+    """
+
+    def __init__(self, ni, nj, nk, h):
+        self.ni, self.nj, self.nk, self.h = ni, nj, nk, h
+        self.ncell = ni * nj * nk
+        hnd = ctypes.c_void_p()
+        check(lib().vflip_particles_create(ni, nj, nk, h, ctypes.byref(hnd)), "particles_create")
+        self.hnd = hnd
+        check(lib().vflip_particles_set_stream(self.hnd, _stream()), "particles_set_stream")
+
+    def close(self):
+        if getattr(self, "hnd", None):
+            lib().vflip_particles_destroy(self.hnd)
+            self.hnd = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def hash(self, pos, clamp=False, bmin=(0.0, 0.0, 0.0), count=True):
+        """hash_FLIP: returns (cell_key[np], cell_start[ncell+1], order[np], nbinned or None)."""
+        npart = pos.shape[0]
+        key = torch.empty(max(npart, 1), dtype=torch.int32, device="cuda")
+        start = torch.empty(self.ncell + 1, dtype=torch.int32, device="cuda")
+        order = torch.empty(max(npart, 1), dtype=torch.int32, device="cuda")
+        nb = ctypes.c_int(0)
+        check(lib().vflip_hash(self.hnd, npart, _ptr(pos, numel=3 * npart, name="pos") if npart else None,
+                               int(clamp), *[float(b) for b in bmin], _ptr(key, torch.int32),
+                               _ptr(start, torch.int32), _ptr(order, torch.int32),
+                               ctypes.byref(nb) if count else None), "hash")
+        return key[:npart], start, order, (nb.value if count else None)
+
+    def p2g(self, pos, vel, start, order, variant=COARSE, method=0, bmin=(0.0, 0.0, 0.0), out=None):
+        """particle_to_grid for u, v, w: returns [(u, u_coef), (v, v_coef), (w, w_coef)]."""
+        ni, nj, nk = self.ni, self.nj, self.nk
+        if out is None:
+            shapes = [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+            out = [(torch.empty(s, device="cuda"), torch.empty(s, device="cuda")) for s in shapes]
+        npart = pos.shape[0]
+        ptrs = []
+        for f, c in out:
+            ptrs += [_ptr(f), _ptr(c)]
+        check(lib().vflip_p2g(self.hnd, variant, method, npart, _ptr(pos) if npart else None,
+                              _ptr(vel) if npart else None, _ptr(start, torch.int32),
+                              _ptr(order, torch.int32), *[float(b) for b in bmin], *ptrs), "p2g")
+        return out
+
+
+def get_delta(new, old, out=None):
+    out = torch.empty_like(new) if out is None else out
+    check(lib().vflip_get_delta(new.numel(), _ptr(new), _ptr(old, numel=new.numel()), _ptr(out, numel=new.numel()),
+                                _stream()), "get_delta")
+    return out
+
+
+def g2p(h, pos, vel, new, old=None, flip=1.0, bmin=(0.0, 0.0, 0.0)):
+    """particle_from_grid in place on vel: new = (u, v, w) grids, old = (u, v, w) or None (new are deltas)."""
+    nk, nj, ni1 = new[0].shape
+    npart = pos.shape[0]
+    olds = [None, None, None] if old is None else [_ptr(o) for o in old]
+    check(lib().vflip_g2p(ni1 - 1, nj, nk, h, npart, _ptr(pos) if npart else None, _ptr(vel) if npart else None,
+                          *[_ptr(g) for g in new], *olds, flip, *[float(b) for b in bmin], _stream()), "g2p")
+
+
+def advect_particles(h, dt, pos, u, v, w, solid_phi, variant=COARSE, bmin=(0.0, 0.0, 0.0)):
+    """advect_FLIP in place on pos (traceRK3 + solid push-out)."""
+    nk, nj, ni1 = u.shape
+    npart = pos.shape[0]
+    check(lib().vflip_advect_particles(ni1 - 1, nj, nk, h, dt, variant, *[float(b) for b in bmin], npart,
+                                       _ptr(pos) if npart else None, _ptr(u), _ptr(v), _ptr(w), _ptr(solid_phi),
+                                       _stream()), "advect_particles")

@@ -10,64 +10,9 @@
 #pragma clang fp contract(off)
 
 #include "common.h"
+#include "sampling.h"
 
 namespace vflip {
-
-struct Grid {
-    const float* f;
-    int nx, ny, nz;
-    double ox, oy, oz;  // grid offsets in cells (u: .5,0,0)
-};
-
-__device__ inline float at(const Grid& g, int i, int j, int k) {
-    i = i < 0 ? 0 : (i > g.nx - 1 ? g.nx - 1 : i);
-    j = j < 0 ? 0 : (j > g.ny - 1 ? g.ny - 1 : j);
-    k = k < 0 ? 0 : (k > g.nz - 1 ? g.nz - 1 : k);
-    return g.f[((size_t)k * g.ny + j) * g.nx + i];
-}
-
-__device__ inline double lerpd(double a, double b, double c) { return (1.0 - c) * a + c * b; }
-
-// Buffer3D::sample_linear
-__device__ inline float sample_linear(const Grid& g, double h, double wx, double wy, double wz) {
-    const double gx = wx / h + g.ox, gy = wy / h + g.oy, gz = wz / h + g.oz;
-    const int i = (int)floor(gx), j = (int)floor(gy), k = (int)floor(gz);
-    const double cx = gx - (double)i, cy = gy - (double)j, cz = gz - (double)k;
-    const double v1 = lerpd(lerpd(at(g, i, j, k), at(g, i + 1, j, k), cx),
-                            lerpd(at(g, i, j + 1, k), at(g, i + 1, j + 1, k), cx), cy);
-    const double v2 = lerpd(lerpd(at(g, i, j, k + 1), at(g, i + 1, j, k + 1), cx),
-                            lerpd(at(g, i, j + 1, k + 1), at(g, i + 1, j + 1, k + 1), cx), cy);
-    return (float)lerpd(v1, v2, cz);
-}
-
-// Buffer3D::cubic_interp with float weights (element type) and double values.
-__device__ inline double cubic_interp(double m1, double v0, double v1, double v2, double fd) {
-    const float f = (float)fd;
-    const float f2 = f * f, f3 = f2 * f;
-    const float wm1 = -float(1. / 3) * f + float(1. / 2) * f2 - float(1. / 6) * f3;
-    const float w0 = 1 - f2 + float(1. / 2) * (f3 - f);
-    const float w1 = f + float(1. / 2) * (f2 - f3);
-    const float w2 = float(1. / 6) * (f3 - f);
-    return wm1 * m1 + w0 * v0 + w1 * v1 + w2 * v2;
-}
-
-// Buffer3D::sample_cubic
-__device__ inline float sample_cubic(const Grid& g, double h, double wx, double wy, double wz) {
-    const double gx = wx / h + g.ox, gy = wy / h + g.oy, gz = wz / h + g.oz;
-    const int i = (int)floor(gx), j = (int)floor(gy), k = (int)floor(gz);
-    const double x = gx - (double)i, y = gy - (double)j, z = gz - (double)k;
-    double fz[4];
-#pragma unroll
-    for (int dk = -1; dk <= 2; ++dk) {
-        double fy[4];
-#pragma unroll
-        for (int dj = -1; dj <= 2; ++dj)
-            fy[dj + 1] = cubic_interp(at(g, i - 1, j + dj, k + dk), at(g, i, j + dj, k + dk), at(g, i + 1, j + dj, k + dk),
-                                      at(g, i + 2, j + dj, k + dk), x);
-        fz[dk + 1] = cubic_interp(fy[0], fy[1], fy[2], fy[3], y);
-    }
-    return (float)cubic_interp(fz[0], fz[1], fz[2], fz[3], z);
-}
 
 // One field (or two fields on the same grid) advected by the MAC velocity.
 __global__ __launch_bounds__(kBlock) void k_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode,

@@ -1,0 +1,781 @@
+// FLIP particle-grid transfer: cell binning, particle-to-grid, grid-to-particle
+// and particle advection.
+//
+// Restates, for device-resident particles (xyz-interleaved float pos/vel):
+//   hash_FLIP            S/Smoke_solver3D.cpp:1327-1346, S/sub_domain.cpp:1087-1107
+//   particle_to_grid     S/Smoke_solver3D.cpp:1451-1510, S/sub_domain.cpp:1204-1265
+//   getDelta             S/Smoke_solver3D.cpp:1297-1318
+//   particle_from_grid   S/Smoke_solver3D.cpp:1398-1406 (+ PIC blend of S/fluid_particle.h:297-336)
+//   advect_FLIP          S/Smoke_solver3D.cpp:141-154, 1408-1430; S/sub_domain.cpp:116-129, 1180-1202
+//
+// Binning is bit-exact: keys from the reference's float arithmetic, and within
+// a cell particles in ascending id order (the append order of the reference's
+// per-cell lists). Compiled without FMA contraction so weights, samples and
+// traces round exactly as the reference's expressions do.
+#pragma clang fp contract(off)
+
+#include <climits>
+
+#include "common.h"
+#include "sampling.h"
+
+struct vflip_particles {
+    int nx = 0, ny = 0, nz = 0;
+    float h = 0;
+    size_t ncell = 0;
+    hipStream_t stream = nullptr;
+    int* slot = nullptr;     // per particle: rank inside its cell's atomic slot range
+    int* tmp = nullptr;      // per particle: merge buffer of the long-segment sort
+    int* big = nullptr;      // [0]: count, [1..]: cells whose segment needs the long sort
+    int* partial = nullptr;  // scan partials
+    size_t cap = 0, big_cap = 0, npartial = 0;
+};
+
+namespace vflip {
+
+// ----------------------------------------------------------------- binning
+// Truncating float->int of Vec3i(Vec3f) (S/vec.h:37-39); NaN and values
+// outside int range give INT_MIN, the x86-64 conversion result.
+__device__ inline int trunc_i32(float t) {
+    if (!(t > -2147483648.0f && t < 2147483648.0f)) return INT_MIN;
+    return (int)t;
+}
+
+struct BinGeom {
+    float h;
+    int nx, ny, nz;
+    int clamp;
+    float bx, by, bz;
+};
+
+// Cell indices of a position; returns false when the coarse-grid rule drops it.
+__device__ inline bool cell_of(float x, float y, float z, const BinGeom& g, int& i, int& j, int& k) {
+    const float half = 0.5f * g.h;
+    if (g.clamp) {
+        x = x - g.bx;
+        y = y - g.by;
+        z = z - g.bz;
+    }
+    x = x + half;
+    y = y + half;
+    z = z + half;
+    i = trunc_i32(x / g.h);
+    j = trunc_i32(y / g.h);
+    k = trunc_i32(z / g.h);
+    if (g.clamp) {
+        i = i < 0 ? 0 : (i > g.nx - 1 ? g.nx - 1 : i);
+        j = j < 0 ? 0 : (j > g.ny - 1 ? g.ny - 1 : j);
+        k = k < 0 ? 0 : (k > g.nz - 1 ? g.nz - 1 : k);
+        return true;
+    }
+    return i >= 0 && j >= 0 && k >= 0 && i < g.nx && j < g.ny && k < g.nz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i32(int* a, size_t n, int v) {
+    for (size_t c = blockIdx.x * (size_t)kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock) a[c] = v;
+}
+
+// Key per particle and its slot among the particles of its cell. Runs of equal
+// keys inside a wave (the common case for particles stored cell-major) take one
+// atomic per run.
+__global__ __launch_bounds__(kBlock) void k_bin_count(int np, const float* __restrict__ pos, BinGeom g,
+                                                      int* __restrict__ key, int* __restrict__ slot,
+                                                      int* __restrict__ count) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    int kk = -1;
+    if (p < np) {
+        int i, j, k;
+        if (cell_of(pos[3 * (size_t)p], pos[3 * (size_t)p + 1], pos[3 * (size_t)p + 2], g, i, j, k))
+            kk = (k * g.ny + j) * g.nx + i;
+    }
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(kk, 1, 64);
+    const bool head = lane == 0 || kk != prev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long le = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int hl = 63 - __clzll((long long)(heads & le));
+    const unsigned long long after = heads & ~le;
+    const int nh = after ? __ffsll((unsigned long long)after) - 1 : 64;
+    int base = 0;
+    if (head && kk >= 0) base = atomicAdd(&count[kk], nh - lane);
+    base = __shfl(base, hl, 64);
+    if (p < np) {
+        key[p] = kk;
+        slot[p] = base + (lane - hl);
+    }
+}
+
+// Exclusive scan of n ints in place, over chunks of kScanChunk.
+constexpr int kScanItems = 8;
+constexpr int kScanChunk = kBlock * kScanItems;
+
+// Block-wide exclusive scan of one value per thread (kBlock threads); returns
+// the exclusive prefix and the block total.
+__device__ inline int block_exclusive_scan(int v, int& total) {
+    __shared__ int wsum[kBlock / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; ++q) {
+        if (q < wid) before += wsum[q];
+        total += wsum[q];
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(size_t n, const int* __restrict__ a, int* __restrict__ partial) {
+    const size_t base = (size_t)blockIdx.x * kScanChunk + (size_t)threadIdx.x * kScanItems;
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q)
+        if (base + q < n) s += a[base + q];
+    s = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// One workgroup of 1024 threads: exclusive scan of the partials, grand total to *total.
+__global__ __launch_bounds__(1024) void k_scan_partials(int nb, int* __restrict__ partial, int* __restrict__ total) {
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per;
+    int s = 0;
+    for (int q = 0; q < per; ++q)
+        if (b0 + q < nb) s += partial[b0 + q];
+    int inc = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int x = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += x;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int q = 0; q < 16; ++q) {
+        if (q < wid) before += wsum[q];
+        all += wsum[q];
+    }
+    int run = before + inc - s;
+    for (int q = 0; q < per; ++q)
+        if (b0 + q < nb) {
+            const int x = partial[b0 + q];
+            partial[b0 + q] = run;
+            run += x;
+        }
+    if (t == 0) *total = all;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(size_t n, int* __restrict__ a, const int* __restrict__ partial) {
+    const size_t base = (size_t)blockIdx.x * kScanChunk + (size_t)threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        v[q] = base + q < n ? a[base + q] : 0;
+        s += v[q];
+    }
+    int total;
+    int run = block_exclusive_scan(s, total) + partial[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q)
+        if (base + q < n) {
+            a[base + q] = run;
+            run += v[q];
+        }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(int np, const int* __restrict__ key, const int* __restrict__ slot,
+                                                        const int* __restrict__ start, int* __restrict__ order) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const int kk = key[p];
+    if (kk >= 0) order[start[kk] + slot[p]] = p;
+}
+
+// Atomic slots order a cell's particles by arrival; the reference order is
+// ascending particle id. Short unsorted segments are insertion-sorted by their
+// thread; longer ones are queued for k_sort_long.
+constexpr int kShortSegment = 64;
+
+__global__ __launch_bounds__(kBlock) void k_cell_fix(size_t ncell, const int* __restrict__ start, int* __restrict__ order,
+                                                     int* __restrict__ big) {
+    const size_t c = blockIdx.x * (size_t)kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    const int s = start[c], e = start[c + 1];
+    if (e - s < 2) return;
+    int prev = order[s];
+    bool sorted = true;
+    for (int a = s + 1; a < e; ++a) {
+        const int x = order[a];
+        if (x < prev) {
+            sorted = false;
+            break;
+        }
+        prev = x;
+    }
+    if (sorted) return;
+    if (e - s <= kShortSegment) {
+        for (int a = s + 1; a < e; ++a) {
+            const int x = order[a];
+            int b = a - 1;
+            while (b >= s && order[b] > x) {
+                order[b + 1] = order[b];
+                --b;
+            }
+            order[b + 1] = x;
+        }
+    } else {
+        const int q = atomicAdd(&big[0], 1);
+        big[1 + q] = (int)c;
+    }
+}
+
+// Number of entries of the sorted run r[0..n) smaller than x.
+__device__ inline int count_less(const int* r, int n, int x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (r[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Long segments: insertion-sorted runs of kShortSegment, then pairwise merges
+// where every element finds its output slot by a binary search in the other
+// run (ids are distinct, so ranks never tie). One workgroup per segment.
+__global__ __launch_bounds__(kBlock) void k_sort_long(const int* __restrict__ start, int* __restrict__ order,
+                                                      int* __restrict__ tmp, const int* __restrict__ big) {
+    const int nbig = big[0];
+    for (int b = blockIdx.x; b < nbig; b += gridDim.x) {
+        const int c = big[1 + b];
+        const int s = start[c], m = start[c + 1] - s;
+        int* src = order + s;
+        int* dst = tmp + s;
+        for (int r0 = threadIdx.x * kShortSegment; r0 < m; r0 += kBlock * kShortSegment) {
+            const int r1 = min(r0 + kShortSegment, m);
+            for (int a = r0 + 1; a < r1; ++a) {
+                const int x = src[a];
+                int q = a - 1;
+                while (q >= r0 && src[q] > x) {
+                    src[q + 1] = src[q];
+                    --q;
+                }
+                src[q + 1] = x;
+            }
+        }
+        __syncthreads();
+        for (int w = kShortSegment; w < m; w <<= 1) {
+            for (int a = threadIdx.x; a < m; a += kBlock) {
+                const int base = (a / (2 * w)) * (2 * w);
+                const int mid = min(base + w, m), end = min(base + 2 * w, m);
+                const int x = src[a];
+                const int at = a < mid ? (a - base) + count_less(src + mid, end - mid, x)
+                                       : (a - mid) + count_less(src + base, mid - base, x);
+                dst[base + at] = x;
+            }
+            __syncthreads();
+            int* t = src;
+            src = dst;
+            dst = t;
+        }
+        if (src != order + s) {
+            for (int a = threadIdx.x; a < m; a += kBlock) order[s + a] = src[a];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------- P2G
+// H(r) of S/Smoke_solver3D.h:217-223.
+__device__ inline float tent(float r) {
+    float res = 0;
+    if (r >= -1 && r < 0) res = 1 + r;
+    if (r >= 0 && r < 1) res = 1 - r;
+    return res;
+}
+
+// World coordinate of grid index f with offset o: ((float)f - o)*h rounded to float.
+__device__ inline float grid_world(int f, double o, double hd) { return (float)(((double)(float)f - o) * hd); }
+
+struct P2GArgs {
+    const float* pos;
+    const float* vel;
+    const int* start;
+    const int* order;
+    int nx, ny, nz;
+    float h;
+    int sub;
+    float bx, by, bz;
+    float* field[3];  // scatter: sum w*v accumulators; then field values
+    float* coef[3];   // scatter: sum w accumulators; then 1e-5 + sum w
+};
+
+__host__ __device__ inline int face_dim(int comp, int axis, int nx, int ny, int nz) {
+    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    return n + (comp == axis ? 1 : 0);
+}
+
+// Subdomain faces within 4 cells of the boundary use the wide kernel
+// (SubDomainSolver::determineDist, S/sub_domain.h:89-103).
+__device__ inline bool sub_shell(int i, int j, int k, int nx, int ny, int nz) {
+    const int d = min(min(min(i, nx - i), min(j, ny - j)), min(k, nz - k));
+    return d <= 4;
+}
+
+// Tile of cells per workgroup for the scatter; its faces plus a one-face
+// halo on every side are accumulated in LDS.
+constexpr int kPTX = 8, kPTY = 8, kPTZ = 8;
+constexpr int kPLX = kPTX + 2, kPLY = kPTY + 2, kPLZ = kPTZ + 2, kPLN = kPLX * kPLY * kPLZ;
+
+// LDS-binned scatter: each workgroup walks the particles of its tile's cells
+// (rows of cells are contiguous in the binned order), accumulates the tent
+// weights of all three components into LDS with LDS atomics, then adds its
+// non-zero LDS faces to the global accumulators with one global atomic each.
+__global__ __launch_bounds__(kBlock) void k_p2g_scatter(P2GArgs a, int tiles_x, int tiles_y, int ntiles) {
+    __shared__ float acc[3][2][kPLN];
+    for (int q = threadIdx.x; q < 3 * 2 * kPLN; q += kBlock) (&acc[0][0][0])[q] = 0.f;
+    __syncthreads();
+    // XCD-aware tile order: consecutive workgroups go round-robin over the 8
+    // XCDs, so hand each XCD a contiguous band of tiles (shared halo faces stay
+    // in one L2).
+    const int L = blockIdx.x;
+    int t = L;
+    if (ntiles % 8 == 0) t = (L % 8) * (ntiles / 8) + L / 8;
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
+    const int x0 = tx * kPTX, y0 = ty * kPTY, z0 = tz * kPTZ;
+    const int x1 = min(x0 + kPTX, a.nx);
+    const double hd = a.h;
+    const BinGeom g{a.h, a.nx, a.ny, a.nz, a.sub, a.bx, a.by, a.bz};
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int r = wid; r < kPTY * kPTZ; r += kBlock / 64) {
+        const int y = y0 + r % kPTY, z = z0 + r / kPTY;
+        if (y >= a.ny || z >= a.nz) continue;
+        const size_t c0 = lin3(x0, y, z, a.nx, a.ny);
+        const int q0 = a.start[c0], q1 = a.start[c0 + (x1 - x0)];
+        for (int q = q0 + lane; q < q1; q += 64) {
+            const int p = a.order[q];
+            const float px = a.pos[3 * (size_t)p], py = a.pos[3 * (size_t)p + 1], pz = a.pos[3 * (size_t)p + 2];
+            const float vv[3] = {a.vel[3 * (size_t)p], a.vel[3 * (size_t)p + 1], a.vel[3 * (size_t)p + 2]};
+            int ci[3];
+            cell_of(px, py, pz, g, ci[0], ci[1], ci[2]);
+            // the binning placed p in this tile; a caller whose bmin differs from
+            // the one it binned with gets the particle skipped, not an LDS overrun
+            if ((unsigned)(ci[0] - x0) >= (unsigned)kPTX || (unsigned)(ci[1] - y0) >= (unsigned)kPTY ||
+                (unsigned)(ci[2] - z0) >= (unsigned)kPTZ)
+                continue;
+            float pp[3] = {px, py, pz};
+            if (a.sub) {
+                pp[0] = px - a.bx;
+                pp[1] = py - a.by;
+                pp[2] = pz - a.bz;
+            }
+            // per axis: weights of the three candidate faces ci-1..ci+1, for
+            // cell-centred (offset 0) and staggered (offset .5) positions
+            float wc[3][3], ws[3][3];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int f = ci[ax] - 1 + d;
+                    wc[ax][d] = tent((pp[ax] - grid_world(f, 0.0, hd)) / a.h);
+                    ws[ax][d] = tent((pp[ax] - grid_world(f, 0.5, hd)) / a.h);
+                }
+            const int lx = ci[0] - 1 - (x0 - 1), ly = ci[1] - 1 - (y0 - 1), lz = ci[2] - 1 - (z0 - 1);
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) {
+                if (!a.field[comp]) continue;
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) {
+                    const float wz = comp == 2 ? ws[2][dz] : wc[2][dz];
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy) {
+                        const float wy = comp == 1 ? ws[1][dy] : wc[1][dy];
+#pragma unroll
+                        for (int dx = 0; dx < 3; ++dx) {
+                            const float wx = comp == 0 ? ws[0][dx] : wc[0][dx];
+                            const float wgt = wx * wy * wz;
+                            if (wgt != 0.f) {
+                                const int li = (lx + dx) + kPLX * ((ly + dy) + kPLY * (lz + dz));
+                                atomicAdd(&acc[comp][0][li], wgt);
+                                atomicAdd(&acc[comp][1][li], wgt * vv[comp]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int comp = 0; comp < 3; ++comp) {
+        if (!a.field[comp]) continue;
+        const int fx = face_dim(comp, 0, a.nx, a.ny, a.nz), fy = face_dim(comp, 1, a.nx, a.ny, a.nz),
+                  fz = face_dim(comp, 2, a.nx, a.ny, a.nz);
+        for (int li = threadIdx.x; li < kPLN; li += kBlock) {
+            const float sw = acc[comp][0][li];
+            if (sw == 0.f) continue;
+            const int i = x0 - 1 + li % kPLX, j = y0 - 1 + (li / kPLX) % kPLY, k = z0 - 1 + li / (kPLX * kPLY);
+            if (i < 0 || j < 0 || k < 0 || i >= fx || j >= fy || k >= fz) continue;
+            if (a.sub && sub_shell(i, j, k, a.nx, a.ny, a.nz)) continue;
+            const size_t o = lin3(i, j, k, fx, fy);
+            unsafeAtomicAdd(&a.coef[comp][o], sw);
+            unsafeAtomicAdd(&a.field[comp][o], acc[comp][1][li]);
+        }
+    }
+}
+
+// coef = 1e-5 + sum w, field = sum w*v / coef (S/Smoke_solver3D.cpp:1475-1503).
+__global__ __launch_bounds__(kBlock) void k_p2g_normalize(int comp, int nx, int ny, int nz, int sub,
+                                                          float* __restrict__ field, float* __restrict__ coef) {
+    const int fx = face_dim(comp, 0, nx, ny, nz), fy = face_dim(comp, 1, nx, ny, nz), fz = face_dim(comp, 2, nx, ny, nz);
+    const size_t n = (size_t)fx * fy * fz;
+    for (size_t o = blockIdx.x * (size_t)kBlock + threadIdx.x; o < n; o += (size_t)gridDim.x * kBlock) {
+        if (sub) {
+            const int i = (int)(o % fx), j = (int)((o / fx) % fy), k = (int)(o / ((size_t)fx * fy));
+            if (sub_shell(i, j, k, nx, ny, nz)) continue;
+        }
+        const float tw = 1e-5f + coef[o];
+        coef[o] = tw;
+        field[o] = field[o] / tw;
+    }
+}
+
+// Per-face gather in the reference's order: cells of the (2l+1)^3 window
+// z-outer/x-inner, particles in binned order, float accumulation from 1e-5.
+// shell_only: only the subdomain's wide-kernel faces (the scatter did the rest).
+__global__ __launch_bounds__(kBlock) void k_p2g_gather(P2GArgs a, int comp, int shell_only) {
+    const int fx = face_dim(comp, 0, a.nx, a.ny, a.nz), fy = face_dim(comp, 1, a.nx, a.ny, a.nz),
+              fz = face_dim(comp, 2, a.nx, a.ny, a.nz);
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= fx || j >= fy || k >= fz) return;
+    int l = 1;
+    float hk = a.h;
+    if (a.sub && sub_shell(i, j, k, a.nx, a.ny, a.nz)) {
+        l = 4;
+        hk = 4.0f * a.h;
+    } else if (shell_only) {
+        return;
+    }
+    const double hd = a.h;
+    const float gx = grid_world(i, comp == 0 ? 0.5 : 0.0, hd), gy = grid_world(j, comp == 1 ? 0.5 : 0.0, hd),
+                gz = grid_world(k, comp == 2 ? 0.5 : 0.0, hd);
+    float tw = 1e-5f, tv = 0.f;
+    for (int c = k - l; c <= k + l; ++c) {
+        if (c < 0 || c >= a.nz) continue;
+        for (int b = j - l; b <= j + l; ++b) {
+            if (b < 0 || b >= a.ny) continue;
+            for (int ia = i - l; ia <= i + l; ++ia) {
+                if (ia < 0 || ia >= a.nx) continue;
+                const size_t cell = lin3(ia, b, c, a.nx, a.ny);
+                const int q1 = a.start[cell + 1];
+                for (int q = a.start[cell]; q < q1; ++q) {
+                    const int p = a.order[q];
+                    float px = a.pos[3 * (size_t)p], py = a.pos[3 * (size_t)p + 1], pz = a.pos[3 * (size_t)p + 2];
+                    if (a.sub) {
+                        px = px - a.bx;
+                        py = py - a.by;
+                        pz = pz - a.bz;
+                    }
+                    const float wgt = tent((px - gx) / hk) * tent((py - gy) / hk) * tent((pz - gz) / hk);
+                    tw += wgt;
+                    tv += wgt * a.vel[3 * (size_t)p + comp];
+                }
+            }
+        }
+    }
+    const size_t o = lin3(i, j, k, fx, fy);
+    a.coef[comp][o] = tw;
+    a.field[comp][o] = tv / tw;
+}
+
+// ---------------------------------------------------------------------- G2P
+__global__ __launch_bounds__(kBlock) void k_get_delta(size_t n, const float* __restrict__ fnew,
+                                                      const float* __restrict__ fold, float* __restrict__ out) {
+    for (size_t c = blockIdx.x * (size_t)kBlock + threadIdx.x; c < n; c += (size_t)gridDim.x * kBlock)
+        out[c] = fnew[c] - fold[c];
+}
+
+struct G2PArgs {
+    int nx, ny, nz;
+    float h;
+    const float* fnew[3];
+    const float* fold[3];  // all NULL: fnew holds the delta grids
+    float flip;
+    float bx, by, bz;
+};
+
+__global__ __launch_bounds__(kBlock) void k_g2p(int np, const float* __restrict__ pos, float* __restrict__ vel,
+                                                G2PArgs a) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const double hd = a.h;
+    const float x = pos[3 * (size_t)p] - a.bx, y = pos[3 * (size_t)p + 1] - a.by, z = pos[3 * (size_t)p + 2] - a.bz;
+    const int dims[3][3] = {{a.nx + 1, a.ny, a.nz}, {a.nx, a.ny + 1, a.nz}, {a.nx, a.ny, a.nz + 1}};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double ox = c == 0 ? 0.5 : 0.0, oy = c == 1 ? 0.5 : 0.0, oz = c == 2 ? 0.5 : 0.0;
+        float d;
+        if (a.fold[c]) {
+            const DeltaGrid dg{a.fnew[c], a.fold[c], dims[c][0], dims[c][1], dims[c][2], ox, oy, oz};
+            d = sample_linear(dg, hd, x, y, z);
+        } else {
+            const Grid dg{a.fnew[c], dims[c][0], dims[c][1], dims[c][2], ox, oy, oz};
+            d = sample_linear(dg, hd, x, y, z);
+        }
+        float& vc = vel[3 * (size_t)p + c];
+        if (a.flip >= 1.0f) {
+            vc += d;
+        } else {
+            const Grid gn{a.fnew[c], dims[c][0], dims[c][1], dims[c][2], ox, oy, oz};
+            const float pic = sample_linear(gn, hd, x, y, z);
+            const double fw = (double)a.flip, pw = 1.0 - fw;
+            vc = (float)(pw * pic + fw * (vc + d));
+        }
+    }
+}
+
+// ------------------------------------------------------- particle advection
+struct AdvArgs {
+    int nx, ny, nz;
+    float h, dt;
+    int sub;
+    float bx, by, bz;
+    const float* u;
+    const float* v;
+    const float* w;
+    const float* phi;
+};
+
+__device__ inline void velocity_at(const AdvArgs& a, const float* x, float* o) {
+    float px = x[0], py = x[1], pz = x[2];
+    if (a.sub) {
+        px = px - a.bx;
+        py = py - a.by;
+        pz = pz - a.bz;
+    }
+    const double hd = a.h;
+    const Grid gu{a.u, a.nx + 1, a.ny, a.nz, 0.5, 0.0, 0.0}, gv{a.v, a.nx, a.ny + 1, a.nz, 0.0, 0.5, 0.0},
+        gw{a.w, a.nx, a.ny, a.nz + 1, 0.0, 0.0, 0.5};
+    o[0] = sample_linear(gu, hd, px, py, pz);
+    o[1] = sample_linear(gv, hd, px, py, pz);
+    o[2] = sample_linear(gw, hd, px, py, pz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_advect_particles(int np, float* __restrict__ pos, AdvArgs a) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= np) return;
+    const float dt = a.dt;
+    const float c1 = (float)(2.0 / 9.0 * dt), c2 = (float)(3.0 / 9.0 * dt);
+    const float c3 = a.sub ? (float)(4.0 / 9.0 * dt) : (float)((1.0 - c1 - c2) * dt);
+    const float t1 = (float)(0.5 * dt), t2 = (float)(0.75 * dt);
+    float x[3] = {pos[3 * (size_t)p], pos[3 * (size_t)p + 1], pos[3 * (size_t)p + 2]};
+    float v1[3], v2[3], v3[3], m[3];
+    velocity_at(a, x, v1);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) m[q] = x[q] + t1 * v1[q];
+    velocity_at(a, m, v2);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) m[q] = x[q] + t2 * v2[q];
+    velocity_at(a, m, v3);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) x[q] = x[q] + c1 * v1[q] + c2 * v2[q] + c3 * v3[q];
+    const double hd = a.h;
+    const Grid gp{a.phi, a.nx + 1, a.ny + 1, a.nz + 1, 0.5, 0.5, 0.5};
+    const float ph = sample_linear(gp, hd, x[0], x[1], x[2]);
+    if (ph < 0) {
+        float n[3];
+        sample_grad(gp, hd, x[0], x[1], x[2], n);
+        // normalize: n /= (float)(sqrt((double)|n|^2) + 1e-6)  (S/vec.h:194-203,218-219)
+        float l = n[0] * n[0];
+        l += n[1] * n[1];
+        l += n[2] * n[2];
+        const float mg = (float)(sqrt((double)l) + 1e-6);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            n[q] /= mg;
+            x[q] -= ph * n[q];
+        }
+    }
+    pos[3 * (size_t)p] = x[0];
+    pos[3 * (size_t)p + 1] = x[1];
+    pos[3 * (size_t)p + 2] = x[2];
+}
+
+// ------------------------------------------------------------------ helpers
+static int ensure_capacity(vflip_particles* P, int np) {
+    if ((size_t)np > P->cap) {
+        const size_t cap = (size_t)np + (size_t)np / 4 + 1024;
+        if (P->slot) VF_HIP(hipFree(P->slot));
+        if (P->tmp) VF_HIP(hipFree(P->tmp));
+        if (P->big) VF_HIP(hipFree(P->big));
+        P->slot = P->tmp = P->big = nullptr;
+        P->cap = 0;
+        const size_t big_cap = cap / (kShortSegment + 1) + 2;
+        if (hipMalloc(&P->slot, cap * sizeof(int)) != hipSuccess || hipMalloc(&P->tmp, cap * sizeof(int)) != hipSuccess ||
+            hipMalloc(&P->big, (big_cap + 1) * sizeof(int)) != hipSuccess) {
+            set_error("particle workspace of %zu entries: out of device memory", cap);
+            return VFLIP_ERR_NOMEM;
+        }
+        P->cap = cap;
+        P->big_cap = big_cap;
+    }
+    return 0;
+}
+
+}  // namespace vflip
+
+using namespace vflip;
+
+extern "C" {
+
+int vflip_particles_create(int ni, int nj, int nk, float h, vflip_particles** out) {
+    VF_ARG(out, "out");
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && h > 0, "grid dims and spacing");
+    VF_ARG((double)ni * nj * nk < 2147483647.0, "cell count must fit in int");
+    auto* P = new vflip_particles();
+    P->nx = ni;
+    P->ny = nj;
+    P->nz = nk;
+    P->h = h;
+    P->ncell = (size_t)ni * nj * nk;
+    P->npartial = (P->ncell + kScanChunk - 1) / kScanChunk;
+    if (hipMalloc(&P->partial, P->npartial * sizeof(int)) != hipSuccess) {
+        delete P;
+        set_error("scan partials: out of device memory");
+        return VFLIP_ERR_NOMEM;
+    }
+    *out = P;
+    return 0;
+}
+
+int vflip_particles_destroy(vflip_particles* P) {
+    if (!P) return 0;
+    if (P->slot) (void)hipFree(P->slot);
+    if (P->tmp) (void)hipFree(P->tmp);
+    if (P->big) (void)hipFree(P->big);
+    if (P->partial) (void)hipFree(P->partial);
+    delete P;
+    return 0;
+}
+
+int vflip_particles_set_stream(vflip_particles* P, void* hip_stream) {
+    VF_ARG(P, "handle");
+    P->stream = (hipStream_t)hip_stream;
+    return 0;
+}
+
+int vflip_hash(vflip_particles* P, int np, const float* pos, int clamp, float bx, float by, float bz, int* cell_key,
+               int* cell_start, int* order, int* nbinned) {
+    VF_ARG(P && np >= 0 && cell_start, "hash args");
+    VF_ARG(np == 0 || (pos && cell_key && order), "hash arrays");
+    if (int rc = ensure_capacity(P, np > 0 ? np : 1)) return rc;
+    hipStream_t s = P->stream;
+    const size_t nc = P->ncell;
+    const BinGeom g{P->h, P->nx, P->ny, P->nz, clamp ? 1 : 0, bx, by, bz};
+    k_fill_i32<<<min(grid_for(nc + 1), 4096u), kBlock, 0, s>>>(cell_start, nc + 1, 0);
+    k_fill_i32<<<1, kBlock, 0, s>>>(P->big, 1, 0);
+    if (np > 0) k_bin_count<<<grid_for(np), kBlock, 0, s>>>(np, pos, g, cell_key, P->slot, cell_start);
+    const unsigned nb = (unsigned)P->npartial;
+    k_scan_reduce<<<nb, kBlock, 0, s>>>(nc, cell_start, P->partial);
+    k_scan_partials<<<1, 1024, 0, s>>>((int)nb, P->partial, cell_start + nc);
+    k_scan_apply<<<nb, kBlock, 0, s>>>(nc, cell_start, P->partial);
+    if (np > 0) {
+        k_bin_scatter<<<grid_for(np), kBlock, 0, s>>>(np, cell_key, P->slot, cell_start, order);
+        k_cell_fix<<<grid_for(nc), kBlock, 0, s>>>(nc, cell_start, order, P->big);
+        k_sort_long<<<256, kBlock, 0, s>>>(cell_start, order, P->tmp, P->big);
+    }
+    VF_CHECK_LAUNCH();
+    if (nbinned) {
+        VF_HIP(hipMemcpyAsync(nbinned, cell_start + nc, sizeof(int), hipMemcpyDeviceToHost, s));
+        VF_HIP(hipStreamSynchronize(s));
+    }
+    return 0;
+}
+
+int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* pos, const float* vel,
+              const int* cell_start, const int* order, float bx, float by, float bz, float* u, float* u_coef, float* v,
+              float* v_coef, float* w, float* w_coef) {
+    VF_ARG(P && np >= 0 && cell_start && (np == 0 || (pos && vel && order)), "p2g args");
+    VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
+    VF_ARG(method == 0 || method == 1, "method");
+    VF_ARG((!u) == (!u_coef) && (!v) == (!v_coef) && (!w) == (!w_coef), "field and coef come in pairs");
+    hipStream_t s = P->stream;
+    const int nx = P->nx, ny = P->ny, nz = P->nz;
+    const int sub = variant == VFLIP_VARIANT_SUB;
+    // positions binned with bmin only in the subdomain form
+    P2GArgs a{pos, vel, cell_start, order, nx, ny, nz, P->h, sub, sub ? bx : 0.f, sub ? by : 0.f, sub ? bz : 0.f,
+              {u, v, w}, {u_coef, v_coef, w_coef}};
+    if (method == 0) {
+        for (int c = 0; c < 3; ++c) {
+            if (!a.field[c]) continue;
+            const size_t n = (size_t)face_dim(c, 0, nx, ny, nz) * face_dim(c, 1, nx, ny, nz) * face_dim(c, 2, nx, ny, nz);
+            k_fill_i32<<<min(grid_for(n), 8192u), kBlock, 0, s>>>((int*)a.field[c], n, 0);
+            k_fill_i32<<<min(grid_for(n), 8192u), kBlock, 0, s>>>((int*)a.coef[c], n, 0);
+        }
+        const int tiles_x = (nx + kPTX - 1) / kPTX, tiles_y = (ny + kPTY - 1) / kPTY, tiles_z = (nz + kPTZ - 1) / kPTZ;
+        const int ntiles = tiles_x * tiles_y * tiles_z;
+        k_p2g_scatter<<<ntiles, kBlock, 0, s>>>(a, tiles_x, tiles_y, ntiles);
+        for (int c = 0; c < 3; ++c) {
+            if (!a.field[c]) continue;
+            const size_t n = (size_t)face_dim(c, 0, nx, ny, nz) * face_dim(c, 1, nx, ny, nz) * face_dim(c, 2, nx, ny, nz);
+            k_p2g_normalize<<<min(grid_for(n), 8192u), kBlock, 0, s>>>(c, nx, ny, nz, sub, a.field[c], a.coef[c]);
+        }
+    }
+    if (method == 1 || sub) {
+        for (int c = 0; c < 3; ++c) {
+            if (!a.field[c]) continue;
+            const int fx = face_dim(c, 0, nx, ny, nz), fy = face_dim(c, 1, nx, ny, nz), fz = face_dim(c, 2, nx, ny, nz);
+            k_p2g_gather<<<dim3((fx + 63) / 64, (fy + 3) / 4, fz), dim3(64, 4), 0, s>>>(a, c, method == 0 ? 1 : 0);
+        }
+    }
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+int vflip_get_delta(size_t n, const float* fnew, const float* fold, float* out, void* hip_stream) {
+    VF_ARG(fnew && fold && out, "get_delta arrays");
+    if (n == 0) return 0;
+    k_get_delta<<<min(grid_for(n), 8192u), kBlock, 0, (hipStream_t)hip_stream>>>(n, fnew, fold, out);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+int vflip_g2p(int ni, int nj, int nk, float h, int np, const float* pos, float* vel, const float* u_new,
+              const float* v_new, const float* w_new, const float* u_old, const float* v_old, const float* w_old,
+              float flip_ratio, float bx, float by, float bz, void* hip_stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && h > 0 && np >= 0, "g2p dims");
+    VF_ARG(np == 0 || (pos && vel), "g2p particles");
+    VF_ARG(u_new && v_new && w_new, "g2p grids");
+    VF_ARG((!u_old) == (!v_old) && (!v_old) == (!w_old), "old grids: all or none");
+    VF_ARG(flip_ratio >= 1.0f || u_old, "a PIC blend (flip_ratio < 1) needs the old grids");
+    if (np == 0) return 0;
+    G2PArgs a{ni, nj, nk, h, {u_new, v_new, w_new}, {u_old, v_old, w_old}, flip_ratio, bx, by, bz};
+    k_g2p<<<grid_for(np), kBlock, 0, (hipStream_t)hip_stream>>>(np, pos, vel, a);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+int vflip_advect_particles(int ni, int nj, int nk, float h, float dt, int variant, float bx, float by, float bz, int np,
+                           float* pos, const float* u, const float* v, const float* w, const float* solid_phi,
+                           void* hip_stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && h > 0 && np >= 0, "advect_particles dims");
+    VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
+    VF_ARG(np == 0 || (pos && u && v && w && solid_phi), "advect_particles arrays");
+    if (np == 0) return 0;
+    AdvArgs a{ni, nj, nk, h, dt, variant == VFLIP_VARIANT_SUB, bx, by, bz, u, v, w, solid_phi};
+    k_advect_particles<<<grid_for(np), kBlock, 0, (hipStream_t)hip_stream>>>(np, pos, a);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+}  // extern "C"
