@@ -245,10 +245,13 @@ int vflip_hash(vflip_particles* P, int np, const float* pos, int clamp, float bx
  * S/sub_domain.h:89-103). For every MAC face: coef = 1e-5 + sum w,
  * field = sum w*vel_c / coef, w = tent weights H(dx/h)H(dy/h)H(dz/h)
  * (S/Smoke_solver3D.h:217-233) over the particles of the 3^3 cells around the
- * face. Takes vflip_hash's cell_start/order. method 0: LDS-binned scatter, then
- * global atomics (FP32 sums in another order than the reference's); method 1:
- * per-face gather in the reference's summation order (bit-exact). A component
- * with NULL field and coef is skipped. */
+ * face. Takes vflip_hash's cell_start/order. method 0: owner-computes tiles
+ * (per-cell register sums, LDS-binned adds, plain stores); method 1: per-face
+ * gather in the reference's summation order (bit-exact); method 2: LDS-binned
+ * scatter over each tile's own cells, then global atomics for every face.
+ * Methods 0 and 2 differ from the reference at FP32 rounding level only: they
+ * sum in another order, and method 0 forms the tent arguments from one scaled
+ * offset per axis. A component with NULL field and coef is skipped. */
 int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* pos, const float* vel,
               const int* cell_start, const int* order, float bx, float by, float bz, float* u, float* u_coef,
               float* v, float* v_coef, float* w, float* w_coef);

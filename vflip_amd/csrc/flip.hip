@@ -305,6 +305,12 @@ __device__ inline float tent(float r) {
     return res;
 }
 
+// Branch-free H(r) for the scatter paths: same values for every r.
+__device__ inline float tent_nb(float r) {
+    const float t = 1.f - fabsf(r);
+    return (r >= -1.f && r < 1.f) ? t : 0.f;
+}
+
 // World coordinate of grid index f with offset o: ((float)f - o)*h rounded to float.
 __device__ inline float grid_world(int f, double o, double hd) { return (float)(((double)(float)f - o) * hd); }
 
@@ -447,6 +453,252 @@ __global__ __launch_bounds__(kBlock) void k_p2g_normalize(int comp, int nx, int 
         const float tw = 1e-5f + coef[o];
         coef[o] = tw;
         field[o] = field[o] / tw;
+    }
+}
+
+// Owner-computes P2G. A workgroup owns the faces of a kOT^3 block of face
+// indices (the last block along an axis also owns the extra staggered face).
+// Every cell within one cell of the block is walked by one thread, which sums
+// its particles' weights in registers, one component at a time: the staggered
+// axis reaches faces c and c+1 (c-1 only when rounding puts a particle a hair
+// below its cell's lower face, handled apart with LDS atomics) and the other
+// axes faces c-1..c+1, so 2 x 3 x 3 faces. The sums go to LDS (one add per face
+// per cell; the lanes of one add target distinct faces), and the block then
+// writes coef = 1e-5 + sum w and field = sum w*v / coef with plain stores: no
+// global atomics, no zero fill, no separate normalisation pass. One component
+// per pass keeps the register sums at 36, so enough waves stay resident to hide
+// the particle loads and the LDS adds.
+constexpr int kOT = 8;
+constexpr int kOH = kOT + 2;  // cells per axis that touch the block's faces
+constexpr int kOF = kOT + 1;  // LDS faces per axis (extra face of the last block)
+constexpr int kOFN = kOF * kOF * kOF;
+
+struct OwnerTile {
+    int b0[3];
+    int hi[3][3];  // owned face range [b0, hi) per component and axis
+};
+
+// Particles of one cell held in registers: the first kOwnChunk are loaded
+// together (independent loads in flight at once) and reused by all three
+// components; cells holding more are rare and read the rest per component.
+constexpr int kOwnChunk = 8;
+
+struct CellParticles {
+    float r[kOwnChunk][3];  // scaled offsets to the centred face c-1 per axis
+    float v[kOwnChunk][3];
+};
+
+// One scaled offset per axis: r = distance in cells from the centred face c-1;
+// the other candidate faces sit whole or half cells from it.
+__device__ inline void scaled_offsets(const P2GArgs& a, size_t p, const float (&g0)[3], float inv_h, float (&r)[3]) {
+    const float bsh[3] = {a.bx, a.by, a.bz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        float x = a.pos[3 * p + ax];
+        if (a.sub) x = x - bsh[ax];
+        r[ax] = (x - g0[ax]) * inv_h;
+    }
+}
+
+// Rare: a particle a rounding hair below its cell's lower staggered face puts
+// face c-1 inside the tent support. Kept out of line so the hot loop stays small.
+template <int C>
+__device__ __attribute__((noinline)) void owner_wedge(int b0x, int b0y, int b0z, int hix, int hiy, int hiz, int cx,
+                                                      int cy, int cz, float wedge, float wu0, float wu1, float wu2,
+                                                      float wv0, float wv1, float wv2, float vc,
+                                                      float2 (&acc)[3][kOFN]) {
+    constexpr int UA = C == 0 ? 1 : 0, VA = C == 2 ? 1 : 2;
+    const int ci[3] = {cx, cy, cz}, b0[3] = {b0x, b0y, b0z}, hi[3] = {hix, hiy, hiz};
+    const float wu[3] = {wu0, wu1, wu2}, wv[3] = {wv0, wv1, wv2};
+    for (int u = 0; u < 3; ++u)
+        for (int v = 0; v < 3; ++v) {
+            int fi[3];
+            fi[C] = ci[C] - 1;
+            fi[UA] = ci[UA] - 1 + u;
+            fi[VA] = ci[VA] - 1 + v;
+            const float fx = C == 0 ? wedge : wu[u];
+            const float fy = C == 1 ? wedge : (UA == 1 ? wu[u] : wv[v]);
+            const float fz = C == 2 ? wedge : wv[v];
+            const float wgt = fx * fy * fz;
+            bool own = wgt != 0.f;
+            for (int ax = 0; ax < 3; ++ax) own = own && fi[ax] >= b0[ax] && fi[ax] < hi[ax];
+            if (own) {
+                const int li = (fi[0] - b0[0]) + kOF * ((fi[1] - b0[1]) + kOF * (fi[2] - b0[2]));
+                atomicAdd(&acc[C][li].x, wgt);
+                atomicAdd(&acc[C][li].y, wgt * vc);
+            }
+        }
+}
+
+template <int C>
+__device__ inline void owner_accumulate(const P2GArgs& a, const OwnerTile& T, const int (&ci)[3], const float (&r)[3],
+                                        float vc, float (&sw)[2][3][3], float (&sv)[2][3][3],
+                                        float2 (&acc)[3][kOFN]) {
+    constexpr int UA = C == 0 ? 1 : 0, VA = C == 2 ? 1 : 2;  // the two centred axes, UA < VA
+    const float ws[2] = {tent_nb(r[C] - 0.5f), tent_nb(r[C] - 1.5f)};
+    const float wu[3] = {tent_nb(r[UA]), tent_nb(r[UA] - 1.f), tent_nb(r[UA] - 2.f)};
+    const float wv[3] = {tent_nb(r[VA]), tent_nb(r[VA] - 1.f), tent_nb(r[VA] - 2.f)};
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                // x*y*z factor order of compute_weight
+                const float fx = C == 0 ? ws[s] : wu[u];
+                const float fy = C == 1 ? ws[s] : (UA == 1 ? wu[u] : wv[v]);
+                const float fz = C == 2 ? ws[s] : wv[v];
+                const float wgt = fx * fy * fz;
+                sw[s][u][v] += wgt;
+                sv[s][u][v] = __builtin_fmaf(wgt, vc, sv[s][u][v]);
+            }
+    const float wedge = tent_nb(r[C] + 0.5f);
+    if (wedge != 0.f)
+        owner_wedge<C>(T.b0[0], T.b0[1], T.b0[2], T.hi[C][0], T.hi[C][1], T.hi[C][2], ci[0], ci[1], ci[2], wedge, wu[0],
+                       wu[1], wu[2], wv[0], wv[1], wv[2], vc, acc);
+}
+
+// Component C of one cell: the register-held particles, then any beyond
+// kOwnChunk. Cells past n (or lanes without a cell) hold offsets outside every
+// tent and add exact zeros.
+template <int C>
+__device__ inline void owner_sums(const P2GArgs& a, const OwnerTile& T, const int (&ci)[3], int q0, int n,
+                                  const CellParticles& cp, const float (&g0)[3], float inv_h, float (&sw)[2][3][3],
+                                  float (&sv)[2][3][3], float2 (&acc)[3][kOFN]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) sw[s][u][v] = sv[s][u][v] = 0.f;
+#pragma unroll
+    for (int k = 0; k < kOwnChunk; ++k) owner_accumulate<C>(a, T, ci, cp.r[k], cp.v[k][C], sw, sv, acc);
+    for (int k = kOwnChunk; k < n; ++k) {
+        const size_t p = (size_t)a.order[q0 + k];
+        float r[3];
+        scaled_offsets(a, p, g0, inv_h, r);
+        owner_accumulate<C>(a, T, ci, r, a.vel[3 * p + C], sw, sv, acc);
+    }
+}
+
+// Adds the cell sums to the tile's faces with plain LDS read-modify-writes in
+// 18 barrier-separated phases, one per relative face offset: within a phase
+// every thread's face is distinct (one cell per thread), so no atomics are
+// needed. Every thread of the workgroup takes part (the barriers are uniform).
+template <int C>
+__device__ inline void owner_flush(const OwnerTile& T, const int (&ci)[3], bool valid, const float (&sw)[2][3][3],
+                                   const float (&sv)[2][3][3], float2 (&acc)[3][kOFN]) {
+    constexpr int UA = C == 0 ? 1 : 0, VA = C == 2 ? 1 : 2;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+            for (int v = 0; v < 3; ++v) {
+                int fi[3];
+                fi[C] = ci[C] + s;
+                fi[UA] = ci[UA] - 1 + u;
+                fi[VA] = ci[VA] - 1 + v;
+                bool own = valid;
+#pragma unroll
+                for (int ax = 0; ax < 3; ++ax) own = own && fi[ax] >= T.b0[ax] && fi[ax] < T.hi[C][ax];
+                if (own) {
+                    const int li = (fi[0] - T.b0[0]) + kOF * ((fi[1] - T.b0[1]) + kOF * (fi[2] - T.b0[2]));
+                    float2 x = acc[C][li];
+                    x.x += sw[s][u][v];
+                    x.y += sv[s][u][v];
+                    acc[C][li] = x;
+                }
+                __syncthreads();
+            }
+}
+
+__global__ __launch_bounds__(kBlock) void k_p2g_owner(P2GArgs a, int tiles_x, int tiles_y, int ntiles) {
+    __shared__ float2 acc[3][kOFN];
+    const int tid = threadIdx.x;
+    for (int q = tid; q < 3 * kOFN; q += kBlock) (&acc[0][0])[q] = make_float2(0.f, 0.f);
+    const int L = blockIdx.x;
+    int t = L;
+    if (ntiles % 8 == 0) t = (L % 8) * (ntiles / 8) + L / 8;
+    const int tx = t % tiles_x, ty = (t / tiles_x) % tiles_y, tz = t / (tiles_x * tiles_y);
+    OwnerTile T;
+    T.b0[0] = tx * kOT;
+    T.b0[1] = ty * kOT;
+    T.b0[2] = tz * kOT;
+    const int nn[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int fd = nn[ax] + (c == ax ? 1 : 0);
+            T.hi[c][ax] = T.b0[ax] + kOT >= nn[ax] ? fd : T.b0[ax] + kOT;
+        }
+    const double hd = a.h;
+    const float inv_h = 1.0f / a.h;
+    for (int base = 0; base < kOH * kOH * kOH; base += kBlock) {
+        const int cid = base + tid;
+        const int ci[3] = {T.b0[0] - 1 + cid % kOH, T.b0[1] - 1 + (cid / kOH) % kOH, T.b0[2] - 1 + cid / (kOH * kOH)};
+        int q0 = 0, n = 0;
+        if (cid < kOH * kOH * kOH && ci[0] >= 0 && ci[1] >= 0 && ci[2] >= 0 && ci[0] < a.nx && ci[1] < a.ny &&
+            ci[2] < a.nz) {
+            const size_t cell = lin3(ci[0], ci[1], ci[2], a.nx, a.ny);
+            q0 = a.start[cell];
+            n = a.start[cell + 1] - q0;
+        }
+        // uniform across the workgroup: skip rounds without particles
+        if (!__syncthreads_or(n > 0)) continue;
+        const bool valid = n > 0;
+        const float g0[3] = {grid_world(ci[0] - 1, 0.0, hd), grid_world(ci[1] - 1, 0.0, hd),
+                             grid_world(ci[2] - 1, 0.0, hd)};
+        CellParticles cp;
+        int pid[kOwnChunk];
+#pragma unroll
+        for (int k = 0; k < kOwnChunk; ++k) pid[k] = k < n ? a.order[q0 + k] : -1;
+#pragma unroll
+        for (int k = 0; k < kOwnChunk; ++k) {
+            if (pid[k] >= 0) {
+                scaled_offsets(a, (size_t)pid[k], g0, inv_h, cp.r[k]);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) cp.v[k][c] = a.vel[3 * (size_t)pid[k] + c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    cp.r[k][c] = 8.f;
+                    cp.v[k][c] = 0.f;
+                }
+            }
+        }
+        float sw[2][3][3], sv[2][3][3];
+        if (a.field[0]) {
+            owner_sums<0>(a, T, ci, q0, n, cp, g0, inv_h, sw, sv, acc);
+            owner_flush<0>(T, ci, valid, sw, sv, acc);
+        }
+        if (a.field[1]) {
+            owner_sums<1>(a, T, ci, q0, n, cp, g0, inv_h, sw, sv, acc);
+            owner_flush<1>(T, ci, valid, sw, sv, acc);
+        }
+        if (a.field[2]) {
+            owner_sums<2>(a, T, ci, q0, n, cp, g0, inv_h, sw, sv, acc);
+            owner_flush<2>(T, ci, valid, sw, sv, acc);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (!a.field[c]) continue;
+        const int fx = face_dim(c, 0, a.nx, a.ny, a.nz), fy = face_dim(c, 1, a.nx, a.ny, a.nz);
+        const int ex = T.hi[c][0] - T.b0[0], ey = T.hi[c][1] - T.b0[1], ez = T.hi[c][2] - T.b0[2];
+        for (int li = tid; li < ex * ey * ez; li += kBlock) {
+            const int lx = li % ex, ly = (li / ex) % ey, lz = li / (ex * ey);
+            const int i = T.b0[0] + lx, j = T.b0[1] + ly, k = T.b0[2] + lz;
+            if (a.sub && sub_shell(i, j, k, a.nx, a.ny, a.nz)) continue;
+            const float2 x = acc[c][lx + kOF * (ly + kOF * lz)];
+            const float tw = 1e-5f + x.x;
+            const size_t o = lin3(i, j, k, fx, fy);
+            a.coef[c][o] = tw;
+            a.field[c][o] = x.y / tw;
+        }
     }
 }
 
@@ -707,7 +959,7 @@ int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* 
               float* v_coef, float* w, float* w_coef) {
     VF_ARG(P && np >= 0 && cell_start && (np == 0 || (pos && vel && order)), "p2g args");
     VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
-    VF_ARG(method == 0 || method == 1, "method");
+    VF_ARG(method == 0 || method == 1 || method == 2, "method");
     VF_ARG((!u) == (!u_coef) && (!v) == (!v_coef) && (!w) == (!w_coef), "field and coef come in pairs");
     hipStream_t s = P->stream;
     const int nx = P->nx, ny = P->ny, nz = P->nz;
@@ -716,6 +968,10 @@ int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* 
     P2GArgs a{pos, vel, cell_start, order, nx, ny, nz, P->h, sub, sub ? bx : 0.f, sub ? by : 0.f, sub ? bz : 0.f,
               {u, v, w}, {u_coef, v_coef, w_coef}};
     if (method == 0) {
+        const int tiles_x = (nx + kOT - 1) / kOT, tiles_y = (ny + kOT - 1) / kOT, tiles_z = (nz + kOT - 1) / kOT;
+        const int ntiles = tiles_x * tiles_y * tiles_z;
+        k_p2g_owner<<<ntiles, kBlock, 0, s>>>(a, tiles_x, tiles_y, ntiles);
+    } else if (method == 2) {
         for (int c = 0; c < 3; ++c) {
             if (!a.field[c]) continue;
             const size_t n = (size_t)face_dim(c, 0, nx, ny, nz) * face_dim(c, 1, nx, ny, nz) * face_dim(c, 2, nx, ny, nz);
@@ -735,7 +991,7 @@ int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* 
         for (int c = 0; c < 3; ++c) {
             if (!a.field[c]) continue;
             const int fx = face_dim(c, 0, nx, ny, nz), fy = face_dim(c, 1, nx, ny, nz), fz = face_dim(c, 2, nx, ny, nz);
-            k_p2g_gather<<<dim3((fx + 63) / 64, (fy + 3) / 4, fz), dim3(64, 4), 0, s>>>(a, c, method == 0 ? 1 : 0);
+            k_p2g_gather<<<dim3((fx + 63) / 64, (fy + 3) / 4, fz), dim3(64, 4), 0, s>>>(a, c, method == 1 ? 0 : 1);
         }
     }
     VF_CHECK_LAUNCH();
