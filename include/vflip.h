@@ -281,6 +281,38 @@ int vflip_advect_particles(int ni, int nj, int nk, float h, float dt, int varian
                            int np, float* pos, const float* u, const float* v, const float* w,
                            const float* solid_phi, void* hip_stream);
 
+/* ------------------------------------------------- coarse/fine coupling */
+/* FineToCoarseOperator — S/coarse_fine_solver.cpp:73-108: coarse(i,j,k) = mean of
+ * the fine cells [i*ratio - offx, i*ratio + ratio - offx) x ... clipped to the
+ * fine grid (0 if none); FineToCoarse (110-118) uses off = ratio/2 along a MAC
+ * face's normal. Bit-exact. Asynchronous on hip_stream. */
+int vflip_fine_to_coarse(const float* fine, int fnx, int fny, int fnz, float* coarse, int cnx, int cny, int cnz,
+                         int ratio, int offx, int offy, int offz, void* hip_stream);
+
+/* CoarseToFineOperator — S/coarse_fine_solver.cpp:146-170: fine(i,j,k) =
+ * coarse.sample_linear(((i,j,k) - (fox,foy,foz))*hf) with the coarse grid's
+ * offsets and spacing hc. With fine_minus != NULL it writes
+ * fine_minus - sample instead (ComputeSmallDifference, 47-71). Bit-exact.
+ * Asynchronous on hip_stream. */
+int vflip_coarse_to_fine(float* fine, int fnx, int fny, int fnz, float fox, float foy, float foz, float hf,
+                         const float* coarse, int cnx, int cny, int cnz, float cox, float coy, float coz, float hc,
+                         const float* fine_minus, void* hip_stream);
+
+/* SetFaceBoundary — S/coarse_fine_solver.cpp:182-230: for each coarse cell, the
+ * fine samples [i*ratio, i*ratio + offx) x [j*ratio, j*ratio + offy) x ...
+ * (the fine faces under one coarse face: extent 1 along its normal) get
+ * coarse - sum/(ratio*ratio) added. Requires 0 <= off <= ratio. Bit-exact.
+ * Asynchronous on hip_stream. */
+int vflip_set_face_boundary(float* fine, int fnx, int fny, int fnz, const float* coarse, int cnx, int cny, int cnz,
+                            int ratio, int offx, int offy, int offz, void* hip_stream);
+
+/* Buffer3D storage — S/fluid_buffer3D.h:186-202: 8^3 bricks, element
+ * ((K*bx*by + J*bx + I) << 9) + (kk << 6) + (jj << 3) + ii. to_bricks writes
+ * bx*by*bz*512 floats (padding 0); from_bricks reads them back to the linear
+ * nx*ny*nz layout of this ABI. Asynchronous on hip_stream. */
+int vflip_to_bricks(const float* src, int nx, int ny, int nz, float* dst, void* hip_stream);
+int vflip_from_bricks(const float* src, int nx, int ny, int nz, float* dst, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,13 @@ SIGNATURES = {
     "vflip_g2p": (c_int, [c_int, c_int, c_int, c_float, c_int, P, P] + [P] * 6 + [c_float] * 4 + [P]),
     "vflip_advect_particles": (c_int, [c_int, c_int, c_int, c_float, c_float, c_int, c_float, c_float, c_float,
                                        c_int, P, P, P, P, P, P]),
+    "vflip_fine_to_coarse": (c_int, [P, c_int, c_int, c_int, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, P]),
+    "vflip_coarse_to_fine": (c_int, [P, c_int, c_int, c_int, c_float, c_float, c_float, c_float, P, c_int, c_int,
+                                     c_int, c_float, c_float, c_float, c_float, P, P]),
+    "vflip_set_face_boundary": (c_int, [P, c_int, c_int, c_int, P, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                        P]),
+    "vflip_to_bricks": (c_int, [P, c_int, c_int, c_int, P, P]),
+    "vflip_from_bricks": (c_int, [P, c_int, c_int, c_int, P, P]),
 }
 
 _lib = None

@@ -271,3 +271,40 @@ def advect_particles(h, dt, pos, u, v, w, solid_phi, variant=COARSE, bmin=(0.0, 
     check(lib().vflip_advect_particles(ni1 - 1, nj, nk, h, dt, variant, *[float(b) for b in bmin], npart,
                                        _ptr(pos) if npart else None, _ptr(u), _ptr(v), _ptr(w), _ptr(solid_phi),
                                        _stream()), "advect_particles")
+
+
+# ------------------------------------------------------ coarse/fine coupling
+def fine_to_coarse(fine, coarse, ratio, offs=(0, 0, 0)):
+    fz, fy, fx = fine.shape
+    cz, cy, cx = coarse.shape
+    check(lib().vflip_fine_to_coarse(_ptr(fine), fx, fy, fz, _ptr(coarse), cx, cy, cz, ratio, *offs, _stream()),
+          "fine_to_coarse")
+
+
+def coarse_to_fine(fine, f_offs, hf, coarse, c_offs, hc, fine_minus=None):
+    """fine = coarse sampled at fine positions; with fine_minus: fine = fine_minus - sample."""
+    fz, fy, fx = fine.shape
+    cz, cy, cx = coarse.shape
+    check(lib().vflip_coarse_to_fine(_ptr(fine), fx, fy, fz, *f_offs, hf, _ptr(coarse), cx, cy, cz, *c_offs, hc,
+                                     _ptr(fine_minus, numel=fine.numel()), _stream()), "coarse_to_fine")
+
+
+def set_face_boundary(fine, coarse, ratio, offs):
+    fz, fy, fx = fine.shape
+    cz, cy, cx = coarse.shape
+    check(lib().vflip_set_face_boundary(_ptr(fine), fx, fy, fz, _ptr(coarse), cx, cy, cz, ratio, *offs, _stream()),
+          "set_face_boundary")
+
+
+def to_bricks(src):
+    nz, ny, nx = src.shape
+    dst = torch.empty(((nx + 7) // 8) * ((ny + 7) // 8) * ((nz + 7) // 8) * 512, device=src.device)
+    check(lib().vflip_to_bricks(_ptr(src), nx, ny, nz, _ptr(dst), _stream()), "to_bricks")
+    return dst
+
+
+def from_bricks(src, shape):
+    nz, ny, nx = shape
+    dst = torch.empty(shape, device=src.device)
+    check(lib().vflip_from_bricks(_ptr(src), nx, ny, nz, _ptr(dst), _stream()), "from_bricks")
+    return dst
