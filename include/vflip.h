@@ -313,6 +313,36 @@ int vflip_set_face_boundary(float* fine, int fnx, int fny, int fnz, const float*
 int vflip_to_bricks(const float* src, int nx, int ny, int nz, float* dst, void* hip_stream);
 int vflip_from_bricks(const float* src, int nx, int ny, int nz, float* dst, void* hip_stream);
 
+/* ------------------------------------------------------ learned warm start */
+/* No counterpart in the reference: a 3D U-Net (4 levels, 16/32/64/128 channels,
+ * 3x3x3 convolutions, BF16 weights/activations, FP32 accumulation on the matrix
+ * cores) that maps (rhs * in_scale, solid mask, liquid mask) to a pressure
+ * guess for vflip_solver_solve's warm_start; MGPCG still runs to the requested
+ * tolerance. Grids: ni, nj, nk positive multiples of 8, cell arrays linear
+ * x-fastest (device). Parameters are fp32 [weights of every layer][biases]
+ * (vflip_unet_num_params floats, layout in vflip_amd/unet.py). */
+typedef struct vflip_unet vflip_unet;
+int vflip_unet_create(unsigned seed, vflip_unet** out);
+int vflip_unet_destroy(vflip_unet* net);
+int vflip_unet_set_stream(vflip_unet* net, void* hip_stream);
+size_t vflip_unet_num_params(const vflip_unet* net);
+int vflip_unet_get_params(const vflip_unet* net, float* dst);
+int vflip_unet_set_params(vflip_unet* net, const float* src);
+int vflip_unet_get_grads(const vflip_unet* net, float* dst);
+int vflip_unet_zero_grad(vflip_unet* net);
+/* pred[v] = network output; solid/liquid may be NULL (0 / 1). Asynchronous. */
+int vflip_unet_forward(vflip_unet* net, int ni, int nj, int nk, const float* rhs, const float* solid,
+                       const float* liquid, float in_scale, float* pred);
+/* Adds d(loss)/d(params) for grad_pred = d(loss)/d(pred) of the last forward. */
+int vflip_unet_backward(vflip_unet* net, const float* grad_pred);
+/* Adam (beta 0.9/0.999, eps 1e-8) on the fp32 master copy with the gradients
+ * scaled by grad_scale; step counts from 1. */
+int vflip_unet_adam(vflip_unet* net, float lr, int step, float grad_scale);
+/* Forward, loss = 0.5 * mean((pred - target)^2), backward (gradients add up);
+ * *loss (host, may be NULL; non-NULL synchronises). */
+int vflip_unet_train_step(vflip_unet* net, int ni, int nj, int nk, const float* rhs, const float* solid,
+                          const float* liquid, float in_scale, const float* target, double* loss);
+
 #ifdef __cplusplus
 }
 #endif

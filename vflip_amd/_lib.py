@@ -93,6 +93,18 @@ SIGNATURES = {
                                         P]),
     "vflip_to_bricks": (c_int, [P, c_int, c_int, c_int, P, P]),
     "vflip_from_bricks": (c_int, [P, c_int, c_int, c_int, P, P]),
+    "vflip_unet_create": (c_int, [ctypes.c_uint, ctypes.POINTER(c_void_p)]),
+    "vflip_unet_destroy": (c_int, [P]),
+    "vflip_unet_set_stream": (c_int, [P, P]),
+    "vflip_unet_num_params": (ctypes.c_size_t, [P]),
+    "vflip_unet_get_params": (c_int, [P, P]),
+    "vflip_unet_set_params": (c_int, [P, P]),
+    "vflip_unet_get_grads": (c_int, [P, P]),
+    "vflip_unet_zero_grad": (c_int, [P]),
+    "vflip_unet_forward": (c_int, [P, c_int, c_int, c_int, P, P, P, c_float, P]),
+    "vflip_unet_backward": (c_int, [P, P]),
+    "vflip_unet_adam": (c_int, [P, c_float, c_int, c_float]),
+    "vflip_unet_train_step": (c_int, [P, c_int, c_int, c_int, P, P, P, c_float, P, PD]),
 }
 
 _lib = None

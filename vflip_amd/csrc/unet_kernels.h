@@ -1,0 +1,404 @@
+// Kernels of the learned warm start: a 3D U-Net over channels-last BF16
+// activations with FP32 accumulation on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16). Internal header of unet.hip.
+//
+// Layouts
+//   activation  [z][y][x][C]  bf16, C padded to a multiple of 16 (pad = 0)
+//   weights     [Cout][Kw]    bf16, k = tap*Cin + ci, tap = (dz+1)*9 + (dy+1)*3 + (dx+1);
+//                             Kw = 27*Cin (28*Cin when Cin == 16: a zero tap pads K to 32s)
+//   gradients   fp32, same shapes as what they differentiate
+#pragma once
+
+#include "common.h"
+
+namespace vflip {
+namespace unet {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kT = 8;            // output tile edge (voxels)
+constexpr int kH = kT + 2;       // halo tile edge
+constexpr int kHV = kH * kH * kH;
+constexpr int kConvThreads = 512;  // 8 waves, wave w owns z-slice w of the tile
+
+// One input of a convolution: channels [c_lo, c_lo + C) of the (virtually
+// concatenated) input come from x, read at (v >> up) on a grid of dims >> up.
+struct ConvSrc {
+    const bf16* x;
+    int C;   // channels stored per voxel in x (padded)
+    int up;  // 1: x lives on the half-resolution grid (nearest upsampling)
+};
+
+struct ConvArgs {
+    ConvSrc src[2];
+    int nsrc;
+    int nx, ny, nz;  // output grid (and full-resolution input grid)
+    int cin;         // total input channels (sum of src C)
+    int cout;        // output channels (multiple of 16)
+    int kw;          // weight row length
+    const bf16* w;   // [cout][kw]
+    const float* bias;
+    int relu;
+    bf16* out;       // [V][cout] bf16 (if not null)
+    float* out32;    // [V][cout] fp32 (if not null), or [V] channel 0 only when out32_ch0
+    int out32_ch0;
+    int tiles_x, tiles_y, ntiles;
+};
+
+__device__ inline int tap_offset_hv(int tap) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    return (dz * kH + dy) * kH + dx;
+}
+
+// Stages the halo tile of input channels [c0, c0 + CK) into LDS as
+// [halo voxel][CK + 8] bf16 (the 8-element pad keeps 16-lane row reads free of
+// bank conflicts). Voxels outside the grid read as zero (zero padding).
+template <int CK>
+__device__ inline void stage_halo(const ConvArgs& a, int c0, int x0, int y0, int z0, bf16* lds) {
+    int s = 0, cl = c0;
+    if (a.nsrc > 1 && c0 >= a.src[0].C) {
+        s = 1;
+        cl = c0 - a.src[0].C;
+    }
+    const ConvSrc src = a.src[s];
+    const int sx = a.nx >> src.up, sy = a.ny >> src.up;
+    constexpr int P = CK / 8;  // 16-byte parts per voxel
+    for (int e = threadIdx.x; e < kHV * P; e += kConvThreads) {
+        const int hv = e / P, part = e % P;
+        const int hx = hv % kH, hy = (hv / kH) % kH, hz = hv / (kH * kH);
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+        bf16x8 val;
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.nx && gy < a.ny && gz < a.nz) {
+            const size_t v = ((size_t)(gz >> src.up) * sy + (gy >> src.up)) * sx + (gx >> src.up);
+            val = *reinterpret_cast<const bf16x8*>(src.x + v * src.C + cl + part * 8);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) val[q] = (bf16)0.f;
+        }
+        *reinterpret_cast<bf16x8*>(lds + hv * (CK + 8) + part * 8) = val;
+    }
+}
+
+// 3x3x3 convolution, implicit GEMM: M = 512 voxels of an 8^3 tile, N = cout,
+// K = 27 * cin. Each K step of 32 is one tap x 32 channels (CK = 32) or two
+// taps x 16 channels (CK = 16). NB = cout / 16.
+template <int CK, int NB>
+__global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) bf16 lds[kHV * (CK + 8)];
+    const int L = blockIdx.x;
+    int t = L;
+    if (a.ntiles % 8 == 0) t = (L % 8) * (a.ntiles / 8) + L / 8;
+    const int x0 = (t % a.tiles_x) * kT, y0 = ((t / a.tiles_x) % a.tiles_y) * kT, z0 = (t / (a.tiles_x * a.tiles_y)) * kT;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kg = lane >> 4;  // MFMA row/col within a block, k group of 8
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // halo voxel of this lane's output row in each M block (tap 0 offset added per tap)
+    int hv0[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        const int m = mb * 16 + r16;
+        hv0[mb] = ((w + 1) * kH + (m / 8 + 1)) * kH + (m % 8 + 1);
+    }
+    for (int c0 = 0; c0 < a.cin; c0 += CK) {
+        __syncthreads();
+        stage_halo<CK>(a, c0, x0, y0, z0, lds);
+        __syncthreads();
+        constexpr int STEPS = CK == 32 ? 27 : 14;
+        for (int st = 0; st < STEPS; ++st) {
+            int tap, coff;
+            if (CK == 32) {
+                tap = st;
+                coff = kg * 8;
+            } else {
+                tap = 2 * st + (kg >> 1);  // tap 27 is the zero pad of the weights
+                coff = (kg & 1) * 8;
+            }
+            const int dhv = tap_offset_hv(tap < 27 ? tap : 0);
+            const size_t kidx = (size_t)tap * a.cin + c0 + coff;
+            bf16x8 bfrag[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb)
+                bfrag[nb] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(nb * 16 + r16) * a.kw + kidx);
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const bf16x8 afrag = *reinterpret_cast<const bf16x8*>(lds + (hv0[mb] + dhv) * (CK + 8) + coff);
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag[nb], acc[mb][nb], 0, 0, 0);
+            }
+        }
+    }
+    // epilogue: D reg r of lane -> voxel row kg*4 + r of the M block, channel nb*16 + r16
+    const int z = z0 + w;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = mb * 16 + kg * 4 + r;
+            const int x = x0 + m % 8, y = y0 + m / 8;
+            if (x >= a.nx || y >= a.ny || z >= a.nz) continue;
+            const size_t v = ((size_t)z * a.ny + y) * a.nx + x;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                const int co = nb * 16 + r16;
+                float val = acc[mb][nb][r];
+                if (a.bias) val += a.bias[co];
+                if (a.relu) val = val > 0.f ? val : 0.f;
+                if (a.out) a.out[v * a.cout + co] = (bf16)val;
+                if (a.out32) {
+                    if (!a.out32_ch0) a.out32[v * a.cout + co] = val;
+                    else if (co == 0) a.out32[v] = val;
+                }
+            }
+        }
+}
+
+// Weight gradient dW[co][tap*cin + ci] += sum_v g[v][co] * x[v + tap][ci]:
+// MFMA with M = 16 output channels, N = 16 input channels of one tap, K = 32
+// voxels per step. A workgroup (4 waves) owns one (tap, ci block) and a chunk of
+// voxels; each wave reduces a quarter of it, the waves' partials are summed in
+// LDS and added to dW with one atomic per entry.
+struct WgradArgs {
+    ConvSrc src[2];
+    int nsrc;
+    int nx, ny, nz;
+    int cin, cout, kw;
+    const bf16* g;  // [V][cout] gradient at the convolution output (after the ReLU mask)
+    float* dw;      // [cout][kw]
+    int chunk;      // voxels per workgroup (multiple of 128)
+};
+
+template <int NCB>
+__global__ __launch_bounds__(256) void k_conv3_wgrad(WgradArgs a) {
+    __shared__ float red[4][NCB][16][16];
+    const int ncib = a.cin / 16;
+    const int tap = blockIdx.x / ncib, cib = blockIdx.x % ncib;
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    int ci0 = cib * 16, s = 0;
+    if (a.nsrc > 1 && ci0 >= a.src[0].C) {
+        s = 1;
+        ci0 -= a.src[0].C;
+    }
+    const ConvSrc src = a.src[s];
+    const int sx = a.nx >> src.up, sy = a.ny >> src.up;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kg = lane >> 4;
+    const size_t V = (size_t)a.nx * a.ny * a.nz;
+    const size_t v_begin = (size_t)blockIdx.y * a.chunk;
+    const size_t v_end = min(V, v_begin + a.chunk);
+    f32x4 acc[NCB];
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (size_t vb = v_begin + (size_t)w * 32; vb < v_end; vb += 128) {
+        bf16x8 bfrag;  // B[k = voxel][n = ci]: x at (voxel + tap), channel ci0 + r16
+        bf16x8 afrag[NCB];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const size_t v = vb + kg * 8 + q;
+            bf16 xv = (bf16)0.f;
+            if (v < v_end) {
+                const int x = (int)(v % a.nx), y = (int)((v / a.nx) % a.ny), z = (int)(v / ((size_t)a.nx * a.ny));
+                const int gx = x + dx, gy = y + dy, gz = z + dz;
+                if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.nx && gy < a.ny && gz < a.nz) {
+                    const size_t sv = ((size_t)(gz >> src.up) * sy + (gy >> src.up)) * sx + (gx >> src.up);
+                    xv = src.x[sv * src.C + ci0 + r16];
+                }
+            }
+            bfrag[q] = xv;
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) afrag[cb][q] = v < v_end ? a.g[v * a.cout + cb * 16 + r16] : (bf16)0.f;
+        }
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+            acc[cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag[cb], bfrag, acc[cb], 0, 0, 0);
+    }
+    // D: row (co within block) kg*4 + r, column (ci within block) r16
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[w][cb][kg * 4 + r][r16] = acc[cb][r];
+    __syncthreads();
+    for (int e = threadIdx.x; e < NCB * 256; e += 256) {
+        const int cb = e / 256, row = (e / 16) % 16, col = e % 16;
+        const float sum = red[0][cb][row][col] + red[1][cb][row][col] + red[2][cb][row][col] + red[3][cb][row][col];
+        if (sum != 0.f)
+            unsafeAtomicAdd(&a.dw[(size_t)(cb * 16 + row) * a.kw + (size_t)tap * a.cin + cib * 16 + col], sum);
+    }
+}
+
+// db[co] += sum over voxels of g[v][co] (g bf16 after the ReLU mask).
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16* __restrict__ g, size_t V, int cout, float* __restrict__ db) {
+    __shared__ float part[256];
+    const int co = threadIdx.x % cout, lanes = 256 / cout * cout;
+    float s = 0.f;
+    if (threadIdx.x < lanes)
+        for (size_t v = blockIdx.x * (size_t)(256 / cout) + threadIdx.x / cout; v < V; v += (size_t)gridDim.x * (256 / cout))
+            s += (float)g[v * cout + co];
+    part[threadIdx.x] = threadIdx.x < lanes ? s : 0.f;
+    __syncthreads();
+    if (threadIdx.x < cout) {
+        float t = 0.f;
+        for (int q = threadIdx.x; q < lanes; q += cout) t += part[q];
+        unsafeAtomicAdd(&db[threadIdx.x], t);
+    }
+}
+
+// 2x2x2 average pooling, channels-last bf16 (fine dims even).
+__global__ __launch_bounds__(256) void k_avgpool(const bf16* __restrict__ x, int nx, int ny, int nz, int C,
+                                                 bf16* __restrict__ y) {
+    const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+    const size_t n = (size_t)cx * cy * cz * C;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const int c = (int)(e % C);
+        const size_t v = e / C;
+        const int i = (int)(v % cx), j = (int)((v / cx) % cy), k = (int)(v / ((size_t)cx * cy));
+        float s = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const size_t f = ((size_t)(2 * k + (q >> 2)) * ny + (2 * j + ((q >> 1) & 1))) * nx + (2 * i + (q & 1));
+            s += (float)x[f * C + c];
+        }
+        y[e] = (bf16)(0.125f * s);
+    }
+}
+
+// gx_fine[v][c] += 0.125 * gy_coarse[v/2][c] (fp32).
+__global__ __launch_bounds__(256) void k_avgpool_bwd(const float* __restrict__ gy, int nx, int ny, int nz, int C,
+                                                     float* __restrict__ gx) {
+    const size_t n = (size_t)nx * ny * nz * C;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const int c = (int)(e % C);
+        const size_t v = e / C;
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((size_t)nx * ny));
+        const size_t cv = ((size_t)(k / 2) * (ny / 2) + j / 2) * (nx / 2) + i / 2;
+        gx[e] += 0.125f * gy[cv * C + c];
+    }
+}
+
+// Splits the gradient of a virtual concat [up(coarse) C0 | skip C1] (fp32
+// [V][C0+C1]): the upsampled part is summed over each 2x2x2 block into the
+// coarse gradient (set), the skip part is added to the skip gradient.
+__global__ __launch_bounds__(256) void k_concat_bwd_up(const float* __restrict__ g, int nx, int ny, int nz, int C0,
+                                                       int C1, float* __restrict__ gcoarse) {
+    const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
+    const size_t n = (size_t)cx * cy * cz * C0;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const int c = (int)(e % C0);
+        const size_t v = e / C0;
+        const int i = (int)(v % cx), j = (int)((v / cx) % cy), k = (int)(v / ((size_t)cx * cy));
+        float s = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const size_t f = ((size_t)(2 * k + (q >> 2)) * ny + (2 * j + ((q >> 1) & 1))) * nx + (2 * i + (q & 1));
+            s += g[f * (C0 + C1) + c];
+        }
+        gcoarse[e] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_concat_bwd_skip(const float* __restrict__ g, size_t V, int C0, int C1,
+                                                         float* __restrict__ gskip) {
+    const size_t n = V * C1;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const size_t v = e / C1;
+        gskip[e] += g[v * (C0 + C1) + C0 + (e % C1)];
+    }
+}
+
+// g_pre = g * (y > 0) as bf16 (y: the layer's ReLU output); relu = 0 passes g.
+__global__ __launch_bounds__(256) void k_relu_bwd(const float* __restrict__ g, const bf16* __restrict__ y, size_t n,
+                                                  int relu, bf16* __restrict__ gp) {
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const float v = g[e];
+        gp[e] = (bf16)((!relu || (float)y[e] > 0.f) ? v : 0.f);
+    }
+}
+
+// Network input: channels 0..2 = rhs * inv_scale, solid, liquid; 3..15 = 0.
+__global__ __launch_bounds__(256) void k_pack_input(const float* __restrict__ rhs, const float* __restrict__ solid,
+                                                    const float* __restrict__ liquid, size_t V, float inv_scale,
+                                                    bf16* __restrict__ x) {
+    for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        bf16x8 lo, hi;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lo[q] = hi[q] = (bf16)0.f;
+        lo[0] = (bf16)(rhs[v] * inv_scale);
+        lo[1] = (bf16)(solid ? solid[v] : 0.f);
+        lo[2] = (bf16)(liquid ? liquid[v] : 1.f);
+        *reinterpret_cast<bf16x8*>(x + v * 16) = lo;
+        *reinterpret_cast<bf16x8*>(x + v * 16 + 8) = hi;
+    }
+}
+
+// MSE loss 0.5/V * sum (y - t)^2 (y: the prediction, [V]); its gradient is
+// written to g[V] and the loss partials are added to *loss.
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t, size_t V,
+                                             float* __restrict__ g, double* __restrict__ loss) {
+    double part = 0.0;
+    const float inv = 1.0f / (float)V;
+    for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const float d = y[v] - t[v];
+        part += 0.5 * (double)d * d / (double)V;
+        g[v] = d * inv;
+    }
+    part = block_sum(part);
+    if (threadIdx.x == 0) atomicAdd(loss, part);
+}
+
+// Gradient of a single-channel output [V] spread to the padded [V][16] layout
+// as bf16 (channels 1..15 zero): the output layer has no ReLU.
+__global__ __launch_bounds__(256) void k_grad_ch0(const float* __restrict__ g, size_t V, bf16* __restrict__ gp) {
+    for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        bf16x8 lo, hi;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lo[q] = hi[q] = (bf16)0.f;
+        lo[0] = (bf16)g[v];
+        *reinterpret_cast<bf16x8*>(gp + v * 16) = lo;
+        *reinterpret_cast<bf16x8*>(gp + v * 16 + 8) = hi;
+    }
+}
+
+// Adam on the fp32 master copy; refreshes the bf16 forward weights.
+__global__ __launch_bounds__(256) void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                                              float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                                              float c1, float c2, float gscale, bf16* __restrict__ pb) {
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const float gr = g[e] * gscale;
+        const float mm = b1 * m[e] + (1.f - b1) * gr;
+        const float vv = b2 * v[e] + (1.f - b2) * gr * gr;
+        m[e] = mm;
+        v[e] = vv;
+        const float upd = lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+        const float np = p[e] - upd;
+        p[e] = np;
+        if (pb) pb[e] = (bf16)np;
+    }
+}
+
+// Backward-data weights: wb[ci][tap' * cout + co] = w[co][tap * cin + ci] with
+// tap' = 26 - tap (the transposed convolution is a convolution with the
+// spatially flipped, channel-transposed kernel).
+__global__ __launch_bounds__(256) void k_flip_weights(const float* __restrict__ w, int cin, int cout, int kw, int kwb,
+                                                      bf16* __restrict__ wb) {
+    const size_t n = (size_t)cin * kwb;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const int ci = (int)(e / kwb), kb = (int)(e % kwb);
+        const int tb = kb / cout, co = kb % cout;
+        float val = 0.f;
+        if (tb < 27) val = w[(size_t)co * kw + (size_t)(26 - tb) * cin + ci];
+        wb[e] = (bf16)val;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ a, size_t n, bf16* __restrict__ b) {
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) b[e] = (bf16)a[e];
+}
+
+}  // namespace unet
+}  // namespace vflip
