@@ -36,7 +36,7 @@ def log(*a):
 
 
 def make_scene(n, seed=0):
-    from tests import scenes
+    from vflip_amd import scenes
     h = 1.0 / n
     spheres = scenes.random_spheres(seed)
     phi = scenes.solid_phi(n, h, spheres)

@@ -326,6 +326,10 @@ int vflip_unet_create(unsigned seed, vflip_unet** out);
 int vflip_unet_destroy(vflip_unet* net);
 int vflip_unet_set_stream(vflip_unet* net, void* hip_stream);
 size_t vflip_unet_num_params(const vflip_unet* net);
+/* Pressure unit of the automatic normalisation (below): pressure / (max|rhs| * gain)
+ * is what the network predicts. Set from the training data; default 1. */
+int vflip_unet_set_gain(vflip_unet* net, float gain);
+float vflip_unet_gain(const vflip_unet* net);
 int vflip_unet_get_params(const vflip_unet* net, float* dst);
 int vflip_unet_set_params(vflip_unet* net, const float* src);
 int vflip_unet_get_grads(const vflip_unet* net, float* dst);
@@ -339,9 +343,28 @@ int vflip_unet_backward(vflip_unet* net, const float* grad_pred);
  * scaled by grad_scale; step counts from 1. */
 int vflip_unet_adam(vflip_unet* net, float lr, int step, float grad_scale);
 /* Forward, loss = 0.5 * mean((pred - target)^2), backward (gradients add up);
- * *loss (host, may be NULL; non-NULL synchronises). */
+ * *loss (host, may be NULL; non-NULL synchronises). in_scale > 0: input
+ * rhs * in_scale, raw output; in_scale == 0: automatic normalisation by
+ * s = max|rhs| (computed on the device), the target taken as
+ * pressure / (s * gain) and vflip_unet_forward's output returned as pressure
+ * (times s * gain). */
 int vflip_unet_train_step(vflip_unet* net, int ni, int nj, int nk, const float* rhs, const float* solid,
                           const float* liquid, float in_scale, const float* target, double* loss);
+
+/* Pressure guess for a solve on an ni x nj x nk grid: rhs averaged by `factor`
+ * per axis onto the network grid (solid fraction from diag == 0), the network
+ * in automatic normalisation, the result prolonged trilinearly into x0 (0 on
+ * cells with diag == 0). Asynchronous on the network's stream. */
+int vflip_unet_warm_start(vflip_unet* net, int ni, int nj, int nk, int factor, const float* rhs, const float* diag,
+                          float* x0);
+
+/* vflip_project with the network's guess as the PCG's initial pressure
+ * (warm_start for this solve); MGPCG runs to the solver's tolerance as
+ * always. factor: solve grid / network grid per axis. */
+int vflip_project_learned(vflip_solver* S, vflip_unet* net, int factor, int variant, float h, float dt, float* u,
+                          float* v, float* w, const float* uw, const float* vw, const float* ww, const float* us,
+                          const float* vs, const float* ws, float* rhs_scratch, float* pressure,
+                          vflip_solve_stats* stats);
 
 #ifdef __cplusplus
 }

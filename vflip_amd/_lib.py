@@ -97,6 +97,8 @@ SIGNATURES = {
     "vflip_unet_destroy": (c_int, [P]),
     "vflip_unet_set_stream": (c_int, [P, P]),
     "vflip_unet_num_params": (ctypes.c_size_t, [P]),
+    "vflip_unet_set_gain": (c_int, [P, c_float]),
+    "vflip_unet_gain": (c_float, [P]),
     "vflip_unet_get_params": (c_int, [P, P]),
     "vflip_unet_set_params": (c_int, [P, P]),
     "vflip_unet_get_grads": (c_int, [P, P]),
@@ -105,6 +107,9 @@ SIGNATURES = {
     "vflip_unet_backward": (c_int, [P, P]),
     "vflip_unet_adam": (c_int, [P, c_float, c_int, c_float]),
     "vflip_unet_train_step": (c_int, [P, c_int, c_int, c_int, P, P, P, c_float, P, PD]),
+    "vflip_unet_warm_start": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P]),
+    "vflip_project_learned": (c_int, [P, P, c_int, c_int, c_float, c_float] + [P] * 11 +
+                              [ctypes.POINTER(SolveStats)]),
 }
 
 _lib = None

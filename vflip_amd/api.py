@@ -130,6 +130,17 @@ class Solver:
         check(lib().vflip_assemble(self.h, variant, h, dt, *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
                                    _ptr(rhs, numel=self.n)), "assemble")
 
+    def project_learned(self, net, factor, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
+        """project() with the U-Net's pressure guess as the PCG's initial iterate."""
+        st = SolveStats()
+        rc = check(lib().vflip_project_learned(self.h, net.h, factor, variant, h, dt,
+                                               *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
+                                               _ptr(rhs, numel=self.n), _ptr(p, numel=self.n), ctypes.byref(st)),
+                   "project_learned")
+        d = st.as_dict()
+        d["status"] = rc
+        return d
+
     def project(self, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
         st = SolveStats()
         rc = check(lib().vflip_project(self.h, variant, h, dt,

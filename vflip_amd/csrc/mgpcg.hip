@@ -946,6 +946,12 @@ void solver_mark_f64_assembled(vflip_solver* S) {
     S->b64_valid = true;
 }
 bool solver_is_mixed(const vflip_solver* S) { return S->opts.precision == VFLIP_PREC_MIXED; }
+// Returns the previous warm_start option and sets it to `on`.
+int solver_set_warm_start(vflip_solver* S, int on) {
+    const int old = S->opts.warm_start;
+    S->opts.warm_start = on;
+    return old;
+}
 // Solve the assembled system: FP32 rhs, plus the FP64 rhs in mixed mode.
 int solver_solve_assembled(vflip_solver* S, const float* rhs32, float* x32, vflip_solve_stats* st) {
     const double* b64 = (S->opts.precision == VFLIP_PREC_MIXED && S->b64_valid) ? S->b64 : nullptr;

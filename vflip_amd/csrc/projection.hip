@@ -297,6 +297,7 @@ int* solver_begin_assembly(vflip_solver* s);
 int solver_f64(vflip_solver* s, double** d64, double** b64, double** x64);
 void solver_mark_f64_assembled(vflip_solver* s);
 bool solver_is_mixed(const vflip_solver* s);
+int solver_set_warm_start(vflip_solver* s, int on);
 int solver_solve_assembled(vflip_solver* s, const float* rhs32, float* x32, vflip_solve_stats* st);
 }  // namespace vflip
 
@@ -325,15 +326,31 @@ int vflip_assemble(vflip_solver* S, int variant, float h, float dt, const float*
     return 0;
 }
 
-int vflip_project(vflip_solver* S, int variant, float h, float dt, float* u, float* v, float* w, const float* uw,
-                  const float* vw, const float* ww, const float* us, const float* vs, const float* ws,
-                  float* rhs_scratch, float* pressure, vflip_solve_stats* stats) {
+namespace {
+int project_impl(vflip_solver* S, vflip_unet* net, int factor, int variant, float h, float dt, float* u, float* v,
+                 float* w, const float* uw, const float* vw, const float* ww, const float* us, const float* vs,
+                 const float* ws, float* rhs_scratch, float* pressure, vflip_solve_stats* stats) {
     VF_ARG(S && pressure && rhs_scratch, "solver/pressure/rhs");
     int rc = vflip_assemble(S, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs_scratch);
     if (rc) return rc;
     rc = vflip_solver_setup(S);
     if (rc) return rc;
-    int status = solver_solve_assembled(S, rhs_scratch, pressure, stats);
+    int status;
+    if (net) {
+        // the network's pressure guess seeds the PCG, which still runs to the tolerance
+        int ni, nj, nk;
+        float *d, *ax, *ay, *az, *lk;
+        hipStream_t s;
+        rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s);
+        if (rc) return rc;
+        if ((rc = vflip_unet_set_stream(net, s))) return rc;
+        if ((rc = vflip_unet_warm_start(net, ni, nj, nk, factor, rhs_scratch, d, pressure))) return rc;
+        const int old = solver_set_warm_start(S, 1);
+        status = solver_solve_assembled(S, rhs_scratch, pressure, stats);
+        solver_set_warm_start(S, old);
+    } else {
+        status = solver_solve_assembled(S, rhs_scratch, pressure, stats);
+    }
     if (status < 0 && status != VFLIP_ERR_BREAKDOWN) return status;
     int ni, nj, nk;
     float *d, *ax, *ay, *az, *lk;
@@ -351,6 +368,21 @@ int vflip_project(vflip_solver* S, int variant, float h, float dt, float* u, flo
     }
     if (rc) return rc;
     return status;
+}
+}  // namespace
+
+int vflip_project(vflip_solver* S, int variant, float h, float dt, float* u, float* v, float* w, const float* uw,
+                  const float* vw, const float* ww, const float* us, const float* vs, const float* ws,
+                  float* rhs_scratch, float* pressure, vflip_solve_stats* stats) {
+    return project_impl(S, nullptr, 1, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs_scratch, pressure, stats);
+}
+
+int vflip_project_learned(vflip_solver* S, vflip_unet* net, int factor, int variant, float h, float dt, float* u,
+                          float* v, float* w, const float* uw, const float* vw, const float* ww, const float* us,
+                          const float* vs, const float* ws, float* rhs_scratch, float* pressure,
+                          vflip_solve_stats* stats) {
+    VF_ARG(net, "network");
+    return project_impl(S, net, factor, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs_scratch, pressure, stats);
 }
 
 }  // extern "C"

@@ -95,7 +95,12 @@ struct vflip_unet {
     float* gx = nullptr;
     float* gpred = nullptr;
     double* loss = nullptr;
+    float* scale = nullptr;  // device: max|rhs| of the current input (automatic normalisation)
+    float gain = 1.0f;       // pressure unit of the automatic normalisation: max|rhs| * gain
     std::vector<void*> grid_allocs;
+    // warm-start workspace on the network grid
+    int wx = 0, wy = 0, wz = 0;
+    float *w_rhs = nullptr, *w_solid = nullptr, *w_liquid = nullptr, *w_pred = nullptr;
 };
 
 namespace {
@@ -327,6 +332,34 @@ struct Rng {
 
 bool dims_ok(int nx, int ny, int nz) { return nx > 0 && ny > 0 && nz > 0 && nx % 8 == 0 && ny % 8 == 0 && nz % 8 == 0; }
 
+// in_scale > 0: the input is rhs * in_scale and the output the raw network
+// value; in_scale == 0: automatic, s = max|rhs| on the device, input rhs / s,
+// output (and training target) in units of s. Returns the device scale or null.
+const float* prepare_scale(vflip_unet* N, const float* rhs, size_t V, float in_scale) {
+    if (in_scale > 0.f) return nullptr;
+    k_zero1<<<1, 1, 0, N->stream>>>(N->scale);
+    k_absmax_to<<<grid256(V), 256, 0, N->stream>>>(rhs, V, N->scale);
+    return N->scale;
+}
+
+int ensure_warm_grid(vflip_unet* N, int cx, int cy, int cz) {
+    if (N->wx == cx && N->wy == cy && N->wz == cz) return 0;
+    for (float* p : {N->w_rhs, N->w_solid, N->w_liquid, N->w_pred})
+        if (p) (void)hipFree(p);
+    N->w_rhs = N->w_solid = N->w_liquid = N->w_pred = nullptr;
+    N->wx = N->wy = N->wz = 0;
+    const size_t n = (size_t)cx * cy * cz;
+    if (hipMalloc(&N->w_rhs, n * 4) != hipSuccess || hipMalloc(&N->w_solid, n * 4) != hipSuccess ||
+        hipMalloc(&N->w_liquid, n * 4) != hipSuccess || hipMalloc(&N->w_pred, n * 4) != hipSuccess) {
+        set_error("warm-start workspace: out of device memory");
+        return VFLIP_ERR_NOMEM;
+    }
+    N->wx = cx;
+    N->wy = cy;
+    N->wz = cz;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -349,7 +382,8 @@ int vflip_unet_create(unsigned seed, vflip_unet** out) {
     const size_t np = nw + nb;
     if (hipMalloc(&N->params, np * 4) != hipSuccess || hipMalloc(&N->grads, np * 4) != hipSuccess ||
         hipMalloc(&N->adam_m, np * 4) != hipSuccess || hipMalloc(&N->adam_v, np * 4) != hipSuccess ||
-        hipMalloc(&N->w16, nw * 2) != hipSuccess || hipMalloc(&N->wb16, nwb * 2) != hipSuccess) {
+        hipMalloc(&N->w16, nw * 2) != hipSuccess || hipMalloc(&N->wb16, nwb * 2) != hipSuccess ||
+        hipMalloc(&N->scale, sizeof(float)) != hipSuccess) {
         delete N;
         set_error("U-Net parameters: out of device memory");
         return VFLIP_ERR_NOMEM;
@@ -384,7 +418,9 @@ int vflip_unet_create(unsigned seed, vflip_unet** out) {
 int vflip_unet_destroy(vflip_unet* N) {
     if (!N) return 0;
     free_grid(N);
-    for (void* p : {(void*)N->params, (void*)N->grads, (void*)N->adam_m, (void*)N->adam_v, (void*)N->w16, (void*)N->wb16})
+    for (void* p : {(void*)N->params, (void*)N->grads, (void*)N->adam_m, (void*)N->adam_v, (void*)N->w16,
+                    (void*)N->wb16, (void*)N->scale, (void*)N->w_rhs, (void*)N->w_solid, (void*)N->w_liquid,
+                    (void*)N->w_pred})
         if (p) (void)hipFree(p);
     delete N;
     return 0;
@@ -397,6 +433,14 @@ int vflip_unet_set_stream(vflip_unet* N, void* hip_stream) {
 }
 
 size_t vflip_unet_num_params(const vflip_unet* N) { return N ? N->nw + N->nb : 0; }
+
+int vflip_unet_set_gain(vflip_unet* N, float gain) {
+    VF_ARG(N && gain > 0.f, "gain");
+    N->gain = gain;
+    return 0;
+}
+
+float vflip_unet_gain(const vflip_unet* N) { return N ? N->gain : 0.f; }
 
 int vflip_unet_get_params(const vflip_unet* N, float* dst) {
     VF_ARG(N && dst, "args");
@@ -434,9 +478,10 @@ int vflip_unet_forward(vflip_unet* N, int ni, int nj, int nk, const float* rhs, 
     VF_ARG(dims_ok(ni, nj, nk), "grid dims must be positive multiples of 8");
     if (int rc = ensure_grid(N, ni, nj, nk)) return rc;
     const size_t V = (size_t)ni * nj * nk;
-    k_pack_input<<<grid256(V), 256, 0, N->stream>>>(rhs, solid, liquid, V, in_scale, N->act[kIn]);
+    const float* sd = prepare_scale(N, rhs, V, in_scale);
+    k_pack_input<<<grid256(V), 256, 0, N->stream>>>(rhs, solid, liquid, V, in_scale, sd, N->act[kIn]);
     forward_layers(N);
-    VF_HIP(hipMemcpyAsync(pred, N->pred, V * 4, hipMemcpyDeviceToDevice, N->stream));
+    k_scale_copy<<<grid256(V), 256, 0, N->stream>>>(N->pred, V, sd, N->gain, pred);
     VF_CHECK_LAUNCH();
     return 0;
 }
@@ -470,16 +515,41 @@ int vflip_unet_train_step(vflip_unet* N, int ni, int nj, int nk, const float* rh
     VF_ARG(dims_ok(ni, nj, nk), "grid dims must be positive multiples of 8");
     if (int rc = ensure_grid(N, ni, nj, nk)) return rc;
     const size_t V = (size_t)ni * nj * nk;
-    k_pack_input<<<grid256(V), 256, 0, N->stream>>>(rhs, solid, liquid, V, in_scale, N->act[kIn]);
+    const float* sd = prepare_scale(N, rhs, V, in_scale);
+    k_pack_input<<<grid256(V), 256, 0, N->stream>>>(rhs, solid, liquid, V, in_scale, sd, N->act[kIn]);
     forward_layers(N);
     VF_HIP(hipMemsetAsync(N->loss, 0, sizeof(double), N->stream));
-    k_mse<<<grid256(V), 256, 0, N->stream>>>(N->pred, target, V, N->gpred, N->loss);
+    k_mse<<<grid256(V), 256, 0, N->stream>>>(N->pred, target, V, sd, N->gain, N->gpred, N->loss);
     backward_layers(N);
     VF_CHECK_LAUNCH();
     if (loss) {
         VF_HIP(hipMemcpyAsync(loss, N->loss, sizeof(double), hipMemcpyDeviceToHost, N->stream));
         VF_HIP(hipStreamSynchronize(N->stream));
     }
+    return 0;
+}
+
+// Warm start for a solve on an ni x nj x nk grid: inputs restricted by
+// `factor` to the network grid, network in automatic normalisation, pressure
+// prolonged back (trilinear) into x0. diag: the solve's level-0 diagonal
+// (0 marks solid and Dirichlet cells; x0 is 0 there). Asynchronous.
+int vflip_unet_warm_start(vflip_unet* N, int ni, int nj, int nk, int factor, const float* rhs, const float* diag,
+                          float* x0) {
+    VF_ARG(N && rhs && x0 && factor >= 1, "warm_start args");
+    VF_ARG(ni % factor == 0 && nj % factor == 0 && nk % factor == 0, "grid dims divisible by factor");
+    const int cx = ni / factor, cy = nj / factor, cz = nk / factor;
+    VF_ARG(dims_ok(cx, cy, cz), "network grid dims must be positive multiples of 8");
+    if (int rc = ensure_warm_grid(N, cx, cy, cz)) return rc;
+    if (int rc = ensure_grid(N, cx, cy, cz)) return rc;
+    const size_t Vc = (size_t)cx * cy * cz;
+    k_restrict_inputs<<<grid256(Vc), 256, 0, N->stream>>>(rhs, diag, ni, nj, nk, factor, N->w_rhs, N->w_solid,
+                                                          N->w_liquid);
+    const float* sd = prepare_scale(N, N->w_rhs, Vc, 0.f);
+    k_pack_input<<<grid256(Vc), 256, 0, N->stream>>>(N->w_rhs, N->w_solid, N->w_liquid, Vc, 0.f, sd, N->act[kIn]);
+    forward_layers(N);
+    const size_t V = (size_t)ni * nj * nk;
+    k_prolong_cells<<<grid256(V), 256, 0, N->stream>>>(N->pred, cx, cy, cz, factor, sd, N->gain, diag, x0);
+    VF_CHECK_LAUNCH();
     return 0;
 }
 

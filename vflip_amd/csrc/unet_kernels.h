@@ -320,10 +320,30 @@ __global__ __launch_bounds__(256) void k_relu_bwd(const float* __restrict__ g, c
     }
 }
 
+// Scale of the automatic normalisation: s = max|rhs| on the device (no host
+// round trip); the network sees rhs / s and predicts pressure / s.
+__device__ inline float inv_of(const float* s_dev, float explicit_inv) {
+    if (!s_dev) return explicit_inv;
+    const float s = s_dev[0];
+    return s > 0.f ? 1.0f / s : 0.f;
+}
+
+// *out = max |x| (out zeroed beforehand); non-negative floats order as their bits.
+__global__ __launch_bounds__(256) void k_absmax_to(const float* __restrict__ x, size_t n, float* __restrict__ out) {
+    float m = 0.f;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) m = fmaxf(m, fabsf(x[e]));
+    m = block_max(m);
+    if (threadIdx.x == 0) atomicMax(reinterpret_cast<int*>(out), __float_as_int(m));
+}
+
+__global__ void k_zero1(float* a) { a[0] = 0.f; }
+
 // Network input: channels 0..2 = rhs * inv_scale, solid, liquid; 3..15 = 0.
+// inv_scale = 1/s_dev[0] when s_dev is given.
 __global__ __launch_bounds__(256) void k_pack_input(const float* __restrict__ rhs, const float* __restrict__ solid,
-                                                    const float* __restrict__ liquid, size_t V, float inv_scale,
-                                                    bf16* __restrict__ x) {
+                                                    const float* __restrict__ liquid, size_t V, float inv_scale_arg,
+                                                    const float* __restrict__ s_dev, bf16* __restrict__ x) {
+    const float inv_scale = inv_of(s_dev, inv_scale_arg);
     for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
         bf16x8 lo, hi;
 #pragma unroll
@@ -338,12 +358,15 @@ __global__ __launch_bounds__(256) void k_pack_input(const float* __restrict__ rh
 
 // MSE loss 0.5/V * sum (y - t)^2 (y: the prediction, [V]); its gradient is
 // written to g[V] and the loss partials are added to *loss.
+// The target is scaled by 1/(s_dev[0] * gain) when s_dev is given (normalised pressure).
 __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const float* __restrict__ t, size_t V,
-                                             float* __restrict__ g, double* __restrict__ loss) {
+                                             const float* __restrict__ s_dev, float gain, float* __restrict__ g,
+                                             double* __restrict__ loss) {
     double part = 0.0;
     const float inv = 1.0f / (float)V;
+    const float tm = inv_of(s_dev, 1.0f) / (s_dev ? gain : 1.0f);
     for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
-        const float d = y[v] - t[v];
+        const float d = y[v] - t[v] * tm;
         part += 0.5 * (double)d * d / (double)V;
         g[v] = d * inv;
     }
@@ -393,6 +416,70 @@ __global__ __launch_bounds__(256) void k_flip_weights(const float* __restrict__ 
         float val = 0.f;
         if (tb < 27) val = w[(size_t)co * kw + (size_t)(26 - tb) * cin + ci];
         wb[e] = (bf16)val;
+    }
+}
+
+// out = y * s_dev[0] * gain (or y when s_dev is null).
+__global__ __launch_bounds__(256) void k_scale_copy(const float* __restrict__ y, size_t n, const float* __restrict__ s_dev,
+                                                    float gain, float* __restrict__ out) {
+    const float m = s_dev ? s_dev[0] * gain : 1.0f;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) out[e] = y[e] * m;
+}
+
+// Solve grid -> network grid by an integer factor f: rhs averaged over each
+// f^3 block; solid = fraction of cells outside the system (diagonal 0, the
+// solid and Dirichlet cells); liquid = 1 - solid.
+__global__ __launch_bounds__(256) void k_restrict_inputs(const float* __restrict__ rhs, const float* __restrict__ diag,
+                                                         int nx, int ny, int nz, int f, float* __restrict__ crhs,
+                                                         float* __restrict__ csolid, float* __restrict__ cliquid) {
+    const int cx = nx / f, cy = ny / f, cz = nz / f;
+    const size_t n = (size_t)cx * cy * cz;
+    const float inv = 1.0f / (float)(f * f * f);
+    for (size_t c = blockIdx.x * (size_t)256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
+        const int i = (int)(c % cx), j = (int)((c / cx) % cy), k = (int)(c / ((size_t)cx * cy));
+        float sr = 0.f, ss = 0.f;
+        for (int dk = 0; dk < f; ++dk)
+            for (int dj = 0; dj < f; ++dj)
+                for (int di = 0; di < f; ++di) {
+                    const size_t v = ((size_t)(k * f + dk) * ny + (j * f + dj)) * nx + (i * f + di);
+                    sr += rhs[v];
+                    ss += (diag && diag[v] == 0.f) ? 1.f : 0.f;
+                }
+        crhs[c] = sr * inv;
+        csolid[c] = ss * inv;
+        cliquid[c] = 1.f - ss * inv;
+    }
+}
+
+// Network grid -> solve grid: trilinear interpolation of cell-centred values
+// (fine cell i sits at coarse coordinate (i + 0.5)/f - 0.5, clamped), times
+// s_dev[0] * gain; cells outside the system (diag 0) get 0.
+__global__ __launch_bounds__(256) void k_prolong_cells(const float* __restrict__ cp, int cx, int cy, int cz, int f,
+                                                       const float* __restrict__ s_dev, float gain,
+                                                       const float* __restrict__ diag, float* __restrict__ x) {
+    const int nx = cx * f, ny = cy * f, nz = cz * f;
+    const size_t n = (size_t)nx * ny * nz;
+    const float m = s_dev ? s_dev[0] * gain : 1.0f;
+    const float invf = 1.0f / (float)f;
+    for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < n; v += (size_t)gridDim.x * 256) {
+        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((size_t)nx * ny));
+        if (diag && diag[v] == 0.f) {
+            x[v] = 0.f;
+            continue;
+        }
+        const float gx = fminf(fmaxf((i + 0.5f) * invf - 0.5f, 0.f), (float)(cx - 1));
+        const float gy = fminf(fmaxf((j + 0.5f) * invf - 0.5f, 0.f), (float)(cy - 1));
+        const float gz = fminf(fmaxf((k + 0.5f) * invf - 0.5f, 0.f), (float)(cz - 1));
+        const int i0 = (int)gx, j0 = (int)gy, k0 = (int)gz;
+        const int i1 = min(i0 + 1, cx - 1), j1 = min(j0 + 1, cy - 1), k1 = min(k0 + 1, cz - 1);
+        const float fx = gx - i0, fy = gy - j0, fz = gz - k0;
+        auto at = [&](int a, int b, int e) { return cp[((size_t)e * cy + b) * cx + a]; };
+        const float c00 = at(i0, j0, k0) + fx * (at(i1, j0, k0) - at(i0, j0, k0));
+        const float c10 = at(i0, j1, k0) + fx * (at(i1, j1, k0) - at(i0, j1, k0));
+        const float c01 = at(i0, j0, k1) + fx * (at(i1, j0, k1) - at(i0, j0, k1));
+        const float c11 = at(i0, j1, k1) + fx * (at(i1, j1, k1) - at(i0, j1, k1));
+        const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+        x[v] = (c0 + fz * (c1 - c0)) * m;
     }
 }
 

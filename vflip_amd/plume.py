@@ -43,6 +43,11 @@ class PlumeSim:
         self.jitter, self.alpha, self.beta, self.seed = jitter, alpha, beta, seed
         self.solver = api.Solver(n, n, n, precision=precision, tolerance=tol, max_iterations=maxit)
         self.last_stats = None
+        self.net, self.net_factor = None, 1
+
+    def use_network(self, net, factor):
+        """Learned warm start: the U-Net (on the grid coarsened by factor) seeds every pressure solve."""
+        self.net, self.net_factor = net, factor
 
     def advect(self):
         h, dt = float(self.h), float(self.dt)
@@ -63,8 +68,12 @@ class PlumeSim:
                         self.temp, self.v)
 
     def project(self):
-        self.last_stats = self.solver.project(api.COARSE, float(self.h), 1.0, self.u, self.v, self.w, self.uw,
-                                              self.vw, self.ww, self.us, self.vs, self.ws, self.rhs, self.p)
+        args = (api.COARSE, float(self.h), 1.0, self.u, self.v, self.w, self.uw, self.vw, self.ww, self.us, self.vs,
+                self.ws, self.rhs, self.p)
+        if self.net is not None:
+            self.last_stats = self.solver.project_learned(self.net, self.net_factor, *args)
+        else:
+            self.last_stats = self.solver.project(*args)
         if self.last_stats["status"] < 0:
             raise RuntimeError(f"projection failed: {self.last_stats}")
         return self.last_stats

@@ -132,3 +132,35 @@ class UNet:
 
     def adam(self, lr, step, grad_scale=1.0):
         check(lib().vflip_unet_adam(self.h, lr, step, grad_scale), "unet_adam")
+
+    def warm_start(self, rhs, diag, factor=1, out=None):
+        """Pressure guess on rhs's grid (network run on the grid coarsened by factor)."""
+        nk, nj, ni = rhs.shape
+        out = torch.empty_like(rhs) if out is None else out
+        check(lib().vflip_unet_warm_start(self.h, ni, nj, nk, factor, _ptr(rhs), _ptr(diag, numel=rhs.numel()),
+                                          _ptr(out, numel=rhs.numel())), "unet_warm_start")
+        return out
+
+    @property
+    def gain(self):
+        return float(lib().vflip_unet_gain(self.h))
+
+    @gain.setter
+    def gain(self, g):
+        check(lib().vflip_unet_set_gain(self.h, float(g)), "unet_set_gain")
+
+    def save(self, path):
+        """Parameters (float16; the kernels use bf16 copies) and the pressure gain to an .npz."""
+        import numpy as np
+        np.savez_compressed(path, params=self.params().cpu().numpy().astype(np.float16),
+                            gain=np.array([self.gain], np.float32))
+
+    def load(self, path):
+        import numpy as np
+        with np.load(path, allow_pickle=False) as z:
+            flat = torch.from_numpy(z["params"].astype(np.float32))
+            gain = float(z["gain"][0])
+        if flat.numel() != self.n:
+            raise ValueError(f"{path}: {flat.numel()} parameters, the network has {self.n}")
+        self.set_params(flat)
+        self.gain = gain
