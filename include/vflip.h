@@ -74,6 +74,8 @@ typedef struct {
     double true_residual; /* |b - A x|_inf / |b|_inf recomputed in FP64 on the GPU */
     double ms_setup;      /* hierarchy build (coarsening kernels) */
     double ms_solve;      /* PCG loop */
+    int warm_rejected;    /* warm_start only: 1 when the guess was no better than zero in the
+                             A-norm (E(x0) = x0'Ax0/2 - b'x0 >= 0) and the solve restarted from 0 */
 } vflip_solve_stats;
 
 /* Reference defaults: tol 1e-6, 50 iterations, FP32, 4/4/200 sweeps, refine 8. */
@@ -350,6 +352,14 @@ int vflip_unet_adam(vflip_unet* net, float lr, int step, float grad_scale);
  * (times s * gain). */
 int vflip_unet_train_step(vflip_unet* net, int ni, int nj, int nk, const float* rhs, const float* solid,
                           const float* liquid, float in_scale, const float* target, double* loss);
+
+/* Energy-norm training step (automatic normalisation): loss =
+ * |pred - x|_A^2 / |x|_A^2 with x = target / (max|rhs| * gain) and A the sample's
+ * 7-point operator (diag, couplings to +x/+y/+z as vflip_solver_set_matrix):
+ * the error norm that governs the warm-started PCG. Gradients add up. */
+int vflip_unet_train_step_energy(vflip_unet* net, int ni, int nj, int nk, const float* rhs, const float* solid,
+                                 const float* liquid, const float* target, const float* diag, const float* ax,
+                                 const float* ay, const float* az, double* loss);
 
 /* Pressure guess for a solve on an ni x nj x nk grid: rhs averaged by `factor`
  * per axis onto the network grid (solid fraction from diag == 0), the network

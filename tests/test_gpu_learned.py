@@ -1,0 +1,124 @@
+"""The learned warm start end to end on the GPU.
+
+* The energy-norm training loss and its gradients against the CPU reference
+  (oracle/unet_ref.py plus a torch 7-point stencil): relative L2 <= 5e-2 per
+  layer (bf16 output gradients on the GPU).
+* MGPCG judges every guess: a guess that is no better than zero in the A-norm
+  (E(x0) >= 0) is dropped and the solve restarts from zero, converging as a
+  zero-guess solve does.
+* What the committed network (vflip_amd/data/unet_plume64.npz, trained by
+  vflip_amd/train_warm_start.py on 2000 pairs of the 64^3 plume workload) gains
+  over the same solve without it, on held-out scenes: fewer iterations to the
+  same 1e-6 FP64 true residual.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+WEIGHTS = os.path.join(ROOT, "vflip_amd", "data", "unet_plume64.npz")
+
+
+def _stencil(x, d, ax, ay, az):
+    """y = A x for the 7-point operator (diag, couplings to +x/+y/+z), (z, y, x) arrays."""
+    y = d * x
+    y[:, :, :-1] += ax[:, :, :-1] * x[:, :, 1:]
+    y[:, :, 1:] += ax[:, :, :-1] * x[:, :, :-1]
+    y[:, :-1, :] += ay[:, :-1, :] * x[:, 1:, :]
+    y[:, 1:, :] += ay[:, :-1, :] * x[:, :-1, :]
+    y[:-1, :, :] += az[:-1, :, :] * x[1:, :, :]
+    y[1:, :, :] += az[:-1, :, :] * x[:-1, :, :]
+    return y
+
+
+def test_energy_loss_gradients_match_reference(cuda):
+    print("energy loss parity", flush=True)
+    import torch
+    from oracle import unet_ref
+    from vflip_amd import api, learned
+    from vflip_amd.unet import UNet, param_layout
+    n = 16
+    phi, centre, radius, steps = learned.scene(7, n)
+    from vflip_amd.plume import PlumeSim
+    sim = PlumeSim(n, phi, source_center_cfg=centre, source_r=radius, seed=7, dt=0.02)
+    for _ in range(3):
+        sim.step()
+    sim.advect()
+    sim.force()
+    exact = api.Solver(n, n, n, precision=api.MIXED, tolerance=1e-6, max_iterations=100)
+    rhs = torch.zeros((n, n, n), device="cuda")
+    p = torch.zeros_like(rhs)
+    exact.project(api.COARSE, float(sim.h), 1.0, sim.u, sim.v, sim.w, sim.uw, sim.vw, sim.ww, sim.us, sim.vs,
+                  sim.ws, rhs, p)
+    d, ax, ay, az = exact.get_level(0)
+    solid = (d == 0).float()
+    net = UNet(seed=11)
+    net.gain = 1e-3
+    net.zero_grad()
+    loss = net.train_step_energy(rhs, solid, 1.0 - solid, p, d, ax, ay, az)
+    g = net.grads().cpu()
+    # reference: same normalisation (s = max|rhs|, pressure unit s * gain)
+    flat = net.params().cpu().clone().requires_grad_(True)
+    r_c, s_c, p_c = rhs.cpu(), solid.cpu(), p.cpu()
+    A = [t.cpu() for t in (d, ax, ay, az)]
+    s = float(r_c.abs().max())
+    pred = unet_ref.forward(flat, r_c, s_c, 1.0 - s_c, 1.0 / s)
+    x = p_c / (s * 1e-3)
+    e = pred - x
+    ref_loss = (e * _stencil(e, *A)).sum() / (x * _stencil(x, *A)).sum()
+    ref_loss.backward()
+    assert abs(loss - float(ref_loss)) <= 2e-2 * float(ref_loss)
+    offs, boffs, _ = param_layout()
+    for l, ((wo, cout, kw, cin), bo) in enumerate(zip(offs, boffs)):
+        gw, rw = g[wo:wo + cout * kw], flat.grad[wo:wo + cout * kw]
+        assert float((gw - rw).norm() / rw.norm()) <= 5e-2, f"layer {l}"
+
+
+def test_bad_guess_is_rejected_and_solve_converges(cuda):
+    print("warm-start judge", flush=True)
+    import torch
+    from vflip_amd import api, scenes
+    n = 32
+    pp = scenes.projection_problem(n, seed=4)
+    u, v, w, us, vs, ws = (pp[k] for k in ("u", "v", "w", "us", "vs", "ws"))
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    uw, vw, ww = api.face_weights(t(pp["phi"]))
+    for prec in (api.FP32, api.MIXED):
+        S = api.Solver(n, n, n, precision=prec, tolerance=1e-6, max_iterations=100)
+        rhs = torch.zeros((n, n, n), device="cuda")
+        S.assemble(api.COARSE, 1.0 / n, 1.0, t(u), t(v), t(w), uw, vw, ww, t(us), t(vs), t(ws), rhs)
+        x0 = torch.zeros_like(rhs)
+        st0 = S.solve(rhs, x0)
+        # a guess far from the solution: the solution with a large random perturbation
+        bad = x0 + 50.0 * x0.abs().max() * torch.randn_like(x0)
+        S.set_opts(warm_start=1)
+        st1 = S.solve(rhs, bad)
+        S.set_opts(warm_start=0)
+        assert st1["warm_rejected"] == 1 and st1["converged"] == 1
+        assert st1["iterations"] == st0["iterations"]
+        # a good guess (the solution itself, slightly perturbed) is kept and converges faster
+        good = x0 * (1 + 1e-3 * torch.randn_like(x0))
+        S.set_opts(warm_start=1)
+        st2 = S.solve(rhs, good)
+        S.set_opts(warm_start=0)
+        assert st2["warm_rejected"] == 0 and st2["converged"] == 1 and st2["iterations"] < st0["iterations"]
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="trained weights not present")
+def test_trained_network_saves_iterations_at_64(cuda):
+    print("learned gain 64^3", flush=True)
+    from vflip_amd import learned
+    from vflip_amd.unet import UNet
+    net = UNet()
+    net.load(WEIGHTS)
+    rows = learned.evaluate(net, list(range(3001, 3009)), n=64, factor=1)
+    plain = np.mean([r["its_plain"] for r in rows])
+    warm = np.mean([r["its_learned"] for r in rows])
+    print(f"iterations: plain {plain:.2f}, learned {warm:.2f}; guess rel err "
+          f"{np.mean([r['guess_rel_err'] for r in rows]):.3f}", flush=True)
+    assert all(r["true_res_learned"] <= 1e-6 for r in rows)
+    assert warm <= plain - 0.5
+    assert np.mean([r["guess_rel_err"] for r in rows]) < 0.5

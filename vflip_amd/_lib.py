@@ -37,6 +37,7 @@ class SolveStats(ctypes.Structure):
         ("true_residual", c_double),
         ("ms_setup", c_double),
         ("ms_solve", c_double),
+        ("warm_rejected", c_int),
     ]
 
     def as_dict(self):
@@ -108,6 +109,7 @@ SIGNATURES = {
     "vflip_unet_adam": (c_int, [P, c_float, c_int, c_float]),
     "vflip_unet_train_step": (c_int, [P, c_int, c_int, c_int, P, P, P, c_float, P, PD]),
     "vflip_unet_warm_start": (c_int, [P, c_int, c_int, c_int, c_int, P, P, P]),
+    "vflip_unet_train_step_energy": (c_int, [P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, PD]),
     "vflip_project_learned": (c_int, [P, P, c_int, c_int, c_float, c_float] + [P] * 11 +
                               [ctypes.POINTER(SolveStats)]),
 }

@@ -565,7 +565,20 @@ constexpr int kMatvecKZ = 8;
 struct Scalars {
     double rho, rho_new, sigma, alpha, beta, res, bnorm, tol_abs, true_res;
     int breakdown;
+    double energy;  // E(x0) = x0'A x0 / 2 - b'x0 of a warm start
 };
+
+// Partial sums of x.(b + r) for the energy of an initial guess:
+// E(x) = x'Ax/2 - b'x = -(x.b + x.r)/2 with r = b - A x.
+__global__ __launch_bounds__(kBlock) void k_energy_dots(unsigned n, const double* __restrict__ x,
+                                                       const double* __restrict__ b64, const float* __restrict__ b32,
+                                                       const float* __restrict__ r, double* __restrict__ partial) {
+    double s = 0;
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock)
+        s += x[c] * ((b64 ? b64[c] : (double)b32[c]) + (double)r[c]);
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
 
 // q = A s and sum(s.q); 3D launch (64 x 4 threads, kMatvecKZ planes per block).
 template <typename CT, typename Acc>
@@ -775,6 +788,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Scalars* sc, const double* 
         sc->bnorm = v;
         sc->tol_abs = tol * v;
         sc->res = v;
+    } else if (what == 6) {
+        sc->energy = -0.5 * v;
     } else {
         sc->true_res = v;
     }

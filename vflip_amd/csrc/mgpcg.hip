@@ -724,6 +724,13 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
         launch_true_residual64(S, mx ? rhs64 : nullptr, rhs32, S->x64, S->r);
         VF_CHECK_LAUNCH();
         if ((rc = finalize(S, 2))) return rc;
+        // The PCG judges the guess: it minimises E(x) = x'Ax/2 - b'x, and
+        // E(x0) < E(0) = 0 exactly when x0 is closer to the solution than zero
+        // in the A-norm. A guess that is not is dropped (restart from zero).
+        k_energy_dots<<<kVecBlocks, kBlock, 0, S->stream>>>(n, S->x64, mx ? rhs64 : nullptr, rhs32, S->r, S->partial);
+        S->np_last = kVecBlocks;
+        VF_CHECK_LAUNCH();
+        if ((rc = finalize(S, 6))) return rc;
     } else {
         if (mx) k_fill64<<<gn, kBlock, 0, S->stream>>>(n, S->x64, 0.0);
         else k_fill<<<gn, kBlock, 0, S->stream>>>(n, S->xw, 0.f);
@@ -733,6 +740,14 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
     if ((rc = fetch_scalars(S))) return rc;
     vflip_solve_stats stats{};
     stats.ms_setup = ms_setup;
+    if (S->opts.warm_start && !(S->h_sc->energy < 0.0)) {
+        if (mx) k_fill64<<<gn, kBlock, 0, S->stream>>>(n, S->x64, 0.0);
+        else k_fill<<<gn, kBlock, 0, S->stream>>>(n, S->xw, 0.f);
+        k_copy<<<gn, kBlock, 0, S->stream>>>(n, S->r, rhs32);
+        VF_CHECK_LAUNCH();
+        S->h_sc->res = S->h_sc->bnorm;
+        stats.warm_rejected = 1;
+    }
     const double bnorm = S->h_sc->bnorm;
     const double tol_abs = S->h_sc->tol_abs;
     double res = S->h_sc->res;

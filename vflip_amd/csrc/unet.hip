@@ -95,6 +95,7 @@ struct vflip_unet {
     float* gx = nullptr;
     float* gpred = nullptr;
     double* loss = nullptr;
+    double* eacc = nullptr;  // device: e'Ae, x'Ax of the energy loss
     float* scale = nullptr;  // device: max|rhs| of the current input (automatic normalisation)
     float gain = 1.0f;       // pressure unit of the automatic normalisation: max|rhs| * gain
     std::vector<void*> grid_allocs;
@@ -383,7 +384,7 @@ int vflip_unet_create(unsigned seed, vflip_unet** out) {
     if (hipMalloc(&N->params, np * 4) != hipSuccess || hipMalloc(&N->grads, np * 4) != hipSuccess ||
         hipMalloc(&N->adam_m, np * 4) != hipSuccess || hipMalloc(&N->adam_v, np * 4) != hipSuccess ||
         hipMalloc(&N->w16, nw * 2) != hipSuccess || hipMalloc(&N->wb16, nwb * 2) != hipSuccess ||
-        hipMalloc(&N->scale, sizeof(float)) != hipSuccess) {
+        hipMalloc(&N->scale, sizeof(float)) != hipSuccess || hipMalloc(&N->eacc, 2 * sizeof(double)) != hipSuccess) {
         delete N;
         set_error("U-Net parameters: out of device memory");
         return VFLIP_ERR_NOMEM;
@@ -419,7 +420,7 @@ int vflip_unet_destroy(vflip_unet* N) {
     if (!N) return 0;
     free_grid(N);
     for (void* p : {(void*)N->params, (void*)N->grads, (void*)N->adam_m, (void*)N->adam_v, (void*)N->w16,
-                    (void*)N->wb16, (void*)N->scale, (void*)N->w_rhs, (void*)N->w_solid, (void*)N->w_liquid,
+                    (void*)N->wb16, (void*)N->scale, (void*)N->eacc, (void*)N->w_rhs, (void*)N->w_solid, (void*)N->w_liquid,
                     (void*)N->w_pred})
         if (p) (void)hipFree(p);
     delete N;
@@ -520,6 +521,36 @@ int vflip_unet_train_step(vflip_unet* N, int ni, int nj, int nk, const float* rh
     forward_layers(N);
     VF_HIP(hipMemsetAsync(N->loss, 0, sizeof(double), N->stream));
     k_mse<<<grid256(V), 256, 0, N->stream>>>(N->pred, target, V, sd, N->gain, N->gpred, N->loss);
+    backward_layers(N);
+    VF_CHECK_LAUNCH();
+    if (loss) {
+        VF_HIP(hipMemcpyAsync(loss, N->loss, sizeof(double), hipMemcpyDeviceToHost, N->stream));
+        VF_HIP(hipStreamSynchronize(N->stream));
+    }
+    return 0;
+}
+
+// Energy-norm training step: forward, loss = |pred - p/(s gain)|_A^2 / |p/(s gain)|_A^2
+// with the sample's operator A (automatic normalisation), backward.
+int vflip_unet_train_step_energy(vflip_unet* N, int ni, int nj, int nk, const float* rhs, const float* solid,
+                                 const float* liquid, const float* target, const float* diag, const float* ax,
+                                 const float* ay, const float* az, double* loss) {
+    VF_ARG(N && rhs && target && diag && ax && ay && az, "train args");
+    VF_ARG(dims_ok(ni, nj, nk), "grid dims must be positive multiples of 8");
+    if (int rc = ensure_grid(N, ni, nj, nk)) return rc;
+    const size_t V = (size_t)ni * nj * nk;
+    const float* sd = prepare_scale(N, rhs, V, 0.f);
+    k_pack_input<<<grid256(V), 256, 0, N->stream>>>(rhs, solid, liquid, V, 0.f, sd, N->act[kIn]);
+    forward_layers(N);
+    // scratch: e and x in gx, A e after them (gx holds at least 16 V floats)
+    float* e = N->gx;
+    float* xn = N->gx + V;
+    float* ae = N->gx + 2 * V;
+    const Stencil A{diag, ax, ay, az, ni, nj, nk};
+    k_energy_prep<<<grid256(V), 256, 0, N->stream>>>(N->pred, target, V, sd, N->gain, e, xn, N->eacc);
+    k_stencil_quad<<<grid256(V), 256, 0, N->stream>>>(A, e, ae, N->eacc, 0);
+    k_stencil_quad<<<grid256(V), 256, 0, N->stream>>>(A, xn, nullptr, N->eacc, 1);
+    k_energy_grad<<<grid256(V), 256, 0, N->stream>>>(ae, V, N->eacc, N->gpred, N->loss);
     backward_layers(N);
     VF_CHECK_LAUNCH();
     if (loss) {

@@ -374,6 +374,64 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, const 
     if (threadIdx.x == 0) atomicAdd(loss, part);
 }
 
+// Energy-norm loss of a warm start. With x = target / (s * gain) (normalised
+// pressure) and e = pred - x: loss = e'Ae / x'Ax, gradient 2 Ae / x'Ax, A the
+// sample's 7-point operator (diag and the couplings to +x, +y, +z).
+struct Stencil {
+    const float *d, *ax, *ay, *az;
+    int nx, ny, nz;
+};
+
+__device__ inline float stencil_apply(const Stencil& A, const float* x, size_t c, int i, int j, int k) {
+    const size_t sx = 1, sy = (size_t)A.nx, sz = (size_t)A.nx * A.ny;
+    float y = A.d[c] * x[c];
+    if (i + 1 < A.nx) y += A.ax[c] * x[c + sx];
+    if (i > 0) y += A.ax[c - sx] * x[c - sx];
+    if (j + 1 < A.ny) y += A.ay[c] * x[c + sy];
+    if (j > 0) y += A.ay[c - sy] * x[c - sy];
+    if (k + 1 < A.nz) y += A.az[c] * x[c + sz];
+    if (k > 0) y += A.az[c - sz] * x[c - sz];
+    return y;
+}
+
+// e = pred - target * tm, xn = target * tm (tm = 1/(s * gain))
+__global__ __launch_bounds__(256) void k_energy_prep(const float* __restrict__ pred, const float* __restrict__ t, size_t V,
+                                                     const float* __restrict__ s_dev, float gain, float* __restrict__ e,
+                                                     float* __restrict__ xn, double* __restrict__ acc) {
+    const float tm = inv_of(s_dev, 1.0f) / gain;
+    if (blockIdx.x == 0 && threadIdx.x < 2) acc[threadIdx.x] = 0.0;
+    for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) {
+        const float x = t[v] * tm;
+        xn[v] = x;
+        e[v] = pred[v] - x;
+    }
+}
+
+// acc[slot] += x'Ax; Ax written to y when y is given.
+__global__ __launch_bounds__(256) void k_stencil_quad(Stencil A, const float* __restrict__ x, float* __restrict__ y,
+                                                      double* __restrict__ acc, int slot) {
+    const size_t V = (size_t)A.nx * A.ny * A.nz;
+    double part = 0.0;
+    for (size_t c = blockIdx.x * (size_t)256 + threadIdx.x; c < V; c += (size_t)gridDim.x * 256) {
+        const int i = (int)(c % A.nx), j = (int)((c / A.nx) % A.ny), k = (int)(c / ((size_t)A.nx * A.ny));
+        const float ax = stencil_apply(A, x, c, i, j, k);
+        if (y) y[c] = ax;
+        part += (double)x[c] * ax;
+    }
+    part = block_sum(part);
+    if (threadIdx.x == 0) atomicAdd(&acc[slot], part);
+}
+
+// g = 2 Ae / x'Ax; loss = e'Ae / x'Ax.
+__global__ __launch_bounds__(256) void k_energy_grad(const float* __restrict__ ae, size_t V,
+                                                     const double* __restrict__ acc, float* __restrict__ g,
+                                                     double* __restrict__ loss) {
+    const double den = acc[1] > 0.0 ? acc[1] : 1.0;
+    const float sc = (float)(2.0 / den);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *loss = acc[0] / den;
+    for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < V; v += (size_t)gridDim.x * 256) g[v] = ae[v] * sc;
+}
+
 // Gradient of a single-channel output [V] spread to the padded [V][16] layout
 // as bf16 (channels 1..15 zero): the output layer has no ReLU.
 __global__ __launch_bounds__(256) void k_grad_ch0(const float* __restrict__ g, size_t V, bf16* __restrict__ gp) {

@@ -43,6 +43,7 @@ def generate_pairs(seeds, n=N_TRAIN, log=None):
     rhs = torch.empty((S, n, n, n), **f32)
     solid = torch.empty((S, n, n, n), **f32)
     pres = torch.empty((S, n, n, n), **f32)
+    ops = torch.empty((S, 4, n, n, n), **f32)  # the sample's operator: diag, +x, +y, +z couplings
     stats = []
     exact = api.Solver(n, n, n, precision=api.MIXED, tolerance=1e-6, max_iterations=100)
     t0 = time.time()
@@ -57,16 +58,22 @@ def generate_pairs(seeds, n=N_TRAIN, log=None):
                            sim.ws, rhs[q], pres[q])
         if st["status"] < 0 or not st["converged"]:
             raise RuntimeError(f"seed {seed}: solve did not converge {st}")
-        diag = exact.get_level(0)[0]
-        solid[q] = (diag == 0).float()
+        d, ax, ay, az = exact.get_level(0)
+        solid[q] = (d == 0).float()
+        for c, a in enumerate((d, ax, ay, az)):
+            ops[q, c] = a
         stats.append(st)
         if log and (q + 1) % 200 == 0:
             log(f"generated {q + 1}/{S} pairs in {time.time() - t0:.1f} s")
-    return dict(rhs=rhs, solid=solid, liquid=1.0 - solid, p=pres, stats=stats)
+    return dict(rhs=rhs, solid=solid, liquid=1.0 - solid, p=pres, ops=ops, stats=stats)
 
 
-def train(net, data, epochs=8, batch=8, lr=1e-3, lr_min=2e-5, seed=0, log=None):
-    """Adam over shuffled mini-batches; cosine learning-rate decay. Returns per-epoch mean losses."""
+def train(net, data, epochs=8, batch=8, lr=1e-3, lr_min=2e-5, seed=0, loss="energy", log=None):
+    """Adam over shuffled mini-batches; cosine learning-rate decay. Returns per-epoch mean losses.
+
+    loss "energy": relative A-norm error of the prediction under each sample's
+    operator (the quantity that sets a warm-started PCG's iteration count);
+    "mse": mean squared error of the normalised pressure."""
     S = data["rhs"].shape[0]
     nbatch = S // batch
     total = epochs * nbatch
@@ -80,8 +87,13 @@ def train(net, data, epochs=8, batch=8, lr=1e-3, lr_min=2e-5, seed=0, log=None):
             idx = perm[b * batch:(b + 1) * batch]
             net.zero_grad()
             for q, i in enumerate(idx.tolist()):
-                loss = net.train_step(data["rhs"][i], data["solid"][i], data["liquid"][i], 0.0, data["p"][i])
-                losses.append(loss)
+                if loss == "energy":
+                    o = data["ops"][i]
+                    val = net.train_step_energy(data["rhs"][i], data["solid"][i], data["liquid"][i], data["p"][i],
+                                                o[0], o[1], o[2], o[3])
+                else:
+                    val = net.train_step(data["rhs"][i], data["solid"][i], data["liquid"][i], 0.0, data["p"][i])
+                losses.append(val)
             step += 1
             a = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * (step - 1) / max(total - 1, 1)))
             net.adam(a, step, 1.0 / batch)

@@ -24,8 +24,9 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--eval", type=int, default=16)
-    ap.add_argument("--eval-n", type=int, default=0, help="also evaluate on this grid (multiple of 64)")
+    ap.add_argument("--eval-n", type=int, nargs="*", default=[], help="also evaluate on these grids (multiples of 64)")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--loss", default="energy", choices=["energy", "mse"])
     ap.add_argument("--out", default=DEFAULT_OUT)
     ap.add_argument("--report", default="")
     a = ap.parse_args(argv)
@@ -49,7 +50,7 @@ def main(argv=None):
     net = UNet(seed=a.seed)
     net.gain = gain
     t1 = time.time()
-    hist = learned.train(net, data, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    hist = learned.train(net, data, epochs=a.epochs, batch=a.batch, lr=a.lr, loss=a.loss, log=log)
     t_train = time.time() - t1
     del data
     torch.cuda.empty_cache()
@@ -57,9 +58,10 @@ def main(argv=None):
     net.save(a.out)
     log(f"saved {a.out}")
     held = list(range(a.pairs + 1, a.pairs + 1 + a.eval))
-    report = dict(pairs=a.pairs, epochs=a.epochs, batch=a.batch, lr=a.lr, gain=gain, seconds_generate=t_gen,
+    report = dict(pairs=a.pairs, epochs=a.epochs, batch=a.batch, lr=a.lr, loss=a.loss, gain=gain,
+                  seconds_generate=t_gen,
                   seconds_train=t_train, loss_per_epoch=hist)
-    for n in [64] + ([a.eval_n] if a.eval_n else []):
+    for n in [64] + list(a.eval_n):
         rows = learned.evaluate(net, held, n=n, factor=n // 64)
         mean = lambda k: sum(r[k] for r in rows) / len(rows)
         summary = {k: mean(k) for k in ("its_plain", "its_learned", "ms_plain", "ms_learned", "guess_rel_err")}

@@ -130,6 +130,18 @@ class UNet:
                                           _ptr(target, numel=rhs.numel()), ctypes.byref(loss)), "unet_train_step")
         return loss.value
 
+    def train_step_energy(self, rhs, solid, liquid, target, diag, ax, ay, az):
+        """Forward, relative energy-norm error of the (normalised) prediction under
+        the sample's operator, backward (gradients accumulate). Returns the loss."""
+        nk, nj, ni = rhs.shape
+        n = rhs.numel()
+        loss = ctypes.c_double()
+        check(lib().vflip_unet_train_step_energy(self.h, ni, nj, nk, _ptr(rhs), _ptr(solid, numel=n),
+                                                 _ptr(liquid, numel=n), _ptr(target, numel=n),
+                                                 *[_ptr(a, numel=n) for a in (diag, ax, ay, az)],
+                                                 ctypes.byref(loss)), "unet_train_step_energy")
+        return loss.value
+
     def adam(self, lr, step, grad_scale=1.0):
         check(lib().vflip_unet_adam(self.h, lr, step, grad_scale), "unet_adam")
 
