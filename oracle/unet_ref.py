@@ -5,8 +5,9 @@ by tests/. It restates the network of vflip_amd/unet.py with plain torch ops
 (conv3d with padding 1, 2x2x2 average pooling, nearest upsampling, channel
 concatenation) and rounds to bf16 where the GPU stores bf16: the packed input,
 the weights, every hidden activation and every pooled tensor. Gradients come
-from torch autograd in fp32 (the GPU additionally rounds the per-layer output
-gradients to bf16, so gradient agreement is to bf16 precision).
+from torch autograd in fp32, with each convolution's output gradient rounded to
+bf16 where the GPU rounds it (optional), so the remaining differences are the
+fp32 accumulation orders.
 
 There is no counterpart in the reference repository: the learned part is this
 build's own design.
@@ -21,9 +22,24 @@ def _bf(t):
     return t.to(torch.bfloat16).to(torch.float32)
 
 
-def forward(flat, rhs, solid=None, liquid=None, in_scale=1.0):
+class _GradBF16(torch.autograd.Function):
+    """Identity forward; backward rounds the incoming gradient to bf16 (the GPU
+    stores each convolution's output gradient, after the ReLU mask, as bf16)."""
+
+    @staticmethod
+    def forward(ctx, x):
+        return x.view_as(x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _bf(g)
+
+
+def forward(flat, rhs, solid=None, liquid=None, in_scale=1.0, grad_bf16=True):
     """flat: fp32 parameter vector (the library's layout, requires_grad allowed);
-    rhs/solid/liquid: (nk, nj, ni) fp32 CPU tensors. Returns the prediction (nk, nj, ni)."""
+    rhs/solid/liquid: (nk, nj, ni) fp32 CPU tensors. Returns the prediction (nk, nj, ni).
+    grad_bf16: round each convolution's output gradient to bf16 in the backward
+    pass, where the GPU does."""
     layers = split_params(flat)
     nk, nj, ni = rhs.shape
     x = torch.zeros(1, 16, nk, nj, ni)
@@ -47,6 +63,8 @@ def forward(flat, rhs, solid=None, liquid=None, in_scale=1.0):
         cin = w.shape[2]
         W = _bf(w).reshape(cout, 3, 3, 3, cin).permute(0, 4, 1, 2, 3)
         y = F.conv3d(xin, W, b, padding=1)
+        if grad_bf16:
+            y = _GradBF16.apply(y)
         if l == len(LAYERS) - 1:
             pred = y[0, 0]
         else:

@@ -56,6 +56,8 @@ def test_unet_forward_and_gradients_match_reference(cuda, shape):
     loss = net.train_step(rhs.cuda(), solid.cuda(), liquid.cuda(), 1.0, target.cuda())
     assert abs(loss - loss_ref) <= 2e-2 * loss_ref
     g = net.grads().cpu()
+    print("per-layer relative gradient error:",
+          [round(_rel(g[w0:w1], grad_ref[w0:w1]), 4) for (w0, w1, _, _) in _layer_slices()], flush=True)
     for l, (w0, w1, b0, b1) in enumerate(_layer_slices()):
         assert _rel(g[w0:w1], grad_ref[w0:w1]) <= 5e-2, f"layer {l} weights"
         assert _rel(g[b0:b1], grad_ref[b0:b1]) <= 5e-2 or float(grad_ref[b0:b1].norm()) < 1e-8, f"layer {l} bias"
