@@ -5,6 +5,8 @@
 // cell indices (grids up to 2^31 cells).
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace vflip {
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* 
             float s = 0.f;
             if (MODE != 1) {
 #pragma unroll
-                for (int e = 0; e < 6; ++e) s += rin[r].a[e] * rin[r].x[e];
+                for (int e = 0; e < 6; ++e) s = __builtin_fmaf(rin[r].a[e], rin[r].x[e], s);
             }
             R[rin[r].slot] = gs_value(rin[r].b, s, rin[r].d);
         }
@@ -289,12 +291,12 @@ __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* 
                     // red neighbours outside the grid were never computed: skip them
                     const int i = i0 + it;
                     float sum = 0.f;
-                    if (q > 0) sum += bin[r].a[0] * Rm[jr * RW + ir];
-                    if (j > 0) sum += bin[r].a[1] * R0[(jr - 1) * RW + ir];
-                    if (i > 0) sum += bin[r].a[2] * R0[jr * RW + ir - 1];
-                    if (i + 1 < ni) sum += bin[r].a[3] * R0[jr * RW + ir + 1];
-                    if (j + 1 < nj) sum += bin[r].a[4] * R0[(jr + 1) * RW + ir];
-                    if (q + 1 < nk) sum += bin[r].a[5] * Rp[jr * RW + ir];
+                    if (q > 0) sum = __builtin_fmaf(bin[r].a[0], Rm[jr * RW + ir], sum);
+                    if (j > 0) sum = __builtin_fmaf(bin[r].a[1], R0[(jr - 1) * RW + ir], sum);
+                    if (i > 0) sum = __builtin_fmaf(bin[r].a[2], R0[jr * RW + ir - 1], sum);
+                    if (i + 1 < ni) sum = __builtin_fmaf(bin[r].a[3], R0[jr * RW + ir + 1], sum);
+                    if (j + 1 < nj) sum = __builtin_fmaf(bin[r].a[4], R0[(jr + 1) * RW + ir], sum);
+                    if (q + 1 < nk) sum = __builtin_fmaf(bin[r].a[5], Rp[jr * RW + ir], sum);
                     const float v = gs_value(bin[r].b, sum, bin[r].d);
                     xout[bin[r].c] = v;
                     if (dot) dacc += (Acc)v * (Acc)bin[r].b;
@@ -315,6 +317,280 @@ __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* 
     if (dot) {
         Acc t = block_sum(dacc);
         if (tid == 0) dot[blockIdx.x] = (double)t;
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Row-wave form of the fused sweep, for levels whose rows are 64*V cells
+// (V = 2 or 4). A wave owns one whole x row: lane l holds cells V*l .. V*l+V-1
+// and reads each field of the row with one vector load. A 1024-thread workgroup
+// stacks kZRows rows: kZOut output rows and one ring row on each side whose red
+// cells are recomputed (as in k_rbgs_zsweep), and marches up a z chunk. At plane
+// p every wave computes the red cells of its row, then the black cells of plane
+// p-1. Within a row the colours alternate, so with e = (j + p) & 1 the red cells
+// of plane p and the black cells of plane p-1 both sit at c = 2h + 1 - e
+// (h < V/2) and the black cells of plane p at c = 2h + e; every pattern below is
+// a select on e, uniform per wave. Neighbour values across lanes come from
+// shuffles, across rows from LDS (double-buffered by plane parity: one barrier
+// per plane), across planes from registers. Fields are loaded one plane ahead and
+// x two planes ahead. Both kernels accumulate the same terms in the same order
+// with explicit fused multiply-adds, so they produce the same iterate bit for bit.
+// ----------------------------------------------------------------------------
+constexpr int kZRows = 16;
+constexpr int kZOut = kZRows - 2;
+
+template <int N> __device__ inline void vld(const float* p, float (&v)[N]) {
+    if constexpr (N == 1) {
+        v[0] = p[0];
+    } else if constexpr (N == 2) {
+        const float2 t = *reinterpret_cast<const float2*>(p);
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        const float4 t = *reinterpret_cast<const float4*>(p);
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+    }
+}
+template <int N> __device__ inline void vst(float* p, const float (&v)[N]) {
+    if constexpr (N == 1) {
+        p[0] = v[0];
+    } else if constexpr (N == 2) {
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+    } else {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+// Components of a row vector at c = 2h + 1 - E (Rc) and c = 2h + E (Bc).
+template <int E, int V> __device__ inline float Rc(const float (&v)[V], int h) { return v[2 * h + 1 - E]; }
+template <int E, int V> __device__ inline float Bc(const float (&v)[V], int h) { return v[2 * h + E]; }
+
+template <int V, int MODE, typename Acc>
+__global__ __launch_bounds__(1024) void k_rbgs_zrow(Op<float> op, const float* __restrict__ b,
+                                                    const float* __restrict__ xin, float* __restrict__ xout,
+                                                    const float* __restrict__ xc, int ci, int cj, int tiles_y,
+                                                    int zchunk, double* __restrict__ dot) {
+    constexpr int H = V / 2;
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    __shared__ __align__(16) float xs[2][kZRows][64 * H];   // black x values of plane p, per row
+    __shared__ __align__(16) float rs[2][kZRows][64 * H];   // red values of plane p, per row
+    __shared__ __align__(16) float ays[2][kZRows][64 * V];  // ay of plane p, per row
+    __shared__ Acc part[kZRows];
+    const int ni = op.ni, nj = op.nj, nk = op.nk;
+    const float sc = op.scale;
+    // consecutive tiles (the y-neighbours of one z chunk) on one XCD: block L runs on XCD L % 8
+    const int nb = gridDim.x, L = blockIdx.x, xcd = L % 8;
+    const int lg = xcd * (nb / 8) + min(xcd, nb % 8) + L / 8;
+    const int ty = lg % tiles_y, zch = lg / tiles_y;
+    const int j0 = ty * kZOut, k0 = zch * zchunk, k1 = min(k0 + zchunk, nk);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int j = j0 - 1 + w;
+    const bool row_ok = j >= 0 && j < nj;
+    const bool out_row = row_ok && w >= 1 && w <= kZOut;
+    const bool ym = j > 0, yp = j + 1 < nj;
+    const int jc = min(max(j, 0), nj - 1);
+    // ring rows read one row further out: j0-2 below the first, j0+kZOut+1 above the last
+    const bool has_ex = w == 0 || w == kZRows - 1;
+    const int jx = min(max(w == 0 ? j - 1 : j + 1, 0), nj - 1);
+    const int i0 = lane * V;
+    const int lprev = (lane + 63) & 63, lnext = (lane + 1) & 63;
+    Acc dacc = 0;
+
+    auto at = [&](int jj, int k) { return (min(max(k, 0), nk - 1) * nj + jj) * ni + i0; };
+    auto load_x = [&](int jj, int k, float (&v)[V]) {
+        const int kk = min(max(k, 0), nk - 1);
+        vld<V>(xin + (kk * nj + jj) * ni + i0, v);
+        if constexpr (MODE == 2) {
+            float cv[H];
+            vld<H>(xc + ((kk >> 1) * cj + (jj >> 1)) * ci + (i0 >> 1), cv);
+#pragma unroll
+            for (int c = 0; c < V; ++c) v[c] += cv[c >> 1];
+        }
+    };
+
+    float cb[V], cd[V], cx[V], cy[V], cz[V];  // fields of plane p
+    float zprev[V];                           // az of plane p-1
+    float xbm[H], xb0[H], xbp[H];             // black x values of planes p-1, p, p+1
+    float exx[V], exy[V];                     // outer row's x and ay at plane p (ring waves)
+    float qb[H], qd[H], qxl[H], qxr[H], qy[H], qyd[H], qz[H], qzd[H], qbr[H];  // black cells of plane p-1
+    float nqm[H], nq[H];                      // red values of planes p-2, p-1
+    {
+        const int k = k0 - 1, o = at(jc, k);
+        vld<V>(b + o, cb); vld<V>(op.d + o, cd); vld<V>(op.ax + o, cx); vld<V>(op.ay + o, cy);
+        vld<V>(op.az + o, cz);
+        vld<V>(op.az + at(jc, k - 1), zprev);
+        if constexpr (MODE != 1) {
+            // black x values of planes k-1, k, k+1 (at c = 2h + ((j + plane) & 1))
+            float t0[V], t1[V], t2[V];
+            load_x(jc, k - 1, t0);
+            load_x(jc, k, t1);
+            load_x(jc, k + 1, t2);
+            auto pick = [&](auto E) {
+                constexpr int e = decltype(E)::value;  // parity of plane k
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    xbm[h] = Bc<1 - e>(t0, h);
+                    xb0[h] = Bc<e>(t1, h);
+                    xbp[h] = Bc<1 - e>(t2, h);
+                }
+            };
+            if ((j + k) & 1) pick(I1()); else pick(I0());
+            if (has_ex) load_x(jx, k, exx);
+        }
+        if (has_ex) vld<V>(op.ay + at(jx, k), exy);
+#pragma unroll
+        for (int h = 0; h < H; ++h) nq[h] = nqm[h] = 0.f;
+    }
+
+    for (int p = k0 - 1; p <= k1; ++p) {
+        const int buf = p & 1;
+        // next plane's fields, x two planes ahead, the ring rows' outer row
+        float nb_[V], nd[V], nx[V], ny[V], nz[V], nxx[V], nex[V], ney[V];
+        {
+            const int o = at(jc, p + 1);
+            vld<V>(b + o, nb_); vld<V>(op.d + o, nd); vld<V>(op.ax + o, nx); vld<V>(op.ay + o, ny);
+            vld<V>(op.az + o, nz);
+            if constexpr (MODE != 1) {
+                load_x(jc, p + 2, nxx);
+                if (has_ex) load_x(jx, p + 1, nex);
+            }
+            if (has_ex) vld<V>(op.ay + at(jx, p + 1), ney);
+        }
+        if constexpr (MODE != 1) vst<H>(&xs[buf][w][lane * H], xb0);
+        vst<V>(&ays[buf][w][lane * V], cy);
+        lds_barrier();
+
+        auto body = [&](auto E) {
+            constexpr int e = decltype(E)::value;  // (j + p) & 1
+            // red cells of plane p
+            float ayd[V];  // ay of row j-1
+            if (w > 0) vld<V>(&ays[buf][w - 1][lane * V], ayd);
+            else {
+#pragma unroll
+                for (int c = 0; c < V; ++c) ayd[c] = exy[c];
+            }
+            const float axe = __shfl(cx[V - 1], lprev, 64);  // ax of cell i0-1
+            float nr[H];
+#pragma unroll
+            for (int h = 0; h < H; ++h) nr[h] = 0.f;
+            if (row_ok && p >= 0 && p < nk) {
+                float xdn[H], xup[H], edge = 0.f;
+                if constexpr (MODE != 1) {
+                    if (w > 0) vld<H>(&xs[buf][w - 1][lane * H], xdn);
+                    else {
+#pragma unroll
+                        for (int h = 0; h < H; ++h) xdn[h] = Rc<e>(exx, h);
+                    }
+                    if (w < kZRows - 1) vld<H>(&xs[buf][w + 1][lane * H], xup);
+                    else {
+#pragma unroll
+                        for (int h = 0; h < H; ++h) xup[h] = Rc<e>(exx, h);
+                    }
+                    edge = e ? __shfl(xb0[H - 1], lprev, 64) : __shfl(xb0[0], lnext, 64);
+                }
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    const int i = i0 + 2 * h + 1 - e;
+                    float s = 0.f;
+                    if constexpr (MODE != 1) {
+                        const bool zm = p > 0, xm = i > 0, xp = i + 1 < ni, zp = p + 1 < nk;
+                        const float axl = e ? (h > 0 ? cx[2 * h - 1] : axe) : cx[2 * h];
+                        const float a0 = zm ? sc * Rc<e>(zprev, h) : 0.f;
+                        const float a1 = ym ? sc * Rc<e>(ayd, h) : 0.f;
+                        const float a2 = xm ? sc * axl : 0.f;
+                        const float a3 = xp ? sc * Rc<e>(cx, h) : 0.f;
+                        const float a4 = yp ? sc * Rc<e>(cy, h) : 0.f;
+                        const float a5 = zp ? sc * Rc<e>(cz, h) : 0.f;
+                        const float xl = e ? (h > 0 ? xb0[h - 1] : edge) : xb0[h];
+                        const float xr = e ? xb0[h] : (h + 1 < H ? xb0[h + 1] : edge);
+                        const float x0 = zm ? xbm[h] : 0.f;
+                        const float x1 = ym ? xdn[h] : 0.f;
+                        const float x2 = xm ? xl : 0.f;
+                        const float x3 = xp ? xr : 0.f;
+                        const float x4 = yp ? xup[h] : 0.f;
+                        const float x5 = zp ? xbp[h] : 0.f;
+                        s = __builtin_fmaf(a0, x0, s);
+                        s = __builtin_fmaf(a1, x1, s);
+                        s = __builtin_fmaf(a2, x2, s);
+                        s = __builtin_fmaf(a3, x3, s);
+                        s = __builtin_fmaf(a4, x4, s);
+                        s = __builtin_fmaf(a5, x5, s);
+                    }
+                    nr[h] = gs_value(Rc<e>(cb, h), s, sc * Rc<e>(cd, h));
+                }
+            }
+            vst<H>(&rs[buf][w][lane * H], nr);
+
+            // black cells of plane q = p-1 (same positions as this plane's red cells)
+            const int q = p - 1;
+            if (out_row && q >= k0 && q < k1) {
+                float rdn[H], rup[H];
+                vld<H>(&rs[buf ^ 1][w - 1][lane * H], rdn);
+                vld<H>(&rs[buf ^ 1][w + 1][lane * H], rup);
+                const float edge = e ? __shfl(nq[H - 1], lprev, 64) : __shfl(nq[0], lnext, 64);
+                float o[V];
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    const int i = i0 + 2 * h + 1 - e;
+                    const float rl = e ? (h > 0 ? nq[h - 1] : edge) : nq[h];
+                    const float rr = e ? nq[h] : (h + 1 < H ? nq[h + 1] : edge);
+                    float sum = 0.f;
+                    if (q > 0) sum = __builtin_fmaf(qzd[h], nqm[h], sum);
+                    if (ym) sum = __builtin_fmaf(qyd[h], rdn[h], sum);
+                    if (i > 0) sum = __builtin_fmaf(qxl[h], rl, sum);
+                    if (i + 1 < ni) sum = __builtin_fmaf(qxr[h], rr, sum);
+                    if (yp) sum = __builtin_fmaf(qy[h], rup[h], sum);
+                    if (q + 1 < nk) sum = __builtin_fmaf(qz[h], nr[h], sum);
+                    const float v = gs_value(qb[h], sum, qd[h]);
+                    o[2 * h + 1 - e] = v;
+                    o[2 * h + e] = nq[h];
+                    if (dot) dacc += (Acc)v * (Acc)qb[h] + (Acc)nq[h] * (Acc)qbr[h];
+                }
+                vst<V>(xout + (q * nj + j) * ni + i0, o);
+            }
+
+            // black cells of plane p for the next step (c = 2h + e), then rotate
+#pragma unroll
+            for (int h = 0; h < H; ++h) {
+                qb[h] = Bc<e>(cb, h);
+                qbr[h] = Rc<e>(cb, h);
+                qd[h] = sc * Bc<e>(cd, h);
+                qxl[h] = sc * (e ? cx[2 * h] : (h > 0 ? cx[2 * h - 1] : axe));
+                qxr[h] = sc * Bc<e>(cx, h);
+                qy[h] = sc * Bc<e>(cy, h);
+                qyd[h] = sc * Bc<e>(ayd, h);
+                qz[h] = sc * Bc<e>(cz, h);
+                qzd[h] = sc * Bc<e>(zprev, h);
+                nqm[h] = nq[h];
+                nq[h] = nr[h];
+            }
+            if constexpr (MODE != 1) {
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    xbm[h] = xb0[h];
+                    xb0[h] = xbp[h];
+                    xbp[h] = Bc<e>(nxx, h);  // plane p+2 has the parity of p
+                }
+            }
+        };
+        if ((j + p) & 1) body(I1()); else body(I0());
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            zprev[c] = cz[c];
+            cb[c] = nb_[c]; cd[c] = nd[c]; cx[c] = nx[c]; cy[c] = ny[c]; cz[c] = nz[c];
+            exy[c] = ney[c];
+            if constexpr (MODE != 1) exx[c] = nex[c];
+        }
+    }
+    if (dot) {
+        const Acc t = wave_sum(dacc);
+        if (lane == 0) part[w] = t;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Acc r = 0;
+            for (int k = 0; k < kZRows; ++k) r += part[k];
+            dot[blockIdx.x] = (double)r;
+        }
     }
 }
 

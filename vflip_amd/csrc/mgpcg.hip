@@ -54,6 +54,8 @@ struct vflip_solver {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool setup_done = false;
     int zsweep_min_n = 32768;  // levels at least this large use the fused z-marching sweep
+    bool use_zrow = true;      // row-wave form of the fused sweep where the row length allows it
+    int cus = 256;             // compute units of the device (launch sizing)
     // coarsest-level operator RBGS^coarse_sweeps tabulated as a dense matrix
     float* ctab = nullptr;
     float* cpart = nullptr;
@@ -97,12 +99,26 @@ dim3 grid3(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, n
 dim3 grid3z(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, (nk + kMatvecKZ - 1) / kMatvecKZ); }
 int nblocks(dim3 g) { return (int)(g.x * g.y * g.z); }
 
-// Fused z-marching sweep geometry for a level.
+// Fused z-marching sweep geometry for a level. Rows of 128 or 256 cells use the
+// row-wave kernel (one 1024-thread workgroup per compute unit, y tiles of kZOut
+// rows, z chunks sized so the grid is one wave of workgroups); other shapes use
+// the 64 x TY tile kernel.
 struct ZGeom {
     int ty, tiles_x, tiles_y, zchunk, blocks;
+    int v;  // cells per lane of the row-wave kernel, 0 for the tile kernel
 };
-ZGeom zgeom(const LevelBuf& lv) {
+ZGeom zgeom(const LevelBuf& lv, bool use_zrow, int cus) {
     ZGeom g;
+    g.v = (use_zrow && (lv.ni == 128 || lv.ni == 256) && lv.nj >= 8) ? lv.ni / 64 : 0;
+    if (g.v) {
+        g.ty = kZOut;
+        g.tiles_x = 1;
+        g.tiles_y = (lv.nj + kZOut - 1) / kZOut;
+        const int chunks = std::max(1, cus / g.tiles_y);
+        g.zchunk = std::max(2, (lv.nk + chunks - 1) / chunks);
+        g.blocks = g.tiles_y * ((lv.nk + g.zchunk - 1) / g.zchunk);
+        return g;
+    }
     g.ty = lv.nj >= 128 ? 8 : 4;
     if (const char* e = getenv("VFLIP_ZTY")) {
         const int t = atoi(e);
@@ -120,11 +136,18 @@ ZGeom zgeom(const LevelBuf& lv) {
     g.blocks = tiles * ((lv.nk + zc - 1) / zc);
     return g;
 }
+ZGeom zgeom(const vflip_solver* S, const LevelBuf& lv) { return zgeom(lv, S->use_zrow, S->cus); }
 
 template <int MODE, typename Acc>
 void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<float>& op, const float* b, const float* xin,
                         float* xout, const float* xc, int ci, int cj, double* dot) {
-    if (g.ty == 16)
+    if (g.v == 4)
+        k_rbgs_zrow<4, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
+                                                                     g.zchunk, dot);
+    else if (g.v == 2)
+        k_rbgs_zrow<2, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
+                                                                     g.zchunk, dot);
+    else if (g.ty == 16)
         k_rbgs_zsweep<16, MODE, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
                                                                        g.tiles_y, g.zchunk, dot);
     else if (g.ty == 8)
@@ -138,7 +161,7 @@ void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<float>& op, co
 template <typename Acc>
 void launch_zsweep(vflip_solver* S, const LevelBuf& lv, const float* b, const float* xin, float* xout, int mode,
                    const float* xc, int ci, int cj, double* dot) {
-    const ZGeom g = zgeom(lv);
+    const ZGeom g = zgeom(S, lv);
     const Op<float> op = make_op<float>(lv);
     if (mode == 1) launch_zsweep_mode<1, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
     else if (mode == 2) launch_zsweep_mode<2, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
@@ -190,7 +213,7 @@ template <typename CT> int smooth_level(vflip_solver* S, int l, bool pre, const 
             if (prof) VF_HIP(hipEventRecord(S->prof_ev[S->prof_used++], S->stream));
             std::swap(*xp, *xtp);
             if (dot) {
-                S->np_last = zgeom(lv).blocks;
+                S->np_last = zgeom(S, lv).blocks;
                 *dot_done = true;
             }
         }
@@ -413,6 +436,13 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
     vflip_default_opts(&S->opts);
     if (opts) S->opts = *opts;
     if (const char* e = getenv("VFLIP_ZSWEEP_MIN_N")) S->zsweep_min_n = atoi(e);
+    if (const char* e = getenv("VFLIP_ZROW")) S->use_zrow = atoi(e) != 0;
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            S->cus = cus;
+    }
     // level shapes, S/GeometricLevelGen.h:36-58
     {
         int a = ni, b = nj, c = nk;
@@ -458,7 +488,8 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
     if (!rc) {
         const LevelBuf& l0 = S->L[0];
         int np = std::max(kVecBlocks, nblocks(grid3z(l0.ni, l0.nj, l0.nk)));
-        np = std::max(np, zgeom(l0).blocks);
+        np = std::max(np, zgeom(l0, false, S->cus).blocks);
+        np = std::max(np, zgeom(l0, true, S->cus).blocks);
         S->np_partial = np;
         rc = alloc((void**)&S->partial, (size_t)np * sizeof(double));
     }
