@@ -1,14 +1,18 @@
-"""The row-wave smoothing sweep (levels whose rows are 128 or 256 cells) on the
-GPU, against the 64 x TY tile sweep and the oracle.
+"""The row-wave kernels (levels whose rows are 128 or 256 cells: the fused
+smoothing sweep, the residual restriction and the PCG matvec) on the GPU,
+against the general kernels and the oracle.
 
-* One V-cycle: the two GPU kernels evaluate the same terms in the same order, so
-  their results are bit-identical; both are within 1e-5 relative L2 of the
+* One V-cycle (smoothing sweeps and residual restrictions): both GPU paths
+  evaluate the same terms in the same order with the same fused multiply-adds,
+  so their results are bit-identical; both are within 1e-5 relative L2 of the
   oracle's FP64 V-cycle.
-* PCG solves (FP32 and mixed precision): the z.r partial sums are grouped
-  differently by the two kernels, so the iterates agree to FP32 round-off and the
-  iteration counts are equal.
+* The matvec: bit-identical to the general kernel, within FP32 rounding of the
+  oracle's FP64 product; an unaligned view falls back to the general kernel.
+* PCG solves (FP32 and mixed precision): the z.r and s.q partial sums are grouped
+  differently, so the iterates agree to FP32 round-off and the iteration counts
+  are equal.
 
-Shapes cover full tiles and ragged ones (nj, nk not multiples of the 14-row tile
+Shapes cover full tiles and ragged ones (nj, nk not multiples of the tile rows
 or the z chunk, odd coarse sizes); the fused sweeps are enabled down to 4096
 cells so that coarse levels with 128-cell rows take the row-wave path too.
 """
@@ -108,3 +112,30 @@ def test_row_wave_pcg_matches_tile_sweep(cuda, precision):
     assert _rel(res[0][1], res[1][1]) < 1e-5
     r = op[4] - O.matvec(*op[:4], res[0][1].astype(np.float64))
     assert np.abs(r).max() <= 1e-5 * np.abs(op[4]).max()
+
+
+@pytest.mark.parametrize("shape", [(128, 37, 19), (256, 40, 20), (256, 64, 48)])
+def test_row_wave_matvec_bitexact(cuda, shape):
+    print(f"row-wave matvec {shape}", flush=True)
+    import torch
+    from oracle import oracle as O
+    from vflip_amd import api
+    ni, nj, nk = shape
+    op = _problem(O, ni, nj, nk, seed=3 * ni + nj)
+    x = torch.randn((nk, nj, ni), generator=torch.Generator().manual_seed(nj), dtype=torch.float32)
+    out = []
+    for row_wave in (True, False):
+        S = _solver(api, shape, op, row_wave)
+        y = torch.zeros((nk, nj, ni), device="cuda")
+        S.matvec(x.cuda(), y)
+        out.append(y.cpu().numpy())
+    assert np.array_equal(out[0], out[1]), float(np.abs(out[0] - out[1]).max())
+    ref = O.matvec(*[a.astype(np.float32).astype(np.float64) for a in op[:4]], x.numpy().astype(np.float64))
+    assert _rel(out[0], ref) < 1e-6
+    # a view that is not 16-byte aligned takes the general kernel and gives the same result
+    S = _solver(api, shape, op, True)
+    xb = torch.zeros(x.numel() + 1, device="cuda")
+    xb[1:] = x.reshape(-1).cuda()
+    yb = torch.zeros(x.numel() + 1, device="cuda")
+    S.matvec(xb[1:].view(nk, nj, ni), yb[1:].view(nk, nj, ni))
+    assert np.array_equal(yb[1:].cpu().numpy().reshape(nk, nj, ni), out[0])

@@ -41,12 +41,12 @@ template <typename CT>
 __device__ inline float offdiag_sum(const Op<CT>& op, const float* __restrict__ x, int c, int i, int j, int k) {
     const int sl = op.ni * op.nj;
     float s = 0.f;
-    if (k > 0) s += op.AZ(c - sl) * x[c - sl];
-    if (j > 0) s += op.AY(c - op.ni) * x[c - op.ni];
-    if (i > 0) s += op.AX(c - 1) * x[c - 1];
-    if (i + 1 < op.ni) s += op.AX(c) * x[c + 1];
-    if (j + 1 < op.nj) s += op.AY(c) * x[c + op.ni];
-    if (k + 1 < op.nk) s += op.AZ(c) * x[c + sl];
+    if (k > 0) s = __builtin_fmaf(op.AZ(c - sl), x[c - sl], s);
+    if (j > 0) s = __builtin_fmaf(op.AY(c - op.ni), x[c - op.ni], s);
+    if (i > 0) s = __builtin_fmaf(op.AX(c - 1), x[c - 1], s);
+    if (i + 1 < op.ni) s = __builtin_fmaf(op.AX(c), x[c + 1], s);
+    if (j + 1 < op.nj) s = __builtin_fmaf(op.AY(c), x[c + op.ni], s);
+    if (k + 1 < op.nk) s = __builtin_fmaf(op.AZ(c), x[c + sl], s);
     return s;
 }
 
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(256) void k_residual_restrict(Op<CT> op, const floa
     float r = 0.f;
     if (i < op.ni && j < op.nj && k < op.nk) {
         const int c = (k * op.nj + j) * op.ni + i;
-        r = 0.125f * (b[c] - (offdiag_sum(op, x, c, i, j, k) + op.D(c) * x[c]));
+        r = 0.125f * (b[c] - __builtin_fmaf(op.D(c), x[c], offdiag_sum(op, x, c, i, j, k)));
     }
     r += __shfl_xor(r, 1, 64);
     const int row = threadIdx.y + 2 * threadIdx.z;
@@ -869,13 +869,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* _
             const int c = (k * nj + j) * ni + i;
             // reference multiply order: -z, -y, -x, diag, +x, +y, +z
             float y = 0.f;
-            if (k > 0) y += op.AZ(c - sl) * s[c - sl];
-            if (j > 0) y += op.AY(c - ni) * s[c - ni];
-            if (i > 0) y += op.AX(c - 1) * s[c - 1];
-            y += op.D(c) * s[c];
-            if (i + 1 < ni) y += op.AX(c) * s[c + 1];
-            if (j + 1 < nj) y += op.AY(c) * s[c + ni];
-            if (k + 1 < nk) y += op.AZ(c) * s[c + sl];
+            if (k > 0) y = __builtin_fmaf(op.AZ(c - sl), s[c - sl], y);
+            if (j > 0) y = __builtin_fmaf(op.AY(c - ni), s[c - ni], y);
+            if (i > 0) y = __builtin_fmaf(op.AX(c - 1), s[c - 1], y);
+            y = __builtin_fmaf(op.D(c), s[c], y);
+            if (i + 1 < ni) y = __builtin_fmaf(op.AX(c), s[c + 1], y);
+            if (j + 1 < nj) y = __builtin_fmaf(op.AY(c), s[c + ni], y);
+            if (k + 1 < nk) y = __builtin_fmaf(op.AZ(c), s[c + sl], y);
             q[c] = y;
             acc += (Acc)s[c] * (Acc)y;
         }
@@ -883,6 +883,168 @@ __global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* _
     Acc t = block_sum(acc);
     if (threadIdx.x == 0 && threadIdx.y == 0)
         partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (double)t;
+}
+
+// Row-wave 7-point stencil for levels whose rows are 64*V cells (V = 2 or 4),
+// organised like k_rbgs_zrow: a wave owns one x row, a 1024-thread workgroup
+// kZRows consecutive rows, marching up a z chunk with the z-neighbours in
+// registers, the y-neighbours through LDS (double-buffered, one barrier per
+// plane; the first and last wave also load the row just outside the tile) and
+// the x-neighbours by lane shuffles.
+//   KIND 0: out = A s and a per-workgroup sum(s.out)      (k_matvec_dot)
+//   KIND 1: out = R (b - A s) on the next coarser level   (k_residual_restrict)
+// Same terms, same order and the same fused multiply-adds as those kernels. The
+// restriction sums each 2x2x2 aggregate as they do: the x pair inside the lane,
+// the y pair (rows 2J, 2J+1 = waves w, w+1) through LDS one plane later, the z
+// pair in registers; z chunks start on even planes.
+template <int V, int KIND, typename Acc>
+__global__ __launch_bounds__(1024) void k_stencil_row(Op<float> op, const float* __restrict__ b,
+                                                      const float* __restrict__ s, float* __restrict__ out, int ci,
+                                                      int cj, int tiles_y, int zchunk, double* __restrict__ partial) {
+    constexpr int H = V / 2;
+    __shared__ __align__(16) float ss[2][kZRows][64 * V];   // s of plane k, per row
+    __shared__ __align__(16) float ays[2][kZRows][64 * V];  // ay of plane k, per row
+    __shared__ __align__(16) float ps[2][kZRows][64 * H];   // restriction: x-pair sums of plane k
+    __shared__ Acc part[kZRows];
+    const int ni = op.ni, nj = op.nj, nk = op.nk;
+    const float sc = op.scale;
+    const int nb = gridDim.x, L = blockIdx.x, xcd = L % 8;
+    const int lg = xcd * (nb / 8) + min(xcd, nb % 8) + L / 8;
+    const int ty = lg % tiles_y, zch = lg / tiles_y;
+    const int j0 = ty * kZRows, k0 = zch * zchunk, k1 = min(k0 + zchunk, nk);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int j = j0 + w;
+    const bool row_ok = j < nj;
+    const bool ym = j > 0, yp = j + 1 < nj;
+    const int jc = min(j, nj - 1);
+    const bool ex_lo = w == 0, ex_hi = w == kZRows - 1;
+    const int jx = ex_lo ? max(j - 1, 0) : min(j + 1, nj - 1);
+    const int i0 = lane * V;
+    const int lprev = (lane + 63) & 63, lnext = (lane + 1) & 63;
+    Acc dacc = 0;
+    auto at = [&](int jj, int k) { return (min(max(k, 0), nk - 1) * nj + jj) * ni + i0; };
+
+    float sm[V], s0[V], sp[V];                 // s of row j at planes k-1, k, k+1
+    float cd[V], cx[V], cy[V], cz[V], cb[V];   // fields of plane k (cb: KIND 1)
+    float zprev[V];                            // az of plane k-1
+    float exs[V], exy[V];                      // row outside the tile: s (and ay below) at plane k
+    float xpl[H], ylo[H];                      // KIND 1: own x-pair sums of plane k-1; y-pair sums of the even plane
+    {
+        vld<V>(s + at(jc, k0 - 1), sm);
+        vld<V>(s + at(jc, k0), s0);
+        vld<V>(s + at(jc, k0 + 1), sp);
+        const int o = at(jc, k0);
+        vld<V>(op.d + o, cd); vld<V>(op.ax + o, cx); vld<V>(op.ay + o, cy); vld<V>(op.az + o, cz);
+        if constexpr (KIND == 1) vld<V>(b + o, cb);
+        vld<V>(op.az + at(jc, k0 - 1), zprev);
+        if (ex_lo || ex_hi) vld<V>(s + at(jx, k0), exs);
+        if (ex_lo) vld<V>(op.ay + at(jx, k0), exy);
+#pragma unroll
+        for (int h = 0; h < H; ++h) xpl[h] = ylo[h] = 0.f;
+    }
+    for (int k = k0; k <= k1; ++k) {
+        const int buf = k & 1;
+        float nd[V], nx[V], ny[V], nz[V], nbv[V], nsp[V], nexs[V], nexy[V];
+        if (k < k1) {
+            const int o = at(jc, k + 1);
+            vld<V>(op.d + o, nd); vld<V>(op.ax + o, nx); vld<V>(op.ay + o, ny); vld<V>(op.az + o, nz);
+            if constexpr (KIND == 1) vld<V>(b + o, nbv);
+            vld<V>(s + at(jc, k + 2), nsp);
+            if (ex_lo || ex_hi) vld<V>(s + at(jx, k + 1), nexs);
+            if (ex_lo) vld<V>(op.ay + at(jx, k + 1), nexy);
+            vst<V>(&ss[buf][w][lane * V], s0);
+            vst<V>(&ays[buf][w][lane * V], cy);
+        }
+        lds_barrier();
+        if constexpr (KIND == 1) {
+            // plane k-1: the even wave of each row pair adds its partner's x-pair sums
+            if (k > k0 && (w & 1) == 0 && row_ok) {
+                const int kq = k - 1;
+                float other[H], out_v[H];
+                vld<H>(&ps[buf ^ 1][w + 1][lane * H], other);
+#pragma unroll
+                for (int h = 0; h < H; ++h) {
+                    const float ys = xpl[h] + other[h];
+                    if (kq & 1) out_v[h] = ylo[h] + ys;
+                    else if (kq == nk - 1) out_v[h] = ys + (0.f + 0.f);
+                    else ylo[h] = ys;
+                }
+                if ((kq & 1) || kq == nk - 1) vst<H>(out + ((kq >> 1) * cj + (j >> 1)) * ci + lane * H, out_v);
+            }
+        }
+        if (k < k1) {
+            float xp[H];
+#pragma unroll
+            for (int h = 0; h < H; ++h) xp[h] = 0.f;
+            if (row_ok) {
+                float sdn[V], sup[V], ayd[V];
+                if (w > 0) {
+                    vld<V>(&ss[buf][w - 1][lane * V], sdn);
+                    vld<V>(&ays[buf][w - 1][lane * V], ayd);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < V; ++c) { sdn[c] = exs[c]; ayd[c] = exy[c]; }
+                }
+                if (w < kZRows - 1) vld<V>(&ss[buf][w + 1][lane * V], sup);
+                else {
+#pragma unroll
+                    for (int c = 0; c < V; ++c) sup[c] = exs[c];
+                }
+                const float sl_e = __shfl(s0[V - 1], lprev, 64);
+                const float sr_e = __shfl(s0[0], lnext, 64);
+                const float ax_e = __shfl(cx[V - 1], lprev, 64);
+                float y[V];
+#pragma unroll
+                for (int c = 0; c < V; ++c) {
+                    const int i = i0 + c;
+                    float t = 0.f;
+                    if (k > 0) t = __builtin_fmaf(sc * zprev[c], sm[c], t);
+                    if (ym) t = __builtin_fmaf(sc * ayd[c], sdn[c], t);
+                    if (i > 0) t = __builtin_fmaf(sc * (c > 0 ? cx[c - 1] : ax_e), c > 0 ? s0[c - 1] : sl_e, t);
+                    if constexpr (KIND == 0) t = __builtin_fmaf(sc * cd[c], s0[c], t);
+                    if (i + 1 < ni) t = __builtin_fmaf(sc * cx[c], c + 1 < V ? s0[c + 1] : sr_e, t);
+                    if (yp) t = __builtin_fmaf(sc * cy[c], sup[c], t);
+                    if (k + 1 < nk) t = __builtin_fmaf(sc * cz[c], sp[c], t);
+                    if constexpr (KIND == 0) {
+                        y[c] = t;
+                        if (partial) dacc += (Acc)s0[c] * (Acc)t;
+                    } else {
+                        y[c] = 0.125f * (cb[c] - __builtin_fmaf(sc * cd[c], s0[c], t));
+                    }
+                }
+                if constexpr (KIND == 0) vst<V>(out + (k * nj + j) * ni + i0, y);
+                else {
+#pragma unroll
+                    for (int h = 0; h < H; ++h) xp[h] = y[2 * h] + y[2 * h + 1];
+                }
+            }
+            if constexpr (KIND == 1) {
+                vst<H>(&ps[buf][w][lane * H], xp);
+#pragma unroll
+                for (int h = 0; h < H; ++h) xpl[h] = xp[h];
+            }
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                zprev[c] = cz[c];
+                sm[c] = s0[c]; s0[c] = sp[c]; sp[c] = nsp[c];
+                cd[c] = nd[c]; cx[c] = nx[c]; cy[c] = ny[c]; cz[c] = nz[c];
+                if constexpr (KIND == 1) cb[c] = nbv[c];
+                exs[c] = nexs[c]; exy[c] = nexy[c];
+            }
+        }
+    }
+    if constexpr (KIND == 0) {
+        if (partial) {
+            const Acc t = wave_sum(dacc);
+            if (lane == 0) part[w] = t;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                Acc r = 0;
+                for (int q = 0; q < kZRows; ++q) r += part[q];
+                partial[blockIdx.x] = (double)r;
+            }
+        }
+    }
 }
 
 // FP64 true residual r = b - A x of the mixed-precision refinement. The level-0

@@ -19,6 +19,10 @@
 #include <cstring>
 #include <vector>
 
+#include <cstdint>
+#include <initializer_list>
+#include <type_traits>
+
 #include "mg_kernels.h"
 
 using namespace vflip;
@@ -138,6 +142,39 @@ ZGeom zgeom(const LevelBuf& lv, bool use_zrow, int cus) {
 }
 ZGeom zgeom(const vflip_solver* S, const LevelBuf& lv) { return zgeom(lv, S->use_zrow, S->cus); }
 
+// Row-wave stencil geometry (matvec, residual restriction): y tiles of kZRows
+// rows, z chunks (even, for the restriction) sized so the grid is one wave of
+// workgroups. v == 0: the level or a pointer does not fit the row kernel.
+struct RowGeom {
+    int v, tiles_y, zchunk, blocks;
+};
+bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+RowGeom rowgeom(const vflip_solver* S, const LevelBuf& lv, std::initializer_list<const void*> ptrs) {
+    RowGeom g{0, 0, 0, 0};
+    if (!S->use_zrow || !(lv.ni == 128 || lv.ni == 256) || lv.nj < 8) return g;
+    for (const void* p : ptrs)
+        if (p && !aligned16(p)) return g;
+    g.v = lv.ni / 64;
+    g.tiles_y = (lv.nj + kZRows - 1) / kZRows;
+    const int chunks = std::max(1, S->cus / g.tiles_y);
+    int zc = (lv.nk + chunks - 1) / chunks;
+    zc = std::max(2, zc + (zc & 1));
+    g.zchunk = zc;
+    g.blocks = g.tiles_y * ((lv.nk + zc - 1) / zc);
+    return g;
+}
+template <int KIND, typename Acc>
+void launch_stencil_row(vflip_solver* S, const RowGeom& g, const LevelBuf& lv, const float* b, const float* s,
+                        float* out, int ci, int cj, double* partial) {
+    const Op<float> op = make_op<float>(lv);
+    if (g.v == 4)
+        k_stencil_row<4, KIND, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y, g.zchunk,
+                                                                       partial);
+    else
+        k_stencil_row<2, KIND, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y, g.zchunk,
+                                                                       partial);
+}
+
 template <int MODE, typename Acc>
 void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<float>& op, const float* b, const float* xin,
                         float* xout, const float* xc, int ci, int cj, double* dot) {
@@ -241,6 +278,14 @@ template <typename CT> int restrict_level(vflip_solver* S, int l) {
     LevelBuf& c = S->L[l + 1];
     const float* b = (l == 0) ? S->r : lv.b;
     const float* x = (l == 0) ? S->z : lv.x;
+    if constexpr (std::is_same<CT, float>::value) {
+        const RowGeom rg = rowgeom(S, lv, {b, x, c.b});
+        if (rg.v) {
+            launch_stencil_row<1, float>(S, rg, lv, b, x, c.b, c.ni, c.nj, nullptr);
+            VF_CHECK_LAUNCH();
+            return 0;
+        }
+    }
     const dim3 g((lv.ni + 63) / 64, (lv.nj + 1) / 2, (lv.nk + 1) / 2);
     k_residual_restrict<CT><<<g, dim3(64, 2, 2), 0, S->stream>>>(make_op<CT>(lv), b, x, c.b, c.ni, c.nj, c.nk);
     VF_CHECK_LAUNCH();
@@ -334,6 +379,13 @@ void launch_true_residual(vflip_solver* S, const float* b, const float* x, float
 }
 void launch_matvec_dot(vflip_solver* S, const float* s, float* q) {
     const LevelBuf& l0 = S->L[0];
+    const RowGeom rg = rowgeom(S, l0, {s, q});
+    if (rg.v) {
+        if (mixed(S)) launch_stencil_row<0, double>(S, rg, l0, nullptr, s, q, 0, 0, S->partial);
+        else launch_stencil_row<0, float>(S, rg, l0, nullptr, s, q, 0, 0, S->partial);
+        S->np_last = rg.blocks;
+        return;
+    }
     dim3 g = grid3z(l0.ni, l0.nj, l0.nk);
     if (mixed(S))
         k_matvec_dot<float, double><<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), s, q, S->partial);
@@ -687,8 +739,12 @@ int vflip_solver_vcycle(vflip_solver* S, const float* r, float* z) {
 int vflip_solver_matvec(vflip_solver* S, const float* x, float* y) {
     VF_ARG(S && x && y, "solver/x/y");
     const LevelBuf& l0 = S->L[0];
-    k_matvec_dot<float, float><<<grid3z(l0.ni, l0.nj, l0.nk), dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), x, y,
-                                                                                           S->partial);
+    const RowGeom rg = rowgeom(S, l0, {x, y});
+    if (rg.v)
+        launch_stencil_row<0, float>(S, rg, l0, nullptr, x, y, 0, 0, nullptr);
+    else
+        k_matvec_dot<float, float><<<grid3z(l0.ni, l0.nj, l0.nk), dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), x,
+                                                                                               y, S->partial);
     VF_CHECK_LAUNCH();
     VF_HIP(hipStreamSynchronize(S->stream));
     return 0;
