@@ -201,8 +201,11 @@ int vflip_project(vflip_solver* s, int variant, float h, float dt, float* u, flo
  * field: fnx x fny x fnz with grid offsets (fox,foy,foz) in cells (u: .5,0,0;
  * cell-centred scalars: 0,0,0). Samplers interpolate in FP64 with index clamping
  * (Buffer3D::sample_linear/sample_cubic, S/fluid_buffer3D.h:226-326); results are
- * bit-identical with that arithmetic. field2/out2 may be NULL. Asynchronous on
- * hip_stream. */
+ * bit-identical with that arithmetic.
+ *   mode 2: mode 0's trilinear backtrace and gather evaluated in FP32 (fused
+ *           lerps), within a few FP32 ulps of mode 0 and several times faster;
+ *           the simulation step's choice.
+ * field2/out2 may be NULL. Asynchronous on hip_stream. */
 int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const float* u, const float* v,
                      const float* w, const float* field, int fnx, int fny, int fnz, float fox, float foy,
                      float foz, float* out, const float* field2, float* out2, void* hip_stream);

@@ -1,5 +1,7 @@
 // Semi-Lagrangian advection: RK2 midpoint backtrace with trilinear or clamped
-// cubic gather, plus the smoke source/buoyancy forcing of the plume workload.
+// cubic gather (FP64, bit-exact with the reference arithmetic; trilinear also in
+// FP32 for the simulation step), plus the smoke source/buoyancy forcing of the
+// plume workload.
 //
 // Restates SubDomainSolver::advect_field (S/sub_domain.cpp:191-235, trilinear)
 // and SmokeSolver3D::advect_field_cubic_clamp (S/Smoke_solver3D.cpp:657-707),
@@ -67,6 +69,98 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2(int ni, int nj, int nk, f
     }
 }
 
+// FP32 form of the trilinear RK2 advection (mode 2), for the simulation step:
+// the same backtrace, clamped corners and interpolation order as k_advect_rk2
+// mode 0, evaluated in FP32 with fused lerps a + c (b - a). Agrees with the FP64
+// form to a few FP32 ulps. Each thread advects two cells of one (i, j) column
+// (planes k, k+1) so the three dependent gather stages of the two cells overlap.
+struct GridF {
+    const float* f;
+    int nx, ny, nz;
+    float ox, oy, oz;
+};
+
+__device__ inline float atf(const GridF& g, int i, int j, int k) {
+    i = min(max(i, 0), g.nx - 1);
+    j = min(max(j, 0), g.ny - 1);
+    k = min(max(k, 0), g.nz - 1);
+    return g.f[(k * g.ny + j) * g.nx + i];
+}
+__device__ inline float lerpf(float a, float b, float c) { return __builtin_fmaf(c, b - a, a); }
+
+// Grid coordinate w/h + o split into its cell index and the fraction inside the
+// cell. q = w * (1/h) carries the FP32 rounding of the quotient; the exact
+// remainder w - q h (one FMA) gives it back, so the fraction is accurate to
+// ~1e-7 of a cell at any index (a plain FP32 w/h + o loses ~n * 6e-8 cells).
+__device__ inline void cell_coord(float w, float h, float inv_h, float o, int& i, float& f) {
+    const float q = w * inv_h;
+    const float c = __builtin_fmaf(-q, h, w) * inv_h;
+    const float g = q + o;  // o is 0 or 0.5: exact for |q| < 2^22
+    const float fl = floorf(g);
+    f = (g - fl) + c;
+    i = (int)fl;
+    if (f < 0.f) { f += 1.f; --i; }
+    else if (f >= 1.f) { f -= 1.f; ++i; }
+}
+
+__device__ inline float sample_linear_f(const GridF& g, float h, float inv_h, float wx, float wy, float wz) {
+    int i, j, k;
+    float cx, cy, cz;
+    cell_coord(wx, h, inv_h, g.ox, i, cx);
+    cell_coord(wy, h, inv_h, g.oy, j, cy);
+    cell_coord(wz, h, inv_h, g.oz, k, cz);
+    const float v1 = lerpf(lerpf(atf(g, i, j, k), atf(g, i + 1, j, k), cx),
+                           lerpf(atf(g, i, j + 1, k), atf(g, i + 1, j + 1, k), cx), cy);
+    const float v2 = lerpf(lerpf(atf(g, i, j, k + 1), atf(g, i + 1, j, k + 1), cx),
+                           lerpf(atf(g, i, j + 1, k + 1), atf(g, i + 1, j + 1, k + 1), cx), cy);
+    return lerpf(v1, v2, cz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int nk, float h, float inv_h, float dt,
+                                                           const float* __restrict__ u, const float* __restrict__ v,
+                                                           const float* __restrict__ w, GridF f0,
+                                                           float* __restrict__ out0, GridF f1,
+                                                           float* __restrict__ out1) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, kb = blockIdx.z * 2;
+    if (i >= f0.nx || j >= f0.ny) return;
+    const GridF gu{u, ni + 1, nj, nk, 0.5f, 0.f, 0.f};
+    const GridF gv{v, ni, nj + 1, nk, 0.f, 0.5f, 0.f};
+    const GridF gw{w, ni, nj, nk + 1, 0.f, 0.f, 0.5f};
+    const float hdt = 0.5f * dt;
+    float wx[2], wy[2], wz[2], uu[2], vv[2], ww[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int k = min(kb + c, f0.nz - 1);
+        wx[c] = ((float)i - f0.ox) * h;
+        wy[c] = ((float)j - f0.oy) * h;
+        wz[c] = ((float)k - f0.oz) * h;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        uu[c] = sample_linear_f(gu, h, inv_h, wx[c], wy[c], wz[c]);
+        vv[c] = sample_linear_f(gv, h, inv_h, wx[c], wy[c], wz[c]);
+        ww[c] = sample_linear_f(gw, h, inv_h, wx[c], wy[c], wz[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const float px = __builtin_fmaf(-hdt, uu[c], wx[c]), py = __builtin_fmaf(-hdt, vv[c], wy[c]);
+        const float pz = __builtin_fmaf(-hdt, ww[c], wz[c]);
+        uu[c] = sample_linear_f(gu, h, inv_h, px, py, pz);
+        vv[c] = sample_linear_f(gv, h, inv_h, px, py, pz);
+        ww[c] = sample_linear_f(gw, h, inv_h, px, py, pz);
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int k = kb + c;
+        if (k >= f0.nz) break;
+        // the reference's float ops: product, then difference
+        const float px = wx[c] - dt * uu[c], py = wy[c] - dt * vv[c], pz = wz[c] - dt * ww[c];
+        const int o = (k * f0.ny + j) * f0.nx + i;
+        out0[o] = sample_linear_f(f0, h, inv_h, px, py, pz);
+        if (out1) out1[o] = sample_linear_f(f1, h, inv_h, px, py, pz);
+    }
+}
+
 // Plume forcing on the reference-convention grid (cell centre (i,j,k)h):
 //   source sphere: rho = T = 1 + jitter(seed, cell) inside |x - c| <= r
 //   buoyancy on v faces: v += dt * (-alpha rho_f + beta T_f), rho_f = average of
@@ -115,8 +209,18 @@ int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const 
                      float* out, const float* field2, float* out2, void* stream) {
     VF_ARG(ni > 0 && nj > 0 && nk > 0 && fnx > 0 && fny > 0 && fnz > 0, "grid dims");
     VF_ARG(u && v && w && field && out, "advection arrays");
-    VF_ARG(mode == 0 || mode == 1, "mode");
+    VF_ARG(mode == 0 || mode == 1 || mode == 2, "mode");
     VF_ARG((field2 == nullptr) == (out2 == nullptr), "second field and its output go together");
+    VF_ARG((size_t)fnx * fny * fnz < (1u << 31) && (size_t)(ni + 1) * (nj + 1) * (nk + 1) < (1u << 31), "grid size");
+    if (mode == 2) {
+        GridF g0{field, fnx, fny, fnz, fox, foy, foz};
+        GridF g1{field2 ? field2 : field, fnx, fny, fnz, fox, foy, foz};
+        dim3 grid((fnx + 63) / 64, (fny + 3) / 4, (fnz + 1) / 2);
+        k_advect_rk2_f32<<<grid, dim3(64, 4), 0, (hipStream_t)stream>>>(ni, nj, nk, h, 1.0f / h, dt, u, v, w, g0, out,
+                                                                         g1, out2);
+        VF_CHECK_LAUNCH();
+        return 0;
+    }
     Grid g0{field, fnx, fny, fnz, fox, foy, foz};
     Grid g1{field2 ? field2 : field, fnx, fny, fnz, fox, foy, foz};
     dim3 grid((fnx + 63) / 64, (fny + 3) / 4, fnz);

@@ -18,7 +18,7 @@ from . import api
 
 class PlumeSim:
     def __init__(self, n, phi, dt=0.01, precision=api.FP32, tol=1e-6, maxit=50, source_center_cfg=(0.5, 0.1, 0.5),
-                 source_r=0.08, jitter=0.05, alpha=0.1, beta=0.3, seed=0, mode=0):
+                 source_r=0.08, jitter=0.05, alpha=0.1, beta=0.3, seed=0, mode=2):
         self.n = n
         self.h = 1.0 / n
         self.dt = dt
