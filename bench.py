@@ -87,13 +87,13 @@ def run_gpu(n, steps, warmup, precision, phi, init_state=None, profile=False, ne
     return sim, (t1 - t0) * 1e3 / steps, its, proj_ms, prof, st
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the fused sweep from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(ROOT, "profiles", "zsweep_pmc_256.json")
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of a row-wave kernel at 256^3 from the committed rocprofv3 --pmc summary."""
+    path = os.path.join(ROOT, "profiles", "row_kernels_pmc_256.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return float(d["hbm_bytes_per_launch"])
+        return float(d["kernels"][kernel]["hbm_bytes_per_launch"])
     except Exception:
         return None
 
@@ -151,7 +151,7 @@ def main():
     log(f"scene n={n} spheres={spheres}")
 
     # plain FP32 MGPCG
-    sim_p, ms_plain, its_plain, proj_plain, _, _ = run_gpu(n, args.steps, args.warmup, api.FP32, phi)
+    sim_p, ms_plain, its_plain, proj_plain, _, st_p = run_gpu(n, args.steps, args.warmup, api.FP32, phi)
     log(f"plain: {ms_plain:.3f} ms/step its={its_plain}")
     state0 = {k: t.clone() for k, t in sim_p.state().items()}
     del sim_p
@@ -163,6 +163,7 @@ def main():
     # learned + mixed: the U-Net's guess (on the grid coarsened to 64^3) seeds every mixed solve
     net = load_network()
     learned = None
+    st_l = None
     if net is not None:
         sim_l, ms_learned, its_learned, proj_learned, _, st_l = run_gpu(n, args.steps, args.warmup, api.MIXED, phi,
                                                                         net=net)
@@ -178,8 +179,9 @@ def main():
         avg_ms = prof["total_ms"] / prof["launches"]
         achieved = prof["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(),
-                    "kernel": "k_rbgs_zsweep (fused red-black sweep, level 0)",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": traffic_from_profiles("k_rbgs_zrow<4, 0, double>") if n == 256 else None,
+                    "kernel": "k_rbgs_zrow<4, 0, double> (fused red-black sweep, level 0, mixed run)",
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": prof["launches"],
                     "algorithmic_bytes_per_launch": prof["bytes_per_launch"]}
 
@@ -239,6 +241,11 @@ def main():
         "speedup_mixed_vs_plain": round(ms_plain / ms_mixed, 3),
         "voxels_per_s": round(voxels / (head * 1e-3), 1),
         "true_residual_last": learned["true_residual_last"] if learned else st["true_residual"],
+        # last solve of each run: recursive |r|_inf/|b|_inf (the stop test) and the FP64 true residual
+        "residuals_last": {"plain": {"recursive": st_p["rel_residual"], "true_fp64": st_p["true_residual"]},
+                           "mixed": {"recursive": st["rel_residual"], "true_fp64": st["true_residual"]},
+                           "learned_mixed": ({"recursive": st_l["rel_residual"], "true_fp64": st_l["true_residual"]}
+                                             if learned else None)},
         "learned_warm_start": learned,
         "fp64_oracle_error": oracle_err,
         "roofline": roofline,

@@ -127,7 +127,8 @@ int vflip_solver_vcycle(vflip_solver* s, const float* r, float* z);
 /* y = A x with the level-0 operator (S/sparse_matrix.h:341-366). */
 int vflip_solver_matvec(vflip_solver* s, const float* x, float* y);
 
-/* Kernel timing of the dominant kernel (the fused level-0 smoothing sweep):
+/* Kernel timing of the dominant kernel (the fused level-0 smoothing sweep in
+ * its common form: the sweeps after the first of each pre/post group):
  * enable=1 records HIP events around each launch on the solver's stream;
  * _read returns the launches since enabling/the last read, their summed
  * duration and the algorithmic HBM bytes of one launch (28 B per cell). */

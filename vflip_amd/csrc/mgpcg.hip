@@ -233,7 +233,7 @@ template <typename CT> int smooth_level(vflip_solver* S, int l, bool pre, const 
             const int mode = (it == 0) ? (pre ? 1 : 2) : 0;
             const bool last = (it == sweeps - 1);
             double* dot = (dot_done && last && l == 0) ? S->partial : nullptr;
-            const bool prof = S->prof && l == 0;
+            const bool prof = S->prof && l == 0 && mode == 0;  // the dominant sweep form
             if (prof) {
                 while ((int)S->prof_ev.size() < S->prof_used + 2) {
                     hipEvent_t e;
