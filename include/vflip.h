@@ -319,6 +319,37 @@ int vflip_set_face_boundary(float* fine, int fnx, int fny, int fnz, const float*
 int vflip_to_bricks(const float* src, int nx, int ny, int nz, float* dst, void* hip_stream);
 int vflip_from_bricks(const float* src, int nx, int ny, int nz, float* dst, void* hip_stream);
 
+/* ------------------------------------------- batched subdomain solves */
+/* The nested fine subdomains' pressure solves (SubDomainSolver::pcg_projection's
+ * AMGPCGSolve call, S/sub_domain.cpp:739, once per subdomain in the reference)
+ * run together: every V-cycle and PCG step is one launch over all subdomains,
+ * the CG scalars and stop tests stay on the GPU, and a fixed number of PCG
+ * iterations is captured once into a hipGraph and replayed until every
+ * subdomain has converged. The batch references (does not own) n solvers, each
+ * FP32 (VFLIP_PREC_FP32) with its operator set (vflip_solver_set_matrix or
+ * vflip_assemble); their per-subdomain tolerance and iteration limit apply. With
+ * the solvers' default tile shapes the results equal n vflip_solver_solve calls
+ * bit for bit. Rebuild the batch after a solver's operator hierarchy is rebuilt
+ * with different options. */
+typedef struct vflip_batch vflip_batch;
+int vflip_batch_create(int n, vflip_solver* const* solvers, vflip_batch** out);
+int vflip_batch_destroy(vflip_batch* batch);
+/* PCG iterations per captured graph (1..64, default 8): one host check per replay. */
+int vflip_batch_set_graph_iterations(vflip_batch* batch, int iterations);
+/* Solves A_q x_q = rhs_q for every subdomain q (device FP32 arrays). use_graph=0
+ * issues the same launches directly. stats: n entries or NULL; ms: wall time of
+ * the batched PCG (NULL to skip). Returns 0 when all converged, 1 when one did
+ * not, <0 on error (VFLIP_ERR_BREAKDOWN if a subdomain broke down). */
+int vflip_batch_solve(vflip_batch* batch, const float* const* rhs, float* const* x, vflip_solve_stats* stats,
+                      int use_graph, double* ms);
+/* Autotuner: times the fused smoothing sweep over the whole batch for each level
+ * depth and tile shape (4/8/16 rows per tile x z chunks of 4..64 planes), keeps
+ * the fastest and rebuilds the batch's launch plan. ty/zc (may be NULL) receive
+ * the chosen shape of the first max_levels depths (0 where no level is fused).
+ * Tuned shapes change the grouping of the z.r partial sums, so results then
+ * agree with vflip_solver_solve to rounding, not bit for bit. */
+int vflip_batch_autotune(vflip_batch* batch, int* ty, int* zc, int max_levels);
+
 /* ------------------------------------------------------ learned warm start */
 /* No counterpart in the reference: a 3D U-Net (4 levels, 16/32/64/128 channels,
  * 3x3x3 convolutions, BF16 weights/activations, FP32 accumulation on the matrix

@@ -59,6 +59,11 @@ SIGNATURES = {
     "vflip_solver_set_matrix": (c_int, [P, P, P, P, P, P]),
     "vflip_solver_setup": (c_int, [P]),
     "vflip_solver_solve": (c_int, [P, P, P, ctypes.POINTER(SolveStats)]),
+    "vflip_batch_create": (c_int, [c_int, P, ctypes.POINTER(c_void_p)]),
+    "vflip_batch_destroy": (c_int, [P]),
+    "vflip_batch_set_graph_iterations": (c_int, [P, c_int]),
+    "vflip_batch_solve": (c_int, [P, P, P, P, c_int, ctypes.POINTER(ctypes.c_double)]),
+    "vflip_batch_autotune": (c_int, [P, P, P, c_int]),
     "vflip_solver_vcycle": (c_int, [P, P, P]),
     "vflip_solver_matvec": (c_int, [P, P, P]),
     "vflip_solver_get_level": (c_int, [P, c_int, P, P, P, P]),

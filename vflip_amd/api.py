@@ -203,6 +203,58 @@ def plume_force(h, dt, center, r, seed, jitter, alpha, beta, rho, temp, v):
                                   _ptr(rho), _ptr(temp), _ptr(v), _stream()), "plume_force")
 
 
+class Batch:
+    """Batched pressure solves of several FP32 subdomain solvers (vflip_batch): every
+    V-cycle / PCG step is one launch over all subdomains, the stop tests run on the
+    GPU and a fixed number of iterations replays from a captured hipGraph."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        h = ctypes.c_void_p()
+        check(lib().vflip_batch_create(len(self.solvers), ctypes.cast(arr, ctypes.c_void_p), ctypes.byref(h)),
+              "batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vflip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_graph_iterations(self, k):
+        check(lib().vflip_batch_set_graph_iterations(self.h, k), "batch_set_graph_iterations")
+
+    def autotune(self, levels=8):
+        """Times the fused sweep's tile shapes per level depth; returns [(rows, z chunk)] per depth."""
+        ty, zc = (ctypes.c_int * levels)(), (ctypes.c_int * levels)()
+        check(lib().vflip_batch_autotune(self.h, ctypes.cast(ty, ctypes.c_void_p), ctypes.cast(zc, ctypes.c_void_p),
+                                         levels), "batch_autotune")
+        return [(ty[i], zc[i]) for i in range(levels)]
+
+    def solve(self, rhs, x, graph=True):
+        """rhs, x: lists of device FP32 tensors (one per solver). Returns (status, [stats], ms)."""
+        n = len(self.solvers)
+        assert len(rhs) == n and len(x) == n
+        r = (ctypes.c_void_p * n)(*[_ptr(t, numel=s.n, name="rhs").value for t, s in zip(rhs, self.solvers)])
+        xs = (ctypes.c_void_p * n)(*[_ptr(t, numel=s.n, name="x").value for t, s in zip(x, self.solvers)])
+        st = (SolveStats * n)()
+        ms = ctypes.c_double()
+        rc = check(lib().vflip_batch_solve(self.h, ctypes.cast(r, ctypes.c_void_p), ctypes.cast(xs, ctypes.c_void_p),
+                                           ctypes.cast(st, ctypes.c_void_p), int(graph), ctypes.byref(ms)),
+                   "batch_solve")
+        out = []
+        for i in range(n):
+            d = st[i].as_dict()
+            out.append(d)
+        return rc, out, ms.value
+
+
 # ---------------------------------------------------------------------- FLIP
 class Particles:
     """Binning/transfer workspace (vflip_particles) for an ni x nj x nk grid of spacing h.

@@ -53,30 +53,47 @@ __device__ inline float offdiag_sum(const Op<CT>& op, const float* __restrict__ 
 // GS update x = (b - offdiag)/diag, diag == 0 -> 0 (S/AlgebraicMultigrid.h:54-62).
 __device__ inline float gs_value(float b, float s, float d) { return (d != 0.f) ? (b - s) / d : 0.f; }
 
+// Block coordinates and grid size. Kernel bodies take them explicitly so that a
+// batched launch (several subdomains' grids in one launch) can run a body with
+// each subdomain's own coordinates; the single-grid kernels pass blockIdx/gridDim.
+struct Blk {
+    unsigned x, y, z, nx, ny, nz;
+};
+__device__ inline Blk this_blk() { return Blk{blockIdx.x, blockIdx.y, blockIdx.z, gridDim.x, gridDim.y, gridDim.z}; }
+
 // ----------------------------------------------------------------------------
 // Small-level smoother: one red-black half sweep in place (flat launch).
 // ----------------------------------------------------------------------------
 template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_rbgs_half(Op<CT> op, const float* __restrict__ b, float* __restrict__ x,
-                                                      int colour) {
+__device__ inline void rbgs_half_body(const Op<CT>& op, const float* __restrict__ b, float* __restrict__ x, int colour,
+                                      Blk B) {
     const unsigned n = (unsigned)(op.ni * op.nj * op.nk);
-    unsigned c = blockIdx.x * kBlock + threadIdx.x;
+    unsigned c = B.x * kBlock + threadIdx.x;
     if (c >= n) return;
     int i, j, k;
     unflat32(c, op.ni, op.nj, i, j, k);
     if (((i + j + k) & 1) != colour) return;
     x[c] = gs_value(b[c], offdiag_sum(op, x, (int)c, i, j, k), op.D(c));
 }
+template <typename CT>
+__global__ __launch_bounds__(kBlock) void k_rbgs_half(Op<CT> op, const float* __restrict__ b, float* __restrict__ x,
+                                                      int colour) {
+    rbgs_half_body(op, b, x, colour, this_blk());
+}
 
 // First half sweep from a zero guess: odd cells b/diag, even cells 0.
 template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_rbgs_first(Op<CT> op, const float* __restrict__ b, float* __restrict__ x) {
+__device__ inline void rbgs_first_body(const Op<CT>& op, const float* __restrict__ b, float* __restrict__ x, Blk B) {
     const unsigned n = (unsigned)(op.ni * op.nj * op.nk);
-    unsigned c = blockIdx.x * kBlock + threadIdx.x;
+    unsigned c = B.x * kBlock + threadIdx.x;
     if (c >= n) return;
     int i, j, k;
     unflat32(c, op.ni, op.nj, i, j, k);
     x[c] = ((i + j + k) & 1) ? gs_value(b[c], 0.f, op.D(c)) : 0.f;
+}
+template <typename CT>
+__global__ __launch_bounds__(kBlock) void k_rbgs_first(Op<CT> op, const float* __restrict__ b, float* __restrict__ x) {
+    rbgs_first_body(op, b, x, this_blk());
 }
 
 // Odd half sweep fused with the preceding prolongation x += P xc: odd cells read
@@ -84,11 +101,10 @@ __global__ __launch_bounds__(kBlock) void k_rbgs_first(Op<CT> op, const float* _
 // Even cells keep the uncorrected value: the next even half sweep overwrites
 // them without reading it.
 template <typename CT>
-__global__ __launch_bounds__(kBlock) void k_rbgs_odd_prolong(Op<CT> op, const float* __restrict__ b,
-                                                             float* __restrict__ x, const float* __restrict__ xc,
-                                                             int ci, int cj) {
+__device__ inline void rbgs_odd_prolong_body(const Op<CT>& op, const float* __restrict__ b, float* __restrict__ x,
+                                             const float* __restrict__ xc, int ci, int cj, Blk B) {
     const unsigned n = (unsigned)(op.ni * op.nj * op.nk);
-    unsigned cu = blockIdx.x * kBlock + threadIdx.x;
+    unsigned cu = B.x * kBlock + threadIdx.x;
     if (cu >= n) return;
     int i, j, k;
     unflat32(cu, op.ni, op.nj, i, j, k);
@@ -104,9 +120,15 @@ __global__ __launch_bounds__(kBlock) void k_rbgs_odd_prolong(Op<CT> op, const fl
     if (k + 1 < op.nk) s += op.AZ(c) * X(c + sl, i, j, k + 1);
     x[c] = gs_value(b[c], s, op.D(c));
 }
+template <typename CT>
+__global__ __launch_bounds__(kBlock) void k_rbgs_odd_prolong(Op<CT> op, const float* __restrict__ b,
+                                                             float* __restrict__ x, const float* __restrict__ xc,
+                                                             int ci, int cj) {
+    rbgs_odd_prolong_body(op, b, x, xc, ci, cj, this_blk());
+}
 
 // x += P xc (injection), used when there are no post-smoothing sweeps.
-__global__ __launch_bounds__(kBlock) void k_prolong_add(int ni, int nj, int nk, float* __restrict__ x,
+static __global__ __launch_bounds__(kBlock) void k_prolong_add(int ni, int nj, int nk, float* __restrict__ x,
                                                         const float* __restrict__ xc, int ci, int cj) {
     const unsigned n = (unsigned)(ni * nj * nk);
     unsigned c = blockIdx.x * kBlock + threadIdx.x;
@@ -146,10 +168,10 @@ struct BlackIn {
 };
 
 template <int TY, int MODE, typename Acc>
-__global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* __restrict__ b,
-                                                     const float* __restrict__ xin, float* __restrict__ xout,
-                                                     const float* __restrict__ xc, int ci, int cj, int tiles_x,
-                                                     int tiles_y, int zchunk, double* __restrict__ dot) {
+__device__ inline void rbgs_zsweep_body(const Op<float>& op, const float* __restrict__ b,
+                                        const float* __restrict__ xin, float* __restrict__ xout,
+                                        const float* __restrict__ xc, int ci, int cj, int tiles_x, int tiles_y,
+                                        int zchunk, double* __restrict__ dot, Blk B) {
     constexpr int TX = kZTX;
     constexpr int RW = TX + 2, RH = TY + 2;
     constexpr int RHALF = RW / 2;
@@ -161,7 +183,7 @@ __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* 
     const int ni = op.ni, nj = op.nj, nk = op.nk;
     const int sl = ni * nj;
     // spatially adjacent tiles on the same XCD (dispatch round-robins XCDs)
-    const int nb = gridDim.x, L = blockIdx.x;
+    const int nb = B.nx, L = B.x;
     const int lg = (nb % 8 == 0) ? (L % 8) * (nb / 8) + (L / 8) : L;
     const int tx = lg % tiles_x, t2 = lg / tiles_x;
     const int ty = t2 % tiles_y, zc = t2 / tiles_y;
@@ -316,8 +338,15 @@ __global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* 
     }
     if (dot) {
         Acc t = block_sum(dacc);
-        if (tid == 0) dot[blockIdx.x] = (double)t;
+        if (tid == 0) dot[L] = (double)t;
     }
+}
+template <int TY, int MODE, typename Acc>
+__global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* __restrict__ b,
+                                                     const float* __restrict__ xin, float* __restrict__ xout,
+                                                     const float* __restrict__ xc, int ci, int cj, int tiles_x,
+                                                     int tiles_y, int zchunk, double* __restrict__ dot) {
+    rbgs_zsweep_body<TY, MODE, Acc>(op, b, xin, xout, xc, ci, cj, tiles_x, tiles_y, zchunk, dot, this_blk());
 }
 
 // ----------------------------------------------------------------------------
@@ -601,13 +630,13 @@ __global__ __launch_bounds__(1024) void k_rbgs_zrow(Op<float> op, const float* _
 // a lane shuffle, the y/z pairs through LDS.
 // ----------------------------------------------------------------------------
 template <typename CT>
-__global__ __launch_bounds__(256) void k_residual_restrict(Op<CT> op, const float* __restrict__ b,
-                                                           const float* __restrict__ x, float* __restrict__ bc,
-                                                           int ci, int cj, int ck) {
+__device__ inline void residual_restrict_body(const Op<CT>& op, const float* __restrict__ b,
+                                              const float* __restrict__ x, float* __restrict__ bc, int ci, int cj,
+                                              int ck, Blk B) {
     __shared__ float part[4][32];
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 2 + threadIdx.y;
-    const int k = blockIdx.z * 2 + threadIdx.z;
+    const int i = B.x * 64 + threadIdx.x;
+    const int j = B.y * 2 + threadIdx.y;
+    const int k = B.z * 2 + threadIdx.z;
     float r = 0.f;
     if (i < op.ni && j < op.nj && k < op.nk) {
         const int c = (k * op.nj + j) * op.ni + i;
@@ -619,10 +648,16 @@ __global__ __launch_bounds__(256) void k_residual_restrict(Op<CT> op, const floa
     __syncthreads();
     if (row == 0 && (threadIdx.x & 1) == 0) {
         const int m = threadIdx.x >> 1;
-        const int I = blockIdx.x * 32 + m, J = blockIdx.y, K = blockIdx.z;
+        const int I = B.x * 32 + m, J = B.y, K = B.z;
         if (I < ci && J < cj && K < ck)
             bc[(K * cj + J) * ci + I] = (part[0][m] + part[1][m]) + (part[2][m] + part[3][m]);
     }
+}
+template <typename CT>
+__global__ __launch_bounds__(256) void k_residual_restrict(Op<CT> op, const float* __restrict__ b,
+                                                           const float* __restrict__ x, float* __restrict__ bc,
+                                                           int ci, int cj, int ck) {
+    residual_restrict_body(op, b, x, bc, ci, cj, ck, this_blk());
 }
 
 // ----------------------------------------------------------------------------
@@ -725,27 +760,35 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
 // y = tab^T b in two passes (64-column chunks, then a fixed-order sum of the
 // chunk partials): deterministic and spread over the whole GPU.
 constexpr int kCoarseChunk = 64;
-__global__ __launch_bounds__(kBlock) void k_coarse_apply_partial(const float* __restrict__ tab,
-                                                                 const float* __restrict__ b,
-                                                                 float* __restrict__ part, int n) {
+__device__ inline void coarse_apply_partial_body(const float* __restrict__ tab, const float* __restrict__ b,
+                                                 float* __restrict__ part, int n, Blk B) {
     __shared__ float bs[kCoarseChunk];
-    const int r = blockIdx.x * kBlock + threadIdx.x;
-    const int c0 = blockIdx.y * kCoarseChunk;
+    const int r = B.x * kBlock + threadIdx.x;
+    const int c0 = B.y * kCoarseChunk;
     const int cn = min(kCoarseChunk, n - c0);
     if (threadIdx.x < cn) bs[threadIdx.x] = b[c0 + threadIdx.x];
     __syncthreads();
     if (r >= n) return;
     float acc = 0.f;
     for (int q = 0; q < cn; ++q) acc += tab[(size_t)(c0 + q) * n + r] * bs[q];
-    part[(size_t)blockIdx.y * n + r] = acc;
+    part[(size_t)B.y * n + r] = acc;
 }
-__global__ __launch_bounds__(kBlock) void k_coarse_apply_reduce(const float* __restrict__ part, float* __restrict__ y,
-                                                                int n, int nchunks) {
-    const int r = blockIdx.x * kBlock + threadIdx.x;
+static __global__ __launch_bounds__(kBlock) void k_coarse_apply_partial(const float* __restrict__ tab,
+                                                                 const float* __restrict__ b,
+                                                                 float* __restrict__ part, int n) {
+    coarse_apply_partial_body(tab, b, part, n, this_blk());
+}
+__device__ inline void coarse_apply_reduce_body(const float* __restrict__ part, float* __restrict__ y, int n,
+                                                int nchunks, Blk B) {
+    const int r = B.x * kBlock + threadIdx.x;
     if (r >= n) return;
     float s = 0.f;
     for (int q = 0; q < nchunks; ++q) s += part[(size_t)q * n + r];
     y[r] = s;
+}
+static __global__ __launch_bounds__(kBlock) void k_coarse_apply_reduce(const float* __restrict__ part, float* __restrict__ y,
+                                                                int n, int nchunks) {
+    coarse_apply_reduce_body(part, y, n, nchunks, this_blk());
 }
 
 // ----------------------------------------------------------------------------
@@ -755,7 +798,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_apply_reduce(const float* __r
 //   diag(I)          = leak(I) + sum of the 6 coarse couplings' magnitudes
 // (equal to 1/16 * sum_{i,k in I} A(i,k); written this way no term cancels).
 // ----------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_coarsen(int ni, int nj, int nk, const float* __restrict__ ax,
+static __global__ __launch_bounds__(kBlock) void k_coarsen(int ni, int nj, int nk, const float* __restrict__ ax,
                                                     const float* __restrict__ ay, const float* __restrict__ az,
                                                     const float* __restrict__ leak, int ci, int cj, int ck,
                                                     float* __restrict__ dc, float* __restrict__ axc,
@@ -803,7 +846,7 @@ __global__ __launch_bounds__(kBlock) void k_coarsen(int ni, int nj, int nk, cons
 }
 
 // leak = diag - sum|couplings| in FP64 (for operators given without it).
-__global__ __launch_bounds__(kBlock) void k_leak_from_diag(int ni, int nj, int nk, const float* __restrict__ d,
+static __global__ __launch_bounds__(kBlock) void k_leak_from_diag(int ni, int nj, int nk, const float* __restrict__ d,
                                                            const float* __restrict__ ax, const float* __restrict__ ay,
                                                            const float* __restrict__ az, float* __restrict__ leak) {
     const unsigned n = (unsigned)(ni * nj * nk);
@@ -824,7 +867,7 @@ __global__ __launch_bounds__(kBlock) void k_leak_from_diag(int ni, int nj, int n
 }
 
 // FP16 copy of a level scaled by 1/scale.
-__global__ __launch_bounds__(kBlock) void k_to_half(unsigned n, const float* __restrict__ src, __half* __restrict__ dst,
+static __global__ __launch_bounds__(kBlock) void k_to_half(unsigned n, const float* __restrict__ src, __half* __restrict__ dst,
                                                     float inv_scale) {
     unsigned c = blockIdx.x * kBlock + threadIdx.x;
     if (c < n) dst[c] = __float2half(src[c] * inv_scale);
@@ -846,7 +889,7 @@ struct Scalars {
 
 // Partial sums of x.(b + r) for the energy of an initial guess:
 // E(x) = x'Ax/2 - b'x = -(x.b + x.r)/2 with r = b - A x.
-__global__ __launch_bounds__(kBlock) void k_energy_dots(unsigned n, const double* __restrict__ x,
+static __global__ __launch_bounds__(kBlock) void k_energy_dots(unsigned n, const double* __restrict__ x,
                                                        const double* __restrict__ b64, const float* __restrict__ b32,
                                                        const float* __restrict__ r, double* __restrict__ partial) {
     double s = 0;
@@ -858,10 +901,10 @@ __global__ __launch_bounds__(kBlock) void k_energy_dots(unsigned n, const double
 
 // q = A s and sum(s.q); 3D launch (64 x 4 threads, kMatvecKZ planes per block).
 template <typename CT, typename Acc>
-__global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* __restrict__ s, float* __restrict__ q,
-                                                       double* __restrict__ partial) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-    const int kb = blockIdx.z * kMatvecKZ;
+__device__ inline void matvec_dot_body(const Op<CT>& op, const float* __restrict__ s, float* __restrict__ q,
+                                       double* __restrict__ partial, Blk B) {
+    const int i = B.x * 64 + threadIdx.x, j = B.y * 4 + threadIdx.y;
+    const int kb = B.z * kMatvecKZ;
     const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
     Acc acc = 0;
     if (i < ni && j < nj) {
@@ -881,8 +924,12 @@ __global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* _
         }
     }
     Acc t = block_sum(acc);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-        partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (double)t;
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[(B.z * B.ny + B.y) * B.nx + B.x] = (double)t;
+}
+template <typename CT, typename Acc>
+__global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* __restrict__ s, float* __restrict__ q,
+                                                       double* __restrict__ partial) {
+    matvec_dot_body<CT, Acc>(op, s, q, partial, this_blk());
 }
 
 // Row-wave 7-point stencil for levels whose rows are 64*V cells (V = 2 or 4),
@@ -1052,7 +1099,7 @@ __global__ __launch_bounds__(1024) void k_stencil_row(Op<float> op, const float*
 // the right-hand side come in FP64 when the assembly provided them (d64, b64),
 // else from the FP32 arrays. x is the FP64 iterate. Writes r rounded to FP32
 // and partial max|r|.
-__global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op, const double* __restrict__ d64,
+static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op, const double* __restrict__ d64,
                                                             const double* __restrict__ b64,
                                                             const float* __restrict__ b32,
                                                             const double* __restrict__ x, float* __restrict__ r,
@@ -1082,7 +1129,7 @@ __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op, const 
 }
 
 // x64 += alpha s; r -= alpha q; partial max|r| (mixed mode, FP64 iterate)
-__global__ __launch_bounds__(kBlock) void k_update_x64r(unsigned n, const Scalars* __restrict__ sc,
+static __global__ __launch_bounds__(kBlock) void k_update_x64r(unsigned n, const Scalars* __restrict__ sc,
                                                         double* __restrict__ x, const float* __restrict__ s,
                                                         float* __restrict__ r, const float* __restrict__ q,
                                                         double* __restrict__ partial) {
@@ -1098,7 +1145,7 @@ __global__ __launch_bounds__(kBlock) void k_update_x64r(unsigned n, const Scalar
     if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
 }
 
-__global__ __launch_bounds__(kBlock) void k_absmax64(unsigned n, const double* __restrict__ a,
+static __global__ __launch_bounds__(kBlock) void k_absmax64(unsigned n, const double* __restrict__ a,
                                                      double* __restrict__ partial) {
     double m = 0;
     for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) m = fmax(m, fabs(a[c]));
@@ -1106,17 +1153,17 @@ __global__ __launch_bounds__(kBlock) void k_absmax64(unsigned n, const double* _
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kBlock) void k_cast_f64_f32(unsigned n, const double* __restrict__ a,
+static __global__ __launch_bounds__(kBlock) void k_cast_f64_f32(unsigned n, const double* __restrict__ a,
                                                          float* __restrict__ o) {
     unsigned c = blockIdx.x * kBlock + threadIdx.x;
     if (c < n) o[c] = (float)a[c];
 }
-__global__ __launch_bounds__(kBlock) void k_cast_f32_f64(unsigned n, const float* __restrict__ a,
+static __global__ __launch_bounds__(kBlock) void k_cast_f32_f64(unsigned n, const float* __restrict__ a,
                                                          double* __restrict__ o) {
     unsigned c = blockIdx.x * kBlock + threadIdx.x;
     if (c < n) o[c] = (double)a[c];
 }
-__global__ __launch_bounds__(kBlock) void k_fill64(unsigned n, double* __restrict__ a, double v) {
+static __global__ __launch_bounds__(kBlock) void k_fill64(unsigned n, double* __restrict__ a, double v) {
     unsigned c = blockIdx.x * kBlock + threadIdx.x;
     if (c < n) a[c] = v;
 }
@@ -1151,45 +1198,63 @@ __global__ __launch_bounds__(kBlock) void k_true_residual(Op<CT> op, const float
 }
 
 template <typename Acc>
+__device__ inline void dot_body(unsigned n, const float* __restrict__ a, const float* __restrict__ b,
+                                double* __restrict__ partial, Blk B) {
+    Acc acc = 0;
+    for (unsigned c = B.x * kBlock + threadIdx.x; c < n; c += B.nx * kBlock) acc += (Acc)a[c] * (Acc)b[c];
+    Acc t = block_sum(acc);
+    if (threadIdx.x == 0) partial[B.x] = (double)t;
+}
+template <typename Acc>
 __global__ __launch_bounds__(kBlock) void k_dot(unsigned n, const float* __restrict__ a, const float* __restrict__ b,
                                                 double* __restrict__ partial) {
-    Acc acc = 0;
-    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) acc += (Acc)a[c] * (Acc)b[c];
-    Acc t = block_sum(acc);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
+    dot_body<Acc>(n, a, b, partial, this_blk());
 }
 
-__global__ __launch_bounds__(kBlock) void k_absmax(unsigned n, const float* __restrict__ a, double* __restrict__ partial) {
+__device__ inline void absmax_body(unsigned n, const float* __restrict__ a, double* __restrict__ partial, Blk B) {
     float m = 0.f;
-    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) m = fmaxf(m, fabsf(a[c]));
+    for (unsigned c = B.x * kBlock + threadIdx.x; c < n; c += B.nx * kBlock) m = fmaxf(m, fabsf(a[c]));
     float t = block_max(m);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
+    if (threadIdx.x == 0) partial[B.x] = (double)t;
+}
+static __global__ __launch_bounds__(kBlock) void k_absmax(unsigned n, const float* __restrict__ a, double* __restrict__ partial) {
+    absmax_body(n, a, partial, this_blk());
 }
 
 // x += alpha s; r -= alpha q; partial max|r| (S/AlgebraicMultigrid.h:259-262)
 template <typename Acc>
-__global__ __launch_bounds__(kBlock) void k_update_xr(unsigned n, const Scalars* __restrict__ sc, float* __restrict__ x,
-                                                      const float* __restrict__ s, float* __restrict__ r,
-                                                      const float* __restrict__ q, double* __restrict__ partial) {
+__device__ inline void update_xr_body(unsigned n, const Scalars* __restrict__ sc, float* __restrict__ x,
+                                      const float* __restrict__ s, float* __restrict__ r, const float* __restrict__ q,
+                                      double* __restrict__ partial, Blk B) {
     const Acc alpha = (Acc)sc->alpha;
     float m = 0.f;
-    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) {
+    for (unsigned c = B.x * kBlock + threadIdx.x; c < n; c += B.nx * kBlock) {
         x[c] = (float)((Acc)x[c] + alpha * (Acc)s[c]);
         const float rv = (float)((Acc)r[c] - alpha * (Acc)q[c]);
         r[c] = rv;
         m = fmaxf(m, fabsf(rv));
     }
     float t = block_max(m);
-    if (threadIdx.x == 0) partial[blockIdx.x] = (double)t;
+    if (threadIdx.x == 0) partial[B.x] = (double)t;
+}
+template <typename Acc>
+__global__ __launch_bounds__(kBlock) void k_update_xr(unsigned n, const Scalars* __restrict__ sc, float* __restrict__ x,
+                                                      const float* __restrict__ s, float* __restrict__ r,
+                                                      const float* __restrict__ q, double* __restrict__ partial) {
+    update_xr_body<Acc>(n, sc, x, s, r, q, partial, this_blk());
 }
 
 // s = z + beta s (S/AlgebraicMultigrid.h:283)
 template <typename Acc>
+__device__ inline void update_s_body(unsigned n, const Scalars* __restrict__ sc, float* __restrict__ s,
+                                     const float* __restrict__ z, Blk B) {
+    const Acc beta = (Acc)sc->beta;
+    for (unsigned c = B.x * kBlock + threadIdx.x; c < n; c += B.nx * kBlock) s[c] = (float)((Acc)z[c] + beta * (Acc)s[c]);
+}
+template <typename Acc>
 __global__ __launch_bounds__(kBlock) void k_update_s(unsigned n, const Scalars* __restrict__ sc, float* __restrict__ s,
                                                      const float* __restrict__ z) {
-    const Acc beta = (Acc)sc->beta;
-    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock)
-        s[c] = (float)((Acc)z[c] + beta * (Acc)s[c]);
+    update_s_body<Acc>(n, sc, s, z, this_blk());
 }
 
 template <typename Acc> __device__ inline double reduce_partials_sum(const double* p, int np) {
@@ -1205,8 +1270,7 @@ __device__ inline double reduce_partials_max(const double* p, int np) {
 
 // what: 0 rho; 1 sigma -> alpha; 2 res; 3 rho_new -> beta; 4 |b| and tolerance; 5 max into true_res
 template <typename Acc>
-__global__ __launch_bounds__(kBlock) void k_finalize(Scalars* sc, const double* __restrict__ partial, int np, int what,
-                                                     double tol) {
+__device__ inline void finalize_body(Scalars* sc, const double* __restrict__ partial, int np, int what, double tol) {
     const double v = (what == 2 || what == 4 || what == 5) ? reduce_partials_max(partial, np)
                                                            : reduce_partials_sum<Acc>(partial, np);
     if (threadIdx.x != 0) return;
@@ -1232,14 +1296,25 @@ __global__ __launch_bounds__(kBlock) void k_finalize(Scalars* sc, const double* 
         sc->true_res = v;
     }
 }
+template <typename Acc>
+__global__ __launch_bounds__(kBlock) void k_finalize(Scalars* sc, const double* __restrict__ partial, int np, int what,
+                                                     double tol) {
+    finalize_body<Acc>(sc, partial, np, what, tol);
+}
 
-__global__ __launch_bounds__(kBlock) void k_fill(unsigned n, float* __restrict__ a, float v) {
-    unsigned c = blockIdx.x * kBlock + threadIdx.x;
+__device__ inline void fill_body(unsigned n, float* __restrict__ a, float v, Blk B) {
+    unsigned c = B.x * kBlock + threadIdx.x;
     if (c < n) a[c] = v;
 }
-__global__ __launch_bounds__(kBlock) void k_copy(unsigned n, float* __restrict__ dst, const float* __restrict__ src) {
-    unsigned c = blockIdx.x * kBlock + threadIdx.x;
+static __global__ __launch_bounds__(kBlock) void k_fill(unsigned n, float* __restrict__ a, float v) {
+    fill_body(n, a, v, this_blk());
+}
+__device__ inline void copy_body(unsigned n, float* __restrict__ dst, const float* __restrict__ src, Blk B) {
+    unsigned c = B.x * kBlock + threadIdx.x;
     if (c < n) dst[c] = src[c];
+}
+static __global__ __launch_bounds__(kBlock) void k_copy(unsigned n, float* __restrict__ dst, const float* __restrict__ src) {
+    copy_body(n, dst, src, this_blk());
 }
 
 }  // namespace vflip

@@ -24,56 +24,9 @@
 #include <type_traits>
 
 #include "mg_kernels.h"
+#include "solver_internal.h"
 
 using namespace vflip;
-
-struct LevelBuf {
-    int ni = 0, nj = 0, nk = 0;
-    size_t n = 0;
-    float *d = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *leak = nullptr;
-    __half *hd = nullptr, *hax = nullptr, *hay = nullptr, *haz = nullptr;
-    float hscale = 1.f;
-    bool half = false;
-    float *x = nullptr, *xt = nullptr, *b = nullptr;
-    void* vec_alloc = nullptr;
-};
-
-struct vflip_solver {
-    int ni, nj, nk;
-    size_t n;
-    vflip_solve_opts opts;
-    hipStream_t stream = nullptr;
-    std::vector<LevelBuf> L;
-    float *r = nullptr, *z = nullptr, *zt = nullptr, *s = nullptr, *q = nullptr, *xw = nullptr, *b32 = nullptr;
-    void* pcg_alloc = nullptr;
-    // FP64 iterate / level-0 diagonal / right-hand side (mixed-precision refinement)
-    double *x64 = nullptr, *d64 = nullptr, *b64 = nullptr;
-    bool d64_valid = false;  // d64 holds the assembled FP64 diagonal of the current operator
-    bool b64_valid = false;  // b64 holds the assembled FP64 right-hand side
-    double* partial = nullptr;
-    int np_partial = 0;  // partial buffer capacity
-    int np_last = 0;     // partials written by the last reduction
-    Scalars* sc = nullptr;
-    Scalars* h_sc = nullptr;  // pinned
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    bool setup_done = false;
-    int zsweep_min_n = 32768;  // levels at least this large use the fused z-marching sweep
-    bool use_zrow = true;      // row-wave form of the fused sweep where the row length allows it
-    int cus = 256;             // compute units of the device (launch sizing)
-    // coarsest-level operator RBGS^coarse_sweeps tabulated as a dense matrix
-    float* ctab = nullptr;
-    float* cpart = nullptr;
-    int ctab_sweeps = -1;
-    bool use_ctab = true;
-    // operator-change detection between assemblies
-    int* d_changed = nullptr;
-    int* h_changed = nullptr;
-    bool check_changed = false;
-    // level-0 smoother timing (profiling)
-    bool prof = false;
-    std::vector<hipEvent_t> prof_ev;
-    int prof_used = 0;
-};
 
 namespace {
 
@@ -103,44 +56,6 @@ dim3 grid3(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, n
 dim3 grid3z(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, (nk + kMatvecKZ - 1) / kMatvecKZ); }
 int nblocks(dim3 g) { return (int)(g.x * g.y * g.z); }
 
-// Fused z-marching sweep geometry for a level. Rows of 128 or 256 cells use the
-// row-wave kernel (one 1024-thread workgroup per compute unit, y tiles of kZOut
-// rows, z chunks sized so the grid is one wave of workgroups); other shapes use
-// the 64 x TY tile kernel.
-struct ZGeom {
-    int ty, tiles_x, tiles_y, zchunk, blocks;
-    int v;  // cells per lane of the row-wave kernel, 0 for the tile kernel
-};
-ZGeom zgeom(const LevelBuf& lv, bool use_zrow, int cus) {
-    ZGeom g;
-    g.v = (use_zrow && (lv.ni == 128 || lv.ni == 256) && lv.nj >= 8) ? lv.ni / 64 : 0;
-    if (g.v) {
-        g.ty = kZOut;
-        g.tiles_x = 1;
-        g.tiles_y = (lv.nj + kZOut - 1) / kZOut;
-        const int chunks = std::max(1, cus / g.tiles_y);
-        g.zchunk = std::max(2, (lv.nk + chunks - 1) / chunks);
-        g.blocks = g.tiles_y * ((lv.nk + g.zchunk - 1) / g.zchunk);
-        return g;
-    }
-    g.ty = lv.nj >= 128 ? 8 : 4;
-    if (const char* e = getenv("VFLIP_ZTY")) {
-        const int t = atoi(e);
-        if ((t == 4 || t == 8 || t == 16) && lv.nj >= t) g.ty = t;
-    }
-    g.tiles_x = (lv.ni + kZTX - 1) / kZTX;
-    g.tiles_y = (lv.nj + g.ty - 1) / g.ty;
-    const int tiles = g.tiles_x * g.tiles_y;
-    // about 2048 workgroups (8 per CU), z chunks of at least 4 planes
-    int zc = std::max(4, (int)((long long)lv.nk * tiles / 2048));
-    const char* env = getenv("VFLIP_ZCHUNK");  // applies to the finest level only (tuning)
-    if (env && atoi(env) > 0 && lv.nk >= 256) zc = atoi(env);
-    zc = std::min(zc, lv.nk);
-    g.zchunk = zc;
-    g.blocks = tiles * ((lv.nk + zc - 1) / zc);
-    return g;
-}
-ZGeom zgeom(const vflip_solver* S, const LevelBuf& lv) { return zgeom(lv, S->use_zrow, S->cus); }
 
 // Row-wave stencil geometry (matvec, residual restriction): y tiles of kZRows
 // rows, z chunks (even, for the restriction) sized so the grid is one wave of
@@ -205,9 +120,6 @@ void launch_zsweep(vflip_solver* S, const LevelBuf& lv, const float* b, const fl
     else launch_zsweep_mode<0, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
 }
 
-bool use_zsweep(const vflip_solver* S, const LevelBuf& lv) {
-    return !lv.half && (int)lv.n >= S->zsweep_min_n && lv.ni >= 16 && lv.nj >= 4 && lv.nk >= 4;
-}
 
 // Pre- or post-smoothing of level l. With dot_done (level 0, post) the last
 // fused sweep also leaves z.r partials in S->partial.
@@ -363,13 +275,13 @@ int check_dims(int ni, int nj, int nk) {
 
 // Reductions record how many partials they wrote.
 void launch_absmax(vflip_solver* S, unsigned n, const float* a) {
-    k_absmax<<<kVecBlocks, kBlock, 0, S->stream>>>(n, a, S->partial);
-    S->np_last = kVecBlocks;
+    k_absmax<<<S->vblocks, kBlock, 0, S->stream>>>(n, a, S->partial);
+    S->np_last = S->vblocks;
 }
 void launch_dot(vflip_solver* S, unsigned n, const float* a, const float* b) {
-    if (mixed(S)) k_dot<double><<<kVecBlocks, kBlock, 0, S->stream>>>(n, a, b, S->partial);
-    else k_dot<float><<<kVecBlocks, kBlock, 0, S->stream>>>(n, a, b, S->partial);
-    S->np_last = kVecBlocks;
+    if (mixed(S)) k_dot<double><<<S->vblocks, kBlock, 0, S->stream>>>(n, a, b, S->partial);
+    else k_dot<float><<<S->vblocks, kBlock, 0, S->stream>>>(n, a, b, S->partial);
+    S->np_last = S->vblocks;
 }
 void launch_true_residual(vflip_solver* S, const float* b, const float* x, float* r) {
     const LevelBuf& l0 = S->L[0];
@@ -395,16 +307,16 @@ void launch_matvec_dot(vflip_solver* S, const float* s, float* q) {
 }
 void launch_update_xr(vflip_solver* S) {
     if (mixed(S))
-        k_update_xr<double><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->xw, S->s, S->r, S->q,
+        k_update_xr<double><<<S->vblocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->xw, S->s, S->r, S->q,
                                                                   S->partial);
     else
-        k_update_xr<float><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->xw, S->s, S->r, S->q,
+        k_update_xr<float><<<S->vblocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->xw, S->s, S->r, S->q,
                                                                  S->partial);
-    S->np_last = kVecBlocks;
+    S->np_last = S->vblocks;
 }
 void launch_update_s(vflip_solver* S) {
-    if (mixed(S)) k_update_s<double><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->s, S->z);
-    else k_update_s<float><<<kVecBlocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->s, S->z);
+    if (mixed(S)) k_update_s<double><<<S->vblocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->s, S->z);
+    else k_update_s<float><<<S->vblocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->s, S->z);
 }
 
 // z = M^-1 r followed by rho = z.r (fused into the last smoothing sweep when possible).
@@ -489,6 +401,8 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
     if (opts) S->opts = *opts;
     if (const char* e = getenv("VFLIP_ZSWEEP_MIN_N")) S->zsweep_min_n = atoi(e);
     if (const char* e = getenv("VFLIP_ZROW")) S->use_zrow = atoi(e) != 0;
+    // grid-stride vector kernels: about 1024 cells per workgroup, at most kVecBlocks partials
+    S->vblocks = (int)std::min<size_t>(kVecBlocks, std::max<size_t>(8, (S->n + 1023) / 1024));
     {
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
@@ -796,8 +710,8 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
     VF_HIP(hipEventRecord(S->e0, S->stream));
     // |b|_inf and the stop threshold (S/AlgebraicMultigrid.h:212-233)
     if (rhs64) {
-        k_absmax64<<<kVecBlocks, kBlock, 0, S->stream>>>(n, rhs64, S->partial);
-        S->np_last = kVecBlocks;
+        k_absmax64<<<S->vblocks, kBlock, 0, S->stream>>>(n, rhs64, S->partial);
+        S->np_last = S->vblocks;
     } else {
         launch_absmax(S, n, rhs32);
     }
@@ -814,8 +728,8 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
         // The PCG judges the guess: it minimises E(x) = x'Ax/2 - b'x, and
         // E(x0) < E(0) = 0 exactly when x0 is closer to the solution than zero
         // in the A-norm. A guess that is not is dropped (restart from zero).
-        k_energy_dots<<<kVecBlocks, kBlock, 0, S->stream>>>(n, S->x64, mx ? rhs64 : nullptr, rhs32, S->r, S->partial);
-        S->np_last = kVecBlocks;
+        k_energy_dots<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->x64, mx ? rhs64 : nullptr, rhs32, S->r, S->partial);
+        S->np_last = S->vblocks;
         VF_CHECK_LAUNCH();
         if ((rc = finalize(S, 6))) return rc;
     } else {
@@ -862,9 +776,9 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
                 VF_CHECK_LAUNCH();
                 if ((rc = finalize(S, 1))) return rc;
                 if (mx) {
-                    k_update_x64r<<<kVecBlocks, kBlock, 0, S->stream>>>(n, S->sc, S->x64, S->s, S->r, S->q,
+                    k_update_x64r<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->sc, S->x64, S->s, S->r, S->q,
                                                                         S->partial);
-                    S->np_last = kVecBlocks;
+                    S->np_last = S->vblocks;
                 } else {
                     launch_update_xr(S);
                 }
@@ -925,6 +839,33 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
 }
 
 }  // namespace
+
+// Outputs of an FP32 solve run by the batched subdomain solver (its PCG left
+// x in S->xw and the scalars in S->sc): x32 = x, then the FP64 true residual
+// and the stats, as vflip_solver_solve reports them.
+int solver_finish(vflip_solver* S, const float* rhs32, float* x32, int iterations, int status,
+                  vflip_solve_stats* st) {
+    int rc;
+    if ((rc = ensure_f64(S))) return rc;
+    const unsigned n = (unsigned)S->n;
+    if (x32) VF_HIP(hipMemcpyAsync(x32, S->xw, n * sizeof(float), hipMemcpyDeviceToDevice, S->stream));
+    k_cast_f32_f64<<<grid_for(n), kBlock, 0, S->stream>>>(n, S->xw, S->x64);
+    launch_true_residual64(S, nullptr, rhs32, S->x64, nullptr);
+    VF_CHECK_LAUNCH();
+    if ((rc = finalize(S, 5))) return rc;
+    if ((rc = fetch_scalars(S))) return rc;
+    if (st) {
+        vflip_solve_stats stats{};
+        const double bnorm = S->h_sc->bnorm;
+        stats.iterations = iterations;
+        stats.converged = status == VFLIP_OK;
+        stats.residual = bnorm > 0 ? S->h_sc->res : 0.0;
+        stats.rel_residual = bnorm > 0 ? S->h_sc->res / bnorm : 0.0;
+        stats.true_residual = bnorm > 0 ? S->h_sc->true_res / bnorm : 0.0;
+        *st = stats;
+    }
+    return 0;
+}
 
 extern "C" {
 

@@ -1,0 +1,114 @@
+// Internal solver state shared by the solver (mgpcg.hip) and the batched
+// subdomain solver (batch.hip). Not part of the C ABI.
+#pragma once
+
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
+#include "mg_kernels.h"
+
+using vflip::kZOut;
+using vflip::kZTX;
+using vflip::kVecBlocks;
+using vflip::Op;
+using vflip::Scalars;
+
+struct LevelBuf {
+    int ni = 0, nj = 0, nk = 0;
+    size_t n = 0;
+    float *d = nullptr, *ax = nullptr, *ay = nullptr, *az = nullptr, *leak = nullptr;
+    __half *hd = nullptr, *hax = nullptr, *hay = nullptr, *haz = nullptr;
+    float hscale = 1.f;
+    bool half = false;
+    float *x = nullptr, *xt = nullptr, *b = nullptr;
+    void* vec_alloc = nullptr;
+};
+
+struct vflip_solver {
+    int ni, nj, nk;
+    size_t n;
+    vflip_solve_opts opts;
+    hipStream_t stream = nullptr;
+    std::vector<LevelBuf> L;
+    float *r = nullptr, *z = nullptr, *zt = nullptr, *s = nullptr, *q = nullptr, *xw = nullptr, *b32 = nullptr;
+    void* pcg_alloc = nullptr;
+    // FP64 iterate / level-0 diagonal / right-hand side (mixed-precision refinement)
+    double *x64 = nullptr, *d64 = nullptr, *b64 = nullptr;
+    bool d64_valid = false;  // d64 holds the assembled FP64 diagonal of the current operator
+    bool b64_valid = false;  // b64 holds the assembled FP64 right-hand side
+    double* partial = nullptr;
+    int np_partial = 0;  // partial buffer capacity
+    int np_last = 0;     // partials written by the last reduction
+    Scalars* sc = nullptr;
+    Scalars* h_sc = nullptr;  // pinned
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool setup_done = false;
+    int zsweep_min_n = 32768;  // levels at least this large use the fused z-marching sweep
+    bool use_zrow = true;      // row-wave form of the fused sweep where the row length allows it
+    int cus = 256;             // compute units of the device (launch sizing)
+    int vblocks = kVecBlocks;  // workgroups (partials) of the grid-stride vector kernels
+    // coarsest-level operator RBGS^coarse_sweeps tabulated as a dense matrix
+    float* ctab = nullptr;
+    float* cpart = nullptr;
+    int ctab_sweeps = -1;
+    bool use_ctab = true;
+    // operator-change detection between assemblies
+    int* d_changed = nullptr;
+    int* h_changed = nullptr;
+    bool check_changed = false;
+    // level-0 smoother timing (profiling)
+    bool prof = false;
+    std::vector<hipEvent_t> prof_ev;
+    int prof_used = 0;
+};
+
+// Fused z-marching sweep geometry for a level. Rows of 128 or 256 cells use the
+// row-wave kernel (one 1024-thread workgroup per compute unit, y tiles of kZOut
+// rows, z chunks sized so the grid is one wave of workgroups); other shapes use
+// the 64 x TY tile kernel.
+struct ZGeom {
+    int ty, tiles_x, tiles_y, zchunk, blocks;
+    int v;  // cells per lane of the row-wave kernel, 0 for the tile kernel
+};
+inline ZGeom zgeom(const LevelBuf& lv, bool use_zrow, int cus) {
+    ZGeom g;
+    g.v = (use_zrow && (lv.ni == 128 || lv.ni == 256) && lv.nj >= 8) ? lv.ni / 64 : 0;
+    if (g.v) {
+        g.ty = kZOut;
+        g.tiles_x = 1;
+        g.tiles_y = (lv.nj + kZOut - 1) / kZOut;
+        const int chunks = std::max(1, cus / g.tiles_y);
+        g.zchunk = std::max(2, (lv.nk + chunks - 1) / chunks);
+        g.blocks = g.tiles_y * ((lv.nk + g.zchunk - 1) / g.zchunk);
+        return g;
+    }
+    g.ty = lv.nj >= 128 ? 8 : 4;
+    if (const char* e = getenv("VFLIP_ZTY")) {
+        const int t = atoi(e);
+        if ((t == 4 || t == 8 || t == 16) && lv.nj >= t) g.ty = t;
+    }
+    g.tiles_x = (lv.ni + kZTX - 1) / kZTX;
+    g.tiles_y = (lv.nj + g.ty - 1) / g.ty;
+    const int tiles = g.tiles_x * g.tiles_y;
+    // about 2048 workgroups (8 per CU), z chunks of at least 4 planes
+    int zc = std::max(4, (int)((long long)lv.nk * tiles / 2048));
+    const char* env = getenv("VFLIP_ZCHUNK");  // applies to the finest level only (tuning)
+    if (env && atoi(env) > 0 && lv.nk >= 256) zc = atoi(env);
+    zc = std::min(zc, lv.nk);
+    g.zchunk = zc;
+    g.blocks = tiles * ((lv.nk + zc - 1) / zc);
+    return g;
+}
+inline ZGeom zgeom(const vflip_solver* S, const LevelBuf& lv) { return zgeom(lv, S->use_zrow, S->cus); }
+
+// Levels at least this large (and not FP16) use the fused z-marching sweep.
+inline bool use_zsweep(const vflip_solver* S, const LevelBuf& lv) {
+    return !lv.half && (int)lv.n >= S->zsweep_min_n && lv.ni >= 16 && lv.nj >= 4 && lv.nk >= 4;
+}
+
+inline Op<float> make_op_f32(const LevelBuf& l) { return Op<float>{l.d, l.ax, l.ay, l.az, 1.f, l.ni, l.nj, l.nk}; }
+
+
+// Outputs of an FP32 solve run by the batched subdomain solver (mgpcg.hip).
+int solver_finish(vflip_solver* S, const float* rhs32, float* x32, int iterations, int status, vflip_solve_stats* st);

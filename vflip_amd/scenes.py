@@ -105,6 +105,22 @@ def projection_problem(n, seed=0, spheres=None, open_top=True, vel_scale=1.0):
     return dict(n=n, h=np.float32(h), phi=phi, u=u, v=v, w=w, us=us, vs=vs, ws=ws, spheres=spheres)
 
 
+def box_problem(ni, nj, nk, seed, h=None, vel_scale=1.0):
+    """Projection inputs on an ni x nj x nk box: nodal solid SDF (walls one cell in,
+    open top, one sphere at a seeded position), seeded random MAC velocity, zero
+    solid velocity."""
+    h = 1.0 / max(ni, nj, nk) if h is None else h
+    u01 = uniform01(seed, 11, np.arange(4))
+    c = (ni * h * (0.3 + 0.4 * u01[0]), nj * h * (0.3 + 0.4 * u01[1]), nk * h * (0.3 + 0.4 * u01[2]))
+    phi = np.minimum(box_phi(ni, nj, nk, h), sphere_phi(ni, nj, nk, h, c, (0.1 + 0.1 * u01[3]) * min(ni, nj, nk) * h))
+    shapes = [(nk, nj, ni + 1), (nk, nj + 1, ni), (nk + 1, nj, ni)]
+    vel = [((uniform01(seed, 100 + q, np.arange(int(np.prod(shp)))) * 2 - 1) * vel_scale).reshape(shp).astype(np.float32)
+           for q, shp in enumerate(shapes)]
+    zeros = [np.zeros(shp, np.float32) for shp in shapes]
+    return dict(h=np.float32(h), phi=phi.astype(np.float32), u=vel[0], v=vel[1], w=vel[2], us=zeros[0], vs=zeros[1],
+                ws=zeros[2])
+
+
 def block_particles(n, ppc, seed, lo=(0.0, 0.0, 0.0), hi=(0.4, 0.6, 1.0), jitter=1.0):
     """ppc jittered particles in every cell whose centre lies in the config-space
     block [lo, hi] of an n^3 grid (the dam-break fill: [0,.4]x[0,.6]x[0,1]).
