@@ -169,6 +169,15 @@ int vflip_assemble(vflip_solver* s, int variant, float h, float dt, const float*
                    const float* w, const float* uw, const float* vw, const float* ww,
                    const float* us, const float* vs, const float* ws, float* rhs);
 
+/* vflip_assemble (coarse variant) with a free surface: cells where liquid_phi >= 0
+ * are air, a Dirichlet p = 0 boundary like the domain ring (empty rows), and
+ * liquid cells couple only to liquid neighbours (first-order free surface; no
+ * reference counterpart, the reference's smoke assembly is the all-liquid case).
+ * liquid_phi: device ni*nj*nk (vflip_liquid_phi). */
+int vflip_assemble_liquid(vflip_solver* s, float h, float dt, const float* u, const float* v, const float* w,
+                          const float* uw, const float* vw, const float* ww, const float* us, const float* vs,
+                          const float* ws, const float* liquid_phi, float* rhs);
+
 /* Velocity update u -= grad p (S/Smoke_solver3D.cpp:1126-1200; variant SUB with
  * dt and wider face ranges: S/sub_domain.cpp:745-819). p: ni*nj*nk FP32. */
 int vflip_grad_apply(int ni, int nj, int nk, int variant, float h, float dt, const float* p,
@@ -286,6 +295,14 @@ int vflip_g2p(int ni, int nj, int nk, float h, int np, const float* pos, float* 
 int vflip_advect_particles(int ni, int nj, int nk, float h, float dt, int variant, float bx, float by, float bz,
                            int np, float* pos, const float* u, const float* v, const float* w,
                            const float* solid_phi, void* hip_stream);
+
+/* Particle level set of the liquid for the free-surface pressure boundary (the
+ * reference has no liquid solver): phi at every cell centre (i h, j h, k h) =
+ * min |centre - p| over the particles in the 3x3x3 cells around it (vflip_hash's
+ * cell_start/order, clamp 0), minus radius; 3h - radius where none is that close.
+ * Liquid cells are phi < 0. Exact (float distances). Asynchronous. */
+int vflip_liquid_phi(int ni, int nj, int nk, float h, float radius, const float* pos, const int* cell_start,
+                     const int* order, float* phi, void* hip_stream);
 
 /* ------------------------------------------------- coarse/fine coupling */
 /* FineToCoarseOperator — S/coarse_fine_solver.cpp:73-108: coarse(i,j,k) = mean of

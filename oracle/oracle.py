@@ -33,7 +33,8 @@ def lib():
         L.or_fraction_inside4.restype = c_float
         L.or_fraction_inside4.argtypes = [c_float] * 4
         L.or_face_weights.argtypes = [c_int, c_int, c_int, f32p, f32p, f32p, f32p]
-        L.or_assemble.argtypes = [c_int, c_int, c_int, c_float, c_float, c_int] + [f32p] * 9 + [f64p] * 5
+        L.or_assemble.argtypes = [c_int, c_int, c_int, c_float, c_float, c_int] + [f32p] * 9 + [ctypes.c_void_p] + [f64p] * 5
+        L.or_liquid_phi.argtypes = [c_int, f32p, c_float, c_int, c_int, c_int, c_float, f32p]
         L.or_level_dims.restype = c_int
         L.or_level_dims.argtypes = [c_int, c_int, c_int, i32p, c_int]
         L.or_coarsen.argtypes = [c_int, c_int, c_int] + [f64p] * 8
@@ -125,14 +126,25 @@ def face_weights(phi):
     return uw, vw, ww
 
 
-def assemble(h, u, v, w, uw, vw, ww, us, vs, ws, dt=1.0, sub=False):
+def assemble(h, u, v, w, uw, vw, ww, us, vs, ws, dt=1.0, sub=False, liquid=None):
+    """liquid: optional per-cell float32 mask (1 liquid, 0 air = Dirichlet p = 0)."""
     nk, nj, ni1 = u.shape
     ni = ni1 - 1
     shp = (nk, nj, ni)
     out = [np.zeros(shp, np.float64) for _ in range(5)]
+    liq = None if liquid is None else c32(liquid)
     lib().or_assemble(ni, nj, nk, h, dt, int(sub), c32(u), c32(v), c32(w), c32(uw), c32(vw), c32(ww),
-                      c32(us), c32(vs), c32(ws), *out)
+                      c32(us), c32(vs), c32(ws), None if liq is None else liq.ctypes.data, *out)
     return out  # diag, ax, ay, az, rhs
+
+
+def liquid_phi(pos, h, shape, radius):
+    """Particle level set at the cell centres of a grid shaped (nk, nj, ni)."""
+    nk, nj, ni = shape
+    p = c32(pos).reshape(-1)
+    out = np.zeros(shape, np.float32)
+    lib().or_liquid_phi(p.size // 3, p, h, ni, nj, nk, radius, out)
+    return out
 
 
 def level_dims(ni, nj, nk):

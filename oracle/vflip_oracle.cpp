@@ -171,10 +171,15 @@ void or_face_weights(int ni, int nj, int nk, const float* phi, float* uw, float*
 // (ax[c] = A(c, c+1) = A(c+1, c), etc.; zero where the reference stores no entry),
 // plus the FP64 right-hand side. All nixnjxnk linear.
 // ---------------------------------------------------------------------------
+// liquid (NULL for smoke): per-cell 1 = liquid, 0 = air. An air cell is a
+// Dirichlet p = 0 cell like the domain ring (empty row), and a liquid cell
+// couples only to liquid neighbours: the first-order free-surface boundary of
+// the standard MAC projection (the reference has no liquid solver; its smoke
+// assembly is the all-liquid case).
 void or_assemble(int ni, int nj, int nk, float h, float dt, int sub,
                  const float* u, const float* v, const float* w,
                  const float* uw, const float* vw, const float* ww,
-                 const float* us, const float* vs, const float* ws,
+                 const float* us, const float* vs, const float* ws, const float* liquid,
                  double* diag, double* ax, double* ay, double* az, double* rhs) {
     const size_t n = (size_t)ni * nj * nk;
     std::memset(diag, 0, n * sizeof(double));
@@ -191,12 +196,14 @@ void or_assemble(int ni, int nj, int nk, float h, float dt, int sub,
         for (int j = 1; j < nj - 1; ++j)
             for (int i = 1; i < ni - 1; ++i) {
                 size_t c = lin(i, j, k, ni, nj);
+                if (liquid && !(liquid[c] > 0.f)) continue;
+                auto wet = [&](size_t q) { return !liquid || liquid[q] > 0.f; };
                 double d = 0, r = 0;
                 float t;
                 // +x face
                 t = sub ? U(uw, i + 1, j, k) * dt / h2 : U(uw, i + 1, j, k) / h2;
                 d += t;
-                if (i < ni - 2) ax[c] = -(double)t;
+                if (i < ni - 2 && wet(c + 1)) ax[c] = -(double)t;
                 r -= (double)(U(uw, i + 1, j, k) * U(u, i + 1, j, k) / h);
                 // -x face
                 t = sub ? U(uw, i, j, k) * dt / h2 : U(uw, i, j, k) / h2;
@@ -205,7 +212,7 @@ void or_assemble(int ni, int nj, int nk, float h, float dt, int sub,
                 // +y face
                 t = sub ? V(vw, i, j + 1, k) * dt / h2 : V(vw, i, j + 1, k) / h2;
                 d += t;
-                if (j < nj - 2) ay[c] = -(double)t;
+                if (j < nj - 2 && wet(c + (size_t)ni)) ay[c] = -(double)t;
                 r -= (double)(V(vw, i, j + 1, k) * V(v, i, j + 1, k) / h);
                 // -y face
                 t = sub ? V(vw, i, j, k) * dt / h2 : V(vw, i, j, k) / h2;
@@ -214,7 +221,7 @@ void or_assemble(int ni, int nj, int nk, float h, float dt, int sub,
                 // +z face
                 t = sub ? W(ww, i, j, k + 1) * dt / h2 : W(ww, i, j, k + 1) / h2;
                 d += t;
-                if (k < nk - 2) az[c] = -(double)t;
+                if (k < nk - 2 && wet(c + (size_t)ni * nj)) az[c] = -(double)t;
                 r -= (double)(W(ww, i, j, k + 1) * W(w, i, j, k + 1) / h);
                 // -z face
                 t = sub ? W(ww, i, j, k) * dt / h2 : W(ww, i, j, k) / h2;
@@ -734,6 +741,31 @@ int or_hash(int np, const float* pos, float h, int nx, int ny, int nz, int clamp
     for (int p = 0; p < np; ++p)
         if (cell_key[p] >= 0) order[fill[cell_key[p]]++] = p;
     return cnt[ncell];
+}
+
+// Particle level set of the liquid (the oracle of vflip_liquid_phi): at every
+// cell centre (i h, j h, k h), phi = min |centre - p| over the particles binned
+// (as or_hash, clamp 0) in the 3x3x3 cells around it, minus radius; 3h - radius
+// where no particle is that close. Distances in float, no contraction.
+void or_liquid_phi(int np, const float* pos, float h, int nx, int ny, int nz, float radius, float* phi) {
+    const int ncell = nx * ny * nz;
+    for (int c = 0; c < ncell; ++c) phi[c] = 3.0f * h;
+    const float half = 0.5f * h;
+    for (int p = 0; p < np; ++p) {
+        const float x = pos[3 * p], y = pos[3 * p + 1], z = pos[3 * p + 2];
+        const int i = trunc_i32((x + half) / h), j = trunc_i32((y + half) / h), k = trunc_i32((z + half) / h);
+        if (i < 0 || j < 0 || k < 0 || i >= nx || j >= ny || k >= nz) continue;
+        for (int ck = k - 1; ck <= k + 1; ++ck)
+            for (int cj = j - 1; cj <= j + 1; ++cj)
+                for (int ci = i - 1; ci <= i + 1; ++ci) {
+                    if (ci < 0 || cj < 0 || ck < 0 || ci >= nx || cj >= ny || ck >= nz) continue;
+                    const float dx = (float)ci * h - x, dy = (float)cj * h - y, dz = (float)ck * h - z;
+                    const float d = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    float& f = phi[(ck * ny + cj) * nx + ci];
+                    if (d < f) f = d;
+                }
+    }
+    for (int c = 0; c < ncell; ++c) phi[c] = phi[c] - radius;
 }
 
 // ---------------------------------------------------------------------------

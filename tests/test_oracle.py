@@ -619,3 +619,49 @@ def test_coarse_fine_round_trip_constant_and_flux():
                 blk = fu[k * N:(k + 1) * N, j * N:(j + 1) * N, i * N:i * N + 1]
                 if blk.size == N * N:
                     assert abs(blk.sum() / (N * N) - cu[k, j, i]) < 1e-5
+
+
+def test_liquid_phi_brute_force():
+    """Particle level set: min distance from each cell centre to the particles binned in the
+    3x3x3 cells around it, minus the radius, in float32 (numpy brute force, bit-exact)."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(5)
+    nx, ny, nz, h = 9, 7, 6, np.float32(0.1)
+    pos = (rng.random((40, 3)) * [1.0, 0.8, 0.7] - 0.05).astype(np.float32)  # a few outside the grid
+    r = np.float32(0.04)
+    got = O.liquid_phi(pos, h, (nz, ny, nx), r)
+    half = np.float32(0.5) * h
+    cell = np.trunc((pos + half) / h).astype(np.int64)
+    inside = np.all((cell >= 0) & (cell < [nx, ny, nz]), axis=1)
+    ref = np.full((nz, ny, nx), np.float32(3.0) * h, np.float32)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                for p in np.nonzero(inside & np.all(np.abs(cell - [i, j, k]) <= 1, axis=1))[0]:
+                    dx = np.float32(i) * h - pos[p, 0]
+                    dy = np.float32(j) * h - pos[p, 1]
+                    dz = np.float32(k) * h - pos[p, 2]
+                    d = np.sqrt(np.float32(dx * dx + dy * dy + dz * dz))
+                    ref[k, j, i] = min(ref[k, j, i], d)
+    assert np.array_equal(got, ref - r)
+
+
+def test_liquid_assembly_closed_forms():
+    """All-liquid mask = the smoke assembly; air rows are empty; no coupling into air; the
+    diagonal of a liquid cell keeps every open face (air faces are Dirichlet)."""
+    from oracle import oracle as O
+    from vflip_amd import scenes
+    pb = scenes.box_problem(10, 9, 8, 2)
+    uw, vw, ww = O.face_weights(pb["phi"])
+    args = (pb["h"], pb["u"], pb["v"], pb["w"], uw, vw, ww, pb["us"], pb["vs"], pb["ws"])
+    smoke = O.assemble(*args)
+    assert all(np.array_equal(a, b) for a, b in zip(smoke, O.assemble(*args, liquid=np.ones((8, 9, 10), np.float32))))
+    liq = (np.random.default_rng(1).random((8, 9, 10)) < 0.6).astype(np.float32)
+    d, ax, ay, az, rhs = O.assemble(*args, liquid=liq)
+    air = liq == 0
+    assert not d[air].any() and not rhs[air].any() and not ax[air].any() and not ay[air].any() and not az[air].any()
+    assert not ax[:, :, :-1][liq[:, :, 1:] == 0].any()
+    assert not ay[:, :-1, :][liq[:, 1:, :] == 0].any()
+    assert not az[:-1, :, :][liq[1:, :, :] == 0].any()
+    wet = ~air
+    assert np.array_equal(d[wet], smoke[0][wet]) and np.array_equal(rhs[wet], smoke[4][wet])

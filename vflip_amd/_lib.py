@@ -60,6 +60,8 @@ SIGNATURES = {
     "vflip_solver_setup": (c_int, [P]),
     "vflip_solver_solve": (c_int, [P, P, P, ctypes.POINTER(SolveStats)]),
     "vflip_batch_create": (c_int, [c_int, P, ctypes.POINTER(c_void_p)]),
+    "vflip_assemble_liquid": (c_int, [P, c_float, c_float, P, P, P, P, P, P, P, P, P, P, P]),
+    "vflip_liquid_phi": (c_int, [c_int, c_int, c_int, c_float, c_float, P, P, P, P, P]),
     "vflip_batch_destroy": (c_int, [P]),
     "vflip_batch_set_graph_iterations": (c_int, [P, c_int]),
     "vflip_batch_solve": (c_int, [P, P, P, P, c_int, ctypes.POINTER(ctypes.c_double)]),

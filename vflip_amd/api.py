@@ -130,6 +130,11 @@ class Solver:
         check(lib().vflip_assemble(self.h, variant, h, dt, *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
                                    _ptr(rhs, numel=self.n)), "assemble")
 
+    def assemble_liquid(self, h, dt, u, v, w, uw, vw, ww, us, vs, ws, liquid_phi, rhs):
+        """Free-surface assembly: cells with liquid_phi >= 0 are air (Dirichlet p = 0)."""
+        check(lib().vflip_assemble_liquid(self.h, h, dt, *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
+                                          _ptr(liquid_phi, numel=self.n), _ptr(rhs, numel=self.n)), "assemble_liquid")
+
     def project_learned(self, net, factor, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
         """project() with the U-Net's pressure guess as the PCG's initial iterate."""
         st = SolveStats()
@@ -309,6 +314,16 @@ class Particles:
                               _ptr(vel) if npart else None, _ptr(start, torch.int32),
                               _ptr(order, torch.int32), *[float(b) for b in bmin], *ptrs), "p2g")
         return out
+
+
+def liquid_phi(h, radius, pos, cell_start, order, shape, out=None):
+    """Particle level set at the cell centres of a grid shaped (nk, nj, ni) from vflip_hash's lists."""
+    nk, nj, ni = shape
+    out = torch.empty(shape, device="cuda") if out is None else out
+    npart = pos.shape[0]
+    check(lib().vflip_liquid_phi(ni, nj, nk, h, radius, _ptr(pos) if npart else None, _ptr(cell_start, torch.int32),
+                                 _ptr(order, torch.int32), _ptr(out, numel=ni * nj * nk), _stream()), "liquid_phi")
+    return out
 
 
 def get_delta(new, old, out=None):

@@ -568,15 +568,18 @@ int launch_list(vflip_batch* B, const std::vector<Phase>& list) {
 
 int ensure_ready(vflip_batch* B) {
     for (auto* S : B->S) {
-        if (!S->setup_done) {
+        if (!S->setup_done || S->check_changed) {  // first use, or assembled since the last setup
             int rc = vflip_solver_setup(S);
             if (rc) return rc;
             VF_HIP(hipStreamSynchronize(S->stream));
         }
-        if (!S->ctab || S->ctab_sweeps != S->opts.coarse_sweeps) {
+        if (!S->use_ctab) {
             set_error("batched solve needs the tabulated coarsest level (VFLIP_COARSE_TABLE unset or 1)");
             return VFLIP_ERR_ARG;
         }
+        int rc = solver_ensure_ctab(S);
+        if (rc) return rc;
+        VF_HIP(hipStreamSynchronize(S->stream));
     }
     if (!B->planned) return build_plan(B);
     return 0;

@@ -888,6 +888,37 @@ static int ensure_capacity(vflip_particles* P, int np) {
 
 using namespace vflip;
 
+// Particle level set of the liquid: at every cell centre (i h, j h, k h),
+// phi = min |centre - p| over the particles binned in the 3x3x3 cells around it,
+// minus radius (3h - radius where no particle is that close). One thread per
+// cell gathers from vflip_hash's cell lists; the minimum does not depend on the
+// visiting order, so the result is exact (float distances, no contraction).
+namespace vflip {
+__global__ __launch_bounds__(kBlock) void k_liquid_phi(int ni, int nj, int nk, float h, float radius,
+                                                       const float* __restrict__ pos,
+                                                       const int* __restrict__ cell_start,
+                                                       const int* __restrict__ order, float* __restrict__ phi) {
+    const int n = ni * nj * nk;
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const int i = c % ni, j = (c / ni) % nj, k = c / (ni * nj);
+    const float cx = (float)i * h, cy = (float)j * h, cz = (float)k * h;
+    float best = 3.0f * h;
+    for (int ck = max(k - 1, 0); ck <= min(k + 1, nk - 1); ++ck)
+        for (int cj = max(j - 1, 0); cj <= min(j + 1, nj - 1); ++cj)
+            for (int ci = max(i - 1, 0); ci <= min(i + 1, ni - 1); ++ci) {
+                const int q = (ck * nj + cj) * ni + ci;
+                for (int s = cell_start[q]; s < cell_start[q + 1]; ++s) {
+                    const int p = order[s];
+                    const float dx = cx - pos[3 * p], dy = cy - pos[3 * p + 1], dz = cz - pos[3 * p + 2];
+                    const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                    best = d < best ? d : best;
+                }
+            }
+    phi[c] = best - radius;
+}
+}  // namespace vflip
+
 extern "C" {
 
 int vflip_particles_create(int ni, int nj, int nk, float h, vflip_particles** out) {
@@ -994,6 +1025,17 @@ int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* 
             k_p2g_gather<<<dim3((fx + 63) / 64, (fy + 3) / 4, fz), dim3(64, 4), 0, s>>>(a, c, method == 1 ? 0 : 1);
         }
     }
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+int vflip_liquid_phi(int ni, int nj, int nk, float h, float radius, const float* pos, const int* cell_start,
+                     const int* order, float* phi, void* hip_stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && (size_t)ni * nj * nk < (1u << 31), "grid dims");
+    VF_ARG(cell_start && order && phi, "level-set arrays");
+    const size_t n = (size_t)ni * nj * nk;
+    k_liquid_phi<<<grid_for(n), kBlock, 0, (hipStream_t)hip_stream>>>(ni, nj, nk, h, radius, pos, cell_start, order,
+                                                                      phi);
     VF_CHECK_LAUNCH();
     return 0;
 }

@@ -252,6 +252,27 @@ int run_vcycle(vflip_solver* S, bool want_dot, bool* dot_done) {
     return 0;
 }
 
+// Tabulate the coarsest solve (coarse_sweeps RB-GS sweeps from a zero guess,
+// a linear map of b): one workgroup per unit right-hand side.
+int build_ctab(vflip_solver* S) {
+    LevelBuf& lv = S->L.back();
+    const int nc = (int)lv.n;
+    if (!S->ctab) {
+        int rc = alloc((void**)&S->ctab, (size_t)nc * nc * sizeof(float));
+        if (!rc) rc = alloc((void**)&S->cpart, (size_t)nc * ((nc + kCoarseChunk - 1) / kCoarseChunk) * sizeof(float));
+        if (rc) return rc;
+    }
+    if (lv.half)
+        k_coarsest<__half><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), nullptr, nullptr,
+                                                                 S->opts.coarse_sweeps, S->ctab);
+    else
+        k_coarsest<float><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), nullptr, nullptr,
+                                                                S->opts.coarse_sweeps, S->ctab);
+    VF_CHECK_LAUNCH();
+    S->ctab_sweeps = S->opts.coarse_sweeps;
+    return 0;
+}
+
 int finalize(vflip_solver* S, int what, double tol = 0) {
     if (mixed(S))
         k_finalize<double><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial, S->np_last, what, tol);
@@ -540,8 +561,14 @@ int vflip_solver_setup(vflip_solver* S) {
         VF_HIP(hipMemcpyAsync(S->h_changed, S->d_changed, sizeof(int), hipMemcpyDeviceToHost, S->stream));
         VF_HIP(hipStreamSynchronize(S->stream));
         S->check_changed = false;
-        if (*S->h_changed == 0) return 0;  // same operator: the hierarchy stays valid
+        if (*S->h_changed == 0) {
+            // same operator: the hierarchy stays valid, and the coarsest-level table
+            // (dropped at the last change) now pays for itself
+            if (S->use_ctab && S->ctab_sweeps != S->opts.coarse_sweeps) return build_ctab(S);
+            return 0;
+        }
     }
+    const bool first = !S->setup_done;
     S->check_changed = false;
     VF_HIP(hipEventRecord(S->e0, S->stream));
     for (size_t l = 0; l + 1 < S->L.size(); ++l) {
@@ -576,24 +603,14 @@ int vflip_solver_setup(vflip_solver* S) {
     } else {
         for (auto& lv : S->L) lv.half = false;
     }
-    // Tabulate the coarsest solve (coarse_sweeps RB-GS sweeps from a zero guess,
-    // a linear map of b): one workgroup per unit right-hand side.
-    if (S->use_ctab) {
-        LevelBuf& lv = S->L.back();
-        const int nc = (int)lv.n;
-        if (!S->ctab) {
-            int rc = alloc((void**)&S->ctab, (size_t)nc * nc * sizeof(float));
-            if (!rc) rc = alloc((void**)&S->cpart, (size_t)nc * ((nc + kCoarseChunk - 1) / kCoarseChunk) * sizeof(float));
-            if (rc) return rc;
-        }
-        if (lv.half)
-            k_coarsest<__half><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), nullptr, nullptr,
-                                                                     S->opts.coarse_sweeps, S->ctab);
-        else
-            k_coarsest<float><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), nullptr, nullptr,
-                                                                    S->opts.coarse_sweeps, S->ctab);
-        VF_CHECK_LAUNCH();
-        S->ctab_sweeps = S->opts.coarse_sweeps;
+    // The coarsest-level table (below) costs about as much as a few solves'
+    // worth of direct coarsest sweeps: build it for the first operator, and for
+    // a changed one only once it is reused (an operator that changes every step,
+    // like a liquid's, keeps the direct sweeps).
+    S->ctab_sweeps = -1;
+    if (S->use_ctab && first) {
+        const int rc = build_ctab(S);
+        if (rc) return rc;
     }
     VF_HIP(hipEventRecord(S->e1, S->stream));
     S->setup_done = true;
@@ -638,7 +655,9 @@ int vflip_solver_get_level(const vflip_solver* S, int l, float* d, float* ax, fl
 
 int vflip_solver_vcycle(vflip_solver* S, const float* r, float* z) {
     VF_ARG(S && r && z, "solver/r/z");
-    if (!S->setup_done) {
+    // an assembly since the last setup may have changed the operator (setup
+    // compares it and keeps the hierarchy when it did not)
+    if (!S->setup_done || S->check_changed) {
         int rc = vflip_solver_setup(S);
         if (rc) return rc;
     }
@@ -697,11 +716,16 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
                vflip_solve_stats* st) {
     int rc;
     float ms_setup = 0.f;
-    if (!S->setup_done) {
+    // an assembly since the last setup may have changed the operator (setup
+    // compares it and keeps the hierarchy when it did not)
+    if (!S->setup_done || S->check_changed) {
+        const bool fresh = !S->setup_done;
         rc = vflip_solver_setup(S);
         if (rc) return rc;
-        VF_HIP(hipEventSynchronize(S->e1));
-        VF_HIP(hipEventElapsedTime(&ms_setup, S->e0, S->e1));
+        if (fresh) {
+            VF_HIP(hipEventSynchronize(S->e1));
+            VF_HIP(hipEventElapsedTime(&ms_setup, S->e0, S->e1));
+        }
     }
     if ((rc = ensure_f64(S))) return rc;
     const unsigned n = (unsigned)S->n;
@@ -839,6 +863,12 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
 }
 
 }  // namespace
+
+// The coarsest-level table for the current operator (built if missing or stale).
+int solver_ensure_ctab(vflip_solver* S) {
+    if (S->ctab && S->ctab_sweeps == S->opts.coarse_sweeps) return 0;
+    return build_ctab(S);
+}
 
 // Outputs of an FP32 solve run by the batched subdomain solver (its PCG left
 // x in S->xw and the scalars in S->sc): x32 = x, then the FP64 true residual

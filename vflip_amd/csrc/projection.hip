@@ -113,8 +113,13 @@ __global__ __launch_bounds__(kBlock) void k_inflow(int ni, int nj, int nk, const
 // Matrix-free assembly over interior cells 1..n-2 (ring rows empty).
 // S/Smoke_solver3D.cpp:986-1111; sub variant S/sub_domain.cpp:612-729.
 // Writes diag/ax/ay/az/leak (FP32, from FP64 accumulation of the FP32 terms)
-// and rhs (FP64 accumulation rounded to FP32).
+// and rhs (FP64 accumulation rounded to FP32). With a liquid level set (lphi !=
+// null; liquid where lphi < 0) an air cell is a Dirichlet p = 0 cell like the
+// ring (empty row) and a liquid cell couples only to liquid neighbours: the
+// first-order free-surface boundary (no reference counterpart: the reference's
+// smoke assembly is the all-liquid case).
 __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, float h, float dt, int sub,
+                                                     const float* __restrict__ lphi,
                                                      const float* __restrict__ u, const float* __restrict__ v,
                                                      const float* __restrict__ w, const float* __restrict__ uw,
                                                      const float* __restrict__ vw, const float* __restrict__ ww,
@@ -130,7 +135,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     int i = (int)(c % ni);
     size_t t = c / ni;
     int j = (int)(t % nj), k = (int)(t / nj);
-    if (!(i >= 1 && i < ni - 1 && j >= 1 && j < nj - 1 && k >= 1 && k < nk - 1)) {
+    auto wet = [&](size_t q) { return !lphi || lphi[q] < 0.f; };
+    if (!(i >= 1 && i < ni - 1 && j >= 1 && j < nj - 1 && k >= 1 && k < nk - 1) || !wet(c)) {
         if (changed && (D[c] != 0.f || AX[c] != 0.f || AY[c] != 0.f || AZ[c] != 0.f || LK[c] != 0.f))
             *changed = 1;  // every writer stores the same value
         D[c] = 0.f; AX[c] = 0.f; AY[c] = 0.f; AZ[c] = 0.f; LK[c] = 0.f; rhs[c] = 0.f;
@@ -152,13 +158,17 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     d += tym; r += (double)(vw[fym] * v[fym] / h);
     d += tzp; r -= (double)(ww[fzp] * w[fzp] / h);
     d += tzm; r += (double)(ww[fzm] * w[fzm] / h);
-    // terms whose neighbour is a ring cell carry no off-diagonal (Dirichlet leak)
-    if (i == ni - 2) lk += txp;
-    if (i == 1) lk += txm;
-    if (j == nj - 2) lk += typ;
-    if (j == 1) lk += tym;
-    if (k == nk - 2) lk += tzp;
-    if (k == 1) lk += tzm;
+    // terms whose neighbour is a ring (or air) cell carry no off-diagonal (Dirichlet leak)
+    const size_t sl = (size_t)ni * nj;
+    const bool oxp = i < ni - 2 && wet(c + 1), oxm = i > 1 && wet(c - 1);
+    const bool oyp = j < nj - 2 && wet(c + ni), oym = j > 1 && wet(c - ni);
+    const bool ozp = k < nk - 2 && wet(c + sl), ozm = k > 1 && wet(c - sl);
+    if (!oxp) lk += txp;
+    if (!oxm) lk += txm;
+    if (!oyp) lk += typ;
+    if (!oym) lk += tym;
+    if (!ozp) lk += tzp;
+    if (!ozm) lk += tzm;
     if (d > 0) {
         float s = (uw[fxp] - 1.0f) * us[fxp];
         s = s - (uw[fxm] - 1.0f) * us[fxm];
@@ -169,8 +179,8 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
         if (sub) r += (double)((1 / h) * s);
         else r += 1.0 / (double)h * (double)s;
     }
-    const float nd = (float)d, nx = (i < ni - 2) ? -txp : 0.f, ny = (j < nj - 2) ? -typ : 0.f;
-    const float nz = (k < nk - 2) ? -tzp : 0.f, nl = (float)lk;
+    const float nd = (float)d, nx = oxp ? -txp : 0.f, ny = oyp ? -typ : 0.f;
+    const float nz = ozp ? -tzp : 0.f, nl = (float)lk;
     // flag an operator change against the previous assembly (bitwise compare)
     if (changed && (__float_as_uint(D[c]) != __float_as_uint(nd) || __float_as_uint(AX[c]) != __float_as_uint(nx) ||
                     __float_as_uint(AY[c]) != __float_as_uint(ny) || __float_as_uint(AZ[c]) != __float_as_uint(nz) ||
@@ -303,9 +313,9 @@ int solver_solve_assembled(vflip_solver* s, const float* rhs32, float* x32, vfli
 
 extern "C" {
 
-int vflip_assemble(vflip_solver* S, int variant, float h, float dt, const float* u, const float* v, const float* w,
-                   const float* uw, const float* vw, const float* ww, const float* us, const float* vs,
-                   const float* ws, float* rhs) {
+static int assemble_impl(vflip_solver* S, int variant, float h, float dt, const float* u, const float* v,
+                         const float* w, const float* uw, const float* vw, const float* ww, const float* us,
+                         const float* vs, const float* ws, const float* liquid_phi, float* rhs) {
     VF_ARG(S, "solver");
     VF_ARG(u && v && w && uw && vw && ww && us && vs && ws && rhs, "assembly arrays");
     VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
@@ -318,12 +328,25 @@ int vflip_assemble(vflip_solver* S, int variant, float h, float dt, const float*
     rc = solver_f64(S, &d64, &b64, &x64);
     if (rc) return rc;
     int* changed = solver_begin_assembly(S);
-    k_assemble<<<grid_for((size_t)ni * nj * nk), kBlock, 0, s>>>(ni, nj, nk, h, dt, variant == VFLIP_VARIANT_SUB, u,
-                                                                 v, w, uw, vw, ww, us, vs, ws, d, ax, ay, az, lk,
-                                                                 rhs, changed, d64, b64);
+    k_assemble<<<grid_for((size_t)ni * nj * nk), kBlock, 0, s>>>(ni, nj, nk, h, dt, variant == VFLIP_VARIANT_SUB,
+                                                                 liquid_phi, u, v, w, uw, vw, ww, us, vs, ws, d, ax,
+                                                                 ay, az, lk, rhs, changed, d64, b64);
     VF_CHECK_LAUNCH();
     solver_mark_f64_assembled(S);
     return 0;
+}
+
+int vflip_assemble(vflip_solver* S, int variant, float h, float dt, const float* u, const float* v, const float* w,
+                   const float* uw, const float* vw, const float* ww, const float* us, const float* vs,
+                   const float* ws, float* rhs) {
+    return assemble_impl(S, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, nullptr, rhs);
+}
+
+int vflip_assemble_liquid(vflip_solver* S, float h, float dt, const float* u, const float* v, const float* w,
+                          const float* uw, const float* vw, const float* ww, const float* us, const float* vs,
+                          const float* ws, const float* liquid_phi, float* rhs) {
+    VF_ARG(liquid_phi, "liquid level set");
+    return assemble_impl(S, VFLIP_VARIANT_COARSE, h, dt, u, v, w, uw, vw, ww, us, vs, ws, liquid_phi, rhs);
 }
 
 namespace {

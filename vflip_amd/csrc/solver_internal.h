@@ -112,3 +112,5 @@ inline Op<float> make_op_f32(const LevelBuf& l) { return Op<float>{l.d, l.ax, l.
 
 // Outputs of an FP32 solve run by the batched subdomain solver (mgpcg.hip).
 int solver_finish(vflip_solver* S, const float* rhs32, float* x32, int iterations, int status, vflip_solve_stats* st);
+// The coarsest-level table for the current operator, built if missing or stale (mgpcg.hip).
+int solver_ensure_ctab(vflip_solver* S);
