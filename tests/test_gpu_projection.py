@@ -274,7 +274,9 @@ def test_project_mixed_fp64_pressure(cuda, mods):
 def test_reassembly_rebuilds_hierarchy(cuda, mods):
     """vflip_solver_solve after vflip_assemble with a different operator rebuilds the
     multigrid hierarchy: the reused solver converges like a fresh one (same iterations
-    +-1, solutions within FP32 round-off), and an unchanged re-assembly keeps it."""
+    +-1, solutions within FP32 round-off), and an unchanged re-assembly keeps it.
+    (Before this was handled, a solver re-assembled with a new liquid region kept
+    its first hierarchy and stalled at the iteration limit.)"""
     print("reassembly", flush=True)
     import torch
     O, api = mods
@@ -296,4 +298,6 @@ def test_reassembly_rebuilds_hierarchy(cuda, mods):
         assert st["converged"] == 1 and abs(st["iterations"] - sf["iterations"]) <= 1
         assert rel_l2(_np(x), _np(xf)) < 1e-5
         outs.append(_np(x))
-    assert np.array_equal(outs[1], outs[2])  # same operator and rhs: same hierarchy, same answer
+    # same operator again: the hierarchy is kept (the coarsest level is now tabulated
+    # instead of swept directly, equal up to FP32 rounding)
+    assert rel_l2(outs[2], outs[1]) < 1e-6
