@@ -33,6 +33,7 @@ extern "C" {
 #define VFLIP_ERR_HIP (-2)
 #define VFLIP_ERR_NOMEM (-3)
 #define VFLIP_ERR_BREAKDOWN (-4) /* rho == 0 or NaN after the first preconditioner apply */
+#define VFLIP_ERR_CAPACITY (-5)  /* an output does not fit the capacity the caller gave */
 
 /* Last error message of the calling thread ("" if none). */
 const char* vflip_last_error(void);
@@ -295,6 +296,28 @@ int vflip_g2p(int ni, int nj, int nk, float h, int np, const float* pos, float* 
 int vflip_advect_particles(int ni, int nj, int nk, float h, float dt, int variant, float bx, float by, float bz,
                            int np, float* pos, const float* u, const float* v, const float* w,
                            const float* solid_phi, void* hip_stream);
+
+/* reorder_FLIP — SmokeSolver3D, S/Smoke_solver3D.cpp:1347-1396
+ * (VFLIP_VARIANT_COARSE) and SubDomainSolver, S/sub_domain.cpp:1110-1159
+ * (VFLIP_VARIANT_SUB: interior cells 1..n-2 only, so particles binned in the
+ * boundary ring are dropped). Takes vflip_hash's lists (clamp 0 / clamp 1 with
+ * the same bmin) and rebuilds the particle set in linear cell order: a cell with
+ * n particles keeps them all when n < 10, else its first 8, in list order; when
+ * n < 8 it also gets 8 - n new particles at [bmin +] (i,j,k) h + 0.49 h r,
+ * r = frand(-1, 1) per axis, with velocity trilinear in u, v, w at that position
+ * (get_velocity_self; [minus bmin] in the subdomain form). Then every particle
+ * whose solid_phi (nodal SDF, (ni+1)(nj+1)(nk+1), trilinear at pos [- bmin]) is
+ * < 0 is dropped. The reference draws r with rand(); here
+ * r = -1 + 2 (randhash(seed * 0x9E3779B1 + 3 (8 cell + slot) + axis) >> 8) / 2^24,
+ * deterministic per (seed, cell, slot). out_pos/out_vel (3*cap floats, not
+ * aliasing pos/vel) receive the new set, *nout (host) its size. When the size
+ * exceeds cap nothing is written and VFLIP_ERR_CAPACITY is returned with *nout
+ * set; np + 8 * (cells) always suffices. Synchronous on the workspace's stream.
+ * Bit-exact. */
+int vflip_reorder(vflip_particles* P, int variant, float bx, float by, float bz, unsigned seed, int np,
+                  const float* pos, const float* vel, const int* cell_start, const int* order, const float* u,
+                  const float* v, const float* w, const float* solid_phi, int cap, float* out_pos, float* out_vel,
+                  int* nout);
 
 /* Particle level set of the liquid for the free-surface pressure boundary (the
  * reference has no liquid solver): phi at every cell centre (i h, j h, k h) =

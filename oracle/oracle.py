@@ -35,6 +35,11 @@ def lib():
         L.or_face_weights.argtypes = [c_int, c_int, c_int, f32p, f32p, f32p, f32p]
         L.or_assemble.argtypes = [c_int, c_int, c_int, c_float, c_float, c_int] + [f32p] * 9 + [ctypes.c_void_p] + [f64p] * 5
         L.or_liquid_phi.argtypes = [c_int, f32p, c_float, c_int, c_int, c_int, c_float, f32p]
+        L.or_reorder.restype = c_int
+        L.or_reorder.argtypes = ([c_int, c_int, c_int, c_float, c_int, c_float, c_float, c_float, ctypes.c_uint,
+                                 f32p, f32p, i32p, i32p, f32p, f32p, f32p, f32p, c_int, f32p, f32p])
+        L.or_reseed_jitter.restype = c_float
+        L.or_reseed_jitter.argtypes = [ctypes.c_uint, ctypes.c_uint]
         L.or_level_dims.restype = c_int
         L.or_level_dims.argtypes = [c_int, c_int, c_int, i32p, c_int]
         L.or_coarsen.argtypes = [c_int, c_int, c_int] + [f64p] * 8
@@ -145,6 +150,31 @@ def liquid_phi(pos, h, shape, radius):
     out = np.zeros(shape, np.float32)
     lib().or_liquid_phi(p.size // 3, p, h, ni, nj, nk, radius, out)
     return out
+
+
+def reorder(pos, vel, h, dims, start, order, u, v, w, phi, seed, sub=False, bmin=(0.0, 0.0, 0.0)):
+    """reorder_FLIP over or_hash's lists; returns (pos, vel) of the new particle set, (n, 3) each."""
+    ni, nj, nk = dims
+    pos = c32(pos).reshape(-1)
+    vel = c32(vel).reshape(-1)
+    if pos.size == 0:
+        pos = np.zeros(3, np.float32)
+        vel = np.zeros(3, np.float32)
+    order = np.ascontiguousarray(order, np.int32)
+    if order.size == 0:
+        order = np.zeros(1, np.int32)
+    cap = int(start[-1]) + 8 * ni * nj * nk
+    out_p = np.zeros(3 * max(cap, 1), np.float32)
+    out_v = np.zeros(3 * max(cap, 1), np.float32)
+    n = lib().or_reorder(ni, nj, nk, h, int(sub), *[float(b) for b in bmin], seed & 0xFFFFFFFF, pos, vel,
+                         np.ascontiguousarray(start, np.int32), order, c32(u), c32(v), c32(w), c32(phi), cap,
+                         out_p, out_v)
+    assert n <= cap
+    return out_p[:3 * n].reshape(-1, 3), out_v[:3 * n].reshape(-1, 3)
+
+
+def reseed_jitter(seed, key):
+    return lib().or_reseed_jitter(seed & 0xFFFFFFFF, key & 0xFFFFFFFF)
 
 
 def level_dims(ni, nj, nk):

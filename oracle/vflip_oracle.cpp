@@ -26,6 +26,12 @@
 //   g2p ................................. affine reproduction
 //   advect_rk2_linear/cubic_clamp ....... affine translation / RK2 exactness
 //   coarse/fine operators ............... constant-field round trip, flux closed form
+//   liquid_phi .......................... brute-force distance over all particles
+//   reorder ............................. brute-force restatement of the per-cell
+//                                         rules on tiny grids, affine velocity and
+//                                         plane solid closed forms, idempotence;
+//                                         the reseed RNG replaces rand() (no vector
+//                                         can pin rand() across platforms)
 //
 // Floating point: compiled with -ffp-contract=off so float expressions round
 // exactly as written (the reference's MSVC /fp:precise build does the same).
@@ -1062,5 +1068,76 @@ void or_advect_particles(int np, float* pos, const float* u, const float* v, con
 // getDelta (S/Smoke_solver3D.cpp:1297-1318): out = new - old, elementwise float.
 void or_get_delta(int n, const float* fnew, const float* fold, float* out) {
     for (int c = 0; c < n; ++c) out[c] = fnew[c] - fold[c];
+}
+
+// frand(-1, 1) of the reseeding (S/Smoke_solver3D.h:36-39: a + (b - a) U) with
+// U drawn from the counter hash instead of rand(): U = (randhash(seed *
+// 0x9E3779B1 + key) >> 8) / 2^24, key = 3 (8 cell + slot) + axis.
+float or_reseed_jitter(unsigned seed, unsigned key) {
+    const unsigned r = or_randhash(seed * 0x9E3779B1u + key);
+    const float u = (float)(r >> 8) * (1.0f / 16777216.0f);
+    return -1.0f + 2.0f * u;
+}
+
+// reorder_FLIP. sub = 0: SmokeSolver3D, S/Smoke_solver3D.cpp:1347-1396 (every
+// cell; new particle at (i,j,k) h + 0.49 h r; velocity get_velocity_self and
+// solid phi at that position). sub = 1: SubDomainSolver,
+// S/sub_domain.cpp:1110-1159 (interior cells 1..n-2 only, the boundary ring's
+// particles are dropped; new particle at bmin + (i,j,k) h + 0.49 h r; velocity
+// get_velocity and solid phi at pos - bmin, S/sub_domain.cpp:70-90).
+// Cell-major over the hash lists (cell_start/order from or_hash): a cell with c
+// particles keeps them all when c < 10, else its first 8; when c < 8 it also
+// gets 8 - c new ones. Then the particles whose trilinear nodal solid phi is
+// < 0 are dropped, keeping the order. Writes min(count, cap) particles and
+// returns the count.
+int or_reorder(int nx, int ny, int nz, float h, int sub, float bx, float by, float bz, unsigned seed,
+               const float* pos, const float* vel, const int* cell_start, const int* order, const float* u,
+               const float* v, const float* w, const float* phi, int cap, float* out_pos, float* out_vel) {
+    const double hd = h;
+    const float b[3] = {bx, by, bz};
+    auto solid_ok = [&](const float* x) {
+        float q[3] = {x[0], x[1], x[2]};
+        if (sub) for (int a = 0; a < 3; ++a) q[a] = x[a] - b[a];
+        return or_sample_linear(phi, nx + 1, ny + 1, nz + 1, hd, 0.5, 0.5, 0.5, q[0], q[1], q[2]) >= 0.0f;
+    };
+    int out = 0;
+    auto push = [&](const float* x, const float* vv) {
+        if (!solid_ok(x)) return;
+        if (out < cap)
+            for (int a = 0; a < 3; ++a) {
+                out_pos[3 * (size_t)out + a] = x[a];
+                out_vel[3 * (size_t)out + a] = vv[a];
+            }
+        ++out;
+    };
+    const int lo = sub ? 1 : 0;
+    for (int k = lo; k < nz - lo; ++k)
+        for (int j = lo; j < ny - lo; ++j)
+            for (int i = lo; i < nx - lo; ++i) {
+                const size_t c = lin(i, j, k, nx, ny);
+                const int s = cell_start[c], n = cell_start[c + 1] - s;
+                const int keep = n < 10 ? n : 8;
+                for (int q = 0; q < keep; ++q) {
+                    const int p = order[s + q];
+                    push(pos + 3 * (size_t)p, vel + 3 * (size_t)p);
+                }
+                const int ijk[3] = {i, j, k};
+                for (int t = 0; t < 8 - n; ++t) {
+                    float x[3], vv[3];
+                    for (int a = 0; a < 3; ++a) {
+                        const float r = or_reseed_jitter(seed, (unsigned)(3 * (8 * c + (size_t)t) + a));
+                        const float cell = (float)ijk[a] * h;
+                        const float base = sub ? b[a] + cell : cell;
+                        x[a] = base + (r * 0.49f) * h;
+                    }
+                    float q[3] = {x[0], x[1], x[2]};
+                    if (sub) for (int a = 0; a < 3; ++a) q[a] = x[a] - b[a];
+                    vv[0] = or_sample_linear(u, nx + 1, ny, nz, hd, 0.5, 0, 0, q[0], q[1], q[2]);
+                    vv[1] = or_sample_linear(v, nx, ny + 1, nz, hd, 0, 0.5, 0, q[0], q[1], q[2]);
+                    vv[2] = or_sample_linear(w, nx, ny, nz + 1, hd, 0, 0, 0.5, q[0], q[1], q[2]);
+                    push(x, vv);
+                }
+            }
+    return out;
 }
 }  // extern "C"

@@ -229,3 +229,93 @@ def test_advect_particles_bitexact(cuda, variant):
     # the solid push-out changed a good number of particles
     free = O.advect_particles(pos, h, dt, u, v, w, np.ones_like(phi), sub=variant == 1, bmin=bmin)
     assert (ref != free).any(axis=1).sum() > 20
+
+
+# ------------------------------------------------------------------ reorder_FLIP
+def _reorder_both(dims, h, pos, vel, grids, phi, seed, sub=False, bmin=(0.0, 0.0, 0.0), out=None):
+    from oracle import oracle as O
+    from vflip_amd import api
+    P = api.Particles(*dims, float(h))
+    gp, gv = _t(pos), _t(vel)
+    _, start, order, _ = P.hash(gp, clamp=sub, bmin=bmin)
+    op, ov = P.reorder(gp, gv, start, order, *[_t(g) for g in grids], _t(phi), seed,
+                       variant=api.SUB if sub else api.COARSE, bmin=bmin, out=out)
+    _, rs, ro = O.bin_particles(pos, h, dims, clamp=sub, bmin=bmin)
+    rp, rv = O.reorder(pos, vel, h, dims, rs, ro, *grids, phi, seed, sub=sub, bmin=bmin)
+    return P, (_np(op), _np(ov)), (rp, rv)
+
+
+@pytest.mark.parametrize("sub", [False, True])
+def test_reorder_bitexact(cuda, sub):
+    """Ragged cells (0..13 particles, some outside the grid), random MAC velocity, a solid
+    sphere: the GPU rebuild equals the oracle's bit for bit (order, kept and new particles)."""
+    print(f"reorder sub={sub}", flush=True)
+    dims, h = (20, 18, 16), np.float32(0.05)
+    bmin = (0.3, -0.2, 0.1) if sub else (0.0, 0.0, 0.0)
+    pos, vel = scenes.ragged_particles(*dims, float(h), seed=6, bmin=bmin, outside=200)
+    rng = np.random.default_rng(3)
+    grids = [rng.standard_normal(s).astype(np.float32) for s in
+             [(dims[2], dims[1], dims[0] + 1), (dims[2], dims[1] + 1, dims[0]), (dims[2] + 1, dims[1], dims[0])]]
+    phi = scenes.sphere_phi(*dims, float(h), (0.5, 0.45, 0.4), 0.22).astype(np.float32)
+    _, (gp, gv), (rp, rv) = _reorder_both(dims, h, pos, vel, grids, phi, 11, sub=sub, bmin=bmin)
+    assert len(gp) == len(rp) and len(rp) > 8 * 1000
+    assert np.array_equal(bits(gp), bits(rp)) and np.array_equal(bits(gv), bits(rv))
+    free = _reorder_both(dims, h, pos, vel, grids, np.ones_like(phi), 11, sub=sub, bmin=bmin)[2][0]
+    assert len(free) - len(rp) > 500  # the sphere dropped particles
+
+
+def test_reorder_capacity_and_empty(cuda):
+    """An output buffer too small is reported and replaced; zero particles give 8 new
+    particles in every cell outside the solid."""
+    from vflip_amd import api
+    dims, h = (12, 10, 9), np.float32(1.0 / 12)
+    pos, vel = scenes.ragged_particles(*dims, float(h), seed=8)
+    grids = scenes.affine_mac(*dims, float(h), [(0.1, 1, 0, 0), (0.2, 0, 1, 0), (0.3, 0, 0, 1)])
+    phi = scenes.box_phi(*dims, float(h), open_top=False).astype(np.float32)
+    import torch
+    small = (torch.empty((10, 3), device="cuda"), torch.empty((10, 3), device="cuda"))
+    _, (gp, gv), (rp, rv) = _reorder_both(dims, h, pos, vel, grids, phi, 4, out=small)
+    assert len(gp) > 10 and np.array_equal(bits(gp), bits(rp)) and np.array_equal(bits(gv), bits(rv))
+    z = np.zeros((0, 3), np.float32)
+    P, (gp, gv), (rp, rv) = _reorder_both(dims, h, z, z, grids, phi, 4)
+    assert np.array_equal(bits(gp), bits(rp)) and np.array_equal(bits(gv), bits(rv))
+    inside = (phi[:-1, :-1, :-1] > 0)  # cells well inside the walls keep all 8
+    assert len(gp) >= 8 * int(inside[2:-2, 2:-2, 2:-2].sum())
+    with pytest.raises(api.VflipError):
+        from vflip_amd._lib import lib
+        import ctypes
+        n = ctypes.c_int(0)
+        zc = torch.zeros(P.ncell + 1, dtype=torch.int32, device="cuda")
+        g = [_t(a) for a in grids]
+        api.check(lib().vflip_reorder(P.hnd, 7, 0.0, 0.0, 0.0, 0, 0, None, None, api._ptr(zc, torch.int32), None,
+                                      *[api._ptr(a) for a in g], api._ptr(_t(phi)), 0, None, None,
+                                      ctypes.byref(n)), "reorder")
+
+
+def test_reorder_smoke_128(cuda):
+    """128^3 FLIP smoke set (8 jittered particles per cell spread over 1.6 cells, shuffled),
+    closed box with a sphere: bit-exact with the oracle at full size; the result is cell
+    sorted with 8 or 9 particles in every cell away from the solid, and a second rebuild
+    without the solid changes nothing."""
+    print("reorder 128^3", flush=True)
+    from vflip_amd import api
+    n = 128
+    h = np.float32(1.0 / n)
+    pos, vel = _cloud(n, 8, 21, spread=1.6, shuffle=True)
+    grids = scenes.random_mac(n, 5, scale=1.0)
+    phi = scenes.solid_phi(n, 1.0 / n, [((0.5, 0.5, 0.5), 0.15)], open_top=False)
+    P, (gp, gv), (rp, rv) = _reorder_both((n, n, n), h, pos, vel, grids, phi, 17)
+    assert np.array_equal(bits(gp), bits(rp)) and np.array_equal(bits(gv), bits(rv))
+    tp, tv = _t(gp), _t(gv)
+    _, start, order, nb = P.hash(tp)
+    counts = _np(start)[1:] - _np(start)[:-1]
+    assert nb == len(gp) and np.array_equal(_np(order)[:nb], np.arange(nb))
+    cphi = phi[:-1, :-1, :-1].reshape(-1)  # nodal phi at the cell's lower corner
+    away = cphi > 2.0 * float(h)
+    assert counts[away].min() >= 8 and counts[away].max() <= 9
+    ones = _t(np.ones_like(phi))
+    g = [_t(a) for a in grids]
+    p2, v2 = P.reorder(tp, tv, start, order, *g, ones, 99)
+    p2, v2 = _np(p2), _np(v2)
+    keep = (counts >= 8)  # no cell rebuilt except those the solid thinned
+    assert len(p2) == len(gp) + int((8 - counts[~keep]).sum())

@@ -665,3 +665,119 @@ def test_liquid_assembly_closed_forms():
     assert not az[:-1, :, :][liq[1:, :, :] == 0].any()
     wet = ~air
     assert np.array_equal(d[wet], smoke[0][wet]) and np.array_equal(rhs[wet], smoke[4][wet])
+
+
+# ------------------------------------------------------------------ reorder_FLIP
+def _brute_lists(pos, h, dims, clamp=False, bmin=(0.0, 0.0, 0.0)):
+    """Per-cell particle id lists by brute-force membership (hash_FLIP's float rule)."""
+    ni, nj, nk = dims
+    h = np.float32(h)
+    p = pos - np.asarray(bmin, np.float32) if clamp else pos
+    cell = np.trunc((p + np.float32(0.5) * h) / h).astype(np.int64)
+    if clamp:
+        cell = np.clip(cell, 0, np.array([ni, nj, nk]) - 1)
+    lists = [[] for _ in range(ni * nj * nk)]
+    for q, (i, j, k) in enumerate(cell):
+        if 0 <= i < ni and 0 <= j < nj and 0 <= k < nk:
+            lists[(k * nj + j) * ni + i].append(q)
+    return lists
+
+
+def _walk_reorder(pos, vel, out_p, out_v, h, dims, sub=False, bmin=(0.0, 0.0, 0.0), coef=None):
+    """Walks the oracle's output cell by cell against the reference's rules: kept rows are
+    the inputs bit for bit, new rows lie within 0.49 h of their cell centre, in that cell,
+    with the affine velocity where the samplers do not clamp. Returns the new-row jitters."""
+    ni, nj, nk = dims
+    lists = _brute_lists(pos, h, dims, clamp=sub, bmin=bmin)
+    b = np.asarray(bmin, np.float64)
+    row, jit = 0, []
+    lo = 1 if sub else 0
+    for k in range(lo, nk - lo):
+        for j in range(lo, nj - lo):
+            for i in range(lo, ni - lo):
+                ids = lists[(k * nj + j) * ni + i]
+                keep = ids if len(ids) < 10 else ids[:8]
+                for q in keep:
+                    assert np.array_equal(bits(out_p[row]), bits(pos[q])) and np.array_equal(bits(out_v[row]), bits(vel[q]))
+                    row += 1
+                for _ in range(8 - len(ids)):
+                    x = out_p[row].astype(np.float64) - b
+                    c = np.array([i, j, k], np.float64) * h
+                    r = (x - c) / (0.49 * h)
+                    assert np.all(np.abs(r) <= 1 + 1e-4), (i, j, k, r)
+                    assert tuple(np.trunc((out_p[row] - np.float32(b) + np.float32(0.5 * h)) / np.float32(h))
+                                 .astype(int)) == (i, j, k)
+                    if coef is not None and min(i, j, k) >= 1 and i < ni - 1 and j < nj - 1 and k < nk - 1:
+                        want = [cf[0] + cf[1] * x[0] + cf[2] * x[1] + cf[3] * x[2] for cf in coef]
+                        assert np.allclose(out_v[row], want, rtol=1e-5, atol=1e-6), (out_v[row], want)
+                    jit.append(r)
+                    row += 1
+    assert row == len(out_p)
+    return np.array(jit)
+
+
+COEF = [(0.3, 1.5, -0.7, 0.2), (-0.2, 0.4, 0.9, -1.1), (0.1, -0.6, 0.25, 0.8)]
+
+
+@pytest.mark.parametrize("sub", [False, True])
+def test_reorder_rules_brute_force(sub):
+    """Cell-major rebuild: keep all of n < 10, the first 8 of n >= 10, 8 - n new particles
+    for n < 8 (in the cell, within 0.49 h, affine velocity reproduced); the subdomain form
+    visits interior cells only and samples at pos - bmin."""
+    dims, h = (7, 6, 5), 0.125
+    bmin = (0.3, -0.2, 0.1) if sub else (0.0, 0.0, 0.0)
+    pos, vel = scenes.ragged_particles(*dims, h, seed=3, bmin=bmin, outside=25)
+    _, start, order = O.bin_particles(pos, h, dims, clamp=sub, bmin=bmin)
+    u, v, w = scenes.affine_mac(*dims, h, COEF)
+    phi = np.ones((dims[2] + 1, dims[1] + 1, dims[0] + 1), np.float32)
+    op, ov = O.reorder(pos, vel, h, dims, start, order, u, v, w, phi, seed=11, sub=sub, bmin=bmin)
+    jit = _walk_reorder(pos, vel, op, ov, h, dims, sub=sub, bmin=bmin, coef=COEF)
+    assert len(jit) > 100
+
+
+def test_reorder_solid_filter_plane():
+    """A plane SDF drops exactly the candidates behind it, in order (trilinear reproduces the
+    plane; no candidate lies within 1e-5 of it)."""
+    dims, h = (8, 7, 6), 0.125
+    pos, vel = scenes.ragged_particles(*dims, h, seed=4)
+    _, start, order = O.bin_particles(pos, h, dims)
+    u, v, w = scenes.affine_mac(*dims, h, COEF)
+    ones = np.ones((dims[2] + 1, dims[1] + 1, dims[0] + 1), np.float32)
+    ap, av = O.reorder(pos, vel, h, dims, start, order, u, v, w, ones, seed=2)
+    n = scenes.nodal_coords
+    x0 = 0.4321
+    gx, gy = n(dims[0], h)[None, None, :], n(dims[1], h)[None, :, None]
+    plane = (0.8 * (gx - x0) + 0.6 * gy + 0.0 * n(dims[2], h)[:, None, None]).astype(np.float32)
+    bp, bv = O.reorder(pos, vel, h, dims, start, order, u, v, w, plane, seed=2)
+    d = 0.8 * (ap[:, 0].astype(np.float64) - x0) + 0.6 * ap[:, 1]
+    assert np.abs(d).min() > 1e-5
+    keep = d >= 0
+    assert 0 < keep.sum() < len(ap)
+    assert np.array_equal(bits(bp), bits(ap[keep])) and np.array_equal(bits(bv), bits(av[keep]))
+
+
+def test_reorder_idempotent():
+    """After one rebuild every cell holds 8 or 9 particles in cell order, so a second
+    rebuild (re-binned, no solid) returns the same particles bit for bit."""
+    dims, h = (9, 6, 5), 0.1
+    pos, vel = scenes.ragged_particles(*dims, h, seed=5, outside=10)
+    u, v, w = scenes.affine_mac(*dims, h, COEF)
+    ones = np.ones((dims[2] + 1, dims[1] + 1, dims[0] + 1), np.float32)
+    _, start, order = O.bin_particles(pos, h, dims)
+    ap, av = O.reorder(pos, vel, h, dims, start, order, u, v, w, ones, seed=9)
+    _, start2, order2 = O.bin_particles(ap, h, dims)
+    counts = np.diff(start2)
+    assert counts.min() >= 8 and counts.max() <= 9
+    assert np.array_equal(order2, np.arange(len(ap)))  # cell-sorted
+    bp, bv = O.reorder(ap, av, h, dims, start2, order2, u, v, w, ones, seed=123)
+    assert np.array_equal(bits(bp), bits(ap)) and np.array_equal(bits(bv), bits(av))
+
+
+def test_reseed_jitter_distribution():
+    """frand(-1, 1) replacement: values on the 2^-23 lattice in [-1, 1), uniform moments."""
+    r = np.array([O.reseed_jitter(7, k) for k in range(40000)], np.float64)
+    assert r.min() >= -1 and r.max() < 1
+    lattice = (r + 1) * 2 ** 23
+    assert np.array_equal(lattice, np.round(lattice))
+    assert abs(r.mean()) < 0.015 and abs((r ** 2).mean() - 1 / 3) < 0.01
+    assert len(np.unique(r)) > 39900

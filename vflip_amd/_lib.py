@@ -62,6 +62,8 @@ SIGNATURES = {
     "vflip_batch_create": (c_int, [c_int, P, ctypes.POINTER(c_void_p)]),
     "vflip_assemble_liquid": (c_int, [P, c_float, c_float, P, P, P, P, P, P, P, P, P, P, P]),
     "vflip_liquid_phi": (c_int, [c_int, c_int, c_int, c_float, c_float, P, P, P, P, P]),
+    "vflip_reorder": (c_int, [P, c_int, c_float, c_float, c_float, ctypes.c_uint, c_int, P, P, P, P, P, P, P, P,
+                              c_int, P, P, PI]),
     "vflip_batch_destroy": (c_int, [P]),
     "vflip_batch_set_graph_iterations": (c_int, [P, c_int]),
     "vflip_batch_solve": (c_int, [P, P, P, P, c_int, ctypes.POINTER(ctypes.c_double)]),
@@ -148,6 +150,9 @@ def exported_symbols():
 
 class VflipError(RuntimeError):
     pass
+
+
+ERR_CAPACITY = -5
 
 
 def check(rc, what=""):

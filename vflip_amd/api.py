@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._lib import SolveOpts, SolveStats, check, lib
+from ._lib import ERR_CAPACITY, SolveOpts, SolveStats, VflipError, check, lib  # noqa: F401
 
 FP32 = 0
 MIXED = 1
@@ -314,6 +314,33 @@ class Particles:
                               _ptr(vel) if npart else None, _ptr(start, torch.int32),
                               _ptr(order, torch.int32), *[float(b) for b in bmin], *ptrs), "p2g")
         return out
+
+
+    def reorder(self, pos, vel, start, order, u, v, w, solid_phi, seed, variant=COARSE, bmin=(0.0, 0.0, 0.0),
+                out=None):
+        """reorder_FLIP over hash()'s lists: returns the new (pos, vel), (n, 3) each.
+
+        out = (pos_buf, vel_buf) of shape (cap, 3) is reused when it fits; the
+        result is then a view of it. Otherwise buffers of the exact size are made.
+        """
+        npart = pos.shape[0]
+        if out is None:
+            cap = npart + npart // 8 + 1024
+            out = (torch.empty((cap, 3), device="cuda"), torch.empty((cap, 3), device="cuda"))
+        for attempt in range(2):
+            cap = min(out[0].shape[0], out[1].shape[0])
+            n = ctypes.c_int(0)
+            rc = lib().vflip_reorder(self.hnd, variant, *[float(b) for b in bmin], int(seed) & 0xFFFFFFFF, npart,
+                                     _ptr(pos) if npart else None, _ptr(vel) if npart else None,
+                                     _ptr(start, torch.int32, numel=self.ncell + 1),
+                                     _ptr(order, torch.int32) if npart else None, _ptr(u), _ptr(v), _ptr(w),
+                                     _ptr(solid_phi), cap, _ptr(out[0]) if cap else None,
+                                     _ptr(out[1]) if cap else None, ctypes.byref(n))
+            if rc == ERR_CAPACITY and attempt == 0:
+                out = (torch.empty((n.value, 3), device="cuda"), torch.empty((n.value, 3), device="cuda"))
+                continue
+            check(rc, "reorder")
+            return out[0][:n.value], out[1][:n.value]
 
 
 def liquid_phi(h, radius, pos, cell_start, order, shape, out=None):

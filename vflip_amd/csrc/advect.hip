@@ -165,15 +165,6 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int n
 //   source sphere: rho = T = 1 + jitter(seed, cell) inside |x - c| <= r
 //   buoyancy on v faces: v += dt * (-alpha rho_f + beta T_f), rho_f = average of
 //   the two cells sharing the face (faces 1..nj-1).
-__device__ inline unsigned randhash_dev(unsigned seed) {
-    unsigned i = (seed ^ 0xA3C59AC3u) * 2654435769u;
-    i ^= (i >> 16);
-    i *= 2654435769u;
-    i ^= (i >> 16);
-    i *= 2654435769u;
-    return i;
-}
-
 __global__ __launch_bounds__(kBlock) void k_plume_source(int ni, int nj, int nk, float h, float cx, float cy, float cz,
                                                          float r, unsigned seed, float jitter, float* __restrict__ rho,
                                                          float* __restrict__ temp) {
@@ -182,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_plume_source(int ni, int nj, int nk,
     const float dx = i * h - cx, dy = j * h - cy, dz = k * h - cz;
     if (dx * dx + dy * dy + dz * dz > r * r) return;
     const size_t c = ((size_t)k * nj + j) * ni + i;
-    const float u = (float)randhash_dev(seed * 0x9E3779B1u + (unsigned)c) * (1.0f / 4294967295.0f);
+    const float u = (float)randhash(seed * 0x9E3779B1u + (unsigned)c) * (1.0f / 4294967295.0f);
     rho[c] = 1.0f + jitter * u;
     temp[c] = 1.0f + jitter * u;
 }

@@ -48,6 +48,17 @@ __host__ __device__ inline size_t lin3(int i, int j, int k, int nx, int ny) {
     return (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k);
 }
 
+// randhash of the reference (S/util.h:238-246): integer hash used as a
+// counter-based random source.
+__host__ __device__ inline unsigned randhash(unsigned seed) {
+    unsigned i = (seed ^ 0xA3C59AC3u) * 2654435769u;
+    i ^= (i >> 16);
+    i *= 2654435769u;
+    i ^= (i >> 16);
+    i *= 2654435769u;
+    return i;
+}
+
 // Wave64 reductions (CDNA wavefront = 64 lanes).
 template <typename T> __device__ inline T wave_sum(T v) {
 #pragma unroll

@@ -28,6 +28,7 @@ struct vflip_particles {
     int* tmp = nullptr;      // per particle: merge buffer of the long-segment sort
     int* big = nullptr;      // [0]: count, [1..]: cells whose segment needs the long sort
     int* partial = nullptr;  // scan partials
+    int* reo = nullptr;      // reorder: per-cell survivor counts/offsets [ncell+1], masks [ncell]
     size_t cap = 0, big_cap = 0, npartial = 0;
 };
 
@@ -864,6 +865,142 @@ __global__ __launch_bounds__(kBlock) void k_advect_particles(int np, float* __re
 }
 
 // ------------------------------------------------------------------ helpers
+// ------------------------------------------------------------ reorder_FLIP
+// Cell-major rebuild of the particle set (S/Smoke_solver3D.cpp:1347-1396,
+// S/sub_domain.cpp:1110-1159). A cell's candidates are its first `keep`
+// binned particles (keep = n < 10 ? n : 8) followed by max(8 - n, 0) new
+// particles; a candidate survives when the nodal solid phi at it is >= 0.
+// k_reorder_count stores each cell's survivor count and survivor bit mask
+// (at most 9 candidates), a scan turns the counts into output offsets, and
+// k_reorder_emit writes the survivors. Thread per cell: after one rebuild the
+// particles are cell-sorted, so a wave's gathers of pos/vel walk one
+// contiguous span of about 8 x 64 particles.
+struct ReoArgs {
+    int nx, ny, nz;
+    float h;
+    int sub;  // SubDomainSolver form: interior cells only, bmin-relative sampling
+    float bx, by, bz;
+    unsigned seed;
+    const float* pos;
+    const float* vel;
+    const int* start;
+    const int* order;
+    const float* u;
+    const float* v;
+    const float* w;
+    const float* phi;
+};
+
+// frand(-1, 1) (S/Smoke_solver3D.h:36-39: a + (b - a) U) with U from the
+// counter hash: U = (randhash(seed * 0x9E3779B1 + key) >> 8) / 2^24.
+__device__ inline float reseed_jitter(unsigned seed, unsigned key) {
+    const unsigned r = randhash(seed * 0x9E3779B1u + key);
+    const float u = (float)(r >> 8) * (1.0f / 16777216.0f);
+    return -1.0f + 2.0f * u;
+}
+
+// New particle `slot` of cell c = (i, j, k): (bmin +) (i,j,k) h + (r * 0.49) h,
+// key 3 (8 c + slot) + axis.
+__device__ inline void reseed_pos(const ReoArgs& a, size_t c, int i, int j, int k, int slot, float* x) {
+    const int ijk[3] = {i, j, k};
+    const float b[3] = {a.bx, a.by, a.bz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const float r = reseed_jitter(a.seed, (unsigned)(3 * (8 * c + (size_t)slot) + ax));
+        const float cell = (float)ijk[ax] * a.h;
+        const float base = a.sub ? b[ax] + cell : cell;
+        x[ax] = base + (r * 0.49f) * a.h;
+    }
+}
+
+// get_solid_phi(pos) >= 0: nodal SDF (nx+1, ny+1, nz+1), offsets 0.5.
+__device__ inline bool reo_solid_ok(const ReoArgs& a, float x, float y, float z) {
+    if (a.sub) {
+        x = x - a.bx;
+        y = y - a.by;
+        z = z - a.bz;
+    }
+    const Grid gp{a.phi, a.nx + 1, a.ny + 1, a.nz + 1, 0.5, 0.5, 0.5};
+    return sample_linear(gp, (double)a.h, x, y, z) >= 0.0f;
+}
+
+// Cell coordinates and whether the cell takes part (all cells; the subdomain
+// form only the interior 1..n-2).
+__device__ inline bool reo_cell(const ReoArgs& a, size_t c, int& i, int& j, int& k) {
+    const size_t plane = (size_t)a.nx * a.ny;
+    k = (int)(c / plane);
+    const int r = (int)(c - (size_t)k * plane);
+    j = r / a.nx;
+    i = r - j * a.nx;
+    if (!a.sub) return true;
+    return i >= 1 && i < a.nx - 1 && j >= 1 && j < a.ny - 1 && k >= 1 && k < a.nz - 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reorder_count(size_t ncell, ReoArgs a, int* __restrict__ cnt,
+                                                          int* __restrict__ mask) {
+    const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    int i, j, k;
+    int m = 0;
+    if (reo_cell(a, c, i, j, k)) {
+        const int s = a.start[c], n = a.start[c + 1] - s;
+        const int keep = n < 10 ? n : 8;
+        for (int q = 0; q < keep; ++q) {
+            const size_t p = (size_t)a.order[s + q];
+            if (reo_solid_ok(a, a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2])) m |= 1 << q;
+        }
+        for (int t = 0; t < 8 - n; ++t) {
+            float x[3];
+            reseed_pos(a, c, i, j, k, t, x);
+            if (reo_solid_ok(a, x[0], x[1], x[2])) m |= 1 << (keep + t);
+        }
+    }
+    cnt[c] = __popc(m);
+    mask[c] = m;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reorder_emit(size_t ncell, ReoArgs a, const int* __restrict__ offset,
+                                                         const int* __restrict__ mask, float* __restrict__ out_pos,
+                                                         float* __restrict__ out_vel) {
+    const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= ncell) return;
+    const int m = mask[c];
+    if (!m) return;
+    int i, j, k;
+    reo_cell(a, c, i, j, k);
+    const int s = a.start[c], n = a.start[c + 1] - s;
+    const int keep = n < 10 ? n : 8;
+    size_t o = (size_t)offset[c];
+    for (int q = 0; q < keep; ++q) {
+        if (!((m >> q) & 1)) continue;
+        const size_t p = (size_t)a.order[s + q];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            out_pos[3 * o + ax] = a.pos[3 * p + ax];
+            out_vel[3 * o + ax] = a.vel[3 * p + ax];
+        }
+        ++o;
+    }
+    const double hd = a.h;
+    const Grid gu{a.u, a.nx + 1, a.ny, a.nz, 0.5, 0.0, 0.0};
+    const Grid gv{a.v, a.nx, a.ny + 1, a.nz, 0.0, 0.5, 0.0};
+    const Grid gw{a.w, a.nx, a.ny, a.nz + 1, 0.0, 0.0, 0.5};
+    for (int t = 0; t < 8 - n; ++t) {
+        if (!((m >> (keep + t)) & 1)) continue;
+        float x[3];
+        reseed_pos(a, c, i, j, k, t, x);
+        // get_velocity_self(x); the subdomain's get_velocity samples at x - bmin
+        const float qx = a.sub ? x[0] - a.bx : x[0], qy = a.sub ? x[1] - a.by : x[1],
+                    qz = a.sub ? x[2] - a.bz : x[2];
+        out_vel[3 * o] = sample_linear(gu, hd, qx, qy, qz);
+        out_vel[3 * o + 1] = sample_linear(gv, hd, qx, qy, qz);
+        out_vel[3 * o + 2] = sample_linear(gw, hd, qx, qy, qz);
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) out_pos[3 * o + ax] = x[ax];
+        ++o;
+    }
+}
+
 static int ensure_capacity(vflip_particles* P, int np) {
     if ((size_t)np > P->cap) {
         const size_t cap = (size_t)np + (size_t)np / 4 + 1024;
@@ -947,6 +1084,7 @@ int vflip_particles_destroy(vflip_particles* P) {
     if (P->tmp) (void)hipFree(P->tmp);
     if (P->big) (void)hipFree(P->big);
     if (P->partial) (void)hipFree(P->partial);
+    if (P->reo) (void)hipFree(P->reo);
     delete P;
     return 0;
 }
@@ -1044,6 +1182,46 @@ int vflip_get_delta(size_t n, const float* fnew, const float* fold, float* out, 
     VF_ARG(fnew && fold && out, "get_delta arrays");
     if (n == 0) return 0;
     k_get_delta<<<min(grid_for(n), 8192u), kBlock, 0, (hipStream_t)hip_stream>>>(n, fnew, fold, out);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+int vflip_reorder(vflip_particles* P, int variant, float bx, float by, float bz, unsigned seed, int np,
+                  const float* pos, const float* vel, const int* cell_start, const int* order, const float* u,
+                  const float* v, const float* w, const float* solid_phi, int cap, float* out_pos, float* out_vel,
+                  int* nout) {
+    VF_ARG(P && nout && cell_start && u && v && w && solid_phi, "reorder args");
+    VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
+    VF_ARG(np >= 0 && cap >= 0, "reorder counts");
+    VF_ARG(np == 0 || (pos && vel && order), "reorder particles");
+    VF_ARG(cap == 0 || (out_pos && out_vel), "reorder outputs");
+    hipStream_t s = P->stream;
+    const size_t nc = P->ncell;
+    if (!P->reo && hipMalloc(&P->reo, (2 * nc + 1) * sizeof(int)) != hipSuccess) {
+        P->reo = nullptr;
+        set_error("reorder workspace of %zu cells: out of device memory", nc);
+        return VFLIP_ERR_NOMEM;
+    }
+    int* cnt = P->reo;
+    int* mask = P->reo + nc + 1;
+    const int sub = variant == VFLIP_VARIANT_SUB;
+    const ReoArgs a{P->nx, P->ny, P->nz, P->h, sub, sub ? bx : 0.f, sub ? by : 0.f, sub ? bz : 0.f, seed,
+                    pos, vel, cell_start, order, u, v, w, solid_phi};
+    k_reorder_count<<<grid_for(nc), kBlock, 0, s>>>(nc, a, cnt, mask);
+    const unsigned nb = (unsigned)P->npartial;
+    k_scan_reduce<<<nb, kBlock, 0, s>>>(nc, cnt, P->partial);
+    k_scan_partials<<<1, 1024, 0, s>>>((int)nb, P->partial, cnt + nc);
+    k_scan_apply<<<nb, kBlock, 0, s>>>(nc, cnt, P->partial);
+    VF_CHECK_LAUNCH();
+    int total = 0;
+    VF_HIP(hipMemcpyAsync(&total, cnt + nc, sizeof(int), hipMemcpyDeviceToHost, s));
+    VF_HIP(hipStreamSynchronize(s));
+    *nout = total;
+    if (total > cap) {
+        set_error("reorder: %d particles do not fit the output capacity %d", total, cap);
+        return VFLIP_ERR_CAPACITY;
+    }
+    if (total > 0) k_reorder_emit<<<grid_for(nc), kBlock, 0, s>>>(nc, a, cnt, mask, out_pos, out_vel);
     VF_CHECK_LAUNCH();
     return 0;
 }

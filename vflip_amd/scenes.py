@@ -142,3 +142,40 @@ def block_particles(n, ppc, seed, lo=(0.0, 0.0, 0.0), hi=(0.4, 0.6, 1.0), jitter
 def particle_velocities(npart, seed, scale=1.0):
     idx = np.arange(npart)
     return (np.stack([uniform01(seed, 300 + a, idx) * 2 - 1 for a in range(3)], 1) * scale).astype(np.float32)
+
+
+def ragged_particles(ni, nj, nk, h, seed, max_count=13, bmin=(0.0, 0.0, 0.0), outside=0):
+    """Particles with an uneven count per cell (U{0..max_count}), each within 0.45 h of
+    its cell centre (i,j,k) h [+ bmin], in a seeded random order, plus `outside`
+    particles up to 2 cells beyond the grid. Returns float32 pos (np, 3), vel (np, 3)."""
+    ncell = ni * nj * nk
+    counts = (uniform01(seed, 400, np.arange(ncell)) * (max_count + 1)).astype(np.int64)
+    cell = np.repeat(np.arange(ncell), counts)
+    ijk = np.stack([cell % ni, (cell // ni) % nj, cell // (ni * nj)], 1).astype(np.float64)
+    idx = np.arange(len(cell))
+    jit = np.stack([uniform01(seed, 410 + a, idx) - 0.5 for a in range(3)], 1) * 0.9
+    pos = (ijk + jit) * h
+    if outside:
+        o = np.arange(outside)
+        dims = np.array([ni, nj, nk], np.float64)
+        far = np.stack([uniform01(seed, 420 + a, o) for a in range(3)], 1) * (dims + 4) - 2
+        pos = np.concatenate([pos, far * h])
+    pos = (pos + np.asarray(bmin, np.float64)).astype(np.float32)
+    perm = np.argsort(uniform01(seed, 430, np.arange(len(pos))), kind="stable")
+    pos = np.ascontiguousarray(pos[perm])
+    return pos, particle_velocities(len(pos), seed + 1)
+
+
+def affine_mac(ni, nj, nk, h, coef):
+    """MAC grids of the affine velocity comp(x) = coef[comp][0] + coef[comp][1:] . x at the
+    face positions (u face (i,j,k) at ((i - 1/2) h, j h, k h), likewise v, w)."""
+    out = []
+    for comp in range(3):
+        shape = [nk, nj, ni]
+        shape[2 - comp] += 1
+        zz, yy, xx = np.meshgrid(*[np.arange(s, dtype=np.float64) for s in shape], indexing="ij")
+        x = [xx * h, yy * h, zz * h]
+        x[comp] = x[comp] - 0.5 * h
+        c = coef[comp]
+        out.append((c[0] + c[1] * x[0] + c[2] * x[1] + c[3] * x[2]).astype(np.float32))
+    return out
