@@ -319,3 +319,31 @@ def test_reorder_smoke_128(cuda):
     p2, v2 = _np(p2), _np(v2)
     keep = (counts >= 8)  # no cell rebuilt except those the solid thinned
     assert len(p2) == len(gp) + int((8 - counts[~keep]).sum())
+
+
+def test_reorder_face_particles_sparse_solid(cuda):
+    """Particles within a few ulp of the cell faces (h = 0.1, so the float binning and the
+    double sampler round differently) and a solid of isolated negative nodes: the GPU's
+    solid-free cell test never changes a decision (bit-exact with the oracle, which samples
+    every candidate)."""
+    print("reorder face particles", flush=True)
+    dims, h = (14, 12, 10), np.float32(0.1)
+    rng = np.random.default_rng(12)
+    ncell = dims[0] * dims[1] * dims[2]
+    cell = rng.integers(0, ncell, 6000)
+    ijk = np.stack([cell % dims[0], (cell // dims[0]) % dims[1], cell // (dims[0] * dims[1])], 1)
+    face = (ijk + rng.choice([-0.5, 0.5], ijk.shape)) * h
+    pos = np.nextafter(face.astype(np.float32), np.float32(np.inf) * rng.choice([-1, 1], ijk.shape))
+    pos = np.where(rng.random(ijk.shape) < 0.5, face.astype(np.float32), pos).astype(np.float32)
+    pos = np.ascontiguousarray(pos)
+    vel = scenes.particle_velocities(len(pos), 3)
+    # tiny positive nodes and isolated very negative ones: a face particle whose sampler base
+    # node is one below its cell is dropped through the far node's tiny weight (94 of the
+    # 382 dropped particles here would survive a test of the cell's own 2x2x2 nodes only)
+    phi = np.full((dims[2] + 1, dims[1] + 1, dims[0] + 1), 1e-6, np.float32)
+    phi.reshape(-1)[rng.integers(0, phi.size, 40)] = -1e3
+    grids = scenes.affine_mac(*dims, float(h), [(0.1, 1, 0, 0), (0.2, 0, 1, 0), (0.3, 0, 0, 1)])
+    _, (gp, gv), (rp, rv) = _reorder_both(dims, h, pos, vel, grids, phi, 5)
+    assert np.array_equal(bits(gp), bits(rp)) and np.array_equal(bits(gv), bits(rv))
+    free = _reorder_both(dims, h, pos, vel, grids, np.ones_like(phi), 5)[2][0]
+    assert len(free) > len(rp)

@@ -864,17 +864,22 @@ __global__ __launch_bounds__(kBlock) void k_advect_particles(int np, float* __re
     pos[3 * (size_t)p + 2] = x[2];
 }
 
-// ------------------------------------------------------------------ helpers
 // ------------------------------------------------------------ reorder_FLIP
 // Cell-major rebuild of the particle set (S/Smoke_solver3D.cpp:1347-1396,
 // S/sub_domain.cpp:1110-1159). A cell's candidates are its first `keep`
 // binned particles (keep = n < 10 ? n : 8) followed by max(8 - n, 0) new
 // particles; a candidate survives when the nodal solid phi at it is >= 0.
 // k_reorder_count stores each cell's survivor count and survivor bit mask
-// (at most 9 candidates), a scan turns the counts into output offsets, and
-// k_reorder_emit writes the survivors. Thread per cell: after one rebuild the
-// particles are cell-sorted, so a wave's gathers of pos/vel walk one
-// contiguous span of about 8 x 64 particles.
+// (at most 9 candidates, bit q = candidate q), a scan turns the counts into
+// output offsets, and k_reorder_emit writes the survivors: a survivor's output
+// slot is its cell's offset plus the survivors before it in the mask, which is
+// the reference's order (kept particles, then new ones, cell after cell).
+//
+// One workgroup per kReoTile consecutive cells, a lane per (cell, candidate)
+// pair: consecutive lanes read consecutive entries of `order` and, once the
+// particles are cell-sorted (after the first rebuild), consecutive particles,
+// and write consecutive output slots. Cells whose every reachable solid-phi node
+// is >= 0 keep all candidates without sampling.
 struct ReoArgs {
     int nx, ny, nz;
     float h;
@@ -924,71 +929,172 @@ __device__ inline bool reo_solid_ok(const ReoArgs& a, float x, float y, float z)
     return sample_linear(gp, (double)a.h, x, y, z) >= 0.0f;
 }
 
+// Solid-free cells. Every node that the solid-phi sample of a particle in cell
+// (i,j,k) can read being >= 0 makes every such sample >= 0 (trilinear weights
+// lie in [0, 1]), so no particle or new particle of the cell is dropped and
+// none needs sampling. A particle binned in cell i has x/h + 1/2 within
+// rounding of [i, i + 1), so the sampler's base node is i - 1, i or i + 1:
+// nodes i-1..i+2 per axis, clamped as the sampler clamps. NaN nodes fail.
+// reo_col_free tests the 4 x 4 (y, z) nodes of one x-node column; a cell is
+// free when the columns at x-nodes i-1..i+2 are.
+__device__ inline bool reo_col_free(const ReoArgs& a, int ii, int j, int k) {
+    ii = min(max(ii, 0), a.nx);
+    bool free = true;
+#pragma unroll
+    for (int dz = -1; dz <= 2; ++dz) {
+        const int kk = min(max(k + dz, 0), a.nz);
+#pragma unroll
+        for (int dy = -1; dy <= 2; ++dy) {
+            const int jj = min(max(j + dy, 0), a.ny);
+            free &= a.phi[((size_t)kk * (a.ny + 1) + jj) * (a.nx + 1) + ii] >= 0.0f;
+        }
+    }
+    return free;
+}
+
 // Cell coordinates and whether the cell takes part (all cells; the subdomain
 // form only the interior 1..n-2).
 __device__ inline bool reo_cell(const ReoArgs& a, size_t c, int& i, int& j, int& k) {
-    const size_t plane = (size_t)a.nx * a.ny;
-    k = (int)(c / plane);
-    const int r = (int)(c - (size_t)k * plane);
+    const unsigned plane = (unsigned)a.nx * (unsigned)a.ny, cu = (unsigned)c;  // cell counts fit in int
+    k = (int)(cu / plane);
+    const int r = (int)(cu - (unsigned)k * plane);
     j = r / a.nx;
     i = r - j * a.nx;
     if (!a.sub) return true;
     return i >= 1 && i < a.nx - 1 && j >= 1 && j < a.ny - 1 && k >= 1 && k < a.nz - 1;
 }
 
+// Cells per workgroup (a multiple of kBlock).
+constexpr int kReoTile = kBlock;
+constexpr int kReoPer = kReoTile / kBlock;
+// Candidates per cell: n < 8 gives n kept + (8 - n) new, 8 or 9 gives n, more
+// gives 8, so at most 9. Candidate s of a cell is its binned entry s when
+// s < keep, else new particle s - keep; it is also the cell's mask bit.
+constexpr int kReoSlots = 9;
+
+// Tile of cells [c0, c0 + nt): cell starts in s_start[0..kReoTile].
+__device__ inline int reo_load_starts(const ReoArgs& a, size_t ncell, int* s_start) {
+    const size_t c0 = (size_t)blockIdx.x * kReoTile;
+    const int nt = (int)min((size_t)kReoTile, ncell - c0);
+    for (int u = threadIdx.x; u <= kReoTile; u += kBlock) s_start[u] = a.start[c0 + min(u, nt)];
+    return nt;
+}
+
+__device__ inline int reo_keep(int n) { return n < 10 ? n : 8; }
+__device__ inline int reo_cand(int n) { return n < 8 ? 8 : reo_keep(n); }
+
 __global__ __launch_bounds__(kBlock) void k_reorder_count(size_t ncell, ReoArgs a, int* __restrict__ cnt,
                                                           int* __restrict__ mask) {
-    const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= ncell) return;
-    int i, j, k;
-    int m = 0;
-    if (reo_cell(a, c, i, j, k)) {
-        const int s = a.start[c], n = a.start[c + 1] - s;
-        const int keep = n < 10 ? n : 8;
-        for (int q = 0; q < keep; ++q) {
-            const size_t p = (size_t)a.order[s + q];
-            if (reo_solid_ok(a, a.pos[3 * p], a.pos[3 * p + 1], a.pos[3 * p + 2])) m |= 1 << q;
+    __shared__ int s_start[kReoTile + 1];
+    __shared__ int s_mask[kReoTile];
+    __shared__ int s_ijk[3][kReoTile];
+    __shared__ unsigned char s_col[kReoTile];
+    __shared__ unsigned char s_test[kReoTile];  // cell takes part and needs the solid test
+    const int tid = threadIdx.x;
+    const size_t c0 = (size_t)blockIdx.x * kReoTile;
+    const int nt = reo_load_starts(a, ncell, s_start);
+    // Own x-node column of every cell; the columns at i-1, i+1, i+2 come from
+    // the neighbouring cells of the same row in this tile, else are read here
+    // directly (node -1 clamps to node 0, the cell's own column).
+    int ci[kReoPer], cj[kReoPer], ck[kReoPer];
+    bool part[kReoPer], edge[kReoPer];
+#pragma unroll
+    for (int u = 0; u < kReoPer; ++u) {
+        const int l = tid + u * kBlock;
+        s_mask[l] = 0;
+        ci[u] = cj[u] = ck[u] = 0;
+        part[u] = l < nt && reo_cell(a, c0 + l, ci[u], cj[u], ck[u]);
+        s_ijk[0][l] = ci[u];
+        s_ijk[1][l] = cj[u];
+        s_ijk[2][l] = ck[u];
+        s_col[l] = l < nt && reo_col_free(a, ci[u], cj[u], ck[u]);
+        bool e = true;
+        if (part[u]) {
+            if (ci[u] > 0 && l == 0) e = e && reo_col_free(a, ci[u] - 1, cj[u], ck[u]);
+            if (!(ci[u] + 1 < a.nx && l + 1 < nt)) e = e && reo_col_free(a, ci[u] + 1, cj[u], ck[u]);
+            if (!(ci[u] + 2 < a.nx && l + 2 < nt)) e = e && reo_col_free(a, ci[u] + 2, cj[u], ck[u]);
         }
-        for (int t = 0; t < 8 - n; ++t) {
-            float x[3];
-            reseed_pos(a, c, i, j, k, t, x);
-            if (reo_solid_ok(a, x[0], x[1], x[2])) m |= 1 << (keep + t);
-        }
+        edge[u] = e;
     }
-    cnt[c] = __popc(m);
-    mask[c] = m;
+    __syncthreads();
+    bool free[kReoPer];
+#pragma unroll
+    for (int u = 0; u < kReoPer; ++u) {
+        const int l = tid + u * kBlock;
+        bool f = part[u] && s_col[l] && edge[u];
+        if (f && ci[u] > 0 && l > 0) f = s_col[l - 1];
+        if (f && ci[u] + 1 < a.nx && l + 1 < nt) f = s_col[l + 1];
+        if (f && ci[u] + 2 < a.nx && l + 2 < nt) f = s_col[l + 2];
+        free[u] = f;
+        s_test[l] = part[u] && !f;
+    }
+    __syncthreads();
+    // a lane per (cell, candidate) pair of the cells that need the test
+    for (int r = tid; r < nt * kReoSlots; r += kBlock) {
+        const int l = r / kReoSlots, sl = r - l * kReoSlots;
+        const int n = s_start[l + 1] - s_start[l], keep = reo_keep(n);
+        if (sl >= reo_cand(n) || !s_test[l]) continue;
+        float x[3];
+        if (sl < keep) {
+            const size_t p = (size_t)a.order[s_start[l] + sl];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) x[ax] = a.pos[3 * p + ax];
+        } else {
+            reseed_pos(a, c0 + l, s_ijk[0][l], s_ijk[1][l], s_ijk[2][l], sl - keep, x);
+        }
+        if (reo_solid_ok(a, x[0], x[1], x[2])) atomicOr(&s_mask[l], 1 << sl);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kReoPer; ++u) {
+        const int l = tid + u * kBlock;
+        if (l >= nt) continue;
+        const int n = s_start[l + 1] - s_start[l];
+        const int m = !part[u] ? 0 : (free[u] ? (1 << reo_cand(n)) - 1 : s_mask[l]);
+        cnt[c0 + l] = __popc(m);
+        mask[c0 + l] = m;
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_reorder_emit(size_t ncell, ReoArgs a, const int* __restrict__ offset,
                                                          const int* __restrict__ mask, float* __restrict__ out_pos,
                                                          float* __restrict__ out_vel) {
-    const size_t c = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (c >= ncell) return;
-    const int m = mask[c];
-    if (!m) return;
-    int i, j, k;
-    reo_cell(a, c, i, j, k);
-    const int s = a.start[c], n = a.start[c + 1] - s;
-    const int keep = n < 10 ? n : 8;
-    size_t o = (size_t)offset[c];
-    for (int q = 0; q < keep; ++q) {
-        if (!((m >> q) & 1)) continue;
-        const size_t p = (size_t)a.order[s + q];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            out_pos[3 * o + ax] = a.pos[3 * p + ax];
-            out_vel[3 * o + ax] = a.vel[3 * p + ax];
-        }
-        ++o;
+    __shared__ int s_start[kReoTile + 1];
+    __shared__ int s_mask[kReoTile];
+    __shared__ int s_off[kReoTile];
+    const int tid = threadIdx.x;
+    const size_t c0 = (size_t)blockIdx.x * kReoTile;
+    const int nt = reo_load_starts(a, ncell, s_start);
+    for (int l = tid; l < kReoTile; l += kBlock) {
+        s_mask[l] = l < nt ? mask[c0 + l] : 0;
+        s_off[l] = l < nt ? offset[c0 + l] : 0;
     }
+    __syncthreads();
     const double hd = a.h;
     const Grid gu{a.u, a.nx + 1, a.ny, a.nz, 0.5, 0.0, 0.0};
     const Grid gv{a.v, a.nx, a.ny + 1, a.nz, 0.0, 0.5, 0.0};
     const Grid gw{a.w, a.nx, a.ny, a.nz + 1, 0.0, 0.0, 0.5};
-    for (int t = 0; t < 8 - n; ++t) {
-        if (!((m >> (keep + t)) & 1)) continue;
+    // a lane per (cell, candidate) pair; survivors go to the cell's offset plus
+    // the survivors before them
+    for (int r = tid; r < nt * kReoSlots; r += kBlock) {
+        const int l = r / kReoSlots, sl = r - l * kReoSlots;
+        const int m = s_mask[l];
+        if (!((m >> sl) & 1)) continue;
+        const int n = s_start[l + 1] - s_start[l], keep = reo_keep(n);
+        const size_t o = (size_t)s_off[l] + __popc(m & ((1 << sl) - 1));
+        if (sl < keep) {
+            const size_t p = (size_t)a.order[s_start[l] + sl];
+#pragma unroll
+            for (int ax = 0; ax < 3; ++ax) {
+                out_pos[3 * o + ax] = a.pos[3 * p + ax];
+                out_vel[3 * o + ax] = a.vel[3 * p + ax];
+            }
+            continue;
+        }
+        int i, j, k;
+        reo_cell(a, c0 + l, i, j, k);
         float x[3];
-        reseed_pos(a, c, i, j, k, t, x);
+        reseed_pos(a, c0 + l, i, j, k, sl - keep, x);
         // get_velocity_self(x); the subdomain's get_velocity samples at x - bmin
         const float qx = a.sub ? x[0] - a.bx : x[0], qy = a.sub ? x[1] - a.by : x[1],
                     qz = a.sub ? x[2] - a.bz : x[2];
@@ -997,10 +1103,10 @@ __global__ __launch_bounds__(kBlock) void k_reorder_emit(size_t ncell, ReoArgs a
         out_vel[3 * o + 2] = sample_linear(gw, hd, qx, qy, qz);
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) out_pos[3 * o + ax] = x[ax];
-        ++o;
     }
 }
 
+// ------------------------------------------------------------------ helpers
 static int ensure_capacity(vflip_particles* P, int np) {
     if ((size_t)np > P->cap) {
         const size_t cap = (size_t)np + (size_t)np / 4 + 1024;
@@ -1207,7 +1313,7 @@ int vflip_reorder(vflip_particles* P, int variant, float bx, float by, float bz,
     const int sub = variant == VFLIP_VARIANT_SUB;
     const ReoArgs a{P->nx, P->ny, P->nz, P->h, sub, sub ? bx : 0.f, sub ? by : 0.f, sub ? bz : 0.f, seed,
                     pos, vel, cell_start, order, u, v, w, solid_phi};
-    k_reorder_count<<<grid_for(nc), kBlock, 0, s>>>(nc, a, cnt, mask);
+    k_reorder_count<<<grid_for(nc, kReoTile), kBlock, 0, s>>>(nc, a, cnt, mask);
     const unsigned nb = (unsigned)P->npartial;
     k_scan_reduce<<<nb, kBlock, 0, s>>>(nc, cnt, P->partial);
     k_scan_partials<<<1, 1024, 0, s>>>((int)nb, P->partial, cnt + nc);
@@ -1221,7 +1327,7 @@ int vflip_reorder(vflip_particles* P, int variant, float bx, float by, float bz,
         set_error("reorder: %d particles do not fit the output capacity %d", total, cap);
         return VFLIP_ERR_CAPACITY;
     }
-    if (total > 0) k_reorder_emit<<<grid_for(nc), kBlock, 0, s>>>(nc, a, cnt, mask, out_pos, out_vel);
+    if (total > 0) k_reorder_emit<<<grid_for(nc, kReoTile), kBlock, 0, s>>>(nc, a, cnt, mask, out_pos, out_vel);
     VF_CHECK_LAUNCH();
     return 0;
 }
