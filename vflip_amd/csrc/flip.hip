@@ -205,39 +205,75 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(int np, const int* __res
 
 // Atomic slots order a cell's particles by arrival; the reference order is
 // ascending particle id. Short unsorted segments are insertion-sorted by their
-// thread; longer ones are queued for k_sort_long.
+// cell's thread; longer ones are queued for k_sort_long. One workgroup per
+// kBlock consecutive cells: their span of `order` is staged in LDS (read
+// coalesced, sorted there, written back only if a segment changed), unless it
+// exceeds kFixSpan entries, when the threads work on global memory directly.
 constexpr int kShortSegment = 64;
+constexpr int kFixSpan = 4096;
+
+// Sorts seg[0..n) (n <= kShortSegment) by insertion; returns whether it changed.
+template <typename P> __device__ inline bool fix_segment(P seg, int n) {
+    bool sorted = true;
+    for (int a = 1; a < n && sorted; ++a) sorted = seg[a - 1] < seg[a];
+    if (sorted) return false;
+    for (int a = 1; a < n; ++a) {
+        const int x = seg[a];
+        int b = a - 1;
+        while (b >= 0 && seg[b] > x) {
+            seg[b + 1] = seg[b];
+            --b;
+        }
+        seg[b + 1] = x;
+    }
+    return true;
+}
+
+__device__ inline bool segment_sorted(const int* seg, int n) {
+    for (int a = 1; a < n; ++a)
+        if (seg[a - 1] > seg[a]) return false;
+    return true;
+}
 
 __global__ __launch_bounds__(kBlock) void k_cell_fix(size_t ncell, const int* __restrict__ start, int* __restrict__ order,
                                                      int* __restrict__ big) {
-    const size_t c = blockIdx.x * (size_t)kBlock + threadIdx.x;
-    if (c >= ncell) return;
-    const int s = start[c], e = start[c + 1];
-    if (e - s < 2) return;
-    int prev = order[s];
-    bool sorted = true;
-    for (int a = s + 1; a < e; ++a) {
-        const int x = order[a];
-        if (x < prev) {
-            sorted = false;
-            break;
-        }
-        prev = x;
+    __shared__ int s_start[kBlock + 1];
+    __shared__ int s_ord[kFixSpan];
+    __shared__ int s_dirty;
+    const int tid = threadIdx.x;
+    const size_t c0 = (size_t)blockIdx.x * kBlock;
+    const int nt = (int)min((size_t)kBlock, ncell - c0);
+    s_start[tid] = start[c0 + min(tid, nt)];
+    if (tid == 0) {
+        s_start[kBlock] = start[c0 + nt];
+        s_dirty = 0;
     }
-    if (sorted) return;
-    if (e - s <= kShortSegment) {
-        for (int a = s + 1; a < e; ++a) {
-            const int x = order[a];
-            int b = a - 1;
-            while (b >= s && order[b] > x) {
-                order[b + 1] = order[b];
-                --b;
+    __syncthreads();
+    const int t0 = s_start[0], span = s_start[nt] - t0;
+    const int s = s_start[min(tid, nt)] - t0, n = tid < nt ? s_start[tid + 1] - s_start[tid] : 0;
+    if (span <= kFixSpan) {
+        for (int t = tid; t < span; t += kBlock) s_ord[t] = order[t0 + t];
+        __syncthreads();
+        if (n >= 2) {
+            if (n <= kShortSegment) {
+                if (fix_segment(s_ord + s, n)) s_dirty = 1;
+            } else if (!segment_sorted(s_ord + s, n)) {
+                const int q = atomicAdd(&big[0], 1);
+                big[1 + q] = (int)(c0 + tid);
             }
-            order[b + 1] = x;
         }
-    } else {
+        __syncthreads();
+        if (s_dirty)
+            for (int t = tid; t < span; t += kBlock) order[t0 + t] = s_ord[t];
+        return;
+    }
+    if (n < 2) return;
+    int* seg = order + t0 + s;
+    if (n <= kShortSegment) {
+        fix_segment(seg, n);
+    } else if (!segment_sorted(seg, n)) {
         const int q = atomicAdd(&big[0], 1);
-        big[1 + q] = (int)c;
+        big[1 + q] = (int)(c0 + tid);
     }
 }
 
