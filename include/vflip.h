@@ -327,6 +327,75 @@ int vflip_reorder(vflip_particles* P, int variant, float bx, float by, float bz,
 int vflip_liquid_phi(int ni, int nj, int nk, float h, float radius, const float* pos, const int* cell_start,
                      const int* order, float* phi, void* hip_stream);
 
+/* ----------------------------------------------- nested FLIP coupling */
+/* The coarse solver's grids (SmokeSolver3D): ni x nj x nk cells of spacing h,
+ * MAC velocity u (ni+1,nj,nk), v (ni,nj+1,nk), w (ni,nj,nk+1), nodal solid phi
+ * ((ni+1)(nj+1)(nk+1)), MAC solid velocity us, vs, ws (NULL = zero). Device
+ * pointers, linear x-fastest. */
+typedef struct vflip_coarse_view {
+    int ni, nj, nk;
+    float h;
+    const float *u, *v, *w, *solid_phi, *us, *vs, *ws;
+} vflip_coarse_view;
+
+/* One subdomain as the coarse solver sees it (SubDomainSolver members,
+ * S/sub_domain.h:160-170): cells, spacing h (_hx), bmin and bmax as the
+ * reference stores them, and the same fields on the subdomain grid, which its
+ * samplers read at pos - bmin (S/sub_domain.cpp:60-95). */
+typedef struct vflip_subdomain_view {
+    int ni, nj, nk;
+    float h;
+    float bmin[3], bmax[3];
+    const float *u, *v, *w, *solid_phi, *us, *vs, *ws;
+} vflip_subdomain_view;
+
+/* get_velocity with the subdomain blend — S/Smoke_solver3D.cpp:71-88,
+ * S/sub_domain.cpp:70-77 and 96-105: the coarse trilinear velocity; inside
+ * is_in2 (bmin + 5h < pos < bmax - 5h, in double; S/sub_domain.h:190-201)
+ * c * v_sub + (1 - c) * v with c = clamp(d / 4h, 0, 1), d the least distance to
+ * the faces of that box. out[3*n]. Asynchronous. Bit-exact. */
+int vflip_nested_velocity(const vflip_coarse_view* C, const vflip_subdomain_view* S, int n, const float* pos,
+                          float* out, void* hip_stream);
+
+/* advect_FLIP of the coarse particles with a subdomain — S/Smoke_solver3D.cpp:
+ * 141-154 (traceRK3, c3 = (1 - c1 - c2) dt) through the blended velocity, and
+ * 1408-1430 (pos -= phi * normal where phi < 0) with get_solid_phi /
+ * get_solid_normal taking the subdomain's phi inside is_in (margin 2h,
+ * S/Smoke_solver3D.cpp:106-124). pos updated in place. Asynchronous. Bit-exact. */
+int vflip_advect_particles_nested(const vflip_coarse_view* C, const vflip_subdomain_view* S, float dt, int np,
+                                  float* pos, void* hip_stream);
+
+/* The halo copy of S/Smoke_solver3D.cpp:393-401: the particles with
+ * is_halo(pos, dx) (bmin - dx < pos < bmax + dx in float, S/sub_domain.h:203-214)
+ * in their order. P: any particle workspace (its scratch and stream).
+ * Synchronous; capacity as vflip_reorder (np always fits). Bit-exact. */
+int vflip_halo_select(vflip_particles* P, const vflip_subdomain_view* S, float dx, int np, const float* pos,
+                      const float* vel, int cap, float* out_pos, float* out_vel, int* nout);
+
+/* seed_ghost — S/Smoke_solver3D.cpp:1529-1560: for every subdomain cell whose
+ * get_solid_phi at bmin + (i,j,k) h is < 3h, 8 candidates at
+ * (i,j,k) h + 0.49 h r + bmin; a candidate whose subdomain phi lies in
+ * (-2h, 2h] is kept, with the blended get_velocity there, or get_solid_vel where
+ * that phi is <= 0. Cell-major order. r = frand(-1, 1) per axis drawn as in
+ * vflip_reorder with key 3 (8 cell + slot) + axis (the reference uses rand()).
+ * P: the subdomain's workspace (vflip_particles_create over its cells).
+ * Synchronous; capacity as vflip_reorder (8 * cells always fits). Bit-exact. */
+int vflip_seed_ghost(vflip_particles* P, const vflip_coarse_view* C, const vflip_subdomain_view* S, unsigned seed,
+                     int cap, float* out_pos, float* out_vel, int* nout);
+
+/* diffuse (PSE viscosity) — S/Smoke_solver3D.cpp:1562-1621 with PSE_kernel of
+ * S/Smoke_solver3D.h:204-209: for each of the first np_real subdomain particles
+ * (pos/vel continue with the ghosts of vflip_seed_ghost), when get_solid_phi at
+ * it is <= 2 h_coarse and its cell trunc((x - bmin)/h + 0.5) lies in the
+ * subdomain, du = sum_j (v_j - v_i) K_ij / (1e-9 + sum_j K_ij) over the particles
+ * that vflip_hash (subdomain workspace, clamp 1, bmin) bins in the 3x3x3 cells
+ * around it, in the reference's order; tau = dt nu + 1e-10. out_vel[3*np_real]
+ * = vel + du (not aliasing vel). Asynchronous. The kernel's exponential is the
+ * device's double exp: within one float rounding of the host's per term. */
+int vflip_pse_diffuse(const vflip_coarse_view* C, const vflip_subdomain_view* S, float dt, float nu, int np_real,
+                      const float* pos, const float* vel, const int* cell_start, const int* order, float* out_vel,
+                      void* hip_stream);
+
 /* ------------------------------------------------- coarse/fine coupling */
 /* FineToCoarseOperator — S/coarse_fine_solver.cpp:73-108: coarse(i,j,k) = mean of
  * the fine cells [i*ratio - offx, i*ratio + ratio - offx) x ... clipped to the

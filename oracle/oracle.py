@@ -39,6 +39,16 @@ def lib():
         L.or_reorder.argtypes = ([c_int, c_int, c_int, c_float, c_int, c_float, c_float, c_float, ctypes.c_uint,
                                  f32p, f32p, i32p, i32p, f32p, f32p, f32p, f32p, c_int, f32p, f32p])
         L.or_reseed_jitter.restype = c_float
+        vp = ctypes.c_void_p
+        L.or_nested_velocity_at.argtypes = [vp, vp, c_int, f32p, f32p]
+        L.or_advect_particles_nested.argtypes = [vp, vp, c_float, c_int, f32p]
+        L.or_halo_select.restype = c_int
+        L.or_halo_select.argtypes = [vp, c_float, c_int, f32p, f32p, c_int, f32p, f32p]
+        L.or_seed_ghost.restype = c_int
+        L.or_seed_ghost.argtypes = [vp, vp, ctypes.c_uint, c_int, f32p, f32p]
+        L.or_pse_kernel.restype = c_float
+        L.or_pse_kernel.argtypes = [f32p, f32p, c_float]
+        L.or_pse_diffuse.argtypes = [vp, vp, c_float, c_float, c_int, f32p, f32p, i32p, i32p, f32p]
         L.or_reseed_jitter.argtypes = [ctypes.c_uint, ctypes.c_uint]
         L.or_level_dims.restype = c_int
         L.or_level_dims.argtypes = [c_int, c_int, c_int, i32p, c_int]
@@ -171,6 +181,84 @@ def reorder(pos, vel, h, dims, start, order, u, v, w, phi, seed, sub=False, bmin
                          out_p, out_v)
     assert n <= cap
     return out_p[:3 * n].reshape(-1, 3), out_v[:3 * n].reshape(-1, 3)
+
+
+class _Coarse(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("h", c_float)] + \
+        [(f, ctypes.c_void_p) for f in ("u", "v", "w", "phi", "us", "vs", "ws")]
+
+
+class _Sub(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("nz", c_int), ("h", c_float), ("bmin", c_float * 3),
+                ("bmax", c_float * 3)] + [(f, ctypes.c_void_p) for f in ("u", "v", "w", "phi", "us", "vs", "ws")]
+
+
+def _views(coarse, sub):
+    """coarse / sub: dicts with h, u, v, w, phi, optional us, vs, ws (numpy, (z, y, x)); sub also bmin, bmax.
+    Returns the ctypes views and the arrays they point into (keep them alive)."""
+    keep = []
+
+    def arr(d, k):
+        a = d.get(k)
+        if a is None:
+            return None
+        a = c32(a)
+        keep.append(a)
+        return a.ctypes.data
+
+    nz, ny, nx1 = coarse["u"].shape
+    C = _Coarse(nx1 - 1, ny, nz, float(coarse["h"]), *[arr(coarse, k) for k in ("u", "v", "w", "phi", "us", "vs", "ws")])
+    snz, sny, snx1 = sub["u"].shape
+    S = _Sub(snx1 - 1, sny, snz, float(sub["h"]), (c_float * 3)(*sub["bmin"]), (c_float * 3)(*sub["bmax"]),
+             *[arr(sub, k) for k in ("u", "v", "w", "phi", "us", "vs", "ws")])
+    return C, S, keep
+
+
+def nested_velocity(coarse, sub, pos):
+    C, S, keep = _views(coarse, sub)
+    p = c32(pos).reshape(-1)
+    out = np.zeros_like(p)
+    lib().or_nested_velocity_at(ctypes.byref(C), ctypes.byref(S), p.size // 3, p, out)
+    return out.reshape(-1, 3)
+
+
+def advect_particles_nested(coarse, sub, dt, pos):
+    C, S, keep = _views(coarse, sub)
+    out = c32(pos).reshape(-1).copy()
+    lib().or_advect_particles_nested(ctypes.byref(C), ctypes.byref(S), dt, out.size // 3, out)
+    return out.reshape(-1, 3)
+
+
+def halo_select(sub, dx, pos, vel):
+    C, S, keep = _views(sub, sub)
+    p, v = c32(pos).reshape(-1), c32(vel).reshape(-1)
+    n0 = p.size // 3
+    op, ov = np.zeros(max(3 * n0, 3), np.float32), np.zeros(max(3 * n0, 3), np.float32)
+    n = lib().or_halo_select(ctypes.byref(S), dx, n0, p if n0 else np.zeros(3, np.float32),
+                             v if n0 else np.zeros(3, np.float32), n0, op, ov)
+    return op[:3 * n].reshape(-1, 3), ov[:3 * n].reshape(-1, 3)
+
+
+def seed_ghost(coarse, sub, seed):
+    C, S, keep = _views(coarse, sub)
+    cap = 8 * S.nx * S.ny * S.nz
+    op, ov = np.zeros(3 * cap, np.float32), np.zeros(3 * cap, np.float32)
+    n = lib().or_seed_ghost(ctypes.byref(C), ctypes.byref(S), seed & 0xFFFFFFFF, cap, op, ov)
+    return op[:3 * n].reshape(-1, 3), ov[:3 * n].reshape(-1, 3)
+
+
+def pse_kernel(xi, xj, tau):
+    return lib().or_pse_kernel(c32(xi), c32(xj), tau)
+
+
+def pse_diffuse(coarse, sub, dt, nu, np_real, pos, vel, start, order):
+    C, S, keep = _views(coarse, sub)
+    p, v = c32(pos).reshape(-1), c32(vel).reshape(-1)
+    out = np.zeros(3 * max(np_real, 1), np.float32)
+    order = np.ascontiguousarray(order, np.int32)
+    lib().or_pse_diffuse(ctypes.byref(C), ctypes.byref(S), dt, nu, np_real, p, v,
+                         np.ascontiguousarray(start, np.int32), order if order.size else np.zeros(1, np.int32), out)
+    return out[:3 * np_real].reshape(-1, 3)
 
 
 def reseed_jitter(seed, key):

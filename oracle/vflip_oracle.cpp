@@ -1140,4 +1140,243 @@ int or_reorder(int nx, int ny, int nz, float h, int sub, float bx, float by, flo
             }
     return out;
 }
+
+// ---------------------------------------------------------------------------
+// Nested FLIP coupling: the coarse solver's view of one subdomain.
+// OrCoarse = SmokeSolver3D's grids (nx,ny,nz cells of spacing h; MAC u, v, w;
+// nodal solid phi; MAC solid velocity us, vs, ws, NULL = zero). OrSub =
+// SubDomainSolver's members (cells, _hx, bmin, bmax, the same fields); its
+// samplers take pos - bmin (S/sub_domain.cpp:60-95).
+// ---------------------------------------------------------------------------
+struct OrCoarse {
+    int nx, ny, nz;
+    float h;
+    const float *u, *v, *w, *phi, *us, *vs, *ws;
+};
+struct OrSub {
+    int nx, ny, nz;
+    float h;
+    float bmin[3], bmax[3];
+    const float *u, *v, *w, *phi, *us, *vs, *ws;
+};
+
+static void or_mac_sample(const float* u, const float* v, const float* w, int nx, int ny, int nz, float h,
+                          const float* x, float* o) {
+    const double hd = h;
+    o[0] = u ? or_sample_linear(u, nx + 1, ny, nz, hd, 0.5, 0, 0, x[0], x[1], x[2]) : 0.f;
+    o[1] = v ? or_sample_linear(v, nx, ny + 1, nz, hd, 0, 0.5, 0, x[0], x[1], x[2]) : 0.f;
+    o[2] = w ? or_sample_linear(w, nx, ny, nz + 1, hd, 0, 0, 0.5, x[0], x[1], x[2]) : 0.f;
+}
+static void or_sub_local(const OrSub& S, const float* x, float* q) {
+    for (int a = 0; a < 3; ++a) q[a] = x[a] - S.bmin[a];
+}
+// is_in (margin 2) / is_in2 (margin 5), S/sub_domain.h:177-201: bmin + m*_hx in double.
+static bool or_sub_inside(const OrSub& S, const float* x, double m) {
+    for (int a = 0; a < 3; ++a)
+        if (!(x[a] > S.bmin[a] + m * S.h && x[a] < S.bmax[a] - m * S.h)) return false;
+    return true;
+}
+// get_boundary_coef, S/sub_domain.cpp:96-105.
+float or_boundary_coef(const OrSub& S, const float* x) {
+    const float e = 5.0f * S.h;
+    float d = 0;
+    for (int side = 0; side < 2; ++side) {
+        float m = 0;
+        for (int a = 0; a < 3; ++a) {
+            const float b = side == 0 ? S.bmin[a] + e : S.bmax[a] - e;
+            const float t = std::fabs(x[a] - b);
+            if (a == 0 || t < m) m = t;
+        }
+        d = side == 0 ? m : (m < d ? m : d);
+    }
+    const float c = d / (4.0f * S.h);
+    return c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+}
+// SmokeSolver3D::get_velocity, S/Smoke_solver3D.cpp:71-88.
+static void or_nested_velocity(const OrCoarse& C, const OrSub& S, const float* x, float* o) {
+    or_mac_sample(C.u, C.v, C.w, C.nx, C.ny, C.nz, C.h, x, o);
+    if (or_sub_inside(S, x, 5.0)) {
+        float q[3], o2[3];
+        or_sub_local(S, x, q);
+        or_mac_sample(S.u, S.v, S.w, S.nx, S.ny, S.nz, S.h, q, o2);
+        const float c = or_boundary_coef(S, x);
+        for (int a = 0; a < 3; ++a) o[a] = c * o2[a] + (1.0f - c) * o[a];
+    }
+}
+// get_solid_phi / get_solid_normal / get_solid_vel, S/Smoke_solver3D.cpp:60-124.
+static float or_nested_phi(const OrCoarse& C, const OrSub& S, const float* x) {
+    if (or_sub_inside(S, x, 2.0)) {
+        float q[3];
+        or_sub_local(S, x, q);
+        return or_sample_linear(S.phi, S.nx + 1, S.ny + 1, S.nz + 1, S.h, 0.5, 0.5, 0.5, q[0], q[1], q[2]);
+    }
+    return or_sample_linear(C.phi, C.nx + 1, C.ny + 1, C.nz + 1, C.h, 0.5, 0.5, 0.5, x[0], x[1], x[2]);
+}
+static void or_nested_normal(const OrCoarse& C, const OrSub& S, const float* x, float* g) {
+    if (or_sub_inside(S, x, 2.0)) {
+        float q[3];
+        or_sub_local(S, x, q);
+        or_sample_grad(S.phi, S.nx + 1, S.ny + 1, S.nz + 1, S.h, 0.5, 0.5, 0.5, q[0], q[1], q[2], g);
+    } else {
+        or_sample_grad(C.phi, C.nx + 1, C.ny + 1, C.nz + 1, C.h, 0.5, 0.5, 0.5, x[0], x[1], x[2], g);
+    }
+    or_normalize3(g);
+}
+static void or_nested_solid_vel(const OrCoarse& C, const OrSub& S, const float* x, float* o) {
+    if (or_sub_inside(S, x, 2.0)) {
+        float q[3];
+        or_sub_local(S, x, q);
+        or_mac_sample(S.us, S.vs, S.ws, S.nx, S.ny, S.nz, S.h, q, o);
+    } else {
+        or_mac_sample(C.us, C.vs, C.ws, C.nx, C.ny, C.nz, C.h, x, o);
+    }
+}
+
+void or_nested_velocity_at(const OrCoarse* C, const OrSub* S, int n, const float* pos, float* out) {
+    for (int p = 0; p < n; ++p) or_nested_velocity(*C, *S, pos + 3 * (size_t)p, out + 3 * (size_t)p);
+}
+
+// advect_FLIP of the coarse particles with a subdomain: traceRK3 through the
+// blended get_velocity, push-out with the subdomain's phi and normal inside
+// is_in (S/Smoke_solver3D.cpp:141-154, 1408-1430).
+void or_advect_particles_nested(const OrCoarse* Cp, const OrSub* Sp, float dt, int np, float* pos) {
+    const OrCoarse& C = *Cp;
+    const OrSub& S = *Sp;
+    const float c1 = (float)(2.0 / 9.0 * dt), c2 = (float)(3.0 / 9.0 * dt);
+    const float c3 = (float)((1.0 - c1 - c2) * dt);
+    const float t1 = (float)(0.5 * dt), t2 = (float)(0.75 * dt);
+#pragma omp parallel for
+    for (int p = 0; p < np; ++p) {
+        float x[3] = {pos[3 * p], pos[3 * p + 1], pos[3 * p + 2]};
+        float v1[3], v2[3], v3[3], m1[3], m2[3];
+        or_nested_velocity(C, S, x, v1);
+        for (int a = 0; a < 3; ++a) m1[a] = x[a] + t1 * v1[a];
+        or_nested_velocity(C, S, m1, v2);
+        for (int a = 0; a < 3; ++a) m2[a] = x[a] + t2 * v2[a];
+        or_nested_velocity(C, S, m2, v3);
+        for (int a = 0; a < 3; ++a) x[a] = x[a] + c1 * v1[a] + c2 * v2[a] + c3 * v3[a];
+        const float ph = or_nested_phi(C, S, x);
+        if (ph < 0) {
+            float g[3];
+            or_nested_normal(C, S, x, g);
+            for (int a = 0; a < 3; ++a) x[a] -= ph * g[a];
+        }
+        for (int a = 0; a < 3; ++a) pos[3 * p + a] = x[a];
+    }
+}
+
+// The halo copy of S/Smoke_solver3D.cpp:393-401: particles with is_halo(pos, dx)
+// (S/sub_domain.h:203-214: bmin - dx < pos < bmax + dx in float), in order.
+int or_halo_select(const OrSub* Sp, float dx, int np, const float* pos, const float* vel, int cap, float* out_pos,
+                   float* out_vel) {
+    const OrSub& S = *Sp;
+    int n = 0;
+    for (int p = 0; p < np; ++p) {
+        bool in = true;
+        for (int a = 0; a < 3; ++a) {
+            const float x = pos[3 * (size_t)p + a];
+            in = in && x > S.bmin[a] - dx && x < S.bmax[a] + dx;
+        }
+        if (!in) continue;
+        if (n < cap)
+            for (int a = 0; a < 3; ++a) {
+                out_pos[3 * (size_t)n + a] = pos[3 * (size_t)p + a];
+                out_vel[3 * (size_t)n + a] = vel[3 * (size_t)p + a];
+            }
+        ++n;
+    }
+    return n;
+}
+
+// seed_ghost, S/Smoke_solver3D.cpp:1529-1560: for every subdomain cell whose
+// (coarse) get_solid_phi at bmin + (i,j,k) h is < 3h, 8 candidates at
+// (i,j,k) h + 0.49 h r + bmin (r = frand(-1,1) per axis, here the counter hash
+// of or_reseed_jitter with key 3 (8 cell + slot) + axis); a candidate whose
+// subdomain phi lies in (-2h, 2h] is kept with velocity get_velocity (blended),
+// or get_solid_vel where that phi is <= 0. Cell-major order. Writes min(n, cap)
+// and returns n.
+int or_seed_ghost(const OrCoarse* Cp, const OrSub* Sp, unsigned seed, int cap, float* out_pos, float* out_vel) {
+    const OrCoarse& C = *Cp;
+    const OrSub& S = *Sp;
+    const float h = S.h;
+    int n = 0;
+    for (int k = 0; k < S.nz; ++k)
+        for (int j = 0; j < S.ny; ++j)
+            for (int i = 0; i < S.nx; ++i) {
+                const int ijk[3] = {i, j, k};
+                float cpos[3];
+                for (int a = 0; a < 3; ++a) cpos[a] = (float)ijk[a] * h + S.bmin[a];
+                if (!(or_nested_phi(C, S, cpos) < 3.0f * h)) continue;
+                const size_t c = lin(i, j, k, S.nx, S.ny);
+                for (int r = 0; r < 8; ++r) {
+                    float x[3], q[3], vv[3];
+                    for (int a = 0; a < 3; ++a) {
+                        const float jit = or_reseed_jitter(seed, (unsigned)(3 * (8 * c + (size_t)r) + a));
+                        x[a] = ((float)ijk[a] * h + (jit * 0.49f) * h) + S.bmin[a];
+                    }
+                    or_sub_local(S, x, q);
+                    const float phi =
+                        or_sample_linear(S.phi, S.nx + 1, S.ny + 1, S.nz + 1, h, 0.5, 0.5, 0.5, q[0], q[1], q[2]);
+                    if (!(phi <= 2.0 * h && phi > -2.0 * h)) continue;
+                    or_nested_velocity(C, S, x, vv);
+                    if (phi <= 0) or_nested_solid_vel(C, S, x, vv);
+                    if (n < cap)
+                        for (int a = 0; a < 3; ++a) {
+                            out_pos[3 * (size_t)n + a] = x[a];
+                            out_vel[3 * (size_t)n + a] = vv[a];
+                        }
+                    ++n;
+                }
+            }
+    return n;
+}
+
+// PSE_kernel, S/Smoke_solver3D.h:204-209 (as written: 1/frac*frac*frac).
+float or_pse_kernel(const float* xi, const float* xj, float tau) {
+    const float frac = (float)std::sqrt(4.0 * 3.1415926 * tau);
+    float d = (xi[0] - xj[0]) * (xi[0] - xj[0]);
+    d += (xi[1] - xj[1]) * (xi[1] - xj[1]);
+    d += (xi[2] - xj[2]) * (xi[2] - xj[2]);
+    return (float)((double)(1.0f / frac * frac * frac) * std::exp(-d * 0.25 / tau));
+}
+
+// diffuse, S/Smoke_solver3D.cpp:1562-1621: for each of the first np_real
+// subdomain particles (the ghosts follow them in pos/vel), when the coarse
+// get_solid_phi at it is <= 2 h_coarse and its cell (trunc((x - bmin)/h + 0.5))
+// lies in the subdomain: du = sum_j (v_j - v_i) K_ij / (1e-9 + sum_j K_ij) over
+// the particles binned (cell_start/order: or_hash clamp 1 over all particles)
+// in the 3x3x3 cells around it, z, y, x ascending, tau = dt nu + 1e-10.
+// out_vel = vel + du for the first np_real particles.
+void or_pse_diffuse(const OrCoarse* Cp, const OrSub* Sp, float dt, float nu, int np_real, const float* pos,
+                    const float* vel, const int* cell_start, const int* order, float* out_vel) {
+    const OrCoarse& C = *Cp;
+    const OrSub& S = *Sp;
+    const float tau = (float)(dt * nu + 1e-10);
+#pragma omp parallel for
+    for (int i = 0; i < np_real; ++i) {
+        const float* xi = pos + 3 * (size_t)i;
+        const float* ui = vel + 3 * (size_t)i;
+        int ijk[3];
+        for (int a = 0; a < 3; ++a) ijk[a] = trunc_i32((xi[a] - S.bmin[a]) / S.h + 0.5f);
+        float du[3] = {0.f, 0.f, 0.f};
+        if (or_nested_phi(C, S, xi) <= 2.0 * C.h && ijk[0] >= 0 && ijk[0] < S.nx && ijk[1] >= 0 && ijk[1] < S.ny &&
+            ijk[2] >= 0 && ijk[2] < S.nz) {
+            float coef = 1e-9f;
+            float ue[3] = {0.f, 0.f, 0.f};
+            for (int kk = std::max(ijk[2] - 1, 0); kk <= std::min(ijk[2] + 1, S.nz - 1); ++kk)
+                for (int jj = std::max(ijk[1] - 1, 0); jj <= std::min(ijk[1] + 1, S.ny - 1); ++jj)
+                    for (int ii = std::max(ijk[0] - 1, 0); ii <= std::min(ijk[0] + 1, S.nx - 1); ++ii) {
+                        const size_t cidx = lin(ii, jj, kk, S.nx, S.ny);
+                        for (int q = cell_start[cidx]; q < cell_start[cidx + 1]; ++q) {
+                            const size_t pj = (size_t)order[q];
+                            const float K = or_pse_kernel(xi, pos + 3 * pj, tau);
+                            for (int a = 0; a < 3; ++a) ue[a] += (vel[3 * pj + a] - ui[a]) * K;
+                            coef += K;
+                        }
+                    }
+            for (int a = 0; a < 3; ++a) du[a] = ue[a] / coef;
+        }
+        for (int a = 0; a < 3; ++a) out_vel[3 * (size_t)i + a] = ui[a] + du[a];
+    }
+}
 }  // extern "C"

@@ -781,3 +781,142 @@ def test_reseed_jitter_distribution():
     assert np.array_equal(lattice, np.round(lattice))
     assert abs(r.mean()) < 0.015 and abs((r ** 2).mean() - 1 / 3) < 0.01
     assert len(np.unique(r)) > 39900
+
+
+# ------------------------------------------------------------ nested FLIP coupling
+A_VEL, B_VEL = (0.3, -0.2, 0.1), (-0.5, 0.4, 0.25)
+
+
+def _inner_distance(sub, x):
+    """Distance (float64) from x to the faces of the box shrunk by 5 h, min over the six."""
+    e = 5 * sub["h"]
+    lo = np.asarray(sub["bmin"], np.float64) + e
+    hi = np.asarray(sub["bmax"], np.float64) - e
+    return np.minimum(np.abs(x - lo).min(1), np.abs(x - hi).min(1))
+
+
+def test_nested_velocity_blend_closed_form():
+    """get_velocity with a subdomain: the coarse value outside is_in2 (bmin + 5h, bmax - 5h), the
+    subdomain value from 4h inside that box on, and the linear ramp c = d / 4h between."""
+    coarse, sub = scenes.nested_flip_case(coarse_vel=A_VEL, sub_vel=B_VEL, box=(0.1, 0.9))
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0.05, 0.95, (4000, 3)).astype(np.float32)
+    got = O.nested_velocity(coarse, sub, x).astype(np.float64)
+    e = 5 * float(sub["h"])
+    inside = np.all((x > np.asarray(sub["bmin"]) + e) & (x < np.asarray(sub["bmax"]) - e), axis=1)
+    A, B = np.asarray(A_VEL), np.asarray(B_VEL)
+    assert np.array_equal(got[~inside], np.broadcast_to(np.float32(A), got[~inside].shape))
+    c = np.clip(_inner_distance(sub, x.astype(np.float64)) / (4 * float(sub["h"])), 0, 1)
+    want = c[:, None] * B + (1 - c[:, None]) * A
+    assert np.abs(got[inside] - want[inside]).max() < 1e-6
+    deep = inside & (c >= 1)
+    assert deep.sum() > 50 and np.array_equal(got[deep], np.broadcast_to(np.float32(B), got[deep].shape))
+    assert ((c > 0.05) & (c < 0.95) & inside).sum() > 50
+
+
+def test_nested_advection_translation_and_pushout():
+    """Equal constant velocities: RK3 moves every particle by (c1 + c2 + c3) v as without the
+    subdomain. A plane solid: particles ending behind it inside is_in land on the plane (the
+    subdomain's phi and normal), those outside are pushed out by the coarse grid's."""
+    coarse, sub = scenes.nested_flip_case(coarse_vel=A_VEL, sub_vel=A_VEL, plane=(1.0, 0.0, 0.0, -5.0))
+    rng = np.random.default_rng(1)
+    x = rng.uniform(0.15, 0.85, (3000, 3)).astype(np.float32)
+    dt = 0.02
+    got = O.advect_particles_nested(coarse, sub, dt, x)
+    ref = O.advect_particles(x, coarse["h"], np.float32(dt), coarse["u"], coarse["v"], coarse["w"], coarse["phi"])
+    assert np.array_equal(bits(got), bits(ref))
+    coarse, sub = scenes.nested_flip_case(coarse_vel=A_VEL, sub_vel=A_VEL, plane=(1.0, 0.0, 0.0, 0.5))
+    got = O.advect_particles_nested(coarse, sub, dt, x).astype(np.float64)
+    moved = x.astype(np.float64) + np.asarray(A_VEL) * (2 / 9 + 3 / 9 + (1 - 2 / 9 * dt - 3 / 9 * dt)) * dt
+    behind = moved[:, 0] < 0.5 - 1e-4
+    assert behind.sum() > 100
+    assert np.abs(got[behind, 0] - 0.5).max() < 2e-6  # on the plane (normalize's 1e-6 epsilon)
+    assert np.abs(got[~behind] - moved[~behind]).max() < 1e-5
+
+
+def test_halo_select_brute_force():
+    coarse, sub = scenes.nested_flip_case()
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(0, 1, (5000, 3)).astype(np.float32)
+    vel = rng.standard_normal((5000, 3)).astype(np.float32)
+    dx = np.float32(0.5 * coarse["h"])
+    lo = np.float32(sub["bmin"]) - dx
+    hi = np.float32(sub["bmax"]) + dx
+    keep = np.all((pos > lo) & (pos < hi), axis=1)
+    op, ov = O.halo_select(sub, float(dx), pos, vel)
+    assert np.array_equal(op, pos[keep]) and np.array_equal(ov, vel[keep])
+
+
+def test_seed_ghost_plane_closed_forms():
+    """Ghosts along a plane solid: cells whose centre lies within 3 h of it (or behind), ghosts
+    within 0.49 h of their cell centre with plane distance in (-2h, 2h], solid velocity behind
+    the plane, blended fluid velocity in front."""
+    sv = (0.7, 0.0, -0.3)
+    pl = (0.6, 0.8, 0.0, 0.62)
+    coarse, sub = scenes.nested_flip_case(coarse_vel=A_VEL, sub_vel=B_VEL, plane=pl, solid_vel=sv)
+    gp, gv = O.seed_ghost(coarse, sub, seed=3)
+    h = float(sub["h"])
+    bmin = np.asarray(sub["bmin"], np.float64)
+    d = gp.astype(np.float64) @ np.asarray(pl[:3]) - pl[3]
+    assert len(gp) > 500
+    assert np.all((d <= 2 * h + 1e-6) & (d > -2 * h - 1e-6))
+    cell = np.floor((gp - bmin) / h + 0.5).astype(int)
+    centre = cell * h + bmin
+    assert np.abs(gp - centre).max() <= 0.49 * h * (1 + 1e-5)
+    assert np.all((centre @ np.asarray(pl[:3]) - pl[3]) < 3 * h + 1e-6)
+    behind = d < -1e-6
+    assert np.abs(gv[behind] - np.asarray(sv)).max() < 1e-6
+    fluid = O.nested_velocity(coarse, sub, gp[d > 1e-6])
+    assert np.array_equal(gv[d > 1e-6], fluid)
+    ncell = sub["u"].shape[0] ** 3
+    assert len(gp) <= 8 * ncell
+
+
+def _pse_brute(coarse, sub, dt, nu, np_real, pos, vel):
+    """float64 brute force: all particles (binned by the subdomain hash rule) of the 27 cells.
+    The kernel's constant factor is frac (its 1/frac*frac*frac)."""
+    h, bmin = float(sub["h"]), np.asarray(sub["bmin"], np.float32)
+    n = sub["u"].shape[0]
+    tau = np.float32(dt * nu + 1e-10)
+    frac = np.float32(np.sqrt(4.0 * 3.1415926 * float(tau)))
+    cells = np.clip(np.trunc((pos - bmin + np.float32(0.5) * np.float32(h)) / np.float32(h)).astype(int), 0, n - 1)
+    own = np.trunc((pos - bmin) / np.float32(h) + np.float32(0.5)).astype(int)
+    out = vel.astype(np.float64).copy()[:np_real]
+    for i in range(np_real):  # (the solid-distance condition is applied by the caller)
+        if np.any(own[i] < 0) or np.any(own[i] >= n):
+            continue
+        nb = np.all(np.abs(cells - own[i]) <= 1, axis=1)
+        d2 = ((pos[nb].astype(np.float64) - pos[i]) ** 2).sum(1)
+        K = float(frac) * np.exp(-d2 * 0.25 / float(tau))
+        out[i] += ((vel[nb] - vel[i]) * K[:, None]).sum(0) / (1e-9 + K.sum())
+    return out
+
+
+def test_pse_diffuse_brute_force_and_invariants():
+    """PSE viscosity: equals a float64 brute force over the 27 neighbour cells to 1e-5; a uniform
+    velocity is unchanged; particles away from the solid (coarse phi > 2 h) are unchanged."""
+    coarse, sub = scenes.nested_flip_case(plane=(1.0, 0.0, 0.0, 0.45))
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(0.26, 0.74, (3000, 3)).astype(np.float32)
+    vel = rng.standard_normal((3000, 3)).astype(np.float32)
+    h = np.float32(sub["h"])
+    dims = (sub["u"].shape[0],) * 3
+    _, start, order = O.bin_particles(pos, h, dims, clamp=True, bmin=sub["bmin"])
+    dt, nu = np.float32(0.01), np.float32(0.02)
+    got = O.pse_diffuse(coarse, sub, dt, nu, 2500, pos, vel, start, order)
+    near = (pos[:2500, 0] - 0.45) <= 2 * float(coarse["h"])
+    assert np.array_equal(got[~near], vel[:2500][~near])
+    ref = _pse_brute(coarse, sub, float(dt), float(nu), 2500, pos, vel)
+    assert np.abs(got[near] - ref[near]).max() < 1e-5
+    assert np.abs(got[near] - vel[:2500][near]).max() > 0.1  # it did diffuse
+    same = np.tile(np.float32([0.3, -0.1, 0.2]), (3000, 1))
+    assert np.array_equal(O.pse_diffuse(coarse, sub, dt, nu, 3000, pos, same, start, order), same)
+
+
+def test_pse_kernel_value():
+    """PSE_kernel as written (1/frac*frac*frac = frac): frac * exp(-r^2 / (4 tau))."""
+    tau = np.float32(2e-4)
+    xi, xj = np.float32([0.1, 0.2, 0.3]), np.float32([0.11, 0.19, 0.305])
+    frac = np.sqrt(4 * 3.1415926 * float(tau))
+    want = frac * np.exp(-((xi.astype(np.float64) - xj) ** 2).sum() / (4 * float(tau)))
+    assert abs(O.pse_kernel(xi, xj, float(tau)) - want) < 1e-6 * want

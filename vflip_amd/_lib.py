@@ -44,6 +44,18 @@ class SolveStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class CoarseView(ctypes.Structure):
+    """vflip_coarse_view."""
+    _fields_ = [("ni", c_int), ("nj", c_int), ("nk", c_int), ("h", c_float)] + \
+        [(f, c_void_p) for f in ("u", "v", "w", "solid_phi", "us", "vs", "ws")]
+
+
+class SubdomainView(ctypes.Structure):
+    """vflip_subdomain_view."""
+    _fields_ = [("ni", c_int), ("nj", c_int), ("nk", c_int), ("h", c_float), ("bmin", c_float * 3),
+                ("bmax", c_float * 3)] + [(f, c_void_p) for f in ("u", "v", "w", "solid_phi", "us", "vs", "ws")]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "vflip_last_error": (ctypes.c_char_p, []),
@@ -64,6 +76,14 @@ SIGNATURES = {
     "vflip_liquid_phi": (c_int, [c_int, c_int, c_int, c_float, c_float, P, P, P, P, P]),
     "vflip_reorder": (c_int, [P, c_int, c_float, c_float, c_float, ctypes.c_uint, c_int, P, P, P, P, P, P, P, P,
                               c_int, P, P, PI]),
+    "vflip_nested_velocity": (c_int, [ctypes.POINTER(CoarseView), ctypes.POINTER(SubdomainView), c_int, P, P, P]),
+    "vflip_advect_particles_nested": (c_int, [ctypes.POINTER(CoarseView), ctypes.POINTER(SubdomainView), c_float,
+                                              c_int, P, P]),
+    "vflip_halo_select": (c_int, [P, ctypes.POINTER(SubdomainView), c_float, c_int, P, P, c_int, P, P, PI]),
+    "vflip_seed_ghost": (c_int, [P, ctypes.POINTER(CoarseView), ctypes.POINTER(SubdomainView), ctypes.c_uint, c_int,
+                                 P, P, PI]),
+    "vflip_pse_diffuse": (c_int, [ctypes.POINTER(CoarseView), ctypes.POINTER(SubdomainView), c_float, c_float, c_int,
+                                  P, P, P, P, P, P]),
     "vflip_batch_destroy": (c_int, [P]),
     "vflip_batch_set_graph_iterations": (c_int, [P, c_int]),
     "vflip_batch_solve": (c_int, [P, P, P, P, c_int, ctypes.POINTER(ctypes.c_double)]),

@@ -8,7 +8,7 @@ import ctypes
 
 import torch
 
-from ._lib import ERR_CAPACITY, SolveOpts, SolveStats, VflipError, check, lib  # noqa: F401
+from ._lib import ERR_CAPACITY, CoarseView, SolveOpts, SolveStats, SubdomainView, VflipError, check, lib  # noqa: F401
 
 FP32 = 0
 MIXED = 1
@@ -341,6 +341,85 @@ class Particles:
                 continue
             check(rc, "reorder")
             return out[0][:n.value], out[1][:n.value]
+
+
+def nested_views(coarse, sub):
+    """ctypes views of the coarse grids and one subdomain.
+
+    coarse / sub: dicts of device float32 tensors u, v, w (MAC shapes), phi (nodal)
+    and optional us, vs, ws, plus h; sub also bmin, bmax (3 floats each)."""
+    def ptr(d, k):
+        t = d.get(k)
+        return None if t is None else _ptr(t, name=k)
+
+    nk, nj, ni1 = coarse["u"].shape
+    C = CoarseView(ni1 - 1, nj, nk, float(coarse["h"]), *[ptr(coarse, k) for k in ("u", "v", "w", "phi", "us", "vs", "ws")])
+    snk, snj, sni1 = sub["u"].shape
+    S = SubdomainView(sni1 - 1, snj, snk, float(sub["h"]), (ctypes.c_float * 3)(*sub["bmin"]),
+                      (ctypes.c_float * 3)(*sub["bmax"]), *[ptr(sub, k) for k in ("u", "v", "w", "phi", "us", "vs", "ws")])
+    return C, S
+
+
+def nested_velocity(coarse, sub, pos):
+    """get_velocity with the subdomain blend at pos (n, 3); returns (n, 3)."""
+    C, S = nested_views(coarse, sub)
+    out = torch.empty_like(pos)
+    n = pos.shape[0]
+    check(lib().vflip_nested_velocity(ctypes.byref(C), ctypes.byref(S), n, _ptr(pos) if n else None,
+                                      _ptr(out) if n else None, _stream()), "nested_velocity")
+    return out
+
+
+def advect_particles_nested(coarse, sub, dt, pos):
+    """advect_FLIP of the coarse particles with a subdomain, in place on pos."""
+    C, S = nested_views(coarse, sub)
+    n = pos.shape[0]
+    check(lib().vflip_advect_particles_nested(ctypes.byref(C), ctypes.byref(S), float(dt), n,
+                                              _ptr(pos) if n else None, _stream()), "advect_particles_nested")
+    return pos
+
+
+def pse_diffuse(coarse, sub, dt, nu, np_real, pos, vel, start, order, out=None):
+    """diffuse (PSE viscosity): new velocities of the first np_real subdomain particles."""
+    C, S = nested_views(coarse, sub)
+    out = torch.empty((np_real, 3), device="cuda") if out is None else out
+    check(lib().vflip_pse_diffuse(ctypes.byref(C), ctypes.byref(S), float(dt), float(nu), np_real,
+                                  _ptr(pos), _ptr(vel), _ptr(start, torch.int32), _ptr(order, torch.int32),
+                                  _ptr(out) if np_real else None, _stream()), "pse_diffuse")
+    return out
+
+
+def _selection(fn, what, cap0):
+    """Runs a synchronous selection entry point fn(cap, out_pos, out_vel, nout), growing the output once."""
+    cap = cap0
+    for attempt in range(2):
+        op = torch.empty((max(cap, 1), 3), device="cuda")
+        ov = torch.empty((max(cap, 1), 3), device="cuda")
+        n = ctypes.c_int(0)
+        rc = fn(cap, _ptr(op), _ptr(ov), ctypes.byref(n))
+        if rc == ERR_CAPACITY and attempt == 0:
+            cap = n.value
+            continue
+        check(rc, what)
+        return op[:n.value], ov[:n.value]
+
+
+def halo_select(P, sub, dx, pos, vel):
+    """The particles with is_halo(pos, dx), in order: (pos, vel)."""
+    _, S = nested_views(sub, sub)
+    n = pos.shape[0]
+    return _selection(lambda cap, a, b, c: lib().vflip_halo_select(P.hnd, ctypes.byref(S), float(dx), n,
+                                                                   _ptr(pos) if n else None,
+                                                                   _ptr(vel) if n else None, cap, a, b, c),
+                      "halo_select", n)
+
+
+def seed_ghost(P, coarse, sub, seed):
+    """seed_ghost over the subdomain (P: its workspace): (pos, vel) of the ghosts."""
+    C, S = nested_views(coarse, sub)
+    return _selection(lambda cap, a, b, c: lib().vflip_seed_ghost(P.hnd, ctypes.byref(C), ctypes.byref(S),
+                                                                  int(seed) & 0xFFFFFFFF, cap, a, b, c),
+                      "seed_ghost", 2 * S.ni * S.nj * S.nk)
 
 
 def liquid_phi(h, radius, pos, cell_start, order, shape, out=None):

@@ -17,30 +17,13 @@
 #include <climits>
 
 #include "common.h"
+#include "flip_internal.h"
 #include "sampling.h"
 
-struct vflip_particles {
-    int nx = 0, ny = 0, nz = 0;
-    float h = 0;
-    size_t ncell = 0;
-    hipStream_t stream = nullptr;
-    int* slot = nullptr;     // per particle: rank inside its cell's atomic slot range
-    int* tmp = nullptr;      // per particle: merge buffer of the long-segment sort
-    int* big = nullptr;      // [0]: count, [1..]: cells whose segment needs the long sort
-    int* partial = nullptr;  // scan partials
-    int* reo = nullptr;      // reorder: per-cell survivor counts/offsets [ncell+1], masks [ncell]
-    size_t cap = 0, big_cap = 0, npartial = 0;
-};
 
 namespace vflip {
 
 // ----------------------------------------------------------------- binning
-// Truncating float->int of Vec3i(Vec3f) (S/vec.h:37-39); NaN and values
-// outside int range give INT_MIN, the x86-64 conversion result.
-__device__ inline int trunc_i32(float t) {
-    if (!(t > -2147483648.0f && t < 2147483648.0f)) return INT_MIN;
-    return (int)t;
-}
 
 struct BinGeom {
     float h;
@@ -932,13 +915,6 @@ struct ReoArgs {
     const float* phi;
 };
 
-// frand(-1, 1) (S/Smoke_solver3D.h:36-39: a + (b - a) U) with U from the
-// counter hash: U = (randhash(seed * 0x9E3779B1 + key) >> 8) / 2^24.
-__device__ inline float reseed_jitter(unsigned seed, unsigned key) {
-    const unsigned r = randhash(seed * 0x9E3779B1u + key);
-    const float u = (float)(r >> 8) * (1.0f / 16777216.0f);
-    return -1.0f + 2.0f * u;
-}
 
 // New particle `slot` of cell c = (i, j, k): (bmin +) (i,j,k) h + (r * 0.49) h,
 // key 3 (8 c + slot) + axis.
@@ -1163,6 +1139,19 @@ static int ensure_capacity(vflip_particles* P, int np) {
     return 0;
 }
 
+size_t scan_chunk() { return kScanChunk; }
+
+void scan_exclusive_i32(int* a, size_t n, int* partial, hipStream_t s) {
+    const unsigned nb = (unsigned)((n + kScanChunk - 1) / kScanChunk);
+    if (nb == 0) {
+        k_fill_i32<<<1, kBlock, 0, s>>>(a, 1, 0);
+        return;
+    }
+    k_scan_reduce<<<nb, kBlock, 0, s>>>(n, a, partial);
+    k_scan_partials<<<1, 1024, 0, s>>>((int)nb, partial, a + n);
+    k_scan_apply<<<nb, kBlock, 0, s>>>(n, a, partial);
+}
+
 }  // namespace vflip
 
 using namespace vflip;
@@ -1227,6 +1216,8 @@ int vflip_particles_destroy(vflip_particles* P) {
     if (P->big) (void)hipFree(P->big);
     if (P->partial) (void)hipFree(P->partial);
     if (P->reo) (void)hipFree(P->reo);
+    if (P->pflag) (void)hipFree(P->pflag);
+    if (P->ppartial) (void)hipFree(P->ppartial);
     delete P;
     return 0;
 }

@@ -179,3 +179,45 @@ def affine_mac(ni, nj, nk, h, coef):
         c = coef[comp]
         out.append((c[0] + c[1] * x[0] + c[2] * x[1] + c[3] * x[2]).astype(np.float32))
     return out
+
+
+def nested_flip_case(n=16, ratio=2, box=(0.25, 0.75), seed=0, coarse_vel=None, sub_vel=None, plane=None,
+                     solid_vel=(0.0, 0.0, 0.0)):
+    """A coarse n^3 grid (h = 1/n) and one subdomain over [box)^3 at `ratio`x (the
+    SubDomainSolver members: cells, h, bmin, bmax = bmin + cells h in float), for
+    the nested FLIP coupling. Velocities: constant vectors when given, else seeded
+    random MAC fields. Solid: the plane phi = x . plane[:3] - plane[3] (unit normal)
+    evaluated at the nodes of both grids, else a sphere of radius 0.2 at the centre.
+    Solid velocity constant. Returns (coarse, sub) dicts of float32 arrays."""
+    h = np.float32(1.0 / n)
+    hs = np.float32(h / ratio)
+    m = int(round((box[1] - box[0]) * n * ratio))
+    bmin = np.array([box[0]] * 3, np.float32)
+    bmax = (bmin + np.float32(m) * hs).astype(np.float32)
+
+    def fields(cells, hh, origin, vel, s):
+        out = {}
+        for comp, name in enumerate("uvw"):
+            shape = [cells, cells, cells]
+            shape[2 - comp] += 1
+            if vel is None:
+                out[name] = (np.stack([uniform01(seed + s, 600 + comp, np.arange(int(np.prod(shape))))], 0)[0]
+                             .reshape(shape) * 2 - 1).astype(np.float32)
+            else:
+                out[name] = np.full(shape, vel[comp], np.float32)
+            out[name + "s"] = np.full(shape, solid_vel[comp], np.float32)
+        g = (np.arange(cells + 1, dtype=np.float64) - 0.5) * float(hh)
+        z, y, x = np.meshgrid(g + origin[2], g + origin[1], g + origin[0], indexing="ij")
+        if plane is not None:
+            phi = x * plane[0] + y * plane[1] + z * plane[2] - plane[3]
+        else:
+            phi = np.sqrt((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2) - 0.2
+        out["phi"] = phi.astype(np.float32)
+        out["us"], out["vs"], out["ws"] = out.pop("us"), out.pop("vs"), out.pop("ws")
+        out["h"] = hh
+        return out
+
+    coarse = fields(n, h, (0.0, 0.0, 0.0), coarse_vel, 0)
+    sub = fields(m, hs, tuple(float(b) for b in bmin), sub_vel, 1)
+    sub["bmin"], sub["bmax"] = tuple(float(b) for b in bmin), tuple(float(b) for b in bmax)
+    return coarse, sub
