@@ -32,6 +32,16 @@
 //                                         plane solid closed forms, idempotence;
 //                                         the reseed RNG replaces rand() (no vector
 //                                         can pin rand() across platforms)
+//   nested coupling: get_velocity blend . closed forms (coarse value outside is_in2,
+//                                         subdomain value deep inside, the 4h ramp)
+//   advect_particles_nested ............. translation equals the plain coarse
+//                                         advection; plane push-out lands on the plane
+//   halo_select ......................... brute-force membership and order
+//   seed_ghost .......................... plane-solid closed forms (cells, bands,
+//                                         solid velocity behind, blend in front)
+//   pse_diffuse ......................... float64 brute force over the 27 cells,
+//                                         uniform velocity unchanged, far particles
+//                                         unchanged, kernel value closed form
 //
 // Floating point: compiled with -ffp-contract=off so float expressions round
 // exactly as written (the reference's MSVC /fp:precise build does the same).
