@@ -20,6 +20,7 @@ timed steps. `cpu_baseline` times the FP64 CPU oracle (a restatement of the
 reference's TBB path) on one step of the same workload on the host cores.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
+  python bench.py --rows   (each scope row's GPU call beside its CPU oracle path)
 """
 import argparse
 import json
@@ -128,6 +129,149 @@ def cpu_baseline(sim_state, phi, n, dt=0.01):
     }
 
 
+def _gpu_ms(fn, reps=10):
+    """Median event-timed duration (ms) of fn() on the current stream, after one untimed call."""
+    import numpy as np
+    import torch
+    fn()
+    ts = []
+    for _ in range(reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        torch.cuda.synchronize()
+        ts.append(a.elapsed_time(b))
+    return float(np.median(ts))
+
+
+def _cpu_ms(fn):
+    t0 = time.perf_counter()
+    out = fn()
+    return (time.perf_counter() - t0) * 1e3, out
+
+
+def row_baselines():
+    """Each scope row's GPU call and the CPU oracle (the reference algorithm restated,
+    OpenMP where the reference's loop is a TBB parallel_for) on the same seeded input.
+    Returns a list of dicts; GPU times are medians of 10 event-timed calls."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    from vflip_amd import api, scenes
+    cores = O.lib().or_num_threads()
+    rows = []
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+    def add(row, workload, gpu_ms, cpu_ms, cpu_cores, parity, **extra):
+        rows.append({"row": row, "workload": workload, "gpu_ms": round(gpu_ms, 4), "cpu_ms": round(cpu_ms, 1),
+                     "cpu_cores": cpu_cores, "cpu_kind": "port", "gpu_over_cpu": round(cpu_ms / gpu_ms, 1),
+                     "parity": parity, **extra})
+        log(f"row {row}: gpu {gpu_ms:.3f} ms, cpu {cpu_ms:.1f} ms ({cpu_cores} cores)")
+
+    n = 128
+    h = np.float32(1.0 / n)
+    pb = scenes.box_problem(n, n, n, 3)
+    g = {k: t(v) for k, v in pb.items() if isinstance(v, np.ndarray)}
+    uw, vw, ww = api.face_weights(g["phi"])
+    # a14 advection of one face component
+    out = torch.empty_like(g["u"])
+    gm0 = _gpu_ms(lambda: api.advect_rk2(float(h), 0.01, g["u"], g["v"], g["w"], g["u"], (0.5, 0, 0), out, mode=0))
+    gm2 = _gpu_ms(lambda: api.advect_rk2(float(h), 0.01, g["u"], g["v"], g["w"], g["u"], (0.5, 0, 0), out, mode=2))
+    cm, _ = _cpu_ms(lambda: O.advect_rk2(h, np.float32(0.01), pb["u"], pb["v"], pb["w"], pb["u"], (0.5, 0, 0)))
+    add("a14 advect_field (RK2, trilinear)", f"u component, {n}^3", gm0, cm, cores, "bit-exact (FP64 mode 0)",
+        gpu_ms_fp32_mode2=round(gm2, 4))
+    # a7 assembly, a13 gradient
+    S = api.Solver(n, n, n, precision=api.FP32)
+    rhs = torch.empty((n, n, n), device="cuda")
+    ga = _gpu_ms(lambda: S.assemble(api.COARSE, float(h), 1.0, g["u"], g["v"], g["w"], uw, vw, ww, g["us"], g["vs"],
+                                    g["ws"], rhs))
+    uwn, vwn, wwn = O.face_weights(pb["phi"])
+    ca, asm = _cpu_ms(lambda: O.assemble(h, pb["u"], pb["v"], pb["w"], uwn, vwn, wwn, pb["us"], pb["vs"], pb["ws"]))
+    add("a7 pcg_projection assembly", f"{n}^3 box + sphere", ga, ca, cores, "within one FP32 rounding of FP64")
+    pres = np.random.default_rng(0).standard_normal((n, n, n))
+    gp = t(pres.astype(np.float32))
+    uu, vv, wv = g["u"].clone(), g["v"].clone(), g["w"].clone()
+    gg = _gpu_ms(lambda: api.grad_apply(api.COARSE, float(h), 1.0, gp, uw, vw, ww, g["us"], g["vs"], g["ws"], uu, vv,
+                                        wv))
+    cg, _ = _cpu_ms(lambda: O.grad_apply(h, pres, uwn, vwn, wwn, pb["us"], pb["vs"], pb["ws"], pb["u"], pb["v"],
+                                         pb["w"]))
+    add("a13 pressure gradient", f"{n}^3", gg, cg, cores, "bit-exact (FP64 pressure)")
+    del S
+    # a17 binning and a18 rebuild: the FLIP smoke set, 8 per cell over 1.6 cells, shuffled
+    pos = scenes.block_particles(n, 8, 21, hi=(1.0, 1.0, 1.0), jitter=1.6)
+    pos = np.ascontiguousarray(pos[np.random.default_rng(21).permutation(len(pos))])
+    vel = scenes.particle_velocities(len(pos), 21)
+    P = api.Particles(n, n, n, float(h))
+    tp, tv = t(pos), t(vel)
+    gh = _gpu_ms(lambda: P.hash(tp, count=False))
+    ch, (rk, rs, ro) = _cpu_ms(lambda: O.bin_particles(pos, h, (n, n, n)))
+    add("a17 hash_FLIP", f"{len(pos) / 1e6:.1f}M particles, {n}^3, random order", gh, ch, 1, "bit-exact")
+    grids = scenes.random_mac(n, 5, scale=1.0)
+    phi = scenes.solid_phi(n, 1.0 / n, [((0.5, 0.5, 0.5), 0.15)], open_top=False)
+    tg = [t(a) for a in grids]
+    tphi = t(phi)
+    _, start, order, _ = P.hash(tp)
+    bufs = (torch.empty((len(pos) + 8 * n ** 3, 3), device="cuda"), torch.empty((len(pos) + 8 * n ** 3, 3),
+                                                                                  device="cuda"))
+    gr = _gpu_ms(lambda: P.reorder(tp, tv, start, order, *tg, tphi, 1, out=bufs))
+    cr, _ = _cpu_ms(lambda: O.reorder(pos, vel, h, (n, n, n), rs, ro, *grids, phi, 1))
+    add("a18 reorder_FLIP", f"{len(pos) / 1e6:.1f}M particles, {n}^3, random order, box + sphere", gr, cr, 1,
+        "bit-exact")
+    del tp, tv, bufs
+    # a19 P2G, a20 G2P, a15 particle advection: the dam-break fill
+    pos = scenes.block_particles(n, 8, 0)
+    vel = scenes.particle_velocities(len(pos), 0)
+    tp, tv = t(pos), t(vel)
+    _, start, order, _ = P.hash(tp)
+    _, rs, ro = O.bin_particles(pos, h, (n, n, n))
+    gp0 = _gpu_ms(lambda: P.p2g(tp, tv, start, order, method=0))
+    gp1 = _gpu_ms(lambda: P.p2g(tp, tv, start, order, method=1), reps=3)
+    specs = ((0, (n, n, n + 1), (0.5, 0, 0)), (1, (n, n + 1, n), (0, 0.5, 0)), (2, (n + 1, n, n), (0, 0, 0.5)))
+    cp, _ = _cpu_ms(lambda: [O.p2g(pos, vel, c, h, (n, n, n), rs, ro, fs, of) for c, fs, of in specs])
+    add("a19 particle_to_grid", f"{len(pos) / 1e6:.2f}M particles (dam-break fill), {n}^3, 3 components", gp0, cp,
+        cores, "FP32 rounding (owner tiles); exact-order method bit-exact", gpu_ms_exact_order=round(gp1, 3))
+    du = [a * np.float32(0.1) for a in grids]
+    tdu = [t(a) for a in du]
+    vv = tv.clone()
+    gq = _gpu_ms(lambda: api.g2p(float(h), tp, vv, tdu))
+    cq, _ = _cpu_ms(lambda: O.g2p(pos, vel, h, (n, n, n), *du))
+    add("a20 particle_from_grid", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gq, cq, cores, "bit-exact")
+    pp = tp.clone()
+    gv = _gpu_ms(lambda: api.advect_particles(float(h), 0.01, pp, *tg, tphi))
+    cv, _ = _cpu_ms(lambda: O.advect_particles(pos, h, np.float32(0.01), *grids, phi))
+    add("a15 advect_FLIP (RK3 + push-out)", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gv, cv, cores, "bit-exact")
+    # C3 particle level set
+    lphi = torch.empty((n, n, n), device="cuda")
+    gl = _gpu_ms(lambda: api.liquid_phi(float(h), 0.9 * float(h), tp, start, order, (n, n, n), out=lphi))
+    cl, _ = _cpu_ms(lambda: O.liquid_phi(pos, h, (n, n, n), np.float32(0.9) * h))
+    add("C3 particle level set", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gl, cl, 1, "bit-exact")
+    del P, tp, tv
+    # nested coupling: ghosts and PSE viscosity
+    coarse, sub = scenes.nested_flip_case(n=n, box=(0.3, 0.7), seed=1, solid_vel=(0.2, -0.1, 0.05))
+    dev = lambda d: {k: (t(v) if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+    C, Sd = dev(coarse), dev(sub)
+    m = sub["u"].shape[0]
+    Ps = api.Particles(m, m, m, float(sub["h"]))
+    gs = _gpu_ms(lambda: api.seed_ghost(Ps, C, Sd, 3))
+    cs, (gpos, gvel) = _cpu_ms(lambda: O.seed_ghost(coarse, sub, 3))
+    add("seed_ghost", f"{m}^3 subdomain at 2x, sphere", gs, cs, 1, "bit-exact")
+    rng = np.random.default_rng(3)
+    real = (np.asarray(sub["bmin"], np.float32) + rng.uniform(0, m * float(sub["h"]), (8 * m ** 3, 3))).astype(
+        np.float32)
+    rvel = rng.standard_normal(real.shape).astype(np.float32)
+    allp, allv = np.concatenate([real, gpos]), np.concatenate([rvel, gvel])
+    tap, tav = t(allp), t(allv)
+    _, sst, sor, _ = Ps.hash(tap, clamp=True, bmin=sub["bmin"])
+    _, rs, ro = O.bin_particles(allp, np.float32(sub["h"]), (m, m, m), clamp=True, bmin=sub["bmin"])
+    gd = _gpu_ms(lambda: api.pse_diffuse(C, Sd, 0.01, 0.05, len(real), tap, tav, sst, sor))
+    cd, _ = _cpu_ms(lambda: O.pse_diffuse(coarse, sub, np.float32(0.01), np.float32(0.05), len(real), allp, allv, rs,
+                                          ro))
+    add("diffuse (PSE viscosity)", f"{len(real) / 1e6:.2f}M subdomain particles + {len(gpos) / 1e6:.2f}M ghosts",
+        gd, cd, cores, "within 1e-6 (device exp)")
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +279,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--rows", action="store_true",
+                    help="instead of the headline step: each scope row's GPU call beside its CPU oracle path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -146,6 +292,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
+    if args.rows:
+        print(json.dumps({"rows": row_baselines()}), flush=True)
+        return
     n = args.n
     phi, spheres = make_scene(n)
     log(f"scene n={n} spheres={spheres}")

@@ -396,6 +396,40 @@ int vflip_pse_diffuse(const vflip_coarse_view* C, const vflip_subdomain_view* S,
                       const float* pos, const float* vel, const int* cell_start, const int* order, float* out_vel,
                       void* hip_stream);
 
+/* ------------------------------------------------------------- tracers */
+/* Smoke tracers are float4 (x, y, z, density) arrays, 16-byte aligned: the
+ * reference's Vec4f tracers (S/Smoke_solver3D.h:45-161). */
+
+/* advect_tracers — S/Smoke_solver3D.h:101-125: traceRK3 through get_velocity
+ * (with the subdomain blend when S is not NULL; coarse c3 = (1 - c1 - c2) dt),
+ * pos -= phi * normal where get_solid_phi <= 0, density d / (1 + decay dt) in
+ * double. In place. Asynchronous. Bit-exact. */
+int vflip_advect_tracers(const vflip_coarse_view* C, const vflip_subdomain_view* S, float dt, float dens_decay,
+                         int n, float* tracers, void* hip_stream);
+
+/* The sphere emitter of emit_tracers — S/Smoke_solver3D.h:67-82: candidates
+ * (x, y, z) = frand(-r - hx, r + hx) per axis, accepted when
+ * x^2 + z^2 + y^2 <= r^2; the first n_emit accepted in candidate order become
+ * tracers (cx + x, cy + y, cz + z, 1). frand(a, b) = a + (b - a) U with
+ * U = (randhash(seed * 0x9E3779B1 + 3 t + axis) >> 8) / 2^24 for candidate t
+ * (the reference draws rand()). out[4 * n_emit]. P: any particle workspace
+ * (scratch and stream). Synchronous. Bit-exact. */
+int vflip_emit_tracers_sphere(vflip_particles* P, unsigned seed, int n_emit, float cx, float cy, float cz, float r,
+                              float hx, float* out);
+
+/* The subdomain part of emit_tracers — S/Smoke_solver3D.h:84-96: per subdomain
+ * cell (cell-major) 4 candidates bmin + (i,j,k) h + U h (U = frand(0, 1) per
+ * axis, key 3 (4 cell + slot) + axis), kept where 0 < get_solid_phi < h, as
+ * tracers with density 1. P: the subdomain's workspace. Synchronous; capacity
+ * as vflip_reorder (4 * cells always fits). Bit-exact. */
+int vflip_emit_tracers_sub(vflip_particles* P, const vflip_coarse_view* C, const vflip_subdomain_view* S,
+                           unsigned seed, int cap, float* out, int* nout);
+
+/* The record of write_tracers — S/Smoke_solver3D.h:126-148: out = (x/lx - 0.5,
+ * y/lx, z/lx - 0.5, d) per tracer; the file Particle_data%04d.bin is these
+ * records, raw float32. Asynchronous. Bit-exact. */
+int vflip_tracer_frame(int n, const float* tracers, float lx, float* out, void* hip_stream);
+
 /* ------------------------------------------------- coarse/fine coupling */
 /* FineToCoarseOperator — S/coarse_fine_solver.cpp:73-108: coarse(i,j,k) = mean of
  * the fine cells [i*ratio - offx, i*ratio + ratio - offx) x ... clipped to the

@@ -46,6 +46,11 @@ def lib():
         L.or_halo_select.argtypes = [vp, c_float, c_int, f32p, f32p, c_int, f32p, f32p]
         L.or_seed_ghost.restype = c_int
         L.or_seed_ghost.argtypes = [vp, vp, ctypes.c_uint, c_int, f32p, f32p]
+        L.or_advect_tracers.argtypes = [vp, vp, c_float, c_float, c_int, f32p]
+        L.or_emit_sphere.argtypes = [ctypes.c_uint, c_int, c_float, c_float, c_float, c_float, c_float, f32p]
+        L.or_emit_sub_tracers.restype = c_int
+        L.or_emit_sub_tracers.argtypes = [vp, vp, ctypes.c_uint, c_int, f32p]
+        L.or_tracer_frame.argtypes = [c_int, f32p, c_float, f32p]
         L.or_pse_kernel.restype = c_float
         L.or_pse_kernel.argtypes = [f32p, f32p, c_float]
         L.or_pse_diffuse.argtypes = [vp, vp, c_float, c_float, c_int, f32p, f32p, i32p, i32p, f32p]
@@ -259,6 +264,40 @@ def pse_diffuse(coarse, sub, dt, nu, np_real, pos, vel, start, order):
     lib().or_pse_diffuse(ctypes.byref(C), ctypes.byref(S), dt, nu, np_real, p, v,
                          np.ascontiguousarray(start, np.int32), order if order.size else np.zeros(1, np.int32), out)
     return out[:3 * np_real].reshape(-1, 3)
+
+
+def advect_tracers(coarse, sub, dt, decay, tracers):
+    """advect_tracers of float4 tracers (n, 4); sub None = no subdomain."""
+    if sub is None:
+        C, _, keep = _views(coarse, dict(coarse, bmin=(0, 0, 0), bmax=(0, 0, 0)))
+        Sref = None
+    else:
+        C, S, keep = _views(coarse, sub)
+        Sref = ctypes.byref(S)
+    out = c32(tracers).reshape(-1).copy()
+    lib().or_advect_tracers(ctypes.byref(C), Sref, dt, decay, out.size // 4, out)
+    return out.reshape(-1, 4)
+
+
+def emit_sphere(seed, n_emit, centre, r, hx):
+    out = np.zeros(4 * max(n_emit, 1), np.float32)
+    lib().or_emit_sphere(seed & 0xFFFFFFFF, n_emit, *[float(c) for c in centre], r, hx, out)
+    return out[:4 * n_emit].reshape(-1, 4)
+
+
+def emit_sub_tracers(coarse, sub, seed):
+    C, S, keep = _views(coarse, sub)
+    cap = 4 * S.nx * S.ny * S.nz
+    out = np.zeros(4 * cap, np.float32)
+    n = lib().or_emit_sub_tracers(ctypes.byref(C), ctypes.byref(S), seed & 0xFFFFFFFF, cap, out)
+    return out[:4 * n].reshape(-1, 4)
+
+
+def tracer_frame(tracers, lx):
+    t = c32(tracers).reshape(-1)
+    out = np.zeros_like(t)
+    lib().or_tracer_frame(t.size // 4, t, lx, out)
+    return out.reshape(-1, 4)
 
 
 def reseed_jitter(seed, key):

@@ -39,6 +39,10 @@
 //   halo_select ......................... brute-force membership and order
 //   seed_ghost .......................... plane-solid closed forms (cells, bands,
 //                                         solid velocity behind, blend in front)
+//   tracers: advect ..................... translation, decay closed form, plain vs
+//                                         nested-without-subdomain identity
+//   tracers: emitters, frame ............ inside-sphere/count, phi band with a
+//                                         plane, frame record closed form
 //   pse_diffuse ......................... float64 brute force over the 27 cells,
 //                                         uniform velocity unchanged, far particles
 //                                         unchanged, kernel value closed form
@@ -1387,6 +1391,120 @@ void or_pse_diffuse(const OrCoarse* Cp, const OrSub* Sp, float dt, float nu, int
             for (int a = 0; a < 3; ++a) du[a] = ue[a] / coef;
         }
         for (int a = 0; a < 3; ++a) out_vel[3 * (size_t)i + a] = ui[a] + du[a];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tracers: Vec4f (x, y, z, density) per tracer, S/Smoke_solver3D.h:45-161.
+// With Sp == NULL the coarse grids alone are sampled (no subdomain).
+// ---------------------------------------------------------------------------
+static OrSub or_no_sub() {
+    OrSub S{};
+    for (int a = 0; a < 3; ++a) {
+        S.bmin[a] = INFINITY;
+        S.bmax[a] = -INFINITY;
+    }
+    S.h = 1.0f;
+    return S;
+}
+
+// advect_tracers, S/Smoke_solver3D.h:101-125: traceRK3 (coarse c3), push-out
+// where get_solid_phi <= 0, density d / (1 + decay dt).
+void or_advect_tracers(const OrCoarse* Cp, const OrSub* Sp, float dt, float decay, int n, float* tr) {
+    const OrCoarse& C = *Cp;
+    const OrSub S = Sp ? *Sp : or_no_sub();
+    const float c1 = (float)(2.0 / 9.0 * dt), c2 = (float)(3.0 / 9.0 * dt);
+    const float c3 = (float)((1.0 - c1 - c2) * dt);
+    const float t1 = (float)(0.5 * dt), t2 = (float)(0.75 * dt);
+#pragma omp parallel for
+    for (int p = 0; p < n; ++p) {
+        float x[3] = {tr[4 * p], tr[4 * p + 1], tr[4 * p + 2]};
+        float v1[3], v2[3], v3[3], m1[3], m2[3];
+        or_nested_velocity(C, S, x, v1);
+        for (int a = 0; a < 3; ++a) m1[a] = x[a] + t1 * v1[a];
+        or_nested_velocity(C, S, m1, v2);
+        for (int a = 0; a < 3; ++a) m2[a] = x[a] + t2 * v2[a];
+        or_nested_velocity(C, S, m2, v3);
+        for (int a = 0; a < 3; ++a) x[a] = x[a] + c1 * v1[a] + c2 * v2[a] + c3 * v3[a];
+        const float ph = or_nested_phi(C, S, x);
+        if (ph <= 0) {
+            float g[3];
+            or_nested_normal(C, S, x, g);
+            for (int a = 0; a < 3; ++a) x[a] -= ph * g[a];
+        }
+        const float d = (float)(tr[4 * p + 3] / (1.0 + decay * dt));
+        tr[4 * p] = x[0];
+        tr[4 * p + 1] = x[1];
+        tr[4 * p + 2] = x[2];
+        tr[4 * p + 3] = d;
+    }
+}
+
+// frand(a, b) = a + (b - a) U with U from the counter hash (as or_reseed_jitter).
+static float or_frand(unsigned seed, unsigned key, float a, float b) {
+    const unsigned r = or_randhash(seed * 0x9E3779B1u + key);
+    const float u = (float)(r >> 8) * (1.0f / 16777216.0f);
+    return a + (b - a) * u;
+}
+
+// The sphere emitter of emit_tracers, S/Smoke_solver3D.h:67-82: candidates
+// (x, y, z) = frand(-r - hx, r + hx) per axis (key 3 t + axis for candidate t)
+// accepted when x^2 + z^2 + y^2 <= r^2, the first n_emit accepted in candidate
+// order, at centre + (x, y, z) with density 1. out[4 * n_emit].
+void or_emit_sphere(unsigned seed, int n_emit, float cx, float cy, float cz, float r, float hx, float* out) {
+    int got = 0;
+    for (unsigned t = 0; got < n_emit; ++t) {
+        const float x = or_frand(seed, 3 * t, -r - hx, r + hx);
+        const float y = or_frand(seed, 3 * t + 1, -r - hx, r + hx);
+        const float z = or_frand(seed, 3 * t + 2, -r - hx, r + hx);
+        if (x * x + z * z + y * y <= r * r) {
+            out[4 * (size_t)got] = cx + x;
+            out[4 * (size_t)got + 1] = cy + y;
+            out[4 * (size_t)got + 2] = cz + z;
+            out[4 * (size_t)got + 3] = 1.0f;
+            ++got;
+        }
+    }
+}
+
+// The subdomain part of emit_tracers, S/Smoke_solver3D.h:84-96: per subdomain
+// cell (cell-major) 4 candidates bmin + (i,j,k) h + U h (U = frand(0, 1) per
+// axis, key 3 (4 cell + slot) + axis), kept where 0 < get_solid_phi < h.
+// Writes min(n, cap) float4 tracers and returns n.
+int or_emit_sub_tracers(const OrCoarse* Cp, const OrSub* Sp, unsigned seed, int cap, float* out) {
+    const OrCoarse& C = *Cp;
+    const OrSub& S = *Sp;
+    int n = 0;
+    for (int k = 0; k < S.nz; ++k)
+        for (int j = 0; j < S.ny; ++j)
+            for (int i = 0; i < S.nx; ++i) {
+                const int ijk[3] = {i, j, k};
+                const size_t c = lin(i, j, k, S.nx, S.ny);
+                for (int q = 0; q < 4; ++q) {
+                    float x[3];
+                    for (int a = 0; a < 3; ++a) {
+                        const float u = or_frand(seed, (unsigned)(3 * (4 * c + (size_t)q) + a), 0.0f, 1.0f);
+                        x[a] = (S.bmin[a] + (float)ijk[a] * S.h) + u * S.h;
+                    }
+                    const float ph = or_nested_phi(C, S, x);
+                    if (!(ph > 0 && ph < S.h)) continue;
+                    if (n < cap) {
+                        for (int a = 0; a < 3; ++a) out[4 * (size_t)n + a] = x[a];
+                        out[4 * (size_t)n + 3] = 1.0f;
+                    }
+                    ++n;
+                }
+            }
+    return n;
+}
+
+// write_tracers' record, S/Smoke_solver3D.h:126-148: (x/lx - 0.5, y/lx, z/lx - 0.5, d).
+void or_tracer_frame(int n, const float* tr, float lx, float* out) {
+    for (int p = 0; p < n; ++p) {
+        out[4 * (size_t)p] = (float)(tr[4 * (size_t)p] / lx - 0.5);
+        out[4 * (size_t)p + 1] = tr[4 * (size_t)p + 1] / lx;
+        out[4 * (size_t)p + 2] = (float)(tr[4 * (size_t)p + 2] / lx - 0.5);
+        out[4 * (size_t)p + 3] = tr[4 * (size_t)p + 3];
     }
 }
 }  // extern "C"

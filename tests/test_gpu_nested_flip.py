@@ -1,8 +1,9 @@
-"""GPU nested FLIP coupling against the oracle.
+"""GPU nested FLIP coupling and smoke tracers against the oracle.
 
 Bit-exact: the blended get_velocity, the nested particle advection (RK3 through
-the blend, push-out with the subdomain's phi inside is_in), the halo copy and the
-ghost seeding. The PSE viscosity sums exp() terms whose double exponential comes
+the blend, push-out with the subdomain's phi inside is_in), the halo copy, the
+ghost seeding, tracer advection (with and without the subdomain), both tracer
+emitters and the frame records. The PSE viscosity sums exp() terms whose double exponential comes
 from the device's math library on the GPU and from the host's in the oracle, so it
 is held to 1e-6 relative to the velocity scale, with nearly all values equal.
 """
@@ -123,3 +124,37 @@ def test_pse_diffuse_matches_oracle(cuda):
     assert changed.sum() > 1000
     assert np.abs(got - ref).max() <= 1e-6 * np.abs(rvel).max()
     assert (bits(got) == bits(ref)).all(1).mean() > 0.99
+
+
+@pytest.mark.parametrize("with_sub", [False, True])
+def test_advect_tracers_bitexact(cuda, with_sub):
+    print(f"advect tracers sub={with_sub}", flush=True)
+    from oracle import oracle as O
+    from vflip_amd import api
+    coarse, sub = _case()
+    rng = np.random.default_rng(8)
+    tr = np.concatenate([rng.uniform(0.05, 0.95, (20000, 3)), rng.uniform(0.5, 1.0, (20000, 1))], 1).astype(np.float32)
+    g = _t(tr)
+    api.advect_tracers(_dev(coarse), _dev(sub) if with_sub else None, 0.03, 0.2, g)
+    ref = O.advect_tracers(coarse, sub if with_sub else None, np.float32(0.03), np.float32(0.2), tr)
+    assert np.array_equal(bits(_np(g)), bits(ref))
+
+
+def test_tracer_emitters_and_frame_bitexact(cuda, tmp_path):
+    print("tracer emitters", flush=True)
+    from oracle import oracle as O
+    from vflip_amd import api
+    coarse, sub = _case()
+    P = api.Particles(24, 24, 24, float(coarse["h"]))
+    e = _np(api.emit_tracers_sphere(P, 11, 20000, (0.5, 0.1, 0.5), 0.08, float(coarse["h"])))
+    assert np.array_equal(bits(e), bits(O.emit_sphere(11, 20000, (0.5, 0.1, 0.5), 0.08, float(coarse["h"]))))
+    m = sub["u"].shape[0]
+    Ps = api.Particles(m, m, m, float(sub["h"]))
+    st = _np(api.emit_tracers_sub(Ps, _dev(coarse), _dev(sub), 12))
+    rs = O.emit_sub_tracers(coarse, sub, 12)
+    assert len(rs) > 500 and np.array_equal(bits(st), bits(rs))
+    allt = np.concatenate([e, st])
+    path = api.write_tracers(str(tmp_path), 7, _t(allt), 1.0)
+    assert path.endswith("Particle_data0007.bin")
+    rec = np.fromfile(path, "<f4").reshape(-1, 4)
+    assert np.array_equal(bits(rec), bits(O.tracer_frame(allt, 1.0)))

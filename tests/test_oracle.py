@@ -920,3 +920,55 @@ def test_pse_kernel_value():
     frac = np.sqrt(4 * 3.1415926 * float(tau))
     want = frac * np.exp(-((xi.astype(np.float64) - xj) ** 2).sum() / (4 * float(tau)))
     assert abs(O.pse_kernel(xi, xj, float(tau)) - want) < 1e-6 * want
+
+
+# ------------------------------------------------------------------------ tracers
+def test_tracer_advection_closed_forms():
+    """Constant velocity, no solid: every tracer moves by (c1 + c2 + c3) v as a FLIP particle does
+    (coarse c3) and its density becomes d / (1 + decay dt); with a subdomain whose velocity equals
+    the coarse one, the same bits."""
+    coarse, sub = scenes.nested_flip_case(coarse_vel=A_VEL, sub_vel=A_VEL, plane=(1.0, 0.0, 0.0, -5.0))
+    rng = np.random.default_rng(6)
+    tr = np.concatenate([rng.uniform(0.15, 0.85, (2000, 3)), rng.uniform(0.5, 1.5, (2000, 1))], 1).astype(np.float32)
+    dt, decay = np.float32(0.02), np.float32(0.3)
+    plain = O.advect_tracers(coarse, None, dt, decay, tr)
+    ref = O.advect_particles(tr[:, :3], coarse["h"], dt, coarse["u"], coarse["v"], coarse["w"], coarse["phi"])
+    assert np.array_equal(bits(plain[:, :3]), bits(ref))
+    want = tr[:, 3].astype(np.float64) / (1.0 + np.float64(np.float32(decay * dt)))  # float d / double
+    assert np.array_equal(plain[:, 3], want.astype(np.float32))
+    assert np.array_equal(bits(O.advect_tracers(coarse, sub, dt, decay, tr)), bits(plain))
+
+
+def test_tracer_pushout_at_zero():
+    """Tracers push out where phi <= 0 (FLIP particles where phi < 0): a tracer advected onto a
+    plane solid's surface or behind it ends on the plane."""
+    coarse, sub = scenes.nested_flip_case(coarse_vel=(-1.0, 0.0, 0.0), sub_vel=(-1.0, 0.0, 0.0),
+                                          plane=(1.0, 0.0, 0.0, 0.5))
+    x = np.random.default_rng(7).uniform(0.5, 0.52, (500, 3)).astype(np.float32)
+    tr = np.concatenate([x, np.ones((500, 1), np.float32)], 1)
+    out = O.advect_tracers(coarse, sub, np.float32(0.05), np.float32(0.0), tr)
+    assert np.abs(out[:, 0] - 0.5).max() < 2e-6
+
+
+def test_tracer_emitters():
+    """Sphere emitter: n tracers inside the sphere, density 1, filling it (mean near the centre);
+    subdomain emitter against a plane solid: every tracer within (0, h) of the plane, inside its
+    subdomain cell."""
+    e = O.emit_sphere(3, 5000, (0.5, 0.1, 0.5), 0.08, 1.0 / 64)
+    assert e.shape == (5000, 4) and np.all(e[:, 3] == 1)
+    r = np.linalg.norm(e[:, :3].astype(np.float64) - [0.5, 0.1, 0.5], axis=1)
+    assert r.max() <= 0.08 * (1 + 1e-6) and abs(np.mean(e[:, 0]) - 0.5) < 0.005
+    assert np.percentile(r, 50) > 0.08 * 0.75  # uniform in volume: median radius 0.79 r
+    coarse, sub = scenes.nested_flip_case(plane=(0.6, 0.8, 0.0, 0.62))
+    t = O.emit_sub_tracers(coarse, sub, 5)
+    h = float(sub["h"])
+    d = t[:, :3].astype(np.float64) @ [0.6, 0.8, 0.0] - 0.62
+    assert len(t) > 500 and np.all((d > -1e-6) & (d < h + 1e-6)) and np.all(t[:, 3] == 1)
+    cell = np.floor((t[:, :3] - np.asarray(sub["bmin"])) / h)
+    assert np.all((t[:, :3] - np.asarray(sub["bmin"])) / h - cell < 1 + 1e-5)
+
+
+def test_tracer_frame_record():
+    tr = np.array([[0.25, 0.5, 0.75, 0.3], [1.0, 0.0, 0.0, 1.0]], np.float32)
+    f = O.tracer_frame(tr, 2.0)
+    assert np.allclose(f, [[0.125 - 0.5, 0.25, 0.375 - 0.5, 0.3], [0.0, 0.0, -0.5, 1.0]], atol=0)

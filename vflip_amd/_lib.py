@@ -84,6 +84,12 @@ SIGNATURES = {
                                  P, P, PI]),
     "vflip_pse_diffuse": (c_int, [ctypes.POINTER(CoarseView), ctypes.POINTER(SubdomainView), c_float, c_float, c_int,
                                   P, P, P, P, P, P]),
+    "vflip_advect_tracers": (c_int, [ctypes.POINTER(CoarseView), ctypes.POINTER(SubdomainView), c_float, c_float,
+                                     c_int, P, P]),
+    "vflip_emit_tracers_sphere": (c_int, [P, ctypes.c_uint, c_int, c_float, c_float, c_float, c_float, c_float, P]),
+    "vflip_emit_tracers_sub": (c_int, [P, ctypes.POINTER(CoarseView), ctypes.POINTER(SubdomainView), ctypes.c_uint,
+                                       c_int, P, PI]),
+    "vflip_tracer_frame": (c_int, [c_int, P, c_float, P, P]),
     "vflip_batch_destroy": (c_int, [P]),
     "vflip_batch_set_graph_iterations": (c_int, [P, c_int]),
     "vflip_batch_solve": (c_int, [P, P, P, P, c_int, ctypes.POINTER(ctypes.c_double)]),

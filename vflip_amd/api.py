@@ -5,6 +5,7 @@ computation runs in libvflip.so's HIP kernels. Arrays are torch tensors shaped
 (nk, nj, ni) (z, y, x) so that the memory layout is the ABI's linear x-fastest.
 """
 import ctypes
+import os
 
 import torch
 
@@ -420,6 +421,55 @@ def seed_ghost(P, coarse, sub, seed):
     return _selection(lambda cap, a, b, c: lib().vflip_seed_ghost(P.hnd, ctypes.byref(C), ctypes.byref(S),
                                                                   int(seed) & 0xFFFFFFFF, cap, a, b, c),
                       "seed_ghost", 2 * S.ni * S.nj * S.nk)
+
+
+def advect_tracers(coarse, sub, dt, decay, tracers):
+    """advect_tracers in place on a (n, 4) float32 tensor; sub None = no subdomain."""
+    if sub is None:
+        C, _ = nested_views(coarse, dict(coarse, bmin=(0, 0, 0), bmax=(0, 0, 0)))
+        sref = None
+    else:
+        C, S = nested_views(coarse, sub)
+        sref = ctypes.byref(S)
+    n = tracers.shape[0]
+    check(lib().vflip_advect_tracers(ctypes.byref(C), sref, float(dt), float(decay), n,
+                                     _ptr(tracers) if n else None, _stream()), "advect_tracers")
+    return tracers
+
+
+def emit_tracers_sphere(P, seed, n_emit, centre, r, hx):
+    out = torch.empty((max(n_emit, 1), 4), device="cuda")
+    check(lib().vflip_emit_tracers_sphere(P.hnd, int(seed) & 0xFFFFFFFF, n_emit, *[float(c) for c in centre],
+                                          float(r), float(hx), _ptr(out)), "emit_tracers_sphere")
+    return out[:n_emit]
+
+
+def emit_tracers_sub(P, coarse, sub, seed):
+    """Subdomain tracers near the solid (P: the subdomain's workspace): (n, 4)."""
+    C, S = nested_views(coarse, sub)
+    cap = 4 * S.ni * S.nj * S.nk
+    out = torch.empty((max(cap, 1), 4), device="cuda")
+    n = ctypes.c_int(0)
+    check(lib().vflip_emit_tracers_sub(P.hnd, ctypes.byref(C), ctypes.byref(S), int(seed) & 0xFFFFFFFF, cap,
+                                       _ptr(out), ctypes.byref(n)), "emit_tracers_sub")
+    return out[:n.value]
+
+
+def tracer_frame(tracers, lx):
+    out = torch.empty_like(tracers)
+    n = tracers.shape[0]
+    check(lib().vflip_tracer_frame(n, _ptr(tracers) if n else None, float(lx), _ptr(out) if n else None, _stream()),
+          "tracer_frame")
+    return out
+
+
+def write_tracers(directory, frame, tracers, lx):
+    """write_tracers: directory/Particle_data%04d.bin, the frame records as raw float32."""
+    rec = tracer_frame(tracers, lx).cpu().numpy().astype("<f4")
+    path = os.path.join(directory, "Particle_data%04d.bin" % frame)
+    with open(path, "wb") as f:
+        f.write(rec.tobytes())
+    return path
 
 
 def liquid_phi(h, radius, pos, cell_start, order, shape, out=None):

@@ -1141,6 +1141,22 @@ static int ensure_capacity(vflip_particles* P, int np) {
 
 size_t scan_chunk() { return kScanChunk; }
 
+int ensure_selection_scratch(vflip_particles* P, size_t n) {
+    if (n + 1 <= P->pflag_cap) return 0;
+    if (P->pflag) VF_HIP(hipFree(P->pflag));
+    if (P->ppartial) VF_HIP(hipFree(P->ppartial));
+    P->pflag = P->ppartial = nullptr;
+    P->pflag_cap = 0;
+    const size_t cap = n + n / 4 + 1024;
+    if (hipMalloc(&P->pflag, (cap + 1) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&P->ppartial, (cap / kScanChunk + 1) * sizeof(int)) != hipSuccess) {
+        set_error("selection workspace of %zu entries: out of device memory", cap);
+        return VFLIP_ERR_NOMEM;
+    }
+    P->pflag_cap = cap;
+    return 0;
+}
+
 void scan_exclusive_i32(int* a, size_t n, int* partial, hipStream_t s) {
     const unsigned nb = (unsigned)((n + kScanChunk - 1) / kScanChunk);
     if (nb == 0) {

@@ -43,5 +43,7 @@ __device__ inline float reseed_jitter(unsigned seed, unsigned key) {
 // ceil(n / scan_chunk()) ints. Asynchronous on s.
 size_t scan_chunk();
 void scan_exclusive_i32(int* a, size_t n, int* partial, hipStream_t s);
+// Grows P's selection scratch (pflag, ppartial) to n + 1 entries.
+int ensure_selection_scratch(vflip_particles* P, size_t n);
 
 }  // namespace vflip
