@@ -350,6 +350,92 @@ int precond_and_dot(vflip_solver* S, int what) {
     return finalize(S, what);
 }
 
+// FNV-1a over the host state the iteration's launches read.
+struct KeyHash {
+    unsigned long long h = 1469598103934665603ull;
+    template <typename T> void add(const T& v) {
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+        for (size_t q = 0; q < sizeof(T); ++q) h = (h ^ p[q]) * 1099511628211ull;
+    }
+};
+
+unsigned long long iteration_key(const vflip_solver* S, int which) {
+    KeyHash k;
+    k.add(which);
+    k.add(S->stream);
+    k.add(S->opts.pre_sweeps); k.add(S->opts.post_sweeps); k.add(S->opts.coarse_sweeps); k.add(S->opts.precision);
+    k.add(S->use_zrow); k.add(S->zsweep_min_n); k.add(S->vblocks); k.add(S->cus); k.add(S->n);
+    k.add(S->r); k.add(S->z); k.add(S->zt); k.add(S->s); k.add(S->q); k.add(S->xw); k.add(S->x64);
+    k.add(S->partial); k.add(S->sc);
+    k.add(S->ctab); k.add(S->cpart); k.add(S->ctab_sweeps);
+    for (const LevelBuf& l : S->L) {
+        k.add(l.ni); k.add(l.nj); k.add(l.nk);
+        k.add(l.d); k.add(l.ax); k.add(l.ay); k.add(l.az); k.add(l.leak);
+        k.add(l.hd); k.add(l.hax); k.add(l.hay); k.add(l.haz); k.add(l.hscale); k.add(l.half);
+        k.add(l.x); k.add(l.xt); k.add(l.b);
+    }
+    return k.h;
+}
+
+// The launches of one PCG iteration before the residual check (which == 0:
+// q = A s with s.q, scalars, x and r updates with |r|, scalars) and after it
+// (which == 1: z = M^-1 r with z.r, scalars, s = z + beta s).
+int iteration_launches(vflip_solver* S, int which) {
+    int rc;
+    if (which == 0) {
+        launch_matvec_dot(S, S->s, S->q);
+        VF_CHECK_LAUNCH();
+        if ((rc = finalize(S, 1))) return rc;
+        if (mixed(S)) {
+            k_update_x64r<<<S->vblocks, kBlock, 0, S->stream>>>((unsigned)S->n, S->sc, S->x64, S->s, S->r, S->q,
+                                                                S->partial);
+            S->np_last = S->vblocks;
+        } else {
+            launch_update_xr(S);
+        }
+        VF_CHECK_LAUNCH();
+        return finalize(S, 2);
+    }
+    if ((rc = precond_and_dot(S, 3))) return rc;
+    launch_update_s(S);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+// Runs an iteration sequence: replays its graph (capturing it first when the
+// key changed), or launches it directly when graphs are off or profiling.
+int run_iteration(vflip_solver* S, int which) {
+    if (!S->use_graphs || S->prof) return iteration_launches(S, which);
+    const unsigned long long key = iteration_key(S, which);
+    if (!S->g_exec[which] || S->g_key[which] != key) {
+        if (S->g_exec[which]) {
+            VF_HIP(hipGraphExecDestroy(S->g_exec[which]));
+            S->g_exec[which] = nullptr;
+        }
+        if (!S->cap_stream) VF_HIP(hipStreamCreateWithFlags(&S->cap_stream, hipStreamNonBlocking));
+        hipStream_t user = S->stream;
+        S->stream = S->cap_stream;
+        VF_HIP(hipStreamBeginCapture(S->cap_stream, hipStreamCaptureModeThreadLocal));
+        int rc = iteration_launches(S, which);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(S->cap_stream, &g);
+        S->stream = user;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        VF_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&S->g_exec[which], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        VF_HIP(ei);
+        S->g_key[which] = key;
+        S->g_np_last[which] = S->np_last;
+    }
+    VF_HIP(hipGraphLaunch(S->g_exec[which], S->stream));
+    S->np_last = S->g_np_last[which];
+    return 0;
+}
+
 }  // namespace
 
 // ============================================================================
@@ -383,6 +469,9 @@ int vflip_solver_destroy(vflip_solver* S) {
     if (S->d_changed) (void)hipFree(S->d_changed);
     if (S->h_changed) (void)hipHostFree(S->h_changed);
     for (auto e : S->prof_ev) (void)hipEventDestroy(e);
+    for (auto g : S->g_exec)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (S->cap_stream) (void)hipStreamDestroy(S->cap_stream);
     if (S->partial) (void)hipFree(S->partial);
     if (S->sc) (void)hipFree(S->sc);
     if (S->h_sc) (void)hipHostFree(S->h_sc);
@@ -422,6 +511,7 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
     if (opts) S->opts = *opts;
     if (const char* e = getenv("VFLIP_ZSWEEP_MIN_N")) S->zsweep_min_n = atoi(e);
     if (const char* e = getenv("VFLIP_ZROW")) S->use_zrow = atoi(e) != 0;
+    if (const char* e = getenv("VFLIP_GRAPHS")) S->use_graphs = atoi(e) != 0;
     // grid-stride vector kernels: about 1024 cells per workgroup, at most kVecBlocks partials
     S->vblocks = (int)std::min<size_t>(kVecBlocks, std::max<size_t>(8, (S->n + 1023) / 1024));
     {
@@ -796,18 +886,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
             VF_CHECK_LAUNCH();
             status = VFLIP_NOT_CONVERGED;
             for (it = 0; it < S->opts.max_iterations; ++it) {
-                launch_matvec_dot(S, S->s, S->q);
-                VF_CHECK_LAUNCH();
-                if ((rc = finalize(S, 1))) return rc;
-                if (mx) {
-                    k_update_x64r<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->sc, S->x64, S->s, S->r, S->q,
-                                                                        S->partial);
-                    S->np_last = S->vblocks;
-                } else {
-                    launch_update_xr(S);
-                }
-                VF_CHECK_LAUNCH();
-                if ((rc = finalize(S, 2))) return rc;
+                if ((rc = run_iteration(S, 0))) return rc;
                 if ((rc = fetch_scalars(S))) return rc;
                 res = S->h_sc->res;
                 if (mx) {
@@ -826,9 +905,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
                     ++it;
                     break;
                 }
-                if ((rc = precond_and_dot(S, 3))) return rc;
-                launch_update_s(S);
-                VF_CHECK_LAUNCH();
+                if ((rc = run_iteration(S, 1))) return rc;
             }
             stats.converged = (status == VFLIP_OK);
         }

@@ -61,6 +61,14 @@ struct vflip_solver {
     bool prof = false;
     std::vector<hipEvent_t> prof_ev;
     int prof_used = 0;
+    // The PCG iteration's two launch sequences (matvec + update, preconditioner
+    // + direction update) captured as graphs, keyed by a hash of every argument
+    // they bake in; captured on a private stream, launched on `stream`.
+    bool use_graphs = true;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t g_exec[2] = {nullptr, nullptr};
+    unsigned long long g_key[2] = {0, 0};
+    int g_np_last[2] = {0, 0};  // np_last after each sequence
 };
 
 // Fused z-marching sweep geometry for a level. Rows of 128 or 256 cells use the
