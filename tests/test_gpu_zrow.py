@@ -62,7 +62,7 @@ def _problem(O, ni, nj, nk, seed):
 
 def _solver(api, shape, op, row_wave, **opts):
     ni, nj, nk = shape
-    with _env(VFLIP_ZROW=int(row_wave), VFLIP_ZSWEEP_MIN_N=4096):
+    with _env(VFLIP_ZROW=int(row_wave), VFLIP_ZSWEEP_MIN_N=4096, VFLIP_FUSED_MIN_N=4096):
         S = api.Solver(ni, nj, nk, **opts)
     S.set_matrix(*[_t(a.astype(np.float32)) for a in op[:4]])
     return S

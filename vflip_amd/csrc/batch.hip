@@ -67,7 +67,7 @@ template <int TY, int MODE>
 __global__ __launch_bounds__(256) void kb_zsweep(const BItem* __restrict__ it, int n, const BState* __restrict__ st) {
     const BItem I = it[bfind(it, n)];
     if (st[I.slot].done) return;
-    rbgs_zsweep_body<TY, MODE, float>(I.op, I.a0, I.a1, I.o0, I.a2, I.i0, I.i1, I.i2, I.i3, I.i4, I.partial, bblk(I));
+    rbgs_zsweep_body<TY, MODE, float, float>(I.op, I.a0, I.a1, I.o0, I.a2, I.i0, I.i1, I.i2, I.i3, I.i4, I.partial, bblk(I));
 }
 // flat half sweeps: a0 b, o0 x (in place), a2 xc, i0 ci, i1 cj
 __global__ __launch_bounds__(kBlock) void kb_first(const BItem* __restrict__ it, int n, const BState* __restrict__ st) {

@@ -167,8 +167,8 @@ struct BlackIn {
     int c2; // partner (odd) cell index, -1 when absent
 };
 
-template <int TY, int MODE, typename Acc>
-__device__ inline void rbgs_zsweep_body(const Op<float>& op, const float* __restrict__ b,
+template <int TY, int MODE, typename Acc, typename CT>
+__device__ inline void rbgs_zsweep_body(const Op<CT>& op, const float* __restrict__ b,
                                         const float* __restrict__ xin, float* __restrict__ xout,
                                         const float* __restrict__ xc, int ci, int cj, int tiles_x, int tiles_y,
                                         int zchunk, double* __restrict__ dot, Blk B) {
@@ -341,12 +341,12 @@ __device__ inline void rbgs_zsweep_body(const Op<float>& op, const float* __rest
         if (tid == 0) dot[L] = (double)t;
     }
 }
-template <int TY, int MODE, typename Acc>
-__global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<float> op, const float* __restrict__ b,
+template <int TY, int MODE, typename Acc, typename CT>
+__global__ __launch_bounds__(256) void k_rbgs_zsweep(Op<CT> op, const float* __restrict__ b,
                                                      const float* __restrict__ xin, float* __restrict__ xout,
                                                      const float* __restrict__ xc, int ci, int cj, int tiles_x,
                                                      int tiles_y, int zchunk, double* __restrict__ dot) {
-    rbgs_zsweep_body<TY, MODE, Acc>(op, b, xin, xout, xc, ci, cj, tiles_x, tiles_y, zchunk, dot, this_blk());
+    rbgs_zsweep_body<TY, MODE, Acc, CT>(op, b, xin, xout, xc, ci, cj, tiles_x, tiles_y, zchunk, dot, this_blk());
 }
 
 // ----------------------------------------------------------------------------

@@ -90,34 +90,41 @@ void launch_stencil_row(vflip_solver* S, const RowGeom& g, const LevelBuf& lv, c
                                                                        partial);
 }
 
-template <int MODE, typename Acc>
-void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<float>& op, const float* b, const float* xin,
+template <int MODE, typename Acc, typename CT>
+void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<CT>& op, const float* b, const float* xin,
                         float* xout, const float* xc, int ci, int cj, double* dot) {
-    if (g.v == 4)
-        k_rbgs_zrow<4, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
-                                                                     g.zchunk, dot);
-    else if (g.v == 2)
-        k_rbgs_zrow<2, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
-                                                                     g.zchunk, dot);
-    else if (g.ty == 16)
-        k_rbgs_zsweep<16, MODE, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
-                                                                       g.tiles_y, g.zchunk, dot);
+    if constexpr (std::is_same<CT, float>::value) {
+        if (g.v == 4) {
+            k_rbgs_zrow<4, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
+                                                                         g.zchunk, dot);
+            return;
+        }
+        if (g.v == 2) {
+            k_rbgs_zrow<2, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
+                                                                         g.zchunk, dot);
+            return;
+        }
+    }
+    if (g.ty == 16)
+        k_rbgs_zsweep<16, MODE, Acc, CT><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
+                                                                           g.tiles_y, g.zchunk, dot);
     else if (g.ty == 8)
-        k_rbgs_zsweep<8, MODE, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
-                                                                      g.tiles_y, g.zchunk, dot);
+        k_rbgs_zsweep<8, MODE, Acc, CT><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
+                                                                          g.tiles_y, g.zchunk, dot);
     else
-        k_rbgs_zsweep<4, MODE, Acc><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
-                                                                      g.tiles_y, g.zchunk, dot);
+        k_rbgs_zsweep<4, MODE, Acc, CT><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
+                                                                          g.tiles_y, g.zchunk, dot);
 }
 
-template <typename Acc>
+// FP16-stored levels always take the tile form (the row-wave kernel reads FP32 coefficients).
+template <typename Acc, typename CT>
 void launch_zsweep(vflip_solver* S, const LevelBuf& lv, const float* b, const float* xin, float* xout, int mode,
                    const float* xc, int ci, int cj, double* dot) {
-    const ZGeom g = zgeom(S, lv);
-    const Op<float> op = make_op<float>(lv);
-    if (mode == 1) launch_zsweep_mode<1, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
-    else if (mode == 2) launch_zsweep_mode<2, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
-    else launch_zsweep_mode<0, Acc>(S, g, op, b, xin, xout, xc, ci, cj, dot);
+    const ZGeom g = zgeom(lv, std::is_same<CT, float>::value && S->use_zrow, S->cus);
+    const Op<CT> op = make_op<CT>(lv);
+    if (mode == 1) launch_zsweep_mode<1, Acc, CT>(S, g, op, b, xin, xout, xc, ci, cj, dot);
+    else if (mode == 2) launch_zsweep_mode<2, Acc, CT>(S, g, op, b, xin, xout, xc, ci, cj, dot);
+    else launch_zsweep_mode<0, Acc, CT>(S, g, op, b, xin, xout, xc, ci, cj, dot);
 }
 
 
@@ -140,7 +147,9 @@ template <typename CT> int smooth_level(vflip_solver* S, int l, bool pre, const 
         VF_CHECK_LAUNCH();
         return 0;
     }
-    if (use_zsweep(S, lv)) {
+    // level 0 keeps the batched solver's rule (its last sweep fuses z.r, so the
+    // summation order, and a batched solve's bits, stay those of the single solve)
+    if (use_zsweep(S, lv, l == 0 ? S->zsweep_min_n : S->fused_min_n)) {
         for (int it = 0; it < sweeps; ++it) {
             const int mode = (it == 0) ? (pre ? 1 : 2) : 0;
             const bool last = (it == sweeps - 1);
@@ -155,9 +164,9 @@ template <typename CT> int smooth_level(vflip_solver* S, int l, bool pre, const 
                 VF_HIP(hipEventRecord(S->prof_ev[S->prof_used++], S->stream));
             }
             if (mixed(S))
-                launch_zsweep<double>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
+                launch_zsweep<double, CT>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
             else
-                launch_zsweep<float>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
+                launch_zsweep<float, CT>(S, lv, b, *xp, *xtp, mode, xc, c ? c->ni : 0, c ? c->nj : 0, dot);
             VF_CHECK_LAUNCH();
             if (prof) VF_HIP(hipEventRecord(S->prof_ev[S->prof_used++], S->stream));
             std::swap(*xp, *xtp);
@@ -364,7 +373,7 @@ unsigned long long iteration_key(const vflip_solver* S, int which) {
     k.add(which);
     k.add(S->stream);
     k.add(S->opts.pre_sweeps); k.add(S->opts.post_sweeps); k.add(S->opts.coarse_sweeps); k.add(S->opts.precision);
-    k.add(S->use_zrow); k.add(S->zsweep_min_n); k.add(S->vblocks); k.add(S->cus); k.add(S->n);
+    k.add(S->use_zrow); k.add(S->zsweep_min_n); k.add(S->fused_min_n); k.add(S->vblocks); k.add(S->cus); k.add(S->n);
     k.add(S->r); k.add(S->z); k.add(S->zt); k.add(S->s); k.add(S->q); k.add(S->xw); k.add(S->x64);
     k.add(S->partial); k.add(S->sc);
     k.add(S->ctab); k.add(S->cpart); k.add(S->ctab_sweeps);
@@ -510,6 +519,7 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
     vflip_default_opts(&S->opts);
     if (opts) S->opts = *opts;
     if (const char* e = getenv("VFLIP_ZSWEEP_MIN_N")) S->zsweep_min_n = atoi(e);
+    if (const char* e = getenv("VFLIP_FUSED_MIN_N")) S->fused_min_n = atoi(e);
     if (const char* e = getenv("VFLIP_ZROW")) S->use_zrow = atoi(e) != 0;
     if (const char* e = getenv("VFLIP_GRAPHS")) S->use_graphs = atoi(e) != 0;
     // grid-stride vector kernels: about 1024 cells per workgroup, at most kVecBlocks partials

@@ -44,7 +44,10 @@ struct vflip_solver {
     Scalars* h_sc = nullptr;  // pinned
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool setup_done = false;
-    int zsweep_min_n = 32768;  // levels at least this large use the fused z-marching sweep
+    int zsweep_min_n = 32768;  // batched subdomain solves: levels this large use the fused sweep
+    // single-grid solves, levels below 0: levels this large use the fused sweep,
+    // smaller ones the per-colour sweeps (measured faster at 64^3 and below)
+    int fused_min_n = 128 * 128 * 128;
     bool use_zrow = true;      // row-wave form of the fused sweep where the row length allows it
     int cus = 256;             // compute units of the device (launch sizing)
     int vblocks = kVecBlocks;  // workgroups (partials) of the grid-stride vector kernels
@@ -64,7 +67,7 @@ struct vflip_solver {
     // The PCG iteration's two launch sequences (matvec + update, preconditioner
     // + direction update) captured as graphs, keyed by a hash of every argument
     // they bake in; captured on a private stream, launched on `stream`.
-    bool use_graphs = true;
+    bool use_graphs = false;  // measured slightly slower than direct launches on ROCm 7; VFLIP_GRAPHS=1
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t g_exec[2] = {nullptr, nullptr};
     unsigned long long g_key[2] = {0, 0};
@@ -111,9 +114,10 @@ inline ZGeom zgeom(const LevelBuf& lv, bool use_zrow, int cus) {
 inline ZGeom zgeom(const vflip_solver* S, const LevelBuf& lv) { return zgeom(lv, S->use_zrow, S->cus); }
 
 // Levels at least this large (and not FP16) use the fused z-marching sweep.
-inline bool use_zsweep(const vflip_solver* S, const LevelBuf& lv) {
-    return !lv.half && (int)lv.n >= S->zsweep_min_n && lv.ni >= 16 && lv.nj >= 4 && lv.nk >= 4;
+inline bool use_zsweep(const vflip_solver* S, const LevelBuf& lv, int min_n) {
+    return (int)lv.n >= min_n && lv.ni >= 16 && lv.nj >= 4 && lv.nk >= 4;
 }
+inline bool use_zsweep(const vflip_solver* S, const LevelBuf& lv) { return use_zsweep(S, lv, S->zsweep_min_n); }
 
 inline Op<float> make_op_f32(const LevelBuf& l) { return Op<float>{l.d, l.ax, l.ay, l.az, 1.f, l.ni, l.nj, l.nk}; }
 
