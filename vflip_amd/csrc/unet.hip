@@ -179,9 +179,15 @@ int ensure_grid(vflip_unet* N, int nx, int ny, int nz) {
 
 ConvSrc make_src(const vflip_unet* N, const In& in) { return ConvSrc{N->act[in.t], in.C, in.up}; }
 
+// Output channels are split into groups until the layer has at least 128
+// workgroups (or one 16-channel block per workgroup).
 template <int CK>
-void launch_conv(const ConvArgs& a, int NB, hipStream_t s) {
-    const dim3 grid(a.ntiles), block(kConvThreads);
+void launch_conv(ConvArgs a, int NBall, hipStream_t s) {
+    int g = 1;
+    while (a.ntiles * g < 128 && NBall % (2 * g) == 0) g *= 2;
+    a.groups = g;
+    const int NB = NBall / g;
+    const dim3 grid(a.ntiles * g), block(kConvThreads);
     switch (NB) {
         case 1: k_conv3<CK, 1><<<grid, block, 0, s>>>(a); break;
         case 2: k_conv3<CK, 2><<<grid, block, 0, s>>>(a); break;

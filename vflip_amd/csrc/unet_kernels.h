@@ -45,6 +45,7 @@ struct ConvArgs {
     float* out32;    // [V][cout] fp32 (if not null), or [V] channel 0 only when out32_ch0
     int out32_ch0;
     int tiles_x, tiles_y, ntiles;
+    int groups;  // output-channel groups: workgroup L computes channels of group L / ntiles
 };
 
 __device__ inline int tap_offset_hv(int tap) {
@@ -83,11 +84,14 @@ __device__ inline void stage_halo(const ConvArgs& a, int c0, int x0, int y0, int
 
 // 3x3x3 convolution, implicit GEMM: M = 512 voxels of an 8^3 tile, N = cout,
 // K = 27 * cin. Each K step of 32 is one tap x 32 channels (CK = 32) or two
-// taps x 16 channels (CK = 16). NB = cout / 16.
+// taps x 16 channels (CK = 16). A workgroup computes NB blocks of 16 output
+// channels, group L / ntiles of the a.groups groups (cout = 16 NB groups); small
+// grids split their channels over groups so that more workgroups share a layer.
 template <int CK, int NB>
 __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
     __shared__ __attribute__((aligned(16))) bf16 lds[kHV * (CK + 8)];
-    const int L = blockIdx.x;
+    const int L = blockIdx.x % a.ntiles, grp = blockIdx.x / a.ntiles;
+    const int cbase = grp * NB * 16;
     int t = L;
     if (a.ntiles % 8 == 0) t = (L % 8) * (a.ntiles / 8) + L / 8;
     const int x0 = (t % a.tiles_x) * kT, y0 = ((t / a.tiles_x) % a.tiles_y) * kT, z0 = (t / (a.tiles_x * a.tiles_y)) * kT;
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
             bf16x8 bfrag[NB];
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
-                bfrag[nb] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(nb * 16 + r16) * a.kw + kidx);
+                bfrag[nb] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(cbase + nb * 16 + r16) * a.kw + kidx);
 #pragma unroll
             for (int mb = 0; mb < 4; ++mb) {
                 const bf16x8 afrag = *reinterpret_cast<const bf16x8*>(lds + (hv0[mb] + dhv) * (CK + 8) + coff);
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
             const size_t v = ((size_t)z * a.ny + y) * a.nx + x;
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
-                const int co = nb * 16 + r16;
+                const int co = cbase + nb * 16 + r16;
                 float val = acc[mb][nb][r];
                 if (a.bias) val += a.bias[co];
                 if (a.relu) val = val > 0.f ? val : 0.f;
