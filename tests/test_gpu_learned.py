@@ -7,7 +7,7 @@
   grow with depth: the layers next to the loss agree to 2e-2, all parameters
   together to 5e-2 and every layer to 1.5e-1 (the MSE-loss test of
   test_gpu_unet.py holds every layer's backward to 5e-2).
-* MGPCG judges every guess: a guess that is no better than zero in the A-norm
+* MGPCG vets every guess: a guess that is no better than zero in the A-norm
   (E(x0) >= 0) is dropped and the solve restarts from zero, converging as a
   zero-guess solve does.
 * What the committed network (vflip_amd/data/unet_plume64.npz, trained by
@@ -87,7 +87,7 @@ def test_energy_loss_gradients_match_reference(cuda):
 
 
 def test_bad_guess_is_rejected_and_solve_converges(cuda):
-    print("warm-start judge", flush=True)
+    print("warm-start vetting", flush=True)
     import torch
     from vflip_amd import api, scenes
     n = 32

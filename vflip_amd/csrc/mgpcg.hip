@@ -849,7 +849,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
         launch_true_residual64(S, mx ? rhs64 : nullptr, rhs32, S->x64, S->r);
         VF_CHECK_LAUNCH();
         if ((rc = finalize(S, 2))) return rc;
-        // The PCG judges the guess: it minimises E(x) = x'Ax/2 - b'x, and
+        // The PCG vets the guess: it minimises E(x) = x'Ax/2 - b'x, and
         // E(x0) < E(0) = 0 exactly when x0 is closer to the solution than zero
         // in the A-norm. A guess that is not is dropped (restart from zero).
         k_energy_dots<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->x64, mx ? rhs64 : nullptr, rhs32, S->r, S->partial);

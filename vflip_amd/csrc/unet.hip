@@ -3,7 +3,7 @@
 // accumulation) that maps (divergence, solid mask, liquid mask) to pressure.
 // Forward, backward and Adam run on the GPU through the kernels of
 // unet_kernels.h (matrix-core convolutions for the forward, backward-data and
-// weight-gradient passes). The solver stays the judge: the prediction only
+// weight-gradient passes). The solver keeps the last word: the prediction only
 // seeds MGPCG, which runs to the same tolerance.
 //
 // Network (level l has grid n / 2^l):
