@@ -99,8 +99,31 @@ def traffic_from_profiles(kernel):
         return None
 
 
-def cpu_baseline(sim_state, phi, n, dt=0.01):
-    """One plume step of the same state on the FP64 CPU oracle (all host cores)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sim_state, phi, n, dt=0.01, reps=5):
+    """The same plume step on the FP64 CPU oracle (all host cores): one warm-up
+    step, then the median of `reps` timed steps from the same state."""
+    import numpy as np
+    runs = [_cpu_step(sim_state, phi, n, dt) for _ in range(1 + reps)]
+    timed = runs[1:]
+    out = dict(runs[0])
+    out["ms_per_step"] = float(np.median([r["ms_per_step"] for r in timed]))
+    out["ms_solve"] = float(np.median([r["ms_solve"] for r in timed]))
+    out["reps"] = reps
+    return out
+
+
+def _cpu_step(sim_state, phi, n, dt):
     import numpy as np
     from oracle import oracle as O
     h = np.float32(1.0 / n)
@@ -338,14 +361,15 @@ def main():
     oracle_err = None
     if not args.no_cpu:
         # one step from the plain run's final state on the CPU oracle, and the same step on the GPU
-        log("cpu baseline: one FP64 oracle step ...")
+        log("cpu baseline: FP64 oracle steps (1 warm-up + 5 timed) ...")
         base = cpu_baseline(state0, phi, n)
         from oracle import oracle as O
         cores = O.lib().or_num_threads()
         cpu = {"value": round(base["ms_per_step"], 1), "unit": "ms/step", "cores": cores, "kind": "port",
                "sample": f"one plume step at {n}^3 (advection of u,v,w,rho,T + forcing + assembly + FP64 AMGPCG "
-                         f"to 1e-6, {base['iterations']} its + gradient), FP64 oracle with OpenMP",
-               "ms_solve": round(base["ms_solve"], 1)}
+                         f"to 1e-6, {base['iterations']} its + gradient), FP64 oracle with OpenMP; median of "
+                         f"{base['reps']} after 1 warm-up",
+               "ms_solve": round(base["ms_solve"], 1), "cpu_model": cpu_model(), "nproc": os.cpu_count()}
         from vflip_amd.plume import PlumeSim
         g = PlumeSim(n, phi, precision=api.FP32)
         g.load_state(state0)
