@@ -186,10 +186,16 @@ def row_baselines():
     rows = []
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
-    def add(row, workload, gpu_ms, cpu_ms, cpu_cores, parity, **extra):
-        rows.append({"row": row, "workload": workload, "gpu_ms": round(gpu_ms, 4), "cpu_ms": round(cpu_ms, 1),
-                     "cpu_cores": cpu_cores, "cpu_kind": "port", "gpu_over_cpu": round(cpu_ms / gpu_ms, 1),
-                     "parity": parity, **extra})
+    def add(row, workload, gpu_ms, cpu_ms, cpu_cores, parity, alg_bytes=None, bytes_note=None, **extra):
+        r = {"row": row, "workload": workload, "gpu_ms": round(gpu_ms, 4), "cpu_ms": round(cpu_ms, 1),
+             "cpu_cores": cpu_cores, "cpu_kind": "port", "gpu_over_cpu": round(cpu_ms / gpu_ms, 1), "parity": parity}
+        if alg_bytes:
+            gbs = alg_bytes / (gpu_ms * 1e-3) / 1e9
+            r["roofline"] = {"bound": "hbm", "algorithmic_bytes": int(alg_bytes), "bytes_per_unit": bytes_note,
+                             "achieved_GBps": round(gbs, 1), "peak_GBps": HBM_PEAK_GBS,
+                             "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        r.update(extra)
+        rows.append(r)
         log(f"row {row}: gpu {gpu_ms:.3f} ms, cpu {cpu_ms:.1f} ms ({cpu_cores} cores)")
 
     n = 128
@@ -202,8 +208,9 @@ def row_baselines():
     gm0 = _gpu_ms(lambda: api.advect_rk2(float(h), 0.01, g["u"], g["v"], g["w"], g["u"], (0.5, 0, 0), out, mode=0))
     gm2 = _gpu_ms(lambda: api.advect_rk2(float(h), 0.01, g["u"], g["v"], g["w"], g["u"], (0.5, 0, 0), out, mode=2))
     cm, _ = _cpu_ms(lambda: O.advect_rk2(h, np.float32(0.01), pb["u"], pb["v"], pb["w"], pb["u"], (0.5, 0, 0)))
+    ncell = n ** 3
     add("a14 advect_field (RK2, trilinear)", f"u component, {n}^3", gm0, cm, cores, "bit-exact (FP64 mode 0)",
-        gpu_ms_fp32_mode2=round(gm2, 4))
+        16 * ncell, "16 B/cell: u, v, w read, one component written", gpu_ms_fp32_mode2=round(gm2, 4))
     # a7 assembly, a13 gradient
     S = api.Solver(n, n, n, precision=api.FP32)
     rhs = torch.empty((n, n, n), device="cuda")
@@ -211,7 +218,8 @@ def row_baselines():
                                     g["ws"], rhs))
     uwn, vwn, wwn = O.face_weights(pb["phi"])
     ca, asm = _cpu_ms(lambda: O.assemble(h, pb["u"], pb["v"], pb["w"], uwn, vwn, wwn, pb["us"], pb["vs"], pb["ws"]))
-    add("a7 pcg_projection assembly", f"{n}^3 box + sphere", ga, ca, cores, "within one FP32 rounding of FP64")
+    add("a7 pcg_projection assembly", f"{n}^3 box + sphere", ga, ca, cores, "within one FP32 rounding of FP64",
+        72 * ncell, "72 B/cell: velocity, weights, solid velocity read; d, ax, ay, az, rhs and FP64 diag/rhs written")
     pres = np.random.default_rng(0).standard_normal((n, n, n))
     gp = t(pres.astype(np.float32))
     uu, vv, wv = g["u"].clone(), g["v"].clone(), g["w"].clone()
@@ -219,7 +227,8 @@ def row_baselines():
                                         wv))
     cg, _ = _cpu_ms(lambda: O.grad_apply(h, pres, uwn, vwn, wwn, pb["us"], pb["vs"], pb["ws"], pb["u"], pb["v"],
                                          pb["w"]))
-    add("a13 pressure gradient", f"{n}^3", gg, cg, cores, "bit-exact (FP64 pressure)")
+    add("a13 pressure gradient", f"{n}^3", gg, cg, cores, "bit-exact (FP64 pressure)",
+        52 * ncell, "52 B/cell: p, weights, solid velocity read; u, v, w read and written")
     del S
     # a17 binning and a18 rebuild: the FLIP smoke set, 8 per cell over 1.6 cells, shuffled
     pos = scenes.block_particles(n, 8, 21, hi=(1.0, 1.0, 1.0), jitter=1.6)
@@ -229,7 +238,8 @@ def row_baselines():
     tp, tv = t(pos), t(vel)
     gh = _gpu_ms(lambda: P.hash(tp, count=False))
     ch, (rk, rs, ro) = _cpu_ms(lambda: O.bin_particles(pos, h, (n, n, n)))
-    add("a17 hash_FLIP", f"{len(pos) / 1e6:.1f}M particles, {n}^3, random order", gh, ch, 1, "bit-exact")
+    add("a17 hash_FLIP", f"{len(pos) / 1e6:.1f}M particles, {n}^3, random order", gh, ch, 1, "bit-exact",
+        72 * len(pos), "72 B/particle (counting sort: position, key, permutation, index)")
     grids = scenes.random_mac(n, 5, scale=1.0)
     phi = scenes.solid_phi(n, 1.0 / n, [((0.5, 0.5, 0.5), 0.15)], open_top=False)
     tg = [t(a) for a in grids]
@@ -240,7 +250,8 @@ def row_baselines():
     gr = _gpu_ms(lambda: P.reorder(tp, tv, start, order, *tg, tphi, 1, out=bufs))
     cr, _ = _cpu_ms(lambda: O.reorder(pos, vel, h, (n, n, n), rs, ro, *grids, phi, 1))
     add("a18 reorder_FLIP", f"{len(pos) / 1e6:.1f}M particles, {n}^3, random order, box + sphere", gr, cr, 1,
-        "bit-exact")
+        "bit-exact", 28 * len(pos) + 24 * len(pos) + 4 * ncell,
+        "28 B per binned particle read, 24 B per output particle written, 4 B per cell")
     del tp, tv, bufs
     # a19 P2G, a20 G2P, a15 particle advection: the dam-break fill
     pos = scenes.block_particles(n, 8, 0)
@@ -253,22 +264,26 @@ def row_baselines():
     specs = ((0, (n, n, n + 1), (0.5, 0, 0)), (1, (n, n + 1, n), (0, 0.5, 0)), (2, (n + 1, n, n), (0, 0, 0.5)))
     cp, _ = _cpu_ms(lambda: [O.p2g(pos, vel, c, h, (n, n, n), rs, ro, fs, of) for c, fs, of in specs])
     add("a19 particle_to_grid", f"{len(pos) / 1e6:.2f}M particles (dam-break fill), {n}^3, 3 components", gp0, cp,
-        cores, "FP32 rounding (owner tiles); exact-order method bit-exact", gpu_ms_exact_order=round(gp1, 3))
+        cores, "FP32 rounding (owner tiles); exact-order method bit-exact", 24 * len(pos) + 24 * ncell,
+        "24 B/particle read + 24 B/cell written (3 fields, 3 weights)", gpu_ms_exact_order=round(gp1, 3))
     du = [a * np.float32(0.1) for a in grids]
     tdu = [t(a) for a in du]
     vv = tv.clone()
     gq = _gpu_ms(lambda: api.g2p(float(h), tp, vv, tdu))
     cq, _ = _cpu_ms(lambda: O.g2p(pos, vel, h, (n, n, n), *du))
-    add("a20 particle_from_grid", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gq, cq, cores, "bit-exact")
+    add("a20 particle_from_grid", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gq, cq, cores, "bit-exact",
+        36 * len(pos) + 12 * ncell, "24 B read + 12 B written per particle, 12 B/cell (FP64 samplers: compute-bound)")
     pp = tp.clone()
     gv = _gpu_ms(lambda: api.advect_particles(float(h), 0.01, pp, *tg, tphi))
     cv, _ = _cpu_ms(lambda: O.advect_particles(pos, h, np.float32(0.01), *grids, phi))
-    add("a15 advect_FLIP (RK3 + push-out)", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gv, cv, cores, "bit-exact")
+    add("a15 advect_FLIP (RK3 + push-out)", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gv, cv, cores, "bit-exact",
+        24 * len(pos) + 16 * ncell, "24 B/particle, 16 B/cell (9 FP64 samples per particle: compute-bound)")
     # C3 particle level set
     lphi = torch.empty((n, n, n), device="cuda")
     gl = _gpu_ms(lambda: api.liquid_phi(float(h), 0.9 * float(h), tp, start, order, (n, n, n), out=lphi))
     cl, _ = _cpu_ms(lambda: O.liquid_phi(pos, h, (n, n, n), np.float32(0.9) * h))
-    add("C3 particle level set", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gl, cl, 1, "bit-exact")
+    add("C3 particle level set", f"{len(pos) / 1e6:.2f}M particles, {n}^3", gl, cl, 1, "bit-exact",
+        16 * len(pos) + 12 * ncell, "16 B/particle (position, order), 12 B/cell (starts, phi)")
     del P, tp, tv
     # nested coupling: ghosts and PSE viscosity
     coarse, sub = scenes.nested_flip_case(n=n, box=(0.3, 0.7), seed=1, solid_vel=(0.2, -0.1, 0.05))

@@ -33,6 +33,30 @@ def test_liquid_phi_bitexact(cuda):
     assert np.array_equal(got.cpu().numpy().view(np.uint32), ref.view(np.uint32))
 
 
+def test_liquid_phi_dense_and_ragged_tiles(cuda):
+    """Tiles whose halo holds more particles than the LDS stage
+    take the global scan (a cluster of ~300 per cell over 27 cells is more than the 6144 staged),
+    the rest the LDS scan; both equal the oracle bit for bit, on a grid whose sides are not
+    multiples of the tile."""
+    print("liquid phi dense", flush=True)
+    from oracle import oracle as O
+    from vflip_amd import api
+    n = (21, 19, 13)
+    h = np.float32(1.0 / 21)
+    rng = np.random.default_rng(4)
+    dense = (np.array([0.3, 0.4, 0.3]) + rng.random((300 * 27, 3)) * 3 * float(h)).astype(np.float32)
+    sparse = (rng.random((3000, 3)) * [1.0, 0.9, 0.6]).astype(np.float32)
+    pos = np.ascontiguousarray(np.concatenate([dense, sparse]))
+    P = api.Particles(*n, float(h))
+    gp = _t(pos)
+    _, start, order, _ = P.hash(gp)
+    counts = np.diff(start.cpu().numpy())
+    assert counts.max() >= 200
+    got = api.liquid_phi(float(h), float(0.9 * h), gp, start, order, (n[2], n[1], n[0]))
+    ref = O.liquid_phi(pos, h, (n[2], n[1], n[0]), np.float32(0.9 * h))
+    assert np.array_equal(got.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
 def test_liquid_assembly(cuda):
     print("liquid assembly", flush=True)
     import torch

@@ -1178,16 +1178,24 @@ using namespace vflip;
 // cell gathers from vflip_hash's cell lists; the minimum does not depend on the
 // visiting order, so the result is exact (float distances, no contraction).
 namespace vflip {
-__global__ __launch_bounds__(kBlock) void k_liquid_phi(int ni, int nj, int nk, float h, float radius,
-                                                       const float* __restrict__ pos,
-                                                       const int* __restrict__ cell_start,
-                                                       const int* __restrict__ order, float* __restrict__ phi) {
-    const int n = ni * nj * nk;
-    const int c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const int i = c % ni, j = (c / ni) % nj, k = c / (ni * nj);
-    const float cx = (float)i * h, cy = (float)j * h, cz = (float)k * h;
-    float best = 3.0f * h;
+// Tile of kLpX x kLpY x kLpZ cells per workgroup: the particles binned in the
+// tile's halo (one cell around it) are staged in LDS once, then every cell
+// scans its 3x3x3 neighbour cells from LDS. A minimum is independent of the
+// order it is taken in, so the result equals the per-cell global scan bit for
+// bit; tiles whose halo holds more than kLpCap particles scan global memory.
+// The scans keep the least squared distance and take one square root at the
+// end: a correctly rounded sqrtf is monotone, so sqrtf(min d^2) equals the
+// minimum of the per-particle sqrtf(d^2).
+constexpr int kLpX = 8, kLpY = 8, kLpZ = 4;
+constexpr int kLpHX = kLpX + 2, kLpHY = kLpY + 2, kLpHZ = kLpZ + 2;
+constexpr int kLpHalo = kLpHX * kLpHY * kLpHZ;  // 600
+constexpr int kLpCap = 5400;  // 8 particles per cell over the whole halo is 4800
+constexpr int kLpRead = 4;    // particles per step of the LDS scan
+constexpr float kLpFar = 3.0e38f;  // pad-word coordinate: its squared distance is +inf
+
+__device__ inline float cell_min2_global(int i, int j, int k, int ni, int nj, int nk, float cx, float cy, float cz,
+                                         const float* __restrict__ pos, const int* __restrict__ cell_start,
+                                         const int* __restrict__ order, float best2) {
     for (int ck = max(k - 1, 0); ck <= min(k + 1, nk - 1); ++ck)
         for (int cj = max(j - 1, 0); cj <= min(j + 1, nj - 1); ++cj)
             for (int ci = max(i - 1, 0); ci <= min(i + 1, ni - 1); ++ci) {
@@ -1195,11 +1203,134 @@ __global__ __launch_bounds__(kBlock) void k_liquid_phi(int ni, int nj, int nk, f
                 for (int s = cell_start[q]; s < cell_start[q + 1]; ++s) {
                     const int p = order[s];
                     const float dx = cx - pos[3 * p], dy = cy - pos[3 * p + 1], dz = cz - pos[3 * p + 2];
-                    const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-                    best = d < best ? d : best;
+                    best2 = fminf(best2, dx * dx + dy * dy + dz * dz);
                 }
             }
-    phi[c] = best - radius;
+    return best2;
+}
+
+__global__ __launch_bounds__(kBlock) void k_liquid_phi(int ni, int nj, int nk, float h, float radius,
+                                                       const float* __restrict__ pos,
+                                                       const int* __restrict__ cell_start,
+                                                       const int* __restrict__ order, float* __restrict__ phi,
+                                                       int tiles_x, int tiles_y) {
+    __shared__ int s_off[kLpHalo + 1];
+    __shared__ int s_src[kLpHalo];  // cell_start of the halo cell minus its staged offset
+    // halo cell e's particles sit at s_off[e] + e on: one pad word per cell, so
+    // the segments of x-adjacent cells (adjacent lanes) start an odd stride apart
+    // (about 9 words for 8 particles a cell) and hit different LDS banks
+    // (plus kLpRead - 1 words that the unrolled scan may read past a run's end and masks)
+    __shared__ float s_x[kLpCap + kLpHalo + kLpRead], s_y[kLpCap + kLpHalo + kLpRead],
+        s_z[kLpCap + kLpHalo + kLpRead];
+    const int t = threadIdx.x;
+    const int tx = blockIdx.x % tiles_x, ty = (blockIdx.x / tiles_x) % tiles_y, tz = blockIdx.x / (tiles_x * tiles_y);
+    const int x0 = tx * kLpX, y0 = ty * kLpY, z0 = tz * kLpZ;
+    // halo cell e <-> (x0 - 1 + hx, y0 - 1 + hy, z0 - 1 + hz); cells outside the grid hold no particles
+    auto halo_cell = [&](int e, int& q) {
+        const int hx = e % kLpHX, hy = (e / kLpHX) % kLpHY, hz = e / (kLpHX * kLpHY);
+        const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+        const bool in = gx >= 0 && gy >= 0 && gz >= 0 && gx < ni && gy < nj && gz < nk;
+        q = in ? (gz * nj + gy) * ni + gx : -1;
+        return in;
+    };
+    // counts per halo cell, then an exclusive scan (thread t owns cells 3t..3t+2 of the 768 slots)
+    int cnt[3], st[3], run = 0;
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int e = 3 * t + u;
+        int q;
+        cnt[u] = 0;
+        st[u] = 0;
+        if (e < kLpHalo && halo_cell(e, q)) {
+            st[u] = cell_start[q];
+            cnt[u] = cell_start[q + 1] - st[u];
+        }
+        run += cnt[u];
+    }
+    int total;
+    int before = block_exclusive_scan(run, total);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) {
+        const int e = 3 * t + u;
+        if (e < kLpHalo) {
+            s_off[e] = before;
+            s_src[e] = st[u] - before;
+        }
+        before += cnt[u];
+        if (e < kLpHalo && total <= kLpCap) {
+            s_x[before + e] = kLpFar;  // the pad word after cell e's particles
+            s_y[before + e] = kLpFar;
+            s_z[before + e] = kLpFar;
+        }
+    }
+    if (t == 0) s_off[kLpHalo] = total;
+    __syncthreads();
+    const bool staged = total <= kLpCap;
+    if (staged) {
+        // one lane per staged particle: its halo cell is the last e with s_off[e] <= slot,
+        // which is a non-empty cell inside the grid (cells outside hold no particles);
+        // it lands at slot + e, after the pad words of cells 0..e-1
+        // (four slots per pass, so each lane has four independent load chains in flight)
+        constexpr int U = 4;
+        for (int base = t; base < total; base += U * kBlock) {
+            int p[U], dst[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int slot = base + u * kBlock;
+                p[u] = -1;
+                dst[u] = 0;
+                if (slot < total) {
+                    // fixed-length search (kLpHalo < 1024), unrolled so the U chains overlap
+                    int lo = 0;
+#pragma unroll
+                    for (int step = 512; step > 0; step >>= 1)
+                        if (lo + step < kLpHalo && s_off[lo + step] <= slot) lo += step;
+                    p[u] = order[s_src[lo] + slot];
+                    dst[u] = slot + lo;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (p[u] >= 0) {
+                    s_x[dst[u]] = pos[3 * p[u]];
+                    s_y[dst[u]] = pos[3 * p[u] + 1];
+                    s_z[dst[u]] = pos[3 * p[u] + 2];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int i = x0 + t % kLpX, j = y0 + (t / kLpX) % kLpY, k = z0 + t / (kLpX * kLpY);
+    if (i >= ni || j >= nj || k >= nk) return;
+    const float cx = (float)i * h, cy = (float)j * h, cz = (float)k * h;
+    float best2 = INFINITY;
+    if (staged) {
+        // The three x-neighbour cells of a (y, z) neighbour row are consecutive
+        // halo cells, so their particles and pad words form one run of LDS; runs
+        // of rows outside the grid are empty. Each run is scanned kLpRead words a
+        // step, the words past its end masked out.
+        const int li = i - x0, lj = j - y0 + 1, lk = k - z0 + 1;  // li: halo x of the cell's left neighbour
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const int e0 = ((lk + r / 3 - 1) * kLpHY + (lj + r % 3 - 1)) * kLpHX + li;
+            const int a = s_off[e0] + e0, b = s_off[e0 + 3] + e0 + 3;
+            for (int s = a; s < b; s += kLpRead) {
+                float d2[kLpRead];
+#pragma unroll
+                for (int u = 0; u < kLpRead; ++u) {
+                    const float dx = cx - s_x[s + u], dy = cy - s_y[s + u], dz = cz - s_z[s + u];
+                    d2[u] = dx * dx + dy * dy + dz * dz;
+                }
+#pragma unroll
+                for (int u = 0; u < kLpRead; ++u) best2 = fminf(best2, s + u < b ? d2[u] : INFINITY);
+            }
+        }
+    } else {
+        best2 = cell_min2_global(i, j, k, ni, nj, nk, cx, cy, cz, pos, cell_start, order, best2);
+    }
+    const float d = sqrtf(best2), cap = 3.0f * h;
+    const float best = d < cap ? d : cap;
+    phi[(k * nj + j) * ni + i] = best - radius;
 }
 }  // namespace vflip
 
@@ -1320,9 +1451,9 @@ int vflip_liquid_phi(int ni, int nj, int nk, float h, float radius, const float*
                      const int* order, float* phi, void* hip_stream) {
     VF_ARG(ni > 0 && nj > 0 && nk > 0 && (size_t)ni * nj * nk < (1u << 31), "grid dims");
     VF_ARG(cell_start && order && phi, "level-set arrays");
-    const size_t n = (size_t)ni * nj * nk;
-    k_liquid_phi<<<grid_for(n), kBlock, 0, (hipStream_t)hip_stream>>>(ni, nj, nk, h, radius, pos, cell_start, order,
-                                                                      phi);
+    const int tiles_x = (ni + kLpX - 1) / kLpX, tiles_y = (nj + kLpY - 1) / kLpY, tiles_z = (nk + kLpZ - 1) / kLpZ;
+    k_liquid_phi<<<tiles_x * tiles_y * tiles_z, kBlock, 0, (hipStream_t)hip_stream>>>(
+        ni, nj, nk, h, radius, pos, cell_start, order, phi, tiles_x, tiles_y);
     VF_CHECK_LAUNCH();
     return 0;
 }

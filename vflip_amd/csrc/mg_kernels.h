@@ -662,29 +662,38 @@ __global__ __launch_bounds__(256) void k_residual_restrict(Op<CT> op, const floa
 
 // ----------------------------------------------------------------------------
 // Coarsest level (<= 4096 unknowns, S/GeometricLevelGen.h:46): all RB-GS sweeps
-// in one workgroup, x in LDS, operator in registers. Each thread owns up to 8
+// in one workgroup, x in LDS, operator in registers. Each thread owns up to 2
 // x-pairs (2m, 2m+1) of a row; a pair holds one cell of each colour, so every
-// lane has work in both half sweeps.
+// lane has work in both half sweeps. LDS holds x split by colour, slot
+// colour * npairs + pair index, so the lanes of a wave touch consecutive words
+// (no bank conflicts) where cell order would give them a stride of two.
 // ----------------------------------------------------------------------------
 constexpr int kCoarseMax = 4096;
 constexpr int kCoarseThreads = 1024;
 constexpr int kCoarsePairs = 2;  // pairs per thread: up to 2048 pairs
 
+// LDS slot of cell (i, j, k): its colour's half, then row-major pair order
+__device__ inline int coarse_slot(int i, int j, int k, int nj, int ppr, int npairs) {
+    return ((i + j + k) & 1) * npairs + (k * nj + j) * ppr + (i >> 1);
+}
+
 template <typename CT>
 __device__ inline void coarse_cell(const Op<CT>& op, const float* __restrict__ b, int unit, int i, int j, int k,
-                                   int& cell, float& d, float& bb, float (&a)[6], int (&nb)[6]) {
+                                   int ppr, int npairs, int& cell, int& slot, float& d, float& bb, float (&a)[6],
+                                   int (&nb)[6]) {
     const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
     if (i >= ni) return;
     const int c = (k * nj + j) * ni + i;
     cell = c;
+    slot = coarse_slot(i, j, k, nj, ppr, npairs);
     d = op.D(c);
     bb = unit >= 0 ? (c == unit ? 1.f : 0.f) : b[c];
-    if (k > 0) { nb[0] = c - sl; a[0] = op.AZ(c - sl); }
-    if (j > 0) { nb[1] = c - ni; a[1] = op.AY(c - ni); }
-    if (i > 0) { nb[2] = c - 1; a[2] = op.AX(c - 1); }
-    if (i + 1 < ni) { nb[3] = c + 1; a[3] = op.AX(c); }
-    if (j + 1 < nj) { nb[4] = c + ni; a[4] = op.AY(c); }
-    if (k + 1 < nk) { nb[5] = c + sl; a[5] = op.AZ(c); }
+    if (k > 0) { nb[0] = coarse_slot(i, j, k - 1, nj, ppr, npairs); a[0] = op.AZ(c - sl); }
+    if (j > 0) { nb[1] = coarse_slot(i, j - 1, k, nj, ppr, npairs); a[1] = op.AY(c - ni); }
+    if (i > 0) { nb[2] = coarse_slot(i - 1, j, k, nj, ppr, npairs); a[2] = op.AX(c - 1); }
+    if (i + 1 < ni) { nb[3] = coarse_slot(i + 1, j, k, nj, ppr, npairs); a[3] = op.AX(c); }
+    if (j + 1 < nj) { nb[4] = coarse_slot(i, j + 1, k, nj, ppr, npairs); a[4] = op.AY(c); }
+    if (k + 1 < nk) { nb[5] = coarse_slot(i, j, k + 1, nj, ppr, npairs); a[5] = op.AZ(c); }
 }
 
 // tab != nullptr: tabulation mode. Workgroup u solves with b = e_u and writes
@@ -701,12 +710,13 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
     const int npairs = ppr * nj * nk;
     // [q][colour]: colour 1 = odd (i+j+k), updated first
     float cd[kCoarsePairs][2], cb[kCoarsePairs][2], ca[kCoarsePairs][2][6];
-    int nbr[kCoarsePairs][2][6], cell[kCoarsePairs][2];
+    int nbr[kCoarsePairs][2][6], cell[kCoarsePairs][2], slot[kCoarsePairs][2];
 #pragma unroll
     for (int q = 0; q < kCoarsePairs; ++q) {
 #pragma unroll
         for (int col = 0; col < 2; ++col) {
             cell[q][col] = -1;
+            slot[q][col] = kCoarseMax;
             cd[q][col] = 0.f;
             cb[q][col] = 0.f;
 #pragma unroll
@@ -721,12 +731,14 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
             const int j = row % nj, k = row / nj;
             // cell 2m+e is odd when e == 1 - ((j+k)&1)
             const int eodd = ((j + k) & 1) ? 0 : 1;
-            coarse_cell(op, b, unit, 2 * m + eodd, j, k, cell[q][1], cd[q][1], cb[q][1], ca[q][1], nbr[q][1]);
-            coarse_cell(op, b, unit, 2 * m + 1 - eodd, j, k, cell[q][0], cd[q][0], cb[q][0], ca[q][0], nbr[q][0]);
+            coarse_cell(op, b, unit, 2 * m + eodd, j, k, ppr, npairs, cell[q][1], slot[q][1], cd[q][1], cb[q][1],
+                        ca[q][1], nbr[q][1]);
+            coarse_cell(op, b, unit, 2 * m + 1 - eodd, j, k, ppr, npairs, cell[q][0], slot[q][0], cd[q][0],
+                        cb[q][0], ca[q][0], nbr[q][0]);
         }
 #pragma unroll
         for (int col = 0; col < 2; ++col)
-            if (cell[q][col] >= 0) xs[cell[q][col]] = 0.f;
+            if (cell[q][col] >= 0) xs[slot[q][col]] = 0.f;
     }
     if (threadIdx.x == 0) xs[kCoarseMax] = 0.f;
     __syncthreads();
@@ -743,7 +755,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
             }
 #pragma unroll
             for (int q = 0; q < kCoarsePairs; ++q)
-                if (cell[q][col] >= 0) xs[cell[q][col]] = nv[q];
+                if (cell[q][col] >= 0) xs[slot[q][col]] = nv[q];
             __syncthreads();
         }
     }
@@ -752,8 +764,8 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
 #pragma unroll
         for (int col = 0; col < 2; ++col)
             if (cell[q][col] >= 0) {
-                if (tab) tab[(size_t)unit * (op.ni * op.nj * op.nk) + cell[q][col]] = xs[cell[q][col]];
-                else x[cell[q][col]] = xs[cell[q][col]];
+                if (tab) tab[(size_t)unit * (op.ni * op.nj * op.nk) + cell[q][col]] = xs[slot[q][col]];
+                else x[cell[q][col]] = xs[slot[q][col]];
             }
 }
 
