@@ -14,7 +14,8 @@ still runs to the tolerance); the mixed path without the network and the plain
 FP32 MGPCG path are timed beside it.
 
 The `roofline` object describes the dominant kernel (the fused level-0
-red-black smoothing sweep): algorithmic bytes per launch (28 B/cell) over the
+red-black smoothing sweep of the mixed run, which reads the FP16 copy of the
+level-0 coefficients): algorithmic bytes per launch (20 B/cell) over the
 average launch time measured with HIP events on the solver's stream inside the
 timed steps. `cpu_baseline` times the FP64 CPU oracle (a restatement of the
 reference's TBB path) on one step of the same workload on the host cores.
@@ -367,8 +368,8 @@ def main():
         achieved = prof["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic_from_profiles("k_rbgs_zrow<4, 0, double>") if n == 256 else None,
-                    "kernel": "k_rbgs_zrow<4, 0, double> (fused red-black sweep, level 0, mixed run)",
+                    "traffic": traffic_from_profiles("k_rbgs_zrow<4, 0, double, __half>") if n == 256 else None,
+                    "kernel": "k_rbgs_zrow<4, 0, double, __half> (fused red-black sweep, level 0, mixed run)",
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": prof["launches"],
                     "algorithmic_bytes_per_launch": prof["bytes_per_launch"]}
 

@@ -132,7 +132,8 @@ int vflip_solver_matvec(vflip_solver* s, const float* x, float* y);
  * its common form: the sweeps after the first of each pre/post group):
  * enable=1 records HIP events around each launch on the solver's stream;
  * _read returns the launches since enabling/the last read, their summed
- * duration and the algorithmic HBM bytes of one launch (28 B per cell). */
+ * duration and the algorithmic HBM bytes of one launch (28 B per cell; 20 B per
+ * cell when a mixed-precision solver reads level 0's FP16 coefficient copy). */
 int vflip_solver_profile(vflip_solver* s, int enable);
 int vflip_solver_profile_read(vflip_solver* s, int* launches, double* total_ms, double* bytes_per_launch);
 

@@ -379,6 +379,23 @@ template <int N> __device__ inline void vld(const float* p, float (&v)[N]) {
         v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
     }
 }
+// FP16 coefficients (the scaled copies of a mixed-precision hierarchy), widened
+// to float; the caller applies the level's scale.
+template <int N> __device__ inline void vld(const __half* p, float (&v)[N]) {
+    if constexpr (N == 1) {
+        v[0] = __half2float(p[0]);
+    } else if constexpr (N == 2) {
+        const float2 t = __half22float2(*reinterpret_cast<const __half2*>(p));
+        v[0] = t.x; v[1] = t.y;
+    } else {
+        const uint2 u = *reinterpret_cast<const uint2*>(p);
+        __half2 h0, h1;
+        __builtin_memcpy(&h0, &u.x, 4);
+        __builtin_memcpy(&h1, &u.y, 4);
+        const float2 t0 = __half22float2(h0), t1 = __half22float2(h1);
+        v[0] = t0.x; v[1] = t0.y; v[2] = t1.x; v[3] = t1.y;
+    }
+}
 template <int N> __device__ inline void vst(float* p, const float (&v)[N]) {
     if constexpr (N == 1) {
         p[0] = v[0];
@@ -393,8 +410,8 @@ template <int N> __device__ inline void vst(float* p, const float (&v)[N]) {
 template <int E, int V> __device__ inline float Rc(const float (&v)[V], int h) { return v[2 * h + 1 - E]; }
 template <int E, int V> __device__ inline float Bc(const float (&v)[V], int h) { return v[2 * h + E]; }
 
-template <int V, int MODE, typename Acc>
-__global__ __launch_bounds__(1024) void k_rbgs_zrow(Op<float> op, const float* __restrict__ b,
+template <int V, int MODE, typename Acc, typename CT = float>
+__global__ __launch_bounds__(1024) void k_rbgs_zrow(Op<CT> op, const float* __restrict__ b,
                                                     const float* __restrict__ xin, float* __restrict__ xout,
                                                     const float* __restrict__ xc, int ci, int cj, int tiles_y,
                                                     int zchunk, double* __restrict__ dot) {
@@ -956,8 +973,8 @@ __global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* _
 // restriction sums each 2x2x2 aggregate as they do: the x pair inside the lane,
 // the y pair (rows 2J, 2J+1 = waves w, w+1) through LDS one plane later, the z
 // pair in registers; z chunks start on even planes.
-template <int V, int KIND, typename Acc>
-__global__ __launch_bounds__(1024) void k_stencil_row(Op<float> op, const float* __restrict__ b,
+template <int V, int KIND, typename Acc, typename CT = float>
+__global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __restrict__ b,
                                                       const float* __restrict__ s, float* __restrict__ out, int ci,
                                                       int cj, int tiles_y, int zchunk, double* __restrict__ partial) {
     constexpr int H = V / 2;

@@ -78,32 +78,30 @@ RowGeom rowgeom(const vflip_solver* S, const LevelBuf& lv, std::initializer_list
     g.blocks = g.tiles_y * ((lv.nk + zc - 1) / zc);
     return g;
 }
-template <int KIND, typename Acc>
+template <int KIND, typename Acc, typename CT = float>
 void launch_stencil_row(vflip_solver* S, const RowGeom& g, const LevelBuf& lv, const float* b, const float* s,
                         float* out, int ci, int cj, double* partial) {
-    const Op<float> op = make_op<float>(lv);
+    const Op<CT> op = make_op<CT>(lv);
     if (g.v == 4)
-        k_stencil_row<4, KIND, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y, g.zchunk,
-                                                                       partial);
+        k_stencil_row<4, KIND, Acc, CT><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y,
+                                                                           g.zchunk, partial);
     else
-        k_stencil_row<2, KIND, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y, g.zchunk,
-                                                                       partial);
+        k_stencil_row<2, KIND, Acc, CT><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y,
+                                                                           g.zchunk, partial);
 }
 
 template <int MODE, typename Acc, typename CT>
 void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<CT>& op, const float* b, const float* xin,
                         float* xout, const float* xc, int ci, int cj, double* dot) {
-    if constexpr (std::is_same<CT, float>::value) {
-        if (g.v == 4) {
-            k_rbgs_zrow<4, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
+    if (g.v == 4) {
+        k_rbgs_zrow<4, MODE, Acc, CT><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
                                                                          g.zchunk, dot);
-            return;
-        }
-        if (g.v == 2) {
-            k_rbgs_zrow<2, MODE, Acc><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
+        return;
+    }
+    if (g.v == 2) {
+        k_rbgs_zrow<2, MODE, Acc, CT><<<g.blocks, 1024, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_y,
                                                                          g.zchunk, dot);
-            return;
-        }
+        return;
     }
     if (g.ty == 16)
         k_rbgs_zsweep<16, MODE, Acc, CT><<<g.blocks, 256, 0, S->stream>>>(op, b, xin, xout, xc, ci, cj, g.tiles_x,
@@ -116,11 +114,10 @@ void launch_zsweep_mode(vflip_solver* S, const ZGeom& g, const Op<CT>& op, const
                                                                           g.tiles_y, g.zchunk, dot);
 }
 
-// FP16-stored levels always take the tile form (the row-wave kernel reads FP32 coefficients).
 template <typename Acc, typename CT>
 void launch_zsweep(vflip_solver* S, const LevelBuf& lv, const float* b, const float* xin, float* xout, int mode,
                    const float* xc, int ci, int cj, double* dot) {
-    const ZGeom g = zgeom(lv, std::is_same<CT, float>::value && S->use_zrow, S->cus);
+    const ZGeom g = zgeom(lv, S->use_zrow, S->cus);
     const Op<CT> op = make_op<CT>(lv);
     if (mode == 1) launch_zsweep_mode<1, Acc, CT>(S, g, op, b, xin, xout, xc, ci, cj, dot);
     else if (mode == 2) launch_zsweep_mode<2, Acc, CT>(S, g, op, b, xin, xout, xc, ci, cj, dot);
@@ -199,10 +196,10 @@ template <typename CT> int restrict_level(vflip_solver* S, int l) {
     LevelBuf& c = S->L[l + 1];
     const float* b = (l == 0) ? S->r : lv.b;
     const float* x = (l == 0) ? S->z : lv.x;
-    if constexpr (std::is_same<CT, float>::value) {
+    {
         const RowGeom rg = rowgeom(S, lv, {b, x, c.b});
         if (rg.v) {
-            launch_stencil_row<1, float>(S, rg, lv, b, x, c.b, c.ni, c.nj, nullptr);
+            launch_stencil_row<1, float, CT>(S, rg, lv, b, x, c.b, c.ni, c.nj, nullptr);
             VF_CHECK_LAUNCH();
             return 0;
         }
@@ -679,9 +676,19 @@ int vflip_solver_setup(vflip_solver* S) {
         VF_CHECK_LAUNCH();
     }
     if (S->opts.precision == VFLIP_PREC_MIXED) {
-        // FP16 copies of levels >= 2, scaled so the largest diagonal maps to 2^10.
-        for (size_t l = 2; l < S->L.size(); ++l) {
+        // FP16 copies of every level for the V-cycle (the PCG matvec and the true
+        // residual keep the FP32 level 0). The scale is the power of two that maps
+        // the largest diagonal into (512, 1024], so integer coefficients (an
+        // all-fluid interior's -1 couplings and 6 diagonals, and their Galerkin
+        // coarsenings) stay exact.
+        for (size_t l = 0; l < S->L.size(); ++l) {
             LevelBuf& lv = S->L[l];
+            if (!lv.hd) {
+                void* p = nullptr;
+                int rc = alloc(&p, 4 * lv.n * sizeof(__half));
+                if (rc) return rc;
+                lv.hd = (__half*)p; lv.hax = lv.hd + lv.n; lv.hay = lv.hax + lv.n; lv.haz = lv.hay + lv.n;
+            }
             launch_absmax(S, (unsigned)lv.n, lv.d);
             VF_CHECK_LAUNCH();
             int rc = finalize(S, 5);
@@ -689,7 +696,7 @@ int vflip_solver_setup(vflip_solver* S) {
             rc = fetch_scalars(S);
             if (rc) return rc;
             const double mx = S->h_sc->true_res;
-            const float scale = (mx > 0) ? (float)(mx / 1024.0) : 1.f;
+            const float scale = (mx > 0) ? (float)std::ldexp(1.0, (int)std::ceil(std::log2(mx / 1024.0))) : 1.f;
             lv.hscale = scale;
             const float inv = 1.f / scale;
             const unsigned g = grid_for(lv.n);
@@ -735,8 +742,10 @@ int vflip_solver_profile_read(vflip_solver* S, int* launches, double* total_ms, 
     }
     if (launches) *launches = S->prof_used / 2;
     if (total_ms) *total_ms = tot;
-    // algorithmic bytes of one fused sweep on level 0: b, x_in, diag + 3 couplings read, x_out written
-    if (bytes_per_launch) *bytes_per_launch = 28.0 * (double)S->n;
+    // algorithmic bytes of one fused sweep on level 0: b and x_in read (4 B each), diag + 3
+    // couplings read (4 B each in FP32, 2 B each from a mixed hierarchy's FP16 copy), x_out written
+    const double coef = (!S->L.empty() && S->L[0].half) ? 2.0 : 4.0;
+    if (bytes_per_launch) *bytes_per_launch = (12.0 + 4.0 * coef) * (double)S->n;
     S->prof_used = 0;
     return 0;
 }

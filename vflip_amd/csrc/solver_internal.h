@@ -113,7 +113,7 @@ inline ZGeom zgeom(const LevelBuf& lv, bool use_zrow, int cus) {
 }
 inline ZGeom zgeom(const vflip_solver* S, const LevelBuf& lv) { return zgeom(lv, S->use_zrow, S->cus); }
 
-// Levels at least this large (and not FP16) use the fused z-marching sweep.
+// Levels at least this large use the fused z-marching sweep.
 inline bool use_zsweep(const vflip_solver* S, const LevelBuf& lv, int min_n) {
     return (int)lv.n >= min_n && lv.ni >= 16 && lv.nj >= 4 && lv.nk >= 4;
 }
