@@ -22,6 +22,7 @@ reference's TBB path) on one step of the same workload on the host cores.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--no-cpu]
   python bench.py --rows   (each scope row's GPU call beside its CPU oracle path)
+  python bench.py --trajectory 20 --n 128   (p/u error against the FP64 oracle after 20 steps)
 """
 import argparse
 import json
@@ -128,7 +129,7 @@ def _cpu_step(sim_state, phi, n, dt):
     import numpy as np
     from oracle import oracle as O
     h = np.float32(1.0 / n)
-    st = {k: v.detach().cpu().numpy() for k, v in sim_state.items()}
+    st = {k: (v if isinstance(v, np.ndarray) else v.detach().cpu().numpy()) for k, v in sim_state.items()}
     uw, vw, ww = O.face_weights(phi)
     z = [np.zeros_like(st[k]) for k in ("u", "v", "w")]
     t0 = time.perf_counter()
@@ -149,8 +150,47 @@ def _cpu_step(sim_state, phi, n, dt):
         "ms_per_step": (t1 - t0) * 1e3,
         "ms_solve": (t_solve - t_asm) * 1e3,
         "iterations": out["iterations"],
-        "p": out["x"], "u": uo, "v": vo, "w": wo,
+        "p": out["x"], "u": uo, "v": vo, "w": wo, "rho": r2, "temp": t2,
     }
+
+
+def trajectory(n, steps, phi, spheres):
+    """Error of the GPU plume after `steps` steps from the initial state against
+    the FP64 CPU oracle run for the same steps: relative L2 of p, u, v, w and rho
+    for the plain, mixed and learned + mixed paths, with per-step iterations."""
+    import numpy as np
+    import torch
+    from vflip_amd import api
+    from vflip_amd.plume import PlumeSim
+    rel = lambda a, b: float(np.linalg.norm(a.astype(np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+    init = {k: t.detach().cpu().numpy().copy() for k, t in PlumeSim(n, phi, precision=api.FP32).state().items()}
+    log(f"trajectory: {steps} oracle steps at {n}^3 ...")
+    st = dict(init)
+    its_cpu = []
+    for s in range(steps):
+        o = _cpu_step(st, phi, n, 0.01)
+        st = {"u": o["u"], "v": o["v"], "w": o["w"], "rho": o["rho"], "temp": o["temp"], "p": o["p"]}
+        its_cpu.append(o["iterations"])
+        log(f"oracle step {s}: {o['ms_per_step']:.0f} ms, {o['iterations']} its")
+    paths = [("plain", api.FP32, None), ("mixed", api.MIXED, None)]
+    net = load_network()
+    if net is not None:
+        paths.append(("learned_mixed", api.MIXED, net))
+    out = {"workload": f"smoke plume {n}^3 ({len(spheres)} sphere obstacles), {steps} steps from the initial state",
+           "oracle": "FP64 CPU restatement, AMGPCG to 1e-6", "oracle_iterations": its_cpu, "paths": {}}
+    for name, prec, nt in paths:
+        sim = PlumeSim(n, phi, precision=prec)
+        if nt is not None:
+            sim.use_network(nt, max(1, n // 64))
+        its = []
+        for _ in range(steps):
+            its.append(sim.step()["iterations"])
+        torch.cuda.synchronize()
+        g = {k: t.detach().cpu().numpy() for k, t in sim.state().items()}
+        out["paths"][name] = {"iterations": its, **{f"{k}_rel_l2": rel(g[k], st[k]) for k in ("p", "u", "v", "w", "rho")}}
+        log(f"{name}: {out['paths'][name]}")
+        del sim
+    return out
 
 
 def _gpu_ms(fn, reps=10):
@@ -320,6 +360,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--rows", action="store_true",
                     help="instead of the headline step: each scope row's GPU call beside its CPU oracle path")
+    ap.add_argument("--trajectory", type=int, default=0, metavar="K",
+                    help="instead of the headline step: p/u error against the FP64 oracle after K steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -337,6 +379,9 @@ def main():
     n = args.n
     phi, spheres = make_scene(n)
     log(f"scene n={n} spheres={spheres}")
+    if args.trajectory > 0:
+        print(json.dumps(trajectory(n, args.trajectory, phi, spheres)), flush=True)
+        return
 
     # plain FP32 MGPCG
     sim_p, ms_plain, its_plain, proj_plain, _, st_p = run_gpu(n, args.steps, args.warmup, api.FP32, phi)
