@@ -62,11 +62,13 @@ def load_network():
     return net
 
 
-def run_gpu(n, steps, warmup, precision, phi, init_state=None, profile=False, net=None):
+def run_gpu(n, steps, warmup, precision, phi, init_state=None, profile=False, net=None, solver_opts=None):
     import torch
     from vflip_amd import api
     from vflip_amd.plume import PlumeSim
     sim = PlumeSim(n, phi, precision=precision)
+    if solver_opts:
+        sim.solver.set_opts(**solver_opts)
     if net is not None:
         sim.use_network(net, max(1, n // 64))
     if init_state is not None:
@@ -393,6 +395,16 @@ def main():
                                                                profile=True)
     log(f"mixed: {ms_mixed:.3f} ms/step its={its_mixed} prof={prof}")
     del sim_m
+    # the mixed form as the plume configuration words it: FP16 copies of levels >= 2 only,
+    # FP64 true-residual refresh every 8 iterations (timed beside the tuned default)
+    cfg_opts = {"half_from_level": 2, "refine_every": 8}
+    sim_c, ms_cfg, its_cfg, proj_cfg, _, st_c = run_gpu(n, args.steps, args.warmup, api.MIXED, phi,
+                                                        solver_opts=cfg_opts)
+    log(f"mixed (configuration form): {ms_cfg:.3f} ms/step its={its_cfg}")
+    mixed_cfg = {"ms_per_step": round(ms_cfg, 3), "iterations": its_cfg,
+                 "projection_ms": round(float(np.mean(proj_cfg)), 3), "true_residual_last": st_c["true_residual"],
+                 "solver_opts": cfg_opts}
+    del sim_c
     # learned + mixed: the U-Net's guess (on the grid coarsened to 64^3) seeds every mixed solve
     net = load_network()
     learned = None
@@ -447,6 +459,7 @@ def main():
 
     voxels = n ** 3
     head = learned["ms_per_step"] if learned else ms_mixed
+    dflt = api.default_opts()
     out = {
         "metric": METRIC,
         "value": round(head, 3),
@@ -481,6 +494,10 @@ def main():
                            "learned_mixed": ({"recursive": st_l["rel_residual"], "true_fp64": st_l["true_residual"]}
                                              if learned else None)},
         "learned_warm_start": learned,
+        # the tuned mixed form above (FP16 coefficients at every V-cycle level, FP64 refresh
+        # every 4 iterations) beside the form the plume configuration words
+        "mixed_solver_opts": {"half_from_level": dflt.half_from_level, "refine_every": dflt.refine_every},
+        "mixed_configuration_form": mixed_cfg,
         "fp64_oracle_error": oracle_err,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -47,8 +47,9 @@ typedef struct vflip_solver vflip_solver;
 
 enum vflip_precision {
     VFLIP_PREC_FP32 = 0,  /* plain: FP32 storage, FP32 smoothing, FP32 CG reductions */
-    VFLIP_PREC_MIXED = 1  /* FP32 smoothing, FP16-stored levels >= 2, FP64 CG dots and
-                             alpha/beta, FP64 true-residual refinement every refine_every */
+    VFLIP_PREC_MIXED = 1  /* FP32 smoothing on FP16-stored levels >= half_from_level, FP64
+                             CG dots and alpha/beta, FP64 true-residual refinement every
+                             refine_every iterations; stops on the FP64 true residual */
 };
 
 enum vflip_variant {
@@ -63,8 +64,10 @@ typedef struct {
     int pre_sweeps;     /* RB-GS sweeps before restriction (reference: 4) */
     int post_sweeps;    /* RB-GS sweeps after prolongation (reference: 4) */
     int coarse_sweeps;  /* RB-GS sweeps on the coarsest level (reference: 200) */
-    int refine_every;   /* mixed: recompute r = b - A x in FP64 every N iterations (8) */
+    int refine_every;   /* mixed: recompute r = b - A x in FP64 every N iterations (4) */
     int warm_start;     /* 1: the x passed to vflip_solver_solve is the initial guess */
+    int half_from_level; /* mixed: the V-cycle reads FP16 copies of levels >= this (0: all;
+                            the PCG matvec and the true residual keep FP32 level 0) */
 } vflip_solve_opts;
 
 typedef struct {

@@ -25,6 +25,7 @@ class SolveOpts(ctypes.Structure):
         ("coarse_sweeps", c_int),
         ("refine_every", c_int),
         ("warm_start", c_int),
+        ("half_from_level", c_int),
     ]
 
 
