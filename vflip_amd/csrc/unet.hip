@@ -579,13 +579,14 @@ int vflip_unet_warm_start(vflip_unet* N, int ni, int nj, int nk, int factor, con
     if (int rc = ensure_warm_grid(N, cx, cy, cz)) return rc;
     if (int rc = ensure_grid(N, cx, cy, cz)) return rc;
     const size_t Vc = (size_t)cx * cy * cz;
-    k_restrict_inputs<<<grid256(Vc), 256, 0, N->stream>>>(rhs, diag, ni, nj, nk, factor, N->w_rhs, N->w_solid,
-                                                          N->w_liquid);
+    k_restrict_inputs<<<dim3((cx + 255) / 256, cy, cz), 256, 0, N->stream>>>(rhs, diag, ni, nj, nk, factor, N->w_rhs,
+                                                                             N->w_solid, N->w_liquid);
     const float* sd = prepare_scale(N, N->w_rhs, Vc, 0.f);
     k_pack_input<<<grid256(Vc), 256, 0, N->stream>>>(N->w_rhs, N->w_solid, N->w_liquid, Vc, 0.f, sd, N->act[kIn]);
     forward_layers(N);
     const size_t V = (size_t)ni * nj * nk;
-    k_prolong_cells<<<grid256(V), 256, 0, N->stream>>>(N->pred, cx, cy, cz, factor, sd, N->gain, diag, x0);
+    k_prolong_cells<<<dim3((ni + 255) / 256, nj, nk), 256, 0, N->stream>>>(N->pred, cx, cy, cz, factor, sd, N->gain,
+                                                                           diag, x0);
     VF_CHECK_LAUNCH();
     return 0;
 }

@@ -491,43 +491,46 @@ __global__ __launch_bounds__(256) void k_scale_copy(const float* __restrict__ y,
 // Solve grid -> network grid by an integer factor f: rhs averaged over each
 // f^3 block; solid = fraction of cells outside the system (diagonal 0, the
 // solid and Dirichlet cells); liquid = 1 - solid.
+// 3D launch: x over coarse i (256 a block), y over coarse j, z over coarse k.
 __global__ __launch_bounds__(256) void k_restrict_inputs(const float* __restrict__ rhs, const float* __restrict__ diag,
                                                          int nx, int ny, int nz, int f, float* __restrict__ crhs,
                                                          float* __restrict__ csolid, float* __restrict__ cliquid) {
-    const int cx = nx / f, cy = ny / f, cz = nz / f;
-    const size_t n = (size_t)cx * cy * cz;
+    const int cx = nx / f, cy = ny / f;
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= cx) return;
     const float inv = 1.0f / (float)(f * f * f);
-    for (size_t c = blockIdx.x * (size_t)256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256) {
-        const int i = (int)(c % cx), j = (int)((c / cx) % cy), k = (int)(c / ((size_t)cx * cy));
-        float sr = 0.f, ss = 0.f;
-        for (int dk = 0; dk < f; ++dk)
-            for (int dj = 0; dj < f; ++dj)
-                for (int di = 0; di < f; ++di) {
-                    const size_t v = ((size_t)(k * f + dk) * ny + (j * f + dj)) * nx + (i * f + di);
-                    sr += rhs[v];
-                    ss += (diag && diag[v] == 0.f) ? 1.f : 0.f;
-                }
-        crhs[c] = sr * inv;
-        csolid[c] = ss * inv;
-        cliquid[c] = 1.f - ss * inv;
-    }
+    float sr = 0.f, ss = 0.f;
+    for (int dk = 0; dk < f; ++dk)
+        for (int dj = 0; dj < f; ++dj) {
+            const size_t row = ((size_t)(k * f + dk) * ny + (j * f + dj)) * nx + (size_t)i * f;
+            for (int di = 0; di < f; ++di) {
+                sr += rhs[row + di];
+                ss += (diag && diag[row + di] == 0.f) ? 1.f : 0.f;
+            }
+        }
+    const size_t c = ((size_t)k * cy + j) * cx + i;
+    crhs[c] = sr * inv;
+    csolid[c] = ss * inv;
+    cliquid[c] = 1.f - ss * inv;
 }
 
 // Network grid -> solve grid: trilinear interpolation of cell-centred values
 // (fine cell i sits at coarse coordinate (i + 0.5)/f - 0.5, clamped), times
 // s_dev[0] * gain; cells outside the system (diag 0) get 0.
+// 3D launch: x over fine i (256 a block), y over fine j, z over fine k.
 __global__ __launch_bounds__(256) void k_prolong_cells(const float* __restrict__ cp, int cx, int cy, int cz, int f,
                                                        const float* __restrict__ s_dev, float gain,
                                                        const float* __restrict__ diag, float* __restrict__ x) {
-    const int nx = cx * f, ny = cy * f, nz = cz * f;
-    const size_t n = (size_t)nx * ny * nz;
+    const int nx = cx * f, ny = cy * f;
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nx) return;
+    const size_t v = ((size_t)k * ny + j) * nx + i;
     const float m = s_dev ? s_dev[0] * gain : 1.0f;
     const float invf = 1.0f / (float)f;
-    for (size_t v = blockIdx.x * (size_t)256 + threadIdx.x; v < n; v += (size_t)gridDim.x * 256) {
-        const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((size_t)nx * ny));
+    {
         if (diag && diag[v] == 0.f) {
             x[v] = 0.f;
-            continue;
+            return;
         }
         const float gx = fminf(fmaxf((i + 0.5f) * invf - 0.5f, 0.f), (float)(cx - 1));
         const float gy = fminf(fmaxf((j + 0.5f) * invf - 0.5f, 0.f), (float)(cy - 1));
