@@ -301,3 +301,37 @@ def test_reassembly_rebuilds_hierarchy(cuda, mods):
     # same operator again: the hierarchy is kept (the coarsest level is now tabulated
     # instead of swept directly, equal up to FP32 rounding)
     assert rel_l2(outs[2], outs[1]) < 1e-6
+
+
+def test_mixed_forms_and_switching(cuda, mods):
+    """The mixed solver's two forms, FP16 coefficient copies of every level with a refresh
+    every 4 iterations (the default) and of levels >= 2 with a refresh every 8 (the form the
+    plume configuration words), both stop on an FP64 true residual within the tolerance. The
+    solver is switched between them and back on one handle, which rebuilds the copies each
+    time. The default form needs no more iterations than the other; the FP64 oracle's
+    count is the reference."""
+    import torch
+    O, api = mods
+    n = 64
+    pb = build_problem(n, 6, O)
+    S, (d, ax, ay, az, rhs) = _solver_with_matrix(api, O, pb, n, precision=api.MIXED)
+    S.set_diag64(_t(d))
+    ref = O.amgpcg(d, ax, ay, az, rhs, tol=1e-6, maxit=50)
+    dflt = api.default_opts()
+    assert (dflt.half_from_level, dflt.refine_every) == (0, 4)
+    its = {}
+    for form, opts in (("default", {}), ("configuration", {"half_from_level": 2, "refine_every": 8}),
+                       ("default again", {"half_from_level": 0, "refine_every": 4})):
+        if opts:
+            S.set_opts(**opts)
+        x = torch.zeros((n, n, n), device="cuda", dtype=torch.float64)
+        st = S.solve_f64(_t(rhs), x)
+        r = rhs - O.matvec(d, ax, ay, az, _np(x))
+        print(f"{form}: its {st['iterations']} (oracle {ref['iterations']}), true residual {st['true_residual']:.2e}",
+              flush=True)
+        assert st["status"] == 0 and st["true_residual"] <= 1e-6
+        assert np.abs(r).max() <= 1.0e-6 * np.abs(rhs).max()
+        its[form] = st["iterations"]
+    assert its["default"] == its["default again"]
+    assert its["default"] <= its["configuration"]
+    assert its["default"] <= ref["iterations"] + 1

@@ -11,6 +11,8 @@
 // are bit-identical with the reference arithmetic.
 #pragma clang fp contract(off)
 
+#include <cmath>
+
 #include "common.h"
 #include "sampling.h"
 
@@ -116,6 +118,53 @@ __device__ inline float sample_linear_f(const GridF& g, float h, float inv_h, fl
     return lerpf(v1, v2, cz);
 }
 
+// The same sample where the axes flagged AX/AY/AZ are known to have a zero
+// fraction (the position is on the grid's own lattice along them): the upper
+// neighbour along such an axis is not loaded. lerpf(a, b, 0) == a for finite b
+// (up to the sign of a zero), so the value equals sample_linear_f's.
+template <bool AX, bool AY, bool AZ>
+__device__ inline float sample_lattice_f(const GridF& g, float h, float inv_h, float wx, float wy, float wz) {
+    int i, j, k;
+    float cx, cy, cz;
+    cell_coord(wx, h, inv_h, g.ox, i, cx);
+    cell_coord(wy, h, inv_h, g.oy, j, cy);
+    cell_coord(wz, h, inv_h, g.oz, k, cz);
+    auto row = [&](int jj, int kk) {
+        const float a = atf(g, i, jj, kk);
+        if constexpr (AX) return a;
+        else return lerpf(a, atf(g, i + 1, jj, kk), cx);
+    };
+    auto plane = [&](int kk) {
+        const float a = row(j, kk);
+        if constexpr (AY) return a;
+        else return lerpf(a, row(j + 1, kk), cy);
+    };
+    const float v1 = plane(k);
+    if constexpr (AZ) return v1;
+    else return lerpf(v1, plane(k + 1), cz);
+}
+
+// First RK2 stage at the output sample's own position: with a power-of-two h
+// the position is exact, so along the axes where a velocity component's grid
+// offset equals the field's the fraction is exactly 0. FIELD: 0 u faces,
+// 1 v faces, 2 w faces, 3 cell centres; -1 general (all eight corners).
+template <int FIELD>
+__device__ inline void first_stage(const GridF& gu, const GridF& gv, const GridF& gw, float h, float inv_h, float x,
+                                   float y, float z, float& u, float& v, float& w) {
+    if constexpr (FIELD < 0) {
+        u = sample_linear_f(gu, h, inv_h, x, y, z);
+        v = sample_linear_f(gv, h, inv_h, x, y, z);
+        w = sample_linear_f(gw, h, inv_h, x, y, z);
+    } else {
+        // field offsets (FIELD == axis: 0.5 on that axis); component c has 0.5 on axis c
+        constexpr bool fx = FIELD == 0, fy = FIELD == 1, fz = FIELD == 2;
+        u = sample_lattice_f<fx == true, fy == false, fz == false>(gu, h, inv_h, x, y, z);
+        v = sample_lattice_f<fx == false, fy == true, fz == false>(gv, h, inv_h, x, y, z);
+        w = sample_lattice_f<fx == false, fy == false, fz == true>(gw, h, inv_h, x, y, z);
+    }
+}
+
+template <int FIELD>
 __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int nk, float h, float inv_h, float dt,
                                                            const float* __restrict__ u, const float* __restrict__ v,
                                                            const float* __restrict__ w, GridF f0,
@@ -136,11 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int n
         wz[c] = ((float)k - f0.oz) * h;
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        uu[c] = sample_linear_f(gu, h, inv_h, wx[c], wy[c], wz[c]);
-        vv[c] = sample_linear_f(gv, h, inv_h, wx[c], wy[c], wz[c]);
-        ww[c] = sample_linear_f(gw, h, inv_h, wx[c], wy[c], wz[c]);
-    }
+    for (int c = 0; c < 2; ++c) first_stage<FIELD>(gu, gv, gw, h, inv_h, wx[c], wy[c], wz[c], uu[c], vv[c], ww[c]);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const float px = __builtin_fmaf(-hdt, uu[c], wx[c]), py = __builtin_fmaf(-hdt, vv[c], wy[c]);
@@ -207,8 +252,26 @@ int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const 
         GridF g0{field, fnx, fny, fnz, fox, foy, foz};
         GridF g1{field2 ? field2 : field, fnx, fny, fnz, fox, foy, foz};
         dim3 grid((fnx + 63) / 64, (fny + 3) / 4, (fnz + 1) / 2);
-        k_advect_rk2_f32<<<grid, dim3(64, 4), 0, (hipStream_t)stream>>>(ni, nj, nk, h, 1.0f / h, dt, u, v, w, g0, out,
-                                                                         g1, out2);
+        // lattice form of the first stage: h a power of two (positions and 1/h exact)
+        // and the field on one of the MAC grid's four lattices
+        int e2 = 0;
+        const bool pow2 = std::frexp(h, &e2) == 0.5f;
+        int field_kind = -1;
+        if (pow2) {
+            if (fox == 0.5f && foy == 0.f && foz == 0.f) field_kind = 0;
+            else if (fox == 0.f && foy == 0.5f && foz == 0.f) field_kind = 1;
+            else if (fox == 0.f && foy == 0.f && foz == 0.5f) field_kind = 2;
+            else if (fox == 0.f && foy == 0.f && foz == 0.f) field_kind = 3;
+        }
+        const float inv_h = 1.0f / h;
+        hipStream_t st = (hipStream_t)stream;
+        switch (field_kind) {
+            case 0: k_advect_rk2_f32<0><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            case 1: k_advect_rk2_f32<1><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            case 2: k_advect_rk2_f32<2><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            case 3: k_advect_rk2_f32<3><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            default: k_advect_rk2_f32<-1><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2);
+        }
         VF_CHECK_LAUNCH();
         return 0;
     }
