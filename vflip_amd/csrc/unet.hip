@@ -99,6 +99,7 @@ struct vflip_unet {
     float* scale = nullptr;  // device: max|rhs| of the current input (automatic normalisation)
     float gain = 1.0f;       // pressure unit of the automatic normalisation: max|rhs| * gain
     std::vector<void*> grid_allocs;
+    float* splitk = nullptr;  // FP32 partials of the split convolutions (per grid)
     // warm-start workspace on the network grid
     int wx = 0, wy = 0, wz = 0;
     float *w_rhs = nullptr, *w_solid = nullptr, *w_liquid = nullptr, *w_pred = nullptr;
@@ -137,6 +138,7 @@ void free_grid(vflip_unet* N) {
     }
     N->pred = N->gx = N->gpred = nullptr;
     N->gpre = nullptr;
+    N->splitk = nullptr;
     N->nx = N->ny = N->nz = 0;
 }
 
@@ -148,6 +150,42 @@ int galloc(vflip_unet* N, T** p, size_t n) {
     }
     N->grid_allocs.push_back(*p);
     return 0;
+}
+
+// Launch plan of a convolution: output channels split into g groups until the
+// layer has 128 workgroups (or one 16-channel block per workgroup); then, for
+// the small grids, the K loop into ks slices (powers of two) until about 256
+// workgroups, each slice at least 9 of the K steps.
+int conv_plan(int ntiles, int NBall, int cin, int ck, int& g) {
+    g = 1;
+    while (ntiles * g < 128 && NBall % (2 * g) == 0) g *= 2;
+    const int total = ((cin + ck - 1) / ck) * (ck == 32 ? 27 : 14);
+    int ks = 1;
+    while (ntiles * g * ks * 2 <= 256 && total / (ks * 2) >= 9) ks *= 2;
+    return ks;
+}
+
+int level_tiles(const vflip_unet* N, int level) {
+    int x, y, z;
+    level_dims(N, level, x, y, z);
+    return ((x + 7) / 8) * ((y + 7) / 8) * ((z + 7) / 8);
+}
+
+// Largest partial buffer of the forward and backward-data convolutions.
+size_t splitk_floats(const vflip_unet* N) {
+    size_t need = 0;
+    for (int l = 0; l < kLayers; ++l) {
+        const LayerDef& d = kNet[l];
+        const size_t V = level_vox(N, d.level);
+        const int nt = level_tiles(N, d.level);
+        int g;
+        int ks = conv_plan(nt, d.cout / 16, cin_of(d), ck_of(d), g);
+        if (ks > 1) need = std::max(need, (size_t)ks * V * d.cout);
+        if (l == 0) continue;
+        ks = conv_plan(nt, cin_of(d) / 16, d.cout, ckb_of(d), g);
+        if (ks > 1) need = std::max(need, (size_t)ks * V * cin_of(d));
+    }
+    return need;
 }
 
 int ensure_grid(vflip_unet* N, int nx, int ny, int nz) {
@@ -174,20 +212,24 @@ int ensure_grid(vflip_unet* N, int nx, int ny, int nz) {
     if (int rc = galloc(N, &N->pred, level_vox(N, 0))) return rc;
     if (int rc = galloc(N, &N->gpred, level_vox(N, 0))) return rc;
     if (int rc = galloc(N, &N->loss, 1)) return rc;
+    if (const size_t nk = splitk_floats(N))
+        if (int rc = galloc(N, &N->splitk, nk)) return rc;
     return 0;
 }
 
 ConvSrc make_src(const vflip_unet* N, const In& in) { return ConvSrc{N->act[in.t], in.C, in.up}; }
 
-// Output channels are split into groups until the layer has at least 128
-// workgroups (or one 16-channel block per workgroup).
+// One convolution by conv_plan: ntiles x groups x ksplit workgroups, and for a
+// split one the reduction of its partials (workspace sized by splitk_floats).
 template <int CK>
-void launch_conv(ConvArgs a, int NBall, hipStream_t s) {
-    int g = 1;
-    while (a.ntiles * g < 128 && NBall % (2 * g) == 0) g *= 2;
+void launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
+    int g;
+    const int ks = conv_plan(a.ntiles, NBall, a.cin, CK, g);
     a.groups = g;
+    a.ksplit = ks;
+    a.part = ks > 1 ? ws : nullptr;
     const int NB = NBall / g;
-    const dim3 grid(a.ntiles * g), block(kConvThreads);
+    const dim3 grid(a.ntiles * g * ks), block(kConvThreads);
     switch (NB) {
         case 1: k_conv3<CK, 1><<<grid, block, 0, s>>>(a); break;
         case 2: k_conv3<CK, 2><<<grid, block, 0, s>>>(a); break;
@@ -198,11 +240,15 @@ void launch_conv(ConvArgs a, int NBall, hipStream_t s) {
         case 12: k_conv3<CK, 12><<<grid, block, 0, s>>>(a); break;
         default: break;
     }
+    if (ks > 1) {
+        const size_t n = (size_t)a.nx * a.ny * a.nz * a.cout;
+        k_conv_splitk_reduce<<<grid256(n), 256, 0, s>>>(a);
+    }
 }
 
-void conv(const ConvArgs& a, int ck, hipStream_t s) {
-    if (ck == 32) launch_conv<32>(a, a.cout / 16, s);
-    else launch_conv<16>(a, a.cout / 16, s);
+void conv(const vflip_unet* N, const ConvArgs& a, int ck, hipStream_t s) {
+    if (ck == 32) launch_conv<32>(a, a.cout / 16, s, N->splitk);
+    else launch_conv<16>(a, a.cout / 16, s, N->splitk);
 }
 
 ConvArgs base_args(const vflip_unet* N, int level) {
@@ -240,7 +286,7 @@ void forward_layers(vflip_unet* N) {
         } else {
             a.out = N->act[l];
         }
-        conv(a, ck_of(d), s);
+        conv(N, a, ck_of(d), s);
     }
 }
 
@@ -292,7 +338,7 @@ void backward_layers(vflip_unet* N) {
         a.kw = kwb_of(d);
         a.w = N->wb16 + N->wboff[l];
         a.out32 = N->gx;
-        conv(a, ckb_of(d), s);
+        conv(N, a, ckb_of(d), s);
         if (d.nin == 1) {
             const In& in = d.in[0];
             const size_t n = V * in.C;

@@ -46,6 +46,8 @@ struct ConvArgs {
     int out32_ch0;
     int tiles_x, tiles_y, ntiles;
     int groups;  // output-channel groups: workgroup L computes channels of group L / ntiles
+    int ksplit;  // > 1: the K loop is split over ksplit workgroups that write FP32 partials
+    float* part; // [ksplit][V][cout] partials, summed by k_conv_splitk_reduce
 };
 
 __device__ inline int tap_offset_hv(int tap) {
@@ -85,12 +87,17 @@ __device__ inline void stage_halo(const ConvArgs& a, int c0, int x0, int y0, int
 // 3x3x3 convolution, implicit GEMM: M = 512 voxels of an 8^3 tile, N = cout,
 // K = 27 * cin. Each K step of 32 is one tap x 32 channels (CK = 32) or two
 // taps x 16 channels (CK = 16). A workgroup computes NB blocks of 16 output
-// channels, group L / ntiles of the a.groups groups (cout = 16 NB groups); small
-// grids split their channels over groups so that more workgroups share a layer.
+// channels, group (blockIdx / ntiles) % groups of the a.groups groups (cout =
+// 16 NB groups); small grids split their channels over groups so that more
+// workgroups share a layer. With ksplit > 1 workgroup slice blockIdx /
+// (ntiles groups) takes a contiguous range of the K steps (channel chunk x
+// step, the unsplit order) and writes its FP32 partial sums instead of the
+// outputs; with ksplit 1 the loop is the unsplit one.
 template <int CK, int NB>
 __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
     __shared__ __attribute__((aligned(16))) bf16 lds[kHV * (CK + 8)];
-    const int L = blockIdx.x % a.ntiles, grp = blockIdx.x / a.ntiles;
+    const int L = blockIdx.x % a.ntiles, grp = (blockIdx.x / a.ntiles) % a.groups;
+    const int ks = blockIdx.x / (a.ntiles * a.groups), nks = a.ksplit > 1 ? a.ksplit : 1;
     const int cbase = grp * NB * 16;
     int t = L;
     if (a.ntiles % 8 == 0) t = (L % 8) * (a.ntiles / 8) + L / 8;
@@ -109,12 +116,19 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
         const int m = mb * 16 + r16;
         hv0[mb] = ((w + 1) * kH + (m / 8 + 1)) * kH + (m % 8 + 1);
     }
-    for (int c0 = 0; c0 < a.cin; c0 += CK) {
-        __syncthreads();
-        stage_halo<CK>(a, c0, x0, y0, z0, lds);
-        __syncthreads();
-        constexpr int STEPS = CK == 32 ? 27 : 14;
-        for (int st = 0; st < STEPS; ++st) {
+    constexpr int STEPS = CK == 32 ? 27 : 14;
+    const int total = ((a.cin + CK - 1) / CK) * STEPS;
+    const int q0 = (int)((long long)ks * total / nks), q1 = (int)((long long)(ks + 1) * total / nks);
+    int staged = -1;
+    for (int q = q0; q < q1; ++q) {
+        const int c0 = (q / STEPS) * CK, st = q % STEPS;
+        if (c0 != staged) {  // uniform over the workgroup
+            __syncthreads();
+            stage_halo<CK>(a, c0, x0, y0, z0, lds);
+            __syncthreads();
+            staged = c0;
+        }
+        {
             int tap, coff;
             if (CK == 32) {
                 tap = st;
@@ -148,6 +162,13 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
             const int x = x0 + m % 8, y = y0 + m / 8;
             if (x >= a.nx || y >= a.ny || z >= a.nz) continue;
             const size_t v = ((size_t)z * a.ny + y) * a.nx + x;
+            if (nks > 1) {
+                const size_t V = (size_t)a.nx * a.ny * a.nz;
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    a.part[((size_t)ks * V + v) * a.cout + cbase + nb * 16 + r16] = acc[mb][nb][r];
+                continue;
+            }
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb) {
                 const int co = cbase + nb * 16 + r16;
@@ -161,6 +182,24 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
                 }
             }
         }
+}
+
+// Sum of a split convolution's partials in slice order, then the epilogue of k_conv3.
+__global__ __launch_bounds__(256) void k_conv_splitk_reduce(ConvArgs a) {
+    const size_t V = (size_t)a.nx * a.ny * a.nz, n = V * a.cout;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const int co = (int)(e % (unsigned)a.cout);
+        const size_t v = e / (unsigned)a.cout;
+        float val = a.part[e];
+        for (int s = 1; s < a.ksplit; ++s) val += a.part[(size_t)s * n + e];
+        if (a.bias) val += a.bias[co];
+        if (a.relu) val = val > 0.f ? val : 0.f;
+        if (a.out) a.out[e] = (bf16)val;
+        if (a.out32) {
+            if (!a.out32_ch0) a.out32[e] = val;
+            else if (co == 0) a.out32[v] = val;
+        }
+    }
 }
 
 // Weight gradient dW[co][tap*cin + ci] += sum_v g[v][co] * x[v + tap][ci]:
