@@ -90,7 +90,7 @@ def test_advect_fp32_mode_close_to_oracle(cuda, n):
     from vflip_amd import api
     import torch
     print(f"advect fp32 n={n}", flush=True)
-    h = np.float32(1.0 / n)  # n = 33: h not a power of two
+    h = np.float32(1.0 / n)  # n = 24, 33: general first RK2 stage; n = 64 (power-of-two h): lattice form
     dt = np.float32(0.6 / n)
     u, v, w = scenes.random_mac(n, 23, scale=1.0)
     rng = np.random.default_rng(n + 1)

@@ -137,9 +137,13 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     int j = (int)(t % nj), k = (int)(t / nj);
     auto wet = [&](size_t q) { return !lphi || lphi[q] < 0.f; };
     if (!(i >= 1 && i < ni - 1 && j >= 1 && j < nj - 1 && k >= 1 && k < nk - 1) || !wet(c)) {
-        if (changed && (D[c] != 0.f || AX[c] != 0.f || AY[c] != 0.f || AZ[c] != 0.f || LK[c] != 0.f))
-            *changed = 1;  // every writer stores the same value
-        D[c] = 0.f; AX[c] = 0.f; AY[c] = 0.f; AZ[c] = 0.f; LK[c] = 0.f; rhs[c] = 0.f;
+        // with change detection the coefficients are stored only where they differ
+        // (an unchanged operator costs no coefficient writes)
+        const bool diff = __float_as_uint(D[c]) | __float_as_uint(AX[c]) | __float_as_uint(AY[c]) |
+                          __float_as_uint(AZ[c]) | __float_as_uint(LK[c]);
+        if (changed && diff) *changed = 1;  // every writer stores the same value
+        if (!changed || diff) { D[c] = 0.f; AX[c] = 0.f; AY[c] = 0.f; AZ[c] = 0.f; LK[c] = 0.f; }
+        rhs[c] = 0.f;
         if (D64) { D64[c] = 0.0; R64[c] = 0.0; }
         return;
     }
@@ -182,19 +186,22 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     const float nd = (float)d, nx = oxp ? -txp : 0.f, ny = oyp ? -typ : 0.f;
     const float nz = ozp ? -tzp : 0.f, nl = (float)lk;
     // flag an operator change against the previous assembly (bitwise compare)
-    if (changed && (__float_as_uint(D[c]) != __float_as_uint(nd) || __float_as_uint(AX[c]) != __float_as_uint(nx) ||
-                    __float_as_uint(AY[c]) != __float_as_uint(ny) || __float_as_uint(AZ[c]) != __float_as_uint(nz) ||
-                    __float_as_uint(LK[c]) != __float_as_uint(nl)))
-        *changed = 1;  // every writer stores the same value
+    const bool diff = !changed ||
+                      (__float_as_uint(D[c]) != __float_as_uint(nd) || __float_as_uint(AX[c]) != __float_as_uint(nx) ||
+                       __float_as_uint(AY[c]) != __float_as_uint(ny) || __float_as_uint(AZ[c]) != __float_as_uint(nz) ||
+                       __float_as_uint(LK[c]) != __float_as_uint(nl));
+    if (changed && diff) *changed = 1;  // every writer stores the same value
     if (D64) {
         D64[c] = d;  // the reference's FP64 diagonal and right-hand side
         R64[c] = r;
     }
-    D[c] = nd;
-    AX[c] = nx;
-    AY[c] = ny;
-    AZ[c] = nz;
-    LK[c] = nl;
+    if (diff) {  // bit-equal coefficients are already stored
+        D[c] = nd;
+        AX[c] = nx;
+        AY[c] = ny;
+        AZ[c] = nz;
+        LK[c] = nl;
+    }
     rhs[c] = (float)r;
 }
 
