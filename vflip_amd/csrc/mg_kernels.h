@@ -1133,28 +1133,41 @@ __global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __
 // the right-hand side come in FP64 when the assembly provided them (d64, b64),
 // else from the FP32 arrays. x is the FP64 iterate. Writes r rounded to FP32
 // and partial max|r|.
+// r = b - A x in FP64 (FP64 iterate, FP64 diagonal/rhs when given), partial
+// max|r|. A thread marches kTrueResKZ planes up z with x and the z coupling of
+// the plane below kept in registers; the terms are summed in the order
+// d x, -z, -y, -x, +x, +y, +z.
+constexpr int kTrueResKZ = 16;
 static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op, const double* __restrict__ d64,
                                                             const double* __restrict__ b64,
                                                             const float* __restrict__ b32,
                                                             const double* __restrict__ x, float* __restrict__ r,
                                                             double* __restrict__ partial) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-    const int kb = blockIdx.z * kMatvecKZ;
+    const int kb = blockIdx.z * kTrueResKZ;
     const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
     double m = 0;
-    if (i < ni && j < nj) {
-        for (int k = kb; k < min(kb + kMatvecKZ, nk); ++k) {
-            const int c = (k * nj + j) * ni + i;
-            double y = (d64 ? d64[c] : (double)op.D(c)) * x[c];
-            if (k > 0) y += (double)op.AZ(c - sl) * x[c - sl];
+    if (i < ni && j < nj && kb < nk) {
+        const int ke = min(kb + kTrueResKZ, nk);
+        int c = (kb * nj + j) * ni + i;
+        double xm = kb > 0 ? x[c - sl] : 0.0, x0 = x[c];
+        float azm = kb > 0 ? op.AZ(c - sl) : 0.f;
+        for (int k = kb; k < ke; ++k, c += sl) {
+            const double xp = k + 1 < nk ? x[c + sl] : 0.0;
+            const float az0 = k + 1 < nk ? op.AZ(c) : 0.f;
+            double y = (d64 ? d64[c] : (double)op.D(c)) * x0;
+            if (k > 0) y += (double)azm * xm;
             if (j > 0) y += (double)op.AY(c - ni) * x[c - ni];
             if (i > 0) y += (double)op.AX(c - 1) * x[c - 1];
             if (i + 1 < ni) y += (double)op.AX(c) * x[c + 1];
             if (j + 1 < nj) y += (double)op.AY(c) * x[c + ni];
-            if (k + 1 < nk) y += (double)op.AZ(c) * x[c + sl];
+            if (k + 1 < nk) y += (double)az0 * xp;
             const double rv = (b64 ? b64[c] : (double)b32[c]) - y;
             if (r) r[c] = (float)rv;
             m = fmax(m, fabs(rv));
+            xm = x0;
+            x0 = xp;
+            azm = az0;
         }
     }
     double t = block_max(m);

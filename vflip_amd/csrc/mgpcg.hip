@@ -817,7 +817,7 @@ int ensure_f64(vflip_solver* S) {
 
 void launch_true_residual64(vflip_solver* S, const double* b64, const float* b32, const double* x, float* r) {
     const LevelBuf& l0 = S->L[0];
-    dim3 g = grid3z(l0.ni, l0.nj, l0.nk);
+    const dim3 g((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
     k_true_residual64<<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr, b64, b32,
                                                          x, r, S->partial);
     S->np_last = nblocks(g);
