@@ -394,7 +394,8 @@ int vflip_seed_ghost(vflip_particles* P, const vflip_coarse_view* C, const vflip
  * subdomain, du = sum_j (v_j - v_i) K_ij / (1e-9 + sum_j K_ij) over the particles
  * that vflip_hash (subdomain workspace, clamp 1, bmin) bins in the 3x3x3 cells
  * around it, in the reference's order; tau = dt nu + 1e-10. out_vel[3*np_real]
- * = vel + du (not aliasing vel). Asynchronous. The kernel's exponential is the
+ * = vel + du (not aliasing vel); every real particle must be in the bins, as
+ * the clamped binning puts every particle. Asynchronous. The kernel's exponential is the
  * device's double exp: within one float rounding of the host's per term. */
 int vflip_pse_diffuse(const vflip_coarse_view* C, const vflip_subdomain_view* S, float dt, float nu, int np_real,
                       const float* pos, const float* vel, const int* cell_start, const int* order, float* out_vel,

@@ -185,15 +185,23 @@ __device__ inline float pse_kernel(const float* xi, const float* xj, float tau, 
     return (float)((double)fr * exp(-d * 0.25 / tau));
 }
 
-// Thread per real particle: du = sum (v_j - v_i) K / (1e-9 + sum K) over the
-// particles binned in the 3x3x3 cells around it (z, y, x ascending, list order).
-__global__ __launch_bounds__(kBlock) void k_pse_diffuse(int np_real, const float* __restrict__ pos,
-                                                        const float* __restrict__ vel, const int* __restrict__ start,
-                                                        const int* __restrict__ order, vflip_coarse_view C,
-                                                        vflip_subdomain_view S, float tau, float fr,
-                                                        float* __restrict__ out_vel) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= np_real) return;
+// du = sum (v_j - v_i) K / (1e-9 + sum K) over the particles binned in the
+// 3x3x3 cells around particle i (z, y, x ascending, list order). The threads
+// walk the bins: thread (cell c, slot s) takes the particles c's list holds at
+// s, s + slots, ... and diffuses the real ones among them (clamped binning
+// lists every particle once). A wave then covers 64 / slots neighbouring cells
+// whose neighbour lists overlap, so the gathers of neighbour positions and
+// velocities are shared through the caches instead of being fetched once per
+// particle in arbitrary order. That pays where the bins are dense (at least
+// kPseDense real particles per cell); sparse bins, like a halo copy of a
+// coarse fill already in cell order, keep one thread per real particle
+// (k_pse_diffuse_particles), which leaves no lane idle.
+constexpr int kPseSlots = 8;
+constexpr int kPseDense = 4;
+__device__ inline void pse_one(int i, const float* __restrict__ pos, const float* __restrict__ vel,
+                               const int* __restrict__ start, const int* __restrict__ order,
+                               const vflip_coarse_view& C, const vflip_subdomain_view& S, float tau, float fr,
+                               float* __restrict__ out_vel) {
     const float xi[3] = {pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]};
     const float ui[3] = {vel[3 * (size_t)i], vel[3 * (size_t)i + 1], vel[3 * (size_t)i + 2]};
     int ijk[3];
@@ -223,6 +231,32 @@ __global__ __launch_bounds__(kBlock) void k_pse_diffuse(int np_real, const float
     }
 #pragma unroll
     for (int a = 0; a < 3; ++a) out_vel[3 * (size_t)i + a] = ui[a] + du[a];
+}
+
+__global__ __launch_bounds__(kBlock) void k_pse_diffuse_bins(int np_real, int ncell, const float* __restrict__ pos,
+                                                             const float* __restrict__ vel,
+                                                             const int* __restrict__ start,
+                                                             const int* __restrict__ order, vflip_coarse_view C,
+                                                             vflip_subdomain_view S, float tau, float fr,
+                                                             float* __restrict__ out_vel) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int c = t / kPseSlots, slot = t % kPseSlots;
+    if (c >= ncell) return;
+    const int e = start[c + 1];
+    for (int q = start[c] + slot; q < e; q += kPseSlots) {
+        const int i = order[q];
+        if (i < np_real) pse_one(i, pos, vel, start, order, C, S, tau, fr, out_vel);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pse_diffuse_particles(int np_real, const float* __restrict__ pos,
+                                                                  const float* __restrict__ vel,
+                                                                  const int* __restrict__ start,
+                                                                  const int* __restrict__ order, vflip_coarse_view C,
+                                                                  vflip_subdomain_view S, float tau, float fr,
+                                                                  float* __restrict__ out_vel) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < np_real) pse_one(i, pos, vel, start, order, C, S, tau, fr, out_vel);
 }
 
 static bool view_ok(const vflip_coarse_view* C) {
@@ -318,9 +352,15 @@ int vflip_pse_diffuse(const vflip_coarse_view* C, const vflip_subdomain_view* S,
     const float tau = (float)(dt * nu + 1e-10);
     const float frac = (float)std::sqrt(4.0 * 3.1415926 * tau);
     const float fr = 1.0f / frac * frac * frac;
-    if (np_real > 0)
-        k_pse_diffuse<<<grid_for(np_real), kBlock, 0, (hipStream_t)hip_stream>>>(np_real, pos, vel, cell_start, order,
-                                                                                   *C, *S, tau, fr, out_vel);
+    const size_t ncell = (size_t)S->ni * S->nj * S->nk;
+    VF_ARG(ncell * kPseSlots < (1u << 31), "subdomain cell count");
+    hipStream_t st = (hipStream_t)hip_stream;
+    if (np_real > 0 && (size_t)np_real >= kPseDense * ncell)
+        k_pse_diffuse_bins<<<grid_for(ncell * kPseSlots), kBlock, 0, st>>>(np_real, (int)ncell, pos, vel, cell_start,
+                                                                           order, *C, *S, tau, fr, out_vel);
+    else if (np_real > 0)
+        k_pse_diffuse_particles<<<grid_for(np_real), kBlock, 0, st>>>(np_real, pos, vel, cell_start, order, *C, *S,
+                                                                       tau, fr, out_vel);
     VF_CHECK_LAUNCH();
     return 0;
 }
