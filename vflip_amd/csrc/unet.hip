@@ -14,6 +14,7 @@
 // There is no counterpart in the reference; the interface mirrors the solver's
 // warm_start option (the PCG takes the prediction as its initial guess).
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -625,8 +626,10 @@ int vflip_unet_warm_start(vflip_unet* N, int ni, int nj, int nk, int factor, con
     if (int rc = ensure_warm_grid(N, cx, cy, cz)) return rc;
     if (int rc = ensure_grid(N, cx, cy, cz)) return rc;
     const size_t Vc = (size_t)cx * cy * cz;
-    k_restrict_inputs<<<dim3((cx + 255) / 256, cy, cz), 256, 0, N->stream>>>(rhs, diag, ni, nj, nk, factor, N->w_rhs,
-                                                                             N->w_solid, N->w_liquid);
+    auto a16 = [](const void* p) { return ((uintptr_t)p & 15) == 0; };
+    const int vec4 = factor == 4 && ni % 4 == 0 && a16(rhs) && (!diag || a16(diag));
+    k_restrict_inputs<<<dim3((cx + 255) / 256, cy, cz), 256, 0, N->stream>>>(rhs, diag, ni, nj, nk, factor, vec4,
+                                                                             N->w_rhs, N->w_solid, N->w_liquid);
     const float* sd = prepare_scale(N, N->w_rhs, Vc, 0.f);
     k_pack_input<<<grid256(Vc), 256, 0, N->stream>>>(N->w_rhs, N->w_solid, N->w_liquid, Vc, 0.f, sd, N->act[kIn]);
     forward_layers(N);

@@ -531,9 +531,11 @@ __global__ __launch_bounds__(256) void k_scale_copy(const float* __restrict__ y,
 // f^3 block; solid = fraction of cells outside the system (diagonal 0, the
 // solid and Dirichlet cells); liquid = 1 - solid.
 // 3D launch: x over coarse i (256 a block), y over coarse j, z over coarse k.
+// vec4: f == 4, nx % 4 == 0 and rhs/diag 16-byte aligned (checked by the caller).
 __global__ __launch_bounds__(256) void k_restrict_inputs(const float* __restrict__ rhs, const float* __restrict__ diag,
-                                                         int nx, int ny, int nz, int f, float* __restrict__ crhs,
-                                                         float* __restrict__ csolid, float* __restrict__ cliquid) {
+                                                         int nx, int ny, int nz, int f, int vec4,
+                                                         float* __restrict__ crhs, float* __restrict__ csolid,
+                                                         float* __restrict__ cliquid) {
     const int cx = nx / f, cy = ny / f;
     const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
     if (i >= cx) return;
@@ -542,6 +544,17 @@ __global__ __launch_bounds__(256) void k_restrict_inputs(const float* __restrict
     for (int dk = 0; dk < f; ++dk)
         for (int dj = 0; dj < f; ++dj) {
             const size_t row = ((size_t)(k * f + dk) * ny + (j * f + dj)) * nx + (size_t)i * f;
+            if (vec4) {
+                // the four fine values of a lane are one aligned float4 (same sum order)
+                const float4 r4 = *reinterpret_cast<const float4*>(rhs + row);
+                sr += r4.x; sr += r4.y; sr += r4.z; sr += r4.w;
+                if (diag) {
+                    const float4 d4 = *reinterpret_cast<const float4*>(diag + row);
+                    ss += d4.x == 0.f ? 1.f : 0.f; ss += d4.y == 0.f ? 1.f : 0.f;
+                    ss += d4.z == 0.f ? 1.f : 0.f; ss += d4.w == 0.f ? 1.f : 0.f;
+                }
+                continue;
+            }
             for (int di = 0; di < f; ++di) {
                 sr += rhs[row + di];
                 ss += (diag && diag[row + di] == 0.f) ? 1.f : 0.f;
