@@ -719,7 +719,19 @@ __device__ inline void coarse_cell(const Op<CT>& op, const float* __restrict__ b
 // tab != nullptr: tabulation mode. Workgroup u solves with b = e_u and writes
 // the result to row u of tab (tab[u*n + r] = (RBGS^sweeps e_u)[r]); the solve
 // from a zero guess is linear in b, so y = tab^T b reproduces it (k_coarse_apply).
-template <typename CT>
+// XREG (the host sets it when a row's pairs, (ni + 1) / 2 of them, divide 16):
+// a row's pairs then sit on consecutive lanes of one 16-lane DPP row, so both
+// x-neighbours of a cell come from registers: one from the thread's own other
+// cell, one from the next or previous lane through a DPP row shift. Only the y
+// and z neighbours are read from LDS; the terms and their order are unchanged.
+__device__ inline float dpp_from_prev(float v) {  // lane l - 1 of the 16-lane row (0 at its start)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+__device__ inline float dpp_from_next(float v) {  // lane l + 1 of the 16-lane row (0 at its end)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x101, 0xf, 0xf, false));
+}
+
+template <typename CT, bool XREG = false>
 __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const float* __restrict__ b,
                                                              float* __restrict__ x, int sweeps,
                                                              float* __restrict__ tab) {
@@ -731,12 +743,16 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
     // [q][colour]: colour 1 = odd (i+j+k), updated first; cd holds 1/diag after set-up
     float cd[kCoarsePairs][2], cb[kCoarsePairs][2], ca[kCoarsePairs][2][6];
     int nbr[kCoarsePairs][2][6], cell[kCoarsePairs][2], slot[kCoarsePairs][2];
+    float xv[kCoarsePairs][2];  // current x of the thread's cells (XREG)
+    int ev[kCoarsePairs];       // which of the pair's cells has the even i (XREG)
 #pragma unroll
     for (int q = 0; q < kCoarsePairs; ++q) {
 #pragma unroll
         for (int col = 0; col < 2; ++col) {
             cell[q][col] = -1;
             slot[q][col] = kCoarseMax;
+            xv[q][col] = 0.f;
+            ev[q] = 0;
             cd[q][col] = 0.f;
             cb[q][col] = 0.f;
 #pragma unroll
@@ -751,6 +767,7 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
             const int j = row % nj, k = row / nj;
             // cell 2m+e is odd when e == 1 - ((j+k)&1)
             const int eodd = ((j + k) & 1) ? 0 : 1;
+            ev[q] = eodd ? 0 : 1;
             coarse_cell(op, b, unit, 2 * m + eodd, j, k, ppr, npairs, cell[q][1], slot[q][1], cd[q][1], cb[q][1],
                         ca[q][1], nbr[q][1]);
             coarse_cell(op, b, unit, 2 * m + 1 - eodd, j, k, ppr, npairs, cell[q][0], slot[q][0], cd[q][0],
@@ -771,13 +788,28 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
 #pragma unroll
             for (int q = 0; q < kCoarsePairs; ++q) {
                 float s = 0.f;
+                if constexpr (XREG) {
+                    const float xe = ev[q] ? xv[q][1] : xv[q][0], xo = ev[q] ? xv[q][0] : xv[q][1];
+                    const float xo_prev = dpp_from_prev(xo), xe_next = dpp_from_next(xe);
+                    const bool even = col == ev[q];
+                    s += ca[q][col][0] * xs[nbr[q][col][0]];
+                    s += ca[q][col][1] * xs[nbr[q][col][1]];
+                    s += ca[q][col][2] * (even ? xo_prev : xe);  // -x
+                    s += ca[q][col][3] * (even ? xo : xe_next);  // +x
+                    s += ca[q][col][4] * xs[nbr[q][col][4]];
+                    s += ca[q][col][5] * xs[nbr[q][col][5]];
+                } else {
 #pragma unroll
-                for (int e = 0; e < 6; ++e) s += ca[q][col][e] * xs[nbr[q][col][e]];
+                    for (int e = 0; e < 6; ++e) s += ca[q][col][e] * xs[nbr[q][col][e]];
+                }
                 nv[q] = (cb[q][col] - s) * cd[q][col];
             }
 #pragma unroll
             for (int q = 0; q < kCoarsePairs; ++q)
-                if (cell[q][col] >= 0) xs[slot[q][col]] = nv[q];
+                if (cell[q][col] >= 0) {
+                    xs[slot[q][col]] = nv[q];
+                    if constexpr (XREG) xv[q][col] = nv[q];
+                }
             __syncthreads();
         }
     }

@@ -240,6 +240,13 @@ int run_vcycle(vflip_solver* S, bool want_dot, bool* dot_done) {
             k_coarse_apply_partial<<<dim3((nc + kBlock - 1) / kBlock, nch), kBlock, 0, S->stream>>>(S->ctab, b,
                                                                                                   S->cpart, nc);
             k_coarse_apply_reduce<<<(nc + kBlock - 1) / kBlock, kBlock, 0, S->stream>>>(S->cpart, x, nc, nch);
+        } else if (16 % ppr == 0) {  // x-neighbours through DPP (see k_coarsest)
+            if (lv.half)
+                k_coarsest<__half, true><<<1, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), b, x,
+                                                                              S->opts.coarse_sweeps, nullptr);
+            else
+                k_coarsest<float, true><<<1, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), b, x,
+                                                                             S->opts.coarse_sweeps, nullptr);
         } else if (lv.half) {
             k_coarsest<__half><<<1, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), b, x, S->opts.coarse_sweeps,
                                                                     nullptr);
@@ -268,9 +275,16 @@ int build_ctab(vflip_solver* S) {
         if (!rc) rc = alloc((void**)&S->cpart, (size_t)nc * ((nc + kCoarseChunk - 1) / kCoarseChunk) * sizeof(float));
         if (rc) return rc;
     }
-    if (lv.half)
+    const bool xreg = 16 % ((lv.ni + 1) / 2) == 0;  // x-neighbours through DPP (see k_coarsest)
+    if (lv.half && xreg)
+        k_coarsest<__half, true><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), nullptr, nullptr,
+                                                                       S->opts.coarse_sweeps, S->ctab);
+    else if (lv.half)
         k_coarsest<__half><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), nullptr, nullptr,
                                                                  S->opts.coarse_sweeps, S->ctab);
+    else if (xreg)
+        k_coarsest<float, true><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), nullptr, nullptr,
+                                                                      S->opts.coarse_sweeps, S->ctab);
     else
         k_coarsest<float><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), nullptr, nullptr,
                                                                 S->opts.coarse_sweeps, S->ctab);
