@@ -350,6 +350,15 @@ def row_baselines():
                                           ro))
     add("diffuse (PSE viscosity)", f"{len(real) / 1e6:.2f}M subdomain particles + {len(gpos) / 1e6:.2f}M ghosts",
         gd, cd, cores, "within 1e-6 (device exp)")
+    # smoke tracers: RK3 through the blended coarse/subdomain velocity, push-out, decay
+    rng = np.random.default_rng(9)
+    ntr = 1 << 20
+    trs = np.concatenate([rng.uniform(0.05, 0.95, (ntr, 3)), rng.uniform(0.5, 1.0, (ntr, 1))], 1).astype(np.float32)
+    ttr = t(trs)
+    gt = _gpu_ms(lambda: api.advect_tracers(C, Sd, 0.01, 0.2, ttr))
+    ct, _ = _cpu_ms(lambda: O.advect_tracers(coarse, sub, np.float32(0.01), np.float32(0.2), trs))
+    add("advect_tracers (with subdomain)", f"{ntr / 1e6:.2f}M tracers, {n}^3 coarse + {m}^3 subdomain", gt, ct, cores,
+        "bit-exact", 32 * ntr, "16 B read + 16 B written per tracer (FP64 nested samples: compute-bound)")
     return rows
 
 
