@@ -1207,6 +1207,36 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
         partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
 }
 
+// Energy of an FP32 initial guess before its FP64 residual: partial sums of
+// x.(2b - A x), A x from the FP32 level-0 operator accumulated in FP64, so that
+// E = -sum / 2 as from k_energy_dots (x.(b + r) with r = b - A x). Launched as
+// k_true_residual64.
+static __global__ __launch_bounds__(kBlock) void k_energy_f32(Op<float> op, const double* __restrict__ b64,
+                                                       const float* __restrict__ b32, const float* __restrict__ x,
+                                                       double* __restrict__ partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int kb = blockIdx.z * kTrueResKZ;
+    const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
+    double acc = 0;
+    if (i < ni && j < nj && kb < nk) {
+        for (int k = kb; k < min(kb + kTrueResKZ, nk); ++k) {
+            const int c = (k * nj + j) * ni + i;
+            const double xc = x[c];
+            double y = (double)op.D(c) * xc;
+            if (k > 0) y += (double)op.AZ(c - sl) * x[c - sl];
+            if (j > 0) y += (double)op.AY(c - ni) * x[c - ni];
+            if (i > 0) y += (double)op.AX(c - 1) * x[c - 1];
+            if (i + 1 < ni) y += (double)op.AX(c) * x[c + 1];
+            if (j + 1 < nj) y += (double)op.AY(c) * x[c + ni];
+            if (k + 1 < nk) y += (double)op.AZ(c) * x[c + sl];
+            acc += xc * (2.0 * (b64 ? b64[c] : (double)b32[c]) - y);
+        }
+    }
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+}
+
 // x64 += alpha s; r -= alpha q; partial max|r| (mixed mode, FP64 iterate)
 static __global__ __launch_bounds__(kBlock) void k_update_x64r(unsigned n, const Scalars* __restrict__ sc,
                                                         double* __restrict__ x, const float* __restrict__ s,

@@ -874,7 +874,23 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
     }
     VF_CHECK_LAUNCH();
     if ((rc = finalize(S, 4, S->opts.tolerance))) return rc;
-    if (S->opts.warm_start) {
+    // A mixed solve seeded with an FP32 guess vets it first: its energy from one FP32
+    // stencil pass (k_energy_f32); only a guess that passes pays for the FP64 cast
+    // and the FP64 true residual below.
+    bool prevetted = false;
+    if (S->opts.warm_start && mx && !x64out && x32) {
+        const LevelBuf& l0 = S->L[0];
+        const dim3 g((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
+        k_energy_f32<<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), rhs64, rhs32, x32, S->partial);
+        S->np_last = nblocks(g);
+        VF_CHECK_LAUNCH();
+        if ((rc = finalize(S, 6))) return rc;
+        if ((rc = fetch_scalars(S))) return rc;
+        prevetted = true;
+    }
+    if (S->opts.warm_start && prevetted && !(S->h_sc->energy < 0.0)) {
+        // rejected: the zero start below (the same as after the FP64 vetting)
+    } else if (S->opts.warm_start) {
         // initial guess from x32 (or x64out when given)
         if (x64out) VF_HIP(hipMemcpyAsync(S->x64, x64out, n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
         else k_cast_f32_f64<<<gn, kBlock, 0, S->stream>>>(n, x32, S->x64);
