@@ -82,7 +82,8 @@ typedef struct {
                              A-norm (E(x0) = x0'Ax0/2 - b'x0 >= 0) and the solve restarted from 0 */
 } vflip_solve_stats;
 
-/* Reference defaults: tol 1e-6, 50 iterations, FP32, 4/4/200 sweeps, refine 8. */
+/* Reference defaults: tol 1e-6, 50 iterations, FP32, 4/4/200 sweeps; mixed-mode
+ * defaults refine_every 4, half_from_level 0. */
 void vflip_default_opts(vflip_solve_opts* opts);
 
 /* Allocates the multigrid hierarchy (levels ceil-halved while > 16^3 unknowns,
@@ -147,10 +148,13 @@ int vflip_solver_get_level(const vflip_solver* s, int level, float* diag, float*
 /* Drop-in for ivock::AMGPCGSolve<double> (S/AlgebraicMultigrid.h:189-197), HOST
  * arrays: the 7-point SPD matrix as diag + couplings (the reference SparseMatrix
  * rows restricted to a 7-point stencil), rhs and result of ni*nj*nk doubles.
- * Solves on the GPU in FP32 (opts->precision selects the mixed mode) with the
- * reference stop test; result, residual_out and iterations_out as the reference.
- * Returns 0 converged, 1 not converged, <0 error. opts may be NULL (defaults
- * with tolerance_factor/max_iterations applied). */
+ * Solves on the GPU with the reference stop test; result, residual_out and
+ * iterations_out as the reference. opts may be NULL: the mixed-precision solver
+ * (VFLIP_PREC_MIXED; FP64 iterate, dots and true-residual stop test, so
+ * residual_out is the FP64 |b - A x|_inf) with the default sweeps;
+ * tolerance_factor and max_iterations always apply. opts->precision =
+ * VFLIP_PREC_FP32 selects plain FP32. Returns 0 converged, 1 not converged,
+ * <0 error. */
 int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const double* ax,
                             const double* ay, const double* az, const double* rhs, double* result,
                             double tolerance_factor, int max_iterations, double* residual_out,
@@ -161,6 +165,25 @@ int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const do
  * bit-exact with S/Smoke_solver3D.cpp:874-944 / S/levelset_util.cpp:23-103. */
 int vflip_face_weights(int ni, int nj, int nk, const float* phi, float* uw, float* vw, float* ww,
                        void* hip_stream);
+
+/* Host callbacks of the boundary set-up: pos points to 3 floats (x, y, z) in
+ * world coordinates, user is passed through. They are called from the calling
+ * thread, once per grid point. */
+typedef float (*vflip_phi_fn)(const float* pos, void* user);
+typedef void (*vflip_vel_fn)(const float* pos, float* vel_out, void* user);
+
+/* set_boundary_phi — S/Smoke_solver3D.cpp:851-873: nodal_phi (device,
+ * (ni+1)(nj+1)(nk+1)) receives phi(pos) at every node, pos = Vec3f(i,j,k)*h -
+ * Vec3f(.5,.5,.5)*h in float. Synchronous. */
+int vflip_set_boundary_phi(int ni, int nj, int nk, float h, vflip_phi_fn phi, void* user, float* nodal_phi,
+                           void* hip_stream);
+
+/* set_boundary_vel — S/Smoke_solver3D.cpp:793-848: us/vs/ws (device MAC shapes)
+ * receive component 0/1/2 of vel(pos) at their faces, pos = Vec3f(i,j,k)*h -
+ * Vec3f(ox,oy,oz)*h with the face grids' offsets (.5,0,0), (0,.5,0), (0,0,.5).
+ * Synchronous. */
+int vflip_set_boundary_vel(int ni, int nj, int nk, float h, vflip_vel_fn vel, void* user, float* us, float* vs,
+                           float* ws, void* hip_stream);
 
 /* Inflow boundary: faces of cells with bdesc==1 get u=u_in, v=w=0
  * (S/Smoke_solver3D.cpp:964-980). bdesc: ni*nj*nk bytes. */

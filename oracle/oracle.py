@@ -99,6 +99,9 @@ def lib():
         L.or_advect_particles.argtypes = ([c_int, f32p, f32p, f32p, f32p, f32p, c_int, c_int, c_int, c_float,
                                           c_float, c_int, c_float, c_float, c_float])
         L.or_get_delta.argtypes = [c_int, f32p, f32p, f32p]
+        L.or_inflow_bc.argtypes = [c_int, c_int, c_int, np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS"),
+                                   c_float, f32p, f32p, f32p]
+        L.or_grid_points.argtypes = [c_int, c_int, c_int, c_float, c_float, c_float, c_float, f32p]
         _LIB = L
     return _LIB
 
@@ -123,6 +126,7 @@ def ref_lib():
         R.ref_blocked_index.argtypes = [c_int] * 6
         R.ref_normalize.argtypes = [c_int, f32p, f32p]
         R.ref_cell_index.argtypes = [c_int, f32p, c_float, c_int, c_float, c_float, c_float, i32p]
+        R.ref_grid_points.argtypes = [c_int, c_int, c_int, c_float, c_double, c_double, c_double, f32p]
         _REF = R
     return _REF
 
@@ -360,6 +364,23 @@ def grad_apply(h, p, uw, vw, ww, us, vs, ws, u, v, w, dt=1.0, sub=False):
     lib().or_grad_apply(ni, nj, nk, h, dt, int(sub), c64(p), c32(uw), c32(vw), c32(ww), c32(us), c32(vs),
                         c32(ws), u, v, w)
     return u, v, w
+
+
+def inflow_bc(bdesc, u_in, u, v, w):
+    """S/Smoke_solver3D.cpp:964-980 on copies of u, v, w; bdesc int8 (nk, nj, ni)."""
+    nk, nj, ni = bdesc.shape
+    u, v, w = c32(u).copy(), c32(v).copy(), c32(w).copy()
+    lib().or_inflow_bc(ni, nj, nk, np.ascontiguousarray(bdesc, np.int8), u_in, u, v, w)
+    return u, v, w
+
+
+def grid_points(shape, h, offs):
+    """World positions (float32, shape + (3,)) where set_boundary_phi/vel sample their
+    callbacks on a grid shaped (nz, ny, nx) with grid offsets offs = (ox, oy, oz)."""
+    nz, ny, nx = shape
+    out = np.zeros((nz, ny, nx, 3), np.float32)
+    lib().or_grid_points(nx, ny, nz, h, offs[0], offs[1], offs[2], out)
+    return out
 
 
 def advect_rk2(h, dt, u, v, w, field, offs, mode=0):

@@ -6,7 +6,8 @@
 //   fraction_inside(2), fraction_inside(4)  S/levelset_util.cpp:4-103
 //   Buffer3D<float>::sample_linear/_cubic/_cube_lerp/_grad   S/fluid_buffer3D.h:86-376
 //   randhash                                S/util.h:238-246
-//   normalize, Vec3f/Vec3i arithmetic       S/vec.h (hash_FLIP cell index)
+//   normalize, Vec3f/Vec3i arithmetic       S/vec.h (hash_FLIP cell index,
+//                                           set_boundary_* sample positions)
 // The standard headers below are included first because the reference headers
 // rely on the MSVC toolchain pulling them in implicitly.
 #include <climits>
@@ -69,6 +70,19 @@ void ref_cell_index(int n, const float* pos, float h, int sub, float bx, float b
         ivock::Vec3i c(xx / h);
         ijk[3 * p] = c[0]; ijk[3 * p + 1] = c[1]; ijk[3 * p + 2] = c[2];
     }
+}
+
+// The sample positions of set_boundary_phi / set_boundary_vel evaluated with the
+// reference's Vec3f arithmetic: Vec3f(i,j,k)*h - Vec3f(ox,oy,oz)*h, where the
+// offsets are the Buffer3D's double _ox/_oy/_oz (S/Smoke_solver3D.cpp:806,864).
+void ref_grid_points(int nx, int ny, int nz, float h, double ox, double oy, double oz, float* out) {
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                ivock::Vec3f pos = ivock::Vec3f(i, j, k) * h - ivock::Vec3f(ox, oy, oz) * h;
+                float* q = out + 3 * (i + (size_t)nx * (j + (size_t)ny * k));
+                q[0] = pos[0]; q[1] = pos[1]; q[2] = pos[2];
+            }
 }
 
 // Blocked storage index of the reference Buffer3D for cell (i,j,k).

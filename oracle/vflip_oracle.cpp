@@ -19,7 +19,14 @@
 //   coarsen ............................. scipy 0.5*R*A*P with the reference's R/P
 //   matvec/rbgs/vcycle/amgpcg ........... scipy spsolve, RB-GS colour-order brute
 //                                         force, PCG invariants
-//   grad_apply .......................... discrete divergence after projection
+//   grad_apply .......................... discrete divergence after projection, coarse
+//                                         and subdomain ranges (all interior cells in
+//                                         the sub form; faces outside the ranges untouched)
+//   assemble, solid-velocity term ....... closed form u = u_solid affine: rhs = -div u_solid
+//                                         at every cell with diag > 0, any face weights
+//   inflow_bc ........................... closed form (explicit face lists)
+//   grid_points ......................... oracle/_ref (the reference's Vec3f arithmetic)
+//                                         + closed form (i - o) h
 //   hash (binning) ...................... brute-force membership
 //   p2g ................................. brute force over all particles; constant
 //                                         velocity closed form
@@ -562,6 +569,44 @@ int or_amgpcg(int ni, int nj, int nk, const double* d, const double* ax, const d
     *iterations_out = it;
     std::copy(x.begin(), x.end(), result);
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Inflow boundary written before the assembly, S/Smoke_solver3D.cpp:964-980:
+// every cell with bdesc == 1 sets its two x faces to u_in and its y and z faces
+// to 0. Faces shared with a neighbour get the same value from either cell.
+// ---------------------------------------------------------------------------
+void or_inflow_bc(int ni, int nj, int nk, const signed char* bdesc, float u_in, float* u, float* v, float* w) {
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                if (bdesc[lin(i, j, k, ni, nj)] != 1) continue;
+                u[lin(i, j, k, ni + 1, nj)] = u_in;
+                u[lin(i + 1, j, k, ni + 1, nj)] = u_in;
+                v[lin(i, j, k, ni, nj + 1)] = 0;
+                v[lin(i, j + 1, k, ni, nj + 1)] = 0;
+                w[lin(i, j, k, ni, nj)] = 0;
+                w[lin(i, j, k + 1, ni, nj)] = 0;
+            }
+}
+
+// ---------------------------------------------------------------------------
+// World positions at which set_boundary_phi / set_boundary_vel sample their
+// callbacks: Vec3f(i,j,k)*h - Vec3f(ox,oy,oz)*h in float, per component
+// (S/Smoke_solver3D.cpp:806,822,838 with the face grids' offsets (0.5,0,0),
+// (0,0.5,0), (0,0,0.5); 864 with (0.5,0.5,0.5) for the nodal SDF).
+// out: 3 floats per point of the nx x ny x nz grid, x fastest.
+// ---------------------------------------------------------------------------
+void or_grid_points(int nx, int ny, int nz, float h, float ox, float oy, float oz, float* out) {
+    const float o[3] = {ox * h, oy * h, oz * h};
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                float* q = out + 3 * lin(i, j, k, nx, ny);
+                q[0] = (float)i * h - o[0];
+                q[1] = (float)j * h - o[1];
+                q[2] = (float)k * h - o[2];
+            }
 }
 
 // ---------------------------------------------------------------------------

@@ -106,6 +106,19 @@ def main():
     out["cellidx_pos"] = np.stack(ci_pos)
     out["cellidx_out"] = np.stack(ci_out)
     out["cellidx_meta"] = np.array(ci_meta, np.float64)
+    # set_boundary_phi / set_boundary_vel sample positions: nodal grid and the three
+    # face grids, with their Buffer3D offsets, for two spacings
+    gp = []
+    for h in (1.0 / 64, 1.0 / 48):
+        hf = np.float32(h)
+        for offs, shp in (((0.5, 0.5, 0.5), (9, 10, 11)), ((0.5, 0.0, 0.0), (8, 9, 11)), ((0.0, 0.5, 0.0), (8, 10, 10)),
+                          ((0.0, 0.0, 0.5), (9, 9, 10))):
+            nz, ny, nx = shp
+            o = np.zeros((nz, ny, nx, 3), np.float32)
+            R.ref_grid_points(nx, ny, nz, float(hf), *offs, o)
+            gp.append((float(hf), offs, shp, o))
+    out["gridpts_meta"] = np.array([[g[0], *g[1], *g[2]] for g in gp], np.float64)
+    out["gridpts_out"] = np.concatenate([g[3].reshape(-1) for g in gp])
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, {k: v.shape for k, v in out.items()})

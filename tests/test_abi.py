@@ -50,3 +50,35 @@ def test_error_path_without_compute():
     assert rc == -1
     assert b"grid dims" in L.vflip_last_error()
     assert L.vflip_version().startswith(b"vflip")
+
+
+def test_mis_shaped_arrays_raise_before_any_launch():
+    """Staggered (MAC) arrays are checked against their face counts before the C ABI is
+    called: a wrongly shaped tensor raises ValueError instead of reaching a kernel."""
+    import torch
+    from vflip_amd import api
+    ni, nj, nk = 6, 5, 4  # distinct dims: the three face counts differ
+    u, v, w = torch.zeros((nk, nj, ni + 1)), torch.zeros((nk, nj + 1, ni)), torch.zeros((nk + 1, nj, ni))
+    p = torch.zeros((nk, nj, ni))
+    with pytest.raises(ValueError, match="vw has"):
+        api.grad_apply(api.COARSE, 0.1, 1.0, p, u, w, w, u, v, w, u, v, w)
+    with pytest.raises(ValueError, match="w has"):
+        api.advect_rk2(0.1, 0.01, u, v, v, p, (0, 0, 0), torch.zeros_like(p))
+    with pytest.raises(ValueError, match="out has"):
+        api.advect_rk2(0.1, 0.01, u, v, w, p, (0, 0, 0), torch.zeros_like(u))
+    with pytest.raises(ValueError, match="u has"):
+        api.inflow_bc(torch.zeros((nk, nj, ni), dtype=torch.int8), 1.0, p, v, w)
+    S = object.__new__(api.Solver)  # the checks run before the handle is used
+    S.ni, S.nj, S.nk = ni, nj, nk
+    S.n = ni * nj * nk
+    S.h = None
+    args = [u, v, w, u, v, w, u, v, w]
+    bad = list(args)
+    bad[7] = w  # vs with w's shape
+    with pytest.raises(ValueError, match="vs has"):
+        S.assemble(api.COARSE, 0.1, 1.0, *bad, p)
+    with pytest.raises(ValueError, match="rhs has"):
+        S.project(api.COARSE, 0.1, 1.0, *args, u, p)
+    # right shapes on the host reach the device check
+    with pytest.raises(ValueError, match="device tensor"):
+        api.grad_apply(api.COARSE, 0.1, 1.0, p, u, v, w, u, v, w, u, v, w)

@@ -194,6 +194,9 @@ def test_project_matches_oracle(cuda, mods):
 
 
 def test_host_dropin(cuda, mods):
+    """vflip_amgpcg_solve_host, the AMGPCGSolve<double> drop-in. Default (no options):
+    the mixed solver, so residual_out is the FP64 |b - A x|_inf of the returned result,
+    as the reference's; explicit FP32: the plain solver's iterate equals the oracle's."""
     O, api = mods
     n = 32
     pb = build_problem(n, 21, O)
@@ -201,9 +204,75 @@ def test_host_dropin(cuda, mods):
                                     pb["us"], pb["vs"], pb["ws"])
     ref = O.amgpcg(d, ax, ay, az, rhs)
     rc, x, res, its = api.amgpcg_solve_host(d, ax, ay, az, rhs)
+    true_res = np.abs(rhs - O.matvec(d, ax, ay, az, x)).max()
+    assert rc == 0 and abs(its - ref["iterations"]) <= 1
+    assert res <= 1e-6 * np.abs(rhs).max() and true_res <= 1e-6 * np.abs(rhs).max()
+    assert abs(res - true_res) <= 1e-9 * np.abs(rhs).max()
+    rc, x, res, its = api.amgpcg_solve_host(d, ax, ay, az, rhs, precision=api.FP32)
     assert rc == 0 and abs(its - ref["iterations"]) <= 1
     assert res <= 1e-6 * np.abs(rhs).max()
     assert rel_l2(x, oracle_iterate(O, d, ax, ay, az, rhs, its)) < 1e-5
+
+
+@pytest.mark.parametrize("dims", [(30, 34, 32), (9, 129, 130)])
+def test_coarsest_beyond_register_capacity(cuda, mods, dims):
+    """Grids whose coarsest level has more x-pairs than the register-resident
+    coarsest kernel holds (15 x 17 x 16: 2176 pairs; 5 x 65 x 65: 12675 > 4096
+    unknowns is never coarsest, 9 x 129 x 130 ends at 3 x 33 x 33 with 2178 pairs):
+    the general coarsest kernel solves them, in the V-cycle and in the tabulated
+    form, and the solve matches the oracle."""
+    import torch
+    O, api = mods
+    ni, nj, nk = dims
+    print(f"coarsest beyond register capacity {dims}", flush=True)
+    pb = scenes.box_problem(ni, nj, nk, 4)
+    uw, vw, ww = O.face_weights(pb["phi"])
+    d, ax, ay, az, rhs = O.assemble(pb["h"], pb["u"], pb["v"], pb["w"], uw, vw, ww, pb["us"], pb["vs"], pb["ws"])
+    S = api.Solver(ni, nj, nk)
+    lv = S.levels()[-1]
+    assert ((lv[0] + 1) // 2) * lv[1] * lv[2] > 2048 and lv[0] * lv[1] * lv[2] <= 4096
+    S.set_matrix(*[_t(a.astype(np.float32)) for a in (d, ax, ay, az)])
+    x = torch.zeros((nk, nj, ni), device="cuda")
+    for rep in range(2):  # the first solve sweeps directly, a reused operator is tabulated
+        x.zero_()
+        st = S.solve(_t(rhs.astype(np.float32)), x)
+        ref = O.amgpcg(d, ax, ay, az, rhs, tol=0.0, maxit=st["iterations"])
+        print(rep, st["iterations"], flush=True)
+        assert st["status"] == 0
+        assert rel_l2(_np(x), ref["x"]) < 1e-5
+        S.set_matrix(*[_t(a.astype(np.float32)) for a in (d, ax, ay, az)])
+
+
+def test_repeated_assembly_before_solve(cuda, mods):
+    """assemble(A); solve; assemble(B); assemble(B); solve: the change flag set by the
+    first B assembly survives the second (which changes no bits), so the second solve
+    rebuilds the hierarchy and matches a fresh solver. Mixed precision, whose V-cycle
+    reads FP16 copies of every level including level 0: a stale hierarchy would
+    precondition with the first operator altogether."""
+    import torch
+    O, api = mods
+    n = 40
+    print("repeated assembly before solve", flush=True)
+    S = api.Solver(n, n, n, precision=api.MIXED)
+    res = []
+    for seeds in ((1,), (2, 2)):
+        for seed in seeds:
+            pb = build_problem(n, seed, O)
+            args = [_t(pb[k]) for k in ("u", "v", "w", "uw", "vw", "ww", "us", "vs", "ws")]
+            rhs = torch.zeros((n, n, n), device="cuda")
+            S.assemble(api.COARSE, float(pb["h"]), 1.0, *args, rhs)
+        x = torch.zeros_like(rhs)
+        st = S.solve(rhs, x)
+        rf = torch.zeros_like(rhs)
+        G = api.Solver(n, n, n, precision=api.MIXED)
+        G.assemble(api.COARSE, float(pb["h"]), 1.0, *args, rf)
+        xf = torch.zeros_like(rhs)
+        sf = G.solve(rf, xf)
+        print(seeds, st["iterations"], sf["iterations"], st["true_residual"], flush=True)
+        assert st["converged"] == 1 and abs(st["iterations"] - sf["iterations"]) <= 1
+        assert st["true_residual"] <= 1e-6
+        assert rel_l2(_np(x), _np(xf)) < 1e-5
+        res.append(st["iterations"])
 
 
 @pytest.mark.parametrize("n,seed", [(64, 2), (96, 8)])

@@ -972,3 +972,128 @@ def test_tracer_frame_record():
     tr = np.array([[0.25, 0.5, 0.75, 0.3], [1.0, 0.0, 0.0, 1.0]], np.float32)
     f = O.tracer_frame(tr, 2.0)
     assert np.allclose(f, [[0.125 - 0.5, 0.25, 0.375 - 0.5, 0.3], [0.0, 0.0, -0.5, 1.0]], atol=0)
+
+
+# ------------------------------------------------------- boundary set-up, inflow
+def test_grid_points_golden_and_closed_form():
+    """set_boundary_phi / set_boundary_vel sample positions: the reference's own
+    Vec3f arithmetic (golden) and the closed form (i - o) h within one rounding."""
+    meta, flat = GOLD["gridpts_meta"], GOLD["gridpts_out"]
+    at = 0
+    for row in meta:
+        h, ox, oy, oz, nz, ny, nx = row
+        nz, ny, nx = int(nz), int(ny), int(nx)
+        m = nz * ny * nx * 3
+        ref = flat[at:at + m].reshape(nz, ny, nx, 3)
+        at += m
+        got = O.grid_points((nz, ny, nx), np.float32(h), (ox, oy, oz))
+        assert np.array_equal(bits(got), bits(ref))
+        k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+        exact = np.stack([(i - ox) * h, (j - oy) * h, (k - oz) * h], -1)
+        assert np.abs(got - exact).max() <= 2 * np.finfo(np.float32).eps * max(nx, ny, nz) * h
+
+
+def test_inflow_bc_closed_form():
+    ni, nj, nk = 7, 6, 5
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal((nk, nj, ni + 1)).astype(np.float32)
+    v = rng.standard_normal((nk, nj + 1, ni)).astype(np.float32)
+    w = rng.standard_normal((nk + 1, nj, ni)).astype(np.float32)
+    bd = np.zeros((nk, nj, ni), np.int8)
+    bd[2, 3, 1] = 1          # one inflow cell
+    bd[1, 1, 4] = bd[1, 1, 5] = 1  # two x-adjacent cells share the face u(5)
+    bd[3, 2, 2] = 2          # other boundary codes are left alone
+    uo, vo, wo = O.inflow_bc(bd, np.float32(2.5), u, v, w)
+    eu = u.copy(); ev = v.copy(); ew = w.copy()
+    for (k, j, i) in ((2, 3, 1), (1, 1, 4), (1, 1, 5)):
+        eu[k, j, i] = eu[k, j, i + 1] = 2.5
+        ev[k, j, i] = ev[k, j + 1, i] = 0
+        ew[k, j, i] = ew[k + 1, j, i] = 0
+    assert np.array_equal(uo, eu) and np.array_equal(vo, ev) and np.array_equal(wo, ew)
+    assert int((uo != u).sum()) == 5 and int((vo != v).sum()) == 6 and int((wo != w).sum()) == 6
+
+
+# ------------------------------------------------------- solid velocity, gradients
+@pytest.mark.parametrize("sub", [False, True])
+def test_assembly_solid_velocity_rigid_closed_form(sub):
+    """u = u_solid = an affine field: w u + (w - 1)(-u_solid) = u_solid on every face, so
+    the right-hand side is -div u_solid = -(a_x + b_y + c_z) in every cell with a non-empty
+    row, whatever the cut-cell face weights (S/Smoke_solver3D.cpp:1099-1108; sub
+    S/sub_domain.cpp:721). A sign, pairing or dropped-term error in the solid term fails it."""
+    n = 20
+    pb = scenes.projection_problem(n, 4)
+    h = pb["h"]
+    uw, vw, ww = O.face_weights(pb["phi"])
+    coef = [(0.3, 0.7, -0.2, 0.1), (-0.4, 0.05, -0.9, 0.3), (0.2, 0.4, 0.25, 0.35)]
+    us, vs, ws = scenes.affine_mac(n, n, n, float(h), coef)
+    dt = np.float32(0.01) if sub else np.float32(1.0)
+    d, ax, ay, az, rhs = O.assemble(h, us, vs, ws, uw, vw, ww, us, vs, ws, dt=dt, sub=sub)
+    live = d > 0
+    cut = live & ((d * h * h / (dt if sub else 1)) < 5.999)
+    assert cut.sum() > 50  # the scene has cut cells next to the spheres and walls
+    div = coef[0][1] + coef[1][2] + coef[2][3]
+    assert np.abs(rhs[live] + div).max() <= 2e-4 * max(1.0, abs(div))
+    # without the solid term the same inputs leave -sum w u / h, which is not -div at cut cells
+    z = scenes.mac_zeros(n)
+    *_, rhs0 = O.assemble(h, us, vs, ws, uw, vw, ww, *z, dt=dt, sub=sub)
+    assert np.abs(rhs0[cut] + div).max() > 1.0
+
+
+def test_grad_apply_sub_variant_divergence_free():
+    """SubDomainSolver: matrix terms w dt / h^2 and u -= dt (p_i - p_{i-1}) / h on faces
+    1..n-1 (S/sub_domain.cpp:612-729, 745-819), so after the solve EVERY interior cell
+    (1..n-2) is divergence-free, including the layer next to the ring that the coarse
+    variant's 2..n-2 ranges leave alone; faces 0 and n are untouched."""
+    n = 22
+    pb = scenes.projection_problem(n, 8)
+    h = pb["h"]
+    dt = np.float32(0.02)
+    uw, vw, ww = O.face_weights(pb["phi"])
+    z = [pb["us"], pb["vs"], pb["ws"]]
+    d, ax, ay, az, rhs = O.assemble(h, pb["u"], pb["v"], pb["w"], uw, vw, ww, *z, dt=dt, sub=True)
+    out = O.amgpcg(d, ax, ay, az, rhs, tol=1e-12, maxit=300)
+    u, v, w = O.grad_apply(h, out["x"], uw, vw, ww, *z, pb["u"], pb["v"], pb["w"], dt=dt, sub=True)
+    d2, *_, rhs2 = O.assemble(h, u, v, w, uw, vw, ww, *z, dt=dt, sub=True)
+    live = d2 > 0
+    assert np.abs(rhs2[live]).max() <= 1e-4 * np.abs(rhs[live]).max()
+    # the coarse variant's ranges leave the cells next to the ring divergent
+    uc, vc, wc = O.grad_apply(h, out["x"] * (1.0 / float(dt)), uw, vw, ww, *z, pb["u"], pb["v"], pb["w"])
+    *_, rhs3 = O.assemble(h, uc, vc, wc, uw, vw, ww, *z, dt=dt, sub=True)
+    ring1 = np.zeros_like(live)
+    ring1[1:-1, 1:-1, 1] = True
+    assert np.abs(rhs3[ring1 & live]).max() > 1e-2 * np.abs(rhs[live]).max()
+    # faces outside 1..n-1 along their axis keep their values
+    assert np.array_equal(u[:, :, [0, n]], pb["u"][:, :, [0, n]])
+    assert np.array_equal(v[:, [0, n], :], pb["v"][:, [0, n], :])
+    assert np.array_equal(w[[0, n], :, :], pb["w"][[0, n], :, :])
+
+
+# ------------------------------------------------------- clamped cubic advection
+def test_cubic_clamp_step_field():
+    """advect_field_cubic_clamp (S/Smoke_solver3D.cpp:690-702): where the tricubic sample
+    leaves [min, max] of the 8 trilinear corners it is replaced by the trilinear value.
+    A step in x sampled a quarter cell to the right overshoots on both sides of the
+    jump (cubic 1 + 7/128 and -5/128), while the cell at the jump stays cubic (15/64)."""
+    n = 16
+    h = np.float32(1.0 / n)
+    dt = np.float32(0.01)
+    U = np.float32(-0.25) * h / dt  # backtrace x - dt U lands 1/4 cell to the right
+    u = np.full((n, n, n + 1), U, np.float32)
+    v = np.zeros((n, n + 1, n), np.float32)
+    w = np.zeros((n + 1, n, n), np.float32)
+    f = np.zeros((n, n, n), np.float32)
+    f[:, :, 8:] = 1.0
+    cub = O.advect_rk2(h, dt, u, v, w, f, (0.0, 0.0, 0.0), mode=1)
+    lin = O.advect_rk2(h, dt, u, v, w, f, (0.0, 0.0, 0.0), mode=0)
+    s = (slice(3, -3), slice(3, -3))
+    # overshooting cells take the trilinear value
+    assert np.array_equal(cub[s + (6,)], lin[s + (6,)]) and np.abs(lin[s + (6,)]).max() < 1e-6
+    assert np.array_equal(cub[s + (8,)], lin[s + (8,)]) and np.abs(lin[s + (8,)] - 1).max() < 1e-6
+    # the cell at the jump keeps the cubic value: weights of {-1,0,1,2} at t = 1/4 on (0,0,1,1)
+    t = 0.25
+    w1 = t + 0.5 * (t * t - t ** 3)
+    w2 = (t ** 3 - t) / 6
+    assert np.abs(cub[s + (7,)] - (w1 + w2)).max() < 1e-5
+    assert np.abs(lin[s + (7,)] - 0.25).max() < 1e-5
+    # away from the jump both equal the field
+    assert np.array_equal(cub[s + (11,)], f[s + (11,)]) and np.array_equal(cub[s + (3,)], f[s + (3,)])

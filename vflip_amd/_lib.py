@@ -57,6 +57,10 @@ class SubdomainView(ctypes.Structure):
                 ("bmax", c_float * 3)] + [(f, c_void_p) for f in ("u", "v", "w", "solid_phi", "us", "vs", "ws")]
 
 
+# host callbacks of vflip_set_boundary_phi / vflip_set_boundary_vel
+PHI_FN = ctypes.CFUNCTYPE(c_float, ctypes.POINTER(c_float), c_void_p)
+VEL_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(c_float), ctypes.POINTER(c_float), c_void_p)
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "vflip_last_error": (ctypes.c_char_p, []),
@@ -107,6 +111,8 @@ SIGNATURES = {
                                         ctypes.POINTER(SolveOpts)]),
     "vflip_face_weights": (c_int, [c_int, c_int, c_int, P, P, P, P, P]),
     "vflip_inflow_bc": (c_int, [c_int, c_int, c_int, P, c_float, P, P, P, P]),
+    "vflip_set_boundary_phi": (c_int, [c_int, c_int, c_int, c_float, P, P, P, P]),
+    "vflip_set_boundary_vel": (c_int, [c_int, c_int, c_int, c_float, P, P, P, P, P, P]),
     "vflip_assemble": (c_int, [P, c_int, c_float, c_float] + [P] * 10),
     "vflip_grad_apply": (c_int, [c_int, c_int, c_int, c_int, c_float, c_float] + [P] * 10 + [P]),
     "vflip_project": (c_int, [P, c_int, c_float, c_float] + [P] * 11 + [ctypes.POINTER(SolveStats)]),

@@ -9,7 +9,8 @@ import os
 
 import torch
 
-from ._lib import ERR_CAPACITY, CoarseView, SolveOpts, SolveStats, SubdomainView, VflipError, check, lib  # noqa: F401
+from ._lib import (ERR_CAPACITY, PHI_FN, VEL_FN, CoarseView, SolveOpts, SolveStats, SubdomainView, VflipError,  # noqa: F401
+                   check, lib)
 
 FP32 = 0
 MIXED = 1
@@ -20,15 +21,35 @@ SUB = 1
 def _ptr(t, dtype=torch.float32, numel=None, name="array"):
     if t is None:
         return None
-    if not t.is_cuda:
-        raise ValueError(f"{name} must be a device tensor")
     if t.dtype != dtype:
         raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
     if numel is not None and t.numel() != numel:
         raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _mac_ptrs(ni, nj, nk, arrays, names):
+    """Pointers of MAC-staggered arrays (u, v, w order repeated), each checked against its
+    face count: u (ni+1) nj nk, v ni (nj+1) nk, w ni nj (nk+1)."""
+    counts = ((ni + 1) * nj * nk, ni * (nj + 1) * nk, ni * nj * (nk + 1))
+    for q, (a, nm) in enumerate(zip(arrays, names)):  # every size first, then the pointers
+        if a is not None and a.numel() != counts[q % 3]:
+            raise ValueError(f"{nm} has {a.numel()} elements, expected {counts[q % 3]}")
+    return [_ptr(a, numel=counts[q % 3], name=nm) for q, (a, nm) in enumerate(zip(arrays, names))]
+
+
+_FACE_NAMES = ("u", "v", "w", "uw", "vw", "ww", "us", "vs", "ws")
+
+
+def _sizes(*items):
+    """(tensor, numel, name) triples: raises ValueError for the first wrong element count."""
+    for t, numel, name in items:
+        if t is not None and t.numel() != numel:
+            raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
 
 
 def _stream():
@@ -127,31 +148,40 @@ class Solver:
         check(lib().vflip_solver_get_level(self.h, l, *[_ptr(o) for o in outs]), "get_level")
         return outs
 
+    def _faces(self, *arrays):
+        return _mac_ptrs(self.ni, self.nj, self.nk, arrays, _FACE_NAMES)
+
     def assemble(self, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs):
-        check(lib().vflip_assemble(self.h, variant, h, dt, *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
-                                   _ptr(rhs, numel=self.n)), "assemble")
+        _sizes((rhs, self.n, "rhs"))
+        check(lib().vflip_assemble(self.h, variant, h, dt, *self._faces(u, v, w, uw, vw, ww, us, vs, ws),
+                                   _ptr(rhs, numel=self.n, name="rhs")), "assemble")
 
     def assemble_liquid(self, h, dt, u, v, w, uw, vw, ww, us, vs, ws, liquid_phi, rhs):
         """Free-surface assembly: cells with liquid_phi >= 0 are air (Dirichlet p = 0)."""
-        check(lib().vflip_assemble_liquid(self.h, h, dt, *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
-                                          _ptr(liquid_phi, numel=self.n), _ptr(rhs, numel=self.n)), "assemble_liquid")
+        _sizes((rhs, self.n, "rhs"), (liquid_phi, self.n, "liquid_phi"))
+        check(lib().vflip_assemble_liquid(self.h, h, dt, *self._faces(u, v, w, uw, vw, ww, us, vs, ws),
+                                          _ptr(liquid_phi, numel=self.n, name="liquid_phi"),
+                                          _ptr(rhs, numel=self.n, name="rhs")), "assemble_liquid")
 
     def project_learned(self, net, factor, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
         """project() with the U-Net's pressure guess as the PCG's initial iterate."""
+        _sizes((rhs, self.n, "rhs"), (p, self.n, "p"))
         st = SolveStats()
         rc = check(lib().vflip_project_learned(self.h, net.h, factor, variant, h, dt,
-                                               *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
-                                               _ptr(rhs, numel=self.n), _ptr(p, numel=self.n), ctypes.byref(st)),
+                                               *self._faces(u, v, w, uw, vw, ww, us, vs, ws),
+                                               _ptr(rhs, numel=self.n, name="rhs"), _ptr(p, numel=self.n, name="p"),
+                                               ctypes.byref(st)),
                    "project_learned")
         d = st.as_dict()
         d["status"] = rc
         return d
 
     def project(self, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
+        _sizes((rhs, self.n, "rhs"), (p, self.n, "p"))
         st = SolveStats()
-        rc = check(lib().vflip_project(self.h, variant, h, dt,
-                                       *[_ptr(a) for a in (u, v, w, uw, vw, ww, us, vs, ws)],
-                                       _ptr(rhs, numel=self.n), _ptr(p, numel=self.n), ctypes.byref(st)),
+        rc = check(lib().vflip_project(self.h, variant, h, dt, *self._faces(u, v, w, uw, vw, ww, us, vs, ws),
+                                       _ptr(rhs, numel=self.n, name="rhs"), _ptr(p, numel=self.n, name="p"),
+                                       ctypes.byref(st)),
                    "project")
         d = st.as_dict()
         d["status"] = rc
@@ -169,17 +199,53 @@ def face_weights(phi):
 
 def inflow_bc(bdesc, u_in, u, v, w):
     nk, nj, ni = bdesc.shape
-    check(lib().vflip_inflow_bc(ni, nj, nk, _ptr(bdesc, torch.int8), u_in, _ptr(u), _ptr(v), _ptr(w), _stream()),
+    faces = _mac_ptrs(ni, nj, nk, (u, v, w), ("u", "v", "w"))
+    check(lib().vflip_inflow_bc(ni, nj, nk, _ptr(bdesc, torch.int8, name="bdesc"), u_in, *faces, _stream()),
           "inflow_bc")
 
 
 def grad_apply(variant, h, dt, p, uw, vw, ww, us, vs, ws, u, v, w):
     nk, nj, ni = p.shape
-    check(lib().vflip_grad_apply(ni, nj, nk, variant, h, dt, _ptr(p), *[_ptr(a) for a in (uw, vw, ww, us, vs, ws)],
-                                 _ptr(u), _ptr(v), _ptr(w), _stream()), "grad_apply")
+    faces = _mac_ptrs(ni, nj, nk, (uw, vw, ww, us, vs, ws, u, v, w), _FACE_NAMES[3:] + _FACE_NAMES[:3])
+    check(lib().vflip_grad_apply(ni, nj, nk, variant, h, dt, _ptr(p, name="p"), *faces, _stream()), "grad_apply")
 
 
-def amgpcg_solve_host(diag, ax, ay, az, rhs, tol=1e-6, maxit=50, precision=FP32):
+def grad_apply_f64(variant, h, dt, p, uw, vw, ww, us, vs, ws, u, v, w):
+    """grad_apply from an FP64 pressure ((float)(p[c] - p[c-1]) as the reference)."""
+    nk, nj, ni = p.shape
+    faces = _mac_ptrs(ni, nj, nk, (uw, vw, ww, us, vs, ws, u, v, w), _FACE_NAMES[3:] + _FACE_NAMES[:3])
+    check(lib().vflip_grad_apply_f64(ni, nj, nk, variant, h, dt, _ptr(p, torch.float64, name="p"), *faces,
+                                     _stream()), "grad_apply_f64")
+
+
+def set_boundary_phi(ni, nj, nk, h, fn, out=None):
+    """set_boundary_phi: nodal (nk+1, nj+1, ni+1) device array of fn(x, y, z) (a Python
+    callable taking three floats) at the reference's node positions."""
+    out = torch.empty((nk + 1, nj + 1, ni + 1), device="cuda") if out is None else out
+    cb = PHI_FN(lambda pos, user: float(fn(pos[0], pos[1], pos[2])))
+    check(lib().vflip_set_boundary_phi(ni, nj, nk, h, ctypes.cast(cb, ctypes.c_void_p), None,
+                                       _ptr(out, numel=(ni + 1) * (nj + 1) * (nk + 1), name="nodal_phi"), _stream()),
+          "set_boundary_phi")
+    return out
+
+
+def set_boundary_vel(ni, nj, nk, h, fn):
+    """set_boundary_vel: MAC solid velocity (us, vs, ws) from fn(x, y, z) -> (vx, vy, vz)."""
+    us = torch.empty((nk, nj, ni + 1), device="cuda")
+    vs = torch.empty((nk, nj + 1, ni), device="cuda")
+    ws = torch.empty((nk + 1, nj, ni), device="cuda")
+
+    def call(pos, out, user):
+        r = fn(pos[0], pos[1], pos[2])
+        out[0], out[1], out[2] = float(r[0]), float(r[1]), float(r[2])
+    cb = VEL_FN(call)
+    check(lib().vflip_set_boundary_vel(ni, nj, nk, h, ctypes.cast(cb, ctypes.c_void_p), None,
+                                       *_mac_ptrs(ni, nj, nk, (us, vs, ws), ("us", "vs", "ws")), _stream()),
+          "set_boundary_vel")
+    return us, vs, ws
+
+
+def amgpcg_solve_host(diag, ax, ay, az, rhs, tol=1e-6, maxit=50, precision=MIXED):
     """AMGPCGSolve drop-in on host float64 numpy arrays shaped (nk, nj, ni)."""
     import numpy as np
     nk, nj, ni = diag.shape
@@ -198,8 +264,14 @@ def advect_rk2(h, dt, u, v, w, field, offs, out, mode=0, field2=None, out2=None)
     """RK2 semi-Lagrangian advection of field (and field2) on a grid shaped like field."""
     nk, nj, ni1 = u.shape
     fz, fy, fx = field.shape
-    check(lib().vflip_advect_rk2(ni1 - 1, nj, nk, h, dt, mode, _ptr(u), _ptr(v), _ptr(w), _ptr(field), fx, fy, fz,
-                                 offs[0], offs[1], offs[2], _ptr(out), _ptr(field2), _ptr(out2), _stream()),
+    nf = field.numel()
+    for t, nm in ((out, "out"), (field2, "field2"), (out2, "out2")):
+        if t is not None and t.numel() != nf:
+            raise ValueError(f"{nm} has {t.numel()} elements, expected {nf}")
+    vel = _mac_ptrs(ni1 - 1, nj, nk, (u, v, w), ("u", "v", "w"))
+    check(lib().vflip_advect_rk2(ni1 - 1, nj, nk, h, dt, mode, *vel, _ptr(field, name="field"), fx, fy, fz,
+                                 offs[0], offs[1], offs[2], _ptr(out, numel=nf, name="out"),
+                                 _ptr(field2, numel=nf, name="field2"), _ptr(out2, numel=nf, name="out2"), _stream()),
           "advect_rk2")
 
 

@@ -823,6 +823,43 @@ __global__ __launch_bounds__(kCoarseThreads) void k_coarsest(Op<CT> op, const fl
             }
 }
 
+// Coarsest level whose pairs exceed the register-resident kernel's capacity
+// (npairs > kCoarsePairs * kCoarseThreads, e.g. 15 x 17 x 16 with 8 pairs per row):
+// the same RB-GS sweeps (odd colour first, reference CSR term order, division by
+// the diagonal as gs_value) with x in LDS at its cell index and the operator read
+// from memory; each thread strides over the level's cells. tab as k_coarsest.
+template <typename CT>
+__global__ __launch_bounds__(kCoarseThreads) void k_coarsest_general(Op<CT> op, const float* __restrict__ b,
+                                                                     float* __restrict__ x, int sweeps,
+                                                                     float* __restrict__ tab) {
+    __shared__ float xs[kCoarseMax];
+    const int unit = tab ? (int)blockIdx.x : -1;
+    const int ni = op.ni, nj = op.nj, n = op.ni * op.nj * op.nk;
+    for (int c = threadIdx.x; c < n; c += kCoarseThreads) xs[c] = 0.f;
+    __syncthreads();
+    for (int it = 0; it < sweeps; ++it) {
+        for (int col = 1; col >= 0; --col) {
+            for (int c = threadIdx.x; c < n; c += kCoarseThreads) {
+                int i, j, k;
+                unflat32((unsigned)c, (unsigned)ni, (unsigned)nj, i, j, k);
+                if (((i + j + k) & 1) != col) continue;
+                const float bc = unit >= 0 ? (c == unit ? 1.f : 0.f) : b[c];
+                xs[c] = gs_value(bc, offdiag_sum(op, xs, c, i, j, k), op.D(c));
+            }
+            __syncthreads();
+        }
+    }
+    for (int c = threadIdx.x; c < n; c += kCoarseThreads) {
+        if (tab) tab[(size_t)unit * n + c] = xs[c];
+        else x[c] = xs[c];
+    }
+}
+
+// Whether the register-resident coarsest kernel covers a level's pairs.
+__host__ __device__ inline bool coarse_fits_registers(int ni, int nj, int nk) {
+    return ((ni + 1) / 2) * nj * nk <= kCoarsePairs * kCoarseThreads;
+}
+
 // y = tab^T b in two passes (64-column chunks, then a fixed-order sum of the
 // chunk partials): deterministic and spread over the whole GPU.
 constexpr int kCoarseChunk = 64;

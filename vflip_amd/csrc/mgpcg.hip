@@ -229,8 +229,7 @@ int run_vcycle(vflip_solver* S, bool want_dot, bool* dot_done) {
         const float* b = (nl == 1) ? S->r : lv.b;
         float* x = (nl == 1) ? S->z : lv.x;
         const int ppr = (lv.ni + 1) / 2;
-        const int npairs = ppr * lv.nj * lv.nk;
-        if (lv.n > (size_t)kCoarseMax || npairs > kCoarsePairs * kCoarseThreads) {
+        if (lv.n > (size_t)kCoarseMax) {
             set_error("coarsest level %dx%dx%d exceeds the single-workgroup solver", lv.ni, lv.nj, lv.nk);
             return VFLIP_ERR_ARG;
         }
@@ -240,6 +239,13 @@ int run_vcycle(vflip_solver* S, bool want_dot, bool* dot_done) {
             k_coarse_apply_partial<<<dim3((nc + kBlock - 1) / kBlock, nch), kBlock, 0, S->stream>>>(S->ctab, b,
                                                                                                   S->cpart, nc);
             k_coarse_apply_reduce<<<(nc + kBlock - 1) / kBlock, kBlock, 0, S->stream>>>(S->cpart, x, nc, nch);
+        } else if (!coarse_fits_registers(lv.ni, lv.nj, lv.nk)) {
+            if (lv.half)
+                k_coarsest_general<__half><<<1, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), b, x,
+                                                                                S->opts.coarse_sweeps, nullptr);
+            else
+                k_coarsest_general<float><<<1, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), b, x,
+                                                                               S->opts.coarse_sweeps, nullptr);
         } else if (16 % ppr == 0) {  // x-neighbours through DPP (see k_coarsest)
             if (lv.half)
                 k_coarsest<__half, true><<<1, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), b, x,
@@ -275,8 +281,19 @@ int build_ctab(vflip_solver* S) {
         if (!rc) rc = alloc((void**)&S->cpart, (size_t)nc * ((nc + kCoarseChunk - 1) / kCoarseChunk) * sizeof(float));
         if (rc) return rc;
     }
+    if (lv.n > (size_t)kCoarseMax) {
+        set_error("coarsest level %dx%dx%d exceeds the single-workgroup solver", lv.ni, lv.nj, lv.nk);
+        return VFLIP_ERR_ARG;
+    }
     const bool xreg = 16 % ((lv.ni + 1) / 2) == 0;  // x-neighbours through DPP (see k_coarsest)
-    if (lv.half && xreg)
+    if (!coarse_fits_registers(lv.ni, lv.nj, lv.nk)) {
+        if (lv.half)
+            k_coarsest_general<__half><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), nullptr, nullptr,
+                                                                             S->opts.coarse_sweeps, S->ctab);
+        else
+            k_coarsest_general<float><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<float>(lv), nullptr, nullptr,
+                                                                            S->opts.coarse_sweeps, S->ctab);
+    } else if (lv.half && xreg)
         k_coarsest<__half, true><<<nc, kCoarseThreads, 0, S->stream>>>(make_op<__half>(lv), nullptr, nullptr,
                                                                        S->opts.coarse_sweeps, S->ctab);
     else if (lv.half)
@@ -1068,6 +1085,10 @@ int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const do
     VF_ARG(check_dims(ni, nj, nk), "grid dims");
     vflip_solve_opts o;
     vflip_default_opts(&o);
+    // The reference solves in FP64: without options the drop-in runs the mixed
+    // solver, whose stop test is on the FP64 true residual, so residual_out is the
+    // FP64 |b - A x|_inf the reference reports.
+    o.precision = VFLIP_PREC_MIXED;
     if (opts) o = *opts;
     o.tolerance = tol;
     o.max_iterations = maxit;
@@ -1179,9 +1200,12 @@ int solver_solve_assembled(vflip_solver* S, const float* rhs32, float* x32, vfli
 }
 // Before an assembly that rewrites level 0: clears the device change flag; the
 // next setup only rebuilds the hierarchy when the assembly set it.
+// A flag that an earlier assembly set and no setup has consumed yet stays set:
+// assemblies in a row accumulate into it.
 int* solver_begin_assembly(vflip_solver* S) {
     int* f = S->d_changed;
-    k_fill<<<1, kBlock, 0, S->stream>>>(1, (float*)f, 0.f);  // 0.0f and int 0 share the bit pattern
+    if (!S->check_changed)
+        k_fill<<<1, kBlock, 0, S->stream>>>(1, (float*)f, 0.f);  // 0.0f and int 0 share the bit pattern
     S->check_changed = true;
     return f;
 }

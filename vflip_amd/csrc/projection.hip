@@ -5,6 +5,8 @@
 // file is compiled without FMA contraction and with IEEE fp32 division.
 #pragma clang fp contract(off)
 
+#include <vector>
+
 #include "common.h"
 
 namespace vflip {
@@ -266,6 +268,66 @@ int vflip_inflow_bc(int ni, int nj, int nk, const int8_t* bdesc, float u_in, flo
     VF_CHECK_LAUNCH();
     VF_HIP(hipStreamSynchronize(s));
     return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// World position of point (i,j,k) of a grid with Buffer3D offsets o, in the
+// reference's float arithmetic: Vec3f(i,j,k)*h - Vec3f(ox,oy,oz)*h
+// (S/Smoke_solver3D.cpp:806,864). The file's contract(off) keeps the host
+// compiler from fusing the product into the subtraction.
+void grid_point(int i, int j, int k, float h, const float o[3], float* q) {
+    const float a = o[0] * h, b = o[1] * h, c = o[2] * h;
+    q[0] = (float)i * h - a;
+    q[1] = (float)j * h - b;
+    q[2] = (float)k * h - c;
+}
+
+// Samples one component of the callback at every point of an nx x ny x nz grid
+// (comp < 0: the scalar callback) and uploads the values.
+int sample_grid(int nx, int ny, int nz, float h, const float o[3], vflip_phi_fn phi, vflip_vel_fn vel, int comp,
+                void* user, float* dst, hipStream_t s) {
+    const size_t n = (size_t)nx * ny * nz;
+    std::vector<float> host(n);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                float q[3], r[3] = {0.f, 0.f, 0.f};
+                grid_point(i, j, k, h, o, q);
+                float val;
+                if (comp < 0) {
+                    val = phi(q, user);
+                } else {
+                    vel(q, r, user);
+                    val = r[comp];
+                }
+                host[lin3(i, j, k, nx, ny)] = val;
+            }
+    VF_HIP(hipMemcpyAsync(dst, host.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    VF_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int vflip_set_boundary_phi(int ni, int nj, int nk, float h, vflip_phi_fn phi, void* user, float* nodal_phi,
+                           void* stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && h > 0 && phi && nodal_phi, "set_boundary_phi args");
+    const float o[3] = {0.5f, 0.5f, 0.5f};
+    return sample_grid(ni + 1, nj + 1, nk + 1, h, o, phi, nullptr, -1, user, nodal_phi, (hipStream_t)stream);
+}
+
+int vflip_set_boundary_vel(int ni, int nj, int nk, float h, vflip_vel_fn vel, void* user, float* us, float* vs,
+                           float* ws, void* stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && h > 0 && vel && us && vs && ws, "set_boundary_vel args");
+    hipStream_t s = (hipStream_t)stream;
+    const float ou[3] = {0.5f, 0.f, 0.f}, ov[3] = {0.f, 0.5f, 0.f}, ow[3] = {0.f, 0.f, 0.5f};
+    int rc = sample_grid(ni + 1, nj, nk, h, ou, nullptr, vel, 0, user, us, s);
+    if (!rc) rc = sample_grid(ni, nj + 1, nk, h, ov, nullptr, vel, 1, user, vs, s);
+    if (!rc) rc = sample_grid(ni, nj, nk + 1, h, ow, nullptr, vel, 2, user, ws, s);
+    return rc;
 }
 
 int vflip_grad_apply(int ni, int nj, int nk, int variant, float h, float dt, const float* p, const float* uw,
