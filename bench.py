@@ -156,10 +156,11 @@ def _cpu_step(sim_state, phi, n, dt):
     }
 
 
-def trajectory(n, steps, phi, spheres):
+def trajectory(n, steps, phi, spheres, paths=("plain", "mixed", "learned_mixed")):
     """Error of the GPU plume after `steps` steps from the initial state against
     the FP64 CPU oracle run for the same steps: relative L2 of p, u, v, w and rho
-    for the plain, mixed and learned + mixed paths, with per-step iterations."""
+    for each of the plain, mixed and learned + mixed paths, with per-step
+    iterations and FP64 true residuals."""
     import numpy as np
     import torch
     from vflip_amd import api
@@ -174,22 +175,31 @@ def trajectory(n, steps, phi, spheres):
         st = {"u": o["u"], "v": o["v"], "w": o["w"], "rho": o["rho"], "temp": o["temp"], "p": o["p"]}
         its_cpu.append(o["iterations"])
         log(f"oracle step {s}: {o['ms_per_step']:.0f} ms, {o['iterations']} its")
-    paths = [("plain", api.FP32, None), ("mixed", api.MIXED, None)]
-    net = load_network()
-    if net is not None:
-        paths.append(("learned_mixed", api.MIXED, net))
+    todo = []
+    for name in paths:
+        if name == "plain":
+            todo.append((name, api.FP32, None))
+        elif name == "mixed":
+            todo.append((name, api.MIXED, None))
+        elif name == "learned_mixed":
+            net = load_network()
+            if net is not None:
+                todo.append((name, api.MIXED, net))
     out = {"workload": f"smoke plume {n}^3 ({len(spheres)} sphere obstacles), {steps} steps from the initial state",
            "oracle": "FP64 CPU restatement, AMGPCG to 1e-6", "oracle_iterations": its_cpu, "paths": {}}
-    for name, prec, nt in paths:
+    for name, prec, nt in todo:
         sim = PlumeSim(n, phi, precision=prec)
         if nt is not None:
             sim.use_network(nt, max(1, n // 64))
-        its = []
+        its, tres = [], []
         for _ in range(steps):
-            its.append(sim.step()["iterations"])
+            r = sim.step()
+            its.append(r["iterations"])
+            tres.append(r["true_residual"])
         torch.cuda.synchronize()
         g = {k: t.detach().cpu().numpy() for k, t in sim.state().items()}
-        out["paths"][name] = {"iterations": its, **{f"{k}_rel_l2": rel(g[k], st[k]) for k in ("p", "u", "v", "w", "rho")}}
+        out["paths"][name] = {"iterations": its, "true_residual_max": max(tres),
+                              **{f"{k}_rel_l2": rel(g[k], st[k]) for k in ("p", "u", "v", "w", "rho")}}
         log(f"{name}: {out['paths'][name]}")
         del sim
     return out

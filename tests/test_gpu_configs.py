@@ -1,0 +1,103 @@
+"""The workloads of the configurations at their own sizes, against the FP64 CPU oracle.
+
+* Plume at 64^3 with the obstacle of the plain-plume configuration (a solid sphere
+  of radius 0.15 at the domain centre), 10 steps.
+* Plume at 128^3 (10 steps) and 256^3 (2 steps) with seeded random sphere
+  obstacles, plain FP32 and mixed-precision projections.
+* Nested subdomains: 256^3 coarse plume + 12 fine subdomains at 2x, batched fine
+  solves against sequential ones, every fine solve's FP64 true residual, and one
+  fine solve against the oracle's AMGPCG on the same assembled operator.
+
+Each step's solve stops at a 1e-6 relative residual, so the fields differ from
+the oracle's by what that tolerance and FP32 storage leave: measured p 1.1e-6 -
+1.7e-6 and u 2e-7 - 3e-7 relative L2. Bounds: p 1e-5, velocities and density
+2e-6 (the same bounds at every size). Plain FP32 takes the oracle's iteration
+count (same stop test on the recursive residual); the mixed path stops on the
+FP64 true residual and may take one iteration more or fewer.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(out, steps, mixed_slack=1):
+    paths = out["paths"]
+    assert {"plain", "mixed"} <= set(paths)
+    for name, r in paths.items():
+        print(name, {k: v for k, v in r.items()}, flush=True)
+        assert len(r["iterations"]) == steps
+        assert r["p_rel_l2"] <= 1e-5, name
+        for k in ("u", "v", "w", "rho"):
+            assert r[f"{k}_rel_l2"] <= 2e-6, (name, k)
+    assert paths["plain"]["iterations"] == out["oracle_iterations"]
+    assert paths["mixed"]["true_residual_max"] <= 1e-6
+    for a, b in zip(paths["mixed"]["iterations"], out["oracle_iterations"]):
+        assert abs(a - b) <= mixed_slack
+
+
+def test_c1_plume_64_centre_sphere(cuda):
+    import bench
+    from vflip_amd import scenes
+    print("C1: plume 64^3, sphere r=0.15 at the centre, 10 steps", flush=True)
+    n = 64
+    spheres = [((0.5, 0.5, 0.5), 0.15)]
+    phi = scenes.solid_phi(n, 1.0 / n, spheres)
+    out = bench.trajectory(n, 10, phi, spheres, paths=("plain", "mixed"))
+    _check(out, 10)
+
+
+@pytest.mark.parametrize("n,steps,seed", [(128, 10, 3), (256, 2, 5)])
+def test_c2_plume(cuda, n, steps, seed):
+    import bench
+    from vflip_amd import scenes
+    spheres = scenes.random_spheres(seed)
+    print(f"C2: plume {n}^3, {len(spheres)} spheres (seed {seed}), {steps} steps", flush=True)
+    phi = scenes.solid_phi(n, 1.0 / n, spheres)
+    out = bench.trajectory(n, steps, phi, spheres, paths=("plain", "mixed"))
+    _check(out, steps)
+
+
+def test_c5_nested_256(cuda):
+    """256^3 coarse + 12 subdomains, 2 steps: batched (solver tile shapes) equals sequential
+    field for field; every fine solve converges and states its FP64 true residual; the
+    last step's largest fine solve equals the oracle's AMGPCG iterate on its operator."""
+    import torch
+    from oracle import oracle as O
+    from vflip_amd.nested import NestedPlume
+    print("C5: nested 256^3 + 12 subdomains, 2 steps", flush=True)
+
+    def run(batched):
+        sim = NestedPlume(n=256, count=12, seed=0, batched=batched, autotune=False)
+        recs = [sim.step() for _ in range(2)]
+        torch.cuda.synchronize()
+        return sim, recs
+
+    sb, rb = run(True)
+    fb = [t.detach().cpu().numpy() for t in (sb.coarse.u, sb.coarse.p)] + \
+         [t.detach().cpu().numpy() for s in sb.subs for t in (s.u, s.p)]
+    # one fine solve against the oracle on the same operator and right-hand side
+    q = int(np.argmax([np.prod(s.nf) for s in sb.subs]))
+    s = sb.subs[q]
+    d, ax, ay, az = (a.detach().cpu().numpy().astype(np.float64) for a in s.solver.get_level(0))
+    rhs = s.rhs.detach().cpu().numpy().astype(np.float64)
+    its = rb[-1]["fine_iterations"][q]
+    ref = O.amgpcg(d, ax, ay, az, rhs, tol=0.0, maxit=its)["x"]
+    p = s.p.detach().cpu().numpy().astype(np.float64)
+    err = np.linalg.norm(p - ref) / np.linalg.norm(ref)
+    print(f"fine solve {q} ({s.nf}): {its} its, rel L2 vs oracle iterate {err:.2e}", flush=True)
+    assert err < 1e-5
+    del sb
+    torch.cuda.empty_cache()
+    ss, rs = run(False)
+    fs = [t.detach().cpu().numpy() for t in (ss.coarse.u, ss.coarse.p)] + \
+         [t.detach().cpu().numpy() for s in ss.subs for t in (s.u, s.p)]
+    for a, b in zip(rb, rs):
+        print("fine its", a["fine_iterations"], "true residuals", ["%.1e" % x for x in a["fine_true_residuals"]],
+              flush=True)
+        assert a["fine_iterations"] == b["fine_iterations"]
+        assert a["fine_converged"] and b["fine_converged"]
+        assert a["fine_max_rel_residual"] <= 1e-6
+        assert max(a["fine_true_residuals"]) <= 1e-4
+    for a, b in zip(fb, fs):
+        assert np.array_equal(a, b)

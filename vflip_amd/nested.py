@@ -169,7 +169,8 @@ class NestedPlume:
                "fine_post_ms": (t4 - t3) * 1e3, "step_ms": (t4 - t0) * 1e3, "coarse_iterations": cst["iterations"],
                "fine_iterations": [st["iterations"] for st in stats],
                "fine_converged": all(st["converged"] for st in stats),
-               "fine_max_rel_residual": max(st["rel_residual"] for st in stats)}
+               "fine_max_rel_residual": max(st["rel_residual"] for st in stats),
+               "fine_true_residuals": [st["true_residual"] for st in stats]}
         self.timing.append(rec)
         return rec
 
