@@ -49,20 +49,31 @@ def make_scene(n, seed=0):
     return phi, spheres
 
 
-WEIGHTS = os.path.join(ROOT, "vflip_amd", "data", "unet_plume64.npz")
+WEIGHTS = os.path.join(ROOT, "vflip_amd", "data", "unet_correction256.npz")
+WEIGHTS_GRID = 256  # the solve grid the correction was trained for (network on 64^3)
 
 
 def load_network():
-    """The trained warm-start U-Net (weights committed with the library), or None."""
+    """The trained correction U-Net of the history warm start (weights committed with the
+    library), or None."""
     from vflip_amd.unet import UNet
     if not os.path.exists(WEIGHTS):
         return None
     net = UNet(seed=0)
     net.load(WEIGHTS)
+    net.trained_gain = net.gain
     return net
 
 
-def run_gpu(n, steps, warmup, precision, phi, init_state=None, profile=False, net=None, solver_opts=None):
+def configure_network(net, n):
+    """Pressure unit for a solve grid of n^3: the correction scales with h^2 (the operator
+    carries 1/h^2), so the gain trained at WEIGHTS_GRID is rescaled by (WEIGHTS_GRID / n)^2."""
+    net.gain = net.trained_gain * (WEIGHTS_GRID / n) ** 2
+    return net
+
+
+def run_gpu(n, steps, warmup, precision, phi, init_state=None, profile=False, net=None, solver_opts=None,
+            history=False):
     import torch
     from vflip_amd import api
     from vflip_amd.plume import PlumeSim
@@ -70,7 +81,10 @@ def run_gpu(n, steps, warmup, precision, phi, init_state=None, profile=False, ne
     if solver_opts:
         sim.solver.set_opts(**solver_opts)
     if net is not None:
-        sim.use_network(net, max(1, n // 64))
+        configure_network(net, n)
+        sim.use_network(net, max(1, n // 64), history=history)
+    elif history:
+        sim.use_history(True)
     if init_state is not None:
         sim.load_state(init_state)
     for _ in range(warmup):
@@ -156,7 +170,7 @@ def _cpu_step(sim_state, phi, n, dt):
     }
 
 
-def trajectory(n, steps, phi, spheres, paths=("plain", "mixed", "learned_mixed")):
+def trajectory(n, steps, phi, spheres, paths=("plain", "mixed", "mixed_history", "learned_mixed")):
     """Error of the GPU plume after `steps` steps from the initial state against
     the FP64 CPU oracle run for the same steps: relative L2 of p, u, v, w and rho
     for each of the plain, mixed and learned + mixed paths, with per-step
@@ -181,16 +195,20 @@ def trajectory(n, steps, phi, spheres, paths=("plain", "mixed", "learned_mixed")
             todo.append((name, api.FP32, None))
         elif name == "mixed":
             todo.append((name, api.MIXED, None))
+        elif name == "mixed_history":
+            todo.append((name, api.MIXED, "history"))
         elif name == "learned_mixed":
             net = load_network()
             if net is not None:
-                todo.append((name, api.MIXED, net))
+                todo.append((name, api.MIXED, configure_network(net, n)))
     out = {"workload": f"smoke plume {n}^3 ({len(spheres)} sphere obstacles), {steps} steps from the initial state",
            "oracle": "FP64 CPU restatement, AMGPCG to 1e-6", "oracle_iterations": its_cpu, "paths": {}}
     for name, prec, nt in todo:
         sim = PlumeSim(n, phi, precision=prec)
-        if nt is not None:
-            sim.use_network(nt, max(1, n // 64))
+        if nt == "history":
+            sim.use_history(True)
+        elif nt is not None:
+            sim.use_network(nt, max(1, n // 64), history=True)
         its, tres = [], []
         for _ in range(steps):
             r = sim.step()
@@ -404,16 +422,15 @@ def main():
         print(json.dumps(trajectory(n, args.trajectory, phi, spheres)), flush=True)
         return
 
-    # plain FP32 MGPCG
+    # plain FP32 MGPCG from a zero guess (the reference algorithm)
     sim_p, ms_plain, its_plain, proj_plain, _, st_p = run_gpu(n, args.steps, args.warmup, api.FP32, phi)
     log(f"plain: {ms_plain:.3f} ms/step its={its_plain}")
     state0 = {k: t.clone() for k, t in sim_p.state().items()}
-    del sim_p
-    # mixed precision (headline), with kernel timing of the dominant kernel
-    sim_m, ms_mixed, its_mixed, proj_mixed, prof, st = run_gpu(n, args.steps, args.warmup, api.MIXED, phi,
-                                                               profile=True)
-    log(f"mixed: {ms_mixed:.3f} ms/step its={its_mixed} prof={prof}")
-    del sim_m
+    sims = {"plain": sim_p}
+    # mixed precision from a zero guess
+    sim_m, ms_mixed, its_mixed, proj_mixed, _, st = run_gpu(n, args.steps, args.warmup, api.MIXED, phi)
+    log(f"mixed: {ms_mixed:.3f} ms/step its={its_mixed}")
+    sims["mixed"] = sim_m
     # the mixed form as the plume configuration words it: FP16 copies of levels >= 2 only,
     # FP64 true-residual refresh every 8 iterations (timed beside the tuned default)
     cfg_opts = {"half_from_level": 2, "refine_every": 8}
@@ -424,19 +441,30 @@ def main():
                  "projection_ms": round(float(np.mean(proj_cfg)), 3), "true_residual_last": st_c["true_residual"],
                  "solver_opts": cfg_opts}
     del sim_c
-    # learned + mixed: the U-Net's guess (on the grid coarsened to 64^3) seeds every mixed solve
+    # the non-learned control: mixed warm-started from the solver's pressure history alone
+    sim_h, ms_hist, its_hist, proj_hist, _, st_h = run_gpu(n, args.steps, args.warmup, api.MIXED, phi, history=True)
+    log(f"mixed + pressure history: {ms_hist:.3f} ms/step its={its_hist}")
+    sims["mixed_history"] = sim_h
+    history = {"ms_per_step": round(ms_hist, 3), "iterations": its_hist,
+               "projection_ms": round(float(np.mean(proj_hist)), 3), "true_residual_last": st_h["true_residual"]}
+    # learned + mixed (headline): the history guess corrected by the U-Net from the divergence it
+    # leaves, on the grid coarsened to 64^3; the dominant kernel is timed in this run
     net = load_network()
     learned = None
     st_l = None
+    prof = None
     if net is not None:
-        sim_l, ms_learned, its_learned, proj_learned, _, st_l = run_gpu(n, args.steps, args.warmup, api.MIXED, phi,
-                                                                        net=net)
-        log(f"learned+mixed: {ms_learned:.3f} ms/step its={its_learned}")
+        sim_l, ms_learned, its_learned, proj_learned, prof, st_l = run_gpu(n, args.steps, args.warmup, api.MIXED,
+                                                                           phi, profile=True, net=net, history=True)
+        log(f"learned+mixed: {ms_learned:.3f} ms/step its={its_learned} prof={prof}")
         learned = {"ms_per_step": round(ms_learned, 3), "iterations": its_learned,
                    "projection_ms": round(float(np.mean(proj_learned)), 3),
-                   "true_residual_last": st_l["true_residual"], "network_grid": 64, "factor": max(1, n // 64),
-                   "weights": os.path.relpath(WEIGHTS, ROOT)}
-        del sim_l
+                   "true_residual_last": st_l["true_residual"], "network_grid": n // max(1, n // 64),
+                   "factor": max(1, n // 64), "weights": os.path.relpath(WEIGHTS, ROOT),
+                   "network_gain_ms_per_step_vs_history": round(ms_hist - ms_learned, 3),
+                   "network_gain_iterations_per_step_vs_history": round(float(np.mean(its_hist) -
+                                                                              np.mean(its_learned)), 3)}
+        sims["learned_mixed"] = sim_l
 
     roofline = None
     if prof and prof["launches"] > 0:
@@ -445,14 +473,14 @@ def main():
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic_from_profiles("k_rbgs_zrow<4, 0, double, __half>") if n == 256 else None,
-                    "kernel": "k_rbgs_zrow<4, 0, double, __half> (fused red-black sweep, level 0, mixed run)",
+                    "kernel": "k_rbgs_zrow<4, 0, double, __half> (fused red-black sweep, level 0, headline run)",
                     "avg_launch_us": round(avg_ms * 1e3, 2), "launches": prof["launches"],
                     "algorithmic_bytes_per_launch": prof["bytes_per_launch"]}
 
     cpu = None
     oracle_err = None
     if not args.no_cpu:
-        # one step from the plain run's final state on the CPU oracle, and the same step on the GPU
+        # one step from the plain run's final state on the CPU oracle, and the same step on each GPU path
         log("cpu baseline: FP64 oracle steps (1 warm-up + 5 timed) ...")
         base = cpu_baseline(state0, phi, n)
         from oracle import oracle as O
@@ -462,22 +490,23 @@ def main():
                          f"to 1e-6, {base['iterations']} its + gradient), FP64 oracle with OpenMP; median of "
                          f"{base['reps']} after 1 warm-up",
                "ms_solve": round(base["ms_solve"], 1), "cpu_model": cpu_model(), "nproc": os.cpu_count()}
-        from vflip_amd.plume import PlumeSim
-        g = PlumeSim(n, phi, precision=api.FP32)
-        g.load_state(state0)
-        stg = g.step()
-        torch.cuda.synchronize()
 
         def rel(a, b):
             a = a.detach().cpu().numpy().astype(np.float64)
             return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
-        oracle_err = {"p_rel_l2": rel(g.p, base["p"]), "u_rel_l2": rel(g.u, base["u"]),
-                      "v_rel_l2": rel(g.v, base["v"]), "w_rel_l2": rel(g.w, base["w"]),
-                      "gpu_iterations": stg["iterations"], "oracle_iterations": base["iterations"]}
+        oracle_err = {"oracle_iterations": base["iterations"]}
+        for name, g in sims.items():
+            g.load_state(state0)
+            stg = g.step()
+            torch.cuda.synchronize()
+            oracle_err[name] = {"p_rel_l2": rel(g.p, base["p"]), "u_rel_l2": rel(g.u, base["u"]),
+                                "v_rel_l2": rel(g.v, base["v"]), "w_rel_l2": rel(g.w, base["w"]),
+                                "gpu_iterations": stg["iterations"], "true_residual": stg["true_residual"]}
         log(f"cpu: {cpu} err: {oracle_err}")
+    del sims
 
     voxels = n ** 3
-    head = learned["ms_per_step"] if learned else ms_mixed
+    head = learned["ms_per_step"] if learned else ms_hist
     dflt = api.default_opts()
     out = {
         "metric": METRIC,
@@ -493,26 +522,32 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"smoke plume {n}^3 (1-4 seeded sphere obstacles), RK2 semi-Lagrangian advection + "
-                               f"MGPCG projection to 1e-6; headline = learned warm start + mixed-precision "
-                               f"projection" + ("" if learned else " (weights missing: mixed only)"),
+                               f"MGPCG projection to 1e-6; headline = learned warm start (pressure-history guess "
+                               f"corrected by the U-Net) + mixed-precision projection" +
+                               ("" if learned else " (weights missing: history guess only)"),
                    "grid": n, "dt": 0.01, "tolerance": 1e-6, "spheres": len(spheres)},
-        "iterations": {"learned_mixed": learned["iterations"] if learned else None, "mixed": its_mixed,
-                       "plain": its_plain},
+        "iterations": {"learned_mixed": learned["iterations"] if learned else None, "mixed_history": its_hist,
+                       "mixed": its_mixed, "plain": its_plain},
         "plain_ms_per_step": round(ms_plain, 3),
         "mixed_ms_per_step": round(ms_mixed, 3),
+        "mixed_history_ms_per_step": round(ms_hist, 3),
         "learned_mixed_ms_per_step": learned["ms_per_step"] if learned else None,
         "projection_ms": {"plain": round(float(np.mean(proj_plain)), 3), "mixed": round(float(np.mean(proj_mixed)), 3),
+                          "mixed_history": history["projection_ms"],
                           "learned_mixed": learned["projection_ms"] if learned else None},
         "speedup_vs_plain": round(ms_plain / head, 3),
         "speedup_mixed_vs_plain": round(ms_plain / ms_mixed, 3),
+        "speedup_mixed_history_vs_plain": round(ms_plain / ms_hist, 3),
         "voxels_per_s": round(voxels / (head * 1e-3), 1),
-        "true_residual_last": learned["true_residual_last"] if learned else st["true_residual"],
+        "true_residual_last": learned["true_residual_last"] if learned else st_h["true_residual"],
         # last solve of each run: recursive |r|_inf/|b|_inf (the stop test) and the FP64 true residual
         "residuals_last": {"plain": {"recursive": st_p["rel_residual"], "true_fp64": st_p["true_residual"]},
                            "mixed": {"recursive": st["rel_residual"], "true_fp64": st["true_residual"]},
+                           "mixed_history": {"recursive": st_h["rel_residual"], "true_fp64": st_h["true_residual"]},
                            "learned_mixed": ({"recursive": st_l["rel_residual"], "true_fp64": st_l["true_residual"]}
                                              if learned else None)},
         "learned_warm_start": learned,
+        "history_warm_start": history,
         # the tuned mixed form above (FP16 coefficients at every V-cycle level, FP64 refresh
         # every 4 iterations) beside the form the plume configuration words
         "mixed_solver_opts": {"half_from_level": dflt.half_from_level, "refine_every": dflt.refine_every},

@@ -80,6 +80,8 @@ typedef struct {
     double ms_solve;      /* PCG loop */
     int warm_rejected;    /* warm_start only: 1 when the guess was no better than zero in the
                              A-norm (E(x0) = x0'Ax0/2 - b'x0 >= 0) and the solve restarted from 0 */
+    double guess_residual; /* |b - A x0|_inf / |b|_inf of the initial iterate the PCG started
+                              from (1 for a zero start; FP64 for a warm start) */
 } vflip_solve_stats;
 
 /* Reference defaults: tol 1e-6, 50 iterations, FP32, 4/4/200 sweeps; mixed-mode
@@ -581,6 +583,31 @@ int vflip_project_learned(vflip_solver* S, vflip_unet* net, int factor, int vari
                           float* v, float* w, const float* uw, const float* vw, const float* ww, const float* us,
                           const float* vs, const float* ws, float* rhs_scratch, float* pressure,
                           vflip_solve_stats* stats);
+
+/* vflip_project warm-started from the solver's own pressure history (no reference
+ * counterpart: the reference zeroes the pressure before every solve,
+ * S/Smoke_solver3D.cpp:960 and S/AlgebraicMultigrid.h:212). The solver keeps the FP64
+ * pressures of its last three solves; the initial guess is 0 before the first,
+ * else the combination of them that minimises x'Ax/2 - b'x (the Galerkin
+ * projection onto previous solutions, Fischer 1998), formed on the GPU. With a
+ * network, the divergence that guess leaves, r = b - A x0, goes through the
+ * network on the grid coarsened by `factor` (vflip_unet_warm_start with r in
+ * place of the right-hand side) and the prolonged correction is added to x0. The
+ * PCG vets the guess (dropped when it is no better than zero in the A-norm) and
+ * runs to the solver's tolerance as always; the solution becomes the newest
+ * history entry (a solve that fails clears the history). net may be NULL: the
+ * extrapolation alone. stats->guess_residual reports |b - A x0|_inf / |b|_inf. */
+int vflip_project_history(vflip_solver* S, vflip_unet* net, int factor, int variant, float h, float dt, float* u,
+                          float* v, float* w, const float* uw, const float* vw, const float* ww, const float* us,
+                          const float* vs, const float* ws, float* rhs_scratch, float* pressure,
+                          vflip_solve_stats* stats);
+/* The initial guess vflip_project_history would start from for the operator and
+ * right-hand side rhs (device FP32, ni*nj*nk) the solver holds now (its FP64 form
+ * when the assembly wrote it), into guess (device FP64, ni*nj*nk). Returns how
+ * many history entries it rests on (0: zero guess), <0 on error. Synchronous. */
+int vflip_solver_history_guess(vflip_solver* S, const float* rhs, double* guess);
+/* Forgets the pressure history (a new scene or a jump in time). */
+int vflip_solver_reset_history(vflip_solver* S);
 
 #ifdef __cplusplus
 }

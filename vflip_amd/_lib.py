@@ -39,6 +39,7 @@ class SolveStats(ctypes.Structure):
         ("ms_setup", c_double),
         ("ms_solve", c_double),
         ("warm_rejected", c_int),
+        ("guess_residual", c_double),
     ]
 
     def as_dict(self):
@@ -154,6 +155,10 @@ SIGNATURES = {
     "vflip_unet_train_step_energy": (c_int, [P, c_int, c_int, c_int, P, P, P, P, P, P, P, P, PD]),
     "vflip_project_learned": (c_int, [P, P, c_int, c_int, c_float, c_float] + [P] * 11 +
                               [ctypes.POINTER(SolveStats)]),
+    "vflip_project_history": (c_int, [P, P, c_int, c_int, c_float, c_float] + [P] * 11 +
+                              [ctypes.POINTER(SolveStats)]),
+    "vflip_solver_reset_history": (c_int, [P]),
+    "vflip_solver_history_guess": (c_int, [P, P, P]),
 }
 
 _lib = None

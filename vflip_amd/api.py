@@ -176,6 +176,31 @@ class Solver:
         d["status"] = rc
         return d
 
+    def project_history(self, net, factor, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
+        """project() from the solver's pressure history (2 p1 - p2), corrected by the
+        network from the divergence the guess leaves when net is given (else None)."""
+        _sizes((rhs, self.n, "rhs"), (p, self.n, "p"))
+        st = SolveStats()
+        rc = check(lib().vflip_project_history(self.h, None if net is None else net.h, factor, variant, h, dt,
+                                               *self._faces(u, v, w, uw, vw, ww, us, vs, ws),
+                                               _ptr(rhs, numel=self.n, name="rhs"), _ptr(p, numel=self.n, name="p"),
+                                               ctypes.byref(st)),
+                   "project_history")
+        d = st.as_dict()
+        d["status"] = rc
+        return d
+
+    def history_guess(self, rhs, out=None):
+        """The FP64 guess project_history would start from for the current operator and rhs;
+        returns (guess, number of history entries used)."""
+        out = torch.empty(rhs.shape, device="cuda", dtype=torch.float64) if out is None else out
+        used = check(lib().vflip_solver_history_guess(self.h, _ptr(rhs, numel=self.n, name="rhs"),
+                                                      _ptr(out, torch.float64, self.n, "guess")), "history_guess")
+        return out, used
+
+    def reset_history(self):
+        check(lib().vflip_solver_reset_history(self.h), "reset_history")
+
     def project(self, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
         _sizes((rhs, self.n, "rhs"), (p, self.n, "p"))
         st = SolveStats()

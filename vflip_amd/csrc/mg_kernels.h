@@ -1309,6 +1309,136 @@ static __global__ __launch_bounds__(kBlock) void k_cast_f32_f64(unsigned n, cons
     unsigned c = blockIdx.x * kBlock + threadIdx.x;
     if (c < n) o[c] = (double)a[c];
 }
+// Energy-optimal initial guess from the pressure history (the projection onto
+// previous solutions of P. F. Fischer, "Projection techniques for iterative
+// solution of Ax = b with successive right-hand sides", CMAME 163, 1998): the
+// x0 in span{p1, p2, p3} that minimises x'Ax/2 - b'x. The span is taken in the
+// difference basis v0 = p1, v1 = p1 - p2, v2 = p1 - 2 p2 + p3 (the last solutions
+// are nearly parallel; their differences keep the M x M Gram system well
+// conditioned). k_hist_gram forms the partial sums of G = V'AV (upper triangle)
+// and f = V'b per workgroup, with A the level-0 operator in FP64 arithmetic (FP64
+// diagonal when the assembly wrote it); 64 x 4 threads, kTrueResKZ planes each.
+// partial[q * nblocks + block], q: G00 G01 G02 G11 G12 G22 f0 f1 f2.
+constexpr int kHistSums = 9;
+template <int M>
+static __global__ __launch_bounds__(kBlock) void k_hist_gram(Op<float> op, const double* __restrict__ d64,
+                                                             const double* __restrict__ b64,
+                                                             const float* __restrict__ b32,
+                                                             const double* __restrict__ p1,
+                                                             const double* __restrict__ p2,
+                                                             const double* __restrict__ p3,
+                                                             double* __restrict__ partial) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int kb = blockIdx.z * kTrueResKZ;
+    const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
+    const double* P[3] = {p1, p2, p3};
+    double acc[kHistSums] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < ni && j < nj && kb < nk) {
+        const int ke = min(kb + kTrueResKZ, nk);
+        for (int k = kb; k < ke; ++k) {
+            const int c = (k * nj + j) * ni + i;
+            const double dc = d64 ? d64[c] : (double)op.D(c);
+            double x[3], y[3];
+#pragma unroll
+            for (int m = 0; m < M; ++m) {
+                const double* v = P[m];
+                double t = dc * v[c];
+                if (k > 0) t += (double)op.AZ(c - sl) * v[c - sl];
+                if (j > 0) t += (double)op.AY(c - ni) * v[c - ni];
+                if (i > 0) t += (double)op.AX(c - 1) * v[c - 1];
+                if (i + 1 < ni) t += (double)op.AX(c) * v[c + 1];
+                if (j + 1 < nj) t += (double)op.AY(c) * v[c + ni];
+                if (k + 1 < nk) t += (double)op.AZ(c) * v[c + sl];
+                x[m] = v[c];
+                y[m] = t;
+            }
+            // difference basis
+            double vb[3] = {x[0], 0, 0}, ab[3] = {y[0], 0, 0};
+            if (M >= 2) { vb[1] = x[0] - x[1]; ab[1] = y[0] - y[1]; }
+            if (M >= 3) { vb[2] = x[0] - 2.0 * x[1] + x[2]; ab[2] = y[0] - 2.0 * y[1] + y[2]; }
+            const double bc = b64 ? b64[c] : (double)b32[c];
+            acc[0] += vb[0] * ab[0];
+            acc[6] += vb[0] * bc;
+            if (M >= 2) { acc[1] += vb[0] * ab[1]; acc[3] += vb[1] * ab[1]; acc[7] += vb[1] * bc; }
+            if (M >= 3) { acc[2] += vb[0] * ab[2]; acc[4] += vb[1] * ab[2]; acc[5] += vb[2] * ab[2]; acc[8] += vb[2] * bc; }
+        }
+    }
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int q = 0; q < kHistSums; ++q) {
+        const double t = block_sum(acc[q]);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[q * nb + blk] = t;
+    }
+}
+
+// One workgroup: sums the Gram partials in a fixed order, solves G c = f by
+// symmetric elimination (a basis vector whose pivot falls below 1e-13 of the first
+// diagonal is dropped: coefficient 0), and writes the weights of p1, p2, p3 of
+// x0 = c0 v0 + c1 v1 + c2 v2 to coef[0..2].
+static __global__ __launch_bounds__(kBlock) void k_hist_solve(const double* __restrict__ partial, int nb, int M,
+                                                              double* __restrict__ coef) {
+    __shared__ double sums[kHistSums];
+    for (int q = 0; q < kHistSums; ++q) {
+        double s = 0;
+        for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[q * nb + b];
+        s = block_sum(s);
+        if (threadIdx.x == 0) sums[q] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double G[3][3] = {{sums[0], sums[1], sums[2]}, {sums[1], sums[3], sums[4]}, {sums[2], sums[4], sums[5]}};
+    double f[3] = {sums[6], sums[7], sums[8]};
+    double c[3] = {0, 0, 0};
+    bool keep[3] = {false, false, false};
+    const double ref = G[0][0];
+    if (ref > 0) {
+        // forward elimination on the kept vectors
+        for (int k = 0; k < M; ++k) {
+            if (!(G[k][k] > 1e-13 * ref)) continue;
+            keep[k] = true;
+            for (int r = k + 1; r < M; ++r) {
+                const double l = G[r][k] / G[k][k];
+                for (int q = k; q < M; ++q) G[r][q] -= l * G[k][q];
+                f[r] -= l * f[k];
+            }
+        }
+        for (int k = M - 1; k >= 0; --k) {
+            if (!keep[k]) continue;
+            double t = f[k];
+            for (int q = k + 1; q < M; ++q) t -= G[k][q] * c[q];
+            c[k] = t / G[k][k];
+        }
+    }
+    coef[0] = c[0] + c[1] + c[2];
+    coef[1] = -(c[1] + 2.0 * c[2]);
+    coef[2] = c[2];
+}
+
+// g = coef[0] p1 + coef[1] p2 + coef[2] p3 (g may alias p3: element-wise).
+static __global__ __launch_bounds__(kBlock) void k_hist_combine(unsigned n, const double* __restrict__ coef,
+                                                                const double* p1, const double* p2, const double* p3,
+                                                                double* g) {
+    const double a0 = coef[0], a1 = coef[1], a2 = coef[2];
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) {
+        double v = a0 * p1[c];
+        if (p2) v += a1 * p2[c];
+        if (p3) v += a2 * p3[c];
+        g[c] = v;
+    }
+}
+
+// Extrapolated initial guess from the pressure history: g = 2 p1 - p2 (p2 null: g = p1).
+static __global__ __launch_bounds__(kBlock) void k_extrapolate64(unsigned n, const double* __restrict__ p1,
+                                                                 const double* __restrict__ p2, double* __restrict__ g) {
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock)
+        g[c] = p2 ? 2.0 * p1[c] - p2[c] : p1[c];
+}
+// g += (double)d
+static __global__ __launch_bounds__(kBlock) void k_add_f32_f64(unsigned n, const float* __restrict__ d,
+                                                               double* __restrict__ g) {
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) g[c] += (double)d[c];
+}
 static __global__ __launch_bounds__(kBlock) void k_fill64(unsigned n, double* __restrict__ a, double v) {
     unsigned c = blockIdx.x * kBlock + threadIdx.x;
     if (c < n) a[c] = v;

@@ -505,6 +505,7 @@ int vflip_solver_destroy(vflip_solver* S) {
     }
     if (S->pcg_alloc) (void)hipFree(S->pcg_alloc);
     if (S->x64) (void)hipFree(S->x64);
+    if (S->hist_alloc) (void)hipFree(S->hist_alloc);
     if (S->ctab) (void)hipFree(S->ctab);
     if (S->cpart) (void)hipFree(S->cpart);
     if (S->d_changed) (void)hipFree(S->d_changed);
@@ -939,6 +940,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
         S->h_sc->res = S->h_sc->bnorm;
         stats.warm_rejected = 1;
     }
+    stats.guess_residual = S->h_sc->bnorm > 0 ? S->h_sc->res / S->h_sc->bnorm : 0.0;
     const double bnorm = S->h_sc->bnorm;
     const double tol_abs = S->h_sc->tol_abs;
     double res = S->h_sc->res;
@@ -1078,6 +1080,29 @@ int vflip_solver_solve_f64(vflip_solver* S, const double* rhs, double* x, vflip_
     return solve_impl(S, S->b32, rhs, nullptr, x, st);
 }
 
+}  // extern "C"
+namespace vflip {
+int solver_history_guess(vflip_solver* S, const float* rhs32, double** guess, int* used);
+}
+extern "C" {
+
+int vflip_solver_history_guess(vflip_solver* S, const float* rhs, double* guess) {
+    VF_ARG(S && rhs && guess, "solver/rhs/guess");
+    double* g = nullptr;
+    int used = 0;
+    int rc = solver_history_guess(S, rhs, &g, &used);
+    if (rc) return rc;
+    VF_HIP(hipMemcpyAsync(guess, g, S->n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+    VF_HIP(hipStreamSynchronize(S->stream));
+    return used;
+}
+
+int vflip_solver_reset_history(vflip_solver* S) {
+    VF_ARG(S, "solver");
+    S->hist_n = 0;
+    return 0;
+}
+
 int vflip_amgpcg_solve_host(int ni, int nj, int nk, const double* diag, const double* ax, const double* ay,
                             const double* az, const double* rhs, double* result, double tol, int maxit,
                             double* residual_out, int* iterations_out, const vflip_solve_opts* opts) {
@@ -1200,6 +1225,90 @@ int solver_solve_assembled(vflip_solver* S, const float* rhs32, float* x32, vfli
 }
 // Before an assembly that rewrites level 0: clears the device change flag; the
 // next setup only rebuilds the hierarchy when the assembly set it.
+// The initial guess of vflip_project_history in the free history slot: zero with
+// no history, else the energy-optimal combination of the last (up to three)
+// pressures (k_hist_gram / k_hist_solve / k_hist_combine; FP64, no host sync).
+// rhs32: the assembled right-hand side (its FP64 form is used when valid).
+// *used receives how many history entries the guess rests on.
+int solver_history_guess(vflip_solver* S, const float* rhs32, double** guess, int* used) {
+    const LevelBuf& l0 = S->L[0];
+    const dim3 gg((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
+    const int nb = nblocks(gg);
+    if (!S->hist_alloc) {
+        const size_t words = 3 * S->n + (size_t)kHistSums * nb + 4;
+        int rc = alloc((void**)&S->hist_alloc, words * sizeof(double));
+        if (rc) return rc;
+        for (int q = 0; q < 3; ++q) S->hist[q] = S->hist_alloc + q * S->n;
+        S->hist_part = S->hist_alloc + 3 * S->n;
+        S->hist_coef = S->hist_part + (size_t)kHistSums * nb;
+        S->hist_n = 0;
+    }
+    int rc = ensure_f64(S);
+    if (rc) return rc;
+    const unsigned n = (unsigned)S->n;
+    double* g = S->hist[(S->hist_head + 1) % 3];
+    const int M = std::min(S->hist_n, 3);
+    if (M == 0) {
+        k_fill64<<<S->vblocks, kBlock, 0, S->stream>>>(n, g, 0.0);
+    } else {
+        const double* p1 = S->hist[S->hist_head];
+        const double* p2 = M >= 2 ? S->hist[(S->hist_head + 2) % 3] : nullptr;
+        const double* p3 = M >= 3 ? S->hist[(S->hist_head + 1) % 3] : nullptr;  // the oldest: g's slot
+        const double* d64 = S->d64_valid ? S->d64 : nullptr;
+        const double* b64 = (mixed(S) && S->b64_valid) ? S->b64 : nullptr;
+        const Op<float> op = make_op<float>(l0);
+        if (M == 1)
+            k_hist_gram<1><<<gg, dim3(64, 4), 0, S->stream>>>(op, d64, b64, rhs32, p1, p2, p3, S->hist_part);
+        else if (M == 2)
+            k_hist_gram<2><<<gg, dim3(64, 4), 0, S->stream>>>(op, d64, b64, rhs32, p1, p2, p3, S->hist_part);
+        else
+            k_hist_gram<3><<<gg, dim3(64, 4), 0, S->stream>>>(op, d64, b64, rhs32, p1, p2, p3, S->hist_part);
+        k_hist_solve<<<1, kBlock, 0, S->stream>>>(S->hist_part, nb, M, S->hist_coef);
+        k_hist_combine<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->hist_coef, p1, p2, p3, g);
+    }
+    VF_CHECK_LAUNCH();
+    *guess = g;
+    *used = M;
+    return 0;
+}
+// r = rhs - A g for a guess (FP64 arithmetic with the assembled FP64 diagonal and,
+// in mixed mode, right-hand side), stored FP32 in the solver's residual buffer.
+int solver_guess_residual(vflip_solver* S, const float* rhs32, const double* g, float** r) {
+    int rc = ensure_f64(S);
+    if (rc) return rc;
+    launch_true_residual64(S, (mixed(S) && S->b64_valid) ? S->b64 : nullptr, rhs32, g, S->r);
+    VF_CHECK_LAUNCH();
+    *r = S->r;
+    return 0;
+}
+// FP32 scratch of the solver's size that is free until the next solve.
+float* solver_scratch32(vflip_solver* S) { return S->z; }
+// guess += correction (FP32)
+int solver_add_correction(vflip_solver* S, double* g, const float* corr) {
+    k_add_f32_f64<<<S->vblocks, kBlock, 0, S->stream>>>((unsigned)S->n, corr, g);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+// Solve the assembled system from the guess in the free history slot (warm start;
+// a guess that is no better than zero in the A-norm is dropped by the solve), then
+// make the solution the newest history entry. x32 receives the FP32 pressure.
+int solver_solve_history(vflip_solver* S, const float* rhs32, double* g, float* x32, vflip_solve_stats* st,
+                         bool warm) {
+    const double* b64 = (mixed(S) && S->b64_valid) ? S->b64 : nullptr;
+    const int old = S->opts.warm_start;
+    S->opts.warm_start = warm ? 1 : 0;
+    const int status = solve_impl(S, rhs32, b64, x32, g, st);
+    S->opts.warm_start = old;
+    if (status < 0 && status != VFLIP_ERR_BREAKDOWN) return status;
+    if (status == VFLIP_OK) {
+        S->hist_head = (S->hist_head + 1) % 3;
+        S->hist_n = std::min(S->hist_n + 1, 3);
+    } else {
+        S->hist_n = 0;  // no trusted history after a failed solve
+    }
+    return status;
+}
+
 // A flag that an earlier assembly set and no setup has consumed yet stays set:
 // assemblies in a row accumulate into it.
 int* solver_begin_assembly(vflip_solver* S) {

@@ -378,6 +378,11 @@ void solver_mark_f64_assembled(vflip_solver* s);
 bool solver_is_mixed(const vflip_solver* s);
 int solver_set_warm_start(vflip_solver* s, int on);
 int solver_solve_assembled(vflip_solver* s, const float* rhs32, float* x32, vflip_solve_stats* st);
+int solver_history_guess(vflip_solver* S, const float* rhs32, double** guess, int* used);
+int solver_guess_residual(vflip_solver* S, const float* rhs32, const double* g, float** r);
+float* solver_scratch32(vflip_solver* S);
+int solver_add_correction(vflip_solver* S, double* g, const float* corr);
+int solver_solve_history(vflip_solver* S, const float* rhs32, double* g, float* x32, vflip_solve_stats* st, bool warm);
 }  // namespace vflip
 
 extern "C" {
@@ -419,6 +424,10 @@ int vflip_assemble_liquid(vflip_solver* S, float h, float dt, const float* u, co
 }
 
 namespace {
+int apply_pressure(vflip_solver* S, int variant, float h, float dt, float* u, float* v, float* w, const float* uw,
+                   const float* vw, const float* ww, const float* us, const float* vs, const float* ws,
+                   const float* pressure);
+
 int project_impl(vflip_solver* S, vflip_unet* net, int factor, int variant, float h, float dt, float* u, float* v,
                  float* w, const float* uw, const float* vw, const float* ww, const float* us, const float* vs,
                  const float* ws, float* rhs_scratch, float* pressure, vflip_solve_stats* stats) {
@@ -444,6 +453,54 @@ int project_impl(vflip_solver* S, vflip_unet* net, int factor, int variant, floa
         status = solver_solve_assembled(S, rhs_scratch, pressure, stats);
     }
     if (status < 0 && status != VFLIP_ERR_BREAKDOWN) return status;
+    rc = apply_pressure(S, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, pressure);
+    if (rc) return rc;
+    return status;
+}
+
+// Pressure-history projection (vflip_project_history): guess from the solver's FP64
+// history, optionally corrected by the network from the divergence the guess leaves.
+int project_history_impl(vflip_solver* S, vflip_unet* net, int factor, int variant, float h, float dt, float* u,
+                         float* v, float* w, const float* uw, const float* vw, const float* ww, const float* us,
+                         const float* vs, const float* ws, float* rhs_scratch, float* pressure,
+                         vflip_solve_stats* stats) {
+    VF_ARG(S && pressure && rhs_scratch, "solver/pressure/rhs");
+    int rc = vflip_assemble(S, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs_scratch);
+    if (rc) return rc;
+    rc = vflip_solver_setup(S);
+    if (rc) return rc;
+    double* g = nullptr;
+    int used = 0;
+    if ((rc = solver_history_guess(S, rhs_scratch, &g, &used))) return rc;
+    bool warm = used > 0;
+    if (net) {
+        int ni, nj, nk;
+        float *d, *ax, *ay, *az, *lk;
+        hipStream_t s;
+        if ((rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s))) return rc;
+        float* r = nullptr;
+        if ((rc = solver_guess_residual(S, rhs_scratch, g, &r))) return rc;
+        float* corr = solver_scratch32(S);
+        if ((rc = vflip_unet_set_stream(net, s))) return rc;
+        if ((rc = vflip_unet_warm_start(net, ni, nj, nk, factor, r, d, corr))) return rc;
+        if ((rc = solver_add_correction(S, g, corr))) return rc;
+        warm = true;
+    }
+    const int status = solver_solve_history(S, rhs_scratch, g, pressure, stats, warm);
+    if (status < 0 && status != VFLIP_ERR_BREAKDOWN) return status;
+    rc = apply_pressure(S, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, pressure);
+    if (rc) return rc;
+    return status;
+}
+}  // namespace
+
+namespace {
+// u -= grad p after a solve: the FP64 pressure in mixed mode, as the reference
+// (double pressure vector), else the FP32 one.
+int apply_pressure(vflip_solver* S, int variant, float h, float dt, float* u, float* v, float* w, const float* uw,
+                   const float* vw, const float* ww, const float* us, const float* vs, const float* ws,
+                   const float* pressure) {
+    int rc;
     int ni, nj, nk;
     float *d, *ax, *ay, *az, *lk;
     hipStream_t s;
@@ -458,8 +515,7 @@ int project_impl(vflip_solver* S, vflip_unet* net, int factor, int variant, floa
     } else {
         rc = vflip_grad_apply(ni, nj, nk, variant, h, dt, pressure, uw, vw, ww, us, vs, ws, u, v, w, s);
     }
-    if (rc) return rc;
-    return status;
+    return rc;
 }
 }  // namespace
 
@@ -475,6 +531,14 @@ int vflip_project_learned(vflip_solver* S, vflip_unet* net, int factor, int vari
                           vflip_solve_stats* stats) {
     VF_ARG(net, "network");
     return project_impl(S, net, factor, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs_scratch, pressure, stats);
+}
+
+int vflip_project_history(vflip_solver* S, vflip_unet* net, int factor, int variant, float h, float dt, float* u,
+                          float* v, float* w, const float* uw, const float* vw, const float* ww, const float* us,
+                          const float* vs, const float* ws, float* rhs_scratch, float* pressure,
+                          vflip_solve_stats* stats) {
+    return project_history_impl(S, net, factor, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs_scratch,
+                                pressure, stats);
 }
 
 }  // extern "C"

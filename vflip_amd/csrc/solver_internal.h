@@ -72,6 +72,13 @@ struct vflip_solver {
     hipGraphExec_t g_exec[2] = {nullptr, nullptr};
     unsigned long long g_key[2] = {0, 0};
     int g_np_last[2] = {0, 0};  // np_last after each sequence
+    // FP64 pressure history of vflip_project_history: hist[hist_head] is the last
+    // solution, hist[(hist_head + 2) % 3] the one before; hist_n of them are valid
+    double* hist_alloc = nullptr;
+    double* hist[3] = {nullptr, nullptr, nullptr};
+    double* hist_part = nullptr;  // Gram partial sums
+    double* hist_coef = nullptr;  // weights of p1, p2, p3
+    int hist_n = 0, hist_head = 0;
 };
 
 // Fused z-marching sweep geometry for a level. Rows of 128 or 256 cells use the

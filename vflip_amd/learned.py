@@ -135,3 +135,102 @@ def evaluate(net, seeds, n=N_TRAIN, factor=1, precision=api.MIXED, tol=1e-6):
 
 def relative_error(pred, p):
     return float((pred - p).norm() / p.norm())
+
+
+# ------------------------------------------------------------------ history correction
+def apply_op(d, ax, ay, az, x):
+    """y = A x for a 7-point operator (diag + couplings to +x/+y/+z, as vflip_solver_set_matrix),
+    in the dtype of x; arrays shaped (nk, nj, ni)."""
+    d, ax, ay, az = (a.to(x.dtype) for a in (d, ax, ay, az))
+    y = d * x
+    y[:, :, :-1] += ax[:, :, :-1] * x[:, :, 1:]
+    y[:, :, 1:] += ax[:, :, :-1] * x[:, :, :-1]
+    y[:, :-1, :] += ay[:, :-1, :] * x[:, 1:, :]
+    y[:, 1:, :] += ay[:, :-1, :] * x[:, :-1, :]
+    y[:-1, :, :] += az[:-1, :, :] * x[1:, :, :]
+    y[1:, :, :] += az[:-1, :, :] * x[:-1, :, :]
+    return y
+
+
+def _pool(a, f):
+    return torch.nn.functional.avg_pool3d(a[None, None], f)[0, 0]
+
+
+def generate_correction_pairs(seeds, n=256, factor=4, steps=12, first=3, log=None):
+    """Training pairs of the history correction: for each seed's plume (scene(seed) at n^3,
+    mixed solves warm-started from the pressure history), every step from `first` on gives
+    the divergence the history guess x0 leaves, r0 = b - A x0, and the error e = p - x0 of
+    that guess against the converged pressure p; both averaged by `factor` onto the network
+    grid, with the solid fraction and the hierarchy's Galerkin operator on that grid (the
+    norm of the energy loss). Returns device tensors [S, ...] like generate_pairs."""
+    level = int(round(math.log2(factor)))
+    m = n // factor
+    rows = {k: [] for k in ("rhs", "solid", "p", "ops")}
+    stats = []
+    t0 = time.time()
+    for seed in seeds:
+        phi, centre, radius, _ = scene(seed, n)
+        sim = PlumeSim(n, phi, dt=0.01, precision=api.MIXED, tol=1e-6, maxit=100, source_center_cfg=centre,
+                       source_r=radius, seed=seed)
+        sim.use_history(True)
+        for s in range(steps):
+            sim.advect()
+            sim.force()
+            if s >= first:
+                sim.solver.assemble(api.COARSE, float(sim.h), 1.0, sim.u, sim.v, sim.w, sim.uw, sim.vw, sim.ww,
+                                    sim.us, sim.vs, sim.ws, sim.rhs)
+                g, used = sim.solver.history_guess(sim.rhs)
+            st = sim.project()
+            if s < first:
+                continue
+            if st["status"] < 0 or not st["converged"]:
+                raise RuntimeError(f"seed {seed} step {s}: solve did not converge {st}")
+            d, ax, ay, az = sim.solver.get_level(0)
+            r0 = sim.rhs.double() - apply_op(d, ax, ay, az, g)
+            r0[d == 0] = 0
+            e = sim.p.double() - g
+            e[d == 0] = 0
+            rows["rhs"].append(_pool(r0.float(), factor))
+            rows["solid"].append(_pool((d == 0).float(), factor))
+            rows["p"].append(_pool(e.float(), factor))
+            rows["ops"].append(torch.stack(sim.solver.get_level(level)))
+            stats.append(dict(seed=seed, step=s, iterations=st["iterations"], history=used))
+        del sim
+        if log and len(stats) and (seeds.index(seed) + 1) % 20 == 0:
+            log(f"correction pairs: {len(stats)} from {seeds.index(seed) + 1} seeds in {time.time() - t0:.1f} s")
+    out = {k: torch.stack(v) for k, v in rows.items()}
+    assert out["rhs"].shape[1:] == (m, m, m)
+    out["liquid"] = 1.0 - out["solid"]
+    out["stats"] = stats
+    return out
+
+
+def evaluate_history(net, seeds, n=256, factor=4, steps=12, first=3):
+    """What the network gains over the pressure-history guess alone: each seed's plume run
+    `steps` steps twice (history only; history + network correction), both mixed to 1e-6;
+    per-seed mean iterations and solve ms over the steps from `first` on."""
+    rows = []
+    for seed in seeds:
+        phi, centre, radius, _ = scene(seed, n)
+        rec = {"seed": seed}
+        for name, use in (("history", False), ("learned", True)):
+            sim = PlumeSim(n, phi, dt=0.01, precision=api.MIXED, tol=1e-6, maxit=100, source_center_cfg=centre,
+                           source_r=radius, seed=seed)
+            if use:
+                sim.use_network(net, factor, history=True)
+            else:
+                sim.use_history(True)
+            its, ms, tres = [], [], []
+            for s in range(steps):
+                st = sim.step()
+                if s >= first:
+                    its.append(st["iterations"])
+                    ms.append(st["ms_solve"])
+                    tres.append(st["true_residual"])
+            torch.cuda.synchronize()
+            rec[f"its_{name}"] = sum(its) / len(its)
+            rec[f"ms_{name}"] = sum(ms) / len(ms)
+            rec[f"true_res_{name}"] = max(tres)
+            del sim
+        rows.append(rec)
+    return rows

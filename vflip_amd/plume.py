@@ -44,10 +44,18 @@ class PlumeSim:
         self.solver = api.Solver(n, n, n, precision=precision, tolerance=tol, max_iterations=maxit)
         self.last_stats = None
         self.net, self.net_factor = None, 1
+        self.history = False
 
-    def use_network(self, net, factor):
-        """Learned warm start: the U-Net (on the grid coarsened by factor) seeds every pressure solve."""
+    def use_network(self, net, factor, history=False):
+        """Learned warm start: the U-Net (on the grid coarsened by factor) seeds every pressure
+        solve; with history, it corrects the pressure-history extrapolation instead."""
         self.net, self.net_factor = net, factor
+        self.history = history
+
+    def use_history(self, on=True):
+        """Warm start from the pressure history alone (2 p1 - p2), no network."""
+        self.history = on
+        self.net = None
 
     def advect(self):
         h, dt = float(self.h), float(self.dt)
@@ -70,7 +78,9 @@ class PlumeSim:
     def project(self):
         args = (api.COARSE, float(self.h), 1.0, self.u, self.v, self.w, self.uw, self.vw, self.ww, self.us, self.vs,
                 self.ws, self.rhs, self.p)
-        if self.net is not None:
+        if self.history:
+            self.last_stats = self.solver.project_history(self.net, self.net_factor, *args)
+        elif self.net is not None:
             self.last_stats = self.solver.project_learned(self.net, self.net_factor, *args)
         else:
             self.last_stats = self.solver.project(*args)
