@@ -29,6 +29,38 @@ HIPCC_FLAGS = [
 ]
 
 
+FACADE_DIR = os.path.join(HERE, "facade")
+FACADE_SO = os.path.join(HERE, "libvflip_ivock.so")
+FACADE_CHECK = os.path.join(FACADE_DIR, "facade_check")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def build_facade(verbose=False):
+    """The C++ drop-in facade (include/vflip_ivock.hpp): host code over libvflip.so and the
+    HIP runtime, compiled with g++, plus its check program."""
+    cxx = shutil.which("g++") or "g++"
+    common = ["-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROCM, "include"),
+              "-D__HIP_PLATFORM_AMD__"]
+    links = ["-L", HERE, "-lvflip", "-L", os.path.join(ROCM, "lib"), "-lamdhip64",
+             "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    steps = [
+        [cxx, *common, "-fPIC", "-shared", "-o", FACADE_SO + ".tmp", os.path.join(FACADE_DIR, "ivock_facade.cpp"),
+         *links, "-Wl,-rpath,$ORIGIN"],
+        [cxx, *common, "-o", FACADE_CHECK + ".tmp", os.path.join(FACADE_DIR, "facade_check.cpp"), "-L", HERE,
+         "-lvflip_ivock", *links, "-Wl,-rpath,$ORIGIN/.."],
+    ]
+    for cmd, out in zip(steps, (FACADE_SO, FACADE_CHECK)):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("facade build failed:\n" + r.stdout)
+        if out == FACADE_SO:
+            os.replace(FACADE_SO + ".tmp", FACADE_SO)
+    os.replace(FACADE_CHECK + ".tmp", FACADE_CHECK)
+    return FACADE_SO
+
+
 def hipcc():
     for cand in (os.environ.get("HIPCC"), shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
         if cand and os.path.exists(cand):
@@ -60,6 +92,7 @@ def build(verbose=False, jobs=8):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout)
     os.replace(tmp, OUT)
+    build_facade(verbose)
     return OUT
 
 
