@@ -2,8 +2,9 @@
 
 * The guess is the energy-optimal combination of the last three FP64 pressures
   (Galerkin projection onto previous solutions): it equals numpy's FP64 solution of
-  the same 3 x 3 Gram system over the oracle's operator, and its energy is no
-  higher than that of the previous pressure or of the linear extrapolation.
+  the same 3 x 3 Gram system over the oracle's operator, its residual is orthogonal
+  to the history's span, and its energy is no higher than that of the previous
+  pressure or of the linear extrapolation.
 * History bookkeeping: a reset or a failed solve leaves no history (zero guess).
 * At 128^3 the history paths (with and without the network's correction) reach the
   FP64 true residual of every step and take fewer iterations than the zero-guess
@@ -58,10 +59,16 @@ def test_history_guess_is_energy_optimal(cuda):
     scale = abs(E(g_np))
     assert E(g) <= E(g_np) + 1e-9 * scale
     assert E(g) <= E(p1) and E(g) <= E(2 * p1 - p2)
-    # Galerkin orthogonality: the residual is A-orthogonal... i.e. orthogonal to the span
+    # the projection starts from this very guess (its reported initial residual is g's)
+    st = sim.project()
+    r0 = np.abs(b - A(g)).max() / np.abs(b).max()
+    print(f"guess residual: solve {st['guess_residual']:.3e}, numpy {r0:.3e}", flush=True)
+    assert abs(st["guess_residual"] - r0) <= 1e-3 * r0
+    # Galerkin orthogonality: the residual is orthogonal to the span
     r = b - A(g)
-    for v in V:
-        assert abs(np.sum(v * r)) <= 1e-6 * np.linalg.norm(v) * np.linalg.norm(b)
+    ortho = [abs(np.sum(v * r)) / (np.linalg.norm(v) * np.linalg.norm(b)) for v in V]
+    print("orthogonality", ortho, flush=True)
+    assert max(ortho) <= 1e-6
 
 
 def test_history_reset_and_failed_solve(cuda):

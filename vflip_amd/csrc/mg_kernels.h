@@ -1207,11 +1207,17 @@ __global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __
 // the plane below kept in registers; the terms are summed in the order
 // d x, -z, -y, -x, +x, +y, +z.
 constexpr int kTrueResKZ = 16;
+// With gram given it also accumulates the pressure history's new Gram row for x:
+// partial sums of x.Ax, p1.Ax and p2.Ax (p1 / p2 may be null) at gram[q * nblocks + block].
 static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op, const double* __restrict__ d64,
                                                             const double* __restrict__ b64,
                                                             const float* __restrict__ b32,
                                                             const double* __restrict__ x, float* __restrict__ r,
-                                                            double* __restrict__ partial) {
+                                                            double* __restrict__ partial,
+                                                            double* __restrict__ gram = nullptr,
+                                                            const double* __restrict__ p1 = nullptr,
+                                                            const double* __restrict__ p2 = nullptr) {
+    double g0 = 0, g1 = 0, g2 = 0;
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     const int kb = blockIdx.z * kTrueResKZ;
     const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
@@ -1233,6 +1239,11 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
             if (k + 1 < nk) y += (double)az0 * xp;
             const double rv = (b64 ? b64[c] : (double)b32[c]) - y;
             if (r) r[c] = (float)rv;
+            if (gram) {
+                g0 += x0 * y;
+                if (p1) g1 += p1[c] * y;
+                if (p2) g2 += p2[c] * y;
+            }
             m = fmax(m, fabs(rv));
             xm = x0;
             x0 = xp;
@@ -1240,8 +1251,17 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
         }
     }
     double t = block_max(m);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-        partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = t;
+    const unsigned blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blk] = t;
+    if (gram) {
+        const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+        const double s0 = block_sum(g0), s1 = block_sum(g1), s2 = block_sum(g2);
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            gram[blk] = s0;
+            gram[nb + blk] = s1;
+            gram[2 * nb + blk] = s2;
+        }
+    }
 }
 
 // Energy of an FP32 initial guess before its FP64 residual: partial sums of
@@ -1312,88 +1332,92 @@ static __global__ __launch_bounds__(kBlock) void k_cast_f32_f64(unsigned n, cons
 // Energy-optimal initial guess from the pressure history (the projection onto
 // previous solutions of P. F. Fischer, "Projection techniques for iterative
 // solution of Ax = b with successive right-hand sides", CMAME 163, 1998): the
-// x0 in span{p1, p2, p3} that minimises x'Ax/2 - b'x. The span is taken in the
-// difference basis v0 = p1, v1 = p1 - p2, v2 = p1 - 2 p2 + p3 (the last solutions
-// are nearly parallel; their differences keep the M x M Gram system well
-// conditioned). k_hist_gram forms the partial sums of G = V'AV (upper triangle)
-// and f = V'b per workgroup, with A the level-0 operator in FP64 arithmetic (FP64
-// diagonal when the assembly wrote it); 64 x 4 threads, kTrueResKZ planes each.
-// partial[q * nblocks + block], q: G00 G01 G02 G11 G12 G22 f0 f1 f2.
+// x0 in span{p1, p2, p3} that minimises x'Ax/2 - b'x, from the Gram matrix
+// G = P'AP (kept incrementally: each history solve's final true residual adds its
+// row) and f = P'b (one streaming pass per guess).
 constexpr int kHistSums = 9;
-template <int M>
-static __global__ __launch_bounds__(kBlock) void k_hist_gram(Op<float> op, const double* __restrict__ d64,
-                                                             const double* __restrict__ b64,
-                                                             const float* __restrict__ b32,
-                                                             const double* __restrict__ p1,
-                                                             const double* __restrict__ p2,
-                                                             const double* __restrict__ p3,
-                                                             double* __restrict__ partial) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-    const int kb = blockIdx.z * kTrueResKZ;
-    const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
-    const double* P[3] = {p1, p2, p3};
-    double acc[kHistSums] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < ni && j < nj && kb < nk) {
-        const int ke = min(kb + kTrueResKZ, nk);
-        for (int k = kb; k < ke; ++k) {
-            const int c = (k * nj + j) * ni + i;
-            const double dc = d64 ? d64[c] : (double)op.D(c);
-            double x[3], y[3];
-#pragma unroll
-            for (int m = 0; m < M; ++m) {
-                const double* v = P[m];
-                double t = dc * v[c];
-                if (k > 0) t += (double)op.AZ(c - sl) * v[c - sl];
-                if (j > 0) t += (double)op.AY(c - ni) * v[c - ni];
-                if (i > 0) t += (double)op.AX(c - 1) * v[c - 1];
-                if (i + 1 < ni) t += (double)op.AX(c) * v[c + 1];
-                if (j + 1 < nj) t += (double)op.AY(c) * v[c + ni];
-                if (k + 1 < nk) t += (double)op.AZ(c) * v[c + sl];
-                x[m] = v[c];
-                y[m] = t;
-            }
-            // difference basis
-            double vb[3] = {x[0], 0, 0}, ab[3] = {y[0], 0, 0};
-            if (M >= 2) { vb[1] = x[0] - x[1]; ab[1] = y[0] - y[1]; }
-            if (M >= 3) { vb[2] = x[0] - 2.0 * x[1] + x[2]; ab[2] = y[0] - 2.0 * y[1] + y[2]; }
-            const double bc = b64 ? b64[c] : (double)b32[c];
-            acc[0] += vb[0] * ab[0];
-            acc[6] += vb[0] * bc;
-            if (M >= 2) { acc[1] += vb[0] * ab[1]; acc[3] += vb[1] * ab[1]; acc[7] += vb[1] * bc; }
-            if (M >= 3) { acc[2] += vb[0] * ab[2]; acc[4] += vb[1] * ab[2]; acc[5] += vb[2] * ab[2]; acc[8] += vb[2] * bc; }
-        }
+
+// Commits a solve's Gram row: sums the partials of k_true_residual64 in a fixed order
+// into G (3 x 3 over history slots, FP64): G[s][s] = x.Ax, G[s][a] = G[a][s] = p_a.Ax,
+// likewise for b (a, b < 0: absent). One workgroup.
+static __global__ __launch_bounds__(kBlock) void k_hist_gram_commit(const double* __restrict__ part, int nb, int s,
+                                                                    int a, int b, double* __restrict__ G) {
+    double v[3];
+    for (int q = 0; q < 3; ++q) {
+        double t = 0;
+        for (int i = threadIdx.x; i < nb; i += kBlock) t += part[q * nb + i];
+        v[q] = block_sum(t);
     }
-    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+    if (threadIdx.x != 0) return;
+    G[3 * s + s] = v[0];
+    if (a >= 0) G[3 * s + a] = G[3 * a + s] = v[1];
+    if (b >= 0) G[3 * s + b] = G[3 * b + s] = v[2];
+}
+
+// Partial sums of f_i = p_i.b over the history entries (M of them), FP64; grid-stride,
+// partials at partial[i * gridDim.x + block].
+template <int M>
+static __global__ __launch_bounds__(kBlock) void k_hist_rhs_dots(unsigned n, const double* __restrict__ b64,
+                                                                 const float* __restrict__ b32,
+                                                                 const double* __restrict__ p1,
+                                                                 const double* __restrict__ p2,
+                                                                 const double* __restrict__ p3,
+                                                                 double* __restrict__ partial) {
+    double f[3] = {0, 0, 0};
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock) {
+        const double bc = b64 ? b64[c] : (double)b32[c];
+        f[0] += p1[c] * bc;
+        if (M >= 2) f[1] += p2[c] * bc;
+        if (M >= 3) f[2] += p3[c] * bc;
+    }
 #pragma unroll
-    for (int q = 0; q < kHistSums; ++q) {
-        const double t = block_sum(acc[q]);
-        if (threadIdx.x == 0 && threadIdx.y == 0) partial[q * nb + blk] = t;
+    for (int q = 0; q < M; ++q) {
+        const double t = block_sum(f[q]);
+        if (threadIdx.x == 0) partial[q * gridDim.x + blockIdx.x] = t;
     }
 }
 
-// One workgroup: sums the Gram partials in a fixed order, solves G c = f by
-// symmetric elimination (a basis vector whose pivot falls below 1e-13 of the first
-// diagonal is dropped: coefficient 0), and writes the weights of p1, p2, p3 of
-// x0 = c0 v0 + c1 v1 + c2 v2 to coef[0..2].
+// One workgroup: f from its partials (fixed order), G over the slots (sa, sb, sc) of
+// p1, p2, p3 from the Gram cache; both taken to the difference basis v0 = p1,
+// v1 = p1 - p2, v2 = p1 - 2 p2 + p3 (exact FP64 combinations: the last solutions are
+// nearly parallel, their differences keep the system well conditioned); G c = f
+// solved by symmetric elimination (a basis vector whose pivot falls below 1e-13 of
+// the first diagonal is dropped: coefficient 0); coef[0..2] = the weights of p1, p2,
+// p3 in x0 = c0 v0 + c1 v1 + c2 v2.
 static __global__ __launch_bounds__(kBlock) void k_hist_solve(const double* __restrict__ partial, int nb, int M,
+                                                              const double* __restrict__ Gc, int sa, int sb, int sc,
                                                               double* __restrict__ coef) {
-    __shared__ double sums[kHistSums];
-    for (int q = 0; q < kHistSums; ++q) {
+    __shared__ double fs[3];
+    for (int q = 0; q < 3; ++q) {
         double s = 0;
-        for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[q * nb + b];
+        if (q < M)
+            for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[q * nb + b];
         s = block_sum(s);
-        if (threadIdx.x == 0) sums[q] = s;
+        if (threadIdx.x == 0) fs[q] = s;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
-    double G[3][3] = {{sums[0], sums[1], sums[2]}, {sums[1], sums[3], sums[4]}, {sums[2], sums[4], sums[5]}};
-    double f[3] = {sums[6], sums[7], sums[8]};
+    const int sl[3] = {sa, sb, sc};
+    double Gp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < M; ++j) Gp[i][j] = Gc[3 * sl[i] + sl[j]];
+    // T: columns are v0, v1, v2 in the p basis
+    const double T[3][3] = {{1, 1, 1}, {0, -1, -2}, {0, 0, 1}};
+    double G[3][3], f[3];
+    for (int k = 0; k < 3; ++k) {
+        f[k] = 0;
+        for (int i = 0; i < M; ++i) f[k] += T[i][k] * fs[i];
+        for (int l = 0; l < 3; ++l) {
+            double t = 0;
+            for (int i = 0; i < M; ++i)
+                for (int j = 0; j < M; ++j) t += T[i][k] * Gp[i][j] * T[j][l];
+            G[k][l] = t;
+        }
+    }
     double c[3] = {0, 0, 0};
     bool keep[3] = {false, false, false};
     const double ref = G[0][0];
     if (ref > 0) {
-        // forward elimination on the kept vectors
         for (int k = 0; k < M; ++k) {
             if (!(G[k][k] > 1e-13 * ref)) continue;
             keep[k] = true;

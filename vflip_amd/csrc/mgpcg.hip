@@ -504,7 +504,7 @@ int vflip_solver_destroy(vflip_solver* S) {
         if (l.vec_alloc) (void)hipFree(l.vec_alloc);
     }
     if (S->pcg_alloc) (void)hipFree(S->pcg_alloc);
-    if (S->x64) (void)hipFree(S->x64);
+    if (S->f64_alloc) (void)hipFree(S->f64_alloc);
     if (S->hist_alloc) (void)hipFree(S->hist_alloc);
     if (S->ctab) (void)hipFree(S->ctab);
     if (S->cpart) (void)hipFree(S->cpart);
@@ -706,6 +706,9 @@ int vflip_solver_setup(vflip_solver* S) {
     }
     const bool first = !S->setup_done;
     S->check_changed = false;
+    // a new operator: the pressure history (and its Gram matrix) no longer applies
+    S->hist_n = 0;
+    S->hist_pending = false;
     VF_HIP(hipEventRecord(S->e0, S->stream));
     for (size_t l = 0; l + 1 < S->L.size(); ++l) {
         LevelBuf& f = S->L[l];
@@ -839,19 +842,24 @@ int vflip_solver_matvec(vflip_solver* S, const float* x, float* y) {
 namespace {
 
 int ensure_f64(vflip_solver* S) {
-    if (S->x64) return 0;
-    int rc = alloc((void**)&S->x64, 3 * S->n * sizeof(double));
+    if (S->f64_alloc) return 0;
+    int rc = alloc((void**)&S->f64_alloc, 3 * S->n * sizeof(double));
     if (rc) return rc;
+    S->x64 = S->f64_alloc;
     S->d64 = S->x64 + S->n;
     S->b64 = S->d64 + S->n;
     return 0;
 }
 
-void launch_true_residual64(vflip_solver* S, const double* b64, const float* b32, const double* x, float* r) {
+// gram: also accumulate the history Gram row of x (vflip_project_history solves)
+void launch_true_residual64(vflip_solver* S, const double* b64, const float* b32, const double* x, float* r,
+                            bool gram = false) {
     const LevelBuf& l0 = S->L[0];
     const dim3 g((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
     k_true_residual64<<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr, b64, b32,
-                                                         x, r, S->partial);
+                                                         x, r, S->partial, gram ? S->hist_part : nullptr,
+                                                         gram ? S->gram_p[0] : nullptr, gram ? S->gram_p[1] : nullptr);
+    if (gram) S->gram_nb = nblocks(g);
     S->np_last = nblocks(g);
 }
 
@@ -867,6 +875,12 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
                vflip_solve_stats* st) {
     int rc;
     float ms_setup = 0.f;
+    // any other solve overwrites x64: a pending history entry (the last history solve's
+    // result) is dropped, and with it the oldest entry, whose Gram row it took over
+    if (!S->hist_solve && S->hist_pending) {
+        S->hist_pending = false;
+        S->hist_n = std::min(S->hist_n, 2);
+    }
     // an assembly since the last setup may have changed the operator (setup
     // compares it and keeps the hierarchy when it did not)
     if (!S->setup_done || S->check_changed) {
@@ -910,8 +924,12 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
         // rejected: the zero start below (the same as after the FP64 vetting)
     } else if (S->opts.warm_start) {
         // initial guess from x32 (or x64out when given)
-        if (x64out) VF_HIP(hipMemcpyAsync(S->x64, x64out, n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
-        else k_cast_f32_f64<<<gn, kBlock, 0, S->stream>>>(n, x32, S->x64);
+        if (x64out) {
+            if (x64out != S->x64)
+                VF_HIP(hipMemcpyAsync(S->x64, x64out, n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+        } else {
+            k_cast_f32_f64<<<gn, kBlock, 0, S->stream>>>(n, x32, S->x64);
+        }
         if (!mx) k_cast_f64_f32<<<gn, kBlock, 0, S->stream>>>(n, S->x64, S->xw);
         launch_true_residual64(S, mx ? rhs64 : nullptr, rhs32, S->x64, S->r);
         VF_CHECK_LAUNCH();
@@ -973,7 +991,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
                     const bool periodic = S->opts.refine_every > 0 && ((it + 1) % S->opts.refine_every) == 0;
                     if (periodic || res <= tol_abs) {
                         // residual replacement with the FP64 true residual
-                        launch_true_residual64(S, rhs64, rhs32, S->x64, S->r);
+                        launch_true_residual64(S, rhs64, rhs32, S->x64, S->r, S->hist_solve);
                         VF_CHECK_LAUNCH();
                         if ((rc = finalize(S, 2))) return rc;
                         if ((rc = fetch_scalars(S))) return rc;
@@ -1011,7 +1029,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
         true_res = res;
         VF_HIP(hipEventSynchronize(S->e1));
     } else {
-        launch_true_residual64(S, rhs64, rhs32, S->x64, nullptr);
+        launch_true_residual64(S, rhs64, rhs32, S->x64, nullptr, S->hist_solve);
         VF_CHECK_LAUNCH();
         if ((rc = finalize(S, 5))) return rc;
         if ((rc = fetch_scalars(S))) return rc;
@@ -1100,6 +1118,7 @@ int vflip_solver_history_guess(vflip_solver* S, const float* rhs, double* guess)
 int vflip_solver_reset_history(vflip_solver* S) {
     VF_ARG(S, "solver");
     S->hist_n = 0;
+    S->hist_pending = false;
     return 0;
 }
 
@@ -1225,45 +1244,51 @@ int solver_solve_assembled(vflip_solver* S, const float* rhs32, float* x32, vfli
 }
 // Before an assembly that rewrites level 0: clears the device change flag; the
 // next setup only rebuilds the hierarchy when the assembly set it.
-// The initial guess of vflip_project_history in the free history slot: zero with
-// no history, else the energy-optimal combination of the last (up to three)
-// pressures (k_hist_gram / k_hist_solve / k_hist_combine; FP64, no host sync).
+// The initial guess of vflip_project_history, written into the solver's FP64 iterate:
+// zero with no history, else the energy-optimal combination of the last (up to
+// three) pressures: f = P'b in one streaming pass, G from the Gram cache, the 3 x 3
+// solve and the FP64 combination on the GPU, no host sync. A pending solution (the
+// last solve's, still in x64) is committed first by swapping x64 with the oldest slot.
 // rhs32: the assembled right-hand side (its FP64 form is used when valid).
 // *used receives how many history entries the guess rests on.
 int solver_history_guess(vflip_solver* S, const float* rhs32, double** guess, int* used) {
-    const LevelBuf& l0 = S->L[0];
-    const dim3 gg((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
-    const int nb = nblocks(gg);
+    int rc = ensure_f64(S);
+    if (rc) return rc;
     if (!S->hist_alloc) {
-        const size_t words = 3 * S->n + (size_t)kHistSums * nb + 4;
-        int rc = alloc((void**)&S->hist_alloc, words * sizeof(double));
+        const LevelBuf& l0 = S->L[0];
+        const int nb3 = ((l0.ni + 63) / 64) * ((l0.nj + 3) / 4) * ((l0.nk + kTrueResKZ - 1) / kTrueResKZ);
+        S->hist_part_n = std::max(S->vblocks, nb3);
+        const size_t words = 3 * S->n + 3 * (size_t)S->hist_part_n + 9 + 4;
+        rc = alloc((void**)&S->hist_alloc, words * sizeof(double));
         if (rc) return rc;
         for (int q = 0; q < 3; ++q) S->hist[q] = S->hist_alloc + q * S->n;
         S->hist_part = S->hist_alloc + 3 * S->n;
-        S->hist_coef = S->hist_part + (size_t)kHistSums * nb;
+        S->hist_G = S->hist_part + 3 * (size_t)S->hist_part_n;
+        S->hist_coef = S->hist_G + 9;
         S->hist_n = 0;
+        S->hist_pending = false;
     }
-    int rc = ensure_f64(S);
-    if (rc) return rc;
+    if (S->hist_pending) {
+        const int slot = (S->hist_head + 1) % 3;  // the oldest entry, dropped
+        std::swap(S->x64, S->hist[slot]);
+        S->hist_head = slot;
+        S->hist_n = std::min(S->hist_n + 1, 3);
+        S->hist_pending = false;
+    }
     const unsigned n = (unsigned)S->n;
-    double* g = S->hist[(S->hist_head + 1) % 3];
+    double* g = S->x64;
     const int M = std::min(S->hist_n, 3);
     if (M == 0) {
         k_fill64<<<S->vblocks, kBlock, 0, S->stream>>>(n, g, 0.0);
     } else {
-        const double* p1 = S->hist[S->hist_head];
-        const double* p2 = M >= 2 ? S->hist[(S->hist_head + 2) % 3] : nullptr;
-        const double* p3 = M >= 3 ? S->hist[(S->hist_head + 1) % 3] : nullptr;  // the oldest: g's slot
-        const double* d64 = S->d64_valid ? S->d64 : nullptr;
+        const int a = S->hist_head, b = (S->hist_head + 2) % 3, c = (S->hist_head + 1) % 3;
         const double* b64 = (mixed(S) && S->b64_valid) ? S->b64 : nullptr;
-        const Op<float> op = make_op<float>(l0);
-        if (M == 1)
-            k_hist_gram<1><<<gg, dim3(64, 4), 0, S->stream>>>(op, d64, b64, rhs32, p1, p2, p3, S->hist_part);
-        else if (M == 2)
-            k_hist_gram<2><<<gg, dim3(64, 4), 0, S->stream>>>(op, d64, b64, rhs32, p1, p2, p3, S->hist_part);
-        else
-            k_hist_gram<3><<<gg, dim3(64, 4), 0, S->stream>>>(op, d64, b64, rhs32, p1, p2, p3, S->hist_part);
-        k_hist_solve<<<1, kBlock, 0, S->stream>>>(S->hist_part, nb, M, S->hist_coef);
+        const double *p1 = S->hist[a], *p2 = M >= 2 ? S->hist[b] : nullptr, *p3 = M >= 3 ? S->hist[c] : nullptr;
+        const int nb = S->vblocks;
+        if (M == 1) k_hist_rhs_dots<1><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+        else if (M == 2) k_hist_rhs_dots<2><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+        else k_hist_rhs_dots<3><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+        k_hist_solve<<<1, kBlock, 0, S->stream>>>(S->hist_part, nb, M, S->hist_G, a, b, c, S->hist_coef);
         k_hist_combine<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->hist_coef, p1, p2, p3, g);
     }
     VF_CHECK_LAUNCH();
@@ -1289,22 +1314,35 @@ int solver_add_correction(vflip_solver* S, double* g, const float* corr) {
     VF_CHECK_LAUNCH();
     return 0;
 }
-// Solve the assembled system from the guess in the free history slot (warm start;
-// a guess that is no better than zero in the A-norm is dropped by the solve), then
-// make the solution the newest history entry. x32 receives the FP32 pressure.
+// Solve the assembled system from the guess in x64 (warm start; a guess that is no
+// better than zero in the A-norm is dropped by the solve). The solution stays in x64,
+// pending, and its final true residual adds its Gram row (against the two newest
+// committed entries) in the slot that the next guess commits it to. x32 receives
+// the FP32 pressure.
 int solver_solve_history(vflip_solver* S, const float* rhs32, double* g, float* x32, vflip_solve_stats* st,
                          bool warm) {
     const double* b64 = (mixed(S) && S->b64_valid) ? S->b64 : nullptr;
     const int old = S->opts.warm_start;
+    const int slot = (S->hist_head + 1) % 3;
+    const int M = std::min(S->hist_n, 3);
+    const int a = M >= 1 ? S->hist_head : -1, b = M >= 2 ? (S->hist_head + 2) % 3 : -1;
     S->opts.warm_start = warm ? 1 : 0;
+    S->hist_solve = true;
+    S->gram_p[0] = a >= 0 ? S->hist[a] : nullptr;
+    S->gram_p[1] = b >= 0 ? S->hist[b] : nullptr;
+    S->gram_nb = 0;
     const int status = solve_impl(S, rhs32, b64, x32, g, st);
+    S->hist_solve = false;
+    S->gram_p[0] = S->gram_p[1] = nullptr;
     S->opts.warm_start = old;
     if (status < 0 && status != VFLIP_ERR_BREAKDOWN) return status;
-    if (status == VFLIP_OK) {
-        S->hist_head = (S->hist_head + 1) % 3;
-        S->hist_n = std::min(S->hist_n + 1, 3);
+    if (status == VFLIP_OK && S->gram_nb > 0) {
+        k_hist_gram_commit<<<1, kBlock, 0, S->stream>>>(S->hist_part, S->gram_nb, slot, a, b, S->hist_G);
+        VF_CHECK_LAUNCH();
+        S->hist_pending = true;
     } else {
         S->hist_n = 0;  // no trusted history after a failed solve
+        S->hist_pending = false;
     }
     return status;
 }

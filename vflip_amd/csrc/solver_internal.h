@@ -74,11 +74,21 @@ struct vflip_solver {
     int g_np_last[2] = {0, 0};  // np_last after each sequence
     // FP64 pressure history of vflip_project_history: hist[hist_head] is the last
     // solution, hist[(hist_head + 2) % 3] the one before; hist_n of them are valid
+    // solutions; a solve's result stays pending in x64 until the next guess commits it by
+    // swapping buffers (x64 <-> the oldest slot). hist_G: Gram matrix p_a'A p_b over the
+    // slots, FP64, one row added per history solve (by its final true residual).
     double* hist_alloc = nullptr;
     double* hist[3] = {nullptr, nullptr, nullptr};
-    double* hist_part = nullptr;  // Gram partial sums
+    double* hist_part = nullptr;  // partial sums (Gram row, right-hand-side dots)
+    int hist_part_n = 0;          // its capacity per sum
+    double* hist_G = nullptr;     // 3 x 3
     double* hist_coef = nullptr;  // weights of p1, p2, p3
     int hist_n = 0, hist_head = 0;
+    bool hist_pending = false;          // x64 holds a solution not yet committed to the history
+    bool hist_solve = false;            // set during a history solve: the final true residual
+    const double* gram_p[2] = {nullptr, nullptr};  // accumulates the Gram row against these
+    int gram_nb = 0;                    // workgroups of that true-residual launch
+    double* f64_alloc = nullptr;        // base of the x64 / d64 / b64 allocation
 };
 
 // Fused z-marching sweep geometry for a level. Rows of 128 or 256 cells use the
