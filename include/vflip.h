@@ -250,6 +250,16 @@ int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const 
                      const float* w, const float* field, int fnx, int fny, int fnz, float fox, float foy,
                      float foz, float* out, const float* field2, float* out2, void* hip_stream);
 
+/* The smoke step's five advections in one launch: u, v, w (MAC faces) and the
+ * cell-centred rho and T, each by vflip_advect_rk2 mode 2 with the MAC velocity
+ * u, v, w (S/sub_domain.cpp:191-235 in FP32; the coarse grid's cubic form is
+ * mode 1 of vflip_advect_rk2). Bit-identical with the five mode-2 calls, in one
+ * launch (each thread the five outputs of a pair of cells). Outputs must not alias
+ * inputs. Asynchronous on hip_stream. */
+int vflip_advect_plume(int ni, int nj, int nk, float h, float dt, const float* u, const float* v, const float* w,
+                       const float* rho, const float* temp, float* u2, float* v2, float* w2, float* rho2, float* temp2,
+                       void* hip_stream);
+
 /* Smoke-plume forcing of the synthetic workload (replaces the reference's
  * gen_heat/add_smokeforce, S/Smoke_solver3D.cpp:1624-1668, with the workload's
  * sphere source and face buoyancy): cells with |x - c| <= r get

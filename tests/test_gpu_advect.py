@@ -114,3 +114,30 @@ def test_advect_fp32_mode_close_to_oracle(cuda, n):
             # the sample by <= 1.2e-7 * n cells; on these rough fields (values change by up
             # to 2 max|f| per cell) that bounds the cell error by 2.4e-7 * n * max|f|
             assert np.abs(g - ref).max() <= 2.5e-7 * n * np.abs(src).max()
+
+
+@pytest.mark.parametrize("n,h", [(40, None), (64, None), (48, 1.0 / 48)])
+def test_fused_plume_advection_bitexact(cuda, n, h):
+    """vflip_advect_plume equals the five mode-2 vflip_advect_rk2 calls bit for bit: with a
+    power-of-two h (lattice first stage) and without, velocities large enough that some
+    backtraces leave the staged tile (global fallback), ragged tile edges."""
+    import torch
+    from vflip_amd import api
+    print(f"fused plume advection n={n}", flush=True)
+    nk, nj, ni = n - 3, n - 6, n  # ragged against the 32 x 8 x 4 tiles
+    hh = 1.0 / 64 if h is None else h
+    g = torch.Generator(device="cuda").manual_seed(n)
+    mk = lambda *s: torch.rand(s, device="cuda", generator=g) * 2 - 1
+    u, v, w = mk(nk, nj, ni + 1) * 2.0, mk(nk, nj + 1, ni) * 2.0, mk(nk + 1, nj, ni) * 2.0
+    rho, temp = mk(nk, nj, ni), mk(nk, nj, ni)
+    dt = 0.02  # |u| dt / h up to ~2.6 cells: beyond the 2-cell halo
+    ref = [torch.empty_like(a) for a in (u, v, w, rho, temp)]
+    api.advect_rk2(hh, dt, u, v, w, u, (0.5, 0, 0), ref[0], mode=2)
+    api.advect_rk2(hh, dt, u, v, w, v, (0, 0.5, 0), ref[1], mode=2)
+    api.advect_rk2(hh, dt, u, v, w, w, (0, 0, 0.5), ref[2], mode=2)
+    api.advect_rk2(hh, dt, u, v, w, rho, (0, 0, 0), ref[3], mode=2, field2=temp, out2=ref[4])
+    got = [torch.empty_like(a) for a in (u, v, w, rho, temp)]
+    api.advect_plume(hh, dt, u, v, w, rho, temp, *got)
+    torch.cuda.synchronize()
+    for a, b in zip(ref, got):
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32))

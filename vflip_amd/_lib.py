@@ -119,6 +119,7 @@ SIGNATURES = {
     "vflip_project": (c_int, [P, c_int, c_float, c_float] + [P] * 11 + [ctypes.POINTER(SolveStats)]),
     "vflip_advect_rk2": (c_int, [c_int, c_int, c_int, c_float, c_float, c_int, P, P, P, P, c_int, c_int, c_int,
                                  c_float, c_float, c_float, P, P, P, P]),
+    "vflip_advect_plume": (c_int, [c_int, c_int, c_int, c_float, c_float] + [P] * 10 + [P]),
     "vflip_plume_force": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float, c_float, c_float, c_float,
                                   ctypes.c_uint, c_float, c_float, c_float, P, P, P, P]),
     "vflip_particles_create": (c_int, [c_int, c_int, c_int, c_float, ctypes.POINTER(c_void_p)]),

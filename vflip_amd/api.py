@@ -300,6 +300,16 @@ def advect_rk2(h, dt, u, v, w, field, offs, out, mode=0, field2=None, out2=None)
           "advect_rk2")
 
 
+def advect_plume(h, dt, u, v, w, rho, temp, u2, v2, w2, rho2, temp2):
+    """The smoke step's advection of u, v, w, rho and T in one launch (mode 2 arithmetic)."""
+    nk, nj, ni = rho.shape
+    faces = _mac_ptrs(ni, nj, nk, (u, v, w, u2, v2, w2), ("u", "v", "w", "u2", "v2", "w2"))
+    cells = [_ptr(t, numel=ni * nj * nk, name=nm) for t, nm in ((rho, "rho"), (temp, "temp"), (rho2, "rho2"),
+                                                                (temp2, "temp2"))]
+    check(lib().vflip_advect_plume(ni, nj, nk, h, dt, *faces[:3], cells[0], cells[1], *faces[3:], cells[2], cells[3],
+                                   _stream()), "advect_plume")
+
+
 def plume_force(h, dt, center, r, seed, jitter, alpha, beta, rho, temp, v):
     nk, nj, ni = rho.shape
     check(lib().vflip_plume_force(ni, nj, nk, h, dt, center[0], center[1], center[2], r, seed, jitter, alpha, beta,

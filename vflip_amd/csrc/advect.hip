@@ -12,6 +12,7 @@
 #pragma clang fp contract(off)
 
 #include <cmath>
+#include <cstdlib>
 
 #include "common.h"
 #include "sampling.h"
@@ -206,6 +207,69 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int n
     }
 }
 
+// ----------------------------------------------------------------------------
+// The plume step's five advections in one launch (mode 2 arithmetic): each thread
+// a pair of cells of one column (planes k, k+1), producing their u, v, w faces and
+// their rho, T centres (the faces i = ni, j = nj, k = nk come from the last cell
+// of each row). Every value is the one k_advect_rk2_f32 computes (same samplers).
+// ----------------------------------------------------------------------------
+// RK2 backtrace of the sample at (x, y, z): k_advect_rk2_f32's stages (FIELD as
+// first_stage; LATTICE: its lattice form, valid for a power-of-two h).
+template <int FIELD, bool LATTICE>
+__device__ inline void backtrace(const GridF& gu, const GridF& gv, const GridF& gw, float h, float inv_h, float dt,
+                                 float x, float y, float z, float& px, float& py, float& pz) {
+    float u, v, w;
+    if constexpr (LATTICE) first_stage<FIELD>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
+    else first_stage<-1>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
+    const float hdt = 0.5f * dt;
+    const float mx = __builtin_fmaf(-hdt, u, x), my = __builtin_fmaf(-hdt, v, y), mz = __builtin_fmaf(-hdt, w, z);
+    u = sample_linear_f(gu, h, inv_h, mx, my, mz);
+    v = sample_linear_f(gv, h, inv_h, mx, my, mz);
+    w = sample_linear_f(gw, h, inv_h, mx, my, mz);
+    px = x - dt * u;
+    py = y - dt * v;
+    pz = z - dt * w;
+}
+
+template <bool LATTICE>
+__global__ __launch_bounds__(256) void k_advect_plume(int ni, int nj, int nk, float h, float inv_h, float dt,
+                                                      const float* __restrict__ u, const float* __restrict__ v,
+                                                      const float* __restrict__ w, const float* __restrict__ rho,
+                                                      const float* __restrict__ temp, float* __restrict__ u2,
+                                                      float* __restrict__ v2, float* __restrict__ w2,
+                                                      float* __restrict__ rho2, float* __restrict__ temp2) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, kb = blockIdx.z * 2;
+    if (i >= ni || j >= nj) return;
+    const GridF gu{u, ni + 1, nj, nk, 0.5f, 0.f, 0.f}, gv{v, ni, nj + 1, nk, 0.f, 0.5f, 0.f},
+        gw{w, ni, nj, nk + 1, 0.f, 0.f, 0.5f};
+    const GridF gr{rho, ni, nj, nk, 0.f, 0.f, 0.f}, gt{temp, ni, nj, nk, 0.f, 0.f, 0.f};
+    for (int dk = 0; dk < 2; ++dk) {
+        const int k = kb + dk;
+        if (k >= nk) break;
+        const float x = (float)i * h, y = (float)j * h, z = (float)k * h;
+        float px, py, pz;
+        backtrace<3, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, z, px, py, pz);
+        const int c = (k * nj + j) * ni + i;
+        rho2[c] = sample_linear_f(gr, h, inv_h, px, py, pz);
+        temp2[c] = sample_linear_f(gt, h, inv_h, px, py, pz);
+        for (int q = 0; q < 1 + (i == ni - 1); ++q) {
+            const int fi = i + q;
+            backtrace<0, LATTICE>(gu, gv, gw, h, inv_h, dt, ((float)fi - 0.5f) * h, y, z, px, py, pz);
+            u2[(k * nj + j) * (ni + 1) + fi] = sample_linear_f(gu, h, inv_h, px, py, pz);
+        }
+        for (int q = 0; q < 1 + (j == nj - 1); ++q) {
+            const int fj = j + q;
+            backtrace<1, LATTICE>(gu, gv, gw, h, inv_h, dt, x, ((float)fj - 0.5f) * h, z, px, py, pz);
+            v2[(k * (nj + 1) + fj) * ni + i] = sample_linear_f(gv, h, inv_h, px, py, pz);
+        }
+        for (int q = 0; q < 1 + (k == nk - 1); ++q) {
+            const int fk = k + q;
+            backtrace<2, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, ((float)fk - 0.5f) * h, px, py, pz);
+            w2[(fk * nj + j) * ni + i] = sample_linear_f(gw, h, inv_h, px, py, pz);
+        }
+    }
+}
+
 // Plume forcing on the reference-convention grid (cell centre (i,j,k)h):
 //   source sphere: rho = T = 1 + jitter(seed, cell) inside |x - c| <= r
 //   buoyancy on v faces: v += dt * (-alpha rho_f + beta T_f), rho_f = average of
@@ -279,6 +343,27 @@ int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const 
     Grid g1{field2 ? field2 : field, fnx, fny, fnz, fox, foy, foz};
     dim3 grid((fnx + 63) / 64, (fny + 3) / 4, fnz);
     k_advect_rk2<<<grid, dim3(64, 4), 0, (hipStream_t)stream>>>(ni, nj, nk, h, dt, mode, u, v, w, g0, out, g1, out2);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+int vflip_advect_plume(int ni, int nj, int nk, float h, float dt, const float* u, const float* v, const float* w,
+                       const float* rho, const float* temp, float* u2, float* v2, float* w2, float* rho2, float* temp2,
+                       void* stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0, "grid dims");
+    VF_ARG(u && v && w && rho && temp && u2 && v2 && w2 && rho2 && temp2, "advection arrays");
+    VF_ARG(u2 != u && v2 != v && w2 != w && rho2 != rho && temp2 != temp, "outputs must not alias the inputs");
+    VF_ARG((size_t)(ni + 1) * (nj + 1) * (nk + 1) < (1u << 31), "grid size");
+    int e2 = 0;
+    const bool pow2 = std::frexp(h, &e2) == 0.5f;
+    const dim3 grid((ni + 63) / 64, (nj + 3) / 4, (nk + 1) / 2);
+    hipStream_t st = (hipStream_t)stream;
+    if (pow2)
+        k_advect_plume<true><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, 1.0f / h, dt, u, v, w, rho, temp, u2, v2, w2,
+                                                           rho2, temp2);
+    else
+        k_advect_plume<false><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, 1.0f / h, dt, u, v, w, rho, temp, u2, v2,
+                                                            w2, rho2, temp2);
     VF_CHECK_LAUNCH();
     return 0;
 }

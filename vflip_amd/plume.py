@@ -2,7 +2,8 @@
 
 One step, on the reference's MAC grid (cell (i,j,k) centre at (i,j,k)h):
   1. RK2 semi-Lagrangian advection of u, v, w, rho and T by the current
-     velocity (SubDomainSolver::advect_field semantics, S/sub_domain.cpp:191-235);
+     velocity (SubDomainSolver::advect_field semantics, S/sub_domain.cpp:191-235),
+     one launch for the five fields;
   2. sphere source (rho = T = 1 + jitter) and face buoyancy
      v += dt(-alpha rho + beta T);
   3. pressure projection with static cut-cell face weights
@@ -23,6 +24,7 @@ class PlumeSim:
         self.h = 1.0 / n
         self.dt = dt
         self.mode = mode
+        self.fused = True  # mode 2: the five advections in one launch (vflip_advect_plume)
         dev = torch.device("cuda")
         f32 = dict(device=dev, dtype=torch.float32)
         self.u = torch.zeros((n, n, n + 1), **f32)
@@ -59,16 +61,23 @@ class PlumeSim:
 
     def advect(self):
         h, dt = float(self.h), float(self.dt)
-        api.advect_rk2(h, dt, self.u, self.v, self.w, self.u, (0.5, 0.0, 0.0), self.u2, mode=self.mode)
-        api.advect_rk2(h, dt, self.u, self.v, self.w, self.v, (0.0, 0.5, 0.0), self.v2, mode=self.mode)
-        api.advect_rk2(h, dt, self.u, self.v, self.w, self.w, (0.0, 0.0, 0.5), self.w2, mode=self.mode)
-        api.advect_rk2(h, dt, self.u, self.v, self.w, self.rho, (0.0, 0.0, 0.0), self.rho2, mode=self.mode,
-                       field2=self.temp, out2=self.temp2)
+        if self.mode == 2 and self.fused:
+            api.advect_plume(h, dt, self.u, self.v, self.w, self.rho, self.temp, self.u2, self.v2, self.w2, self.rho2,
+                             self.temp2)
+        else:
+            self._advect_separate(h, dt)
         self.u, self.u2 = self.u2, self.u
         self.v, self.v2 = self.v2, self.v
         self.w, self.w2 = self.w2, self.w
         self.rho, self.rho2 = self.rho2, self.rho
         self.temp, self.temp2 = self.temp2, self.temp
+
+    def _advect_separate(self, h, dt):
+        api.advect_rk2(h, dt, self.u, self.v, self.w, self.u, (0.5, 0.0, 0.0), self.u2, mode=self.mode)
+        api.advect_rk2(h, dt, self.u, self.v, self.w, self.v, (0.0, 0.5, 0.0), self.v2, mode=self.mode)
+        api.advect_rk2(h, dt, self.u, self.v, self.w, self.w, (0.0, 0.0, 0.5), self.w2, mode=self.mode)
+        api.advect_rk2(h, dt, self.u, self.v, self.w, self.rho, (0.0, 0.0, 0.0), self.rho2, mode=self.mode,
+                       field2=self.temp, out2=self.temp2)
 
     def force(self):
         c, r = self.source
