@@ -1,12 +1,13 @@
 """The 64^3 plume (one sphere obstacle) run for 10 steps from its initial state
-on the GPU, for the plain, mixed and learned + mixed projections, against the
+on the GPU, for the plain, mixed and pressure-history projections, against the
 FP64 CPU oracle run for the same 10 steps (AMGPCG to 1e-6 each step).
 
 Every step's solve stops at a 1e-6 relative residual, so the fields differ by
 what that tolerance and FP32 storage leave: measured on MI355X p 1.1e-6 and u
 2.0e-7 relative L2 (plain). The bounds below are 10x those figures. Plain
-iteration counts equal the oracle's; the learned guess saves one iteration at
-64^3 (the network's own grid).
+iteration counts equal the oracle's; the history guess takes fewer iterations
+than the zero guess. (The learned correction is trained for 256^3 solves; its
+test is at the configurations' grids, tests/test_gpu_configs.py.)
 """
 import numpy as np
 import pytest
@@ -18,7 +19,7 @@ def test_plume_trajectory_64(cuda):
     print("plume trajectory 64^3 x 10 steps", flush=True)
     import bench
     phi, spheres = bench.make_scene(64)
-    out = bench.trajectory(64, 10, phi, spheres)
+    out = bench.trajectory(64, 10, phi, spheres, paths=("plain", "mixed", "mixed_history"))
     paths = out["paths"]
     assert set(paths) >= {"plain", "mixed"}
     for name, r in paths.items():
@@ -27,5 +28,4 @@ def test_plume_trajectory_64(cuda):
         for k in ("u", "v", "w", "rho"):
             assert r[f"{k}_rel_l2"] <= 2e-6, (name, k)
     assert paths["plain"]["iterations"] == out["oracle_iterations"]
-    if "learned_mixed" in paths:
-        assert np.mean(paths["learned_mixed"]["iterations"]) < np.mean(paths["mixed"]["iterations"])
+    assert np.mean(paths["mixed_history"]["iterations"]) < np.mean(paths["mixed"]["iterations"])
