@@ -294,6 +294,19 @@ int vflip_particles_set_stream(vflip_particles* P, void* hip_stream);
 int vflip_hash(vflip_particles* P, int np, const float* pos, int clamp, float bx, float by, float bz,
                int* cell_key, int* cell_start, int* order, int* nbinned);
 
+/* vflip_hash without a workspace object and without any allocation. No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, kernels and bit-exact results as vflip_hash on an ni x nj x nk grid of
+ * spacing h; the binning's scratch (slots, merge buffer, scan partials) is one
+ * static device array per GPU (about 34 MB, held until the process ends). Limits:
+ * np <= 4194304 particles and ni*nj*nk <= 16777216 cells; larger inputs return
+ * VFLIP_ERR_ARG before any launch. Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams or threads) race. Asynchronous on
+ * hip_stream. */
+int vflip_hash_noalloc(int ni, int nj, int nk, float h, int np, const float* pos, int clamp, float bx, float by,
+                       float bz, int* cell_key, int* cell_start, int* order, void* hip_stream);
+
 /* particle_to_grid — S/Smoke_solver3D.cpp:1451-1510 (VFLIP_VARIANT_COARSE) and
  * S/sub_domain.cpp:1204-1265 (VFLIP_VARIANT_SUB: positions relative to bmin;
  * faces within 4 cells of the boundary use a 4h kernel over 9^3 cells,

@@ -126,6 +126,8 @@ SIGNATURES = {
     "vflip_particles_destroy": (c_int, [P]),
     "vflip_particles_set_stream": (c_int, [P, P]),
     "vflip_hash": (c_int, [P, c_int, P, c_int, c_float, c_float, c_float, P, P, P, PI]),
+    "vflip_hash_noalloc": (c_int, [c_int, c_int, c_int, c_float, c_int, P, c_int, c_float, c_float, c_float, P, P, P,
+                                   P]),
     "vflip_p2g": (c_int, [P, c_int, c_int, c_int, P, P, P, P, c_float, c_float, c_float] + [P] * 6),
     "vflip_get_delta": (c_int, [ctypes.c_size_t, P, P, P, P]),
     "vflip_g2p": (c_int, [c_int, c_int, c_int, c_float, c_int, P, P] + [P] * 6 + [c_float] * 4 + [P]),

@@ -15,6 +15,7 @@
 #pragma clang fp contract(off)
 
 #include <climits>
+#include <mutex>
 
 #include "common.h"
 #include "flip_internal.h"
@@ -1375,6 +1376,61 @@ int vflip_particles_set_stream(vflip_particles* P, void* hip_stream) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// The binning's launch sequence over caller-provided scratch: slot and tmp hold np
+// ints, big 2 + np / (kShortSegment + 1), partial ceil(ncell / kScanChunk).
+int hash_launches(hipStream_t s, const BinGeom& g, size_t nc, int np, const float* pos, int* cell_key,
+                  int* cell_start, int* order, int* slot, int* tmp, int* big, int* partial, unsigned npartial) {
+    k_fill_i32<<<min(grid_for(nc + 1), 4096u), kBlock, 0, s>>>(cell_start, nc + 1, 0);
+    k_fill_i32<<<1, kBlock, 0, s>>>(big, 1, 0);
+    if (np > 0) k_bin_count<<<grid_for(np), kBlock, 0, s>>>(np, pos, g, cell_key, slot, cell_start);
+    k_scan_reduce<<<npartial, kBlock, 0, s>>>(nc, cell_start, partial);
+    k_scan_partials<<<1, 1024, 0, s>>>((int)npartial, partial, cell_start + nc);
+    k_scan_apply<<<npartial, kBlock, 0, s>>>(nc, cell_start, partial);
+    if (np > 0) {
+        k_bin_scatter<<<grid_for(np), kBlock, 0, s>>>(np, cell_key, slot, cell_start, order);
+        k_cell_fix<<<grid_for(nc), kBlock, 0, s>>>(nc, cell_start, order, big);
+        k_sort_long<<<256, kBlock, 0, s>>>(cell_start, order, tmp, big);
+    }
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+
+// Scratch of vflip_hash_noalloc: one array per device, as large as the stated limits
+// need (about 34 MB); its device address is looked up once per device.
+constexpr int kNoAllocMaxParticles = 1 << 22;
+constexpr int kNoAllocMaxCells = 1 << 24;
+constexpr size_t kNoAllocBig = (size_t)kNoAllocMaxParticles / (kShortSegment + 1) + 2;
+constexpr size_t kNoAllocPartials = ((size_t)kNoAllocMaxCells + kScanChunk - 1) / kScanChunk;
+constexpr size_t kNoAllocWords = 2 * (size_t)kNoAllocMaxParticles + kNoAllocBig + kNoAllocPartials;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) int scratch[kNoAllocWords];
+
+namespace {
+// The calling thread's current device's copy of `scratch` (looked up on its first use).
+int scratch_ptr(int** out) {
+    int dev = -1;
+    VF_HIP(hipGetDevice(&dev));
+    VF_ARG(dev >= 0 && dev < kMaxDevices, "device index");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        VF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (int*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int vflip_hash(vflip_particles* P, int np, const float* pos, int clamp, float bx, float by, float bz, int* cell_key,
                int* cell_start, int* order, int* nbinned) {
     VF_ARG(P && np >= 0 && cell_start, "hash args");
@@ -1383,24 +1439,32 @@ int vflip_hash(vflip_particles* P, int np, const float* pos, int clamp, float bx
     hipStream_t s = P->stream;
     const size_t nc = P->ncell;
     const BinGeom g{P->h, P->nx, P->ny, P->nz, clamp ? 1 : 0, bx, by, bz};
-    k_fill_i32<<<min(grid_for(nc + 1), 4096u), kBlock, 0, s>>>(cell_start, nc + 1, 0);
-    k_fill_i32<<<1, kBlock, 0, s>>>(P->big, 1, 0);
-    if (np > 0) k_bin_count<<<grid_for(np), kBlock, 0, s>>>(np, pos, g, cell_key, P->slot, cell_start);
-    const unsigned nb = (unsigned)P->npartial;
-    k_scan_reduce<<<nb, kBlock, 0, s>>>(nc, cell_start, P->partial);
-    k_scan_partials<<<1, 1024, 0, s>>>((int)nb, P->partial, cell_start + nc);
-    k_scan_apply<<<nb, kBlock, 0, s>>>(nc, cell_start, P->partial);
-    if (np > 0) {
-        k_bin_scatter<<<grid_for(np), kBlock, 0, s>>>(np, cell_key, P->slot, cell_start, order);
-        k_cell_fix<<<grid_for(nc), kBlock, 0, s>>>(nc, cell_start, order, P->big);
-        k_sort_long<<<256, kBlock, 0, s>>>(cell_start, order, P->tmp, P->big);
-    }
-    VF_CHECK_LAUNCH();
+    if (int rc = hash_launches(s, g, nc, np, pos, cell_key, cell_start, order, P->slot, P->tmp, P->big, P->partial,
+                               (unsigned)P->npartial))
+        return rc;
     if (nbinned) {
         VF_HIP(hipMemcpyAsync(nbinned, cell_start + nc, sizeof(int), hipMemcpyDeviceToHost, s));
         VF_HIP(hipStreamSynchronize(s));
     }
     return 0;
+}
+
+int vflip_hash_noalloc(int ni, int nj, int nk, float h, int np, const float* pos, int clamp, float bx, float by,
+                       float bz, int* cell_key, int* cell_start, int* order, void* hip_stream) {
+    VF_ARG(ni > 0 && nj > 0 && nk > 0 && h > 0, "grid dims and spacing");
+    VF_ARG(np >= 0 && np <= kNoAllocMaxParticles, "particle count beyond the no-alloc limit of 4194304");
+    VF_ARG((double)ni * nj * nk <= (double)kNoAllocMaxCells, "cell count beyond the no-alloc limit of 16777216");
+    VF_ARG(cell_start && (np == 0 || (pos && cell_key && order)), "hash arrays");
+    int* base = nullptr;
+    if (int rc = scratch_ptr(&base)) return rc;
+    int* slot = base;
+    int* tmp = slot + kNoAllocMaxParticles;
+    int* big = tmp + kNoAllocMaxParticles;
+    int* partial = big + kNoAllocBig;
+    const size_t nc = (size_t)ni * nj * nk;
+    const BinGeom g{h, ni, nj, nk, clamp ? 1 : 0, bx, by, bz};
+    return hash_launches((hipStream_t)hip_stream, g, nc, np, pos, cell_key, cell_start, order, slot, tmp, big, partial,
+                         (unsigned)((nc + kScanChunk - 1) / kScanChunk));
 }
 
 int vflip_p2g(vflip_particles* P, int variant, int method, int np, const float* pos, const float* vel,
