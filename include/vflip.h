@@ -632,6 +632,49 @@ int vflip_solver_history_guess(vflip_solver* S, const float* rhs, double* guess)
 /* Forgets the pressure history (a new scene or a jump in time). */
 int vflip_solver_reset_history(vflip_solver* S);
 
+/* ---- Geometric multigrid on a uniform grid: MultiGridSolver3D<T> -----------
+ * S/Multigrid3D.h:152-184 (class), 1438-1439 (MGSolverf / MGSolverd). The
+ * constant-coefficient 7-point operator (A x)(c) = sum of the six neighbours -
+ * 6 x(c), a neighbour past the grid clamped to the cell itself (Neumann) or 0
+ * (dirichlet != 0, the reference's _b_Dirichlet). Arrays are device arrays of
+ * the chosen type, linear x-fastest (the reference's Buffer3D bricks convert
+ * with vflip_to_bricks / vflip_from_bricks for float). Results are
+ * bit-identical with the reference's arithmetic in that type. Calls enqueue on
+ * `stream` and return after the work completes. */
+typedef enum { VFLIP_MG3D_FLOAT = 0, VFLIP_MG3D_DOUBLE = 1 } vflip_mg3d_type;
+typedef struct vflip_mg3d vflip_mg3d;
+/* Constructor + m_InitialSystem (S/Multigrid3D.h:139-150, levels 100-137): every
+ * extent halves (floor, at least 1) until the grid has at most 512 cells; the
+ * level buffers (x, b of the coarse levels, x_new of every level) are allocated
+ * here. One object serves one stream at a time. */
+int vflip_mg3d_create(int gridx, int gridy, int gridz, int type, int dirichlet, vflip_mg3d** out);
+int vflip_mg3d_destroy(vflip_mg3d* m);
+/* m_max_level and systemk: the level count and, when dims is not NULL, the
+ * extents of every level (3 ints per level, finest first). */
+int vflip_mg3d_levels(const vflip_mg3d* m, int* levels, int* dims);
+/* m_Vcycle(x, b, tol, residual, 0) (S/Multigrid3D.h:1328-1392): one V-cycle on
+ * x in place (5 + 5 red-black sweeps per level, 100 at the bottom). The
+ * reference's tol and residual are unused there and have no counterpart. */
+int vflip_mg3d_vcycle(vflip_mg3d* m, void* x, const void* b, void* stream);
+/* m_FullMultiGrid(x, b, tol, residual) (S/Multigrid3D.h:1394-1436). */
+int vflip_mg3d_full_multigrid(vflip_mg3d* m, void* x, const void* b, void* stream);
+/* m_Red_Black_Gauss(system, b, x, x_new, iter_time, omega) (S/Multigrid3D.h:840-1129)
+ * on an nx x ny x nz grid: iter_time sweeps (parity-1 cells, then parity-0) on
+ * x in place; x_new is scratch of the same size (its contents are overwritten). */
+int vflip_mg3d_rbgs(int nx, int ny, int nz, int type, int dirichlet, const void* b, void* x, void* x_new,
+                    int iter_time, double omega, void* stream);
+/* m_ComputeResidual(system, res, b, x) (S/Multigrid3D.h:1130-1186): res = b - A x. */
+int vflip_mg3d_residual(int nx, int ny, int nz, int type, int dirichlet, void* res, const void* b, const void* x,
+                        void* stream);
+/* m_Restrict (S/Multigrid3D.h:1193-1245): dst (the coarser grid, extents
+ * max(n/2, 1)) = 0.5 * the sum of the 8 children in src (nx x ny x nz). */
+int vflip_mg3d_restrict(int nx, int ny, int nz, int type, const void* src, void* dst, void* stream);
+/* m_Prolongate (S/Multigrid3D.h:1298-1326): dst (nx x ny x nz) += its parent in
+ * src (the coarser grid). A parent past the coarser grid (odd extents) adds 0;
+ * the reference reads Buffer3D padding there, which is 0 unless the coarser
+ * extent is a multiple of 8 (it then reads a cell of the next brick). */
+int vflip_mg3d_prolongate(int nx, int ny, int nz, int type, const void* src, void* dst, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -169,6 +169,72 @@ bool AMGPCGSolve(const SparseMatrix<T>& A, const std::vector<T>& rhs, std::vecto
     return rc == VFLIP_OK;  // 1 = not converged, <0 = error (vflip_last_error())
 }
 
+// ---------------------------------------------------- MultiGridSolver3D
+// The geometric multigrid of S/Multigrid3D.h:12-184 (uniform_grid_descriptor_3D,
+// MultiGrid<T>, MultiGridSolver3D<T>, MGSolverf / MGSolverd) over vflip_mg3d_*:
+// the same levels, members and methods; each method uploads the Buffer3D fields it
+// reads, runs the gfx950 kernels and downloads what it writes (bit-identical with
+// the reference's arithmetic in T). m_Vcycle / m_FullMultiGrid take tol and residual
+// and leave them unused, as the reference does. The open-boundary far-field members
+// (m_applyOpenBoundaryCondition, m_FastSummation, m_BarnesHutSummation, m_Toplevel,
+// m_bottomlevel, m_cumulate_poles, m_MonopleToMonopole) are not provided.
+struct uniform_grid_descriptor_3D {
+    int gridx, gridy, gridz, system_size;
+    uniform_grid_descriptor_3D() : gridx(0), gridy(0), gridz(0), system_size(0) {}
+    uniform_grid_descriptor_3D(int x, int y, int z) : gridx(x), gridy(y), gridz(z), system_size(x * y * z) {}
+    void set_grid_information(int x, int y, int z) {
+        gridx = x; gridy = y; gridz = z;
+        system_size = x * y * z;
+    }
+};
+
+template <class T> class MultiGrid {
+public:
+    MultiGrid() : m_bInitialized(false), m_max_level(0) {}
+    virtual ~MultiGrid() { m_FinalMemoryEachLevel(); }
+    void m_InitialSystem(int gridx, int gridy, int gridz);
+    void m_SetGridInformation(int x_, int y_, int z_) { m_system_descriptor.set_grid_information(x_, y_, z_); }
+    void m_AssignMemoryEachLevel();
+    void m_FinalMemoryEachLevel();
+    void m_ComputeLevels();
+    bool m_bInitialized;
+    uniform_grid_descriptor_3D m_system_descriptor;
+    int m_max_level;
+    std::vector<Buffer3D<T>*> xk;
+    std::vector<Buffer3D<T>*> xk_new;
+    std::vector<Buffer3D<T>*> bk;
+    std::vector<Buffer3D<T>*> rk;
+    std::vector<uniform_grid_descriptor_3D> systemk;
+
+protected:
+    std::vector<Buffer3D<T>*> owned_;  // level buffers this object allocated
+};
+
+struct MGDevice;  // device copies and vflip_mg3d handles (facade implementation)
+
+template <class T> class MultiGridSolver3D : public MultiGrid<T> {
+public:
+    MultiGridSolver3D() : _b_Dirichlet(false) {}
+    ~MultiGridSolver3D();
+    bool _b_Dirichlet;
+    void m_Vcycle(Buffer3D<T>* x, Buffer3D<T>* b, T tol, T& residual, int level);
+    void m_FullMultiGrid(Buffer3D<T>* x, Buffer3D<T>* b, T tol, T& residual);
+    void m_Red_Black_Gauss(uniform_grid_descriptor_3D& system, Buffer3D<T>& b, Buffer3D<T>& x, Buffer3D<T>& x_new,
+                           int iter_time, double omega);
+    void m_Restrict(uniform_grid_descriptor_3D& next_level, uniform_grid_descriptor_3D& curr_level,
+                    Buffer3D<T>& src_level, Buffer3D<T>& dst_level);
+    void m_ExactSolve();
+    void m_Prolongate(uniform_grid_descriptor_3D& next_level, uniform_grid_descriptor_3D& curr_level,
+                      Buffer3D<T>& src_level, Buffer3D<T>& dst_level);
+    void m_ComputeResidual(uniform_grid_descriptor_3D& system, Buffer3D<T>& res, Buffer3D<T>& b, Buffer3D<T>& x);
+
+private:
+    MGDevice* dev_ = nullptr;
+    MGDevice& dev();
+};
+typedef MultiGridSolver3D<float> MGSolverf;
+typedef MultiGridSolver3D<double> MGSolverd;
+
 struct DeviceState;  // device fields and vflip handles of one solver (facade implementation)
 
 // --------------------------------------------------------- SubDomainSolver

@@ -102,6 +102,14 @@ def lib():
         L.or_inflow_bc.argtypes = [c_int, c_int, c_int, np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS"),
                                    c_float, f32p, f32p, f32p]
         L.or_grid_points.argtypes = [c_int, c_int, c_int, c_float, c_float, c_float, c_float, f32p]
+        vp = ctypes.c_void_p
+        L.or_mg3d_levels.restype = c_int
+        L.or_mg3d_levels.argtypes = [c_int, c_int, c_int, vp]
+        L.or_mg3d_rbgs.argtypes = [c_int] * 5 + [vp, vp, c_int, c_double]
+        L.or_mg3d_residual.argtypes = [c_int] * 5 + [vp] * 3
+        L.or_mg3d_restrict.argtypes = [c_int] * 4 + [vp] * 2
+        L.or_mg3d_prolongate.argtypes = [c_int] * 4 + [vp] * 2
+        L.or_mg3d_solve.argtypes = [c_int] * 5 + [vp, vp, c_int, c_int]
         _LIB = L
     return _LIB
 
@@ -477,3 +485,73 @@ def get_delta(new, old):
     out = np.zeros_like(c32(new))
     lib().or_get_delta(out.size, c32(new), c32(old), out)
     return out
+
+
+# --------------------------------------------------------------------------- geometric multigrid
+# MultiGridSolver3D<T> (S/Multigrid3D.h), arrays (nz, ny, nx) of float32 (MGSolverf) or
+# float64 (MGSolverd); the restatement is oracle/vflip_oracle.cpp, namespace mg3d.
+def _mg_type(a):
+    if a.dtype == np.float32:
+        return 0
+    if a.dtype == np.float64:
+        return 1
+    raise ValueError("float32 or float64 arrays")
+
+
+def _vp(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def mg3d_levels(nx, ny, nz):
+    """[(nx, ny, nz)] per level (MultiGrid::m_ComputeLevels / m_AssignMemoryEachLevel)."""
+    n = lib().or_mg3d_levels(nx, ny, nz, None)
+    d = np.zeros(3 * n, np.int32)
+    lib().or_mg3d_levels(nx, ny, nz, _vp(d))
+    return [tuple(int(v) for v in d[3 * l:3 * l + 3]) for l in range(n)]
+
+
+def mg3d_rbgs(b, x, iters, omega=1.0, dirichlet=False):
+    """m_Red_Black_Gauss: returns the smoothed copy of x."""
+    x = np.ascontiguousarray(x).copy()
+    b = np.ascontiguousarray(b, dtype=x.dtype)
+    nz, ny, nx = x.shape
+    lib().or_mg3d_rbgs(nx, ny, nz, _mg_type(x), int(dirichlet), _vp(b), _vp(x), int(iters), float(omega))
+    return x
+
+
+def mg3d_residual(b, x, dirichlet=False):
+    x = np.ascontiguousarray(x)
+    b = np.ascontiguousarray(b, dtype=x.dtype)
+    res = np.empty_like(x)
+    nz, ny, nx = x.shape
+    lib().or_mg3d_residual(nx, ny, nz, _mg_type(x), int(dirichlet), _vp(res), _vp(b), _vp(x))
+    return res
+
+
+def mg3d_restrict(src):
+    src = np.ascontiguousarray(src)
+    nz, ny, nx = src.shape
+    dst = np.empty((max(nz // 2, 1), max(ny // 2, 1), max(nx // 2, 1)), src.dtype)
+    lib().or_mg3d_restrict(nx, ny, nz, _mg_type(src), _vp(src), _vp(dst))
+    return dst
+
+
+def mg3d_prolongate(src, dst):
+    """m_Prolongate: returns dst + the parents' values (dst is not modified)."""
+    dst = np.ascontiguousarray(dst).copy()
+    src = np.ascontiguousarray(src, dtype=dst.dtype)
+    nz, ny, nx = dst.shape
+    assert src.shape == (max(nz // 2, 1), max(ny // 2, 1), max(nx // 2, 1))
+    lib().or_mg3d_prolongate(nx, ny, nz, _mg_type(dst), _vp(src), _vp(dst))
+    return dst
+
+
+def mg3d_solve(b, x, cycles=1, fmg=False, dirichlet=False):
+    """One solver object: m_FullMultiGrid first when fmg, then `cycles` m_Vcycle calls;
+    returns the new x."""
+    x = np.ascontiguousarray(x).copy()
+    b = np.ascontiguousarray(b, dtype=x.dtype)
+    nz, ny, nx = x.shape
+    lib().or_mg3d_solve(nx, ny, nz, _mg_type(x), int(dirichlet), _vp(x), _vp(b), int(cycles), int(bool(fmg)))
+    return x

@@ -1553,3 +1553,251 @@ void or_tracer_frame(int n, const float* tr, float lx, float* out) {
     }
 }
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Geometric multigrid, MultiGridSolver3D<T> (S/Multigrid3D.h), T = float or
+// double, on linear x-fastest arrays. A read past a grid that the reference
+// makes into its Buffer3D brick padding (zeroed, never written) reads 0 here.
+namespace mg3d {
+
+struct Dims {
+    int x, y, z;
+    size_t n() const { return (size_t)x * y * z; }
+};
+
+// MultiGrid::m_ComputeLevels + m_AssignMemoryEachLevel (S/Multigrid3D.h:100-137)
+std::vector<Dims> levels(int gx, int gy, int gz) {
+    int level = 0, x = gx, y = gy, z = gz;
+    while ((size_t)x * y * z > 512) {
+        level = level + 1;
+        x = x / 2; if (x == 0) x = 1;
+        y = y / 2; if (y == 0) y = 1;
+        z = z / 2; if (z == 0) z = 1;
+    }
+    std::vector<Dims> s(level + 1);
+    s[0] = {gx, gy, gz};
+    for (int i = 1; i <= level; ++i)
+        s[i] = {(s[i - 1].x / 2) == 0 ? 1 : s[i - 1].x / 2, (s[i - 1].y / 2) == 0 ? 1 : s[i - 1].y / 2,
+                (s[i - 1].z / 2) == 0 ? 1 : s[i - 1].z / 2};
+    return s;
+}
+
+// The six neighbour values of (i,j,k): indexes clamped into the grid, zeroed at
+// the walls with Dirichlet (S/Multigrid3D.h:1040-1062).
+template <class T>
+inline void neighbours(const Dims& g, bool dir, const T* x, unsigned i, unsigned j, unsigned k, T& lv, T& rv, T& tv,
+                       T& dv, T& fv, T& bv) {
+    unsigned li = (i == 0) ? i : i - 1;
+    unsigned ri = (i == (unsigned)g.x - 1) ? i : i + 1;
+    unsigned ti = (j == (unsigned)g.y - 1) ? j : j + 1;
+    unsigned di = (j == 0) ? j : j - 1;
+    unsigned fi = (k == 0) ? k : k - 1;
+    unsigned bi = (k == (unsigned)g.z - 1) ? k : k + 1;
+    lv = x[lin(li, j, k, g.x, g.y)];
+    rv = x[lin(ri, j, k, g.x, g.y)];
+    tv = x[lin(i, ti, k, g.x, g.y)];
+    dv = x[lin(i, di, k, g.x, g.y)];
+    fv = x[lin(i, j, fi, g.x, g.y)];
+    bv = x[lin(i, j, bi, g.x, g.y)];
+    if (dir) {
+        if (i == 0) lv = 0;
+        if (i == (unsigned)g.x - 1) rv = 0;
+        if (j == (unsigned)g.y - 1) tv = 0;
+        if (j == 0) dv = 0;
+        if (k == 0) fv = 0;
+        if (k == (unsigned)g.z - 1) bv = 0;
+    }
+}
+
+// m_Red_Black_Gauss (S/Multigrid3D.h:840-1129): the parity-1 pass, then the
+// parity-0 pass, each in place.
+template <class T>
+void rbgs(const Dims& g, bool dir, const T* b, T* x, int iter_time, double omega) {
+    for (int it = 0; it < iter_time; ++it)
+        for (int parity = 1; parity >= 0; --parity)
+            for (unsigned k = 0; k < (unsigned)g.z; ++k)
+                for (unsigned j = 0; j < (unsigned)g.y; ++j)
+                    for (unsigned i = 0; i < (unsigned)g.x; ++i) {
+                        if ((int)((j + k + i) % 2) != parity) continue;
+                        const size_t c = lin(i, j, k, g.x, g.y);
+                        T rhs = b[c];
+                        T lv, rv, tv, dv, fv, bv;
+                        neighbours(g, dir, x, i, j, k, lv, rv, tv, dv, fv, bv);
+                        T rr = lv + rv;
+                        rr += tv + dv;
+                        rr += fv + bv;
+                        rr -= rhs;
+                        x[c] = (1 - omega) * x[c] + omega * rr / 6.0;
+                    }
+}
+
+// m_ComputeResidual (S/Multigrid3D.h:1130-1186)
+template <class T> void residual(const Dims& g, bool dir, T* res, const T* b, const T* x) {
+    for (unsigned k = 0; k < (unsigned)g.z; ++k)
+        for (unsigned j = 0; j < (unsigned)g.y; ++j)
+            for (unsigned i = 0; i < (unsigned)g.x; ++i) {
+                const size_t c = lin(i, j, k, g.x, g.y);
+                T rhs = b[c];
+                T center = x[c];
+                T lv, rv, tv, dv, fv, bv;
+                neighbours(g, dir, x, i, j, k, lv, rv, tv, dv, fv, bv);
+                T rr = lv + rv;
+                rr += tv + dv;
+                rr += fv + bv;
+                rr -= 6 * center;
+                res[c] = rhs - rr;
+            }
+}
+
+template <class T> inline T read0(const Dims& g, const T* a, unsigned i, unsigned j, unsigned k) {
+    return (i < (unsigned)g.x && j < (unsigned)g.y && k < (unsigned)g.z) ? a[lin(i, j, k, g.x, g.y)] : T(0);
+}
+
+// m_Restrict (S/Multigrid3D.h:1193-1245)
+template <class T> void restrict_(const Dims& cg, const Dims& fg, const T* src, T* dst) {
+    for (unsigned k = 0; k < (unsigned)cg.z; ++k)
+        for (unsigned j = 0; j < (unsigned)cg.y; ++j)
+            for (unsigned i = 0; i < (unsigned)cg.x; ++i) {
+                unsigned i_fine = i * 2, j_fine = j * 2, k_fine = k * 2;
+                T r = 0;
+                r += read0(fg, src, i_fine, j_fine, k_fine);
+                r += read0(fg, src, i_fine + 1, j_fine, k_fine);
+                r += read0(fg, src, i_fine, j_fine + 1, k_fine);
+                r += read0(fg, src, i_fine + 1, j_fine + 1, k_fine);
+                r += read0(fg, src, i_fine, j_fine, k_fine + 1);
+                r += read0(fg, src, i_fine + 1, j_fine, k_fine + 1);
+                r += read0(fg, src, i_fine, j_fine + 1, k_fine + 1);
+                r += read0(fg, src, i_fine + 1, j_fine + 1, k_fine + 1);
+                r *= 0.5;
+                dst[lin(i, j, k, cg.x, cg.y)] = r;
+            }
+}
+
+// m_Prolongate (S/Multigrid3D.h:1298-1326)
+template <class T> void prolongate(const Dims& fg, const Dims& cg, const T* src, T* dst) {
+    for (unsigned k = 0; k < (unsigned)fg.z; ++k)
+        for (unsigned j = 0; j < (unsigned)fg.y; ++j)
+            for (unsigned i = 0; i < (unsigned)fg.x; ++i) {
+                T r = read0(cg, src, i / 2, j / 2, k / 2);
+                dst[lin(i, j, k, fg.x, fg.y)] += r;
+            }
+}
+
+template <class T> struct Solver {
+    std::vector<Dims> sys;
+    bool dir;
+    std::vector<std::vector<T>> xs, bs, rs;  // level storage (levels >= 1 for x and b)
+    std::vector<T*> xk, bk;
+
+    Solver(int gx, int gy, int gz, bool d) : sys(levels(gx, gy, gz)), dir(d) {
+        const size_t L = sys.size();
+        xs.resize(L); bs.resize(L); rs.resize(L);
+        xk.assign(L, nullptr); bk.assign(L, nullptr);
+        for (size_t i = 0; i < L; ++i) {
+            rs[i].assign(sys[i].n(), T(0));
+            if (i > 0) {
+                xs[i].assign(sys[i].n(), T(0));
+                bs[i].assign(sys[i].n(), T(0));
+                xk[i] = xs[i].data();
+                bk[i] = bs[i].data();
+            }
+        }
+    }
+    int max_level() const { return (int)sys.size(); }
+    void zero_from(int level) {
+        for (int i = level + 1; i < max_level(); ++i) {
+            std::fill(xs[i].begin(), xs[i].end(), T(0));
+            std::fill(bs[i].begin(), bs[i].end(), T(0));
+            std::fill(rs[i].begin(), rs[i].end(), T(0));
+        }
+    }
+    // m_ExactSolve (S/Multigrid3D.h:1187-1191)
+    void exact() { rbgs(sys[max_level() - 1], dir, bk[max_level() - 1], xk[max_level() - 1], 100, 1.0); }
+
+    // m_Vcycle (S/Multigrid3D.h:1328-1392)
+    void vcycle(T* x, const T* b, int level) {
+        xk[level] = x;
+        bk[level] = const_cast<T*>(b);
+        zero_from(level);
+        std::fill(rs[level].begin(), rs[level].end(), T(0));
+        for (int i = level; i < max_level() - 1; ++i) {
+            rbgs(sys[i], dir, bk[i], xk[i], 5, 1.0);
+            residual(sys[i], dir, rs[i].data(), bk[i], xk[i]);
+            restrict_(sys[i + 1], sys[i], rs[i].data(), bk[i + 1]);
+        }
+        exact();
+        for (int i = max_level() - 2; i >= level; --i) {
+            prolongate(sys[i], sys[i + 1], xk[i + 1], xk[i]);
+            rbgs(sys[i], dir, bk[i], xk[i], 5, 1.0);
+        }
+        zero_from(level);
+    }
+
+    // m_FullMultiGrid (S/Multigrid3D.h:1394-1436)
+    void fmg(T* x, const T* b) {
+        xk[0] = x;
+        bk[0] = const_cast<T*>(b);
+        zero_from(0);
+        std::fill(rs[0].begin(), rs[0].end(), T(0));
+        for (int i = 0; i < max_level() - 1; ++i) {
+            residual(sys[i], dir, rs[i].data(), bk[i], xk[i]);
+            restrict_(sys[i + 1], sys[i], rs[i].data(), bk[i + 1]);
+        }
+        exact();
+        for (int i = max_level() - 2; i >= 0; --i) {
+            prolongate(sys[i], sys[i + 1], xk[i + 1], xk[i]);
+            vcycle(xk[i], bk[i], i);
+        }
+        vcycle(xk[0], b, 0);
+    }
+};
+
+template <class T> void vcycle_run(int gx, int gy, int gz, int dir, T* x, const T* b, int cycles, int fmg) {
+    Solver<T> S(gx, gy, gz, dir != 0);
+    if (fmg) S.fmg(x, b);
+    for (int c = 0; c < cycles; ++c) S.vcycle(x, b, 0);
+}
+
+}  // namespace mg3d
+
+extern "C" {
+
+// Level extents, 3 per level; returns the level count (dims may be NULL).
+int or_mg3d_levels(int gx, int gy, int gz, int* dims) {
+    auto s = mg3d::levels(gx, gy, gz);
+    if (dims)
+        for (size_t l = 0; l < s.size(); ++l) {
+            dims[3 * l] = s[l].x; dims[3 * l + 1] = s[l].y; dims[3 * l + 2] = s[l].z;
+        }
+    return (int)s.size();
+}
+// type 0 = float, 1 = double (MGSolverf / MGSolverd)
+void or_mg3d_rbgs(int nx, int ny, int nz, int type, int dir, const void* b, void* x, int iters, double omega) {
+    const mg3d::Dims g{nx, ny, nz};
+    if (type) mg3d::rbgs(g, dir != 0, (const double*)b, (double*)x, iters, omega);
+    else mg3d::rbgs(g, dir != 0, (const float*)b, (float*)x, iters, omega);
+}
+void or_mg3d_residual(int nx, int ny, int nz, int type, int dir, void* res, const void* b, const void* x) {
+    const mg3d::Dims g{nx, ny, nz};
+    if (type) mg3d::residual(g, dir != 0, (double*)res, (const double*)b, (const double*)x);
+    else mg3d::residual(g, dir != 0, (float*)res, (const float*)b, (const float*)x);
+}
+void or_mg3d_restrict(int nx, int ny, int nz, int type, const void* src, void* dst) {
+    const mg3d::Dims fg{nx, ny, nz};
+    const mg3d::Dims cg{std::max(nx / 2, 1), std::max(ny / 2, 1), std::max(nz / 2, 1)};
+    if (type) mg3d::restrict_(cg, fg, (const double*)src, (double*)dst);
+    else mg3d::restrict_(cg, fg, (const float*)src, (float*)dst);
+}
+void or_mg3d_prolongate(int nx, int ny, int nz, int type, const void* src, void* dst) {
+    const mg3d::Dims fg{nx, ny, nz};
+    const mg3d::Dims cg{std::max(nx / 2, 1), std::max(ny / 2, 1), std::max(nz / 2, 1)};
+    if (type) mg3d::prolongate(fg, cg, (const double*)src, (double*)dst);
+    else mg3d::prolongate(fg, cg, (const float*)src, (float*)dst);
+}
+// fmg != 0: one m_FullMultiGrid first; then `cycles` m_Vcycle calls, one solver object.
+void or_mg3d_solve(int nx, int ny, int nz, int type, int dir, void* x, const void* b, int cycles, int fmg) {
+    if (type) mg3d::vcycle_run(nx, ny, nz, dir, (double*)x, (const double*)b, cycles, fmg);
+    else mg3d::vcycle_run(nx, ny, nz, dir, (float*)x, (const float*)b, cycles, fmg);
+}
+
+}  // extern "C"

@@ -162,6 +162,15 @@ SIGNATURES = {
                               [ctypes.POINTER(SolveStats)]),
     "vflip_solver_reset_history": (c_int, [P]),
     "vflip_solver_history_guess": (c_int, [P, P, P]),
+    "vflip_mg3d_create": (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "vflip_mg3d_destroy": (c_int, [P]),
+    "vflip_mg3d_levels": (c_int, [P, PI, PI]),
+    "vflip_mg3d_vcycle": (c_int, [P, P, P, P]),
+    "vflip_mg3d_full_multigrid": (c_int, [P, P, P, P]),
+    "vflip_mg3d_rbgs": (c_int, [c_int, c_int, c_int, c_int, c_int, P, P, P, c_int, c_double, P]),
+    "vflip_mg3d_residual": (c_int, [c_int, c_int, c_int, c_int, c_int, P, P, P, P]),
+    "vflip_mg3d_restrict": (c_int, [c_int, c_int, c_int, c_int, P, P, P]),
+    "vflip_mg3d_prolongate": (c_int, [c_int, c_int, c_int, c_int, P, P, P]),
 }
 
 _lib = None

@@ -649,3 +649,106 @@ def from_bricks(src, shape):
     dst = torch.empty(shape, device=src.device)
     check(lib().vflip_from_bricks(_ptr(src), nx, ny, nz, _ptr(dst), _stream()), "from_bricks")
     return dst
+
+
+# ------------------------------------------------------ geometric multigrid (MultiGridSolver3D<T>)
+_MG3D_TYPES = {torch.float32: 0, torch.float64: 1}
+
+
+def _mg3d_type(t, name="x"):
+    if t.dtype not in _MG3D_TYPES:
+        raise ValueError(f"{name} must be float32 or float64, got {t.dtype}")
+    return _MG3D_TYPES[t.dtype]
+
+
+class MG3D:
+    """MultiGridSolver3D<T> (S/Multigrid3D.h:152-184) on an (nz, ny, nx) grid, T = dtype
+    (float32: MGSolverf, float64: MGSolverd); dirichlet is the reference's _b_Dirichlet."""
+
+    def __init__(self, nx, ny, nz, dtype=torch.float32, dirichlet=False):
+        if dtype not in _MG3D_TYPES:
+            raise ValueError(f"dtype must be float32 or float64, got {dtype}")
+        self.shape, self.dtype, self.dirichlet = (nz, ny, nx), dtype, bool(dirichlet)
+        h = ctypes.c_void_p()
+        check(lib().vflip_mg3d_create(nx, ny, nz, _MG3D_TYPES[dtype], int(dirichlet), ctypes.byref(h)), "mg3d_create")
+        self.h = h
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().vflip_mg3d_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def levels(self):
+        """[(nx, ny, nz)] of every level, finest first (m_ComputeLevels / systemk)."""
+        n = ctypes.c_int()
+        check(lib().vflip_mg3d_levels(self.h, ctypes.byref(n), None), "mg3d_levels")
+        dims = (ctypes.c_int * (3 * n.value))()
+        check(lib().vflip_mg3d_levels(self.h, ctypes.byref(n), dims), "mg3d_levels")
+        return [tuple(dims[3 * l:3 * l + 3]) for l in range(n.value)]
+
+    def _args(self, x, b):
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        _sizes((x, n, "x"), (b, n, "b"))
+        return _ptr(x, self.dtype, n, "x"), _ptr(b, self.dtype, n, "b")
+
+    def vcycle(self, x, b):
+        """m_Vcycle(x, b, tol, residual, 0): one V-cycle on x in place."""
+        check(lib().vflip_mg3d_vcycle(self.h, *self._args(x, b), _stream()), "mg3d_vcycle")
+
+    def full_multigrid(self, x, b):
+        """m_FullMultiGrid(x, b, tol, residual) on x in place."""
+        check(lib().vflip_mg3d_full_multigrid(self.h, *self._args(x, b), _stream()), "mg3d_full_multigrid")
+
+
+def mg3d_rbgs(b, x, iters, omega=1.0, dirichlet=False, x_new=None):
+    """m_Red_Black_Gauss on x (nz, ny, nx) in place; x_new is scratch of x's shape."""
+    ty = _mg3d_type(x)
+    nz, ny, nx = x.shape
+    if x_new is None:
+        x_new = torch.empty_like(x)
+    _sizes((b, x.numel(), "b"), (x_new, x.numel(), "x_new"))
+    check(lib().vflip_mg3d_rbgs(nx, ny, nz, ty, int(dirichlet), _ptr(b, x.dtype, name="b"), _ptr(x, x.dtype),
+                                _ptr(x_new, x.dtype, name="x_new"), int(iters), float(omega), _stream()), "mg3d_rbgs")
+
+
+def mg3d_residual(b, x, dirichlet=False, out=None):
+    """m_ComputeResidual: b - A x."""
+    ty = _mg3d_type(x)
+    nz, ny, nx = x.shape
+    out = torch.empty_like(x) if out is None else out
+    _sizes((b, x.numel(), "b"), (out, x.numel(), "out"))
+    check(lib().vflip_mg3d_residual(nx, ny, nz, ty, int(dirichlet), _ptr(out, x.dtype, name="out"),
+                                    _ptr(b, x.dtype, name="b"), _ptr(x, x.dtype), _stream()), "mg3d_residual")
+    return out
+
+
+def mg3d_coarse_shape(shape):
+    return tuple(max(n // 2, 1) for n in shape)
+
+
+def mg3d_restrict(src):
+    """m_Restrict: the coarser grid's 0.5 * sum of 8 children."""
+    ty = _mg3d_type(src, "src")
+    nz, ny, nx = src.shape
+    dst = torch.empty(mg3d_coarse_shape(src.shape), dtype=src.dtype, device=src.device)
+    check(lib().vflip_mg3d_restrict(nx, ny, nz, ty, _ptr(src, src.dtype, name="src"), _ptr(dst, src.dtype, name="dst"),
+                                    _stream()), "mg3d_restrict")
+    return dst
+
+
+def mg3d_prolongate(src, dst):
+    """m_Prolongate: dst += parent values from the coarser grid src (in place on dst)."""
+    ty = _mg3d_type(dst, "dst")
+    nz, ny, nx = dst.shape
+    cshape = mg3d_coarse_shape(dst.shape)
+    if tuple(src.shape) != cshape:
+        raise ValueError(f"src has shape {tuple(src.shape)}, expected {cshape}")
+    check(lib().vflip_mg3d_prolongate(nx, ny, nz, ty, _ptr(src, dst.dtype, name="src"),
+                                      _ptr(dst, dst.dtype, name="dst"), _stream()), "mg3d_prolongate")

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libvflip.so")
 
-SOURCES = ["runtime.hip", "mgpcg.hip", "batch.hip", "projection.hip", "advect.hip", "flip.hip", "nested_flip.hip", "tracers.hip", "subdomain.hip", "unet.hip"]
+SOURCES = ["runtime.hip", "mgpcg.hip", "batch.hip", "projection.hip", "advect.hip", "flip.hip", "nested_flip.hip", "tracers.hip", "subdomain.hip", "unet.hip", "mg3d.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

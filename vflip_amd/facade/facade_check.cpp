@@ -5,7 +5,8 @@
 //      a seeded MAC velocity, pcg_projection;
 //   2. the FLIP particles init() seeded: velocities set, hash_FLIP, particle_to_grid
 //      for the three components (the reference's summation order);
-//   3. ivock::AMGPCGSolve<double> on a SparseMatrixd holding a 7-point cut-cell system.
+//   3. ivock::AMGPCGSolve<double> on a SparseMatrixd holding a 7-point cut-cell system;
+//   4. MGSolverf / MGSolverd (MultiGridSolver3D) V-cycles, full multigrid and level operators.
 // Usage: facade_check <output directory>
 #include <cmath>
 #include <cstdint>
@@ -144,6 +145,55 @@ int main(int argc, char** argv) {
     save("amg_b.f64", b);
     save("amg_x.f64", x);
     save("amg_stats.f64", std::vector<double>{(double)ok, (double)its, res});
+
+    // 4. MGSolverf / MGSolverd: m_InitialSystem, two m_Vcycle calls, m_FullMultiGrid, the
+    //    level operators on the finest level, Dirichlet walls.
+    {
+        const int gx = 20, gy = 18, gz = 24;
+        MGSolverf mf;
+        mf._b_Dirichlet = true;
+        mf.m_InitialSystem(gx, gy, gz);
+        MGSolverd md;
+        md.m_InitialSystem(gx, gy, gz);
+        buffer3Df bf, xf, xf2, rf, cf;
+        Buffer3D<double> bd, xd;
+        for (auto* q : {&bf, &xf, &xf2, &rf}) q->init(gx, gy, gz);
+        cf.init(gx / 2, gy / 2, gz / 2);
+        bd.init(gx, gy, gz);
+        xd.init(gx, gy, gz);
+        std::vector<float> lb, lx;
+        for (int k = 0; k < gz; ++k)
+            for (int j = 0; j < gy; ++j)
+                for (int i = 0; i < gx; ++i) {
+                    const uint32_t c = (uint32_t)(i + gx * (j + gy * k));
+                    bf(i, j, k) = urand(15000001u + c);
+                    xf(i, j, k) = 0.1f * urand(16000003u + c);
+                    bd(i, j, k) = bf(i, j, k);
+                    xd(i, j, k) = xf(i, j, k);
+                }
+        xf2.copy(xf);
+        save("mg_b.f32", linear(bf));
+        save("mg_x0.f32", linear(xf));
+        std::printf("MGSolverf: %d levels, coarsest %dx%dx%d\n", mf.m_max_level, mf.systemk.back().gridx,
+                    mf.systemk.back().gridy, mf.systemk.back().gridz);
+        float tol = 0.f, resid = 0.f;
+        mf.m_ComputeResidual(mf.systemk[0], rf, bf, xf);
+        save("mg_res.f32", linear(rf));
+        mf.m_Restrict(mf.systemk[1], mf.systemk[0], rf, cf);
+        save("mg_restrict.f32", linear(cf));
+        mf.m_Vcycle(&xf, &bf, tol, resid, 0);
+        mf.m_Vcycle(&xf, &bf, tol, resid, 0);
+        save("mg_vcycle2.f32", linear(xf));
+        mf.m_FullMultiGrid(&xf2, &bf, tol, resid);
+        save("mg_fmg.f32", linear(xf2));
+        double told = 0, residd = 0;
+        md.m_Vcycle(&xd, &bd, told, residd, 0);
+        std::vector<double> outd((size_t)gx * gy * gz);
+        for (int k = 0; k < gz; ++k)
+            for (int j = 0; j < gy; ++j)
+                for (int i = 0; i < gx; ++i) outd[i + (size_t)gx * (j + (size_t)gy * k)] = xd(i, j, k);
+        save("mg_vcycle_neumann.f64", outd);
+    }
     std::printf("facade check done\n");
     return 0;
 }

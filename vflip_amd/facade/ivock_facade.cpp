@@ -866,4 +866,201 @@ void CoarseFineSmokeSolver::LargeScaleAdvection(float dt) {
     CoarseToFineOperator(fine_solver._wnp1, coarse_solver._wnp1);
 }
 
+// ------------------------------------------------------------ MultiGridSolver3D
+// Device side of one MultiGridSolver3D: a vflip_mg3d handle per level it has been
+// asked to V-cycle from (the levels of a level's grid are the finer grid's levels
+// below it) and scratch copies of the fields.
+struct MGDevice {
+    int type = VFLIP_MG3D_FLOAT;
+    int dirichlet = -1;
+    std::vector<vflip_mg3d*> handles;
+    DBuf x, b, t;
+    ~MGDevice() {
+        for (auto* h : handles)
+            if (h) vflip_mg3d_destroy(h);
+    }
+    vflip_mg3d* handle(int level, const uniform_grid_descriptor_3D& g, int dir) {
+        if (dir != dirichlet) {
+            for (auto*& h : handles)
+                if (h) { vflip_mg3d_destroy(h); h = nullptr; }
+            dirichlet = dir;
+        }
+        if ((int)handles.size() <= level) handles.resize(level + 1, nullptr);
+        if (!handles[level]) ck(vflip_mg3d_create(g.gridx, g.gridy, g.gridz, type, dir, &handles[level]), "mg3d_create");
+        return handles[level];
+    }
+};
+
+namespace {
+template <class T> int mg_type();
+template <> int mg_type<float>() { return VFLIP_MG3D_FLOAT; }
+template <> int mg_type<double>() { return VFLIP_MG3D_DOUBLE; }
+
+// Buffer3D bricks <-> a linear x-fastest device array of g's extents.
+template <class T> T* mg_upload(DBuf& d, const Buffer3D<T>& b, const uniform_grid_descriptor_3D& g) {
+    const size_t n = (size_t)g.gridx * g.gridy * g.gridz;
+    std::vector<T> lin(n);
+    for (int k = 0; k < g.gridz; ++k)
+        for (int j = 0; j < g.gridy; ++j)
+            for (int i = 0; i < g.gridx; ++i) lin[i + (size_t)g.gridx * (j + (size_t)g.gridy * k)] = b(i, j, k);
+    T* p = d.as<T>(n);
+    hk(hipMemcpy(p, lin.data(), n * sizeof(T), hipMemcpyHostToDevice), "mg upload");
+    return p;
+}
+template <class T> void mg_download(const T* p, Buffer3D<T>& b, const uniform_grid_descriptor_3D& g) {
+    const size_t n = (size_t)g.gridx * g.gridy * g.gridz;
+    std::vector<T> lin(n);
+    hk(hipMemcpy(lin.data(), p, n * sizeof(T), hipMemcpyDeviceToHost), "mg download");
+    for (int k = 0; k < g.gridz; ++k)
+        for (int j = 0; j < g.gridy; ++j)
+            for (int i = 0; i < g.gridx; ++i) b(i, j, k) = lin[i + (size_t)g.gridx * (j + (size_t)g.gridy * k)];
+}
+uniform_grid_descriptor_3D mg_coarser(const uniform_grid_descriptor_3D& g) {
+    return uniform_grid_descriptor_3D(std::max(g.gridx / 2, 1), std::max(g.gridy / 2, 1), std::max(g.gridz / 2, 1));
+}
+void mg_same(const uniform_grid_descriptor_3D& a, const uniform_grid_descriptor_3D& b, const char* what) {
+    if (a.gridx != b.gridx || a.gridy != b.gridy || a.gridz != b.gridz)
+        throw std::runtime_error(std::string("vflip facade: ") + what + ": the coarser level must be max(n/2, 1)");
+}
+}  // namespace
+
+// S/Multigrid3D.h:124-137
+template <class T> void MultiGrid<T>::m_ComputeLevels() {
+    int level = 0;
+    int x = m_system_descriptor.gridx, y = m_system_descriptor.gridy, z = m_system_descriptor.gridz;
+    while ((size_t)x * y * z > 512) {
+        ++level;
+        x = std::max(x / 2, 1);
+        y = std::max(y / 2, 1);
+        z = std::max(z / 2, 1);
+    }
+    m_max_level = level + 1;
+}
+// S/Multigrid3D.h:71-118: x, b, r and x_new for the coarse levels, x_new and r for the finest
+template <class T> void MultiGrid<T>::m_AssignMemoryEachLevel() {
+    xk.assign(m_max_level, nullptr); xk_new.assign(m_max_level, nullptr);
+    bk.assign(m_max_level, nullptr); rk.assign(m_max_level, nullptr);
+    systemk.assign(m_max_level, uniform_grid_descriptor_3D());
+    systemk[0] = m_system_descriptor;
+    for (int i = 1; i < m_max_level; ++i) systemk[i] = mg_coarser(systemk[i - 1]);
+    auto make = [&](int l, bool init) {
+        auto* b = new Buffer3D<T>();
+        if (init) b->init(systemk[l].gridx, systemk[l].gridy, systemk[l].gridz, 1.0 / systemk[l].gridx, -0.5, -0.5, -0.5);
+        owned_.push_back(b);
+        return b;
+    };
+    for (int l = 0; l < m_max_level; ++l) {
+        xk[l] = make(l, l > 0);
+        bk[l] = make(l, l > 0);
+        xk_new[l] = make(l, true);
+        rk[l] = make(l, true);
+    }
+    m_bInitialized = true;
+}
+template <class T> void MultiGrid<T>::m_FinalMemoryEachLevel() {
+    for (auto* b : owned_) delete b;
+    owned_.clear();
+    xk.clear(); xk_new.clear(); bk.clear(); rk.clear();
+    m_bInitialized = false;
+}
+// S/Multigrid3D.h:138-150
+template <class T> void MultiGrid<T>::m_InitialSystem(int gridx, int gridy, int gridz) {
+    m_FinalMemoryEachLevel();
+    m_SetGridInformation(gridx, gridy, gridz);
+    m_ComputeLevels();
+    m_AssignMemoryEachLevel();
+}
+
+template <class T> MultiGridSolver3D<T>::~MultiGridSolver3D() { delete dev_; }
+template <class T> MGDevice& MultiGridSolver3D<T>::dev() {
+    if (!dev_) {
+        dev_ = new MGDevice();
+        dev_->type = mg_type<T>();
+    }
+    return *dev_;
+}
+// S/Multigrid3D.h:1328-1392; the coarse levels' host buffers are left zero, as there
+template <class T> void MultiGridSolver3D<T>::m_Vcycle(Buffer3D<T>* x, Buffer3D<T>* b, T, T&, int level) {
+    if (level < 0 || level >= this->m_max_level) throw std::runtime_error("vflip facade: m_Vcycle level");
+    this->xk[level] = x;
+    this->bk[level] = b;
+    MGDevice& D = dev();
+    const auto& g = this->systemk[level];
+    vflip_mg3d* h = D.handle(level, g, _b_Dirichlet ? 1 : 0);
+    T* dx = mg_upload(D.x, *x, g);
+    T* db = mg_upload(D.b, *b, g);
+    ck(vflip_mg3d_vcycle(h, dx, db, nullptr), "mg3d_vcycle");
+    mg_download(dx, *x, g);
+}
+// S/Multigrid3D.h:1394-1436
+template <class T> void MultiGridSolver3D<T>::m_FullMultiGrid(Buffer3D<T>* x, Buffer3D<T>* b, T, T&) {
+    this->xk[0] = x;
+    this->bk[0] = b;
+    MGDevice& D = dev();
+    const auto& g = this->systemk[0];
+    vflip_mg3d* h = D.handle(0, g, _b_Dirichlet ? 1 : 0);
+    T* dx = mg_upload(D.x, *x, g);
+    T* db = mg_upload(D.b, *b, g);
+    ck(vflip_mg3d_full_multigrid(h, dx, db, nullptr), "mg3d_full_multigrid");
+    mg_download(dx, *x, g);
+}
+// S/Multigrid3D.h:840-1129 (x_new is untouched here: the device keeps its own scratch)
+template <class T>
+void MultiGridSolver3D<T>::m_Red_Black_Gauss(uniform_grid_descriptor_3D& g, Buffer3D<T>& b, Buffer3D<T>& x,
+                                             Buffer3D<T>&, int iter_time, double omega) {
+    MGDevice& D = dev();
+    T* dx = mg_upload(D.x, x, g);
+    T* db = mg_upload(D.b, b, g);
+    T* dt = D.t.as<T>((size_t)g.gridx * g.gridy * g.gridz);
+    ck(vflip_mg3d_rbgs(g.gridx, g.gridy, g.gridz, mg_type<T>(), _b_Dirichlet ? 1 : 0, db, dx, dt, iter_time, omega,
+                       nullptr), "mg3d_rbgs");
+    mg_download(dx, x, g);
+}
+// S/Multigrid3D.h:1130-1186
+template <class T>
+void MultiGridSolver3D<T>::m_ComputeResidual(uniform_grid_descriptor_3D& g, Buffer3D<T>& res, Buffer3D<T>& b,
+                                             Buffer3D<T>& x) {
+    MGDevice& D = dev();
+    T* dx = mg_upload(D.x, x, g);
+    T* db = mg_upload(D.b, b, g);
+    T* dr = D.t.as<T>((size_t)g.gridx * g.gridy * g.gridz);
+    ck(vflip_mg3d_residual(g.gridx, g.gridy, g.gridz, mg_type<T>(), _b_Dirichlet ? 1 : 0, dr, db, dx, nullptr),
+       "mg3d_residual");
+    mg_download(dr, res, g);
+}
+// S/Multigrid3D.h:1187-1191
+template <class T> void MultiGridSolver3D<T>::m_ExactSolve() {
+    const int l = this->m_max_level - 1;
+    m_Red_Black_Gauss(this->systemk[l], *this->bk[l], *this->xk[l], *this->xk_new[l], 100, 1.0);
+}
+// S/Multigrid3D.h:1193-1245
+template <class T>
+void MultiGridSolver3D<T>::m_Restrict(uniform_grid_descriptor_3D& next_level, uniform_grid_descriptor_3D& curr_level,
+                                      Buffer3D<T>& src, Buffer3D<T>& dst) {
+    mg_same(next_level, mg_coarser(curr_level), "m_Restrict");
+    MGDevice& D = dev();
+    T* ds = mg_upload(D.x, src, curr_level);
+    T* dd = D.t.as<T>((size_t)next_level.gridx * next_level.gridy * next_level.gridz);
+    ck(vflip_mg3d_restrict(curr_level.gridx, curr_level.gridy, curr_level.gridz, mg_type<T>(), ds, dd, nullptr),
+       "mg3d_restrict");
+    mg_download(dd, dst, next_level);
+}
+// S/Multigrid3D.h:1298-1326 (next_level: the finer grid of dst, curr_level: src's)
+template <class T>
+void MultiGridSolver3D<T>::m_Prolongate(uniform_grid_descriptor_3D& next_level, uniform_grid_descriptor_3D& curr_level,
+                                        Buffer3D<T>& src, Buffer3D<T>& dst) {
+    mg_same(curr_level, mg_coarser(next_level), "m_Prolongate");
+    MGDevice& D = dev();
+    T* ds = mg_upload(D.b, src, curr_level);
+    T* dd = mg_upload(D.x, dst, next_level);
+    ck(vflip_mg3d_prolongate(next_level.gridx, next_level.gridy, next_level.gridz, mg_type<T>(), ds, dd, nullptr),
+       "mg3d_prolongate");
+    mg_download(dd, dst, next_level);
+}
+
+template class MultiGrid<float>;
+template class MultiGrid<double>;
+template class MultiGridSolver3D<float>;
+template class MultiGridSolver3D<double>;
+
 }  // namespace ivock
