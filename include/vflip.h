@@ -567,6 +567,12 @@ int vflip_unet_get_params(const vflip_unet* net, float* dst);
 int vflip_unet_set_params(vflip_unet* net, const float* src);
 int vflip_unet_get_grads(const vflip_unet* net, float* dst);
 int vflip_unet_zero_grad(vflip_unet* net);
+/* A stored activation of the last forward as fp32, [z][y][x][C] on its level's grid
+ * (C = the tensor's channels): tensor 0..13 = output of layer 0..13 (after the ReLU),
+ * 15..17 = the pooled inputs of levels 1..3, 18 = the packed network input (16
+ * channels, 0..2 used). For tests that check the backward pass against a reference
+ * fed with the same forward values. Synchronous. */
+int vflip_unet_get_activation(const vflip_unet* net, int tensor, float* dst);
 /* pred[v] = network output; solid/liquid may be NULL (0 / 1). Asynchronous. */
 int vflip_unet_forward(vflip_unet* net, int ni, int nj, int nk, const float* rhs, const float* solid,
                        const float* liquid, float in_scale, float* pred);

@@ -35,23 +35,33 @@ class _GradBF16(torch.autograd.Function):
         return _bf(g)
 
 
-def forward(flat, rhs, solid=None, liquid=None, in_scale=1.0, grad_bf16=True):
+def _forced(a, key, forced):
+    """a with the value forced[key] (the GPU's stored tensor, (C, z, y, x)) and a's gradient
+    path: forced + (a - a.detach()), so the backward runs on the same forward values."""
+    if forced is None or key not in forced:
+        return a
+    return forced[key].unsqueeze(0) + (a - a.detach())
+
+
+def forward(flat, rhs, solid=None, liquid=None, in_scale=1.0, grad_bf16=True, forced=None):
     """flat: fp32 parameter vector (the library's layout, requires_grad allowed);
     rhs/solid/liquid: (nk, nj, ni) fp32 CPU tensors. Returns the prediction (nk, nj, ni).
     grad_bf16: round each convolution's output gradient to bf16 in the backward
-    pass, where the GPU does."""
+    pass, where the GPU does. forced: optional {"in", "P0".."P2", layer index:
+    (C, z, y, x) tensor} of stored forward values to use in place of the ones
+    recomputed here (the gradients still flow through this forward)."""
     layers = split_params(flat)
     nk, nj, ni = rhs.shape
     x = torch.zeros(1, 16, nk, nj, ni)
     x[0, 0] = _bf(rhs * in_scale)
     x[0, 1] = _bf(solid) if solid is not None else 0.0
     x[0, 2] = _bf(liquid) if liquid is not None else 1.0
-    acts = {"in": x}
+    acts = {"in": _forced(x, "in", forced)}
     pred = None
     for l, (ins, cout, relu, level, _) in enumerate(LAYERS):
         for p, src in POOL_OF.items():
             if src == l - 1 and p not in acts and l in (2, 4, 6):
-                acts[p] = _bf(F.avg_pool3d(acts[src], 2))
+                acts[p] = _forced(_bf(F.avg_pool3d(acts[src], 2)), p, forced)
         parts = []
         for (t, c, up) in ins:
             a = acts[t]
@@ -68,7 +78,7 @@ def forward(flat, rhs, solid=None, liquid=None, in_scale=1.0, grad_bf16=True):
         if l == len(LAYERS) - 1:
             pred = y[0, 0]
         else:
-            acts[l] = _bf(F.relu(y) if relu else y)
+            acts[l] = _forced(_bf(F.relu(y) if relu else y), l, forced)
     return pred
 
 

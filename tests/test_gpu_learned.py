@@ -2,11 +2,18 @@
 
 * The energy-norm training loss and its gradients against the CPU reference
   (oracle/unet_ref.py with the GPU's bf16 rounding points, plus a torch 7-point
-  stencil). The loss gradient 2Ae/x'Ax is dominated by grid-scale components
-  that cancel as they are pooled toward the bottleneck, so rounding differences
-  grow with depth: the layers next to the loss agree to 2e-2, all parameters
-  together to 5e-2 and every layer to 1.5e-1 (the MSE-loss test of
-  test_gpu_unet.py holds every layer's backward to 5e-2).
+  stencil), in two ways. (1) The backward pass alone: the reference runs on the
+  GPU's own stored forward values (activations, pooled tensors, packed input),
+  so both sides differentiate the same forward; every layer's weight gradient
+  agrees to 1e-2 (measured 1.6e-3..2.4e-3). (2) Free running, each side with its own
+  forward: the loss agrees to 2e-2, all parameters together to 5e-2 and the
+  layers next to the loss to 2e-2. The deeper layers then differ by a few
+  percent because a different FP32 accumulation order flips some activations to
+  the neighbouring bf16 value and the energy-loss gradient cancels as it is
+  pooled toward the bottleneck: the reference against itself with FP64 instead of
+  FP32 accumulation (same rounding points) moves layers 2-8 by 2-5%, while FP32
+  against FP64 without bf16 rounding agrees to 1e-4. The MSE-loss test of
+  test_gpu_unet.py holds every layer's free-running backward to 5e-2.
 * MGPCG vets every guess: a guess that is no better than zero in the A-norm
   (E(x0) >= 0) is dropped and the solve restarts from zero, converging as a
   zero-guess solve does.
@@ -43,7 +50,7 @@ def test_energy_loss_gradients_match_reference(cuda):
     import torch
     from oracle import unet_ref
     from vflip_amd import api, learned
-    from vflip_amd.unet import UNet, param_layout
+    from vflip_amd.unet import LAYERS, UNet, param_layout
     n = 32
     phi, centre, radius, steps = learned.scene(7, n)
     from vflip_amd.plume import PlumeSim
@@ -76,14 +83,34 @@ def test_energy_loss_gradients_match_reference(cuda):
     ref_loss.backward()
     assert abs(loss - float(ref_loss.detach())) <= 2e-2 * float(ref_loss.detach())
     offs, boffs, _ = param_layout()
-    errs = []
-    for l, ((wo, cout, kw, cin), bo) in enumerate(zip(offs, boffs)):
-        gw, rw = g[wo:wo + cout * kw], flat.grad[wo:wo + cout * kw]
-        errs.append(float((gw - rw).norm() / rw.norm()))
-    print("per-layer relative gradient error:", [round(e, 4) for e in errs], flush=True)
+
+    def layer_errs(ref_grad):
+        out = []
+        for (wo, cout, kw, cin), bo in zip(offs, boffs):
+            gw, rw = g[wo:wo + cout * kw], ref_grad[wo:wo + cout * kw]
+            out.append(float((gw - rw).norm() / rw.norm()))
+        return out
+    errs = layer_errs(flat.grad)
+    print("free-running per-layer relative gradient error:", [round(e, 4) for e in errs], flush=True)
     wall = offs[-1][0] + offs[-1][1] * offs[-1][2]
     overall = float((g[:wall] - flat.grad[:wall]).norm() / flat.grad[:wall].norm())
-    assert max(errs[12:]) <= 2e-2 and overall <= 5e-2 and max(errs) <= 1.5e-1, (overall, errs)
+    assert max(errs[12:]) <= 2e-2 and overall <= 5e-2, (overall, errs)
+
+    # the backward pass on the GPU's own forward values
+    levels = {"in": 0, "P0": 1, "P1": 2, "P2": 3}
+    forced = {}
+    for key, t in [("in", 18), ("P0", 15), ("P1", 16), ("P2", 17)] + [(l, l) for l in range(14)]:
+        lev = levels[key] if isinstance(key, str) else LAYERS[key][3]
+        forced[key] = net.activation(t, (n >> lev, n >> lev, n >> lev)).cpu()
+    flat2 = net.params().cpu().clone().requires_grad_(True)
+    pred2 = unet_ref.forward(flat2, r_c, s_c, 1.0 - s_c, 1.0 / s, forced=forced)
+    e2 = pred2 - x
+    loss2 = (e2 * _stencil(e2, *A)).sum() / (x * _stencil(x, *A)).sum()
+    loss2.backward()
+    assert abs(loss - float(loss2.detach())) <= 2e-3 * float(loss2.detach())
+    errs2 = layer_errs(flat2.grad)
+    print("backward-only per-layer relative gradient error:", [round(e, 4) for e in errs2], flush=True)
+    assert max(errs2) <= 1e-2, errs2
 
 
 def test_bad_guess_is_rejected_and_solve_converges(cuda):

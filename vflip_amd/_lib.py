@@ -150,6 +150,7 @@ SIGNATURES = {
     "vflip_unet_set_params": (c_int, [P, P]),
     "vflip_unet_get_grads": (c_int, [P, P]),
     "vflip_unet_zero_grad": (c_int, [P]),
+    "vflip_unet_get_activation": (c_int, [P, c_int, P]),
     "vflip_unet_forward": (c_int, [P, c_int, c_int, c_int, P, P, P, c_float, P]),
     "vflip_unet_backward": (c_int, [P, P]),
     "vflip_unet_adam": (c_int, [P, c_float, c_int, c_float]),

@@ -73,6 +73,11 @@ int kwb_of(const LayerDef& d) { return (ckb_of(d) == 32 ? 27 : 28) * d.cout; }
 
 unsigned grid256(size_t n) { return (unsigned)std::min<size_t>((n + 255) / 256, 16384); }
 
+__global__ void k_bf16_to_f32(const bf16* __restrict__ a, size_t n, float* __restrict__ out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)a[i];
+}
+
 }  // namespace
 
 struct vflip_unet {
@@ -516,6 +521,18 @@ int vflip_unet_set_params(vflip_unet* N, const float* src) {
 int vflip_unet_get_grads(const vflip_unet* N, float* dst) {
     VF_ARG(N && dst, "args");
     VF_HIP(hipMemcpyAsync(dst, N->grads, (N->nw + N->nb) * 4, hipMemcpyDeviceToDevice, N->stream));
+    VF_HIP(hipStreamSynchronize(N->stream));
+    return 0;
+}
+
+int vflip_unet_get_activation(const vflip_unet* N, int tensor, float* dst) {
+    VF_ARG(N && dst, "args");
+    VF_ARG(tensor >= 0 && tensor < kTensors && tensor != kLayers - 1, "tensor (0..13 layer outputs, 15..17 pooled, 18 input)");
+    const bf16* a = N->act[tensor];
+    VF_ARG(a != nullptr, "no forward has run on a grid");
+    const size_t n = level_vox(N, tensor_level(tensor)) * tensor_channels(tensor);
+    k_bf16_to_f32<<<grid256(n), 256, 0, N->stream>>>(a, n, dst);
+    VF_CHECK_LAUNCH();
     VF_HIP(hipStreamSynchronize(N->stream));
     return 0;
 }

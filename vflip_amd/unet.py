@@ -110,6 +110,17 @@ class UNet:
     def zero_grad(self):
         check(lib().vflip_unet_zero_grad(self.h), "unet_zero_grad")
 
+    def activation(self, tensor, shape):
+        """Stored activation of the last forward as (C, z, y, x) float32: tensor 0..13 = layer
+        outputs, 15..17 = pooled inputs of levels 1..3, 18 = the packed input; shape = the
+        (z, y, x) grid of the tensor's level."""
+        z, y, x = shape
+        ch = 16 if tensor == 18 else (LAYERS[tensor][1] if tensor < len(LAYERS) else
+                                      LAYERS[POOL_OF[f"P{tensor - len(LAYERS)}"]][1])
+        out = torch.empty((z, y, x, ch), device="cuda")
+        check(lib().vflip_unet_get_activation(self.h, tensor, _ptr(out)), "unet_get_activation")
+        return out.permute(3, 0, 1, 2).contiguous()
+
     def forward(self, rhs, solid=None, liquid=None, in_scale=1.0, out=None):
         nk, nj, ni = rhs.shape
         out = torch.empty_like(rhs) if out is None else out
