@@ -115,6 +115,14 @@ int vflip_solver_setup(vflip_solver* s);
  * VFLIP_ERR_BREAKDOWN. stats may be NULL. */
 int vflip_solver_solve(vflip_solver* s, const float* rhs, float* x, vflip_solve_stats* stats);
 
+/* FP64 defect correction around this solver's own solves: the single-grid form of
+ * vflip_batch_solve_refined (same rounds, same kernels; a batch of solvers with the
+ * default tile shapes gives the same bits). */
+int vflip_solver_solve_refined(vflip_solver* s, const float* rhs, float* x, double* x64, int assembled, double tol,
+                               int max_rounds, vflip_solve_stats* stats);
+/* The FP64 level-0 diagonal and right-hand side the last assembly wrote (device
+ * doubles, ni*nj*nk each; either may be NULL), for checks of FP64 results. */
+int vflip_solver_get_assembled64(vflip_solver* s, double* diag, double* rhs);
 /* FP64 right-hand side and solution (device double arrays). In mixed precision
  * the iterate is accumulated in FP64 and the stop test is on the FP64 true
  * residual |b - A x|_inf (FP64 diagonal from vflip_solver_set_diag64 or the
@@ -538,6 +546,19 @@ int vflip_batch_set_graph_iterations(vflip_batch* batch, int iterations);
  * not, <0 on error (VFLIP_ERR_BREAKDOWN if a subdomain broke down). */
 int vflip_batch_solve(vflip_batch* batch, const float* const* rhs, float* const* x, vflip_solve_stats* stats,
                       int use_graph, double* ms);
+/* The subdomain solves to the reference's FP64 tolerance (SubDomainSolver::pcg_projection
+ * solves in FP64 to a relative 1e-9 in at most 1000 iterations, S/sub_domain.cpp:739):
+ * FP64 defect correction around the batched FP32 MGPCG. Each round takes every
+ * unfinished subdomain's FP64 true residual b - A x (the assembled FP64 diagonal; with
+ * `assembled`, the assembled FP64 right-hand side, else rhs), solves all corrections
+ * in one vflip_batch_solve to the solvers' own (FP32) tolerance and adds them to the
+ * FP64 iterates, until |b - A x|_inf <= tol |b|_inf or max_rounds corrections.
+ * x: FP32 solutions; x64 (may be NULL, else per subdomain NULL or device doubles):
+ * the FP64 iterates. stats: iterations summed over the rounds, true_residual the
+ * final FP64 |b - A x|_inf / |b|_inf, ms_solve the call's wall time. */
+int vflip_batch_solve_refined(vflip_batch* batch, const float* const* rhs, float* const* x, double* const* x64,
+                              int assembled, double tol, int max_rounds, vflip_solve_stats* stats, int use_graph,
+                              double* ms);
 /* Autotuner: times the fused smoothing sweep over the whole batch for each level
  * depth and tile shape (4/8/16 rows per tile x z chunks of 4..64 planes), keeps
  * the fastest and rebuilds the batch's launch plan. ty/zc (may be NULL) receive

@@ -91,8 +91,9 @@ def test_c4_learned_warm_start(cuda, n, steps, seed):
 
 def test_c5_nested_256(cuda):
     """256^3 coarse + 12 subdomains, 2 steps: batched (solver tile shapes) equals sequential
-    field for field; every fine solve converges and states its FP64 true residual; the
-    last step's largest fine solve equals the oracle's AMGPCG iterate on its operator."""
+    field for field; every fine solve reaches the reference's FP64 tolerance, 1e-9 relative
+    (S/sub_domain.cpp:739), which numpy confirms on the largest subdomain's assembled FP64
+    system, and that solution matches the oracle's FP64 AMGPCG solve."""
     import torch
     from oracle import oracle as O
     from vflip_amd.nested import NestedPlume
@@ -106,29 +107,37 @@ def test_c5_nested_256(cuda):
 
     sb, rb = run(True)
     fb = [t.detach().cpu().numpy() for t in (sb.coarse.u, sb.coarse.p)] + \
-         [t.detach().cpu().numpy() for s in sb.subs for t in (s.u, s.p)]
-    # one fine solve against the oracle on the same operator and right-hand side
+         [t.detach().cpu().numpy() for s in sb.subs for t in (s.u, s.p, s.p64)]
+    # the largest fine solve: FP64 residual in numpy, and against the oracle's solve
     q = int(np.argmax([np.prod(s.nf) for s in sb.subs]))
     s = sb.subs[q]
-    d, ax, ay, az = (a.detach().cpu().numpy().astype(np.float64) for a in s.solver.get_level(0))
-    rhs = s.rhs.detach().cpu().numpy().astype(np.float64)
+    d64, b64 = (t.cpu().numpy() for t in s.solver.assembled64())
+    _, ax, ay, az = (a.detach().cpu().numpy().astype(np.float64) for a in s.solver.get_level(0))
+    p64 = s.p64.cpu().numpy()
+    y = d64 * p64
+    y[:, :, :-1] += ax[:, :, :-1] * p64[:, :, 1:]
+    y[:, :, 1:] += ax[:, :, :-1] * p64[:, :, :-1]
+    y[:, :-1, :] += ay[:, :-1, :] * p64[:, 1:, :]
+    y[:, 1:, :] += ay[:, :-1, :] * p64[:, :-1, :]
+    y[:-1, :, :] += az[:-1, :, :] * p64[1:, :, :]
+    y[1:, :, :] += az[:-1, :, :] * p64[:-1, :, :]
+    res = np.abs(b64 - y).max() / np.abs(b64).max()
+    ref = O.amgpcg(d64, ax, ay, az, b64, tol=1e-12, maxit=1000)["x"]
+    err = np.linalg.norm(p64 - ref) / np.linalg.norm(ref)
     its = rb[-1]["fine_iterations"][q]
-    ref = O.amgpcg(d, ax, ay, az, rhs, tol=0.0, maxit=its)["x"]
-    p = s.p.detach().cpu().numpy().astype(np.float64)
-    err = np.linalg.norm(p - ref) / np.linalg.norm(ref)
-    print(f"fine solve {q} ({s.nf}): {its} its, rel L2 vs oracle iterate {err:.2e}", flush=True)
-    assert err < 1e-5
+    print(f"fine solve {q} ({s.nf}): {its} its, FP64 residual {res:.2e}, rel L2 vs oracle {err:.2e}", flush=True)
+    assert res <= 1.01e-9 and err < 1e-6
     del sb
     torch.cuda.empty_cache()
     ss, rs = run(False)
     fs = [t.detach().cpu().numpy() for t in (ss.coarse.u, ss.coarse.p)] + \
-         [t.detach().cpu().numpy() for s in ss.subs for t in (s.u, s.p)]
+         [t.detach().cpu().numpy() for s in ss.subs for t in (s.u, s.p, s.p64)]
     for a, b in zip(rb, rs):
         print("fine its", a["fine_iterations"], "true residuals", ["%.1e" % x for x in a["fine_true_residuals"]],
               flush=True)
         assert a["fine_iterations"] == b["fine_iterations"]
         assert a["fine_converged"] and b["fine_converged"]
-        assert a["fine_max_rel_residual"] <= 1e-6
-        assert max(a["fine_true_residuals"]) <= 1e-4
+        assert a["fine_max_rel_residual"] <= 1e-9
+        assert max(a["fine_true_residuals"]) <= 1e-9
     for a, b in zip(fb, fs):
         assert np.array_equal(a, b)

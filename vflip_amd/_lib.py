@@ -163,6 +163,9 @@ SIGNATURES = {
                               [ctypes.POINTER(SolveStats)]),
     "vflip_solver_reset_history": (c_int, [P]),
     "vflip_solver_history_guess": (c_int, [P, P, P]),
+    "vflip_batch_solve_refined": (c_int, [P, P, P, P, c_int, c_double, c_int, P, c_int, PD]),
+    "vflip_solver_get_assembled64": (c_int, [P, P, P]),
+    "vflip_solver_solve_refined": (c_int, [P, P, P, P, c_int, c_double, c_int, ctypes.POINTER(SolveStats)]),
     "vflip_mg3d_create": (c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.POINTER(c_void_p)]),
     "vflip_mg3d_destroy": (c_int, [P]),
     "vflip_mg3d_levels": (c_int, [P, PI, PI]),

@@ -201,6 +201,26 @@ class Solver:
     def reset_history(self):
         check(lib().vflip_solver_reset_history(self.h), "reset_history")
 
+    def assembled64(self):
+        """(diag, rhs) in FP64 as the last assembly wrote them (device tensors)."""
+        d = torch.empty((self.nk, self.nj, self.ni), device="cuda", dtype=torch.float64)
+        b = torch.empty_like(d)
+        check(lib().vflip_solver_get_assembled64(self.h, _ptr(d, torch.float64), _ptr(b, torch.float64)),
+              "get_assembled64")
+        return d, b
+
+    def solve_refined(self, rhs, x, tol=1e-9, max_rounds=8, assembled=True, x64=None):
+        """FP64 defect correction around this solver's own solves (vflip_solver_solve_refined)."""
+        _sizes((rhs, self.n, "rhs"), (x, self.n, "x"), (x64, self.n, "x64"))
+        st = SolveStats()
+        rc = check(lib().vflip_solver_solve_refined(self.h, _ptr(rhs, numel=self.n, name="rhs"),
+                                                    _ptr(x, numel=self.n, name="x"),
+                                                    _ptr(x64, torch.float64, self.n, "x64"), int(assembled),
+                                                    float(tol), int(max_rounds), ctypes.byref(st)), "solve_refined")
+        d = st.as_dict()
+        d["status"] = rc
+        return d
+
     def project(self, variant, h, dt, u, v, w, uw, vw, ww, us, vs, ws, rhs, p):
         _sizes((rhs, self.n, "rhs"), (p, self.n, "p"))
         st = SolveStats()
@@ -366,6 +386,25 @@ class Batch:
             d = st[i].as_dict()
             out.append(d)
         return rc, out, ms.value
+
+    def solve_refined(self, rhs, x, tol=1e-9, max_rounds=8, assembled=True, x64=None, graph=True):
+        """FP64 defect correction around the batched FP32 solve (vflip_batch_solve_refined):
+        every subdomain to an FP64 true residual of tol. Returns (status, [stats], ms)."""
+        n = len(self.solvers)
+        assert len(rhs) == n and len(x) == n and (x64 is None or len(x64) == n)
+        r = (ctypes.c_void_p * n)(*[_ptr(t, numel=s.n, name="rhs").value for t, s in zip(rhs, self.solvers)])
+        xs = (ctypes.c_void_p * n)(*[_ptr(t, numel=s.n, name="x").value for t, s in zip(x, self.solvers)])
+        x64s = None if x64 is None else (ctypes.c_void_p * n)(
+            *[_ptr(t, torch.float64, s.n, "x64").value for t, s in zip(x64, self.solvers)])
+        st = (SolveStats * n)()
+        ms = ctypes.c_double()
+        rc = check(lib().vflip_batch_solve_refined(self.h, ctypes.cast(r, ctypes.c_void_p),
+                                                   ctypes.cast(xs, ctypes.c_void_p),
+                                                   None if x64s is None else ctypes.cast(x64s, ctypes.c_void_p),
+                                                   int(assembled), float(tol), int(max_rounds),
+                                                   ctypes.cast(st, ctypes.c_void_p), int(graph), ctypes.byref(ms)),
+                   "batch_solve_refined")
+        return rc, [st[i].as_dict() for i in range(n)], ms.value
 
 
 # ---------------------------------------------------------------------- FLIP

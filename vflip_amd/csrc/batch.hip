@@ -21,6 +21,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -671,9 +672,14 @@ int vflip_batch_set_graph_iterations(vflip_batch* B, int iterations) {
     return 0;
 }
 
-int vflip_batch_solve(vflip_batch* B, const float* const* rhs, float* const* x, vflip_solve_stats* stats,
-                      int use_graph, double* ms) {
-    VF_ARG(B && rhs && x, "batch/rhs/x");
+}  // extern "C"
+
+namespace {
+// finish: outputs and stats as vflip_solver_solve reports them (FP64 true residual per
+// subdomain); without it only x and the iteration counts (its, may be null), for the
+// correction solves of vflip_batch_solve_refined.
+int batch_solve_impl(vflip_batch* B, const float* const* rhs, float* const* x, vflip_solve_stats* stats,
+                     int use_graph, double* ms, bool finish, int* its) {
     const int ns = (int)B->S.size();
     for (int q = 0; q < ns; ++q) VF_ARG(rhs[q] && x[q], "rhs/x of a subdomain");
     // the solvers' own streams may hold earlier work on these buffers (assembly)
@@ -717,10 +723,92 @@ int vflip_batch_solve(vflip_batch* B, const float* const* rhs, float* const* x, 
     for (int q = 0; q < ns; ++q) {
         const BState& st = B->h_st[q];
         const int status = st.done ? st.status : VFLIP_NOT_CONVERGED;
-        if ((rc = solver_finish(B->S[q], rhs[q], x[q], st.iters, status, stats ? &stats[q] : nullptr))) return rc;
+        if (finish) {
+            if ((rc = solver_finish(B->S[q], rhs[q], x[q], st.iters, status, stats ? &stats[q] : nullptr))) return rc;
+        } else {
+            VF_HIP(hipMemcpyAsync(x[q], B->S[q]->xw, B->S[q]->n * sizeof(float), hipMemcpyDeviceToDevice, B->stream));
+            if (its) its[q] = st.iters;
+        }
         if (status < 0) worst = status;
         else if (worst == VFLIP_OK && status != VFLIP_OK) worst = status;
     }
+    return worst;
+}
+}  // namespace
+
+extern "C" {
+
+int vflip_batch_solve(vflip_batch* B, const float* const* rhs, float* const* x, vflip_solve_stats* stats,
+                      int use_graph, double* ms) {
+    VF_ARG(B && rhs && x, "batch/rhs/x");
+    return batch_solve_impl(B, rhs, x, stats, use_graph, ms, true, nullptr);
+}
+
+int vflip_batch_solve_refined(vflip_batch* B, const float* const* rhs, float* const* x, double* const* x64,
+                              int assembled, double tol, int max_rounds, vflip_solve_stats* stats, int use_graph,
+                              double* ms) {
+    VF_ARG(B && rhs && x, "batch/rhs/x");
+    VF_ARG(tol > 0 && max_rounds >= 0, "tolerance / rounds");
+    const int ns = (int)B->S.size();
+    for (int q = 0; q < ns; ++q) VF_ARG(rhs[q] && x[q], "rhs/x of a subdomain");
+    for (auto* S : B->S) VF_HIP(hipStreamSynchronize(S->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = B->stream;
+    int rc;
+    for (auto* S : B->S)
+        if ((rc = solver_refine_begin(S, s, assembled != 0))) return rc;
+    std::vector<int> its(ns, 0);
+    std::vector<double> bnorm(ns, 0.0), tr(ns, 0.0);
+    std::vector<char> done(ns, 0);
+    std::vector<const float*> in_rhs(ns);
+    std::vector<float*> in_x(ns);
+    std::vector<int> in_its(ns, 0);
+    for (int round = 0;; ++round) {
+        // FP64 true residual of every unfinished subdomain, one read-back for all
+        for (int q = 0; q < ns; ++q)
+            if (!done[q]) {
+                vflip_solver* S = B->S[q];
+                if ((rc = solver_refine_residual(S, s, rhs[q], assembled != 0))) return rc;
+                VF_HIP(hipMemcpyAsync(S->h_sc, S->sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+            }
+        VF_HIP(hipStreamSynchronize(s));
+        bool all = true;
+        for (int q = 0; q < ns; ++q) {
+            if (done[q]) continue;
+            tr[q] = B->S[q]->h_sc->true_res;
+            if (round == 0) bnorm[q] = tr[q];
+            if (bnorm[q] == 0 || tr[q] <= tol * bnorm[q]) done[q] = 1;
+            all = all && done[q];
+        }
+        if (all || round == max_rounds) break;
+        // one batched FP32 solve of the corrections; finished subdomains get a zero
+        // right-hand side, which their solve takes as converged at once
+        for (int q = 0; q < ns; ++q) {
+            vflip_solver* S = B->S[q];
+            if (done[q]) k_fill<<<grid_for(S->n), kBlock, 0, s>>>((unsigned)S->n, S->ref_r, 0.f);
+            in_rhs[q] = S->ref_r;
+            in_x[q] = S->ref_e;
+        }
+        VF_CHECK_LAUNCH();
+        VF_HIP(hipStreamSynchronize(s));
+        rc = batch_solve_impl(B, in_rhs.data(), in_x.data(), nullptr, use_graph, nullptr, false, in_its.data());
+        if (rc < 0) return rc;
+        for (int q = 0; q < ns; ++q)
+            if (!done[q]) {
+                its[q] += in_its[q];
+                if ((rc = solver_refine_add(B->S[q], s))) return rc;
+            }
+    }
+    int worst = VFLIP_OK;
+    for (int q = 0; q < ns; ++q) {
+        if ((rc = solver_refine_end(B->S[q], s, x[q], x64 ? x64[q] : nullptr))) return rc;
+        if (!done[q]) worst = VFLIP_NOT_CONVERGED;
+    }
+    VF_HIP(hipStreamSynchronize(s));
+    const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (ms) *ms = t;
+    if (stats)
+        for (int q = 0; q < ns; ++q) solver_refine_stats(its[q], done[q] != 0, tr[q], bnorm[q], t, &stats[q]);
     return worst;
 }
 

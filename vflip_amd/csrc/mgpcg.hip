@@ -13,6 +13,8 @@
 //                        sweeps (level 0 uses the PCG vectors r and z/zt)
 #include "common.h"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -506,6 +508,7 @@ int vflip_solver_destroy(vflip_solver* S) {
     if (S->pcg_alloc) (void)hipFree(S->pcg_alloc);
     if (S->f64_alloc) (void)hipFree(S->f64_alloc);
     if (S->hist_alloc) (void)hipFree(S->hist_alloc);
+    if (S->ref_alloc) (void)hipFree(S->ref_alloc);
     if (S->ctab) (void)hipFree(S->ctab);
     if (S->cpart) (void)hipFree(S->cpart);
     if (S->d_changed) (void)hipFree(S->d_changed);
@@ -1081,11 +1084,118 @@ int solver_finish(vflip_solver* S, const float* rhs32, float* x32, int iteration
     return 0;
 }
 
+// FP64 defect correction around FP32 solves (SubDomainSolver::pcg_projection solves in
+// FP64 to 1e-9, S/sub_domain.cpp:739): the iterate is kept in FP64, each round solves
+// A e = b - A x for the correction in FP32 and adds it. Buffers of the rounds; x = 0.
+int solver_refine_begin(vflip_solver* S, hipStream_t s, bool assembled) {
+    if (assembled && !S->b64_valid) {
+        set_error("refined solve: the solver holds no assembled FP64 right-hand side");
+        return VFLIP_ERR_ARG;
+    }
+    int rc;
+    if (!S->setup_done || S->check_changed) {
+        if ((rc = vflip_solver_setup(S))) return rc;
+        VF_HIP(hipStreamSynchronize(S->stream));
+    }
+    if ((rc = ensure_f64(S))) return rc;
+    if (!S->ref_alloc) {
+        const size_t n = S->n;
+        if ((rc = alloc(&S->ref_alloc, n * sizeof(double) + 2 * ((n + 63) & ~(size_t)63) * sizeof(float)))) return rc;
+        S->ref_x64 = (double*)S->ref_alloc;
+        S->ref_r = (float*)(S->ref_x64 + n);
+        S->ref_e = S->ref_r + ((n + 63) & ~(size_t)63);
+    }
+    k_fill64<<<grid_for(S->n), kBlock, 0, s>>>((unsigned)S->n, S->ref_x64, 0.0);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+// ref_r = FP32(b - A x) in FP64 (b = the assembled FP64 right-hand side when `assembled`,
+// else rhs32; the assembled FP64 diagonal when valid) and its max |.| into sc->true_res.
+int solver_refine_residual(vflip_solver* S, hipStream_t s, const float* rhs32, bool assembled) {
+    const LevelBuf& l0 = S->L[0];
+    const dim3 g((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
+    k_true_residual64<<<g, dim3(64, 4), 0, s>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr,
+                                                 assembled ? S->b64 : nullptr, rhs32, S->ref_x64, S->ref_r, S->partial,
+                                                 nullptr, nullptr, nullptr);
+    S->np_last = nblocks(g);
+    k_finalize<double><<<1, kBlock, 0, s>>>(S->sc, S->partial, S->np_last, 5, 0.0);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+// ref_x64 += ref_e
+int solver_refine_add(vflip_solver* S, hipStream_t s) {
+    k_add_f32_f64<<<grid_for(S->n), kBlock, 0, s>>>((unsigned)S->n, S->ref_e, S->ref_x64);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+// x32 = FP32(ref_x64), x64 (may be null) = ref_x64
+int solver_refine_end(vflip_solver* S, hipStream_t s, float* x32, double* x64) {
+    if (x32) k_cast_f64_f32<<<grid_for(S->n), kBlock, 0, s>>>((unsigned)S->n, S->ref_x64, x32);
+    if (x64) VF_HIP(hipMemcpyAsync(x64, S->ref_x64, S->n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+void solver_refine_stats(int iterations, bool converged, double true_res, double bnorm, double ms,
+                         vflip_solve_stats* st) {
+    if (!st) return;
+    vflip_solve_stats o{};
+    o.iterations = iterations;
+    o.converged = converged ? 1 : 0;
+    o.residual = true_res;
+    o.rel_residual = bnorm > 0 ? true_res / bnorm : 0.0;
+    o.true_residual = o.rel_residual;
+    o.ms_solve = ms;
+    o.guess_residual = 1.0;
+    *st = o;
+}
+
 extern "C" {
 
 int vflip_solver_solve(vflip_solver* S, const float* rhs, float* x, vflip_solve_stats* st) {
     VF_ARG(S && rhs && x, "solver/rhs/x");
     return solve_impl(S, rhs, nullptr, x, nullptr, st);
+}
+
+int vflip_solver_get_assembled64(vflip_solver* S, double* diag, double* rhs) {
+    VF_ARG(S && (diag || rhs), "solver/outputs");
+    VF_ARG(S->d64_valid && S->b64_valid, "the solver holds no assembled FP64 diagonal and right-hand side");
+    if (diag) VF_HIP(hipMemcpyAsync(diag, S->d64, S->n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+    if (rhs) VF_HIP(hipMemcpyAsync(rhs, S->b64, S->n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
+    VF_HIP(hipStreamSynchronize(S->stream));
+    return 0;
+}
+
+int vflip_solver_solve_refined(vflip_solver* S, const float* rhs, float* x, double* x64, int assembled, double tol,
+                               int max_rounds, vflip_solve_stats* st) {
+    VF_ARG(S && rhs && (x || x64), "solver/rhs/x");
+    VF_ARG(tol > 0 && max_rounds >= 0, "tolerance / rounds");
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = solver_refine_begin(S, S->stream, assembled != 0);
+    if (rc) return rc;
+    int its = 0;
+    double bnorm = 0, tr = 0;
+    bool ok = false;
+    for (int round = 0;; ++round) {
+        if ((rc = solver_refine_residual(S, S->stream, rhs, assembled != 0))) return rc;
+        if ((rc = fetch_scalars(S))) return rc;
+        tr = S->h_sc->true_res;
+        if (round == 0) bnorm = tr;
+        if (bnorm == 0 || tr <= tol * bnorm) {
+            ok = true;
+            break;
+        }
+        if (round == max_rounds) break;
+        vflip_solve_stats in{};
+        rc = solve_impl(S, S->ref_r, nullptr, S->ref_e, nullptr, &in);
+        if (rc < 0) return rc;
+        its += in.iterations;
+        if ((rc = solver_refine_add(S, S->stream))) return rc;
+    }
+    if ((rc = solver_refine_end(S, S->stream, x, x64))) return rc;
+    VF_HIP(hipStreamSynchronize(S->stream));
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    solver_refine_stats(its, ok, tr, bnorm, ms, st);
+    return ok ? VFLIP_OK : VFLIP_NOT_CONVERGED;
 }
 
 int vflip_solver_solve_f64(vflip_solver* S, const double* rhs, double* x, vflip_solve_stats* st) {

@@ -89,6 +89,11 @@ struct vflip_solver {
     const double* gram_p[2] = {nullptr, nullptr};  // accumulates the Gram row against these
     int gram_nb = 0;                    // workgroups of that true-residual launch
     double* f64_alloc = nullptr;        // base of the x64 / d64 / b64 allocation
+    // FP64 defect correction around FP32 solves (vflip_solver_solve_refined and the
+    // batched form): the FP64 iterate, the FP32 residual and correction
+    double* ref_x64 = nullptr;
+    float *ref_r = nullptr, *ref_e = nullptr;
+    void* ref_alloc = nullptr;
 };
 
 // Fused z-marching sweep geometry for a level. Rows of 128 or 256 cells use the
@@ -141,5 +146,14 @@ inline Op<float> make_op_f32(const LevelBuf& l) { return Op<float>{l.d, l.ax, l.
 
 // Outputs of an FP32 solve run by the batched subdomain solver (mgpcg.hip).
 int solver_finish(vflip_solver* S, const float* rhs32, float* x32, int iterations, int status, vflip_solve_stats* st);
+// FP64 defect correction rounds shared by vflip_solver_solve_refined and the batched
+// form (mgpcg.hip): begin (x = 0, buffers, setup), residual into ref_r and sc->true_res,
+// add the correction ref_e, end (outputs), stats.
+int solver_refine_begin(vflip_solver* S, hipStream_t s, bool assembled);
+int solver_refine_residual(vflip_solver* S, hipStream_t s, const float* rhs32, bool assembled);
+int solver_refine_add(vflip_solver* S, hipStream_t s);
+int solver_refine_end(vflip_solver* S, hipStream_t s, float* x32, double* x64);
+void solver_refine_stats(int iterations, bool converged, double true_res, double bnorm, double ms,
+                         vflip_solve_stats* st);
 // The coarsest-level table for the current operator, built if missing or stale (mgpcg.hip).
 int solver_ensure_ctab(vflip_solver* S);

@@ -18,8 +18,10 @@ its fine grid (S/coarse_fine_solver.cpp):
 Subdomain shapes: fine cells per axis drawn from {32, 48, 64, 96} per subdomain
 (seeded), boxes placed along the plume column and kept two coarse cells away
 from the walls. The solids are the coarse scene's spheres evaluated at the fine
-nodes. Fine solves run in FP32 to a 1e-6 relative residual with at most 100
-iterations (the reference's subdomain solve is FP64 to 1e-9).
+nodes. Fine solves reach the reference's subdomain tolerance, an FP64 true
+residual of 1e-9 relative (S/sub_domain.cpp:739): FP64 defect correction rounds
+around FP32 MGPCG solves of the corrections (vflip_batch_solve_refined, or
+vflip_solver_solve_refined per subdomain), each correction to `inner_tol`.
 """
 import time
 
@@ -46,7 +48,7 @@ def layout(n, count, seed, ratio=2):
 
 
 class Subdomain:
-    def __init__(self, c0, m, nf, ratio, hc, spheres, dt):
+    def __init__(self, c0, m, nf, ratio, hc, spheres, dt, inner_tol=1e-5):
         self.c0, self.m, self.nf, self.ratio = c0, m, nf, ratio
         self.hc, self.hf, self.dt = hc, hc / ratio, dt
         nx, ny, nz = nf
@@ -72,7 +74,8 @@ class Subdomain:
         self.uw, self.vw, self.ww = api.face_weights(self.phi)
         self.rhs = torch.zeros((nz, ny, nx), **f32)
         self.p = torch.zeros_like(self.rhs)
-        self.solver = api.Solver(nx, ny, nz, precision=api.FP32, tolerance=1e-6, max_iterations=100)
+        self.p64 = torch.zeros((nz, ny, nx), device=dev, dtype=torch.float64)
+        self.solver = api.Solver(nx, ny, nz, precision=api.FP32, tolerance=inner_tol, max_iterations=1000)
 
     def coarse_box(self, a, comp):
         """The coarse faces of component comp (0/1/2) inside and on this box."""
@@ -104,8 +107,9 @@ class Subdomain:
                              self.ww, self.us, self.vs, self.ws, self.rhs)
 
     def apply_gradient(self):
-        api.grad_apply(api.SUB, float(self.hf), float(self.dt), self.p, self.uw, self.vw, self.ww, self.us, self.vs,
-                       self.ws, self.u, self.v, self.w)
+        # from the FP64 pressure, as the reference's double pressure (S/sub_domain.cpp:745-819)
+        api.grad_apply_f64(api.SUB, float(self.hf), float(self.dt), self.p64, self.uw, self.vw, self.ww, self.us,
+                           self.vs, self.ws, self.u, self.v, self.w)
 
     def write_back(self, coarse):
         """Coarse faces strictly inside the box = mean of the fine faces over them."""
@@ -122,12 +126,15 @@ class Subdomain:
 class NestedPlume:
     """Coarse plume of n^3 cells plus `count` fine subdomains at `ratio`x."""
 
-    def __init__(self, n=256, count=12, seed=0, ratio=2, dt=0.01, batched=True, autotune=True, graph=True):
+    def __init__(self, n=256, count=12, seed=0, ratio=2, dt=0.01, batched=True, autotune=True, graph=True,
+                 fine_tol=1e-9, inner_tol=1e-5, max_rounds=8):
         self.n, self.ratio, self.dt = n, ratio, dt
         self.spheres = scenes.random_spheres(seed)
         phi = scenes.solid_phi(n, 1.0 / n, self.spheres)
         self.coarse = PlumeSim(n, phi, dt=dt)
-        self.subs = [Subdomain(c0, m, nf, ratio, 1.0 / n, self.spheres, dt) for c0, m, nf in layout(n, count, seed, ratio)]
+        self.subs = [Subdomain(c0, m, nf, ratio, 1.0 / n, self.spheres, dt, inner_tol)
+                     for c0, m, nf in layout(n, count, seed, ratio)]
+        self.fine_tol, self.max_rounds = fine_tol, max_rounds
         self.batched, self.graph = batched, graph
         self.batch = api.Batch([s.solver for s in self.subs]) if batched else None
         self.tiles = None
@@ -138,9 +145,12 @@ class NestedPlume:
         if self.batched:
             if self.autotune and self.tiles is None:
                 self.tiles = self.batch.autotune()
-            rc, stats, _ = self.batch.solve([s.rhs for s in self.subs], [s.p for s in self.subs], graph=self.graph)
+            rc, stats, _ = self.batch.solve_refined([s.rhs for s in self.subs], [s.p for s in self.subs],
+                                                    tol=self.fine_tol, max_rounds=self.max_rounds,
+                                                    x64=[s.p64 for s in self.subs], graph=self.graph)
             return stats
-        return [s.solver.solve(s.rhs, s.p) for s in self.subs]
+        return [s.solver.solve_refined(s.rhs, s.p, tol=self.fine_tol, max_rounds=self.max_rounds, x64=s.p64)
+                for s in self.subs]
 
     def step(self):
         sync = torch.cuda.synchronize
