@@ -83,39 +83,64 @@ struct GridF {
     float ox, oy, oz;
 };
 
-__device__ inline float atf(const GridF& g, int i, int j, int k) {
-    i = min(max(i, 0), g.nx - 1);
-    j = min(max(j, 0), g.ny - 1);
-    k = min(max(k, 0), g.nz - 1);
-    return g.f[(k * g.ny + j) * g.nx + i];
-}
 __device__ inline float lerpf(float a, float b, float c) { return __builtin_fmaf(c, b - a, a); }
 
 // Grid coordinate w/h + o split into its cell index and the fraction inside the
 // cell. q = w * (1/h) carries the FP32 rounding of the quotient; the exact
 // remainder w - q h (one FMA) gives it back, so the fraction is accurate to
 // ~1e-7 of a cell at any index (a plain FP32 w/h + o loses ~n * 6e-8 cells).
+// POW2 (h a power of two): q is exact, the remainder is exactly 0 and g - floor(g)
+// lies in [0, 1), so the general form reduces to one FMA, floor and difference
+// with the same result bits.
+template <bool POW2>
 __device__ inline void cell_coord(float w, float h, float inv_h, float o, int& i, float& f) {
-    const float q = w * inv_h;
-    const float c = __builtin_fmaf(-q, h, w) * inv_h;
-    const float g = q + o;  // o is 0 or 0.5: exact for |q| < 2^22
-    const float fl = floorf(g);
-    f = (g - fl) + c;
-    i = (int)fl;
-    if (f < 0.f) { f += 1.f; --i; }
-    else if (f >= 1.f) { f -= 1.f; ++i; }
+    if constexpr (POW2) {
+        const float g = __builtin_fmaf(w, inv_h, o);
+        const float fl = floorf(g);
+        f = g - fl;
+        i = (int)fl;
+    } else {
+        const float q = w * inv_h;
+        const float c = __builtin_fmaf(-q, h, w) * inv_h;
+        const float g = q + o;  // o is 0 or 0.5: exact for |q| < 2^22
+        const float fl = floorf(g);
+        f = (g - fl) + c;
+        i = (int)fl;
+        if (f < 0.f) { f += 1.f; --i; }
+        else if (f >= 1.f) { f -= 1.f; ++i; }
+    }
 }
 
+// The two clamped lattice lines around a coordinate along one axis, as byte
+// offsets (index * stride): the corner loads are one 32-bit add each from a
+// uniform base. stride < 2^24 and n * stride < 2^32 (checked by the entry points).
+template <bool POW2>
+__device__ inline void axis_lines(float w, float h, float inv_h, float o, int n, unsigned stride, unsigned& a0,
+                                  unsigned& a1, float& f) {
+    int i;
+    cell_coord<POW2>(w, h, inv_h, o, i, f);
+    a0 = __umul24((unsigned)min(max(i, 0), n - 1), stride);
+    a1 = __umul24((unsigned)min(max(i + 1, 0), n - 1), stride);
+}
+
+__device__ inline float ldb(const float* base, unsigned off) {
+    return *(const float*)((const char*)base + off);
+}
+
+// Trilinear sample in the order of the FP64 sampler: x lerps, then y, then z,
+// with every corner index clamped to the grid.
+template <bool POW2>
 __device__ inline float sample_linear_f(const GridF& g, float h, float inv_h, float wx, float wy, float wz) {
-    int i, j, k;
+    unsigned x0, x1, y0, y1, z0, z1;
     float cx, cy, cz;
-    cell_coord(wx, h, inv_h, g.ox, i, cx);
-    cell_coord(wy, h, inv_h, g.oy, j, cy);
-    cell_coord(wz, h, inv_h, g.oz, k, cz);
-    const float v1 = lerpf(lerpf(atf(g, i, j, k), atf(g, i + 1, j, k), cx),
-                           lerpf(atf(g, i, j + 1, k), atf(g, i + 1, j + 1, k), cx), cy);
-    const float v2 = lerpf(lerpf(atf(g, i, j, k + 1), atf(g, i + 1, j, k + 1), cx),
-                           lerpf(atf(g, i, j + 1, k + 1), atf(g, i + 1, j + 1, k + 1), cx), cy);
+    axis_lines<POW2>(wx, h, inv_h, g.ox, g.nx, 4u, x0, x1, cx);
+    axis_lines<POW2>(wy, h, inv_h, g.oy, g.ny, 4u * g.nx, y0, y1, cy);
+    axis_lines<POW2>(wz, h, inv_h, g.oz, g.nz, 4u * g.nx * g.ny, z0, z1, cz);
+    const unsigned p00 = y0 + z0, p10 = y1 + z0, p01 = y0 + z1, p11 = y1 + z1;
+    const float v1 = lerpf(lerpf(ldb(g.f, p00 + x0), ldb(g.f, p00 + x1), cx),
+                           lerpf(ldb(g.f, p10 + x0), ldb(g.f, p10 + x1), cx), cy);
+    const float v2 = lerpf(lerpf(ldb(g.f, p01 + x0), ldb(g.f, p01 + x1), cx),
+                           lerpf(ldb(g.f, p11 + x0), ldb(g.f, p11 + x1), cx), cy);
     return lerpf(v1, v2, cz);
 }
 
@@ -125,37 +150,38 @@ __device__ inline float sample_linear_f(const GridF& g, float h, float inv_h, fl
 // (up to the sign of a zero), so the value equals sample_linear_f's.
 template <bool AX, bool AY, bool AZ>
 __device__ inline float sample_lattice_f(const GridF& g, float h, float inv_h, float wx, float wy, float wz) {
-    int i, j, k;
+    unsigned x0, x1, y0, y1, z0, z1;
     float cx, cy, cz;
-    cell_coord(wx, h, inv_h, g.ox, i, cx);
-    cell_coord(wy, h, inv_h, g.oy, j, cy);
-    cell_coord(wz, h, inv_h, g.oz, k, cz);
-    auto row = [&](int jj, int kk) {
-        const float a = atf(g, i, jj, kk);
+    axis_lines<true>(wx, h, inv_h, g.ox, g.nx, 4u, x0, x1, cx);
+    axis_lines<true>(wy, h, inv_h, g.oy, g.ny, 4u * g.nx, y0, y1, cy);
+    axis_lines<true>(wz, h, inv_h, g.oz, g.nz, 4u * g.nx * g.ny, z0, z1, cz);
+    auto row = [&](unsigned p) {
+        const float a = ldb(g.f, p + x0);
         if constexpr (AX) return a;
-        else return lerpf(a, atf(g, i + 1, jj, kk), cx);
+        else return lerpf(a, ldb(g.f, p + x1), cx);
     };
-    auto plane = [&](int kk) {
-        const float a = row(j, kk);
+    auto plane = [&](unsigned z) {
+        const float a = row(y0 + z);
         if constexpr (AY) return a;
-        else return lerpf(a, row(j + 1, kk), cy);
+        else return lerpf(a, row(y1 + z), cy);
     };
-    const float v1 = plane(k);
+    const float v1 = plane(z0);
     if constexpr (AZ) return v1;
-    else return lerpf(v1, plane(k + 1), cz);
+    else return lerpf(v1, plane(z1), cz);
 }
 
 // First RK2 stage at the output sample's own position: with a power-of-two h
 // the position is exact, so along the axes where a velocity component's grid
 // offset equals the field's the fraction is exactly 0. FIELD: 0 u faces,
-// 1 v faces, 2 w faces, 3 cell centres; -1 general (all eight corners).
-template <int FIELD>
+// 1 v faces, 2 w faces, 3 cell centres; -1 general (all eight corners; POW2 as
+// in cell_coord).
+template <int FIELD, bool POW2>
 __device__ inline void first_stage(const GridF& gu, const GridF& gv, const GridF& gw, float h, float inv_h, float x,
                                    float y, float z, float& u, float& v, float& w) {
     if constexpr (FIELD < 0) {
-        u = sample_linear_f(gu, h, inv_h, x, y, z);
-        v = sample_linear_f(gv, h, inv_h, x, y, z);
-        w = sample_linear_f(gw, h, inv_h, x, y, z);
+        u = sample_linear_f<POW2>(gu, h, inv_h, x, y, z);
+        v = sample_linear_f<POW2>(gv, h, inv_h, x, y, z);
+        w = sample_linear_f<POW2>(gw, h, inv_h, x, y, z);
     } else {
         // field offsets (FIELD == axis: 0.5 on that axis); component c has 0.5 on axis c
         constexpr bool fx = FIELD == 0, fy = FIELD == 1, fz = FIELD == 2;
@@ -165,7 +191,7 @@ __device__ inline void first_stage(const GridF& gu, const GridF& gv, const GridF
     }
 }
 
-template <int FIELD>
+template <int FIELD, bool POW2>
 __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int nk, float h, float inv_h, float dt,
                                                            const float* __restrict__ u, const float* __restrict__ v,
                                                            const float* __restrict__ w, GridF f0,
@@ -186,14 +212,15 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int n
         wz[c] = ((float)k - f0.oz) * h;
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) first_stage<FIELD>(gu, gv, gw, h, inv_h, wx[c], wy[c], wz[c], uu[c], vv[c], ww[c]);
+    for (int c = 0; c < 2; ++c)
+        first_stage<FIELD, POW2>(gu, gv, gw, h, inv_h, wx[c], wy[c], wz[c], uu[c], vv[c], ww[c]);
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const float px = __builtin_fmaf(-hdt, uu[c], wx[c]), py = __builtin_fmaf(-hdt, vv[c], wy[c]);
         const float pz = __builtin_fmaf(-hdt, ww[c], wz[c]);
-        uu[c] = sample_linear_f(gu, h, inv_h, px, py, pz);
-        vv[c] = sample_linear_f(gv, h, inv_h, px, py, pz);
-        ww[c] = sample_linear_f(gw, h, inv_h, px, py, pz);
+        uu[c] = sample_linear_f<POW2>(gu, h, inv_h, px, py, pz);
+        vv[c] = sample_linear_f<POW2>(gv, h, inv_h, px, py, pz);
+        ww[c] = sample_linear_f<POW2>(gw, h, inv_h, px, py, pz);
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
@@ -202,8 +229,8 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int n
         // the reference's float ops: product, then difference
         const float px = wx[c] - dt * uu[c], py = wy[c] - dt * vv[c], pz = wz[c] - dt * ww[c];
         const int o = (k * f0.ny + j) * f0.nx + i;
-        out0[o] = sample_linear_f(f0, h, inv_h, px, py, pz);
-        if (out1) out1[o] = sample_linear_f(f1, h, inv_h, px, py, pz);
+        out0[o] = sample_linear_f<POW2>(f0, h, inv_h, px, py, pz);
+        if (out1) out1[o] = sample_linear_f<POW2>(f1, h, inv_h, px, py, pz);
     }
 }
 
@@ -219,13 +246,13 @@ template <int FIELD, bool LATTICE>
 __device__ inline void backtrace(const GridF& gu, const GridF& gv, const GridF& gw, float h, float inv_h, float dt,
                                  float x, float y, float z, float& px, float& py, float& pz) {
     float u, v, w;
-    if constexpr (LATTICE) first_stage<FIELD>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
-    else first_stage<-1>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
+    if constexpr (LATTICE) first_stage<FIELD, true>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
+    else first_stage<-1, false>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
     const float hdt = 0.5f * dt;
     const float mx = __builtin_fmaf(-hdt, u, x), my = __builtin_fmaf(-hdt, v, y), mz = __builtin_fmaf(-hdt, w, z);
-    u = sample_linear_f(gu, h, inv_h, mx, my, mz);
-    v = sample_linear_f(gv, h, inv_h, mx, my, mz);
-    w = sample_linear_f(gw, h, inv_h, mx, my, mz);
+    u = sample_linear_f<LATTICE>(gu, h, inv_h, mx, my, mz);
+    v = sample_linear_f<LATTICE>(gv, h, inv_h, mx, my, mz);
+    w = sample_linear_f<LATTICE>(gw, h, inv_h, mx, my, mz);
     px = x - dt * u;
     py = y - dt * v;
     pz = z - dt * w;
@@ -250,22 +277,22 @@ __global__ __launch_bounds__(256) void k_advect_plume(int ni, int nj, int nk, fl
         float px, py, pz;
         backtrace<3, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, z, px, py, pz);
         const int c = (k * nj + j) * ni + i;
-        rho2[c] = sample_linear_f(gr, h, inv_h, px, py, pz);
-        temp2[c] = sample_linear_f(gt, h, inv_h, px, py, pz);
+        rho2[c] = sample_linear_f<LATTICE>(gr, h, inv_h, px, py, pz);
+        temp2[c] = sample_linear_f<LATTICE>(gt, h, inv_h, px, py, pz);
         for (int q = 0; q < 1 + (i == ni - 1); ++q) {
             const int fi = i + q;
             backtrace<0, LATTICE>(gu, gv, gw, h, inv_h, dt, ((float)fi - 0.5f) * h, y, z, px, py, pz);
-            u2[(k * nj + j) * (ni + 1) + fi] = sample_linear_f(gu, h, inv_h, px, py, pz);
+            u2[(k * nj + j) * (ni + 1) + fi] = sample_linear_f<LATTICE>(gu, h, inv_h, px, py, pz);
         }
         for (int q = 0; q < 1 + (j == nj - 1); ++q) {
             const int fj = j + q;
             backtrace<1, LATTICE>(gu, gv, gw, h, inv_h, dt, x, ((float)fj - 0.5f) * h, z, px, py, pz);
-            v2[(k * (nj + 1) + fj) * ni + i] = sample_linear_f(gv, h, inv_h, px, py, pz);
+            v2[(k * (nj + 1) + fj) * ni + i] = sample_linear_f<LATTICE>(gv, h, inv_h, px, py, pz);
         }
         for (int q = 0; q < 1 + (k == nk - 1); ++q) {
             const int fk = k + q;
             backtrace<2, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, ((float)fk - 0.5f) * h, px, py, pz);
-            w2[(fk * nj + j) * ni + i] = sample_linear_f(gw, h, inv_h, px, py, pz);
+            w2[(fk * nj + j) * ni + i] = sample_linear_f<LATTICE>(gw, h, inv_h, px, py, pz);
         }
     }
 }
@@ -302,6 +329,15 @@ __global__ __launch_bounds__(kBlock) void k_buoyancy(int ni, int nj, int nk, flo
 
 using namespace vflip;
 
+namespace {
+// Every MAC component of an ni x nj x nk grid within the FP32 samplers' byte
+// offsets: fewer than 2^30 samples (offsets < 2^32) and 2^22 per plane (the
+// plane stride in bytes < 2^24 for the 24-bit multiply).
+bool f32_addressable(int ni, int nj, int nk) {
+    return (size_t)(ni + 1) * (nj + 1) * (nk + 1) < (1u << 30) && (size_t)(ni + 1) * (nj + 1) < (1u << 22);
+}
+}  // namespace
+
 extern "C" {
 
 int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const float* u, const float* v,
@@ -313,6 +349,9 @@ int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const 
     VF_ARG((field2 == nullptr) == (out2 == nullptr), "second field and its output go together");
     VF_ARG((size_t)fnx * fny * fnz < (1u << 31) && (size_t)(ni + 1) * (nj + 1) * (nk + 1) < (1u << 31), "grid size");
     if (mode == 2) {
+        // the FP32 samplers address corners by 32-bit byte offsets with 24-bit strides
+        VF_ARG(f32_addressable(ni, nj, nk) && (size_t)fnx * fny * fnz < (1u << 30) && (size_t)fnx * fny < (1u << 22),
+               "grid too large for mode 2 (at most 2^30 samples per grid, 2^22 per plane)");
         GridF g0{field, fnx, fny, fnz, fox, foy, foz};
         GridF g1{field2 ? field2 : field, fnx, fny, fnz, fox, foy, foz};
         dim3 grid((fnx + 63) / 64, (fny + 3) / 4, (fnz + 1) / 2);
@@ -330,11 +369,15 @@ int vflip_advect_rk2(int ni, int nj, int nk, float h, float dt, int mode, const 
         const float inv_h = 1.0f / h;
         hipStream_t st = (hipStream_t)stream;
         switch (field_kind) {
-            case 0: k_advect_rk2_f32<0><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
-            case 1: k_advect_rk2_f32<1><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
-            case 2: k_advect_rk2_f32<2><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
-            case 3: k_advect_rk2_f32<3><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
-            default: k_advect_rk2_f32<-1><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2);
+            case 0: k_advect_rk2_f32<0, true><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            case 1: k_advect_rk2_f32<1, true><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            case 2: k_advect_rk2_f32<2, true><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            case 3: k_advect_rk2_f32<3, true><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2); break;
+            default:
+                if (pow2)
+                    k_advect_rk2_f32<-1, true><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2);
+                else
+                    k_advect_rk2_f32<-1, false><<<grid, dim3(64, 4), 0, st>>>(ni, nj, nk, h, inv_h, dt, u, v, w, g0, out, g1, out2);
         }
         VF_CHECK_LAUNCH();
         return 0;
@@ -353,7 +396,7 @@ int vflip_advect_plume(int ni, int nj, int nk, float h, float dt, const float* u
     VF_ARG(ni > 0 && nj > 0 && nk > 0, "grid dims");
     VF_ARG(u && v && w && rho && temp && u2 && v2 && w2 && rho2 && temp2, "advection arrays");
     VF_ARG(u2 != u && v2 != v && w2 != w && rho2 != rho && temp2 != temp, "outputs must not alias the inputs");
-    VF_ARG((size_t)(ni + 1) * (nj + 1) * (nk + 1) < (1u << 31), "grid size");
+    VF_ARG(f32_addressable(ni, nj, nk), "grid too large (at most 2^30 samples per MAC grid, 2^22 per plane)");
     int e2 = 0;
     const bool pow2 = std::frexp(h, &e2) == 0.5f;
     const dim3 grid((ni + 63) / 64, (nj + 3) / 4, (nk + 1) / 2);
