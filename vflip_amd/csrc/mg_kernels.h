@@ -1264,6 +1264,77 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
     }
 }
 
+// Residual r = b - A x of an FP64 guess (the terms and FP64 arithmetic of
+// k_true_residual64, rounded to FP32 per cell), restricted by 4 per axis for the
+// network's input without storing r: crhs = mean of r over each 4^3 block, csolid =
+// the fraction of its cells with diagonal 0, cliquid = 1 - csolid. Block 64 x 4: a
+// thread per fine cell of 4 rows (one coarse row), marching kRR4Planes planes with
+// x and the z coupling in registers, as k_true_residual64. Sum order: the 4 planes
+// of a coarse plane per thread, then the 4 x-lanes of a coarse cell in pairs
+// (shuffles), then the 4 rows in pairs (LDS). Needs ni, nj, nk divisible by 4.
+constexpr int kRR4Planes = 16;
+static __global__ __launch_bounds__(256) void k_residual_restrict4(Op<float> op, const double* __restrict__ d64,
+                                                                   const double* __restrict__ b64,
+                                                                   const float* __restrict__ b32,
+                                                                   const double* __restrict__ x,
+                                                                   float* __restrict__ crhs,
+                                                                   float* __restrict__ csolid,
+                                                                   float* __restrict__ cliquid) {
+    __shared__ float sr[4][16], ss[4][16];
+    const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
+    const int cx = ni / 4, cy = nj / 4;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = blockIdx.x * 64 + tx, j = blockIdx.y * 4 + ty;
+    const int kb = blockIdx.z * kRR4Planes, ke = min(kb + kRR4Planes, nk);
+    const bool in = i < ni;
+    int c = (kb * nj + j) * ni + i;
+    double xm = (in && kb > 0) ? x[c - sl] : 0.0, x0 = in ? x[c] : 0.0;
+    float azm = (in && kb > 0) ? op.AZ(c - sl) : 0.f;
+    float acc = 0.f, cnt = 0.f;
+    for (int k = kb; k < ke; ++k, c += sl) {
+        if (in) {
+            const double xp = k + 1 < nk ? x[c + sl] : 0.0;
+            const float az0 = k + 1 < nk ? op.AZ(c) : 0.f;
+            double y = (d64 ? d64[c] : (double)op.D(c)) * x0;
+            if (k > 0) y += (double)azm * xm;
+            if (j > 0) y += (double)op.AY(c - ni) * x[c - ni];
+            if (i > 0) y += (double)op.AX(c - 1) * x[c - 1];
+            if (i + 1 < ni) y += (double)op.AX(c) * x[c + 1];
+            if (j + 1 < nj) y += (double)op.AY(c) * x[c + ni];
+            if (k + 1 < nk) y += (double)az0 * xp;
+            acc += (float)((b64 ? b64[c] : (double)b32[c]) - y);
+            cnt += op.D(c) == 0.f ? 1.f : 0.f;
+            xm = x0;
+            x0 = xp;
+            azm = az0;
+        }
+        if ((k & 3) == 3) {  // the last plane of a coarse plane (uniform over the block)
+            acc += __shfl_xor(acc, 1, 64);
+            acc += __shfl_xor(acc, 2, 64);
+            cnt += __shfl_xor(cnt, 1, 64);
+            cnt += __shfl_xor(cnt, 2, 64);
+            if ((tx & 3) == 0) {
+                sr[ty][tx >> 2] = acc;
+                ss[ty][tx >> 2] = cnt;
+            }
+            __syncthreads();
+            const int ic = blockIdx.x * 16 + tx;
+            if (ty == 0 && tx < 16 && ic < cx) {
+                const float inv = 1.0f / 64.0f;
+                const float r = ((sr[0][tx] + sr[1][tx]) + (sr[2][tx] + sr[3][tx])) * inv;
+                const float so = ((ss[0][tx] + ss[1][tx]) + (ss[2][tx] + ss[3][tx])) * inv;
+                const size_t cc = ((size_t)(k >> 2) * cy + blockIdx.y) * cx + ic;
+                crhs[cc] = r;
+                csolid[cc] = so;
+                cliquid[cc] = 1.f - so;
+            }
+            __syncthreads();
+            acc = 0.f;
+            cnt = 0.f;
+        }
+    }
+}
+
 // Energy of an FP32 initial guess before its FP64 residual: partial sums of
 // x.(2b - A x), A x from the FP32 level-0 operator accumulated in FP64, so that
 // E = -sum / 2 as from k_energy_dots (x.(b + r) with r = b - A x). Launched as

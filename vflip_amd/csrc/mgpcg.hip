@@ -1416,6 +1416,24 @@ int solver_guess_residual(vflip_solver* S, const float* rhs32, const double* g, 
     *r = S->r;
     return 0;
 }
+// The guess residual restricted by 4 per axis into the network's input grid
+// (k_residual_restrict4): no full-resolution residual is stored.
+int solver_guess_residual_restricted(vflip_solver* S, const float* rhs32, const double* g, float* crhs,
+                                     float* csolid, float* cliquid) {
+    int rc = ensure_f64(S);
+    if (rc) return rc;
+    const LevelBuf& l0 = S->L[0];
+    if (l0.ni % 4 || l0.nj % 4 || l0.nk % 4) {
+        set_error("restricted guess residual: grid %dx%dx%d not divisible by 4", l0.ni, l0.nj, l0.nk);
+        return VFLIP_ERR_ARG;
+    }
+    const dim3 grid((l0.ni + 63) / 64, l0.nj / 4, (l0.nk + kRR4Planes - 1) / kRR4Planes);
+    k_residual_restrict4<<<grid, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr,
+                                                               (mixed(S) && S->b64_valid) ? S->b64 : nullptr, rhs32,
+                                                               g, crhs, csolid, cliquid);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
 // FP32 scratch of the solver's size that is free until the next solve.
 float* solver_scratch32(vflip_solver* S) { return S->z; }
 // guess += correction (FP32)

@@ -383,6 +383,10 @@ int solver_guess_residual(vflip_solver* S, const float* rhs32, const double* g, 
 float* solver_scratch32(vflip_solver* S);
 int solver_add_correction(vflip_solver* S, double* g, const float* corr);
 int solver_solve_history(vflip_solver* S, const float* rhs32, double* g, float* x32, vflip_solve_stats* st, bool warm);
+int solver_guess_residual_restricted(vflip_solver* S, const float* rhs32, const double* g, float* crhs,
+                                     float* csolid, float* cliquid);
+int unet_warm_inputs(vflip_unet* N, int cx, int cy, int cz, float** crhs, float** csolid, float** cliquid);
+int unet_warm_predict_add(vflip_unet* N, int ni, int nj, int nk, int factor, const float* diag, double* g);
 }  // namespace vflip
 
 extern "C" {
@@ -474,16 +478,26 @@ int project_history_impl(vflip_solver* S, vflip_unet* net, int factor, int varia
     if ((rc = solver_history_guess(S, rhs_scratch, &g, &used))) return rc;
     bool warm = used > 0;
     if (net) {
+        // the correction from the network, on the residual the guess leaves: restricted
+        // by 4 straight from the guess (no full-resolution residual), prolonged back
+        // and added into the FP64 guess
         int ni, nj, nk;
         float *d, *ax, *ay, *az, *lk;
         hipStream_t s;
         if ((rc = solver_level0(S, &ni, &nj, &nk, &d, &ax, &ay, &az, &lk, &s))) return rc;
-        float* r = nullptr;
-        if ((rc = solver_guess_residual(S, rhs_scratch, g, &r))) return rc;
-        float* corr = solver_scratch32(S);
         if ((rc = vflip_unet_set_stream(net, s))) return rc;
-        if ((rc = vflip_unet_warm_start(net, ni, nj, nk, factor, r, d, corr))) return rc;
-        if ((rc = solver_add_correction(S, g, corr))) return rc;
+        if (factor == 4 && ni % 4 == 0 && nj % 4 == 0 && nk % 4 == 0) {
+            float *crhs, *csolid, *cliquid;
+            if ((rc = unet_warm_inputs(net, ni / 4, nj / 4, nk / 4, &crhs, &csolid, &cliquid))) return rc;
+            if ((rc = solver_guess_residual_restricted(S, rhs_scratch, g, crhs, csolid, cliquid))) return rc;
+            if ((rc = unet_warm_predict_add(net, ni, nj, nk, 4, d, g))) return rc;
+        } else {
+            float* r = nullptr;
+            if ((rc = solver_guess_residual(S, rhs_scratch, g, &r))) return rc;
+            float* corr = solver_scratch32(S);
+            if ((rc = vflip_unet_warm_start(net, ni, nj, nk, factor, r, d, corr))) return rc;
+            if ((rc = solver_add_correction(S, g, corr))) return rc;
+        }
         warm = true;
     }
     const int status = solver_solve_history(S, rhs_scratch, g, pressure, stats, warm);

@@ -421,6 +421,39 @@ int ensure_warm_grid(vflip_unet* N, int cx, int cy, int cz) {
 
 }  // namespace
 
+namespace vflip {
+// The network's warm-start input buffers for a network grid of cx x cy x cz
+// (restricted residual, solid and liquid fractions), allocated on first use.
+int unet_warm_inputs(vflip_unet* N, int cx, int cy, int cz, float** crhs, float** csolid, float** cliquid) {
+    VF_ARG(N && crhs && csolid && cliquid, "warm inputs");
+    VF_ARG(dims_ok(cx, cy, cz), "network grid dims must be positive multiples of 8");
+    if (int rc = ensure_warm_grid(N, cx, cy, cz)) return rc;
+    if (int rc = ensure_grid(N, cx, cy, cz)) return rc;
+    *crhs = N->w_rhs;
+    *csolid = N->w_solid;
+    *cliquid = N->w_liquid;
+    return 0;
+}
+
+// The network on its filled input buffers (automatic normalisation), its prediction
+// prolonged by `factor` and added into the FP64 guess g of the ni x nj x nk solve
+// (cells with diag 0 unchanged). Asynchronous.
+int unet_warm_predict_add(vflip_unet* N, int ni, int nj, int nk, int factor, const float* diag, double* g) {
+    VF_ARG(N && g && factor >= 1, "warm predict args");
+    const int cx = ni / factor, cy = nj / factor, cz = nk / factor;
+    VF_ARG(cx * factor == ni && cy * factor == nj && cz * factor == nk && N->wx == cx && N->wy == cy && N->wz == cz,
+           "warm inputs not prepared for this grid");
+    const size_t Vc = (size_t)cx * cy * cz;
+    const float* sd = prepare_scale(N, N->w_rhs, Vc, 0.f);
+    k_pack_input<<<grid256(Vc), 256, 0, N->stream>>>(N->w_rhs, N->w_solid, N->w_liquid, Vc, 0.f, sd, N->act[kIn]);
+    forward_layers(N);
+    k_prolong_add64<<<dim3((ni + 255) / 256, nj, nk), 256, 0, N->stream>>>(N->pred, cx, cy, cz, factor, sd, N->gain,
+                                                                          diag, g);
+    VF_CHECK_LAUNCH();
+    return 0;
+}
+}  // namespace vflip
+
 extern "C" {
 
 int vflip_unet_create(unsigned seed, vflip_unet** out) {

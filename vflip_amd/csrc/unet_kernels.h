@@ -119,38 +119,51 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
     constexpr int STEPS = CK == 32 ? 27 : 14;
     const int total = ((a.cin + CK - 1) / CK) * STEPS;
     const int q0 = (int)((long long)ks * total / nks), q1 = (int)((long long)(ks + 1) * total / nks);
+    // K step q: tap and channel offset of this lane's 8 k values
+    auto step_k = [&](int q, int& tap, int& coff) {
+        const int st = q % STEPS;
+        if (CK == 32) {
+            tap = st;
+            coff = kg * 8;
+        } else {
+            tap = 2 * st + (kg >> 1);  // tap 27 is the zero pad of the weights
+            coff = (kg & 1) * 8;
+        }
+    };
+    // the weight fragments of step q (global loads; issued one step ahead so that
+    // their latency overlaps the previous step's matrix-core work)
+    auto load_b = [&](int q, bf16x8 (&bf)[NB]) {
+        int tap, coff;
+        step_k(q, tap, coff);
+        const size_t kidx = (size_t)tap * a.cin + (q / STEPS) * CK + coff;
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb)
+            bf[nb] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(cbase + nb * 16 + r16) * a.kw + kidx);
+    };
+    bf16x8 bcur[NB], bnext[NB];
+    if (q0 < q1) load_b(q0, bcur);
     int staged = -1;
     for (int q = q0; q < q1; ++q) {
-        const int c0 = (q / STEPS) * CK, st = q % STEPS;
+        const int c0 = (q / STEPS) * CK;
+        if (q + 1 < q1) load_b(q + 1, bnext);
         if (c0 != staged) {  // uniform over the workgroup
             __syncthreads();
             stage_halo<CK>(a, c0, x0, y0, z0, lds);
             __syncthreads();
             staged = c0;
         }
-        {
-            int tap, coff;
-            if (CK == 32) {
-                tap = st;
-                coff = kg * 8;
-            } else {
-                tap = 2 * st + (kg >> 1);  // tap 27 is the zero pad of the weights
-                coff = (kg & 1) * 8;
-            }
-            const int dhv = tap_offset_hv(tap < 27 ? tap : 0);
-            const size_t kidx = (size_t)tap * a.cin + c0 + coff;
-            bf16x8 bfrag[NB];
+        int tap, coff;
+        step_k(q, tap, coff);
+        const int dhv = tap_offset_hv(tap < 27 ? tap : 0);
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+            const bf16x8 afrag = *reinterpret_cast<const bf16x8*>(lds + (hv0[mb] + dhv) * (CK + 8) + coff);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
-                bfrag[nb] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(cbase + nb * 16 + r16) * a.kw + kidx);
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) {
-                const bf16x8 afrag = *reinterpret_cast<const bf16x8*>(lds + (hv0[mb] + dhv) * (CK + 8) + coff);
-#pragma unroll
-                for (int nb = 0; nb < NB; ++nb)
-                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bfrag[nb], acc[mb][nb], 0, 0, 0);
-            }
+                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bcur[nb], acc[mb][nb], 0, 0, 0);
         }
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) bcur[nb] = bnext[nb];
     }
     // epilogue: D reg r of lane -> voxel row kg*4 + r of the M block, channel nb*16 + r16
     const int z = z0 + w;
@@ -598,6 +611,33 @@ __global__ __launch_bounds__(256) void k_prolong_cells(const float* __restrict__
         const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
         x[v] = (c0 + fz * (c1 - c0)) * m;
     }
+}
+
+// k_prolong_cells added into an FP64 guess: g += the prolonged value (same
+// interpolation and scale; cells with diag 0 are left as they are).
+__global__ __launch_bounds__(256) void k_prolong_add64(const float* __restrict__ cp, int cx, int cy, int cz, int f,
+                                                       const float* __restrict__ s_dev, float gain,
+                                                       const float* __restrict__ diag, double* __restrict__ g) {
+    const int nx = cx * f, ny = cy * f;
+    const int i = blockIdx.x * 256 + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= nx) return;
+    const size_t v = ((size_t)k * ny + j) * nx + i;
+    if (diag && diag[v] == 0.f) return;
+    const float m = s_dev ? s_dev[0] * gain : 1.0f;
+    const float invf = 1.0f / (float)f;
+    const float gx = fminf(fmaxf((i + 0.5f) * invf - 0.5f, 0.f), (float)(cx - 1));
+    const float gy = fminf(fmaxf((j + 0.5f) * invf - 0.5f, 0.f), (float)(cy - 1));
+    const float gz = fminf(fmaxf((k + 0.5f) * invf - 0.5f, 0.f), (float)(cz - 1));
+    const int i0 = (int)gx, j0 = (int)gy, k0 = (int)gz;
+    const int i1 = min(i0 + 1, cx - 1), j1 = min(j0 + 1, cy - 1), k1 = min(k0 + 1, cz - 1);
+    const float fx = gx - i0, fy = gy - j0, fz = gz - k0;
+    auto at = [&](int a, int b, int e) { return cp[((size_t)e * cy + b) * cx + a]; };
+    const float c00 = at(i0, j0, k0) + fx * (at(i1, j0, k0) - at(i0, j0, k0));
+    const float c10 = at(i0, j1, k0) + fx * (at(i1, j1, k0) - at(i0, j1, k0));
+    const float c01 = at(i0, j0, k1) + fx * (at(i1, j0, k1) - at(i0, j0, k1));
+    const float c11 = at(i0, j1, k1) + fx * (at(i1, j1, k1) - at(i0, j1, k1));
+    const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+    g[v] += (double)((c0 + fz * (c1 - c0)) * m);
 }
 
 __global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ a, size_t n, bf16* __restrict__ b) {
