@@ -621,7 +621,9 @@ int vflip_batch_create(int n, vflip_solver* const* solvers, vflip_batch** out) {
     vflip_batch* B = new vflip_batch();
     B->S.assign(solvers, solvers + n);
     int rc = 0;
-    if (hipStreamCreate(&B->stream) != hipSuccess || hipEventCreate(&B->e0) != hipSuccess ||
+    // non-blocking: the batch orders itself against the solvers' streams explicitly (it
+    // synchronises them on entry and its own stream on exit)
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&B->e0) != hipSuccess ||
         hipEventCreate(&B->e1) != hipSuccess) {
         set_error("hipStreamCreate/hipEventCreate failed");
         rc = VFLIP_ERR_HIP;
