@@ -29,6 +29,23 @@ HIPCC_FLAGS = [
 ]
 
 
+PROBE_SRC = os.path.join(HERE, "probe", "hbm_probe.hip")
+PROBE_SO = os.path.join(HERE, "libvflip_probe.so")
+
+
+def build_probe(verbose=False):
+    """The machine probes behind the roofline statements (vflip_amd.bench_stream)."""
+    tmp = PROBE_SO + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-shared", "-o", tmp, PROBE_SRC]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("probe build failed:\n" + r.stdout)
+    os.replace(tmp, PROBE_SO)
+    return PROBE_SO
+
+
 FACADE_DIR = os.path.join(HERE, "facade")
 FACADE_SO = os.path.join(HERE, "libvflip_ivock.so")
 FACADE_CHECK = os.path.join(FACADE_DIR, "facade_check")
@@ -93,6 +110,7 @@ def build(verbose=False, jobs=8):
         raise RuntimeError("link failed:\n" + r.stdout)
     os.replace(tmp, OUT)
     build_facade(verbose)
+    build_probe(verbose)
     return OUT
 
 
