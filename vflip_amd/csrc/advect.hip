@@ -212,8 +212,10 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int n
         wz[c] = ((float)k - f0.oz) * h;
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c)
-        first_stage<FIELD, POW2>(gu, gv, gw, h, inv_h, wx[c], wy[c], wz[c], uu[c], vv[c], ww[c]);
+    for (int c = 0; c < 2; ++c) {
+        if constexpr (FIELD >= 0) first_stage_idx<FIELD>(gu, gv, gw, i, j, min(kb + c, f0.nz - 1), uu[c], vv[c], ww[c]);
+        else first_stage<FIELD, POW2>(gu, gv, gw, h, inv_h, wx[c], wy[c], wz[c], uu[c], vv[c], ww[c]);
+    }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         const float px = __builtin_fmaf(-hdt, uu[c], wx[c]), py = __builtin_fmaf(-hdt, vv[c], wy[c]);
@@ -240,13 +242,49 @@ __global__ __launch_bounds__(kBlock) void k_advect_rk2_f32(int ni, int nj, int n
 // their rho, T centres (the faces i = ni, j = nj, k = nk come from the last cell
 // of each row). Every value is the one k_advect_rk2_f32 computes (same samplers).
 // ----------------------------------------------------------------------------
+// first_stage<FIELD, true> with the lattice point's integer indices: the output
+// sample of field FIELD at index (i, j, k) of its own grid (u face i, v face j, w
+// face k, or cell centre) sits on each velocity component's lattice or halfway
+// between two of its lines, so every fraction is 0 or 0.5 and every clamp is known;
+// the loads and lerps are sample_lattice_f's in its order (same values, no
+// coordinate arithmetic). ni = gv.nx, nj = gu.ny, nk = gu.nz.
+template <int FIELD>
+__device__ inline void first_stage_idx(const GridF& gu, const GridF& gv, const GridF& gw, int i, int j, int k,
+                                       float& u, float& v, float& w) {
+    const int ni = gv.nx, nj = gu.ny, nk = gu.nz;
+    // 32-bit byte offsets from the uniform base, as the samplers (indices are in range)
+    auto U = [&](int a, int b, int c) { return ldb(gu.f, 4u * (unsigned)((c * gu.ny + b) * gu.nx + a)); };
+    auto Vv = [&](int a, int b, int c) { return ldb(gv.f, 4u * (unsigned)((c * gv.ny + b) * gv.nx + a)); };
+    auto W = [&](int a, int b, int c) { return ldb(gw.f, 4u * (unsigned)((c * gw.ny + b) * gw.nx + a)); };
+    if constexpr (FIELD == 3) {  // cell centre (i, j, k)
+        u = lerpf(U(i, j, k), U(i + 1, j, k), 0.5f);
+        v = lerpf(Vv(i, j, k), Vv(i, j + 1, k), 0.5f);
+        w = lerpf(W(i, j, k), W(i, j, k + 1), 0.5f);
+    } else if constexpr (FIELD == 0) {  // u face i of row j, plane k
+        const int i0 = max(i - 1, 0), i1 = min(i, ni - 1);
+        u = U(i, j, k);
+        v = lerpf(lerpf(Vv(i0, j, k), Vv(i1, j, k), 0.5f), lerpf(Vv(i0, j + 1, k), Vv(i1, j + 1, k), 0.5f), 0.5f);
+        w = lerpf(lerpf(W(i0, j, k), W(i1, j, k), 0.5f), lerpf(W(i0, j, k + 1), W(i1, j, k + 1), 0.5f), 0.5f);
+    } else if constexpr (FIELD == 1) {  // v face j of column i, plane k
+        const int j0 = max(j - 1, 0), j1 = min(j, nj - 1);
+        u = lerpf(lerpf(U(i, j0, k), U(i + 1, j0, k), 0.5f), lerpf(U(i, j1, k), U(i + 1, j1, k), 0.5f), 0.5f);
+        v = Vv(i, j, k);
+        w = lerpf(lerpf(W(i, j0, k), W(i, j1, k), 0.5f), lerpf(W(i, j0, k + 1), W(i, j1, k + 1), 0.5f), 0.5f);
+    } else {  // w face k of column i, row j
+        const int k0 = max(k - 1, 0), k1 = min(k, nk - 1);
+        u = lerpf(lerpf(U(i, j, k0), U(i + 1, j, k0), 0.5f), lerpf(U(i, j, k1), U(i + 1, j, k1), 0.5f), 0.5f);
+        v = lerpf(lerpf(Vv(i, j, k0), Vv(i, j + 1, k0), 0.5f), lerpf(Vv(i, j, k1), Vv(i, j + 1, k1), 0.5f), 0.5f);
+        w = W(i, j, k);
+    }
+}
+
 // RK2 backtrace of the sample at (x, y, z): k_advect_rk2_f32's stages (FIELD as
 // first_stage; LATTICE: its lattice form, valid for a power-of-two h).
 template <int FIELD, bool LATTICE>
 __device__ inline void backtrace(const GridF& gu, const GridF& gv, const GridF& gw, float h, float inv_h, float dt,
-                                 float x, float y, float z, float& px, float& py, float& pz) {
+                                 float x, float y, float z, int i, int j, int k, float& px, float& py, float& pz) {
     float u, v, w;
-    if constexpr (LATTICE) first_stage<FIELD, true>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
+    if constexpr (LATTICE) first_stage_idx<FIELD>(gu, gv, gw, i, j, k, u, v, w);
     else first_stage<-1, false>(gu, gv, gw, h, inv_h, x, y, z, u, v, w);
     const float hdt = 0.5f * dt;
     const float mx = __builtin_fmaf(-hdt, u, x), my = __builtin_fmaf(-hdt, v, y), mz = __builtin_fmaf(-hdt, w, z);
@@ -275,23 +313,23 @@ __global__ __launch_bounds__(256) void k_advect_plume(int ni, int nj, int nk, fl
         if (k >= nk) break;
         const float x = (float)i * h, y = (float)j * h, z = (float)k * h;
         float px, py, pz;
-        backtrace<3, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, z, px, py, pz);
+        backtrace<3, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, z, i, j, k, px, py, pz);
         const int c = (k * nj + j) * ni + i;
         rho2[c] = sample_linear_f<LATTICE>(gr, h, inv_h, px, py, pz);
         temp2[c] = sample_linear_f<LATTICE>(gt, h, inv_h, px, py, pz);
         for (int q = 0; q < 1 + (i == ni - 1); ++q) {
             const int fi = i + q;
-            backtrace<0, LATTICE>(gu, gv, gw, h, inv_h, dt, ((float)fi - 0.5f) * h, y, z, px, py, pz);
+            backtrace<0, LATTICE>(gu, gv, gw, h, inv_h, dt, ((float)fi - 0.5f) * h, y, z, fi, j, k, px, py, pz);
             u2[(k * nj + j) * (ni + 1) + fi] = sample_linear_f<LATTICE>(gu, h, inv_h, px, py, pz);
         }
         for (int q = 0; q < 1 + (j == nj - 1); ++q) {
             const int fj = j + q;
-            backtrace<1, LATTICE>(gu, gv, gw, h, inv_h, dt, x, ((float)fj - 0.5f) * h, z, px, py, pz);
+            backtrace<1, LATTICE>(gu, gv, gw, h, inv_h, dt, x, ((float)fj - 0.5f) * h, z, i, fj, k, px, py, pz);
             v2[(k * (nj + 1) + fj) * ni + i] = sample_linear_f<LATTICE>(gv, h, inv_h, px, py, pz);
         }
         for (int q = 0; q < 1 + (k == nk - 1); ++q) {
             const int fk = k + q;
-            backtrace<2, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, ((float)fk - 0.5f) * h, px, py, pz);
+            backtrace<2, LATTICE>(gu, gv, gw, h, inv_h, dt, x, y, ((float)fk - 0.5f) * h, i, j, fk, px, py, pz);
             w2[(fk * nj + j) * ni + i] = sample_linear_f<LATTICE>(gw, h, inv_h, px, py, pz);
         }
     }
