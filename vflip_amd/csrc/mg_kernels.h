@@ -990,18 +990,6 @@ struct Scalars {
     double energy;  // E(x0) = x0'A x0 / 2 - b'x0 of a warm start
 };
 
-// Partial sums of x.(b + r) for the energy of an initial guess:
-// E(x) = x'Ax/2 - b'x = -(x.b + x.r)/2 with r = b - A x.
-static __global__ __launch_bounds__(kBlock) void k_energy_dots(unsigned n, const double* __restrict__ x,
-                                                       const double* __restrict__ b64, const float* __restrict__ b32,
-                                                       const float* __restrict__ r, double* __restrict__ partial) {
-    double s = 0;
-    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n; c += gridDim.x * kBlock)
-        s += x[c] * ((b64 ? b64[c] : (double)b32[c]) + (double)r[c]);
-    const double t = block_sum(s);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
-}
-
 // q = A s and sum(s.q); 3D launch (64 x 4 threads, kMatvecKZ planes per block).
 template <typename CT, typename Acc>
 __device__ inline void matvec_dot_body(const Op<CT>& op, const float* __restrict__ s, float* __restrict__ q,
@@ -1205,7 +1193,8 @@ __global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __
 // r = b - A x in FP64 (FP64 iterate, FP64 diagonal/rhs when given), partial
 // max|r|. A thread marches kTrueResKZ planes up z with x and the z coupling of
 // the plane below kept in registers; the terms are summed in the order
-// d x, -z, -y, -x, +x, +y, +z.
+// d x, -z, -y, -x, +x, +y, +z. epart (optional): per-block sums of x (b + r)
+// for the energy of x: E(x) = x'Ax/2 - b'x = -(x.b + x.r)/2 with r = b - A x.
 constexpr int kTrueResKZ = 16;
 // With gram given it also accumulates the pressure history's new Gram row for x:
 // partial sums of x.Ax, p1.Ax and p2.Ax (p1 / p2 may be null) at gram[q * nblocks + block].
@@ -1216,8 +1205,9 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
                                                             double* __restrict__ partial,
                                                             double* __restrict__ gram = nullptr,
                                                             const double* __restrict__ p1 = nullptr,
-                                                            const double* __restrict__ p2 = nullptr) {
-    double g0 = 0, g1 = 0, g2 = 0;
+                                                            const double* __restrict__ p2 = nullptr,
+                                                            double* __restrict__ epart = nullptr) {
+    double g0 = 0, g1 = 0, g2 = 0, e = 0;
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     const int kb = blockIdx.z * kTrueResKZ;
     const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
@@ -1237,8 +1227,11 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
             if (i + 1 < ni) y += (double)op.AX(c) * x[c + 1];
             if (j + 1 < nj) y += (double)op.AY(c) * x[c + ni];
             if (k + 1 < nk) y += (double)az0 * xp;
-            const double rv = (b64 ? b64[c] : (double)b32[c]) - y;
+            const double bv = b64 ? b64[c] : (double)b32[c];
+            const double rv = bv - y;
             if (r) r[c] = (float)rv;
+            // energy of x: x (b + r) with r as stored in FP32
+            if (epart) e += x0 * (bv + (double)(float)rv);
             if (gram) {
                 g0 += x0 * y;
                 if (p1) g1 += p1[c] * y;
@@ -1253,6 +1246,10 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
     double t = block_max(m);
     const unsigned blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0 && threadIdx.y == 0) partial[blk] = t;
+    if (epart) {
+        const double se = block_sum(e);
+        if (threadIdx.x == 0 && threadIdx.y == 0) epart[blk] = se;
+    }
     if (gram) {
         const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
         const double s0 = block_sum(g0), s1 = block_sum(g1), s2 = block_sum(g2);
@@ -1337,7 +1334,7 @@ static __global__ __launch_bounds__(256) void k_residual_restrict4(Op<float> op,
 
 // Energy of an FP32 initial guess before its FP64 residual: partial sums of
 // x.(2b - A x), A x from the FP32 level-0 operator accumulated in FP64, so that
-// E = -sum / 2 as from k_energy_dots (x.(b + r) with r = b - A x). Launched as
+// E = -sum / 2 as from k_true_residual64's energy sums (x.(b + r) with r = b - A x). Launched as
 // k_true_residual64.
 static __global__ __launch_bounds__(kBlock) void k_energy_f32(Op<float> op, const double* __restrict__ b64,
                                                        const float* __restrict__ b32, const float* __restrict__ x,

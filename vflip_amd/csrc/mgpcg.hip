@@ -614,6 +614,8 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
     if (!rc) {
         const LevelBuf& l0 = S->L[0];
         int np = std::max(kVecBlocks, nblocks(grid3z(l0.ni, l0.nj, l0.nk)));
+        // the true residual's max|r| and energy partials side by side
+        np = std::max(np, 2 * nblocks(dim3((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ)));
         np = std::max(np, zgeom(l0, false, S->cus).blocks);
         np = std::max(np, zgeom(l0, true, S->cus).blocks);
         S->np_partial = np;
@@ -855,13 +857,16 @@ int ensure_f64(vflip_solver* S) {
 }
 
 // gram: also accumulate the history Gram row of x (vflip_project_history solves)
+// energy: also the energy partials of x, in S->partial after the max|r| partials
+// (finalize_energy reads them).
 void launch_true_residual64(vflip_solver* S, const double* b64, const float* b32, const double* x, float* r,
-                            bool gram = false) {
+                            bool gram = false, bool energy = false) {
     const LevelBuf& l0 = S->L[0];
     const dim3 g((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
     k_true_residual64<<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr, b64, b32,
                                                          x, r, S->partial, gram ? S->hist_part : nullptr,
-                                                         gram ? S->gram_p[0] : nullptr, gram ? S->gram_p[1] : nullptr);
+                                                         gram ? S->gram_p[0] : nullptr, gram ? S->gram_p[1] : nullptr,
+                                                         energy ? S->partial + nblocks(g) : nullptr);
     if (gram) S->gram_nb = nblocks(g);
     S->np_last = nblocks(g);
 }
@@ -934,16 +939,17 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
             k_cast_f32_f64<<<gn, kBlock, 0, S->stream>>>(n, x32, S->x64);
         }
         if (!mx) k_cast_f64_f32<<<gn, kBlock, 0, S->stream>>>(n, S->x64, S->xw);
-        launch_true_residual64(S, mx ? rhs64 : nullptr, rhs32, S->x64, S->r);
-        VF_CHECK_LAUNCH();
-        if ((rc = finalize(S, 2))) return rc;
         // The PCG vets the guess: it minimises E(x) = x'Ax/2 - b'x, and
         // E(x0) < E(0) = 0 exactly when x0 is closer to the solution than zero
-        // in the A-norm. A guess that is not is dropped (restart from zero).
-        k_energy_dots<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->x64, mx ? rhs64 : nullptr, rhs32, S->r, S->partial);
-        S->np_last = S->vblocks;
+        // in the A-norm. A guess that is not is dropped (restart from zero). The
+        // energy's sums come from the same pass as the residual.
+        launch_true_residual64(S, mx ? rhs64 : nullptr, rhs32, S->x64, S->r, false, true);
         VF_CHECK_LAUNCH();
-        if ((rc = finalize(S, 6))) return rc;
+        const int ne = S->np_last;
+        if ((rc = finalize(S, 2))) return rc;
+        if (mx) k_finalize<double><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial + ne, ne, 6, 0.0);
+        else k_finalize<float><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial + ne, ne, 6, 0.0);
+        VF_CHECK_LAUNCH();
     } else {
         if (mx) k_fill64<<<gn, kBlock, 0, S->stream>>>(n, S->x64, 0.0);
         else k_fill<<<gn, kBlock, 0, S->stream>>>(n, S->xw, 0.f);

@@ -207,39 +207,54 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int ni, int nj, int nk, flo
     rhs[c] = (float)r;
 }
 
-// u -= grad p, S/Smoke_solver3D.cpp:1126-1200 and S/sub_domain.cpp:745-819.
-// P = double reproduces the reference exactly: (float)(p[c] - p[c-1]) with an
-// FP64 pressure. P = float is the plain FP32 path.
+// u -= grad p on the three MAC components in one pass (S/Smoke_solver3D.cpp:1126-1200;
+// SUB: S/sub_domain.cpp:745-819, dt/h scaling and faces 1..n-1 along the axis). A thread
+// takes lattice point (i, j, k) of the (ni+1) x (nj+1) x (nk+1) box and the u, v and w
+// faces of that index that exist, so each pressure value is read once for the three
+// faces on its low side. Per face: along the component's axis the faces lo..hi
+// (COARSE 2..n-2, SUB 1..n-1), across it 1..n-2; where the face weight is > 0 the
+// velocity drops (float)(p[c] - p[c - e]) / h (times dt for SUB), else it takes the
+// solid velocity. P = double reproduces the reference exactly (its pressure vector is
+// double); P = float is the plain FP32 path.
 template <typename P>
-__global__ __launch_bounds__(kBlock) void k_grad(int axis, int ni, int nj, int nk, int sub, float h, float dt,
-                                                 const P* __restrict__ p, const float* __restrict__ wgt,
-                                                 const float* __restrict__ sol, float* __restrict__ vel) {
-    const int fx = ni + (axis == 0), fy = nj + (axis == 1), fz = nk + (axis == 2);
-    const size_t nf = (size_t)fx * fy * fz;
-    size_t f = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    if (f >= nf) return;
-    int i = (int)(f % fx);
-    size_t t = f / fx;
-    int j = (int)(t % fy), k = (int)(t / fy);
+__device__ inline void grad_face(int axis, int i, int j, int k, int ni, int nj, int nk, int sub, float h, float dt,
+                                 const P* __restrict__ p, const float* __restrict__ wgt,
+                                 const float* __restrict__ sol, float* __restrict__ vel) {
+    const int fx = ni + (axis == 0), fy = nj + (axis == 1);
+    const int a[3] = {i, j, k}, nn[3] = {ni, nj, nk};
     const int lo = sub ? 1 : 2;
-    int a[3] = {i, j, k};
-    int nn[3] = {ni, nj, nk};
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
         if (q == axis) {
-            int hi = sub ? nn[q] - 1 : nn[q] - 2;
+            const int hi = sub ? nn[q] - 1 : nn[q] - 2;
             if (a[q] < lo || a[q] > hi) return;
         } else if (a[q] < 1 || a[q] > nn[q] - 2) {
             return;
         }
     }
+    const size_t f = ((size_t)k * fy + j) * fx + i;
     if (wgt[f] > 0) {
-        size_t c = lin3(i, j, k, ni, nj);
-        size_t cm = (axis == 0) ? c - 1 : (axis == 1) ? c - ni : c - (size_t)ni * nj;
+        const size_t c = lin3(i, j, k, ni, nj);
+        const size_t cm = (axis == 0) ? c - 1 : (axis == 1) ? c - ni : c - (size_t)ni * nj;
         const float g = (float)(p[c] - p[cm]);
         vel[f] -= sub ? dt * g / h : g / h;
     } else {
         vel[f] = sol[f];
     }
+}
+
+template <typename P>
+__global__ __launch_bounds__(256) void k_grad3(int ni, int nj, int nk, int sub, float h, float dt,
+                                               const P* __restrict__ p, const float* __restrict__ uw,
+                                               const float* __restrict__ vw, const float* __restrict__ ww,
+                                               const float* __restrict__ us, const float* __restrict__ vs,
+                                               const float* __restrict__ ws, float* __restrict__ u,
+                                               float* __restrict__ v, float* __restrict__ w) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i > ni || j > nj || k > nk) return;
+    grad_face<P>(0, i, j, k, ni, nj, nk, sub, h, dt, p, uw, us, u);
+    grad_face<P>(1, i, j, k, ni, nj, nk, sub, h, dt, p, vw, vs, v);
+    grad_face<P>(2, i, j, k, ni, nj, nk, sub, h, dt, p, ww, ws, w);
 }
 
 }  // namespace vflip
@@ -338,12 +353,9 @@ int vflip_grad_apply(int ni, int nj, int nk, int variant, float h, float dt, con
     VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
     hipStream_t s = (hipStream_t)stream;
     int sub = variant == VFLIP_VARIANT_SUB;
-    size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
-    k_grad<float><<<grid_for(nu), kBlock, 0, s>>>(0, ni, nj, nk, sub, h, dt, p, uw, us, u);
-    k_grad<float><<<grid_for(nv), kBlock, 0, s>>>(1, ni, nj, nk, sub, h, dt, p, vw, vs, v);
-    k_grad<float><<<grid_for(nw), kBlock, 0, s>>>(2, ni, nj, nk, sub, h, dt, p, ww, ws, w);
+    k_grad3<float><<<dim3(ni / 64 + 1, nj / 4 + 1, nk + 1), dim3(64, 4), 0, s>>>(ni, nj, nk, sub, h, dt, p, uw, vw, ww,
+                                                                              us, vs, ws, u, v, w);
     VF_CHECK_LAUNCH();
-    VF_HIP(hipStreamSynchronize(s));
     return 0;
 }
 
@@ -355,12 +367,9 @@ int vflip_grad_apply_f64(int ni, int nj, int nk, int variant, float h, float dt,
     VF_ARG(variant == VFLIP_VARIANT_COARSE || variant == VFLIP_VARIANT_SUB, "variant");
     hipStream_t s = (hipStream_t)stream;
     int sub = variant == VFLIP_VARIANT_SUB;
-    size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
-    k_grad<double><<<grid_for(nu), kBlock, 0, s>>>(0, ni, nj, nk, sub, h, dt, p, uw, us, u);
-    k_grad<double><<<grid_for(nv), kBlock, 0, s>>>(1, ni, nj, nk, sub, h, dt, p, vw, vs, v);
-    k_grad<double><<<grid_for(nw), kBlock, 0, s>>>(2, ni, nj, nk, sub, h, dt, p, ww, ws, w);
+    k_grad3<double><<<dim3(ni / 64 + 1, nj / 4 + 1, nk + 1), dim3(64, 4), 0, s>>>(ni, nj, nk, sub, h, dt, p, uw, vw,
+                                                                               ww, us, vs, ws, u, v, w);
     VF_CHECK_LAUNCH();
-    VF_HIP(hipStreamSynchronize(s));
     return 0;
 }
 
