@@ -44,16 +44,50 @@ __device__ inline float at(const DeltaGrid& g, int i, int j, int k) {
     return g.fnew[c] - g.fold[c];
 }
 
-// Buffer3D::sample_linear
+// Value at element offset c (in range).
+__device__ inline float val(const Grid& g, size_t c) { return g.f[c]; }
+__device__ inline float val(const DeltaGrid& g, size_t c) { return g.fnew[c] - g.fold[c]; }
+
+// Grid coordinates w / h + o of a sample. For a power-of-two h (the usual grid
+// spacings) w / h is an exact scaling, the same value as w times the exact
+// reciprocal, which takes one multiply instead of a division.
+__device__ inline void grid_coords(double h, double wx, double wy, double wz, double ox, double oy, double oz,
+                                   double& gx, double& gy, double& gz) {
+    const unsigned long long hb = (unsigned long long)__double_as_longlong(h);
+    const unsigned e = (unsigned)(hb >> 52) & 0x7FFu;
+    if ((hb & 0x000FFFFFFFFFFFFFull) == 0 && e >= 1 && e <= 2045) {
+        const double ih = __longlong_as_double((long long)(((unsigned long long)(2046 - e) << 52) |
+                                                           (hb & 0x8000000000000000ull)));
+        gx = wx * ih + ox;
+        gy = wy * ih + oy;
+        gz = wz * ih + oz;
+    } else {
+        gx = wx / h + ox;
+        gy = wy / h + oy;
+        gz = wz / h + oz;
+    }
+}
+
+// Buffer3D::sample_linear: floor of the grid coordinate, corners clamped to the
+// grid (Buffer3D::at), lerps in double in the order x, y, z. Each axis is clamped
+// once for its two lattice lines and the eight corners are offsets from the four
+// (y, z) rows (same loads and values as at()).
 template <typename G>
 __device__ inline float sample_linear(const G& g, double h, double wx, double wy, double wz) {
-    const double gx = wx / h + g.ox, gy = wy / h + g.oy, gz = wz / h + g.oz;
+    double gx, gy, gz;
+    grid_coords(h, wx, wy, wz, g.ox, g.oy, g.oz, gx, gy, gz);
     const int i = (int)floor(gx), j = (int)floor(gy), k = (int)floor(gz);
     const double cx = gx - (double)i, cy = gy - (double)j, cz = gz - (double)k;
-    const double v1 = lerpd(lerpd(at(g, i, j, k), at(g, i + 1, j, k), cx),
-                            lerpd(at(g, i, j + 1, k), at(g, i + 1, j + 1, k), cx), cy);
-    const double v2 = lerpd(lerpd(at(g, i, j, k + 1), at(g, i + 1, j, k + 1), cx),
-                            lerpd(at(g, i, j + 1, k + 1), at(g, i + 1, j + 1, k + 1), cx), cy);
+    const int i0 = min(max(i, 0), g.nx - 1), i1 = min(max(i + 1, 0), g.nx - 1);
+    const int j0 = min(max(j, 0), g.ny - 1), j1 = min(max(j + 1, 0), g.ny - 1);
+    const int k0 = min(max(k, 0), g.nz - 1), k1 = min(max(k + 1, 0), g.nz - 1);
+    const size_t nx = (size_t)g.nx, ny = (size_t)g.ny;
+    const size_t r00 = ((size_t)k0 * ny + j0) * nx, r10 = ((size_t)k0 * ny + j1) * nx;
+    const size_t r01 = ((size_t)k1 * ny + j0) * nx, r11 = ((size_t)k1 * ny + j1) * nx;
+    const float c000 = val(g, r00 + i0), c100 = val(g, r00 + i1), c010 = val(g, r10 + i0), c110 = val(g, r10 + i1);
+    const float c001 = val(g, r01 + i0), c101 = val(g, r01 + i1), c011 = val(g, r11 + i0), c111 = val(g, r11 + i1);
+    const double v1 = lerpd(lerpd(c000, c100, cx), lerpd(c010, c110, cx), cy);
+    const double v2 = lerpd(lerpd(c001, c101, cx), lerpd(c011, c111, cx), cy);
     return (float)lerpd(v1, v2, cz);
 }
 
