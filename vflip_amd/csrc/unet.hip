@@ -159,13 +159,14 @@ int galloc(vflip_unet* N, T** p, size_t n) {
 }
 
 // Launch plan of a convolution: output channels split into g groups until the
-// layer has 128 workgroups (or one 16-channel block per workgroup); then, for
-// the small grids, the K loop into ks slices (powers of two) until about 256
-// workgroups, each slice at least 9 of the K steps.
+// layer has 128 workgroups and each workgroup at most 6 16-channel blocks (or
+// one block per workgroup); then, for the small grids, the K loop into ks
+// slices (powers of two) until about 256 workgroups, each slice at least 9 of
+// the K steps.
 int conv_plan(int ntiles, int NBall, int cin, int ck, int& g) {
     g = 1;
-    while (ntiles * g < 128 && NBall % (2 * g) == 0) g *= 2;
-    const int total = ((cin + ck - 1) / ck) * (ck == 32 ? 27 : 14);
+    while ((ntiles * g < 128 || NBall / g > 6) && NBall % (2 * g) == 0) g *= 2;
+    const int total = ((cin + ck - 1) / ck) * (ck == 32 ? 27 : 15);
     int ks = 1;
     while (ntiles * g * ks * 2 <= 256 && total / (ks * 2) >= 9) ks *= 2;
     return ks;
@@ -174,7 +175,7 @@ int conv_plan(int ntiles, int NBall, int cin, int ck, int& g) {
 int level_tiles(const vflip_unet* N, int level) {
     int x, y, z;
     level_dims(N, level, x, y, z);
-    return ((x + 7) / 8) * ((y + 7) / 8) * ((z + 7) / 8);
+    return ((x + kTX - 1) / kTX) * ((y + kTY - 1) / kTY) * ((z + kTZ - 1) / kTZ);
 }
 
 // Largest partial buffer of the forward and backward-data convolutions.
@@ -236,15 +237,28 @@ void launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
     a.part = ks > 1 ? ws : nullptr;
     const int NB = NBall / g;
     const dim3 grid(a.ntiles * g * ks), block(kConvThreads);
-    switch (NB) {
-        case 1: k_conv3<CK, 1><<<grid, block, 0, s>>>(a); break;
-        case 2: k_conv3<CK, 2><<<grid, block, 0, s>>>(a); break;
-        case 3: k_conv3<CK, 3><<<grid, block, 0, s>>>(a); break;
-        case 4: k_conv3<CK, 4><<<grid, block, 0, s>>>(a); break;
-        case 6: k_conv3<CK, 6><<<grid, block, 0, s>>>(a); break;
-        case 8: k_conv3<CK, 8><<<grid, block, 0, s>>>(a); break;
-        case 12: k_conv3<CK, 12><<<grid, block, 0, s>>>(a); break;
-        default: break;
+    if (ks > 1) {
+        switch (NB) {
+            case 1: k_conv3<CK, 1, true><<<grid, block, 0, s>>>(a); break;
+            case 2: k_conv3<CK, 2, true><<<grid, block, 0, s>>>(a); break;
+            case 3: k_conv3<CK, 3, true><<<grid, block, 0, s>>>(a); break;
+            case 4: k_conv3<CK, 4, true><<<grid, block, 0, s>>>(a); break;
+            case 6: k_conv3<CK, 6, true><<<grid, block, 0, s>>>(a); break;
+            case 8: k_conv3<CK, 8, true><<<grid, block, 0, s>>>(a); break;
+            case 12: k_conv3<CK, 12, true><<<grid, block, 0, s>>>(a); break;
+            default: break;
+        }
+    } else {
+        switch (NB) {
+            case 1: k_conv3<CK, 1, false><<<grid, block, 0, s>>>(a); break;
+            case 2: k_conv3<CK, 2, false><<<grid, block, 0, s>>>(a); break;
+            case 3: k_conv3<CK, 3, false><<<grid, block, 0, s>>>(a); break;
+            case 4: k_conv3<CK, 4, false><<<grid, block, 0, s>>>(a); break;
+            case 6: k_conv3<CK, 6, false><<<grid, block, 0, s>>>(a); break;
+            case 8: k_conv3<CK, 8, false><<<grid, block, 0, s>>>(a); break;
+            case 12: k_conv3<CK, 12, false><<<grid, block, 0, s>>>(a); break;
+            default: break;
+        }
     }
     if (ks > 1) {
         const size_t n = (size_t)a.nx * a.ny * a.nz * a.cout;
@@ -260,9 +274,9 @@ void conv(const vflip_unet* N, const ConvArgs& a, int ck, hipStream_t s) {
 ConvArgs base_args(const vflip_unet* N, int level) {
     ConvArgs a{};
     level_dims(N, level, a.nx, a.ny, a.nz);
-    a.tiles_x = (a.nx + kT - 1) / kT;
-    a.tiles_y = (a.ny + kT - 1) / kT;
-    a.ntiles = a.tiles_x * a.tiles_y * ((a.nz + kT - 1) / kT);
+    a.tiles_x = (a.nx + kTX - 1) / kTX;
+    a.tiles_y = (a.ny + kTY - 1) / kTY;
+    a.ntiles = a.tiles_x * a.tiles_y * ((a.nz + kTZ - 1) / kTZ);
     return a;
 }
 

@@ -18,10 +18,13 @@ typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kT = 8;            // output tile edge (voxels)
-constexpr int kH = kT + 2;       // halo tile edge
-constexpr int kHV = kH * kH * kH;
-constexpr int kConvThreads = 512;  // 8 waves, wave w owns z-slice w of the tile
+// Output tile of a convolution workgroup: 16 x 8 x 4 voxels, 32 rows of 16 x
+// voxels. Wave w owns rows y = 4 (w & 1) + mb (mb = 0..3) of z-slice w >> 1, one
+// row per 16-row M block of the matrix cores.
+constexpr int kTX = 16, kTY = 8, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;  // halo tile 18 x 10 x 6
+constexpr int kHVox = kHX * kHY * kHZ;
+constexpr int kConvThreads = 512;  // 8 waves
 
 // One input of a convolution: channels [c_lo, c_lo + C) of the (virtually
 // concatenated) input come from x, read at (v >> up) on a grid of dims >> up.
@@ -50,16 +53,54 @@ struct ConvArgs {
     float* part; // [ksplit][V][cout] partials, summed by k_conv_splitk_reduce
 };
 
-__device__ inline int tap_offset_hv(int tap) {
+// LDS layout of the halo tile of CK input channels: P = CK / 8 channel planes of
+// 16 bytes per voxel, voxel (hx, hy, hz) at plane * PS + hz * Z + hy * R + hx.
+// gfx950 serves a 16-byte LDS read 16 lanes at a time from 64 four-byte banks;
+// it is free of bank conflicts when each 16-lane group reads 256 contiguous
+// bytes and the four groups of the wave start at the same offset modulo 256
+// bytes. A matrix-core A fragment is 16 consecutive x voxels (one group per
+// channel plane, or per tap of a pair), so: PS is a multiple of 16 voxels, and
+// for CK = 16, whose K steps pair two taps, Z = 191 makes the offsets of the
+// paired taps (kTapPairs16) equal modulo 16 voxels.
+template <int CK>
+struct Halo {
+    static constexpr int P = CK / 8;
+    static constexpr int R = kHX;
+    static constexpr int Z = CK == 16 ? 191 : kHX * kHY;
+    static constexpr int PS = (kHZ * Z + 15) / 16 * 16;
+    static constexpr int LDS = P * PS * 8;  // bf16 elements
+    // 16-byte elements, in blocks of 64 (one wave's load): lane l of block b
+    // takes part (l >> 4) % P of voxel b * 64 / P + (l & 15) + 16 ((l >> 4) / P)
+    static constexpr int BLOCKS = (kHVox * P + 63) / 64;
+    static constexpr int N = (BLOCKS * 64 + kConvThreads - 1) / kConvThreads;  // elements per thread
+};
+
+__device__ inline int tap_offset(int tap, int R, int Z) {
     const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    return (dz * kH + dy) * kH + dx;
+    return dz * Z + dy * R + dx;
 }
 
-// Stages the halo tile of input channels [c0, c0 + CK) into LDS as
-// [halo voxel][CK + 8] bf16 (the 8-element pad keeps 16-lane row reads free of
-// bank conflicts). Voxels outside the grid read as zero (zero padding).
-template <int CK>
-__device__ inline void stage_halo(const ConvArgs& a, int c0, int x0, int y0, int z0, bf16* lds) {
+// K step st of a 16-channel chunk: taps (kTapPairs16 >> 10 st) & 31 and
+// (>> 10 st + 5) & 31; tap 27 is the zero pad of the weights. Pairs:
+// (0,10) (1,11) (2,3) (4,14) (5,6) (7,17) (8,27) (9,19) (12,22) (13,23)
+// (15,25) (16,26) (18,27) (20,21) (24,27), four steps per 64-bit word.
+__device__ inline void tap_pair16(int st, int& ta, int& tb) {
+    constexpr unsigned long long W[4] = {
+        (0ull | 10ull << 5) | (1ull | 11ull << 5) << 16 | (2ull | 3ull << 5) << 32 | (4ull | 14ull << 5) << 48,
+        (5ull | 6ull << 5) | (7ull | 17ull << 5) << 16 | (8ull | 27ull << 5) << 32 | (9ull | 19ull << 5) << 48,
+        (12ull | 22ull << 5) | (13ull | 23ull << 5) << 16 | (15ull | 25ull << 5) << 32 | (16ull | 26ull << 5) << 48,
+        (18ull | 27ull << 5) | (20ull | 21ull << 5) << 16 | (24ull | 27ull << 5) << 32};
+    const int q = st >> 2;
+    const unsigned long long w = q == 0 ? W[0] : q == 1 ? W[1] : q == 2 ? W[2] : W[3];
+    const unsigned v = (unsigned)(w >> ((st & 3) * 16));
+    ta = v & 31;
+    tb = (v >> 5) & 31;
+}
+
+// Global loads of G of this thread's halo elements (element threadIdx + (i0 + i) *
+// 512), all issued before any is used; voxels outside the grid read as zero.
+template <int CK, int G>
+__device__ inline void load_halo(const ConvArgs& a, int c0, int x0, int y0, int z0, int i0, bf16x8 (&hr)[G]) {
     int s = 0, cl = c0;
     if (a.nsrc > 1 && c0 >= a.src[0].C) {
         s = 1;
@@ -67,41 +108,140 @@ __device__ inline void stage_halo(const ConvArgs& a, int c0, int x0, int y0, int
     }
     const ConvSrc src = a.src[s];
     const int sx = a.nx >> src.up, sy = a.ny >> src.up;
-    constexpr int P = CK / 8;  // 16-byte parts per voxel
-    for (int e = threadIdx.x; e < kHV * P; e += kConvThreads) {
-        const int hv = e / P, part = e % P;
-        const int hx = hv % kH, hy = (hv / kH) % kH, hz = hv / (kH * kH);
+    constexpr int P = Halo<CK>::P;
+    const int lane = threadIdx.x & 63;
+    const int part = (lane >> 4) % P, vl = (lane & 15) + 16 * ((lane >> 4) / P);
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const int blk = (threadIdx.x + (i0 + i) * kConvThreads) >> 6;
+        const int hv = blk * (64 / P) + vl;
+        const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
         const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
         bf16x8 val;
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < a.nx && gy < a.ny && gz < a.nz) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) val[q] = (bf16)0.f;
+        if (hv < kHVox && gx >= 0 && gy >= 0 && gz >= 0 && gx < a.nx && gy < a.ny && gz < a.nz) {
             const size_t v = ((size_t)(gz >> src.up) * sy + (gy >> src.up)) * sx + (gx >> src.up);
             val = *reinterpret_cast<const bf16x8*>(src.x + v * src.C + cl + part * 8);
-        } else {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) val[q] = (bf16)0.f;
         }
-        *reinterpret_cast<bf16x8*>(lds + hv * (CK + 8) + part * 8) = val;
+        hr[i] = val;
     }
 }
 
-// 3x3x3 convolution, implicit GEMM: M = 512 voxels of an 8^3 tile, N = cout,
-// K = 27 * cin. Each K step of 32 is one tap x 32 channels (CK = 32) or two
-// taps x 16 channels (CK = 16). A workgroup computes NB blocks of 16 output
+template <int CK, int G>
+__device__ inline void store_halo(const bf16x8 (&hr)[G], int i0, bf16* lds) {
+    using H = Halo<CK>;
+    const int lane = threadIdx.x & 63;
+    const int part = (lane >> 4) % H::P, vl = (lane & 15) + 16 * ((lane >> 4) / H::P);
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+        const int blk = (threadIdx.x + (i0 + i) * kConvThreads) >> 6;
+        const int hv = blk * (64 / H::P) + vl;
+        if (hv < kHVox) {
+            const int hx = hv % kHX, hy = (hv / kHX) % kHY, hz = hv / (kHX * kHY);
+            *reinterpret_cast<bf16x8*>(lds + (part * H::PS + hz * H::Z + hy * H::R + hx) * 8) = hr[i];
+        }
+    }
+}
+
+// Stages the whole halo tile into LDS, at most four loads per thread in flight.
+template <int CK>
+__device__ inline void stage_halo(const ConvArgs& a, int c0, int x0, int y0, int z0, bf16* lds) {
+    constexpr int N = Halo<CK>::N, G = CK == 16 ? 1 : 3;
+#pragma unroll 1
+    for (int i0 = 0; i0 < N; i0 += G) {  // elements past the tile are skipped by load and store
+        bf16x8 hr[G];
+        load_halo<CK, G>(a, c0, x0, y0, z0, i0, hr);
+        store_halo<CK, G>(hr, i0, lds);
+    }
+}
+
+// The weights of K steps [lo, lo + n) of one channel chunk, staged into LDS
+// beside the halo: 16-byte element e = ((s * NB + nb) * 4 + kg) * 16 + r16 (s =
+// step - lo) holds the 8 k values of lane (r16, kg) of output block nb, so each
+// 16-lane group of a B fragment reads 256 contiguous bytes.
+template <int CK, int NB>
+struct WChunk {
+    static constexpr int STEPS = CK == 32 ? 27 : 15;
+    static constexpr int E = STEPS * NB * 64;  // a whole chunk
+    static constexpr int N = (E + kConvThreads - 1) / kConvThreads;  // elements per thread, at most
+};
+
+template <int CK, int NB>
+__device__ inline void load_wchunk(const ConvArgs& a, int chunk, int lo, int n, int cbase,
+                                   bf16x8 (&wr)[WChunk<CK, NB>::N]) {
+#pragma unroll
+    for (int i = 0; i < WChunk<CK, NB>::N; ++i) {
+        const int e = threadIdx.x + i * kConvThreads;
+        if (e < n * NB * 64) {
+            const int st = lo + e / (NB * 64), rem = e % (NB * 64);
+            const int nb = rem >> 6, kg = (rem >> 4) & 3, r16 = rem & 15;
+            int tap = st, coff = kg * 8;
+            if (CK == 16) {
+                int ta, tb;
+                tap_pair16(st, ta, tb);
+                tap = (kg >> 1) ? tb : ta;
+                coff = (kg & 1) * 8;
+            }
+            wr[i] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(cbase + nb * 16 + r16) * a.kw +
+                                                     (size_t)tap * a.cin + chunk * CK + coff);
+        }
+    }
+}
+
+template <int CK, int NB>
+__device__ inline void store_wchunk(const bf16x8 (&wr)[WChunk<CK, NB>::N], int n, bf16* ldw) {
+#pragma unroll
+    for (int i = 0; i < WChunk<CK, NB>::N; ++i) {
+        const int e = threadIdx.x + i * kConvThreads;
+        if (e < n * NB * 64) *reinterpret_cast<bf16x8*>(ldw + e * 8) = wr[i];
+    }
+}
+
+// LDS of a convolution workgroup (bf16 elements): the halo, plus the weights of
+// the chunk's K steps where they fit (up to 3 output blocks: 16-channel chunks,
+// and 32-channel chunks of split K loops, which run on small grids at one
+// workgroup per CU). The compiler is held
+// to the register budget of the waves per SIMD that this LDS allows (2 per
+// workgroup, 160 KB per CU), unless the workgroup needs more registers.
+template <int CK, int NB, bool SPLIT>
+constexpr bool conv_wlds() { return CK == 16 ? NB <= 3 : SPLIT && NB <= 3; }
+template <int CK, int NB, bool SPLIT>
+constexpr int conv_lds() { return Halo<CK>::LDS + (conv_wlds<CK, NB, SPLIT>() ? WChunk<CK, NB>::E * 8 : 0); }
+template <int CK, int NB, bool SPLIT>
+constexpr int conv_waves_per_simd() {
+    const int b = (160 * 1024) / (conv_lds<CK, NB, SPLIT>() * 2);
+    const int w = 2 * (b < 1 ? 1 : (b > 4 ? 4 : b));
+    // registers: accumulators, loads in flight, A fragments, addressing
+    const int ring = conv_wlds<CK, NB, SPLIT>() ? 4 * (Halo<CK>::N + WChunk<CK, NB>::N) + 16
+                                                : 4 * NB * (SPLIT ? (NB == 1 ? 2 : NB == 2 ? 4 : 1) : 3);
+    const int cap = 512 / (16 * NB + ring + 56);
+    return w < cap ? w : (cap < 2 ? 2 : cap);
+}
+
+// 3x3x3 convolution, implicit GEMM: M = 512 voxels of a 16 x 8 x 4 tile, N =
+// cout, K = 27 * cin. Each K step of 32 is one tap x 32 channels (CK = 32, 27
+// steps per chunk) or a pair of taps x 16 channels (CK = 16, 15 steps by
+// tap_pair16, three of them with the zero pad tap). A workgroup computes NB blocks of 16 output
 // channels, group (blockIdx / ntiles) % groups of the a.groups groups (cout =
 // 16 NB groups); small grids split their channels over groups so that more
 // workgroups share a layer. With ksplit > 1 workgroup slice blockIdx /
 // (ntiles groups) takes a contiguous range of the K steps (channel chunk x
 // step, the unsplit order) and writes its FP32 partial sums instead of the
 // outputs; with ksplit 1 the loop is the unsplit one.
-template <int CK, int NB>
-__global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
-    __shared__ __attribute__((aligned(16))) bf16 lds[kHV * (CK + 8)];
+template <int CK, int NB, bool SPLIT>
+__global__ __launch_bounds__(kConvThreads, (conv_waves_per_simd<CK, NB, SPLIT>())) void k_conv3(ConvArgs a) {
+    using H = Halo<CK>;
+    constexpr bool WLDS = conv_wlds<CK, NB, SPLIT>();  // the chunk's weights in LDS
+    constexpr int LDS_ALL = conv_lds<CK, NB, SPLIT>();
+    __shared__ __attribute__((aligned(16))) bf16 lds[LDS_ALL];
     const int L = blockIdx.x % a.ntiles, grp = (blockIdx.x / a.ntiles) % a.groups;
     const int ks = blockIdx.x / (a.ntiles * a.groups), nks = a.ksplit > 1 ? a.ksplit : 1;
     const int cbase = grp * NB * 16;
     int t = L;
     if (a.ntiles % 8 == 0) t = (L % 8) * (a.ntiles / 8) + L / 8;
-    const int x0 = (t % a.tiles_x) * kT, y0 = ((t / a.tiles_x) % a.tiles_y) * kT, z0 = (t / (a.tiles_x * a.tiles_y)) * kT;
+    const int x0 = (t % a.tiles_x) * kTX, y0 = ((t / a.tiles_x) % a.tiles_y) * kTY;
+    const int z0 = (t / (a.tiles_x * a.tiles_y)) * kTZ;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r16 = lane & 15, kg = lane >> 4;  // MFMA row/col within a block, k group of 8
     f32x4 acc[4][NB];
@@ -109,70 +249,208 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // halo voxel of this lane's output row in each M block (tap 0 offset added per tap)
-    int hv0[4];
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) {
-        const int m = mb * 16 + r16;
-        hv0[mb] = ((w + 1) * kH + (m / 8 + 1)) * kH + (m % 8 + 1);
-    }
-    constexpr int STEPS = CK == 32 ? 27 : 14;
+    // LDS slot of tap (-1,-1,-1) of this lane's output voxel in M block 0, in its
+    // channel plane; M block mb and tap t add mb * R + tap_offset(t) + Z + R + 1 >= 0,
+    // compile-time offsets of the unrolled steps
+    const int plane = CK == 32 ? kg : (kg & 1);
+    const int hbase = plane * H::PS + (w >> 1) * H::Z + (w & 1) * 4 * H::R + r16;
+    const int lane_hi = CK == 16 ? (kg >> 1) : 0;  // CK = 16: lanes of the second tap of the pair
+    constexpr int STEPS = CK == 32 ? 27 : 15;
     const int total = ((a.cin + CK - 1) / CK) * STEPS;
     const int q0 = (int)((long long)ks * total / nks), q1 = (int)((long long)(ks + 1) * total / nks);
-    // K step q: tap and channel offset of this lane's 8 k values
-    auto step_k = [&](int q, int& tap, int& coff) {
+    // K step q: the taps of its two halves of k (the same tap for CK = 32; tap 27
+    // is the zero pad of the weights, read at its partner's voxels)
+    auto step_taps = [&](int q, int& ta, int& tb) {
         const int st = q % STEPS;
-        if (CK == 32) {
-            tap = st;
-            coff = kg * 8;
-        } else {
-            tap = 2 * st + (kg >> 1);  // tap 27 is the zero pad of the weights
-            coff = (kg & 1) * 8;
-        }
+        ta = tb = st;
+        if (CK == 16) tap_pair16(st, ta, tb);
     };
-    // the weight fragments of step q (global loads; issued one step ahead so that
-    // their latency overlaps the previous step's matrix-core work)
+    // weights: a uniform base per tap plus one per-lane 32-bit byte offset (row
+    // cbase + r16, k group, chunk; the second tap of a pair on lanes lane_hi)
+    const unsigned wlane = ((unsigned)(cbase + r16) * (unsigned)a.kw + (CK == 32 ? kg * 8 : (kg & 1) * 8)) * 2u;
     auto load_b = [&](int q, bf16x8 (&bf)[NB]) {
-        int tap, coff;
-        step_k(q, tap, coff);
-        const size_t kidx = (size_t)tap * a.cin + (q / STEPS) * CK + coff;
+        int ta, tb;
+        step_taps(q, ta, tb);
+        const char* base = reinterpret_cast<const char*>(a.w) + (size_t)ta * a.cin * 2;
+        const unsigned off = wlane + (unsigned)((q / STEPS) * CK * 2) + (unsigned)(lane_hi * (tb - ta) * a.cin * 2);
 #pragma unroll
         for (int nb = 0; nb < NB; ++nb)
-            bf[nb] = *reinterpret_cast<const bf16x8*>(a.w + (size_t)(cbase + nb * 16 + r16) * a.kw + kidx);
+            bf[nb] = *reinterpret_cast<const bf16x8*>(base + (size_t)(off + (unsigned)(nb * 16 * a.kw * 2)));
     };
-    bf16x8 bcur[NB], bnext[NB];
-    if (q0 < q1) load_b(q0, bcur);
-    int staged = -1;
-    for (int q = q0; q < q1; ++q) {
-        const int c0 = (q / STEPS) * CK;
-        if (q + 1 < q1) load_b(q + 1, bnext);
-        if (c0 != staged) {  // uniform over the workgroup
-            __syncthreads();
-            stage_halo<CK>(a, c0, x0, y0, z0, lds);
-            __syncthreads();
-            staged = c0;
-        }
-        int tap, coff;
-        step_k(q, tap, coff);
-        const int dhv = tap_offset_hv(tap < 27 ? tap : 0);
+    // the A fragments of step st of a chunk for the four M blocks
+    auto load_a = [&](int st, bf16x8 (&af)[4]) {
+        int ta, tb;
+        step_taps(st, ta, tb);
+        const int oa = tap_offset(ta, H::R, H::Z) + H::Z + H::R + 1;
+        const int ob = tap_offset(tb < 27 ? tb : ta, H::R, H::Z) + H::Z + H::R + 1;
+        const bf16* p = lds + (hbase + lane_hi * (ob - oa)) * 8;
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-            const bf16x8 afrag = *reinterpret_cast<const bf16x8*>(lds + (hv0[mb] + dhv) * (CK + 8) + coff);
+        for (int mb = 0; mb < 4; ++mb) af[mb] = *reinterpret_cast<const bf16x8*>(p + (oa + mb * H::R) * 8);
+    };
+    if constexpr (WLDS) {
+        // Halo and weights of each chunk staged together (every load issued before
+        // the first store), then the chunk's K steps from LDS alone: all of them,
+        // unrolled, for a whole K loop; this slice's range for a split one.
+        bf16* ldw = lds + H::LDS;
+        auto step = [&](int st, int s) {
+            bf16x8 af[4], bf[NB];
+            load_a(st, af);
 #pragma unroll
             for (int nb = 0; nb < NB; ++nb)
-                acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afrag, bcur[nb], acc[mb][nb], 0, 0, 0);
-        }
+                bf[nb] = *reinterpret_cast<const bf16x8*>(ldw + (((s * NB + nb) * 4 + kg) * 16 + r16) * 8);
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) bcur[nb] = bnext[nb];
+            for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+                for (int nb = 0; nb < NB; ++nb)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], bf[nb], acc[mb][nb], 0, 0, 0);
+        };
+        const int c_hi = (q1 - 1) / STEPS;
+        for (int c = q0 / STEPS; c <= c_hi; ++c) {
+            const int lo = SPLIT ? max(q0 - c * STEPS, 0) : 0;
+            const int hi = SPLIT ? min(q1 - c * STEPS, STEPS) : STEPS;
+            __syncthreads();
+            {
+                bf16x8 hr[H::N], wr[WChunk<CK, NB>::N];
+                load_halo<CK, H::N>(a, c * CK, x0, y0, z0, 0, hr);
+                load_wchunk<CK, NB>(a, c, lo, hi - lo, cbase, wr);
+                store_halo<CK, H::N>(hr, 0, lds);
+                store_wchunk<CK, NB>(wr, hi - lo, ldw);
+            }
+            __syncthreads();
+            if constexpr (!SPLIT) {
+#pragma unroll
+                for (int st = 0; st < STEPS; ++st) step(st, st);
+            } else {
+                for (int st = lo; st < hi; ++st) step(st, st - lo);
+            }
+        }
+    } else if constexpr (!SPLIT) {
+        // Whole chunks: the chunk's K steps unrolled, the weight fragments of step
+        // st + 3 (of this chunk or the next) loaded while step st computes. 3
+        // divides the steps per chunk, so ring slot st % 3 holds step st in every
+        // chunk; past the last chunk the prefetch repeats it (unused).
+        constexpr int D = 3;
+        static_assert(STEPS % D == 0, "ring slots");
+        const int nch = (a.cin + CK - 1) / CK;
+        bf16x8 ring[D][NB];
+#pragma unroll
+        for (int d = 0; d < D; ++d) load_b(d, ring[d]);
+        for (int c = 0; c < nch; ++c) {
+            __syncthreads();
+            stage_halo<CK>(a, c * CK, x0, y0, z0, lds);
+            __syncthreads();
+#pragma unroll
+            for (int st = 0; st < STEPS; ++st) {
+                const int q = c * STEPS + st;
+                bf16x8 af[4];
+                load_a(st, af);
+#pragma unroll
+                for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[mb][nb] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], ring[st % D][nb], acc[mb][nb], 0, 0, 0);
+                load_b(min(q + D, nch * STEPS - 1), ring[st % D]);
+            }
+        }
+    } else {
+        constexpr int D = NB == 1 ? 2 : (NB == 2 ? 4 : 1);
+        // Every step issues exactly one prefetch (past the end: the last step
+        // again, unused).
+        bf16x8 ring[D][NB];
+#pragma unroll
+        for (int d = 0; d < D; ++d) load_b(min(q0 + d, q1 - 1), ring[d]);
+        int staged = -1;
+        for (int qb = q0; qb < q1; qb += D) {
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int q = qb + d;
+                if (q >= q1) break;  // uniform over the workgroup
+                const int c0 = (q / STEPS) * CK;
+                if (c0 != staged) {  // uniform over the workgroup
+                    __syncthreads();
+                    stage_halo<CK>(a, c0, x0, y0, z0, lds);
+                    __syncthreads();
+                    staged = c0;
+                }
+                bf16x8 af[4];
+                load_a(q % STEPS, af);
+#pragma unroll
+                for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb)
+                        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[mb], ring[d][nb], acc[mb][nb], 0, 0, 0);
+                load_b(min(q + D, q1 - 1), ring[d]);
+            }
+        }
     }
-    // epilogue: D reg r of lane -> voxel row kg*4 + r of the M block, channel nb*16 + r16
-    const int z = z0 + w;
+    // epilogue: D reg r of lane -> voxel x = kg*4 + r of the M block's row, channel nb*16 + r16.
+    // The bias is read before any store: the FP32 output could alias it as far as
+    // the compiler knows, and a load issued after stores waits for all of them.
+    float bias[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) bias[nb] = a.bias && nks == 1 ? a.bias[cbase + nb * 16 + r16] : 0.f;
+    if constexpr (!SPLIT && 512 * NB * 16 <= LDS_ALL) {
+        if (a.out && !a.out32) {  // uniform: BF16 output only
+            // the tile's outputs, [voxel m][NB * 16] bf16 with m = (zl * 8 + yl) * 16 + xl,
+            // gathered in LDS and written as 16-byte chunks of whole voxels
+            __syncthreads();  // every wave is done with the halo and weights
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = ((w >> 1) * 8 + (w & 1) * 4 + mb) * 16 + kg * 4 + r;
+#pragma unroll
+                    for (int nb = 0; nb < NB; ++nb) {
+                        float val = acc[mb][nb][r] + bias[nb];
+                        if (a.relu) val = val > 0.f ? val : 0.f;
+                        lds[m * NB * 16 + nb * 16 + r16] = (bf16)val;
+                    }
+                }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 2 * NB; ++i) {
+                const int j = threadIdx.x + i * kConvThreads;
+                const int m = j / (2 * NB), part = j % (2 * NB);
+                const int x = x0 + (m & 15), y = y0 + ((m >> 4) & 7), z = z0 + (m >> 7);
+                if (x < a.nx && y < a.ny && z < a.nz) {
+                    const size_t v = ((size_t)z * a.ny + y) * a.nx + x;
+                    *reinterpret_cast<bf16x8*>(a.out + v * a.cout + cbase + part * 8) =
+                        *reinterpret_cast<const bf16x8*>(lds + m * NB * 16 + part * 8);
+                }
+            }
+            return;
+        }
+    }
+    const int z = z0 + (w >> 1);
+    if (!SPLIT && a.out32 && a.out32_ch0 && !a.out && (a.nx & 3) == 0) {  // uniform
+        // FP32 channel 0 only: lanes r16 = 0 of block 0 hold it for 4 consecutive x
+        if (cbase == 0 && r16 == 0 && z < a.nz) {
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const int x = x0 + kg * 4, y = y0 + (w & 1) * 4 + mb;
+                if (x >= a.nx || y >= a.ny) continue;
+                float4 o;
+                o.x = acc[mb][0][0] + bias[0];
+                o.y = acc[mb][0][1] + bias[0];
+                o.z = acc[mb][0][2] + bias[0];
+                o.w = acc[mb][0][3] + bias[0];
+                if (a.relu) {
+                    o.x = fmaxf(o.x, 0.f);
+                    o.y = fmaxf(o.y, 0.f);
+                    o.z = fmaxf(o.z, 0.f);
+                    o.w = fmaxf(o.w, 0.f);
+                }
+                *reinterpret_cast<float4*>(a.out32 + ((size_t)z * a.ny + y) * a.nx + x) = o;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int m = mb * 16 + kg * 4 + r;
-            const int x = x0 + m % 8, y = y0 + m / 8;
+            const int x = x0 + kg * 4 + r, y = y0 + (w & 1) * 4 + mb;
             if (x >= a.nx || y >= a.ny || z >= a.nz) continue;
             const size_t v = ((size_t)z * a.ny + y) * a.nx + x;
             if (nks > 1) {
@@ -186,7 +464,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(ConvArgs a) {
             for (int nb = 0; nb < NB; ++nb) {
                 const int co = cbase + nb * 16 + r16;
                 float val = acc[mb][nb][r];
-                if (a.bias) val += a.bias[co];
+                if (a.bias) val += bias[nb];
                 if (a.relu) val = val > 0.f ? val : 0.f;
                 if (a.out) a.out[v * a.cout + co] = (bf16)val;
                 if (a.out32) {
