@@ -45,14 +45,15 @@ def _forced(a, key, forced):
 
 def forward(flat, rhs, solid=None, liquid=None, in_scale=1.0, grad_bf16=True, forced=None):
     """flat: fp32 parameter vector (the library's layout, requires_grad allowed);
-    rhs/solid/liquid: (nk, nj, ni) fp32 CPU tensors. Returns the prediction (nk, nj, ni).
+    rhs/solid/liquid: (nk, nj, ni) fp32 tensors (CPU, or all on one GPU for large
+    grids). Returns the prediction (nk, nj, ni).
     grad_bf16: round each convolution's output gradient to bf16 in the backward
     pass, where the GPU does. forced: optional {"in", "P0".."P2", layer index:
     (C, z, y, x) tensor} of stored forward values to use in place of the ones
     recomputed here (the gradients still flow through this forward)."""
     layers = split_params(flat)
     nk, nj, ni = rhs.shape
-    x = torch.zeros(1, 16, nk, nj, ni)
+    x = torch.zeros(1, 16, nk, nj, ni, device=rhs.device)
     x[0, 0] = _bf(rhs * in_scale)
     x[0, 1] = _bf(solid) if solid is not None else 0.0
     x[0, 2] = _bf(liquid) if liquid is not None else 1.0

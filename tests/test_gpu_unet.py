@@ -63,6 +63,35 @@ def test_unet_forward_and_gradients_match_reference(cuda, shape):
         assert _rel(g[b0:b1], grad_ref[b0:b1]) <= 5e-2 or float(grad_ref[b0:b1].norm()) < 1e-8, f"layer {l} bias"
 
 
+@pytest.mark.parametrize("n", [64, 128])
+def test_unet_large_grid_matches_reference(cuda, n):
+    """Grids whose convolutions run unsplit (64^3) and, at full resolution, with
+    persistent workgroups (128^3). The reference runs in fp32 on the GPU (torch
+    conv3d with TF32 off); same tolerances."""
+    print(f"unet parity {n}^3", flush=True)
+    import torch
+    from oracle import unet_ref
+    from vflip_amd.unet import UNet, param_layout
+    net = UNet(seed=3)
+    rhs, solid, liquid, target = _sample((n, n, n), 2)
+    rhs, solid, liquid, target = rhs.cuda(), solid.cuda(), liquid.cuda(), target.cuda()
+    flat = net.params().detach().clone()
+    with torch.backends.cudnn.flags(enabled=True, allow_tf32=False):
+        loss_ref, grad_ref, pred_ref = unet_ref.loss_and_grads(flat, rhs, solid, liquid, 1.0, target)
+    pred = net.forward(rhs, solid, liquid, 1.0)
+    torch.cuda.synchronize()
+    assert _rel(pred, pred_ref) <= 2e-2
+    net.zero_grad()
+    loss = net.train_step(rhs, solid, liquid, 1.0, target)
+    assert abs(loss - loss_ref) <= 2e-2 * loss_ref
+    g = net.grads()
+    errs = [round(_rel(g[w0:w1], grad_ref[w0:w1]), 4) for (w0, w1, _, _) in _layer_slices()]
+    print("per-layer relative gradient error:", errs, flush=True)
+    for l, (w0, w1, b0, b1) in enumerate(_layer_slices()):
+        assert _rel(g[w0:w1], grad_ref[w0:w1]) <= 5e-2, f"layer {l} weights"
+        assert _rel(g[b0:b1], grad_ref[b0:b1]) <= 5e-2 or float(grad_ref[b0:b1].norm()) < 1e-8, f"layer {l} bias"
+
+
 def test_unet_adam_reduces_loss(cuda):
     print("unet adam", flush=True)
     import torch

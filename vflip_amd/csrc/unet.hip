@@ -159,13 +159,14 @@ int galloc(vflip_unet* N, T** p, size_t n) {
 }
 
 // Launch plan of a convolution: output channels split into g groups until the
-// layer has 128 workgroups and each workgroup at most 6 16-channel blocks (or
+// layer has 256 workgroups and each workgroup at most 6 16-channel blocks (or
 // one block per workgroup); then, for the small grids, the K loop into ks
 // slices (powers of two) until about 256 workgroups, each slice at least 9 of
-// the K steps.
+// the K steps. Groups come first: a split K loop adds an FP32 partial per
+// slice and a reduction launch.
 int conv_plan(int ntiles, int NBall, int cin, int ck, int& g) {
     g = 1;
-    while ((ntiles * g < 128 || NBall / g > 6) && NBall % (2 * g) == 0) g *= 2;
+    while ((ntiles * g < 256 || NBall / g > 6) && NBall % (2 * g) == 0) g *= 2;
     const int total = ((cin + ck - 1) / ck) * (ck == 32 ? 27 : 15);
     int ks = 1;
     while (ntiles * g * ks * 2 <= 256 && total / (ks * 2) >= 9) ks *= 2;
@@ -226,8 +227,26 @@ int ensure_grid(vflip_unet* N, int nx, int ny, int nz) {
 
 ConvSrc make_src(const vflip_unet* N, const In& in) { return ConvSrc{N->act[in.t], in.C, in.up}; }
 
+// Workgroups of k_conv3_tiles that fit on the current device at once (0 if the
+// query fails), cached per device.
+int tiles_grid() {
+    static int cache[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    if (cache[dev] == 0) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_conv3_tiles, kConvThreads, 0) != hipSuccess)
+            return 0;
+        cache[dev] = cus * per;
+    }
+    return cache[dev];
+}
+
 // One convolution by conv_plan: ntiles x groups x ksplit workgroups, and for a
 // split one the reduction of its partials (workspace sized by splitk_floats).
+// Unsplit 16-channel layers into 16 outputs on grids of at least four rounds of
+// workgroups run persistent (k_conv3_tiles).
 template <int CK>
 void launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
     int g;
@@ -236,6 +255,15 @@ void launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
     a.ksplit = ks;
     a.part = ks > 1 ? ws : nullptr;
     const int NB = NBall / g;
+    if (CK == 16 && ks == 1 && g == 1 && NB == 1) {
+        const bool bf16_out = a.out && !a.out32;
+        const bool ch0_out = !a.out && a.out32 && a.out32_ch0 && (a.nx & 3) == 0;
+        const int pg = tiles_grid();
+        if ((bf16_out || ch0_out) && pg > 0 && a.ntiles >= 4 * pg) {
+            k_conv3_tiles<<<pg, kConvThreads, 0, s>>>(a);
+            return;
+        }
+    }
     const dim3 grid(a.ntiles * g * ks), block(kConvThreads);
     if (ks > 1) {
         switch (NB) {

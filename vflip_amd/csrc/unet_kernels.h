@@ -198,6 +198,86 @@ __device__ inline void store_wchunk(const bf16x8 (&wr)[WChunk<CK, NB>::N], int n
     }
 }
 
+// Epilogue pieces shared by the convolution kernels. D register r of lane
+// (r16, kg) in M block mb, block nb holds voxel x = kg*4 + r of row (w & 1) * 4 +
+// mb of z-slice w >> 1 of the tile, channel nb*16 + r16.
+
+// The tile's BF16 outputs (bias, ReLU) into LDS as [voxel m][NB * 16], m =
+// (zl * 8 + yl) * 16 + xl.
+template <int NB>
+__device__ inline void tile_to_lds(const ConvArgs& a, const f32x4 (&acc)[4][NB], const float (&bias)[NB], bf16* out) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kg = lane >> 4;
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = ((w >> 1) * 8 + (w & 1) * 4 + mb) * 16 + kg * 4 + r;
+#pragma unroll
+            for (int nb = 0; nb < NB; ++nb) {
+                float val = acc[mb][nb][r] + bias[nb];
+                if (a.relu) val = val > 0.f ? val : 0.f;
+                out[m * NB * 16 + nb * 16 + r16] = (bf16)val;
+            }
+        }
+}
+
+// The tile's outputs from LDS to a.out, 16-byte chunks of whole voxels.
+template <int NB>
+__device__ inline void lds_to_out(const ConvArgs& a, const bf16* out, int x0, int y0, int z0, int cbase) {
+#pragma unroll
+    for (int i = 0; i < 2 * NB; ++i) {
+        const int j = threadIdx.x + i * kConvThreads;
+        const int m = j / (2 * NB), part = j % (2 * NB);
+        const int x = x0 + (m & 15), y = y0 + ((m >> 4) & 7), z = z0 + (m >> 7);
+        if (x < a.nx && y < a.ny && z < a.nz) {
+            const size_t v = ((size_t)z * a.ny + y) * a.nx + x;
+            *reinterpret_cast<bf16x8*>(a.out + v * a.cout + cbase + part * 8) =
+                *reinterpret_cast<const bf16x8*>(out + m * NB * 16 + part * 8);
+        }
+    }
+}
+
+// FP32 channel 0 alone (a.out32_ch0; a.nx a multiple of 4): lanes r16 = 0 of
+// block 0 hold it for 4 consecutive x. Written to out32, or (lds != nullptr)
+// to LDS as [m] floats.
+template <int NB>
+__device__ inline void tile_ch0(const ConvArgs& a, const f32x4 (&acc)[4][NB], float bias0, int x0, int y0, int z0,
+                                float* lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r16 = lane & 15, kg = lane >> 4;
+    if (r16 != 0) return;
+    const int z = z0 + (w >> 1);
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) {
+        float4 o;
+        o.x = acc[mb][0][0] + bias0;
+        o.y = acc[mb][0][1] + bias0;
+        o.z = acc[mb][0][2] + bias0;
+        o.w = acc[mb][0][3] + bias0;
+        if (a.relu) {
+            o.x = fmaxf(o.x, 0.f);
+            o.y = fmaxf(o.y, 0.f);
+            o.z = fmaxf(o.z, 0.f);
+            o.w = fmaxf(o.w, 0.f);
+        }
+        if (lds) {
+            *reinterpret_cast<float4*>(lds + ((w >> 1) * 8 + (w & 1) * 4 + mb) * 16 + kg * 4) = o;
+        } else {
+            const int x = x0 + kg * 4, y = y0 + (w & 1) * 4 + mb;
+            if (x < a.nx && y < a.ny && z < a.nz) *reinterpret_cast<float4*>(a.out32 + ((size_t)z * a.ny + y) * a.nx + x) = o;
+        }
+    }
+}
+
+// Channel 0 from LDS ([m] floats) to out32, 16 bytes (4 x voxels) per thread.
+__device__ inline void lds_ch0_to_out32(const ConvArgs& a, const float* lds, int x0, int y0, int z0) {
+    if (threadIdx.x >= 128) return;
+    const int m = threadIdx.x * 4;
+    const int x = x0 + (m & 15), y = y0 + ((m >> 4) & 7), z = z0 + (m >> 7);
+    if (x < a.nx && y < a.ny && z < a.nz)
+        *reinterpret_cast<float4*>(a.out32 + ((size_t)z * a.ny + y) * a.nx + x) =
+            *reinterpret_cast<const float4*>(lds + m);
+}
+
 // LDS of a convolution workgroup (bf16 elements): the halo, plus the weights of
 // the chunk's K steps where they fit (up to 3 output blocks: 16-channel chunks,
 // and 32-channel chunks of split K loops, which run on small grids at one
@@ -392,58 +472,17 @@ __global__ __launch_bounds__(kConvThreads, (conv_waves_per_simd<CK, NB, SPLIT>()
     for (int nb = 0; nb < NB; ++nb) bias[nb] = a.bias && nks == 1 ? a.bias[cbase + nb * 16 + r16] : 0.f;
     if constexpr (!SPLIT && 512 * NB * 16 <= LDS_ALL) {
         if (a.out && !a.out32) {  // uniform: BF16 output only
-            // the tile's outputs, [voxel m][NB * 16] bf16 with m = (zl * 8 + yl) * 16 + xl,
-            // gathered in LDS and written as 16-byte chunks of whole voxels
+            // the tile's outputs gathered in LDS, written as 16-byte chunks of whole voxels
             __syncthreads();  // every wave is done with the halo and weights
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int m = ((w >> 1) * 8 + (w & 1) * 4 + mb) * 16 + kg * 4 + r;
-#pragma unroll
-                    for (int nb = 0; nb < NB; ++nb) {
-                        float val = acc[mb][nb][r] + bias[nb];
-                        if (a.relu) val = val > 0.f ? val : 0.f;
-                        lds[m * NB * 16 + nb * 16 + r16] = (bf16)val;
-                    }
-                }
+            tile_to_lds<NB>(a, acc, bias, lds);
             __syncthreads();
-#pragma unroll
-            for (int i = 0; i < 2 * NB; ++i) {
-                const int j = threadIdx.x + i * kConvThreads;
-                const int m = j / (2 * NB), part = j % (2 * NB);
-                const int x = x0 + (m & 15), y = y0 + ((m >> 4) & 7), z = z0 + (m >> 7);
-                if (x < a.nx && y < a.ny && z < a.nz) {
-                    const size_t v = ((size_t)z * a.ny + y) * a.nx + x;
-                    *reinterpret_cast<bf16x8*>(a.out + v * a.cout + cbase + part * 8) =
-                        *reinterpret_cast<const bf16x8*>(lds + m * NB * 16 + part * 8);
-                }
-            }
+            lds_to_out<NB>(a, lds, x0, y0, z0, cbase);
             return;
         }
     }
     const int z = z0 + (w >> 1);
     if (!SPLIT && a.out32 && a.out32_ch0 && !a.out && (a.nx & 3) == 0) {  // uniform
-        // FP32 channel 0 only: lanes r16 = 0 of block 0 hold it for 4 consecutive x
-        if (cbase == 0 && r16 == 0 && z < a.nz) {
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) {
-                const int x = x0 + kg * 4, y = y0 + (w & 1) * 4 + mb;
-                if (x >= a.nx || y >= a.ny) continue;
-                float4 o;
-                o.x = acc[mb][0][0] + bias[0];
-                o.y = acc[mb][0][1] + bias[0];
-                o.z = acc[mb][0][2] + bias[0];
-                o.w = acc[mb][0][3] + bias[0];
-                if (a.relu) {
-                    o.x = fmaxf(o.x, 0.f);
-                    o.y = fmaxf(o.y, 0.f);
-                    o.z = fmaxf(o.z, 0.f);
-                    o.w = fmaxf(o.w, 0.f);
-                }
-                *reinterpret_cast<float4*>(a.out32 + ((size_t)z * a.ny + y) * a.nx + x) = o;
-            }
-        }
+        if (cbase == 0) tile_ch0<NB>(a, acc, bias[0], x0, y0, z0, nullptr);
         return;
     }
 #pragma unroll
@@ -473,6 +512,111 @@ __global__ __launch_bounds__(kConvThreads, (conv_waves_per_simd<CK, NB, SPLIT>()
                 }
             }
         }
+}
+
+// Persistent form of k_conv3 for whole K loops of 16-channel chunks into one
+// block of 16 outputs on large grids (a.groups 1, BF16 output or FP32 channel
+// 0). gridDim.x workgroups, as many as fit on the GPU at once, walk tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ..., each over its chunks. The halo (and,
+// with several chunks, the weights) of the next (tile, chunk) is loaded into
+// registers while the current one computes and is stored into LDS after it;
+// with one chunk the weights are staged once. A tile's outputs wait in LDS and
+// are written to global memory after the next halo has arrived, so that the
+// wait for that halo never includes their stores. Per tile the arithmetic and
+// K order are k_conv3's: the outputs are the same bits.
+constexpr int kOutLds = 512 * 16;  // bf16: one tile of 16 BF16 outputs, or 512 FP32 channel-0 values
+__global__ __launch_bounds__(kConvThreads, 4) void k_conv3_tiles(ConvArgs a) {
+    constexpr int CK = 16, NB = 1, STEPS = 15;
+    using H = Halo<CK>;
+    using W = WChunk<CK, NB>;
+    __shared__ __attribute__((aligned(16))) bf16 lds[H::LDS + W::E * 8 + kOutLds];
+    bf16* ldw = lds + H::LDS;
+    bf16* ldo = ldw + W::E * 8;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kg = lane >> 4, lane_hi = kg >> 1;
+    const int hbase = (kg & 1) * H::PS + (w >> 1) * H::Z + (w & 1) * 4 * H::R + r16;
+    const int nch = (a.cin + CK - 1) / CK;
+    if ((int)blockIdx.x >= a.ntiles) return;  // uniform
+    const int nseq = ((a.ntiles - 1 - (int)blockIdx.x) / (int)gridDim.x + 1) * nch;
+    const bool ch0 = a.out32 != nullptr;  // FP32 channel 0 instead of BF16 outputs
+    // sequence item k: tile origin and chunk
+    auto at = [&](int k, int& x0, int& y0, int& z0, int& c) {
+        c = k % nch;
+        const int L = (int)blockIdx.x + (k / nch) * (int)gridDim.x;
+        int t = L;
+        if (a.ntiles % 8 == 0) t = (L % 8) * (a.ntiles / 8) + L / 8;
+        x0 = (t % a.tiles_x) * kTX;
+        y0 = ((t / a.tiles_x) % a.tiles_y) * kTY;
+        z0 = (t / (a.tiles_x * a.tiles_y)) * kTZ;
+    };
+    // bias before any store (a load after stores would wait for them)
+    const float bias = a.bias ? a.bias[r16] : 0.f;
+    const float bias0 = __shfl(bias, 0);
+    bf16x8 hr[H::N], wr[W::N];
+    {
+        int x0, y0, z0, c;
+        at(0, x0, y0, z0, c);
+        load_halo<CK, H::N>(a, c * CK, x0, y0, z0, 0, hr);
+        load_wchunk<CK, NB>(a, c, 0, STEPS, 0, wr);
+    }
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) acc[mb][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int px0 = 0, py0 = 0, pz0 = 0;
+    bool pending = false;
+    for (int k = 0; k < nseq; ++k) {
+        int x0, y0, z0, c;
+        at(k, x0, y0, z0, c);
+        __syncthreads();  // the previous item's K loop and the pending outputs' LDS reads are done
+        store_halo<CK, H::N>(hr, 0, lds);
+        if (nch > 1 || k == 0) store_wchunk<CK, NB>(wr, STEPS, ldw);
+        __syncthreads();
+        if (k + 1 < nseq) {
+            int x1, y1, z1, c1;
+            at(k + 1, x1, y1, z1, c1);
+            load_halo<CK, H::N>(a, c1 * CK, x1, y1, z1, 0, hr);
+            if (nch > 1) load_wchunk<CK, NB>(a, c1, 0, STEPS, 0, wr);
+        }
+        if (pending) {
+            if (ch0) lds_ch0_to_out32(a, reinterpret_cast<const float*>(ldo), px0, py0, pz0);
+            else lds_to_out<NB>(a, ldo, px0, py0, pz0, 0);
+            pending = false;
+        }
+#pragma unroll
+        for (int st = 0; st < STEPS; ++st) {
+            int ta, tb;
+            tap_pair16(st, ta, tb);
+            const int oa = tap_offset(ta, H::R, H::Z) + H::Z + H::R + 1;
+            const int ob = tap_offset(tb < 27 ? tb : ta, H::R, H::Z) + H::Z + H::R + 1;
+            const bf16* p = lds + (hbase + lane_hi * (ob - oa)) * 8;
+            const bf16x8 bf = *reinterpret_cast<const bf16x8*>(ldw + ((st * 4 + kg) * 16 + r16) * 8);
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) {
+                const bf16x8 af = *reinterpret_cast<const bf16x8*>(p + (oa + mb * H::R) * 8);
+                acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[mb][0], 0, 0, 0);
+            }
+        }
+        if (c == nch - 1) {
+            __syncthreads();  // every wave has read the pending outputs
+            if (ch0) {
+                tile_ch0<NB>(a, acc, bias0, x0, y0, z0, reinterpret_cast<float*>(ldo));
+            } else {
+                const float b1[1] = {bias};
+                tile_to_lds<NB>(a, acc, b1, ldo);
+            }
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) acc[mb][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+            px0 = x0;
+            py0 = y0;
+            pz0 = z0;
+            pending = true;
+        }
+    }
+    __syncthreads();
+    if (pending) {
+        if (ch0) lds_ch0_to_out32(a, reinterpret_cast<const float*>(ldo), px0, py0, pz0);
+        else lds_to_out<NB>(a, ldo, px0, py0, pz0, 0);
+    }
 }
 
 // Sum of a split convolution's partials in slice order, then the epilogue of k_conv3.
