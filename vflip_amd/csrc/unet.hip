@@ -247,21 +247,25 @@ int tiles_grid() {
 // split one the reduction of its partials (workspace sized by splitk_floats).
 // Unsplit 16-channel layers into 16 outputs on grids of at least four rounds of
 // workgroups run persistent (k_conv3_tiles).
+// Returns whether the kernels also wrote a.pool (otherwise the caller pools).
 template <int CK>
-void launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
+bool launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
     int g;
     const int ks = conv_plan(a.ntiles, NBall, a.cin, CK, g);
     a.groups = g;
     a.ksplit = ks;
     a.part = ks > 1 ? ws : nullptr;
     const int NB = NBall / g;
+    const bool pool_req = a.pool && a.out && !a.out32;
+    const bool pooled = pool_req && (ks > 1 ? (a.nx | a.ny | a.nz) % 2 == 0 : conv_tile_in_lds(CK, NB, false));
+    if (!pooled) a.pool = nullptr;
     if (CK == 16 && ks == 1 && g == 1 && NB == 1) {
         const bool bf16_out = a.out && !a.out32;
         const bool ch0_out = !a.out && a.out32 && a.out32_ch0 && (a.nx & 3) == 0;
         const int pg = tiles_grid();
         if ((bf16_out || ch0_out) && pg > 0 && a.ntiles >= 4 * pg) {
             k_conv3_tiles<<<pg, kConvThreads, 0, s>>>(a);
-            return;
+            return pooled;
         }
     }
     const dim3 grid(a.ntiles * g * ks), block(kConvThreads);
@@ -290,13 +294,15 @@ void launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
     }
     if (ks > 1) {
         const size_t n = (size_t)a.nx * a.ny * a.nz * a.cout;
-        k_conv_splitk_reduce<<<grid256(n), 256, 0, s>>>(a);
+        if (pooled) k_conv_splitk_reduce_pool<<<grid256(n / 8), 256, 0, s>>>(a);
+        else k_conv_splitk_reduce<<<grid256(n), 256, 0, s>>>(a);
     }
+    return pooled;
 }
 
-void conv(const vflip_unet* N, const ConvArgs& a, int ck, hipStream_t s) {
-    if (ck == 32) launch_conv<32>(a, a.cout / 16, s, N->splitk);
-    else launch_conv<16>(a, a.cout / 16, s, N->splitk);
+bool conv(const vflip_unet* N, const ConvArgs& a, int ck, hipStream_t s) {
+    if (ck == 32) return launch_conv<32>(a, a.cout / 16, s, N->splitk);
+    return launch_conv<16>(a, a.cout / 16, s, N->splitk);
 }
 
 ConvArgs base_args(const vflip_unet* N, int level) {
@@ -312,13 +318,8 @@ void forward_layers(vflip_unet* N) {
     hipStream_t s = N->stream;
     for (int l = 0; l < kLayers; ++l) {
         const LayerDef& d = kNet[l];
-        if (l == 2 || l == 4 || l == 6) {  // pool the skip of the level above
-            const int p = kP0 + (l / 2 - 1), src = kPoolOf[l / 2 - 1];
-            int x, y, z;
-            level_dims(N, kNet[src].level, x, y, z);
-            const size_t n = level_vox(N, kNet[src].level + 1) * kNet[src].cout;
-            k_avgpool<<<grid256(n), 256, 0, s>>>(N->act[src], x, y, z, kNet[src].cout, N->act[p]);
-        }
+        // layers 1, 3, 5 feed the pooled input of the level below
+        const int pool_t = l == 1 ? kP0 : l == 3 ? kP1 : l == 5 ? kP2 : -1;
         ConvArgs a = base_args(N, d.level);
         a.nsrc = d.nin;
         for (int q = 0; q < d.nin; ++q) a.src[q] = make_src(N, d.in[q]);
@@ -334,7 +335,13 @@ void forward_layers(vflip_unet* N) {
         } else {
             a.out = N->act[l];
         }
-        conv(N, a, ck_of(d), s);
+        if (pool_t >= 0) a.pool = N->act[pool_t];
+        if (!conv(N, a, ck_of(d), s) && pool_t >= 0) {
+            int x, y, z;
+            level_dims(N, d.level, x, y, z);
+            const size_t n = level_vox(N, d.level + 1) * d.cout;
+            k_avgpool<<<grid256(n), 256, 0, s>>>(N->act[l], x, y, z, d.cout, N->act[pool_t]);
+        }
     }
 }
 

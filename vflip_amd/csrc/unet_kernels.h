@@ -51,6 +51,7 @@ struct ConvArgs {
     int groups;  // output-channel groups: workgroup L computes channels of group L / ntiles
     int ksplit;  // > 1: the K loop is split over ksplit workgroups that write FP32 partials
     float* part; // [ksplit][V][cout] partials, summed by k_conv_splitk_reduce
+    bf16* pool;  // with a.out: also the 2x2x2 means of the outputs, [V / 8][cout] (or nullptr)
 };
 
 // LDS layout of the halo tile of CK input channels: P = CK / 8 channel planes of
@@ -237,6 +238,54 @@ __device__ inline void lds_to_out(const ConvArgs& a, const bf16* out, int x0, in
     }
 }
 
+// 2x2x2 means of the tile's BF16 outputs in LDS, each the FP32 sum of the 8 BF16
+// values in k_avgpool's order times 0.125 (the same bits as k_avgpool on the
+// stored outputs), two channels per thread. Written to a.pool (the half-
+// resolution grid, a.cout channels), or (pl != nullptr) to LDS as [64 coarse
+// voxels of the tile][NB * 16] for lds_pool_copy.
+template <int NB>
+__device__ inline void lds_pool(const ConvArgs& a, const bf16* out, int x0, int y0, int z0, int cbase, bf16* pl) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    constexpr int P = 8 * NB;  // channel pairs per voxel
+    const int CX = a.nx / 2, CY = a.ny / 2, CZ = a.nz / 2;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int j = threadIdx.x + i * kConvThreads;  // 64 coarse voxels x P pairs
+        const int cm = j / P, cp = j % P;
+        const int ci = cm & 7, cj = (cm >> 3) & 3, ck = cm >> 5;  // coarse voxel in the tile's 8 x 4 x 2
+        float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int m = ((2 * ck + (q >> 2)) * 8 + (2 * cj + ((q >> 1) & 1))) * 16 + 2 * ci + (q & 1);
+            const bf16x2 v = *reinterpret_cast<const bf16x2*>(out + m * NB * 16 + cp * 2);
+            s0 += (float)v[0];
+            s1 += (float)v[1];
+        }
+        bf16x2 o;
+        o[0] = (bf16)(0.125f * s0);
+        o[1] = (bf16)(0.125f * s1);
+        if (pl) {
+            *reinterpret_cast<bf16x2*>(pl + cm * NB * 16 + cp * 2) = o;
+        } else {
+            const int cx = x0 / 2 + ci, cy = y0 / 2 + cj, cz = z0 / 2 + ck;
+            if (cx < CX && cy < CY && cz < CZ)
+                *reinterpret_cast<bf16x2*>(a.pool + (((size_t)cz * CY + cy) * CX + cx) * a.cout + cbase + cp * 2) = o;
+        }
+    }
+}
+
+// The pooled values of a tile from LDS (lds_pool) to a.pool, 16-byte chunks.
+template <int NB>
+__device__ inline void lds_pool_copy(const ConvArgs& a, const bf16* pl, int x0, int y0, int z0, int cbase) {
+    const int j = threadIdx.x;  // 64 coarse voxels x 2 NB 16-byte parts
+    if (j >= 64 * 2 * NB) return;
+    const int cm = j / (2 * NB), part = j % (2 * NB);
+    const int cx = x0 / 2 + (cm & 7), cy = y0 / 2 + ((cm >> 3) & 3), cz = z0 / 2 + (cm >> 5);
+    if (cx < a.nx / 2 && cy < a.ny / 2 && cz < a.nz / 2)
+        *reinterpret_cast<bf16x8*>(a.pool + (((size_t)cz * (a.ny / 2) + cy) * (a.nx / 2) + cx) * a.cout + cbase +
+                                   part * 8) = *reinterpret_cast<const bf16x8*>(pl + cm * NB * 16 + part * 8);
+}
+
 // FP32 channel 0 alone (a.out32_ch0; a.nx a multiple of 4): lanes r16 = 0 of
 // block 0 hold it for 4 consecutive x. Written to out32, or (lds != nullptr)
 // to LDS as [m] floats.
@@ -284,10 +333,19 @@ __device__ inline void lds_ch0_to_out32(const ConvArgs& a, const float* lds, int
 // workgroup per CU). The compiler is held
 // to the register budget of the waves per SIMD that this LDS allows (2 per
 // workgroup, 160 KB per CU), unless the workgroup needs more registers.
+__host__ __device__ constexpr bool conv_wlds(int ck, int nb, bool split) { return (ck == 16 || split) && nb <= 3; }
+__host__ __device__ constexpr int conv_lds(int ck, int nb, bool split) {
+    return (ck == 16 ? Halo<16>::LDS : Halo<32>::LDS) + (conv_wlds(ck, nb, split) ? (ck == 32 ? 27 : 15) * nb * 64 * 8 : 0);
+}
+// The outputs of a tile (512 voxels x 16 NB BF16) fit in the workgroup's LDS: an
+// unsplit epilogue writes through LDS (16-byte stores, pooling).
+__host__ __device__ constexpr bool conv_tile_in_lds(int ck, int nb, bool split) {
+    return 512 * nb * 16 <= conv_lds(ck, nb, split);
+}
 template <int CK, int NB, bool SPLIT>
-constexpr bool conv_wlds() { return CK == 16 ? NB <= 3 : SPLIT && NB <= 3; }
+constexpr bool conv_wlds() { return conv_wlds(CK, NB, SPLIT); }
 template <int CK, int NB, bool SPLIT>
-constexpr int conv_lds() { return Halo<CK>::LDS + (conv_wlds<CK, NB, SPLIT>() ? WChunk<CK, NB>::E * 8 : 0); }
+constexpr int conv_lds() { return conv_lds(CK, NB, SPLIT); }
 template <int CK, int NB, bool SPLIT>
 constexpr int conv_waves_per_simd() {
     const int b = (160 * 1024) / (conv_lds<CK, NB, SPLIT>() * 2);
@@ -470,13 +528,14 @@ __global__ __launch_bounds__(kConvThreads, (conv_waves_per_simd<CK, NB, SPLIT>()
     float bias[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; ++nb) bias[nb] = a.bias && nks == 1 ? a.bias[cbase + nb * 16 + r16] : 0.f;
-    if constexpr (!SPLIT && 512 * NB * 16 <= LDS_ALL) {
+    if constexpr (!SPLIT && conv_tile_in_lds(CK, NB, false)) {
         if (a.out && !a.out32) {  // uniform: BF16 output only
             // the tile's outputs gathered in LDS, written as 16-byte chunks of whole voxels
             __syncthreads();  // every wave is done with the halo and weights
             tile_to_lds<NB>(a, acc, bias, lds);
             __syncthreads();
             lds_to_out<NB>(a, lds, x0, y0, z0, cbase);
+            if (a.pool) lds_pool<NB>(a, lds, x0, y0, z0, cbase, nullptr);
             return;
         }
     }
@@ -514,6 +573,38 @@ __global__ __launch_bounds__(kConvThreads, (conv_waves_per_simd<CK, NB, SPLIT>()
         }
 }
 
+// k_conv_splitk_reduce for a BF16 output that also feeds a 2x2x2 average pool
+// (a.pool; even grid dims): one thread per (coarse voxel, channel) reduces its 8
+// fine voxels in k_avgpool's order and writes them and their mean, the same
+// bits as k_conv_splitk_reduce followed by k_avgpool.
+__global__ __launch_bounds__(256) void k_conv_splitk_reduce_pool(ConvArgs a) {
+    const int CX = a.nx / 2, CY = a.ny / 2, CZ = a.nz / 2;
+    const size_t V = (size_t)a.nx * a.ny * a.nz, nf = V * a.cout, n = (size_t)CX * CY * CZ * a.cout;
+    for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) {
+        const int co = (int)(e % (unsigned)a.cout);
+        const size_t cv = e / (unsigned)a.cout;
+        const int i = (int)(cv % CX), j = (int)((cv / CX) % CY), k = (int)(cv / ((size_t)CX * CY));
+        const float b = a.bias ? a.bias[co] : 0.f;
+        size_t fe[8];
+        bf16 o[8];
+        float sum = 0.f;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {  // every load before the first store (a load waits for earlier stores)
+            const size_t f = ((size_t)(2 * k + (q >> 2)) * a.ny + (2 * j + ((q >> 1) & 1))) * a.nx + (2 * i + (q & 1));
+            fe[q] = f * a.cout + co;
+            float val = a.part[fe[q]];
+            for (int sl = 1; sl < a.ksplit; ++sl) val += a.part[(size_t)sl * nf + fe[q]];
+            val += b;
+            if (a.relu) val = val > 0.f ? val : 0.f;
+            o[q] = (bf16)val;
+            sum += (float)o[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a.out[fe[q]] = o[q];
+        a.pool[e] = (bf16)(0.125f * sum);
+    }
+}
+
 // Persistent form of k_conv3 for whole K loops of 16-channel chunks into one
 // block of 16 outputs on large grids (a.groups 1, BF16 output or FP32 channel
 // 0). gridDim.x workgroups, as many as fit on the GPU at once, walk tiles
@@ -523,15 +614,17 @@ __global__ __launch_bounds__(kConvThreads, (conv_waves_per_simd<CK, NB, SPLIT>()
 // with one chunk the weights are staged once. A tile's outputs wait in LDS and
 // are written to global memory after the next halo has arrived, so that the
 // wait for that halo never includes their stores. Per tile the arithmetic and
-// K order are k_conv3's: the outputs are the same bits.
+// K order are k_conv3's: the outputs are the same bits. The tile's 2x2x2 means
+// (a.pool) are formed from its outputs in LDS and written with them.
 constexpr int kOutLds = 512 * 16;  // bf16: one tile of 16 BF16 outputs, or 512 FP32 channel-0 values
 __global__ __launch_bounds__(kConvThreads, 4) void k_conv3_tiles(ConvArgs a) {
     constexpr int CK = 16, NB = 1, STEPS = 15;
     using H = Halo<CK>;
     using W = WChunk<CK, NB>;
-    __shared__ __attribute__((aligned(16))) bf16 lds[H::LDS + W::E * 8 + kOutLds];
+    __shared__ __attribute__((aligned(16))) bf16 lds[H::LDS + W::E * 8 + kOutLds + 64 * 16];
     bf16* ldw = lds + H::LDS;
     bf16* ldo = ldw + W::E * 8;
+    bf16* ldp = ldo + kOutLds;  // the pending tile's pooled outputs
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r16 = lane & 15, kg = lane >> 4, lane_hi = kg >> 1;
     const int hbase = (kg & 1) * H::PS + (w >> 1) * H::Z + (w & 1) * 4 * H::R + r16;
@@ -560,8 +653,6 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_conv3_tiles(ConvArgs a) {
         load_wchunk<CK, NB>(a, c, 0, STEPS, 0, wr);
     }
     f32x4 acc[4][NB];
-#pragma unroll
-    for (int mb = 0; mb < 4; ++mb) acc[mb][0] = f32x4{0.f, 0.f, 0.f, 0.f};
     int px0 = 0, py0 = 0, pz0 = 0;
     bool pending = false;
     for (int k = 0; k < nseq; ++k) {
@@ -578,9 +669,17 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_conv3_tiles(ConvArgs a) {
             if (nch > 1) load_wchunk<CK, NB>(a, c1, 0, STEPS, 0, wr);
         }
         if (pending) {
-            if (ch0) lds_ch0_to_out32(a, reinterpret_cast<const float*>(ldo), px0, py0, pz0);
-            else lds_to_out<NB>(a, ldo, px0, py0, pz0, 0);
+            if (ch0) {
+                lds_ch0_to_out32(a, reinterpret_cast<const float*>(ldo), px0, py0, pz0);
+            } else {
+                lds_to_out<NB>(a, ldo, px0, py0, pz0, 0);
+                if (a.pool) lds_pool_copy<NB>(a, ldp, px0, py0, pz0, 0);
+            }
             pending = false;
+        }
+        if (c == 0) {
+#pragma unroll
+            for (int mb = 0; mb < 4; ++mb) acc[mb][0] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
 #pragma unroll
         for (int st = 0; st < STEPS; ++st) {
@@ -595,6 +694,7 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_conv3_tiles(ConvArgs a) {
                 const bf16x8 af = *reinterpret_cast<const bf16x8*>(p + (oa + mb * H::R) * 8);
                 acc[mb][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[mb][0], 0, 0, 0);
             }
+            if (st % 2 == 1) __builtin_amdgcn_sched_barrier(0);  // bounds the loads issued ahead: no spills
         }
         if (c == nch - 1) {
             __syncthreads();  // every wave has read the pending outputs
@@ -603,9 +703,11 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_conv3_tiles(ConvArgs a) {
             } else {
                 const float b1[1] = {bias};
                 tile_to_lds<NB>(a, acc, b1, ldo);
+                if (a.pool) {
+                    __syncthreads();
+                    lds_pool<NB>(a, ldo, x0, y0, z0, 0, ldp);
+                }
             }
-#pragma unroll
-            for (int mb = 0; mb < 4; ++mb) acc[mb][0] = f32x4{0.f, 0.f, 0.f, 0.f};
             px0 = x0;
             py0 = y0;
             pz0 = z0;
@@ -614,8 +716,12 @@ __global__ __launch_bounds__(kConvThreads, 4) void k_conv3_tiles(ConvArgs a) {
     }
     __syncthreads();
     if (pending) {
-        if (ch0) lds_ch0_to_out32(a, reinterpret_cast<const float*>(ldo), px0, py0, pz0);
-        else lds_to_out<NB>(a, ldo, px0, py0, pz0, 0);
+        if (ch0) {
+            lds_ch0_to_out32(a, reinterpret_cast<const float*>(ldo), px0, py0, pz0);
+        } else {
+            lds_to_out<NB>(a, ldo, px0, py0, pz0, 0);
+            if (a.pool) lds_pool_copy<NB>(a, ldp, px0, py0, pz0, 0);
+        }
     }
 }
 
