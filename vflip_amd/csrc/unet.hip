@@ -256,8 +256,12 @@ bool launch_conv(ConvArgs a, int NBall, hipStream_t s, float* ws) {
     a.ksplit = ks;
     a.part = ks > 1 ? ws : nullptr;
     const int NB = NBall / g;
+    // (a split layer pools in its reduction when that still has 256 workgroups of
+    // 8-voxel sums; on smaller grids the separate passes are faster)
     const bool pool_req = a.pool && a.out && !a.out32;
-    const bool pooled = pool_req && (ks > 1 ? (a.nx | a.ny | a.nz) % 2 == 0 : conv_tile_in_lds(CK, NB, false));
+    const size_t coarse = (size_t)(a.nx / 2) * (a.ny / 2) * (a.nz / 2) * a.cout;
+    const bool pooled = pool_req && (ks > 1 ? (a.nx | a.ny | a.nz) % 2 == 0 && coarse >= 256 * 256
+                                            : conv_tile_in_lds(CK, NB, false));
     if (!pooled) a.pool = nullptr;
     if (CK == 16 && ks == 1 && g == 1 && NB == 1) {
         const bool bf16_out = a.out && !a.out32;
