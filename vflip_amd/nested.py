@@ -23,6 +23,7 @@ residual of 1e-9 relative (S/sub_domain.cpp:739): FP64 defect correction rounds
 around FP32 MGPCG solves of the corrections (vflip_batch_solve_refined, or
 vflip_solver_solve_refined per subdomain), each correction to `inner_tol`.
 """
+import gc
 import time
 
 import numpy as np
@@ -198,6 +199,11 @@ def run(steps=20, n=256, count=12, seed=0, warmup=2, **kw):
         sim = NestedPlume(n=n, count=count, seed=seed, batched=(mode == "batched"), **kw)
         for _ in range(warmup):
             sim.step()
+        # a full pass of Python's garbage collector over the simulation's objects
+        # takes about 10 ms; freezing what exists keeps it out of the timed steps
+        # (it had landed in the batched runs' post phase)
+        gc.collect()
+        gc.freeze()
         sim.timing.clear()
         for _ in range(steps):
             sim.step()
@@ -217,5 +223,6 @@ def run(steps=20, n=256, count=12, seed=0, warmup=2, **kw):
         out["subdomains"] = [{"box_origin": c0, "coarse_cells": m, "fine_cells": nf} for c0, m, nf in
                              layout(n, count, seed)]
         del sim
+        gc.unfreeze()
         torch.cuda.empty_cache()
     return out
