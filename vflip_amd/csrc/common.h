@@ -82,25 +82,25 @@ __device__ inline void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// Block reductions for kBlock threads in any (x, y) shape; result valid in the
-// first thread.
+// Block reductions for NT threads (kBlock by default) in any (x, y) shape;
+// result valid in the first thread.
 __device__ inline int flat_tid() { return threadIdx.x + threadIdx.y * blockDim.x; }
 
-template <typename T> __device__ inline T block_sum(T v) {
-    __shared__ T red[kBlock / 64];
+template <typename T, int NT = kBlock> __device__ inline T block_sum(T v) {
+    __shared__ T red[NT / 64];
     v = wave_sum(v);
     const int t = flat_tid(), lane = t & 63, wid = t >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
     T r = 0;
     if (t == 0) {
-        for (int q = 0; q < kBlock / 64; ++q) r += red[q];
+        for (int q = 0; q < NT / 64; ++q) r += red[q];
     }
     __syncthreads();
     return r;
 }
-template <typename T> __device__ inline T block_max(T v) {
-    __shared__ T red[kBlock / 64];
+template <typename T, int NT = kBlock> __device__ inline T block_max(T v) {
+    __shared__ T red[NT / 64];
     v = wave_max(v);
     const int t = flat_tid(), lane = t & 63, wid = t >> 6;
     if (lane == 0) red[wid] = v;
@@ -108,7 +108,7 @@ template <typename T> __device__ inline T block_max(T v) {
     T r = 0;
     if (t == 0) {
         r = red[0];
-        for (int q = 1; q < kBlock / 64; ++q) r = r > red[q] ? r : red[q];
+        for (int q = 1; q < NT / 64; ++q) r = r > red[q] ? r : red[q];
     }
     __syncthreads();
     return r;

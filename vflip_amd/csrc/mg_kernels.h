@@ -1196,9 +1196,15 @@ __global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __
 // d x, -z, -y, -x, +x, +y, +z. epart (optional): per-block sums of x (b + r)
 // for the energy of x: E(x) = x'Ax/2 - b'x = -(x.b + x.r)/2 with r = b - A x.
 constexpr int kTrueResKZ = 16;
+// Block of 64 x kTrueResTY cells: the rows just outside it, read for the y
+// neighbours, are a fraction 2 / kTrueResTY of the x traffic.
+constexpr int kTrueResTY = 8;
+inline dim3 true_res_grid(int ni, int nj, int nk) {
+    return dim3((ni + 63) / 64, (nj + kTrueResTY - 1) / kTrueResTY, (nk + kTrueResKZ - 1) / kTrueResKZ);
+}
 // With gram given it also accumulates the pressure history's new Gram row for x:
 // partial sums of x.Ax, p1.Ax and p2.Ax (p1 / p2 may be null) at gram[q * nblocks + block].
-static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op, const double* __restrict__ d64,
+static __global__ __launch_bounds__(64 * kTrueResTY) void k_true_residual64(Op<float> op, const double* __restrict__ d64,
                                                             const double* __restrict__ b64,
                                                             const float* __restrict__ b32,
                                                             const double* __restrict__ x, float* __restrict__ r,
@@ -1208,7 +1214,7 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
                                                             const double* __restrict__ p2 = nullptr,
                                                             double* __restrict__ epart = nullptr) {
     double g0 = 0, g1 = 0, g2 = 0, e = 0;
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * kTrueResTY + threadIdx.y;
     const int kb = blockIdx.z * kTrueResKZ;
     const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
     double m = 0;
@@ -1217,17 +1223,38 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
         int c = (kb * nj + j) * ni + i;
         double xm = kb > 0 ? x[c - sl] : 0.0, x0 = x[c];
         float azm = kb > 0 ? op.AZ(c - sl) : 0.f;
+        // the fields of plane k are loaded while plane k - 1 computes (zero where a
+        // neighbour is outside the grid, whose term is skipped)
+        struct Plane {
+            double d, b, xym, xxm, xxp, xyp;
+            float aym, axm, axp, ayp;
+        };
+        auto load = [&](int cc, Plane& q) {
+            q.d = d64 ? d64[cc] : (double)op.D(cc);
+            q.b = b64 ? b64[cc] : (double)b32[cc];
+            q.aym = j > 0 ? op.AY(cc - ni) : 0.f;
+            q.xym = j > 0 ? x[cc - ni] : 0.0;
+            q.axm = i > 0 ? op.AX(cc - 1) : 0.f;
+            q.xxm = i > 0 ? x[cc - 1] : 0.0;
+            q.axp = i + 1 < ni ? op.AX(cc) : 0.f;
+            q.xxp = i + 1 < ni ? x[cc + 1] : 0.0;
+            q.ayp = j + 1 < nj ? op.AY(cc) : 0.f;
+            q.xyp = j + 1 < nj ? x[cc + ni] : 0.0;
+        };
+        Plane cur, nxt;
+        load(c, cur);
         for (int k = kb; k < ke; ++k, c += sl) {
             const double xp = k + 1 < nk ? x[c + sl] : 0.0;
             const float az0 = k + 1 < nk ? op.AZ(c) : 0.f;
-            double y = (d64 ? d64[c] : (double)op.D(c)) * x0;
+            if (k + 1 < ke) load(c + sl, nxt);
+            double y = cur.d * x0;
             if (k > 0) y += (double)azm * xm;
-            if (j > 0) y += (double)op.AY(c - ni) * x[c - ni];
-            if (i > 0) y += (double)op.AX(c - 1) * x[c - 1];
-            if (i + 1 < ni) y += (double)op.AX(c) * x[c + 1];
-            if (j + 1 < nj) y += (double)op.AY(c) * x[c + ni];
+            if (j > 0) y += (double)cur.aym * cur.xym;
+            if (i > 0) y += (double)cur.axm * cur.xxm;
+            if (i + 1 < ni) y += (double)cur.axp * cur.xxp;
+            if (j + 1 < nj) y += (double)cur.ayp * cur.xyp;
             if (k + 1 < nk) y += (double)az0 * xp;
-            const double bv = b64 ? b64[c] : (double)b32[c];
+            const double bv = cur.b;
             const double rv = bv - y;
             if (r) r[c] = (float)rv;
             // energy of x: x (b + r) with r as stored in FP32
@@ -1241,18 +1268,20 @@ static __global__ __launch_bounds__(kBlock) void k_true_residual64(Op<float> op,
             xm = x0;
             x0 = xp;
             azm = az0;
+            cur = nxt;
         }
     }
-    double t = block_max(m);
+    double t = block_max<double, 64 * kTrueResTY>(m);
     const unsigned blk = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
     if (threadIdx.x == 0 && threadIdx.y == 0) partial[blk] = t;
     if (epart) {
-        const double se = block_sum(e);
+        const double se = block_sum<double, 64 * kTrueResTY>(e);
         if (threadIdx.x == 0 && threadIdx.y == 0) epart[blk] = se;
     }
     if (gram) {
         const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-        const double s0 = block_sum(g0), s1 = block_sum(g1), s2 = block_sum(g2);
+        const double s0 = block_sum<double, 64 * kTrueResTY>(g0), s1 = block_sum<double, 64 * kTrueResTY>(g1),
+                     s2 = block_sum<double, 64 * kTrueResTY>(g2);
         if (threadIdx.x == 0 && threadIdx.y == 0) {
             gram[blk] = s0;
             gram[nb + blk] = s1;

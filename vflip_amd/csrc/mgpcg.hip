@@ -615,7 +615,8 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
         const LevelBuf& l0 = S->L[0];
         int np = std::max(kVecBlocks, nblocks(grid3z(l0.ni, l0.nj, l0.nk)));
         // the true residual's max|r| and energy partials side by side
-        np = std::max(np, 2 * nblocks(dim3((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ)));
+        np = std::max(np, 2 * nblocks(true_res_grid(l0.ni, l0.nj, l0.nk)));
+        np = std::max(np, nblocks(dim3((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ)));
         np = std::max(np, zgeom(l0, false, S->cus).blocks);
         np = std::max(np, zgeom(l0, true, S->cus).blocks);
         S->np_partial = np;
@@ -862,8 +863,8 @@ int ensure_f64(vflip_solver* S) {
 void launch_true_residual64(vflip_solver* S, const double* b64, const float* b32, const double* x, float* r,
                             bool gram = false, bool energy = false) {
     const LevelBuf& l0 = S->L[0];
-    const dim3 g((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
-    k_true_residual64<<<g, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr, b64, b32,
+    const dim3 g = true_res_grid(l0.ni, l0.nj, l0.nk);
+    k_true_residual64<<<g, dim3(64, kTrueResTY), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr, b64, b32,
                                                          x, r, S->partial, gram ? S->hist_part : nullptr,
                                                          gram ? S->gram_p[0] : nullptr, gram ? S->gram_p[1] : nullptr,
                                                          energy ? S->partial + nblocks(g) : nullptr);
@@ -1119,8 +1120,8 @@ int solver_refine_begin(vflip_solver* S, hipStream_t s, bool assembled) {
 // else rhs32; the assembled FP64 diagonal when valid) and its max |.| into sc->true_res.
 int solver_refine_residual(vflip_solver* S, hipStream_t s, const float* rhs32, bool assembled) {
     const LevelBuf& l0 = S->L[0];
-    const dim3 g((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ);
-    k_true_residual64<<<g, dim3(64, 4), 0, s>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr,
+    const dim3 g = true_res_grid(l0.ni, l0.nj, l0.nk);
+    k_true_residual64<<<g, dim3(64, kTrueResTY), 0, s>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr,
                                                  assembled ? S->b64 : nullptr, rhs32, S->ref_x64, S->ref_r, S->partial,
                                                  nullptr, nullptr, nullptr);
     S->np_last = nblocks(g);
@@ -1372,7 +1373,7 @@ int solver_history_guess(vflip_solver* S, const float* rhs32, double** guess, in
     if (rc) return rc;
     if (!S->hist_alloc) {
         const LevelBuf& l0 = S->L[0];
-        const int nb3 = ((l0.ni + 63) / 64) * ((l0.nj + 3) / 4) * ((l0.nk + kTrueResKZ - 1) / kTrueResKZ);
+        const int nb3 = nblocks(true_res_grid(l0.ni, l0.nj, l0.nk));
         S->hist_part_n = std::max(S->vblocks, nb3);
         const size_t words = 3 * S->n + 3 * (size_t)S->hist_part_n + 9 + 4;
         rc = alloc((void**)&S->hist_alloc, words * sizeof(double));
