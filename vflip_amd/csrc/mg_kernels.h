@@ -1290,6 +1290,179 @@ static __global__ __launch_bounds__(64 * kTrueResTY) void k_true_residual64(Op<f
     }
 }
 
+// Row-wave form of k_true_residual64 for levels whose rows are 64*V cells (V = 2
+// or 4), organised like k_stencil_row: a wave owns one x row (lane l: cells
+// V*l .. V*l+V-1), a 512-thread workgroup kTrRows consecutive rows, marching up a
+// z chunk with the z-neighbours in registers, the y-neighbours through LDS
+// (double-buffered, one barrier per plane; the first and last wave also load the
+// row just outside the tile) and the x-neighbours by lane shuffles; the next
+// plane's fields are loaded while the current one computes. Each cell sums the
+// same terms in the same order as k_true_residual64 (r, max|r| exactly as it);
+// the per-workgroup energy and Gram partial sums cover other cell sets.
+constexpr int kTrRows = 8;
+template <int V, bool GRAM>  // GRAM: the Gram partials (gram, p1, p2) are accumulated
+__global__ __launch_bounds__(64 * kTrRows) void k_true_residual_row(
+    Op<float> op, const double* __restrict__ d64, const double* __restrict__ b64, const float* __restrict__ b32,
+    const double* __restrict__ x, float* __restrict__ r, double* __restrict__ partial, int tiles_y, int zchunk,
+    double* __restrict__ gram, const double* __restrict__ p1, const double* __restrict__ p2,
+    double* __restrict__ epart) {
+    __shared__ __align__(16) double xs[2][kTrRows][64 * V];  // x of plane k, per row
+    __shared__ __align__(16) float ays[2][kTrRows][64 * V];  // ay of plane k, per row
+    const int ni = op.ni, nj = op.nj, nk = op.nk;
+    const int nb = gridDim.x, L = blockIdx.x, xcd = L % 8;
+    const int lg = xcd * (nb / 8) + min(xcd, nb % 8) + L / 8;
+    const int ty = lg % tiles_y, zch = lg / tiles_y;
+    const int j0 = ty * kTrRows, k0 = zch * zchunk, k1 = min(k0 + zchunk, nk);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int j = j0 + w;
+    const bool row_ok = j < nj;
+    const bool ym = j > 0, yp = j + 1 < nj;
+    const int jc = min(j, nj - 1);
+    const bool ex_lo = w == 0, ex_hi = w == kTrRows - 1;
+    const int jx = ex_lo ? max(j - 1, 0) : min(j + 1, nj - 1);
+    const int i0 = lane * V;
+    const int lprev = (lane + 63) & 63, lnext = (lane + 1) & 63;
+    auto at = [&](int jj, int k) { return (min(max(k, 0), nk - 1) * nj + jj) * ni + i0; };
+    auto vldd = [&](const double* p, double (&v)[V]) {
+#pragma unroll
+        for (int c = 0; c < V; c += 2) {
+            const double2 t = *reinterpret_cast<const double2*>(p + c);
+            v[c] = t.x;
+            v[c + 1] = t.y;
+        }
+    };
+    // fields of a plane: diagonal and right-hand side widened to FP64
+    auto load_db = [&](int o, double (&d)[V], double (&bb)[V]) {
+        if (d64) {
+            vldd(d64 + o, d);
+        } else {
+            float t[V];
+            vld<V>(op.d + o, t);
+#pragma unroll
+            for (int c = 0; c < V; ++c) d[c] = (double)t[c];
+        }
+        if (b64) {
+            vldd(b64 + o, bb);
+        } else {
+            float t[V];
+            vld<V>(b32 + o, t);
+#pragma unroll
+            for (int c = 0; c < V; ++c) bb[c] = (double)t[c];
+        }
+    };
+    double xm[V], x0[V], xp[V];        // x of row j at planes k-1, k, k+1
+    double cd[V], cb[V], q1[V], q2[V];  // plane k: diagonal, rhs, history directions
+    float cx[V], cy[V], cz[V], zprev[V];
+    double exs[V];                      // row outside the tile: x at plane k
+    float exy[V];                       // and (first wave) its ay
+    double m = 0, e = 0, g0 = 0, g1 = 0, g2 = 0;
+    {
+        vldd(x + at(jc, k0 - 1), xm);
+        vldd(x + at(jc, k0), x0);
+        vldd(x + at(jc, k0 + 1), xp);
+        const int o = at(jc, k0);
+        load_db(o, cd, cb);
+        vld<V>(op.ax + o, cx);
+        vld<V>(op.ay + o, cy);
+        vld<V>(op.az + o, cz);
+        vld<V>(op.az + at(jc, k0 - 1), zprev);
+        if (GRAM && p1) vldd(p1 + o, q1);
+        if (GRAM && p2) vldd(p2 + o, q2);
+        if (ex_lo || ex_hi) vldd(x + at(jx, k0), exs);
+        if (ex_lo) vld<V>(op.ay + at(jx, k0), exy);
+    }
+    for (int k = k0; k < k1; ++k) {
+        const int buf = k & 1;
+        double nd[V], nbv[V], nq1[V], nq2[V], nxp[V], nexs[V];
+        float nx[V], ny[V], nz[V], nexy[V];
+        if (k + 1 < k1) {
+            const int o = at(jc, k + 1);
+            load_db(o, nd, nbv);
+            vld<V>(op.ax + o, nx);
+            vld<V>(op.ay + o, ny);
+            vld<V>(op.az + o, nz);
+            if (GRAM && p1) vldd(p1 + o, nq1);
+            if (GRAM && p2) vldd(p2 + o, nq2);
+            vldd(x + at(jc, k + 2), nxp);
+            if (ex_lo || ex_hi) vldd(x + at(jx, k + 1), nexs);
+            if (ex_lo) vld<V>(op.ay + at(jx, k + 1), nexy);
+        }
+#pragma unroll
+        for (int c = 0; c < V; ++c) {
+            xs[buf][w][lane * V + c] = x0[c];
+            ays[buf][w][lane * V + c] = cy[c];
+        }
+        lds_barrier();
+        if (row_ok) {
+            double xdn[V], xup[V];
+            float ayd[V];
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                xdn[c] = w > 0 ? xs[buf][w - 1][lane * V + c] : exs[c];
+                ayd[c] = w > 0 ? ays[buf][w - 1][lane * V + c] : exy[c];
+                xup[c] = w < kTrRows - 1 ? xs[buf][w + 1][lane * V + c] : exs[c];
+            }
+            const double xl_e = __shfl(x0[V - 1], lprev, 64);
+            const double xr_e = __shfl(x0[0], lnext, 64);
+            const float axl_e = __shfl(cx[V - 1], lprev, 64);
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                const int i = i0 + c;
+                double y = cd[c] * x0[c];
+                if (k > 0) y += (double)zprev[c] * xm[c];
+                if (ym) y += (double)ayd[c] * xdn[c];
+                if (i > 0) y += (double)(c > 0 ? cx[c - 1] : axl_e) * (c > 0 ? x0[c - 1] : xl_e);
+                if (i + 1 < ni) y += (double)cx[c] * (c + 1 < V ? x0[c + 1] : xr_e);
+                if (yp) y += (double)cy[c] * xup[c];
+                if (k + 1 < nk) y += (double)cz[c] * xp[c];
+                const double bv = cb[c];
+                const double rv = bv - y;
+                if (r) r[(k * nj + j) * ni + i] = (float)rv;
+                if (epart) e += x0[c] * (bv + (double)(float)rv);
+                if (GRAM && gram) {
+                    g0 += x0[c] * y;
+                    if (p1) g1 += q1[c] * y;
+                    if (p2) g2 += q2[c] * y;
+                }
+                m = fmax(m, fabs(rv));
+            }
+        }
+        if (k + 1 < k1) {
+#pragma unroll
+            for (int c = 0; c < V; ++c) {
+                zprev[c] = cz[c];
+                xm[c] = x0[c];
+                x0[c] = xp[c];
+                xp[c] = nxp[c];
+                cd[c] = nd[c];
+                cb[c] = nbv[c];
+                cx[c] = nx[c];
+                cy[c] = ny[c];
+                cz[c] = nz[c];
+                if (GRAM && p1) q1[c] = nq1[c];
+                if (GRAM && p2) q2[c] = nq2[c];
+                exs[c] = nexs[c];
+                exy[c] = nexy[c];
+            }
+        }
+    }
+    const double t = block_max<double, 64 * kTrRows>(m);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    if (epart) {
+        const double se = block_sum<double, 64 * kTrRows>(e);
+        if (threadIdx.x == 0) epart[blockIdx.x] = se;
+    }
+    if (GRAM && gram) {
+        const double s0 = block_sum<double, 64 * kTrRows>(g0), s1 = block_sum<double, 64 * kTrRows>(g1),
+                     s2 = block_sum<double, 64 * kTrRows>(g2);
+        if (threadIdx.x == 0) {
+            gram[blockIdx.x] = s0;
+            gram[nb + blockIdx.x] = s1;
+            gram[2 * nb + blockIdx.x] = s2;
+        }
+    }
+}
+
 // Residual r = b - A x of an FP64 guess (the terms and FP64 arithmetic of
 // k_true_residual64, rounded to FP32 per cell), restricted by 4 per axis for the
 // network's input without storing r: crhs = mean of r over each 4^3 block, csolid =

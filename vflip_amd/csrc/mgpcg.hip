@@ -857,19 +857,44 @@ int ensure_f64(vflip_solver* S) {
     return 0;
 }
 
+// The FP64 true residual r = b - A x on level 0 (k_true_residual_row for rows of
+// 128 or 256 cells, else k_true_residual64): max|r| partials in part, and
+// optionally the energy partials after them and the Gram partials in gram.
+// Returns the number of partials (workgroups).
+int true_residual_pass(vflip_solver* S, hipStream_t st, const double* b64, const float* b32, const double* x, float* r,
+                       double* part, double* gram, const double* p1, const double* p2, bool energy) {
+    const LevelBuf& l0 = S->L[0];
+    const double* d64 = S->d64_valid ? S->d64 : nullptr;
+    if (l0.ni == 256 || l0.ni == 128) {
+        constexpr int kZChunk = 16;
+        const int tiles_y = (l0.nj + kTrRows - 1) / kTrRows, nzc = (l0.nk + kZChunk - 1) / kZChunk;
+        const int nb = tiles_y * nzc;
+        double* ep = energy ? part + nb : nullptr;
+        const Op<float> op = make_op<float>(l0);
+        if (l0.ni == 256) {
+            if (gram) k_true_residual_row<4, true><<<nb, 64 * kTrRows, 0, st>>>(op, d64, b64, b32, x, r, part, tiles_y, kZChunk, gram, p1, p2, ep);
+            else k_true_residual_row<4, false><<<nb, 64 * kTrRows, 0, st>>>(op, d64, b64, b32, x, r, part, tiles_y, kZChunk, nullptr, nullptr, nullptr, ep);
+        } else {
+            if (gram) k_true_residual_row<2, true><<<nb, 64 * kTrRows, 0, st>>>(op, d64, b64, b32, x, r, part, tiles_y, kZChunk, gram, p1, p2, ep);
+            else k_true_residual_row<2, false><<<nb, 64 * kTrRows, 0, st>>>(op, d64, b64, b32, x, r, part, tiles_y, kZChunk, nullptr, nullptr, nullptr, ep);
+        }
+        return nb;
+    }
+    const dim3 g = true_res_grid(l0.ni, l0.nj, l0.nk);
+    k_true_residual64<<<g, dim3(64, kTrueResTY), 0, st>>>(make_op<float>(l0), d64, b64, b32, x, r, part, gram, p1, p2,
+                                                        energy ? part + nblocks(g) : nullptr);
+    return nblocks(g);
+}
+
 // gram: also accumulate the history Gram row of x (vflip_project_history solves)
 // energy: also the energy partials of x, in S->partial after the max|r| partials
 // (finalize_energy reads them).
 void launch_true_residual64(vflip_solver* S, const double* b64, const float* b32, const double* x, float* r,
                             bool gram = false, bool energy = false) {
-    const LevelBuf& l0 = S->L[0];
-    const dim3 g = true_res_grid(l0.ni, l0.nj, l0.nk);
-    k_true_residual64<<<g, dim3(64, kTrueResTY), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr, b64, b32,
-                                                         x, r, S->partial, gram ? S->hist_part : nullptr,
-                                                         gram ? S->gram_p[0] : nullptr, gram ? S->gram_p[1] : nullptr,
-                                                         energy ? S->partial + nblocks(g) : nullptr);
-    if (gram) S->gram_nb = nblocks(g);
-    S->np_last = nblocks(g);
+    const int nb = true_residual_pass(S, S->stream, b64, b32, x, r, S->partial, gram ? S->hist_part : nullptr,
+                                      gram ? S->gram_p[0] : nullptr, gram ? S->gram_p[1] : nullptr, energy);
+    if (gram) S->gram_nb = nb;
+    S->np_last = nb;
 }
 
 // PCG driver for both precisions (S/AlgebraicMultigrid.h:189-301).
@@ -1119,12 +1144,8 @@ int solver_refine_begin(vflip_solver* S, hipStream_t s, bool assembled) {
 // ref_r = FP32(b - A x) in FP64 (b = the assembled FP64 right-hand side when `assembled`,
 // else rhs32; the assembled FP64 diagonal when valid) and its max |.| into sc->true_res.
 int solver_refine_residual(vflip_solver* S, hipStream_t s, const float* rhs32, bool assembled) {
-    const LevelBuf& l0 = S->L[0];
-    const dim3 g = true_res_grid(l0.ni, l0.nj, l0.nk);
-    k_true_residual64<<<g, dim3(64, kTrueResTY), 0, s>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr,
-                                                 assembled ? S->b64 : nullptr, rhs32, S->ref_x64, S->ref_r, S->partial,
-                                                 nullptr, nullptr, nullptr);
-    S->np_last = nblocks(g);
+    S->np_last = true_residual_pass(S, s, assembled ? S->b64 : nullptr, rhs32, S->ref_x64, S->ref_r, S->partial,
+                                    nullptr, nullptr, nullptr, false);
     k_finalize<double><<<1, kBlock, 0, s>>>(S->sc, S->partial, S->np_last, 5, 0.0);
     VF_CHECK_LAUNCH();
     return 0;
