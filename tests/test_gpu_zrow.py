@@ -13,7 +13,8 @@ against the general kernels and the oracle.
   are equal.
 
 Shapes cover full tiles and ragged ones (nj, nk not multiples of the tile rows
-or the z chunk, odd coarse sizes); the fused sweeps are enabled down to 4096
+or the z chunk, odd coarse sizes), tiles of unequal row and chunk counts, and
+z chunks marched in both directions; the fused sweeps are enabled down to 4096
 cells so that coarse levels with 128-cell rows take the row-wave path too.
 """
 import contextlib
@@ -73,7 +74,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-@pytest.mark.parametrize("shape", [(128, 37, 19), (256, 40, 20), (256, 64, 48), (128, 128, 64)])
+@pytest.mark.parametrize("shape", [(128, 37, 19), (256, 40, 20), (256, 64, 48), (128, 128, 64), (256, 256, 96)])
 def test_row_wave_vcycle_bitexact(cuda, shape):
     print(f"row-wave vcycle {shape}", flush=True)
     from oracle import oracle as O

@@ -1024,8 +1024,9 @@ __global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* _
 }
 
 // Row-wave 7-point stencil for levels whose rows are 64*V cells (V = 2 or 4),
-// organised like k_rbgs_zrow: a wave owns one x row, a 1024-thread workgroup
-// kZRows consecutive rows, marching up a z chunk with the z-neighbours in
+// organised like k_rbgs_zrow: a wave owns one x row, a 512-thread workgroup
+// kSRows consecutive rows (two workgroups per compute unit: one marches while the
+// other waits at its barrier), marching up a z chunk with the z-neighbours in
 // registers, the y-neighbours through LDS (double-buffered, one barrier per
 // plane; the first and last wave also load the row just outside the tile) and
 // the x-neighbours by lane shuffles.
@@ -1035,27 +1036,28 @@ __global__ __launch_bounds__(kBlock) void k_matvec_dot(Op<CT> op, const float* _
 // restriction sums each 2x2x2 aggregate as they do: the x pair inside the lane,
 // the y pair (rows 2J, 2J+1 = waves w, w+1) through LDS one plane later, the z
 // pair in registers; z chunks start on even planes.
+constexpr int kSRows = 8;  // rows per workgroup of k_stencil_row (even: restriction row pairs)
 template <int V, int KIND, typename Acc, typename CT = float>
-__global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __restrict__ b,
+__global__ __launch_bounds__(64 * kSRows) void k_stencil_row(Op<CT> op, const float* __restrict__ b,
                                                       const float* __restrict__ s, float* __restrict__ out, int ci,
                                                       int cj, int tiles_y, int zchunk, double* __restrict__ partial) {
     constexpr int H = V / 2;
-    __shared__ __align__(16) float ss[2][kZRows][64 * V];   // s of plane k, per row
-    __shared__ __align__(16) float ays[2][kZRows][64 * V];  // ay of plane k, per row
-    __shared__ __align__(16) float ps[2][kZRows][64 * H];   // restriction: x-pair sums of plane k
-    __shared__ Acc part[kZRows];
+    __shared__ __align__(16) float ss[2][kSRows][64 * V];   // s of plane k, per row
+    __shared__ __align__(16) float ays[2][kSRows][64 * V];  // ay of plane k, per row
+    __shared__ __align__(16) float ps[2][kSRows][64 * H];   // restriction: x-pair sums of plane k
+    __shared__ Acc part[kSRows];
     const int ni = op.ni, nj = op.nj, nk = op.nk;
     const float sc = op.scale;
     const int nb = gridDim.x, L = blockIdx.x, xcd = L % 8;
     const int lg = xcd * (nb / 8) + min(xcd, nb % 8) + L / 8;
     const int ty = lg % tiles_y, zch = lg / tiles_y;
-    const int j0 = ty * kZRows, k0 = zch * zchunk, k1 = min(k0 + zchunk, nk);
+    const int j0 = ty * kSRows, k0 = zch * zchunk, k1 = min(k0 + zchunk, nk);
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int j = j0 + w;
     const bool row_ok = j < nj;
     const bool ym = j > 0, yp = j + 1 < nj;
     const int jc = min(j, nj - 1);
-    const bool ex_lo = w == 0, ex_hi = w == kZRows - 1;
+    const bool ex_lo = w == 0, ex_hi = w == kSRows - 1;
     const int jx = ex_lo ? max(j - 1, 0) : min(j + 1, nj - 1);
     const int i0 = lane * V;
     const int lprev = (lane + 63) & 63, lnext = (lane + 1) & 63;
@@ -1123,7 +1125,7 @@ __global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __
 #pragma unroll
                     for (int c = 0; c < V; ++c) { sdn[c] = exs[c]; ayd[c] = exy[c]; }
                 }
-                if (w < kZRows - 1) vld<V>(&ss[buf][w + 1][lane * V], sup);
+                if (w < kSRows - 1) vld<V>(&ss[buf][w + 1][lane * V], sup);
                 else {
 #pragma unroll
                     for (int c = 0; c < V; ++c) sup[c] = exs[c];
@@ -1178,7 +1180,7 @@ __global__ __launch_bounds__(1024) void k_stencil_row(Op<CT> op, const float* __
             __syncthreads();
             if (threadIdx.x == 0) {
                 Acc r = 0;
-                for (int q = 0; q < kZRows; ++q) r += part[q];
+                for (int q = 0; q < kSRows; ++q) r += part[q];
                 partial[blockIdx.x] = (double)r;
             }
         }

@@ -59,9 +59,10 @@ dim3 grid3z(int ni, int nj, int nk) { return dim3((ni + 63) / 64, (nj + 3) / 4, 
 int nblocks(dim3 g) { return (int)(g.x * g.y * g.z); }
 
 
-// Row-wave stencil geometry (matvec, residual restriction): y tiles of kZRows
+// Row-wave stencil geometry (matvec, residual restriction): y tiles of kSRows
 // rows, z chunks (even, for the restriction) sized so the grid is one wave of
-// workgroups. v == 0: the level or a pointer does not fit the row kernel.
+// workgroups, two per compute unit. v == 0: the level or a pointer does not fit
+// the row kernel.
 struct RowGeom {
     int v, tiles_y, zchunk, blocks;
 };
@@ -72,8 +73,8 @@ RowGeom rowgeom(const vflip_solver* S, const LevelBuf& lv, std::initializer_list
     for (const void* p : ptrs)
         if (p && !aligned16(p)) return g;
     g.v = lv.ni / 64;
-    g.tiles_y = (lv.nj + kZRows - 1) / kZRows;
-    const int chunks = std::max(1, S->cus / g.tiles_y);
+    g.tiles_y = (lv.nj + kSRows - 1) / kSRows;
+    const int chunks = std::max(1, 2 * S->cus / g.tiles_y);
     int zc = (lv.nk + chunks - 1) / chunks;
     zc = std::max(2, zc + (zc & 1));
     g.zchunk = zc;
@@ -85,10 +86,10 @@ void launch_stencil_row(vflip_solver* S, const RowGeom& g, const LevelBuf& lv, c
                         float* out, int ci, int cj, double* partial) {
     const Op<CT> op = make_op<CT>(lv);
     if (g.v == 4)
-        k_stencil_row<4, KIND, Acc, CT><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y,
+        k_stencil_row<4, KIND, Acc, CT><<<g.blocks, 64 * kSRows, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y,
                                                                            g.zchunk, partial);
     else
-        k_stencil_row<2, KIND, Acc, CT><<<g.blocks, 1024, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y,
+        k_stencil_row<2, KIND, Acc, CT><<<g.blocks, 64 * kSRows, 0, S->stream>>>(op, b, s, out, ci, cj, g.tiles_y,
                                                                            g.zchunk, partial);
 }
 
@@ -619,6 +620,7 @@ int vflip_solver_create(int ni, int nj, int nk, const vflip_solve_opts* opts, vf
         np = std::max(np, nblocks(dim3((l0.ni + 63) / 64, (l0.nj + 3) / 4, (l0.nk + kTrueResKZ - 1) / kTrueResKZ)));
         np = std::max(np, zgeom(l0, false, S->cus).blocks);
         np = std::max(np, zgeom(l0, true, S->cus).blocks);
+        np = std::max(np, rowgeom(S, l0, {}).blocks);  // the row-wave matvec's s.q partials
         S->np_partial = np;
         rc = alloc((void**)&S->partial, (size_t)np * sizeof(double));
     }
