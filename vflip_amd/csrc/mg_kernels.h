@@ -493,14 +493,16 @@ __global__ __launch_bounds__(1024) void k_rbgs_zrow(Op<CT> op, const float* __re
         // next plane's fields, x two planes ahead, the ring rows' outer row
         float nb_[V], nd[V], nx[V], ny[V], nz[V], nxx[V], nex[V], ney[V];
         {
-            const int o = at(jc, p + 1);
+            // the last plane reloads planes already read (nothing beyond k1+1 is needed)
+            const int pn = p < k1 ? p + 1 : p, px = p < k1 ? p + 2 : p + 1;
+            const int o = at(jc, pn);
             vld<V>(b + o, nb_); vld<V>(op.d + o, nd); vld<V>(op.ax + o, nx); vld<V>(op.ay + o, ny);
             vld<V>(op.az + o, nz);
             if constexpr (MODE != 1) {
-                load_x(jc, p + 2, nxx);
-                if (has_ex) load_x(jx, p + 1, nex);
+                load_x(jc, px, nxx);
+                if (has_ex) load_x(jx, pn, nex);
             }
-            if (has_ex) vld<V>(op.ay + at(jx, p + 1), ney);
+            if (has_ex) vld<V>(op.ay + at(jx, pn), ney);
         }
         if constexpr (MODE != 1) vst<H>(&xs[buf][w][lane * H], xb0);
         vst<V>(&ays[buf][w][lane * V], cy);
