@@ -500,8 +500,12 @@ int unet_warm_predict_add(vflip_unet* N, int ni, int nj, int nk, int factor, con
     const float* sd = prepare_scale(N, N->w_rhs, Vc, 0.f);
     k_pack_input<<<grid256(Vc), 256, 0, N->stream>>>(N->w_rhs, N->w_solid, N->w_liquid, Vc, 0.f, sd, N->act[kIn]);
     forward_layers(N);
-    k_prolong_add64<<<dim3((ni + 255) / 256, nj, nk), 256, 0, N->stream>>>(N->pred, cx, cy, cz, factor, sd, N->gain,
-                                                                          diag, g);
+    if (ni % 4 == 0 && ((uintptr_t)g & 31) == 0 && ((uintptr_t)diag & 15) == 0)
+        k_prolong_add64_x4<<<dim3((ni / 4 + 255) / 256, nj, nk), 256, 0, N->stream>>>(N->pred, cx, cy, cz, factor, sd,
+                                                                                      N->gain, diag, g);
+    else
+        k_prolong_add64<<<dim3((ni + 255) / 256, nj, nk), 256, 0, N->stream>>>(N->pred, cx, cy, cz, factor, sd,
+                                                                              N->gain, diag, g);
     VF_CHECK_LAUNCH();
     return 0;
 }

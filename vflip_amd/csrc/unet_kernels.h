@@ -1168,6 +1168,47 @@ __global__ __launch_bounds__(256) void k_prolong_add64(const float* __restrict__
     g[v] += (double)((c0 + fz * (c1 - c0)) * m);
 }
 
+// k_prolong_add64 for rows of a multiple of 4 cells: a thread takes 4 consecutive
+// cells of a row (one 16-byte diag load, two 16-byte guess loads and stores) and
+// evaluates each cell's interpolation with the same expression as k_prolong_add64.
+__global__ __launch_bounds__(256) void k_prolong_add64_x4(const float* __restrict__ cp, int cx, int cy, int cz,
+                                                          int f, const float* __restrict__ s_dev, float gain,
+                                                          const float* __restrict__ diag, double* __restrict__ g) {
+    const int nx = cx * f, ny = cy * f;
+    const int i4 = (blockIdx.x * 256 + threadIdx.x) * 4, j = blockIdx.y, k = blockIdx.z;
+    if (i4 >= nx) return;
+    const size_t v = ((size_t)k * ny + j) * nx + i4;
+    const float m = s_dev ? s_dev[0] * gain : 1.0f;
+    const float invf = 1.0f / (float)f;
+    const float gy = fminf(fmaxf((j + 0.5f) * invf - 0.5f, 0.f), (float)(cy - 1));
+    const float gz = fminf(fmaxf((k + 0.5f) * invf - 0.5f, 0.f), (float)(cz - 1));
+    const int j0 = (int)gy, k0 = (int)gz;
+    const int j1 = min(j0 + 1, cy - 1), k1 = min(k0 + 1, cz - 1);
+    const float fy = gy - j0, fz = gz - k0;
+    auto at = [&](int a, int b, int e) { return cp[((size_t)e * cy + b) * cx + a]; };
+    float dv[4] = {1.f, 1.f, 1.f, 1.f};
+    if (diag) {
+        const float4 t = *reinterpret_cast<const float4*>(diag + v);
+        dv[0] = t.x; dv[1] = t.y; dv[2] = t.z; dv[3] = t.w;
+    }
+    const double2 ga = *reinterpret_cast<const double2*>(g + v), gb = *reinterpret_cast<const double2*>(g + v + 2);
+    double gv[4] = {ga.x, ga.y, gb.x, gb.y};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float gx = fminf(fmaxf((i4 + c + 0.5f) * invf - 0.5f, 0.f), (float)(cx - 1));
+        const int i0 = (int)gx, i1 = min(i0 + 1, cx - 1);
+        const float fx = gx - i0;
+        const float c00 = at(i0, j0, k0) + fx * (at(i1, j0, k0) - at(i0, j0, k0));
+        const float c10 = at(i0, j1, k0) + fx * (at(i1, j1, k0) - at(i0, j1, k0));
+        const float c01 = at(i0, j0, k1) + fx * (at(i1, j0, k1) - at(i0, j0, k1));
+        const float c11 = at(i0, j1, k1) + fx * (at(i1, j1, k1) - at(i0, j1, k1));
+        const float c0 = c00 + fy * (c10 - c00), c1 = c01 + fy * (c11 - c01);
+        if (dv[c] != 0.f) gv[c] += (double)((c0 + fz * (c1 - c0)) * m);
+    }
+    *reinterpret_cast<double2*>(g + v) = make_double2(gv[0], gv[1]);
+    *reinterpret_cast<double2*>(g + v + 2) = make_double2(gv[2], gv[3]);
+}
+
 __global__ __launch_bounds__(256) void k_to_bf16(const float* __restrict__ a, size_t n, bf16* __restrict__ b) {
     for (size_t e = blockIdx.x * (size_t)256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) b[e] = (bf16)a[e];
 }
