@@ -1470,25 +1470,34 @@ __global__ __launch_bounds__(64 * kTrRows) void k_true_residual_row(
 // Residual r = b - A x of an FP64 guess (the terms and FP64 arithmetic of
 // k_true_residual64, rounded to FP32 per cell), restricted by 4 per axis for the
 // network's input without storing r: crhs = mean of r over each 4^3 block, csolid =
-// the fraction of its cells with diagonal 0, cliquid = 1 - csolid. Block 64 x 4: a
-// thread per fine cell of 4 rows (one coarse row), marching kRR4Planes planes with
-// x and the z coupling in registers, as k_true_residual64. Sum order: the 4 planes
-// of a coarse plane per thread, then the 4 x-lanes of a coarse cell in pairs
-// (shuffles), then the 4 rows in pairs (LDS). Needs ni, nj, nk divisible by 4.
+// the fraction of its cells with diagonal 0, cliquid = 1 - csolid. Block 256 x 4: a
+// thread per fine cell of 4 whole rows of up to 256 cells (one coarse row, so no
+// x halo is read from a neighbouring block), marching kRR4Planes planes with x and
+// the z coupling in registers, as k_true_residual64; consecutive blocks of one XCD
+// take y-adjacent coarse rows, so the rows above and below a block come through
+// that XCD's L2. Sum order: the 4 planes of a coarse plane per thread, then the 4
+// x-lanes of a coarse cell in pairs (shuffles), then the 4 rows in pairs (LDS).
+// Needs ni, nj, nk divisible by 4.
 constexpr int kRR4Planes = 16;
-static __global__ __launch_bounds__(256) void k_residual_restrict4(Op<float> op, const double* __restrict__ d64,
+constexpr int kRR4X = 256;
+static __global__ __launch_bounds__(kRR4X * 4) void k_residual_restrict4(Op<float> op, const double* __restrict__ d64,
                                                                    const double* __restrict__ b64,
                                                                    const float* __restrict__ b32,
                                                                    const double* __restrict__ x,
                                                                    float* __restrict__ crhs,
                                                                    float* __restrict__ csolid,
                                                                    float* __restrict__ cliquid) {
-    __shared__ float sr[4][16], ss[4][16];
+    __shared__ float sr[4][kRR4X / 4], ss[4][kRR4X / 4];
     const int ni = op.ni, nj = op.nj, nk = op.nk, sl = ni * nj;
     const int cx = ni / 4, cy = nj / 4;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = blockIdx.x * 64 + tx, j = blockIdx.y * 4 + ty;
-    const int kb = blockIdx.z * kRR4Planes, ke = min(kb + kRR4Planes, nk);
+    // block L runs on XCD L % 8: consecutive logical blocks (x fastest, then the
+    // coarse row, then the z chunk) on one XCD
+    const int nbx = gridDim.x, nb = nbx * gridDim.y * gridDim.z, L = blockIdx.x + nbx * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int xcd = L % 8, lg = xcd * (nb / 8) + min(xcd, nb % 8) + L / 8;
+    const int bx = lg % nbx, by = (lg / nbx) % gridDim.y, bz = lg / (nbx * gridDim.y);
+    const int i = bx * kRR4X + tx, j = by * 4 + ty;
+    const int kb = bz * kRR4Planes, ke = min(kb + kRR4Planes, nk);
     const bool in = i < ni;
     int c = (kb * nj + j) * ni + i;
     double xm = (in && kb > 0) ? x[c - sl] : 0.0, x0 = in ? x[c] : 0.0;
@@ -1521,12 +1530,12 @@ static __global__ __launch_bounds__(256) void k_residual_restrict4(Op<float> op,
                 ss[ty][tx >> 2] = cnt;
             }
             __syncthreads();
-            const int ic = blockIdx.x * 16 + tx;
-            if (ty == 0 && tx < 16 && ic < cx) {
+            const int ic = bx * (kRR4X / 4) + tx;
+            if (ty == 0 && tx < kRR4X / 4 && ic < cx) {
                 const float inv = 1.0f / 64.0f;
                 const float r = ((sr[0][tx] + sr[1][tx]) + (sr[2][tx] + sr[3][tx])) * inv;
                 const float so = ((ss[0][tx] + ss[1][tx]) + (ss[2][tx] + ss[3][tx])) * inv;
-                const size_t cc = ((size_t)(k >> 2) * cy + blockIdx.y) * cx + ic;
+                const size_t cc = ((size_t)(k >> 2) * cy + by) * cx + ic;
                 crhs[cc] = r;
                 csolid[cc] = so;
                 cliquid[cc] = 1.f - so;

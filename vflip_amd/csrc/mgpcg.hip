@@ -1457,8 +1457,8 @@ int solver_guess_residual_restricted(vflip_solver* S, const float* rhs32, const 
         set_error("restricted guess residual: grid %dx%dx%d not divisible by 4", l0.ni, l0.nj, l0.nk);
         return VFLIP_ERR_ARG;
     }
-    const dim3 grid((l0.ni + 63) / 64, l0.nj / 4, (l0.nk + kRR4Planes - 1) / kRR4Planes);
-    k_residual_restrict4<<<grid, dim3(64, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr,
+    const dim3 grid((l0.ni + kRR4X - 1) / kRR4X, l0.nj / 4, (l0.nk + kRR4Planes - 1) / kRR4Planes);
+    k_residual_restrict4<<<grid, dim3(kRR4X, 4), 0, S->stream>>>(make_op<float>(l0), S->d64_valid ? S->d64 : nullptr,
                                                                (mixed(S) && S->b64_valid) ? S->b64 : nullptr, rhs32,
                                                                g, crhs, csolid, cliquid);
     VF_CHECK_LAUNCH();
