@@ -1660,6 +1660,47 @@ static __global__ __launch_bounds__(kBlock) void k_hist_rhs_dots(unsigned n, con
     }
 }
 
+// k_hist_rhs_dots two cells per thread and step (16-byte loads; arrays 16-byte
+// aligned, the FP32 right-hand side 8-byte aligned; n even).
+template <int M>
+static __global__ __launch_bounds__(kBlock) void k_hist_rhs_dots2(unsigned n, const double* __restrict__ b64,
+                                                                  const float* __restrict__ b32,
+                                                                  const double* __restrict__ p1,
+                                                                  const double* __restrict__ p2,
+                                                                  const double* __restrict__ p3,
+                                                                  double* __restrict__ partial) {
+    double f[3] = {0, 0, 0};
+    const unsigned n2 = n / 2;
+#pragma unroll 2
+    for (unsigned c = blockIdx.x * kBlock + threadIdx.x; c < n2; c += gridDim.x * kBlock) {
+        double2 bc;
+        if (b64) {
+            bc = reinterpret_cast<const double2*>(b64)[c];
+        } else {
+            const float2 t = reinterpret_cast<const float2*>(b32)[c];
+            bc = make_double2((double)t.x, (double)t.y);
+        }
+        const double2 a = reinterpret_cast<const double2*>(p1)[c];
+        f[0] += a.x * bc.x;
+        f[0] += a.y * bc.y;
+        if (M >= 2) {
+            const double2 q = reinterpret_cast<const double2*>(p2)[c];
+            f[1] += q.x * bc.x;
+            f[1] += q.y * bc.y;
+        }
+        if (M >= 3) {
+            const double2 q = reinterpret_cast<const double2*>(p3)[c];
+            f[2] += q.x * bc.x;
+            f[2] += q.y * bc.y;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+        const double t = block_sum(f[q]);
+        if (threadIdx.x == 0) partial[q * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 // One workgroup: f from its partials (fixed order), G over the slots (sa, sb, sc) of
 // p1, p2, p3 from the Gram cache; both taken to the difference basis v0 = p1,
 // v1 = p1 - p2, v2 = p1 - 2 p2 + p3 (exact FP64 combinations: the last solutions are
@@ -1667,10 +1708,14 @@ static __global__ __launch_bounds__(kBlock) void k_hist_rhs_dots(unsigned n, con
 // solved by symmetric elimination (a basis vector whose pivot falls below 1e-13 of
 // the first diagonal is dropped: coefficient 0); coef[0..2] = the weights of p1, p2,
 // p3 in x0 = c0 v0 + c1 v1 + c2 v2.
-static __global__ __launch_bounds__(kBlock) void k_hist_solve(const double* __restrict__ partial, int nb, int M,
+// M is a template parameter so that the small arrays below stay in registers (a
+// runtime M indexes them dynamically, i.e. through scratch memory).
+template <int M>
+static __global__ __launch_bounds__(kBlock) void k_hist_solve(const double* __restrict__ partial, int nb,
                                                               const double* __restrict__ Gc, int sa, int sb, int sc,
                                                               double* __restrict__ coef) {
     __shared__ double fs[3];
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
         double s = 0;
         if (q < M)
@@ -1682,18 +1727,25 @@ static __global__ __launch_bounds__(kBlock) void k_hist_solve(const double* __re
     if (threadIdx.x != 0) return;
     const int sl[3] = {sa, sb, sc};
     double Gp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+#pragma unroll
     for (int i = 0; i < M; ++i)
-        for (int j = 0; j < M; ++j) Gp[i][j] = Gc[3 * sl[i] + sl[j]];
+    #pragma unroll
+    for (int j = 0; j < M; ++j) Gp[i][j] = Gc[3 * sl[i] + sl[j]];
     // T: columns are v0, v1, v2 in the p basis
     const double T[3][3] = {{1, 1, 1}, {0, -1, -2}, {0, 0, 1}};
     double G[3][3], f[3];
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         f[k] = 0;
-        for (int i = 0; i < M; ++i) f[k] += T[i][k] * fs[i];
-        for (int l = 0; l < 3; ++l) {
+    #pragma unroll
+    for (int i = 0; i < M; ++i) f[k] += T[i][k] * fs[i];
+    #pragma unroll
+    for (int l = 0; l < 3; ++l) {
             double t = 0;
-            for (int i = 0; i < M; ++i)
-                for (int j = 0; j < M; ++j) t += T[i][k] * Gp[i][j] * T[j][l];
+        #pragma unroll
+    for (int i = 0; i < M; ++i)
+            #pragma unroll
+    for (int j = 0; j < M; ++j) t += T[i][k] * Gp[i][j] * T[j][l];
             G[k][l] = t;
         }
     }
@@ -1701,19 +1753,24 @@ static __global__ __launch_bounds__(kBlock) void k_hist_solve(const double* __re
     bool keep[3] = {false, false, false};
     const double ref = G[0][0];
     if (ref > 0) {
-        for (int k = 0; k < M; ++k) {
+    #pragma unroll
+    for (int k = 0; k < M; ++k) {
             if (!(G[k][k] > 1e-13 * ref)) continue;
             keep[k] = true;
-            for (int r = k + 1; r < M; ++r) {
+        #pragma unroll
+    for (int r = k + 1; r < M; ++r) {
                 const double l = G[r][k] / G[k][k];
-                for (int q = k; q < M; ++q) G[r][q] -= l * G[k][q];
+            #pragma unroll
+    for (int q = k; q < M; ++q) G[r][q] -= l * G[k][q];
                 f[r] -= l * f[k];
             }
         }
-        for (int k = M - 1; k >= 0; --k) {
+    #pragma unroll
+    for (int k = M - 1; k >= 0; --k) {
             if (!keep[k]) continue;
             double t = f[k];
-            for (int q = k + 1; q < M; ++q) t -= G[k][q] * c[q];
+        #pragma unroll
+    for (int q = k + 1; q < M; ++q) t -= G[k][q] * c[q];
             c[k] = t / G[k][k];
         }
     }

@@ -1425,10 +1425,22 @@ int solver_history_guess(vflip_solver* S, const float* rhs32, double** guess, in
         const double* b64 = (mixed(S) && S->b64_valid) ? S->b64 : nullptr;
         const double *p1 = S->hist[a], *p2 = M >= 2 ? S->hist[b] : nullptr, *p3 = M >= 3 ? S->hist[c] : nullptr;
         const int nb = S->vblocks;
-        if (M == 1) k_hist_rhs_dots<1><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
-        else if (M == 2) k_hist_rhs_dots<2><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
-        else k_hist_rhs_dots<3><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
-        k_hist_solve<<<1, kBlock, 0, S->stream>>>(S->hist_part, nb, M, S->hist_G, a, b, c, S->hist_coef);
+        const bool pairs = n % 2 == 0 && ((uintptr_t)rhs32 & 7) == 0 && ((uintptr_t)b64 & 15) == 0 &&
+                           ((uintptr_t)p1 & 15) == 0 && ((uintptr_t)p2 & 15) == 0 && ((uintptr_t)p3 & 15) == 0;
+        if (pairs) {
+            if (M == 1) k_hist_rhs_dots2<1><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+            else if (M == 2) k_hist_rhs_dots2<2><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+            else k_hist_rhs_dots2<3><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+        } else if (M == 1) {
+            k_hist_rhs_dots<1><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+        } else if (M == 2) {
+            k_hist_rhs_dots<2><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+        } else {
+            k_hist_rhs_dots<3><<<nb, kBlock, 0, S->stream>>>(n, b64, rhs32, p1, p2, p3, S->hist_part);
+        }
+        if (M == 1) k_hist_solve<1><<<1, kBlock, 0, S->stream>>>(S->hist_part, nb, S->hist_G, a, b, c, S->hist_coef);
+        else if (M == 2) k_hist_solve<2><<<1, kBlock, 0, S->stream>>>(S->hist_part, nb, S->hist_G, a, b, c, S->hist_coef);
+        else k_hist_solve<3><<<1, kBlock, 0, S->stream>>>(S->hist_part, nb, S->hist_G, a, b, c, S->hist_coef);
         k_hist_combine<<<S->vblocks, kBlock, 0, S->stream>>>(n, S->hist_coef, p1, p2, p3, g);
     }
     VF_CHECK_LAUNCH();
