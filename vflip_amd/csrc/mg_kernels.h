@@ -875,6 +875,7 @@ __device__ inline void coarse_apply_partial_body(const float* __restrict__ tab, 
     __syncthreads();
     if (r >= n) return;
     float acc = 0.f;
+#pragma unroll 16
     for (int q = 0; q < cn; ++q) acc += tab[(size_t)(c0 + q) * n + r] * bs[q];
     part[(size_t)B.y * n + r] = acc;
 }
@@ -887,7 +888,9 @@ __device__ inline void coarse_apply_reduce_body(const float* __restrict__ part, 
                                                 int nchunks, Blk B) {
     const int r = B.x * kBlock + threadIdx.x;
     if (r >= n) return;
+    // the loads of 16 chunks are issued together; the sum keeps the chunk order
     float s = 0.f;
+#pragma unroll 16
     for (int q = 0; q < nchunks; ++q) s += part[(size_t)q * n + r];
     y[r] = s;
 }
