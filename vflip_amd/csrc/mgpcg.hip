@@ -313,6 +313,14 @@ int build_ctab(vflip_solver* S) {
     return 0;
 }
 
+// o = (float)a over n cells, two per thread where the sizes and alignments allow.
+void cast_f64_f32(unsigned n, const double* a, float* o, hipStream_t st) {
+    if (n % 2 == 0 && ((uintptr_t)a & 15) == 0 && ((uintptr_t)o & 7) == 0)
+        k_cast_f64_f32_x2<<<(n / 2 + kBlock - 1) / kBlock, kBlock, 0, st>>>(n, a, o);
+    else
+        k_cast_f64_f32<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(n, a, o);
+}
+
 int finalize(vflip_solver* S, int what, double tol = 0) {
     if (mixed(S))
         k_finalize<double><<<1, kBlock, 0, S->stream>>>(S->sc, S->partial, S->np_last, what, tol);
@@ -966,7 +974,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
         } else {
             k_cast_f32_f64<<<gn, kBlock, 0, S->stream>>>(n, x32, S->x64);
         }
-        if (!mx) k_cast_f64_f32<<<gn, kBlock, 0, S->stream>>>(n, S->x64, S->xw);
+        if (!mx) cast_f64_f32(n, S->x64, S->xw, S->stream);
         // The PCG vets the guess: it minimises E(x) = x'Ax/2 - b'x, and
         // E(x0) < E(0) = 0 exactly when x0 is closer to the solution than zero
         // in the A-norm. A guess that is not is dropped (restart from zero). The
@@ -1048,7 +1056,7 @@ int solve_impl(vflip_solver* S, const float* rhs32, const double* rhs64, float* 
     }
     // outputs
     if (mx) {
-        if (x32) k_cast_f64_f32<<<gn, kBlock, 0, S->stream>>>(n, S->x64, x32);
+        if (x32) cast_f64_f32(n, S->x64, x32, S->stream);
         if (x64out && x64out != S->x64)
             VF_HIP(hipMemcpyAsync(x64out, S->x64, n * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
     } else {
@@ -1160,7 +1168,7 @@ int solver_refine_add(vflip_solver* S, hipStream_t s) {
 }
 // x32 = FP32(ref_x64), x64 (may be null) = ref_x64
 int solver_refine_end(vflip_solver* S, hipStream_t s, float* x32, double* x64) {
-    if (x32) k_cast_f64_f32<<<grid_for(S->n), kBlock, 0, s>>>((unsigned)S->n, S->ref_x64, x32);
+    if (x32) cast_f64_f32((unsigned)S->n, S->ref_x64, x32, s);
     if (x64) VF_HIP(hipMemcpyAsync(x64, S->ref_x64, S->n * sizeof(double), hipMemcpyDeviceToDevice, s));
     VF_CHECK_LAUNCH();
     return 0;
@@ -1233,7 +1241,7 @@ int vflip_solver_solve_f64(vflip_solver* S, const double* rhs, double* x, vflip_
     int rc = ensure_f64(S);
     if (rc) return rc;
     // FP32 copy of the right-hand side for the FP32 parts of the iteration
-    k_cast_f64_f32<<<grid_for(S->n), kBlock, 0, S->stream>>>((unsigned)S->n, rhs, S->b32);
+    cast_f64_f32((unsigned)S->n, rhs, S->b32, S->stream);
     VF_CHECK_LAUNCH();
     return solve_impl(S, S->b32, rhs, nullptr, x, st);
 }
